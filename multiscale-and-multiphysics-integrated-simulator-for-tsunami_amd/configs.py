@@ -1,0 +1,221 @@
+"""Synthetic workloads of the VOF-SMAC step, written as CADMAS-SURF/3D input decks.
+
+Every case is expressed in the reference's own deck grammar (Source code/A11111.in):
+GRID, OBST, POROUS, B.C., TIME, MATE, COMP, OPTION and DEBUG lines.  Decks are
+produced as text so that the same case can be fed to the reference solver (to make
+golden fixtures) and to this package's deck reader (deck.py).
+
+Initial conditions that the grammar cannot express (a solitary wave) are returned
+as a separate function applied to the fields after set-up, the way the reference
+injects them through a restart file (Source code/vf_ir1inp.f:184-205).
+"""
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+
+
+def _coords(n, d, x0=0.0):
+    return [x0 + i * d for i in range(n + 1)]
+
+
+def _grid_lines(axis, coords, per_line=12):
+    """GRID {X|Y|Z} c1 c2 ... END; continuation lines hold bare coordinates
+    (Source code/vf_iigrid.f:70-111 reads words until END across lines)."""
+    words = [f"{c:.6f}" for c in coords] + ["END"]
+    lines = []
+    for i in range(0, len(words), per_line):
+        lines.append(" ".join(words[i:i + per_line]))
+    lines[0] = f"GRID {axis} " + lines[0]
+    return lines
+
+
+@dataclass
+class Case:
+    """A named workload: deck text, sizes and an optional initial-condition hook."""
+    name: str
+    deck: str
+    ni: int
+    nj: int
+    nk: int
+    steps: int
+    dt: float
+    ic: Optional[Callable] = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def cells(self):
+        return self.ni * self.nj * self.nk
+
+
+def _common(lines, dt, steps, eps_r=1.0e-10, rho=1000.0, nu=1.0e-6, g=9.81, wlevel=None):
+    lines += [
+        f"TIME CONST {dt:.6e}",
+        f"TIME END {steps} 1.0D9",
+        f"MATE DENSITY {rho}",
+        f"MATE K-VISC {nu:.6e}",
+        f"MATE GRAVITY {g}",
+        "COMP MTRX M-ILUBCGSTAB 0.95",
+        f"COMP MTRX R-ERROR {eps_r:.6e}",
+        "OPTION T-DOOR BUB OFF",
+        "OPTION T-DOOR DROP OFF",
+    ]
+    if wlevel is not None:
+        lines.append(f"MATE W-LEVEL {wlevel:.6f}")
+
+
+def dam_break(ni=64, nj=16, nk=48, d=0.01, col_x=0.20, col_z=0.30, steps=500, dt=1.0e-4,
+              eps_r=1.0e-10, name=None):
+    """Closed rectangular tank with a water column in the corner (dam break).
+
+    Faces of the column align with grid lines so the initial F is exactly 0/1.
+    """
+    ic_i = int(round(col_x / d))
+    ic_k = int(round(col_z / d))
+    lines = [f"# dam break {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE"]
+    _common(lines, dt, steps, eps_r)
+    lines.append(f"DEBUG F-BOX 1 1 1 {ic_i} {nj} {ic_k} 1.0")
+    return Case(name or f"dam_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
+def _beach_obst(lines, xs, dz, ni, nj, x0, slope, zcap=None):
+    """Plane beach of obstacle cells: column i is solid up to the beach surface."""
+    for i in range(ni):
+        xc = 0.5 * (xs[i] + xs[i + 1])
+        if xc <= x0:
+            continue
+        zb = (xc - x0) * slope
+        if zcap is not None:
+            zb = min(zb, zcap)
+        kc = int(np.floor(zb / dz + 0.5))
+        if kc >= 1:
+            lines.append(f"OBST {i + 1} 1 1 {i + 1} {nj} {kc}")
+
+
+def _porous_block(lines, i1, i2, nj, nk, por, cm, cd):
+    lines.append(f"POROUS V {i1} 1 1 {i2} {nj} {nk} {por} 0.0")
+    lines.append(f"POROUS X {i1} 1 1 {i2 + 1} {nj} {nk} {por} 0.0")
+    lines.append(f"POROUS Y {i1} 1 1 {i2} {nj + 1} {nk} {por} 0.0")
+    lines.append(f"POROUS Z {i1} 1 1 {i2} {nj} {nk + 1} {por} 0.0")
+    lines.append(f"POROUS CM {i1} 1 1 {i2} {nj} {nk} {cm}")
+    lines.append(f"POROUS CD {i1} 1 1 {i2} {nj} {nk} {cd}")
+
+
+def beach_small(ni=40, nj=4, nk=20, dx=0.02, dz=0.01, steps=50, dt=2.0e-4, eps_r=1.0e-10,
+                name=None):
+    """Small deck-expressible run-up case exercising every static feature of the path.
+
+    A dam-break column (DEBUG F-BOX) over still water (MATE W-LEVEL) runs through a
+    porous rubble block (POROUS V/X/Y/Z/CM/CD) towards a plane beach of obstacle cells
+    (OBST); the upstream x- face is an open free boundary (B.C. X ... VP FREE).
+    """
+    xs = _coords(ni, dx)
+    lines = [f"# beach {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", xs)
+    lines += _grid_lines("Y", _coords(nj, dx))
+    lines += _grid_lines("Z", _coords(nk, dz))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE",
+              f"B.C. X 1 1 1 1 {nj} {nk} VP FREE"]
+    _beach_obst(lines, xs, dz, ni, nj, x0=0.55 * ni * dx, slope=0.25, zcap=0.6 * nk * dz)
+    pi1 = int(0.35 * ni) + 1
+    _porous_block(lines, pi1, pi1 + 3, nj, nk, 0.4, 1.2, 1.0)
+    _common(lines, dt, steps, eps_r, wlevel=0.40 * nk * dz)
+    lines.append(f"DEBUG F-BOX 1 1 1 {int(0.2 * ni)} {nj} {int(0.7 * nk)} 1.0")
+    return Case(name or f"beach_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
+def solitary_ic(h, H, x0, g=9.81, rho=1000.0):
+    """Boussinesq solitary wave eta = H sech^2(sqrt(3H/4h^3)(x-x0)), u = sqrt(g/h) eta.
+
+    Returns a function (layout) -> None that overwrites F, U and P of a set-up field
+    layout: F per column from the free-surface height, U on x-faces of wet cells,
+    P hydrostatic under the local surface.
+    """
+    kk = np.sqrt(3.0 * H / (4.0 * h ** 3))
+
+    def apply(lay):
+        xx, zz = lay.XX, lay.ZZ
+        nf = lay.NF
+        numi, numj, numk = nf.shape
+        xc = 0.5 * (xx[0, :-1] + xx[0, 1:])
+        xc = np.append(xc, xc[-1] + xx[1, -1])
+        eta_c = H / np.cosh(kk * (xc - x0)) ** 2
+        eta_f = H / np.cosh(kk * (xx[0, :] - x0)) ** 2
+        zb = zz[0, :]
+        zt = np.append(zb[1:], zb[-1] + zz[1, -1])
+        surf = h + eta_c
+        frac = np.clip((surf[:, None] - zb[None, :]) / (zt - zb)[None, :], 0.0, 1.0)
+        F = np.broadcast_to(frac[:, None, :], (numi, numj, numk)).copy()
+        F[nf == -1] = 0.0
+        F[0, :, :] = F[-1, :, :] = 0.0
+        F[:, 0, :] = F[:, -1, :] = 0.0
+        F[:, :, 0] = F[:, :, -1] = 0.0
+        lay.FF[...] = F
+        zc = 0.5 * (zb + zt)
+        P = rho * g * (surf[:, None] - zc[None, :])
+        lay.PP[...] = np.where(lay.FF > 0.0, np.broadcast_to(P[:, None, :], (numi, numj, numk)), 0.0)
+        u = np.sqrt(g / h) * eta_f
+        wet = zc[None, :] < (h + eta_f)[:, None]
+        U = np.where(wet, u[:, None], 0.0)
+        lay.UU[...] = np.broadcast_to(U[:, None, :], (numi, numj, numk))
+        lay.UU[nf == -1] = 0.0
+    return apply
+
+
+def solitary_runup(ni=512, nj=32, nk=96, dx=0.02, dz=0.01, h=0.5, H_over_h=0.1, steps=2000,
+                   dt=2.0e-4, eps_r=1.0e-10, name=None):
+    """Solitary-wave run-up on a 1:20 beach from x=6 m behind a porous rubble block."""
+    xs = _coords(ni, dx)
+    lines = [f"# solitary run-up {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", xs)
+    lines += _grid_lines("Y", _coords(nj, dx))
+    lines += _grid_lines("Z", _coords(nk, dz))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE", f"B.C. X 1 1 1 1 {nj} {nk} VP FREE"]
+    _beach_obst(lines, xs, dz, ni, nj, x0=6.0, slope=1.0 / 20.0, zcap=(nk - 4) * dz)
+    i1 = int(round(4.0 / dx)) + 1
+    i2 = int(round(4.4 / dx))
+    if i2 <= ni:
+        _porous_block(lines, i1, i2, nj, nk, 0.4, 1.2, 1.0)
+    _common(lines, dt, steps, eps_r, wlevel=h)
+    case = Case(name or f"runup_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt,
+                ic=solitary_ic(h, H_over_h * h, 2.0))
+    return case
+
+
+def buildings_tank(ni=512, nj=256, nk=128, d=0.02, h=0.8, H_over_h=0.15, nbuild=40, seed=0,
+                   steps=200, dt=2.0e-4, eps_r=1.0e-10, name=None):
+    """Solitary wave over a 1:20 beach with seeded rectangular buildings on the beach half."""
+    rng = np.random.default_rng(seed)
+    xs = _coords(ni, d)
+    lines = [f"# buildings {ni}x{nj}x{nk} seed {seed}"]
+    lines += _grid_lines("X", xs)
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE", f"B.C. X 1 1 1 1 {nj} {nk} VP FREE"]
+    x0 = min(6.0, 0.5 * ni * d)
+    _beach_obst(lines, xs, d, ni, nj, x0=x0, slope=1.0 / 20.0, zcap=(nk - 4) * d)
+    for _ in range(nbuild):
+        fx, fy = rng.integers(4, 17, size=2)
+        ht = int(rng.integers(8, 41))
+        ia = int(rng.integers(ni // 2 + 1, ni - fx))
+        ja = int(rng.integers(1, nj - fy))
+        lines.append(f"OBST {ia} {ja} 1 {ia + fx - 1} {ja + fy - 1} {min(ht, nk - 2)}")
+    _common(lines, dt, steps, eps_r, wlevel=h)
+    return Case(name or f"buildings_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt,
+                ic=solitary_ic(h, H_over_h * h, 2.0))
+
+
+def named(name):
+    """The workloads by name (sizes as in the benchmark table)."""
+    table = {
+        "dam_a": lambda: dam_break(),
+        "runup_b": lambda: solitary_runup(),
+        "buildings_c": lambda: buildings_tank(),
+        "tiny": lambda: dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5),
+        "beach_small": lambda: beach_small(),
+    }
+    return table[name]()

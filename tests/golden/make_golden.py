@@ -1,0 +1,119 @@
+"""Generate golden fixtures from the reference CADMAS-SURF/3D solver.
+
+Runs oracle/_ref/cadmas_ref.x (the reference compiled from its own Fortran
+sources by oracle/build_ref.sh, with the phase recorder oracle/ref_dump.c) on
+the synthetic decks of configs.py and stores what each phase read and wrote as
+compressed .npz files next to this script:
+
+  <case>_static.npz          set-up arrays (metrics, porosities, face indices,
+                             boundary list) and solver settings
+  <case>_sNNNNN.npz          for step NNNNN: v1_in/ps_in/ps_out/v1_out/f1_in/
+                             f1_out records, keys prefixed "<tag>:"
+  <case>_trajectory.npz      per-step list-file quantities for every step:
+                             STEP, FSUM, FCUT, DV, BNORM, RNORM, ITR
+
+Usage: python tests/golden/make_golden.py [case ...]
+Only runs in the build container (needs /root/reference and flang); the
+fixtures it writes are what the tests read everywhere else.
+"""
+import os
+import re
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(HERE))
+from dumpio import read_dump  # noqa: E402
+
+REF_EXE = ROOT / "oracle" / "_ref" / "cadmas_ref.x"
+
+
+def _load_pkg():
+    import importlib.util
+    name = "cadmas_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    pkg = ROOT / "multiscale-and-multiphysics-integrated-simulator-for-tsunami_amd"
+    spec = importlib.util.spec_from_file_location(name, pkg / "__init__.py",
+                                                  submodule_search_locations=[str(pkg)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+# case name -> (case factory, steps whose phase records are kept)
+def _cases():
+    cfg = _load_pkg().configs
+    return {
+        "tiny": (lambda: cfg.dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5, name="tiny"),
+                 [1, 2, 3, 4, 5]),
+        "beach_small": (lambda: cfg.beach_small(steps=40, name="beach_small"), [1, 2, 3, 20, 40]),
+        "dam_a": (lambda: cfg.dam_break(name="dam_a"), [1, 500]),
+    }
+
+
+_STEP_RE = re.compile(
+    r"STEP=\s*(\d+)\s*:\s*TIME=\s*(\S+)\s*:\s*DT\s*=\s*(\S+)\s*:\s*FSUM=\s*(\S+)\s*:\s*FCUT=\s*(\S+)"
+    r"\s*:\s*!VD!=\s*(\S+)\s*\n\s*:\s*!B!\s*=\s*(\S+)\s*:\s*!R!\s*=\s*(\S+)\s*:\s*ITR\s*=\s*(\S+)")
+
+
+def run_case(name, factory, keep_steps):
+    case = factory()
+    with tempfile.TemporaryDirectory(prefix=f"golden_{name}_") as td:
+        td = Path(td)
+        (td / "dump").mkdir()
+        (td / f"{name}.in").write_text(case.deck)
+        (td / "data.env").write_text(f"1\n{name} 1 0\n")
+        env = dict(os.environ, CADMAS_DUMP_DIR=str(td / "dump"),
+                   CADMAS_DUMP_STEPS=",".join(str(s) for s in keep_steps))
+        r = subprocess.run([str(REF_EXE)], cwd=td, env=env, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run {name} failed rc={r.returncode}\n{r.stdout[-2000:]}")
+        listing = (td / f"{name}.list").read_text()
+        traj = {k: [] for k in ("STEP", "TIME", "DT", "FSUM", "FCUT", "DV", "BNORM", "RNORM", "ITR")}
+        for m in _STEP_RE.finditer(listing):
+            vals = m.groups()
+            traj["STEP"].append(int(vals[0]))
+            for key, v in zip(("TIME", "DT", "FSUM", "FCUT", "DV", "BNORM", "RNORM"), vals[1:8]):
+                traj[key].append(float(v))
+            traj["ITR"].append(int(vals[8]))
+        # full-precision per-step scalars from the recorder
+        dv = {}
+        for f in sorted((td / "dump").glob("s*_dv.bin")):
+            d = read_dump(f)
+            dv[int(d["NNOW"][0])] = float(d["DV"][0])
+        np.savez_compressed(HERE / f"{name}_trajectory.npz",
+                            **{k: np.array(v) for k, v in traj.items()},
+                            DV_STEPS=np.array(sorted(dv)), DV_FULL=np.array([dv[s] for s in sorted(dv)]),
+                            DECK=np.array(case.deck))
+        static = read_dump(td / "dump" / "static.bin")
+        np.savez_compressed(HERE / f"{name}_static.npz", **static, DECK=np.array(case.deck))
+        for s in keep_steps:
+            rec = {}
+            for tag in ("v1_in", "ps_in", "ps_out", "v1_out", "f1_in", "f1_out"):
+                p = td / "dump" / f"s{s:05d}_{tag}.bin"
+                if p.exists():
+                    for k, v in read_dump(p).items():
+                        rec[f"{tag}:{k}"] = v
+            np.savez_compressed(HERE / f"{name}_s{s:05d}.npz", **rec)
+        print(f"{name}: {len(traj['STEP'])} list steps, kept {keep_steps}")
+
+
+def main(argv):
+    if not REF_EXE.exists():
+        subprocess.run([str(ROOT / "oracle" / "build_ref.sh")], check=True)
+    cases = _cases()
+    names = argv or list(cases)
+    for n in names:
+        fac, keep = cases[n]
+        run_case(n, fac, keep)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
