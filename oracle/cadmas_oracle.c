@@ -1,0 +1,1458 @@
+/*
+ * cadmas_oracle.c -- serial CPU restatement of the CADMAS-SURF/3D single-phase
+ * VOF-SMAC time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see cadmas_oracle.h).  Every routine follows one
+ * reference subroutine of Simulators/CADMAS-SURF-3D/Source code/ with the same
+ * loop order, loop ranges and floating-point expression order, so that on the
+ * same inputs it reproduces the reference bit for bit (build with
+ * -ffp-contract=off; x86-64 baseline has no FMA).  Reference citations are
+ * given per routine as file:line.
+ *
+ * Supported feature set (anything else returns a negative code):
+ *   boundary types VP SLIP / NON-S / FIX-V / FREE (INDB(3,*) = 1..4),
+ *   F FIX / FREE (INDB(4,*) = 1,2), static porosity (IPRNT<=1), donor scheme
+ *   (ISCMVP=0, ISCMFF=0), drag by CD (IDRGN=0), no sponge (IDAMP<0), no wave
+ *   maker / wave source / open boundary, no air-pressure model (PVCP0<ZERO),
+ *   T-DOOR bubble and drop models off, LOOPS=1.
+ */
+#include "cadmas_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ZERO 1.0e-20
+
+/* ---- indexing (1-based, column major) ---------------------------------- */
+#define DECL_DIMS(s)                                   \
+    const int ni = (s)->numi, nj = (s)->numj, nk = (s)->numk; \
+    const size_t nij = (size_t)ni * (size_t)nj;        \
+    (void)nk; (void)nij
+#define ID(i, j, k) ((size_t)((i) - 1) + (size_t)ni * (size_t)((j) - 1) + nij * (size_t)((k) - 1))
+#define XX(m, i) (s->xx[(size_t)((i) - 1) * 6 + (m) - 1])
+#define YY(m, j) (s->yy[(size_t)((j) - 1) * 6 + (m) - 1])
+#define ZZ(m, k) (s->zz[(size_t)((k) - 1) * 6 + (m) - 1])
+#define INDB(m, l) (s->indb[(size_t)((l) - 1) * 4 + (m) - 1])
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+/* metric of axis a (0=x,1=y,2=z), entry m (1..6), index i (1-based) */
+static inline double MET(const orc_state *s, int a, int m, int i)
+{
+    const double *t = a == 0 ? s->xx : (a == 1 ? s->yy : s->zz);
+    return t[(size_t)(i - 1) * 6 + m - 1];
+}
+
+static void decode(const orc_state *s, int ijk, int *i, int *j, int *k)
+{
+    DECL_DIMS(s);
+    int r = ijk;
+    *k = (r - 1) / (int)nij + 1;
+    r = r - (int)nij * (*k - 1);
+    *j = (r - 1) / ni + 1;
+    *i = r - ni * (*j - 1);
+}
+
+/* ======================================================================
+ * Phase (a): momentum predictor
+ * ====================================================================== */
+
+/* Normal flux of component a at cell centres: the (G u) u term blended by
+ * SCMVP and G nu 2 du/dx.  vf_vflxdu.f:71-90 (u), vf_vflxdv.f:71-90 (v),
+ * vf_vflxdw.f:66-85 (w). */
+static void flux_normal(orc_state *s, int a, const double *q, const double *gl,
+                        const double *gg, double *fl)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const size_t sa = st[a];
+    const double su = s->scmvp, sc = 1.0 - s->scmvp;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->indc[n] == -1) continue;
+                int ia = a == 0 ? i : (a == 1 ? j : k);
+                double win = (gl[n] * q[n] + gl[n + sa] * q[n + sa]) * 0.5;
+                double fg = win * (q[n] + q[n + sa]) * 0.5;
+                fl[n] = fl[n] - su * (dmax(win, 0.0) * q[n] - dmax(-win, 0.0) * q[n + sa]) - sc * fg;
+                fl[n] = fl[n] + (gg[n] + gg[n + sa]) * s->anu[n] * MET(s, a, 4, ia) * (q[n + sa] - q[n]);
+            }
+}
+
+/* Cross flux of component a (q) through faces normal to axis b: (G_b v_b) q
+ * and G_b nu (dq/db + dv_b/da) on the cell edge (a-face, b-face), including
+ * the wall/free-surface special branch.  The six instances in the reference
+ * are axis permutations of one formula:
+ *   q=u,b=y vf_vflxdu.f:94-219   q=u,b=z vf_vflxdu.f:222-347
+ *   q=v,b=x vf_vflxdv.f:94-219   q=v,b=z vf_vflxdv.f:222-347
+ *   q=w,b=x vf_vflxdw.f:88-213   q=w,b=y vf_vflxdw.f:216-341
+ * Loop ranges per instance are passed in (lo/hi for i, j, k). */
+static void flux_cross(orc_state *s, int a, int b, const double *q, const double *vb,
+                       const double *glb, const double *ggb, const int *ind_a,
+                       const int *ind_b, const double *bcq, double *fl,
+                       int i0, int i1, int j0, int j1, int k0, int k1)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const size_t sa = st[a], sb = st[b];
+    const double su = s->scmvp, sc = 1.0 - s->scmvp;
+    const int nsb = 2 * b + 1; /* INDB(2,*) of a boundary on the minus-b side */
+    for (int k = k0; k <= k1; k++)
+        for (int j = j0; j <= j1; j++)
+            for (int i = i0; i <= i1; i++) {
+                size_t n = ID(i, j, k);
+                const int *nf = s->nf;
+                int ic = nf[n - sa - sb] * nf[n - sb] * nf[n - sa] * nf[n];
+                int it = ind_a[n];
+                int ib = ind_a[n - sb];
+                int ia = a == 0 ? i : (a == 1 ? j : k);
+                int jb = b == 0 ? i : (b == 1 ? j : k);
+                if (ic != 0 || (it != 0 && ib != 0)) {
+                    continue;
+                } else if (it == 0 && ib == 0) {
+                    double win = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * glb[n] * vb[n]
+                                                     + MET(s, a, 2, ia) * glb[n - sa] * vb[n - sa]);
+                    double fg = win * MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * q[n] + MET(s, b, 2, jb) * q[n - sb]);
+                    fl[n] = fl[n] - su * (dmax(win, 0.0) * q[n - sb] - dmax(-win, 0.0) * q[n]) - sc * fg;
+                    double gg = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * ggb[n] + MET(s, a, 2, ia) * ggb[n - sa]);
+                    double anr = MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * s->anu[n] + MET(s, b, 2, jb) * s->anu[n - sb]);
+                    double anl = MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * s->anu[n - sa] + MET(s, b, 2, jb) * s->anu[n - sa - sb]);
+                    double an = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * anr + MET(s, a, 2, ia) * anl);
+                    double d1 = MET(s, b, 5, jb) * (q[n] - q[n - sb]);
+                    double d2 = MET(s, a, 5, ia) * (vb[n] - vb[n - sa]);
+                    fl[n] = fl[n] + gg * an * (d1 + d2);
+                } else {
+                    double br, bl;
+                    int is = ind_b[n];
+                    if (is >= 1) {
+                        if (INDB(3, is) == 1 || INDB(3, is) == 4)
+                            br = INDB(2, is) == nsb ? q[n] : q[n - sb];
+                        else
+                            br = bcq[is - 1];
+                    } else {
+                        br = MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * (q[n] + q[n + sa])
+                                                 + MET(s, b, 2, jb) * (q[n - sb] + q[n - sb + sa])) * 0.5;
+                    }
+                    is = ind_b[n - sa];
+                    if (is >= 1) {
+                        if (INDB(3, is) == 1 || INDB(3, is) == 4)
+                            bl = INDB(2, is) == nsb ? q[n] : q[n - sb];
+                        else
+                            bl = bcq[is - 1];
+                    } else {
+                        bl = MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * (q[n] + q[n - sa])
+                                                 + MET(s, b, 2, jb) * (q[n - sb] + q[n - sb - sa])) * 0.5;
+                    }
+                    double win = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * glb[n] * vb[n]
+                                                     + MET(s, a, 2, ia) * glb[n - sa] * vb[n - sa]);
+                    double fg = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * glb[n] * vb[n] * br
+                                                    + MET(s, a, 2, ia - 1) * glb[n - sa] * vb[n - sa] * bl);
+                    if (it == 0 && win < 0.0)
+                        fl[n] = fl[n] - su * win * q[n] - sc * fg;
+                    else if (ib == 0 && win > 0.0)
+                        fl[n] = fl[n] - su * win * q[n - sb] - sc * fg;
+                    else
+                        fl[n] = fl[n] - fg;
+                    double gg = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * ggb[n] + MET(s, a, 2, ia) * ggb[n - sa]);
+                    fg = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * br + MET(s, a, 2, ia - 1) * bl);
+                    double anr, anl;
+                    if (nf[n] == -1)
+                        anr = s->anu[n - sb];
+                    else if (nf[n - sb] == -1)
+                        anr = s->anu[n];
+                    else
+                        anr = MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * s->anu[n] + MET(s, b, 2, jb) * s->anu[n - sb]);
+                    if (nf[n - sa] == -1)
+                        anl = s->anu[n - sa - sb];
+                    else if (nf[n - sa - sb] == -1)
+                        anl = s->anu[n - sa];
+                    else
+                        anl = MET(s, b, 6, jb) * (MET(s, b, 2, jb - 1) * s->anu[n - sa] + MET(s, b, 2, jb) * s->anu[n - sa - sb]);
+                    double an = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * anr + MET(s, a, 2, ia) * anl);
+                    double d1;
+                    if (it == 0)
+                        d1 = MET(s, b, 4, jb) * (q[n] - fg) * 2.0;
+                    else
+                        d1 = MET(s, b, 4, jb - 1) * (fg - q[n - sb]) * 2.0;
+                    double d2 = MET(s, a, 5, ia) * (vb[n] - vb[n - sa]);
+                    fl[n] = fl[n] + gg * an * (d1 + d2);
+                }
+            }
+}
+
+/* Source terms of component a: pressure gradient, porous drag, gravity.
+ * vf_vgene.f:84-163 (u), :166-241 (v), :244-330 (w); IDRGN<=0 branch. */
+static void vgene(orc_state *s, int a, double *qsrc)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const size_t sa = st[a];
+    const int *ind = a == 0 ? s->indx : (a == 1 ? s->indy : s->indz);
+    const double *gga = a == 0 ? s->ggx : (a == 1 ? s->ggy : s->ggz);
+    const double *vel[3] = {s->uu, s->vv, s->ww};
+    const double ri = 1.0 / s->rho0;
+    int i0 = 2, j0 = 2, k0 = 2;
+    if (a == 0) i0 = 3;
+    if (a == 1) j0 = 3;
+    if (a == 2) k0 = 3;
+    for (int k = k0; k <= nk - 1; k++)
+        for (int j = j0; j <= nj - 1; j++)
+            for (int i = i0; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (!(ind[n] == 0 && s->nf[n - sa] * s->nf[n] == 0)) continue;
+                int ia = a == 0 ? i : (a == 1 ? j : k);
+                double gv = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * s->ggv[n] + MET(s, a, 2, ia - 1) * s->ggv[n - sa]);
+                double cd = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * s->cd0[n] + MET(s, a, 2, ia - 1) * s->cd0[n - sa]);
+                qsrc[n] = qsrc[n] + gv * ri * MET(s, a, 5, ia) * (s->pp[n - sa] - s->pp[n]);
+                double c[3];
+                for (int d = 0; d < 3; d++) {
+                    if (d == a) {
+                        c[d] = vel[a][n];
+                    } else {
+                        const double *vd = vel[d];
+                        size_t sd = st[d];
+                        c[d] = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * (vd[n] + vd[n + sd])
+                                                   + MET(s, a, 2, ia) * (vd[n - sa] + vd[n - sa + sd])) * 0.5;
+                    }
+                }
+                double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                qsrc[n] = qsrc[n] - 0.5 * cd * MET(s, a, 5, ia) * (1.0 - gga[n]) * c[a] * uvw;
+                if (a == 2) {
+                    qsrc[n] = qsrc[n] - gv * s->grz0;
+                    double dr = 0.0;
+                    qsrc[n] = qsrc[n] - gv * s->grz0 * dr / s->rho0;
+                }
+            }
+}
+
+/* Explicit update of component a and the VVMAX clip.
+ * vf_veuler.f:116-158 (u), :161-203 (v), :206-249 (w); GV0 = GV for static
+ * porosity (vf_v1cal.f:218-227 passes GLV twice). */
+static void veuler(orc_state *s, int a, double *const fl[3], const double *qsrc)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const size_t sa = st[a];
+    const int *ind = a == 0 ? s->indx : (a == 1 ? s->indy : s->indz);
+    double *q = a == 0 ? s->uu : (a == 1 ? s->vv : s->ww);
+    int i0 = 2, j0 = 2, k0 = 2;
+    if (a == 0) i0 = 3;
+    if (a == 1) j0 = 3;
+    if (a == 2) k0 = 3;
+    for (int k = k0; k <= nk - 1; k++)
+        for (int j = j0; j <= nj - 1; j++)
+            for (int i = i0; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (!(ind[n] == 0 && s->nf[n - sa] * s->nf[n] == 0)) continue;
+                int idx[3] = {i, j, k};
+                int ia = idx[a];
+                double gv = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * s->glv[n] + MET(s, a, 2, ia - 1) * s->glv[n - sa]);
+                double gv0 = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * s->glv[n] + MET(s, a, 2, ia - 1) * s->glv[n - sa]);
+                double t[3];
+                for (int d = 0; d < 3; d++) {
+                    const double *f = fl[d];
+                    if (d == a)
+                        t[d] = MET(s, d, 5, idx[d]) * (f[n] - f[n - st[d]]);
+                    else
+                        t[d] = MET(s, d, 4, idx[d]) * (f[n + st[d]] - f[n]);
+                }
+                double du = (t[0] + t[1] + t[2] + qsrc[n]);
+                q[n] = (gv * q[n] + s->dtnow * du) / gv0;
+                if (q[n] > s->vvmax) q[n] = s->vvmax;
+                if (q[n] < -s->vvmax) q[n] = -s->vvmax;
+            }
+}
+
+/* Normal velocity on boundary faces.  vf_bwuwn.f:84-254 (types 1-4, 6-8). */
+static int bwuwn(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int l = 1; l <= s->numb; l++) {
+        int i, j, k;
+        decode(s, INDB(1, l), &i, &j, &k);
+        int ns = INDB(2, l), ib = INDB(3, l);
+        size_t n = ID(i, j, k);
+        if (ib == 1 || ib == 6 || ib == 8) {
+            if (ns == 1 || ns == 2) { s->bcu[l - 1] = 0.0; s->uu[n] = 0.0; }
+            else if (ns == 3 || ns == 4) { s->bcv[l - 1] = 0.0; s->vv[n] = 0.0; }
+            else { s->bcw[l - 1] = 0.0; s->ww[n] = 0.0; }
+        } else if (ib == 2) {
+            s->bcu[l - 1] = 0.0; s->bcv[l - 1] = 0.0; s->bcw[l - 1] = 0.0;
+            if (ns == 1 || ns == 2) s->uu[n] = s->bcu[l - 1];
+            else if (ns == 3 || ns == 4) s->vv[n] = s->bcv[l - 1];
+            else s->ww[n] = s->bcw[l - 1];
+        } else if (ib == 3) {
+            if (ns == 1 || ns == 2) s->uu[n] = s->bcu[l - 1];
+            else if (ns == 3 || ns == 4) s->vv[n] = s->bcv[l - 1];
+            else s->ww[n] = s->bcw[l - 1];
+        } else if (ib == 4) {
+            double val;
+            switch (ns) {
+            case 1: val = s->uu[n + 1]; s->bcu[l - 1] = val; s->uu[n] = val; break;
+            case 2: val = s->uu[n - 1]; s->bcu[l - 1] = val; s->uu[n] = val; break;
+            case 3: val = s->vv[n + ni]; s->bcv[l - 1] = val; s->vv[n] = val; break;
+            case 4: val = s->vv[n - ni]; s->bcv[l - 1] = val; s->vv[n] = val; break;
+            case 5: val = s->ww[n + nij]; s->bcw[l - 1] = val; s->ww[n] = val; break;
+            case 6: val = s->ww[n - nij]; s->bcw[l - 1] = val; s->ww[n] = val; break;
+            }
+        } else if (ib == 7) {
+        } else {
+            return -10; /* wave-maker boundaries are outside the restated path */
+        }
+    }
+    return 0;
+}
+
+int orc_predictor(orc_state *s)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    double *w = (double *)calloc(12 * nc, sizeof(double));
+    if (!w) return -1;
+    /* WK01..WK12 of vf_v1cal.f:126-146 */
+    double *flu[3] = {w + 0 * nc, w + 1 * nc, w + 2 * nc};
+    double *qu = w + 3 * nc;
+    double *flv[3] = {w + 4 * nc, w + 5 * nc, w + 6 * nc};
+    double *qv = w + 7 * nc;
+    double *flw[3] = {w + 8 * nc, w + 9 * nc, w + 10 * nc};
+    double *qw = w + 11 * nc;
+
+    /* VF_VFLXDU (vf_v1cal.f:155) */
+    flux_normal(s, 0, s->uu, s->glx, s->ggx, flu[0]);
+    flux_cross(s, 0, 1, s->uu, s->vv, s->gly, s->ggy, s->indx, s->indy, s->bcu, flu[1],
+               3, ni - 1, 2, nj, 2, nk);
+    flux_cross(s, 0, 2, s->uu, s->ww, s->glz, s->ggz, s->indx, s->indz, s->bcu, flu[2],
+               3, ni - 1, 2, nj - 1, 2, nk);
+    /* VF_VFLXDV (vf_v1cal.f:169) */
+    flux_normal(s, 1, s->vv, s->gly, s->ggy, flv[1]);
+    flux_cross(s, 1, 0, s->vv, s->uu, s->glx, s->ggx, s->indy, s->indx, s->bcv, flv[0],
+               2, ni, 3, nj - 1, 2, nk);
+    flux_cross(s, 1, 2, s->vv, s->ww, s->glz, s->ggz, s->indy, s->indz, s->bcv, flv[2],
+               2, ni - 1, 3, nj - 1, 2, nk);
+    /* VF_VFLXDW (vf_v1cal.f:183) */
+    flux_normal(s, 2, s->ww, s->glz, s->ggz, flw[2]);
+    flux_cross(s, 2, 0, s->ww, s->uu, s->glx, s->ggx, s->indz, s->indx, s->bcw, flw[0],
+               2, ni, 2, nj - 1, 3, nk - 1);
+    flux_cross(s, 2, 1, s->ww, s->vv, s->gly, s->ggy, s->indz, s->indy, s->bcw, flw[1],
+               2, ni - 1, 2, nj, 3, nk - 1);
+    /* VF_VGENE (vf_v1cal.f:199) */
+    vgene(s, 0, qu);
+    vgene(s, 1, qv);
+    vgene(s, 2, qw);
+    /* VF_VEULER (vf_v1cal.f:218) */
+    veuler(s, 0, flu, qu);
+    veuler(s, 1, flv, qv);
+    veuler(s, 2, flw, qw);
+    free(w);
+    /* VF_BWUWN (vf_v1cal.f:251) */
+    return bwuwn(s);
+}
+
+/* ======================================================================
+ * Phase (b): pressure Poisson
+ * ====================================================================== */
+
+/* Matrix and right-hand side.  vf_vpcoef.f:76-437 (wave-source and
+ * air-pressure branches excluded). */
+int orc_vpcoef(orc_state *s, orc_matrix *m)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    memset(m->ad, 0, nc * sizeof(double));
+    memset(m->ali, 0, nc * sizeof(double));
+    memset(m->alj, 0, nc * sizeof(double));
+    memset(m->alk, 0, nc * sizeof(double));
+    memset(m->aui, 0, nc * sizeof(double));
+    memset(m->auj, 0, nc * sizeof(double));
+    memset(m->auk, 0, nc * sizeof(double));
+    memset(m->bb, 0, nc * sizeof(double));
+    const double dti = 1.0 / s->dtnow;
+    const double tr = s->dtnow / s->rho0;
+    const double pps = 0.0;
+    for (int k = 1; k <= nk; k++)
+        for (int j = 1; j <= nj; j++)
+            for (int i = 1; i <= ni; i++) {
+                size_t n = ID(i, j, k);
+                int nn = s->nf[n];
+                if (s->indc[n] == -1) {
+                    m->ad[n] = 1.0;
+                    m->ali[n] = 0.0; m->alj[n] = 0.0; m->alk[n] = 0.0;
+                    m->aui[n] = 0.0; m->auj[n] = 0.0; m->auk[n] = 0.0;
+                    m->bb[n] = 0.0;
+                } else if (nn == 0) {
+                    double axy = XX(2, i) * YY(2, j);
+                    double axz = XX(2, i) * ZZ(2, k);
+                    double ayz = YY(2, j) * ZZ(2, k);
+                    double avl = XX(2, i) * YY(2, j) * ZZ(2, k);
+                    double aim, aip, ajm, ajp, akm, akp, gv;
+                    const double *ggv = s->ggv, *glv = s->glv;
+                    int is;
+                    /* ALI */
+                    is = s->indx[n];
+                    if (is == 0) {
+                        gv = XX(6, i) * (XX(2, i) * ggv[n] / glv[n] + XX(2, i - 1) * ggv[n - 1] / glv[n - 1]);
+                        aim = ayz * XX(5, i) * s->ggx[n] * gv;
+                        m->ali[n] = -aim;
+                    } else if (is >= 1) {
+                        if (INDB(3, is) == 4) aim = 2.0 * ayz * XX(4, i) * s->ggx[n] * ggv[n] / glv[n];
+                        else aim = 0.0;
+                        m->ali[n] = 0.0;
+                    } else return -20;
+                    /* AUI */
+                    is = s->indx[n + 1];
+                    if (is == 0) {
+                        gv = XX(6, i + 1) * (XX(2, i + 1) * ggv[n + 1] / glv[n + 1] + XX(2, i) * ggv[n] / glv[n]);
+                        aip = ayz * XX(5, i + 1) * s->ggx[n + 1] * gv;
+                        m->aui[n] = -aip;
+                    } else if (is >= 1) {
+                        if (INDB(3, is) == 4) aip = 2.0 * ayz * XX(4, i) * s->ggx[n + 1] * ggv[n] / glv[n];
+                        else aip = 0.0;
+                        m->aui[n] = 0.0;
+                    } else return -20;
+                    /* ALJ */
+                    is = s->indy[n];
+                    if (is == 0) {
+                        gv = YY(6, j) * (YY(2, j) * ggv[n] / glv[n] + YY(2, j - 1) * ggv[n - ni] / glv[n - ni]);
+                        ajm = axz * YY(5, j) * s->ggy[n] * gv;
+                        m->alj[n] = -ajm;
+                    } else if (is >= 1) {
+                        if (INDB(3, is) == 4) ajm = 2.0 * axz * YY(4, j) * s->ggy[n] * ggv[n] / glv[n];
+                        else ajm = 0.0;
+                        m->alj[n] = 0.0;
+                    } else return -20;
+                    /* AUJ */
+                    is = s->indy[n + ni];
+                    if (is == 0) {
+                        gv = YY(6, j + 1) * (YY(2, j + 1) * ggv[n + ni] / glv[n + ni] + YY(2, j) * ggv[n] / glv[n]);
+                        ajp = axz * YY(5, j + 1) * s->ggy[n + ni] * gv;
+                        m->auj[n] = -ajp;
+                    } else if (is >= 1) {
+                        if (INDB(3, is) == 4) ajp = 2.0 * axz * YY(4, j) * s->ggy[n + ni] * ggv[n] / glv[n];
+                        else ajp = 0.0;
+                        m->auj[n] = 0.0;
+                    } else return -20;
+                    /* ALK */
+                    is = s->indz[n];
+                    if (is == 0) {
+                        gv = ZZ(6, k) * (ZZ(2, k) * ggv[n] / glv[n] + ZZ(2, k - 1) * ggv[n - nij] / glv[n - nij]);
+                        akm = axy * ZZ(5, k) * s->ggz[n] * gv;
+                        m->alk[n] = -akm;
+                    } else if (is >= 1) {
+                        if (INDB(3, is) != 4) akm = 0.0;
+                        else akm = 2.0 * axy * ZZ(4, k) * s->ggz[n] * ggv[n] / glv[n];
+                        m->alk[n] = 0.0;
+                    } else return -20;
+                    /* AUK */
+                    is = s->indz[n + nij];
+                    if (is == 0) {
+                        gv = ZZ(6, k + 1) * (ZZ(2, k + 1) * ggv[n + nij] / glv[n + nij] + ZZ(2, k) * ggv[n] / glv[n]);
+                        akp = axy * ZZ(5, k + 1) * s->ggz[n + nij] * gv;
+                        m->auk[n] = -akp;
+                    } else if (is >= 1) {
+                        if (INDB(3, is) != 4) akp = 0.0;
+                        else akp = 2.0 * axy * ZZ(4, k) * s->ggz[n + nij] * ggv[n] / glv[n];
+                        m->auk[n] = 0.0;
+                    } else return -20;
+                    m->ad[n] = akm + ajm + aim + aip + ajp + akp;
+                    double vd = ayz * (s->ggx[n + 1] * s->uu[n + 1] - s->ggx[n] * s->uu[n])
+                              + axz * (s->ggy[n + ni] * s->vv[n + ni] - s->ggy[n] * s->vv[n])
+                              + axy * (s->ggz[n + nij] * s->ww[n + nij] - s->ggz[n] * s->ww[n]);
+                    double vq = -(ggv[n] - ggv[n]) * dti * avl;
+                    m->bb[n] = vd - vq;
+                } else if (nn >= 1 && nn <= 6) {
+                    /* surface cell: Dirichlet row toward the fluid neighbour */
+                    double psi, pf;
+                    m->ad[n] = 1.0;
+                    m->ali[n] = 0.0; m->alj[n] = 0.0; m->alk[n] = 0.0;
+                    m->aui[n] = 0.0; m->auj[n] = 0.0; m->auk[n] = 0.0;
+                    switch (nn) {
+                    case 5: psi = ZZ(3, k) / (0.5 * ZZ(2, k - 1) + s->ff[n] * ZZ(2, k)); m->alk[n] = psi - 1.0; pf = s->pp[n - nij]; break;
+                    case 6: psi = ZZ(3, k + 1) / (0.5 * ZZ(2, k + 1) + s->ff[n] * ZZ(2, k)); m->auk[n] = psi - 1.0; pf = s->pp[n + nij]; break;
+                    case 3: psi = YY(3, j) / (0.5 * YY(2, j - 1) + s->ff[n] * YY(2, j)); m->alj[n] = psi - 1.0; pf = s->pp[n - ni]; break;
+                    case 4: psi = YY(3, j + 1) / (0.5 * YY(2, j + 1) + s->ff[n] * YY(2, j)); m->auj[n] = psi - 1.0; pf = s->pp[n + ni]; break;
+                    case 1: psi = XX(3, i) / (0.5 * XX(2, i - 1) + s->ff[n] * XX(2, i)); m->ali[n] = psi - 1.0; pf = s->pp[n - 1]; break;
+                    default: psi = XX(3, i + 1) / (0.5 * XX(2, i + 1) + s->ff[n] * XX(2, i)); m->aui[n] = psi - 1.0; pf = s->pp[n + 1]; break;
+                    }
+                    m->bb[n] = -tr * ((1.0 - psi) * pf + psi * pps - s->pp[n]);
+                } else {
+                    return -21;
+                }
+            }
+    return 0;
+}
+
+/* MILU factorisation.  vf_mzdcmp.f:49-67. */
+int orc_mzdcmp(int ni, int nj, int nk, const orc_matrix *m, double *dd, double cgpara,
+               const int *indc, int icgtyp)
+{
+    const size_t nij = (size_t)ni * nj;
+    double u = 0.0;
+    if (icgtyp != 0) u = cgpara;
+    int npos = 1;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (indc[n] == -1) continue;
+                double au1 = m->aui[n - 1] + u * (m->auj[n - 1] + m->auk[n - 1]);
+                double au2 = m->auj[n - ni] + u * (m->aui[n - ni] + m->auk[n - ni]);
+                double au3 = m->auk[n - nij] + u * (m->aui[n - nij] + m->auj[n - nij]);
+                dd[n] = 1.0 / (m->ad[n] - m->ali[n] * dd[n - 1] * au1
+                                         - m->alj[n] * dd[n - ni] * au2
+                                         - m->alk[n] * dd[n - nij] * au3);
+                if (dd[n] <= 0.0) npos = 0;
+            }
+    return npos;
+}
+
+/* (L D^-1 + I)(D + U) x = y.  vf_mzminv.f:38-41, vf_mzfrwd.f:37-48,
+ * vf_mzbkwd.f:37-47 (called with X aliased to Y). */
+void orc_mzminv(int ni, int nj, int nk, const double *dd, const orc_matrix *m, double *x,
+                const double *y, const int *indc)
+{
+    const size_t nij = (size_t)ni * nj;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (indc[n] == -1) continue;
+                x[n] = y[n] - m->ali[n] * dd[n - 1] * x[n - 1]
+                            - m->alj[n] * dd[n - ni] * x[n - ni]
+                            - m->alk[n] * dd[n - nij] * x[n - nij];
+            }
+    for (int k = nk - 1; k >= 2; k--)
+        for (int j = nj - 1; j >= 2; j--)
+            for (int i = ni - 1; i >= 2; i--) {
+                size_t n = ID(i, j, k);
+                if (indc[n] == -1) continue;
+                x[n] = (x[n] - m->aui[n] * x[n + 1]
+                             - m->auj[n] * x[n + ni]
+                             - m->auk[n] * x[n + nij]) * dd[n];
+            }
+}
+
+/* y = A x.  vf_mzax.f:42-56. */
+void orc_mzax(int ni, int nj, int nk, const orc_matrix *m, const double *x, double *y,
+              const int *indc)
+{
+    const size_t nij = (size_t)ni * nj;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (indc[n] == -1) continue;
+                y[n] = m->ad[n] * x[n] + m->ali[n] * x[n - 1] + m->alj[n] * x[n - ni]
+                     + m->alk[n] * x[n - nij] + m->aui[n] * x[n + 1] + m->auj[n] * x[n + ni]
+                     + m->auk[n] * x[n + nij];
+            }
+}
+
+/* Masked dot product, lexicographic order.  vf_mzip.f:32-43. */
+double orc_mzip(int ni, int nj, int nk, const double *x, const double *y, const int *indc)
+{
+    const size_t nij = (size_t)ni * nj;
+    double sum = 0.0;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (indc[n] != -1) sum = sum + x[n] * y[n];
+            }
+    return sum;
+}
+
+/* MILU-preconditioned BiCGSTAB on M^-1 A x = M^-1 b.  vf_m1bcgs.f:78-262. */
+int orc_m1bcgs(int ni, int nj, int nk, const orc_matrix *m, double *x0, const int *indc,
+               double cgpara, double cgepsa, double cgepsr, double cgdiv, int icgtyp, int icgmax,
+               double *bnorm_out, double *xnorm_out, int *iter_out)
+{
+    const size_t nij = (size_t)ni * nj;
+    const size_t nc = nij * (size_t)nk;
+    double *w = (double *)calloc(8 * nc, sizeof(double));
+    if (!w) return -1;
+    double *dd = w, *xw = w + nc, *yy = w + 2 * nc, *r0 = w + 3 * nc, *rr = w + 4 * nc;
+    double *pp = w + 5 * nc, *ap = w + 6 * nc, *ae = w + 7 * nc;
+    memcpy(xw, x0, nc * sizeof(double));
+    int iter = -1;
+    double bnorm = 0.0, xnorm = 0.0, s = 0.0;
+    int isw = orc_mzdcmp(ni, nj, nk, m, dd, cgpara, indc, icgtyp);
+    if (isw == 0) goto done_noconv;
+    {
+        double epsa2 = cgepsa * cgepsa, epsr2 = cgepsr * cgepsr, div2 = cgdiv * cgdiv;
+        orc_mzminv(ni, nj, nk, dd, m, yy, m->bb, indc);
+        bnorm = orc_mzip(ni, nj, nk, yy, yy, indc);
+        iter = 0;
+        orc_mzax(ni, nj, nk, m, xw, yy, indc);
+        for (int k = 2; k <= nk - 1; k++)
+            for (int j = 2; j <= nj - 1; j++)
+                for (int i = 2; i <= ni - 1; i++) {
+                    size_t n = ID(i, j, k);
+                    if (indc[n] != -1) yy[n] = m->bb[n] - yy[n];
+                }
+        orc_mzminv(ni, nj, nk, dd, m, r0, yy, indc);
+        s = orc_mzip(ni, nj, nk, r0, r0, indc);
+        xnorm = s;
+        if (xnorm <= bnorm * epsr2 + epsa2) goto converged;
+        for (size_t n = 0; n < nc; n++)
+            if (indc[n] != -1) { rr[n] = r0[n]; pp[n] = r0[n]; }
+        for (;;) {
+            iter = iter + 1;
+            if (iter > icgmax) break;
+            orc_mzax(ni, nj, nk, m, pp, yy, indc);
+            orc_mzminv(ni, nj, nk, dd, m, ap, yy, indc);
+            double w1 = orc_mzip(ni, nj, nk, r0, ap, indc);
+            double alpha = s / w1;
+            for (size_t n = 0; n < nc; n++)
+                if (indc[n] != -1) rr[n] = rr[n] - alpha * ap[n];
+            orc_mzax(ni, nj, nk, m, rr, yy, indc);
+            orc_mzminv(ni, nj, nk, dd, m, ae, yy, indc);
+            w1 = orc_mzip(ni, nj, nk, rr, ae, indc);
+            double w2 = orc_mzip(ni, nj, nk, ae, ae, indc);
+            double omega = w1 / w2;
+            for (size_t n = 0; n < nc; n++)
+                if (indc[n] != -1) {
+                    xw[n] = xw[n] + (alpha * pp[n] + omega * rr[n]);
+                    rr[n] = rr[n] - omega * ae[n];
+                }
+            xnorm = orc_mzip(ni, nj, nk, rr, rr, indc);
+            if (xnorm <= bnorm * epsr2 + epsa2) break;
+            if (xnorm > div2) { iter = -2; break; }
+            w1 = orc_mzip(ni, nj, nk, r0, rr, indc);
+            double beta = w1 * alpha / (s * omega);
+            for (size_t n = 0; n < nc; n++)
+                if (indc[n] != -1) pp[n] = rr[n] + beta * (pp[n] - omega * ap[n]);
+            s = w1;
+        }
+    }
+converged:
+    bnorm = sqrt(bnorm);
+    xnorm = sqrt(xnorm);
+    if (iter >= 0) {
+        for (int k = 2; k <= nk - 1; k++)
+            for (int j = 2; j <= nj - 1; j++)
+                for (int i = 2; i <= ni - 1; i++) {
+                    size_t n = ID(i, j, k);
+                    if (indc[n] != -1) x0[n] = xw[n];
+                }
+    }
+done_noconv:
+    free(w);
+    *bnorm_out = bnorm;
+    *xnorm_out = xnorm;
+    *iter_out = iter;
+    return 0;
+}
+
+/* ======================================================================
+ * Phase (d): correction and boundary values
+ * ====================================================================== */
+
+/* vf_vmodif.f:65-210 (wave-boundary branches excluded). */
+int orc_vmodif(orc_state *s, const double *pt)
+{
+    DECL_DIMS(s);
+    const double *ggv = s->ggv, *glv = s->glv;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni; i++) {
+                size_t n = ID(i, j, k);
+                if (s->nf[n - 1] * s->nf[n] != 0) continue;
+                int is = s->indx[n];
+                if (is == 0) {
+                    double gv = XX(6, i) * (XX(2, i) * ggv[n] / glv[n] + XX(2, i - 1) * ggv[n - 1] / glv[n - 1]);
+                    s->uu[n] = s->uu[n] + XX(5, i) * gv * (pt[n] - pt[n - 1]);
+                } else if (is >= 1 && INDB(3, is) == 4) {
+                    if (INDB(2, is) == 1)
+                        s->uu[n] = s->uu[n] + 2.0 * XX(4, i) * ggv[n] / glv[n] * pt[n];
+                    else
+                        s->uu[n] = s->uu[n] - 2.0 * XX(4, i - 1) * ggv[n - 1] / glv[n - 1] * pt[n - 1];
+                    s->bcu[is - 1] = s->uu[n];
+                }
+            }
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->nf[n - ni] * s->nf[n] != 0) continue;
+                int is = s->indy[n];
+                if (is == 0) {
+                    double gv = YY(6, j) * (YY(2, j) * ggv[n] / glv[n] + YY(2, j - 1) * ggv[n - ni] / glv[n - ni]);
+                    s->vv[n] = s->vv[n] + YY(5, j) * gv * (pt[n] - pt[n - ni]);
+                } else if (is >= 1 && INDB(3, is) == 4) {
+                    if (INDB(2, is) == 3)
+                        s->vv[n] = s->vv[n] + 2.0 * YY(4, j) * ggv[n] / glv[n] * pt[n];
+                    else
+                        s->vv[n] = s->vv[n] - 2.0 * YY(4, j - 1) * ggv[n - ni] / glv[n - ni] * pt[n - ni];
+                    s->bcv[is - 1] = s->vv[n];
+                }
+            }
+    for (int k = 2; k <= nk; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->nf[n - nij] * s->nf[n] != 0) continue;
+                int is = s->indz[n];
+                if (is == 0) {
+                    double gv = ZZ(6, k) * (ZZ(2, k) * ggv[n] / glv[n] + ZZ(2, k - 1) * ggv[n - nij] / glv[n - nij]);
+                    s->ww[n] = s->ww[n] + ZZ(5, k) * gv * (pt[n] - pt[n - nij]);
+                } else if (is >= 1 && INDB(3, is) == 4) {
+                    if (INDB(2, is) == 5)
+                        s->ww[n] = s->ww[n] + 2.0 * ZZ(4, k) * ggv[n] / glv[n] * pt[n];
+                    else
+                        s->ww[n] = s->ww[n] - 2.0 * ZZ(4, k - 1) * ggv[n - nij] / glv[n - nij] * pt[n - nij];
+                    s->bcw[is - 1] = s->ww[n];
+                }
+            }
+    const double rt = s->rho0 / s->dtnow;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->indc[n] != -1) s->pp[n] = s->pp[n] - rt * pt[n];
+            }
+    return 0;
+}
+
+/* Tangential velocity on faces that do not touch fluid.  vf_bsuwt.f:47-463.
+ * Face of component a between cells n-sa and n; the surface orientation of
+ * each side selects a neighbouring face along axis d (1..6 -> -x,+x,-y,+y,
+ * -z,+z) whose value is extrapolated (IBSUW0=0) and blended by distance. */
+static void bsuwt_axis(orc_state *s, int a)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const size_t sa = st[a];
+    const int *ind = a == 0 ? s->indx : (a == 1 ? s->indy : s->indz);
+    double *q = a == 0 ? s->uu : (a == 1 ? s->vv : s->ww);
+    int i0 = 2, j0 = 2, k0 = 2;
+    if (a == 0) i0 = 3;
+    if (a == 1) j0 = 3;
+    if (a == 2) k0 = 3;
+    for (int k = k0; k <= nk - 1; k++)
+        for (int j = j0; j <= nj - 1; j++)
+            for (int i = i0; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (ind[n] != 0) continue;
+                int nl = s->nf[n - sa], nr = s->nf[n];
+                if (nl == 8 && nr == 8) {
+                    q[n] = 0.0;
+                } else if (nl != 0 && nr != 0) {
+                    double dl = 0.0, vl = 0.0, dr = 0.0, vr = 0.0;
+                    int idx[3] = {i, j, k};
+                    /* orientations in the reference's order; skip those normal to a */
+                    for (int t = 1; t <= 6; t++) {
+                        int d = (t - 1) / 2;
+                        if (d == a) continue;
+                        if (!(nl == t || nr == t)) continue;
+                        int plus = (t % 2 == 0);
+                        size_t sd = st[d];
+                        int id = idx[d];
+                        double v0 = plus ? q[n + sd] : q[n - sd];
+                        if (s->ibsuw0 == 0) {
+                            size_t n1 = plus ? n + sd : n - sd;
+                            size_t n2 = plus ? n + 2 * sd : n - 2 * sd;
+                            if (s->nf[n1 - sa] != -1 && s->nf[n1] != -1) {
+                                if (s->nf[n2 - sa] != -1 && s->nf[n2] != -1) {
+                                    if (s->nf[n2 - sa] == 0 || s->nf[n2] == 0) {
+                                        if (plus)
+                                            v0 = v0 + (v0 - q[n2]) * MET(s, d, 3, id + 1) / MET(s, d, 3, id + 2);
+                                        else
+                                            v0 = v0 + (v0 - q[n2]) * MET(s, d, 3, id) / MET(s, d, 3, id - 1);
+                                    }
+                                }
+                            }
+                        }
+                        double dist = plus ? MET(s, d, 3, id + 1) : MET(s, d, 3, id);
+                        if (nl == t) { dl = dist; vl = v0; }
+                        if (nr == t) { dr = dist; vr = v0; }
+                    }
+                    if (nl == 8) q[n] = vr;
+                    else if (nr == 8) q[n] = vl;
+                    else q[n] = (dr * vl + dl * vr) / (dr + dl);
+                }
+            }
+}
+
+/* vf_bsuwt2.f:39-117: tangential velocities on the gas side of a surface
+ * cell copied across a wall corner. */
+static void bsuwt2(orc_state *s)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    double *vel[3] = {s->uu, s->vv, s->ww};
+    const int *ind[3] = {s->indx, s->indy, s->indz};
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k;
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        size_t n = ID(i, j, k);
+        int nf0 = s->nf[n];
+        if (nf0 < 1 || nf0 > 6) continue;
+        int d = (nf0 - 1) / 2;                 /* normal axis */
+        size_t lb = (nf0 % 2 == 1) ? n - st[d] : n + st[d];
+        /* the two tangential axes in the reference's order (x before y before z) */
+        for (int t = 0; t < 3; t++) {
+            if (t == d) continue;
+            size_t s_t = st[t];
+            size_t l1 = n, l2 = n;
+            if (s->nf[n - s_t] == 8 && ind[t][lb] != 0) l2 = l2 + s_t;
+            if (s->nf[n + s_t] == 8 && ind[t][lb + s_t] != 0) l1 = l1 + s_t;
+            vel[t][l1] = vel[t][l2];
+        }
+    }
+}
+
+/* vf_bsuwn.f:43-76: normal velocity on the gas-side face of a surface cell
+ * from discrete continuity. */
+static void bsuwn(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k;
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        size_t n = ID(i, j, k);
+        int nf0 = s->nf[n];
+        double fxp = s->ggx[n + 1] * s->uu[n + 1];
+        double fxm = s->ggx[n] * s->uu[n];
+        double fyp = s->ggy[n + ni] * s->vv[n + ni];
+        double fym = s->ggy[n] * s->vv[n];
+        double fzp = s->ggz[n + nij] * s->ww[n + nij];
+        double fzm = s->ggz[n] * s->ww[n];
+        if (nf0 == 5)
+            s->ww[n + nij] = (fzm - ZZ(2, k) * (XX(4, i) * (fxp - fxm) + YY(4, j) * (fyp - fym))) / s->ggz[n + nij];
+        else if (nf0 == 6)
+            s->ww[n] = (fzp + ZZ(2, k) * (XX(4, i) * (fxp - fxm) + YY(4, j) * (fyp - fym))) / s->ggz[n];
+        else if (nf0 == 3)
+            s->vv[n + ni] = (fym - YY(2, j) * (XX(4, i) * (fxp - fxm) + ZZ(4, k) * (fzp - fzm))) / s->ggy[n + ni];
+        else if (nf0 == 4)
+            s->vv[n] = (fyp + YY(2, j) * (XX(4, i) * (fxp - fxm) + ZZ(4, k) * (fzp - fzm))) / s->ggy[n];
+        else if (nf0 == 1)
+            s->uu[n + 1] = (fxm - XX(2, i) * (YY(4, j) * (fyp - fym) + ZZ(4, k) * (fzp - fzm))) / s->ggx[n + 1];
+        else if (nf0 == 2)
+            s->uu[n] = (fxp + XX(2, i) * (YY(4, j) * (fyp - fym) + ZZ(4, k) * (fzp - fzm))) / s->ggx[n];
+    }
+}
+
+/* Tangential wall velocities on slip/free boundaries.  vf_bwuwt.f:42-176
+ * (types 1,4; 2,3 leave them; 7 radiation does nothing). */
+static int bwuwt(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int l = 1; l <= s->numb; l++) {
+        int i, j, k;
+        decode(s, INDB(1, l), &i, &j, &k);
+        int ns = INDB(2, l), ib = INDB(3, l);
+        size_t n = ID(i, j, k);
+        const double *u = s->uu, *v = s->vv, *w = s->ww;
+        if (ib == 1 || ib == 4) {
+            switch (ns) {
+            case 1: s->bcv[l - 1] = (v[n] + v[n + ni]) * 0.5; s->bcw[l - 1] = (w[n] + w[n + nij]) * 0.5; break;
+            case 2: s->bcv[l - 1] = (v[n - 1] + v[n - 1 + ni]) * 0.5; s->bcw[l - 1] = (w[n - 1] + w[n - 1 + nij]) * 0.5; break;
+            case 3: s->bcu[l - 1] = (u[n] + u[n + 1]) * 0.5; s->bcw[l - 1] = (w[n] + w[n + nij]) * 0.5; break;
+            case 4: s->bcu[l - 1] = (u[n - ni] + u[n - ni + 1]) * 0.5; s->bcw[l - 1] = (w[n - ni] + w[n - ni + nij]) * 0.5; break;
+            case 5: s->bcu[l - 1] = (u[n] + u[n + 1]) * 0.5; s->bcv[l - 1] = (v[n] + v[n + ni]) * 0.5; break;
+            case 6: s->bcu[l - 1] = (u[n - nij] + u[n - nij + 1]) * 0.5; s->bcv[l - 1] = (v[n - nij] + v[n - nij + ni]) * 0.5; break;
+            }
+        } else if (ib == 2 || ib == 3 || ib == 7) {
+        } else {
+            return -11; /* log-law and wave boundaries are outside the restated path */
+        }
+    }
+    return 0;
+}
+
+/* vf_bwpp.f:28-47 */
+static void bwpp(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int l = 1; l <= s->numb; l++) {
+        int i, j, k;
+        decode(s, INDB(1, l), &i, &j, &k);
+        size_t n = ID(i, j, k);
+        switch (INDB(2, l)) {
+        case 1: s->bcp[l - 1] = s->pp[n]; break;
+        case 2: s->bcp[l - 1] = s->pp[n - 1]; break;
+        case 3: s->bcp[l - 1] = s->pp[n]; break;
+        case 4: s->bcp[l - 1] = s->pp[n - ni]; break;
+        case 5: s->bcp[l - 1] = s->pp[n]; break;
+        case 6: s->bcp[l - 1] = s->pp[n - nij]; break;
+        }
+    }
+}
+
+/* vf_v1cal.f:293-297 */
+int orc_post_bc(orc_state *s)
+{
+    bsuwt_axis(s, 0);
+    bsuwt_axis(s, 1);
+    bsuwt_axis(s, 2);
+    bsuwt2(s);
+    bsuwn(s);
+    int rc = bwuwt(s);
+    if (rc) return rc;
+    bwpp(s);
+    return 0;
+}
+
+/* VF_V1CAL (vf_v1cal.f:111-297) for ILOOP=1, static porosity. */
+int orc_v1cal(orc_state *s, double *pt)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    int rc = orc_predictor(s);
+    if (rc) return rc;
+    double *w = (double *)calloc(8 * nc, sizeof(double));
+    if (!w) return -1;
+    orc_matrix m = {w, w + nc, w + 2 * nc, w + 3 * nc, w + 4 * nc, w + 5 * nc, w + 6 * nc, w + 7 * nc};
+    rc = orc_vpcoef(s, &m);
+    if (rc) { free(w); return rc; }
+    /* VF_VPSOL: PT=0 then MILU-BiCGSTAB (vf_vpsol.f:55-76) */
+    memset(pt, 0, nc * sizeof(double));
+    orc_m1bcgs(ni, nj, nk, &m, pt, s->indc, s->cgpara, s->cgepsa, s->cgepsr, s->cgdiv,
+               s->icgtyp, s->icgmax, &s->cgbnrm, &s->cgxnrm, &s->icgitr);
+    free(w);
+    if (s->icgitr > s->icgmax) return -30;
+    if (s->icgitr == -2) return -31;
+    if (s->icgitr == -1) return -32;
+    rc = orc_vmodif(s, pt);
+    if (rc) return rc;
+    return orc_post_bc(s);
+}
+
+/* ======================================================================
+ * Phase (c): VOF
+ * ====================================================================== */
+
+/* Donor-acceptor F flux through faces normal to axis a.  vf_fconv.f:56-200. */
+static void fconv_axis(orc_state *s, int a, double *flf)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const size_t sa = st[a];
+    const int *ind = a == 0 ? s->indx : (a == 1 ? s->indy : s->indz);
+    const double *gga = a == 0 ? s->ggx : (a == 1 ? s->ggy : s->ggz);
+    const double *q = a == 0 ? s->uu : (a == 1 ? s->vv : s->ww);
+    int i1 = ni - 1, j1 = nj - 1, k1 = nk - 1;
+    if (a == 0) i1 = ni;
+    if (a == 1) j1 = nj;
+    if (a == 2) k1 = nk;
+    /* tangential orientations that let FAD be replaced by FD */
+    for (int k = 2; k <= k1; k++)
+        for (int j = 2; j <= j1; j++)
+            for (int i = 2; i <= i1; i++) {
+                size_t n = ID(i, j, k);
+                if (ind[n] == -1) continue;
+                int idx[3] = {i, j, k};
+                double c = s->dtnow * gga[n] * q[n];
+                double ac = fabs(c);
+                int lsn;
+                size_t la, ld, ldm;
+                int ild;
+                if (c >= 0.0) { lsn = 1; la = n; ld = n - sa; ldm = n - 2 * sa; ild = idx[a] - 1; }
+                else { lsn = -1; la = n - sa; ld = n; ldm = n + sa; ild = idx[a]; }
+                int nn = s->nf[ld];
+                double fd, fdmw, v, fa;
+                if (nn != -1) {
+                    fd = s->ff[ld];
+                    fdmw = 1.0;
+                    if (s->nf[ldm] != -1) fdmw = s->ff[ldm];
+                    v = MET(s, a, 2, ild) * s->ggv[ld];
+                } else {
+                    fd = s->bcf[ind[n] - 1];
+                    fdmw = 1.0;
+                    v = MET(s, a, 2, ild);
+                }
+                if (s->nf[la] != -1) fa = s->ff[la];
+                else fa = fd;
+                double fad = fa;
+                int tang = (nn == -1) || ((nn >= 1 && nn <= 6) && ((nn - 1) / 2 != a));
+                if (tang) {
+                    if (fdmw != 0.0 && fa != 0.0) fad = fd;
+                }
+                double fdm = dmax(fdmw, fd);
+                if (fdmw < s->flower && fa < s->flower) fdm = dmax(fdm, 0.1);
+                double cfx = dmax((fdm - fad) * ac - (fdm - fd) * v, 0.0);
+                double fvx = fad * ac + cfx;
+                double fvm = fd * v;
+                flf[n] = flf[n] - (double)lsn * dmin(fvx, fvm);
+                if (fvx > fvm) s->nlim[ld] = 1;
+            }
+}
+
+/* vf_feuler.f:43-58 (static porosity: G0 = GGV, LOOPS = 1, QF = 0) */
+static void feuler(orc_state *s, const double *flu, const double *flv, const double *flw)
+{
+    DECL_DIMS(s);
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->nf[n] == -1) continue;
+                double g0 = s->ggv[n] - (s->ggv[n] - s->ggv[n]) * (double)s->loops;
+                double df = (XX(4, i) * (flu[n + 1] - flu[n])
+                           + YY(4, j) * (flv[n + ni] - flv[n])
+                           + ZZ(4, k) * (flw[n + nij] - flw[n])
+                           + s->dtnow * 0.0);
+                s->ff[n] = (g0 * s->ff[n] + df) / s->ggv[n];
+            }
+}
+
+static void surf_pair(const orc_state *s, int i, int j, int k, int nn, int *e, int *f)
+{
+    /* (IEC,JEC,KEC) gas side and (IFC,JFC,KFC) fluid side; vf_fmod1.f:64-86 */
+    e[0] = f[0] = i; e[1] = f[1] = j; e[2] = f[2] = k;
+    (void)s;
+    switch (nn) {
+    case 5: e[2] = k + 1; f[2] = k - 1; break;
+    case 6: e[2] = k - 1; f[2] = k + 1; break;
+    case 3: e[1] = j + 1; f[1] = j - 1; break;
+    case 4: e[1] = j - 1; f[1] = j + 1; break;
+    case 1: e[0] = i + 1; f[0] = i - 1; break;
+    case 2: e[0] = i - 1; f[0] = i + 1; break;
+    }
+}
+
+/* Surface-cell redistribution.  vf_fmod1.f:46-191. */
+static void fmod1(orc_state *s)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    memset(s->nlim, 0, nc * sizeof(int));
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k, e[3], f[3];
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        surf_pair(s, i, j, k, s->nf[ID(i, j, k)], e, f);
+        s->nlim[ID(e[0], e[1], e[2])] += 1;
+        s->nlim[ID(f[0], f[1], f[2])] += 1;
+    }
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k, e[3], f[3];
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        surf_pair(s, i, j, k, s->nf[ID(i, j, k)], e, f);
+        size_t ne = ID(e[0], e[1], e[2]), n = ID(i, j, k), nf_ = ID(f[0], f[1], f[2]);
+        double ve = XX(2, e[0]) * YY(2, e[1]) * ZZ(2, e[2]) * s->ggv[ne];
+        double vs = XX(2, i) * YY(2, j) * ZZ(2, k) * s->ggv[n];
+        double vf = XX(2, f[0]) * YY(2, f[1]) * ZZ(2, f[2]) * s->ggv[nf_];
+        double ss = vs * s->ff[n];
+        if (s->nlim[ne] == 1) ss = ss + ve * s->ff[ne];
+        if (s->nlim[nf_] == 1) ss = ss + vf * s->ff[nf_];
+        double ds;
+        if (s->nlim[nf_] == 1) {
+            ds = dmin(vf, ss);
+            ss = ss - ds;
+            s->ff[nf_] = ds / vf;
+        }
+        ds = dmin(vs, ss);
+        ss = ss - ds;
+        s->ff[n] = ds / vs;
+        if (s->nlim[ne] == 1) {
+            ds = dmin(ve, ss);
+            ss = ss - ds;
+            s->ff[ne] = ds / ve;
+        }
+        if (ss > 0.0) {
+            if (s->nlim[ne] == 1) s->ff[ne] = s->ff[ne] + ss / ve;
+            else s->ff[n] = s->ff[n] + ss / vs;
+        }
+    }
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k, e[3], f[3];
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        surf_pair(s, i, j, k, s->nf[ID(i, j, k)], e, f);
+        size_t ne = ID(e[0], e[1], e[2]), n = ID(i, j, k), nf_ = ID(f[0], f[1], f[2]);
+        double ve = XX(2, e[0]) * YY(2, e[1]) * ZZ(2, e[2]) * s->ggv[ne];
+        double vs = XX(2, i) * YY(2, j) * ZZ(2, k) * s->ggv[n];
+        double vf = XX(2, f[0]) * YY(2, f[1]) * ZZ(2, f[2]) * s->ggv[nf_];
+        if (s->ff[n] > 1.0) {
+            s->ff[ne] = s->ff[ne] + vs * (s->ff[n] - 1.0) / ve;
+            s->ff[n] = 1.0;
+        }
+        if (s->ff[n] < 0.0) {
+            s->ff[nf_] = s->ff[nf_] + vs * s->ff[n] / vf;
+            s->ff[n] = 0.0;
+        }
+    }
+}
+
+/* Clip F and volume sums.  vf_fcut01.f:34-55. */
+static void fcut01(orc_state *s)
+{
+    DECL_DIMS(s);
+    double fsum = 0.0, fcut = 0.0;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->nf[n] == -1) continue;
+                double v = XX(2, i) * YY(2, j) * ZZ(2, k) * s->ggv[n];
+                if (s->ff[n] > s->fupper) {
+                    fcut = fcut + (s->ff[n] - 1.0) * v;
+                    s->ff[n] = 1.0;
+                } else if (s->ff[n] < s->flower) {
+                    fcut = fcut + (s->ff[n] - 0.0) * v;
+                    s->ff[n] = 0.0;
+                }
+                fsum = fsum + s->ff[n] * v;
+            }
+    s->fsum = fsum;
+    s->fcut = fcut;
+}
+
+/* vf_bwff.f:33-87 (fix / free) */
+static int bwff(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int l = 1; l <= s->numb; l++) {
+        int i, j, k;
+        decode(s, INDB(1, l), &i, &j, &k);
+        int ns = INDB(2, l), ib = INDB(4, l);
+        size_t n = ID(i, j, k);
+        if (ib == 1) {
+        } else if (ib == 2) {
+            switch (ns) {
+            case 1: s->bcf[l - 1] = s->ff[n]; break;
+            case 2: s->bcf[l - 1] = s->ff[n - 1]; break;
+            case 3: s->bcf[l - 1] = s->ff[n]; break;
+            case 4: s->bcf[l - 1] = s->ff[n - ni]; break;
+            case 5: s->bcf[l - 1] = s->ff[n]; break;
+            case 6: s->bcf[l - 1] = s->ff[n - nij]; break;
+            }
+        } else {
+            return -12;
+        }
+    }
+    return 0;
+}
+
+/* Normal velocity on the gas side of a surface cell from its tangential
+ * neighbours.  vf_bsuwn3.f:37-335. */
+static void bsuwn3(orc_state *s)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    double *vel[3] = {s->uu, s->vv, s->ww};
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k;
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        size_t n = ID(i, j, k);
+        int nf0 = s->nf[n];
+        if (nf0 < 1 || nf0 > 6) continue;
+        int idx[3] = {i, j, k};
+        int d = (nf0 - 1) / 2;
+        int plus = (nf0 % 2 == 1); /* fluid on the minus side -> gas-side face is n+sd */
+        size_t sd = st[d];
+        double *q = vel[d];
+        double d0 = MET(s, d, 2, idx[d]);
+        double v0 = plus ? q[n] : q[n + sd];
+        size_t l1 = plus ? n + sd : n - sd;  /* neighbour cell across the gas-side face */
+        size_t wface = plus ? n + sd : n;    /* the face that is written */
+        int is_[4] = {0, 0, 0, 0};
+        double dd[4] = {0, 0, 0, 0}, vv_[4] = {0, 0, 0, 0};
+        int c = 0;
+        for (int t = 0; t < 3; t++) {
+            if (t == d) continue;
+            for (int pm = 0; pm < 2; pm++) {
+                size_t ls = pm ? n + st[t] : n - st[t];
+                size_t ls1 = pm ? l1 + st[t] : l1 - st[t];
+                if (s->nf[ls] == 0 || s->nf[ls1] == 0) {
+                    if (s->nf[ls] != -1 && s->nf[ls1] != -1) {
+                        is_[c] = 1;
+                        dd[c] = pm ? MET(s, t, 3, idx[t] + 1) : MET(s, t, 3, idx[t]);
+                        vv_[c] = pm ? q[wface + st[t]] : q[wface - st[t]];
+                    }
+                }
+                c++;
+            }
+        }
+        int iss = is_[0] + is_[1] + is_[2] + is_[3];
+        if (iss <= 0) {
+            if (plus) {
+                if (s->ibsuw0 == 0 && s->nf[n - 2 * sd] == 0)
+                    v0 = v0 + (v0 - q[n - sd]) * MET(s, d, 2, idx[d]) / MET(s, d, 2, idx[d] - 1);
+            } else {
+                if (s->ibsuw0 == 0 && s->nf[n + 2 * sd] == 0)
+                    v0 = v0 + (v0 - q[n + 2 * sd]) * MET(s, d, 2, idx[d]) / MET(s, d, 2, idx[d] + 1);
+            }
+            q[wface] = v0;
+        } else {
+            double ds = d0 + dd[0] + dd[1] + dd[2] + dd[3];
+            q[wface] = ((ds - d0) * v0 + (ds - dd[0]) * vv_[0] + (ds - dd[1]) * vv_[1]
+                        + (ds - dd[2]) * vv_[2] + (ds - dd[3]) * vv_[3]) / ds / (double)iss;
+        }
+    }
+}
+
+/* Velocities of gas cells boxed in by surface cells.  vf_bsuwem.f:36-94. */
+static void bsuwem(orc_state *s)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    int *nl = s->nlim;
+    memset(nl, 0, nc * sizeof(int));
+    for (int l = 1; l <= s->nums; l++) {
+        int i, j, k;
+        decode(s, s->inds[l - 1], &i, &j, &k);
+        size_t n = ID(i, j, k);
+        int nn = s->nf[n];
+        if (nn == 5 || nn == 6) {
+            nl[n + 1] += 1; nl[n - 1] += 1; nl[n + ni] += 1; nl[n - ni] += 1;
+        } else if (nn == 3 || nn == 4) {
+            nl[n + 1] += 1; nl[n - 1] += 1; nl[n + nij] += 1; nl[n - nij] += 1;
+        } else {
+            nl[n + ni] += 1; nl[n - ni] += 1; nl[n + nij] += 1; nl[n - nij] += 1;
+        }
+    }
+    const int *nf = s->nf;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (!(nl[n] >= 2 && nf[n] == 8)) continue;
+                if (nf[n - 1] == 8 && nl[n - 1] < 2) { if (nf[n + 1] != 8) s->uu[n] = s->uu[n + 1]; }
+                if (nf[n + 1] == 8 && nl[n + 1] < 2) { if (nf[n - 1] != 8) s->uu[n + 1] = s->uu[n]; }
+                if (nf[n - ni] == 8 && nl[n - ni] < 2) { if (nf[n + ni] != 8) s->vv[n] = s->vv[n + ni]; }
+                if (nf[n + ni] == 8 && nl[n + ni] < 2) { if (nf[n - ni] != 8) s->vv[n + ni] = s->vv[n]; }
+                if (nf[n - nij] == 8 && nl[n - nij] < 2) { if (nf[n + nij] != 8) s->ww[n] = s->ww[n + nij]; }
+                if (nf[n + nij] == 8 && nl[n + nij] < 2) { if (nf[n - nij] != 8) s->ww[n + nij] = s->ww[n]; }
+            }
+}
+
+/* F value of the face neighbour across the face with index ind (BCF on a
+ * boundary face); used by vf_fnfini.f and vf_fnfprv.f. */
+static double fnb(const orc_state *s, int ind, size_t nb)
+{
+    return ind >= 1 ? s->bcf[ind - 1] : s->ff[nb];
+}
+
+/* Cell flags fluid(0) / surface(7) / gas(8) and the surface list.
+ * vf_fnfini.f:36-189. */
+static void fnfini(orc_state *s)
+{
+    DECL_DIMS(s);
+    int *nf = s->nf;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 1; j <= nj; j++)
+            for (int i = 1; i <= ni; i++) {
+                size_t n = ID(i, j, k);
+                if (nf[n] != -1) nf[n] = 0;
+            }
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (nf[n] == -1 || !(s->ff[n] < s->flower)) continue;
+                int ix1 = s->indx[n], ix2 = s->indx[n + 1];
+                int iy1 = s->indy[n], iy2 = s->indy[n + ni];
+                int iz1 = s->indz[n], iz2 = s->indz[n + nij];
+                double fx1 = fnb(s, ix1, n - 1), fx2 = fnb(s, ix2, n + 1);
+                double fy1 = fnb(s, iy1, n - ni), fy2 = fnb(s, iy2, n + ni);
+                double fz1 = fnb(s, iz1, n - nij), fz2 = fnb(s, iz2, n + nij);
+                if (!(fz1 > s->fupper && iz1 == 0) && !(fy1 > s->fupper && iy1 == 0)
+                    && !(fy2 > s->fupper && iy2 == 0) && !(fx1 > s->fupper && ix1 == 0)
+                    && !(fx2 > s->fupper && ix2 == 0) && !(fz2 > s->fupper && iz2 == 0))
+                    nf[n] = 8;
+            }
+    for (;;) {
+        int ns = 0;
+        for (int k = 2; k <= nk - 1; k++)
+            for (int j = 2; j <= nj - 1; j++)
+                for (int i = 2; i <= ni - 1; i++) {
+                    size_t n = ID(i, j, k);
+                    if (nf[n] == -1 || nf[n] == 8) continue;
+                    if (nf[n + nij] == 8 || nf[n + ni] == 8 || nf[n - ni] == 8 || nf[n + 1] == 8
+                        || nf[n - 1] == 8 || nf[n - nij] == 8) {
+                        s->inds[ns++] = (int)n + 1;
+                        nf[n] = 7;
+                    }
+                }
+        int nums = 0;
+        for (int l = 0; l < ns; l++) {
+            int i00 = s->inds[l];
+            size_t n = (size_t)i00 - 1;
+            if (nf[n + nij] != 0 && nf[n + ni] != 0 && nf[n - ni] != 0 && nf[n + 1] != 0
+                && nf[n - 1] != 0 && nf[n - nij] != 0)
+                nf[n] = 8;
+            else
+                s->inds[nums++] = i00;
+        }
+        s->nums = nums;
+        int nch = 0;
+        for (int l = 0; l < nums; l++) {
+            size_t n = (size_t)s->inds[l] - 1;
+            if ((nf[n - nij] != 0 || nf[n + nij] != 8) && (nf[n - 1] != 0 || nf[n + 1] != 8)
+                && (nf[n + 1] != 0 || nf[n - 1] != 8) && (nf[n - ni] != 0 || nf[n + ni] != 8)
+                && (nf[n + ni] != 0 || nf[n - ni] != 8) && (nf[n + nij] != 0 || nf[n - nij] != 8)) {
+                nf[n] = -7;
+                nch++;
+            }
+        }
+        if (nch <= 0) break;
+        for (int l = 0; l < nums; l++) {
+            size_t n = (size_t)s->inds[l] - 1;
+            if (nf[n] == -7) nf[n] = 8;
+        }
+    }
+}
+
+/* Surface orientation.  vf_fnfprv.f:34-223. */
+static int fnfprv(orc_state *s)
+{
+    DECL_DIMS(s);
+    const size_t st[3] = {1, (size_t)ni, nij};
+    const int *ind[3] = {s->indx, s->indy, s->indz};
+    int *nf = s->nf;
+    /* candidate order: z-, z+, y-, y+, x-, x+ -> NF 5, 6, 3, 4, 1, 2 */
+    static const int order[6] = {5, 6, 3, 4, 1, 2};
+    for (int l = 0; l < s->nums; l++) {
+        size_t n = (size_t)s->inds[l] - 1;
+        double fm = -1.0;
+        for (int o = 0; o < 6; o++) {
+            int code = order[o];
+            int d = (code - 1) / 2;
+            int minus = (code % 2 == 1);
+            size_t nb = minus ? n - st[d] : n + st[d];     /* fluid neighbour */
+            size_t op = minus ? n + st[d] : n - st[d];     /* must be gas */
+            if (!(nf[nb] == 0 && nf[op] == 8)) continue;
+            double fs = 2.0 * s->ff[nb];
+            for (int t = 0; t < 3; t++) {
+                if (t == d) continue;
+                fs = fs + fnb(s, ind[t][nb], nb - st[t]);
+                fs = fs + fnb(s, ind[t][nb + st[t]], nb + st[t]);
+            }
+            if (fs > fm) { nf[n] = code; fm = fs; }
+        }
+        if (nf[n] == 7) return -40;
+    }
+    return 0;
+}
+
+/* vf_cindc.f:35-50 */
+static void cindc(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                int v = s->nf[n];
+                s->indc[n] = (v == 8 || v == -1) ? -1 : 0;
+            }
+}
+
+/* Pressure of cells that turned from gas to fluid.  vf_bsppfl.f:40-62. */
+static void bsppfl(orc_state *s)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    double *pp0 = (double *)malloc(nc * sizeof(double));
+    memcpy(pp0, s->pp, nc * sizeof(double));
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (!(s->nf[n] == 0 && s->nlim[n] == 8)) continue;
+                double sumf = 0.0, sump = 0.0;
+                for (int kp = -1; kp <= 1; kp++)
+                    for (int jp = -1; jp <= 1; jp++)
+                        for (int ip = -1; ip <= 1; ip++) {
+                            size_t m = ID(i + ip, j + jp, k + kp);
+                            if (s->nf[m] == 0) {
+                                double f = s->ff[m];
+                                sumf = sumf + f;
+                                sump = sump + f * pp0[m];
+                            }
+                        }
+                if (sumf < ZERO) s->pp[n] = 0.0;
+                else s->pp[n] = sump / sumf;
+            }
+    free(pp0);
+}
+
+/* Surface and gas pressures.  vf_bspp.f:36-62 (no air-pressure model). */
+static void bspp(orc_state *s)
+{
+    DECL_DIMS(s);
+    const double pps = 0.0;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                int v = s->nf[n];
+                double psi;
+                switch (v) {
+                case 8: s->pp[n] = 0.0; break;
+                case 5: psi = ZZ(3, k) / (0.5 * ZZ(2, k - 1) + s->ff[n] * ZZ(2, k)); s->pp[n] = (1.0 - psi) * s->pp[n - nij] + psi * pps; break;
+                case 6: psi = ZZ(3, k + 1) / (0.5 * ZZ(2, k + 1) + s->ff[n] * ZZ(2, k)); s->pp[n] = (1.0 - psi) * s->pp[n + nij] + psi * pps; break;
+                case 3: psi = YY(3, j) / (0.5 * YY(2, j - 1) + s->ff[n] * YY(2, j)); s->pp[n] = (1.0 - psi) * s->pp[n - ni] + psi * pps; break;
+                case 4: psi = YY(3, j + 1) / (0.5 * YY(2, j + 1) + s->ff[n] * YY(2, j)); s->pp[n] = (1.0 - psi) * s->pp[n + ni] + psi * pps; break;
+                case 1: psi = XX(3, i) / (0.5 * XX(2, i - 1) + s->ff[n] * XX(2, i)); s->pp[n] = (1.0 - psi) * s->pp[n - 1] + psi * pps; break;
+                case 2: psi = XX(3, i + 1) / (0.5 * XX(2, i + 1) + s->ff[n] * XX(2, i)); s->pp[n] = (1.0 - psi) * s->pp[n + 1] + psi * pps; break;
+                default: break;
+                }
+            }
+}
+
+/* VF_F1CAL (vf_f1cal.f:124-239) for the restated feature set. */
+int orc_f1cal(orc_state *s)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    double *w = (double *)calloc(3 * nc, sizeof(double));
+    if (!w) return -1;
+    double *flfu = w, *flfv = w + nc, *flfw = w + 2 * nc;
+    memset(s->nlim, 0, nc * sizeof(int));
+    fconv_axis(s, 0, flfu);
+    fconv_axis(s, 1, flfv);
+    fconv_axis(s, 2, flfw);
+    feuler(s, flfu, flfv, flfw);
+    free(w);
+    fmod1(s);
+    fcut01(s);
+    int rc = bwff(s);
+    if (rc) return rc;
+    bsuwn3(s);
+    bsuwem(s);
+    memcpy(s->nlim, s->nf, nc * sizeof(int));
+    fnfini(s);
+    rc = fnfprv(s);
+    if (rc) return rc;
+    cindc(s);
+    bsppfl(s);
+    bsuwt_axis(s, 0);
+    bsuwt_axis(s, 1);
+    bsuwt_axis(s, 2);
+    bsuwt2(s);
+    bsuwn3(s);
+    bspp(s);
+    rc = bwuwt(s);
+    if (rc) return rc;
+    bwpp(s);
+    return 0;
+}
+
+/* vf_cdiv00.f:38-56 */
+double orc_cdiv00(const orc_state *s)
+{
+    DECL_DIMS(s);
+    double val = 0.0;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                size_t n = ID(i, j, k);
+                if (s->indc[n] == -1) continue;
+                double vd = YY(2, j) * ZZ(2, k) * (s->ggx[n + 1] * s->uu[n + 1] - s->ggx[n] * s->uu[n])
+                          + XX(2, i) * ZZ(2, k) * (s->ggy[n + ni] * s->vv[n + ni] - s->ggy[n] * s->vv[n])
+                          + XX(2, i) * YY(2, j) * (s->ggz[n + nij] * s->ww[n + nij] - s->ggz[n] * s->ww[n]);
+                val = val + vd * vd;
+            }
+    return sqrt(val);
+}
+
+int orc_step(orc_state *s, double *pt)
+{
+    int rc = orc_v1cal(s, pt);
+    if (rc) return rc;
+    return orc_f1cal(s);
+}

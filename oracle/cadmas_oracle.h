@@ -1,0 +1,87 @@
+/*
+ * cadmas_oracle.h -- CPU restatement of the CADMAS-SURF/3D single-phase
+ * VOF-SMAC time step (plain C, serial, the reference's loop and expression
+ * order).
+ *
+ * TEST INFRASTRUCTURE: this library is the checker for the HIP product path
+ * and the CPU baseline timed beside it.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  It is never linked into the
+ * product library.
+ *
+ * Conventions follow the reference's Fortran (Source code/ of
+ * Simulators/CADMAS-SURF-3D):
+ *   - arrays are column major, (NUMI,NUMJ,NUMK), I fastest, 1-based indices
+ *     in the restatement (Source code/vf_a1main.f:150-151);
+ *   - metric tables XX(6,NUMI), YY(6,NUMJ), ZZ(6,NUMK) (vf_cgrid.f:44-67);
+ *   - boundary list INDB(4,NUMB) (vf_cindb.f);
+ *   - one rank: MYIS=MYJS=2, MYIE=NUMI-1, MYJE=NUMJ-1, MYMIS..MYMJE=1
+ *     (vf_cpara.f:81-98).
+ */
+#ifndef CADMAS_ORACLE_H
+#define CADMAS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_state {
+    int numi, numj, numk, numb;
+    const double *xx, *yy, *zz;
+    double *uu, *vv, *ww, *pp, *ff;
+    const double *anu, *cd0;
+    const double *ggv, *ggx, *ggy, *ggz, *glv, *glx, *gly, *glz;
+    double *bcu, *bcv, *bcw, *bcp, *bcf;
+    const double *bcvi;
+    int *nf;
+    const int *indx, *indy, *indz;
+    int *indc;
+    const int *indb;
+    int *inds;       /* surface-cell list, NUMI*NUMJ*NUMK capacity */
+    int *nlim;       /* VF_F1CAL work (NUMI,NUMJ,NUMK); holds old NF on return */
+    int nums;
+    /* physics and numerics (COMMON /VF_APHYSR/, /VF_ACOMPR/, /VF_ACOMPI/) */
+    double rho0, grz0, anu0, dtnow;
+    double scmvp, vvmax, flower, fupper, plower;
+    double cgpara, cgepsa, cgepsr, cgdiv;
+    int icgtyp, icgmax, ibsuw0, loops;
+    /* per-step outputs */
+    double cgbnrm, cgxnrm, fsum, fcut;
+    int icgitr;
+} orc_state;
+
+/* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */
+typedef struct orc_matrix {
+    double *ad, *ali, *alj, *alk, *aui, *auj, *auk, *bb;
+} orc_matrix;
+
+/* phase (a): fluxes, sources, Euler update, normal wall velocities */
+int orc_predictor(orc_state *s);
+/* phase (b): matrix assembly; solve with MILU-BiCGSTAB into pt */
+int orc_vpcoef(orc_state *s, orc_matrix *m);
+int orc_m1bcgs(int numi, int numj, int numk, const orc_matrix *m, double *x0,
+               const int *indc, double cgpara, double cgepsa, double cgepsr, double cgdiv,
+               int icgtyp, int icgmax, double *bnorm, double *xnorm, int *iter);
+int orc_mzdcmp(int numi, int numj, int numk, const orc_matrix *m, double *dd,
+               double cgpara, const int *indc, int icgtyp);
+void orc_mzminv(int numi, int numj, int numk, const double *dd, const orc_matrix *m,
+                double *x, const double *y, const int *indc);
+void orc_mzax(int numi, int numj, int numk, const orc_matrix *m, const double *x,
+              double *y, const int *indc);
+double orc_mzip(int numi, int numj, int numk, const double *x, const double *y,
+                const int *indc);
+/* phase (d): velocity / pressure correction and the boundary values after it */
+int orc_vmodif(orc_state *s, const double *pt);
+int orc_post_bc(orc_state *s);
+/* VF_V1CAL: (a) + (b) + (d); pt receives the potential (WK09) */
+int orc_v1cal(orc_state *s, double *pt);
+/* VF_F1CAL: phase (c) */
+int orc_f1cal(orc_state *s);
+/* VF_CDIV00 */
+double orc_cdiv00(const orc_state *s);
+/* one SMAC step: V1CAL then F1CAL */
+int orc_step(orc_state *s, double *pt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -17,7 +17,8 @@
  * VF_ACOMPI.h, VF_ATIMER.h, VF_ATIMEI.h (leading members only).
  *
  * Dumping is enabled by the environment variable CADMAS_DUMP_DIR; with
- * CADMAS_DUMP_STEPS="a,b,c" only those step numbers are written.
+ * CADMAS_DUMP_STEPS="a,b,c" only those step numbers are written (the
+ * per-step divergence norm of VF_CDIV00 is always written).
  *
  * Record format (little endian): "CDMP", char name[24], char type ('d' or
  * 'i'), int32 ndim, int32 dims[ndim] (Fortran order, fastest first), data.
@@ -62,7 +63,8 @@ static int open_dump(const char *tag)
 {
     const char *dir = getenv("CADMAS_DUMP_DIR");
     if (!dir || !*dir) return 0;
-    if (!step_selected(vf_atimei_.nnow) && strcmp(tag, "static") != 0) return 0;
+    if (!step_selected(vf_atimei_.nnow) && strcmp(tag, "static") != 0 && strcmp(tag, "dv") != 0)
+        return 0;
     char path[4096];
     if (strcmp(tag, "static") == 0)
         snprintf(path, sizeof path, "%s/static.bin", dir);

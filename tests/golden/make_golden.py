@@ -54,8 +54,14 @@ def _cases():
         "tiny": (lambda: cfg.dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5, name="tiny"),
                  [1, 2, 3, 4, 5]),
         "beach_small": (lambda: cfg.beach_small(steps=40, name="beach_small"), [1, 2, 3, 20, 40]),
-        "dam_a": (lambda: cfg.dam_break(name="dam_a"), [1, 500]),
+        "dam_a": (lambda: cfg.dam_break(name="dam_a"), [1, 100, 500]),
     }
+
+
+# keep only these records for large cases (tag -> keys or None for all)
+_KEEP = {
+    "dam_a": {1: {"v1_in": None, "f1_in": ("BCF",)}, 100: {"f1_out": None}, 500: {"f1_out": None}},
+}
 
 
 _STEP_RE = re.compile(
@@ -96,10 +102,15 @@ def run_case(name, factory, keep_steps):
         np.savez_compressed(HERE / f"{name}_static.npz", **static, DECK=np.array(case.deck))
         for s in keep_steps:
             rec = {}
+            keep = _KEEP.get(name, {}).get(s)
             for tag in ("v1_in", "ps_in", "ps_out", "v1_out", "f1_in", "f1_out"):
                 p = td / "dump" / f"s{s:05d}_{tag}.bin"
+                if keep is not None and tag not in keep:
+                    continue
                 if p.exists():
                     for k, v in read_dump(p).items():
+                        if keep is not None and keep[tag] is not None and k not in keep[tag]:
+                            continue
                         rec[f"{tag}:{k}"] = v
             np.savez_compressed(HERE / f"{name}_s{s:05d}.npz", **rec)
         print(f"{name}: {len(traj['STEP'])} list steps, kept {keep_steps}")
