@@ -1,0 +1,128 @@
+/*
+ * cadmas_hip.h -- C ABI of the MI355X (gfx950) CADMAS-SURF/3D single-phase
+ * VOF-SMAC time step.
+ *
+ * The reference's replaceable unit is the Fortran subroutine, linked
+ * statically (Simulators/CADMAS-SURF-3D/Source code/Makefile:14-83); its
+ * arrays are explicit-shape column-major REAL*8 (NUMI,NUMJ,NUMK) fields,
+ * INTEGER*4 flags, INDB(4,NUMB) and metric tables XX(6,NUMI), YY(6,NUMJ),
+ * ZZ(6,NUMK).  Every pointer below takes exactly those host arrays.  Fields stay
+ * resident in HBM inside a cad_ctx between steps; host copies happen only when
+ * the caller uploads or downloads them (the reference's .rsl output steps).
+ *
+ * All functions return CAD_OK (0) or a negative CAD_ERR_* code; the message of
+ * the last failure is available from cad_last_error().  Solver outcomes keep
+ * the reference's ITER convention (vf_m1bcgs.f:60-64): >=0 iterations,
+ * -1 MILU pivot <= 0, -2 divergence, >ICGMAX not converged.
+ */
+#ifndef CADMAS_HIP_H
+#define CADMAS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAD_OK 0
+#define CAD_ERR_ARG (-1)
+#define CAD_ERR_HIP (-2)
+#define CAD_ERR_STATE (-3)
+#define CAD_ERR_UNSUPPORTED (-4)
+#define CAD_ERR_NOT_CONVERGED (-30)   /* VF_VPSOL 'NOT CONVERGED.'  vf_vpsol.f:71 */
+#define CAD_ERR_INSTABILITY (-31)     /* VF_VPSOL 'INSTABILITY.'    vf_vpsol.f:73 */
+#define CAD_ERR_DECOMPOSITION (-32)   /* VF_VPSOL 'CAN NOT DECOMPOSITION.' :75 */
+#define CAD_ERR_ORIENTATION (-40)     /* VF_FNFPRV 'P.G ERROR.'     vf_fnfprv.f:223 */
+
+#define CAD_PRECISION_FP64 0          /* plain form: all FP64 */
+#define CAD_PRECISION_MIXED 1         /* FP32 stencils/MILU/momentum, FP64 dots */
+
+typedef struct cad_ctx cad_ctx;
+
+/* Scalars the reference keeps in COMMON /VF_APHYSR/ (RHO0, GRZ0, ANU0, VVMAX),
+ * /VF_ACOMPR/ (SCMVP, FLOWER, FUPPER, PLOWER, CGPARA, CGEPSA, CGEPSR, CGDIV),
+ * /VF_ACOMPI/ (ICGTYP, ICGMAX, IBSUW0), /VF_ATIMER/ (DTNOW), /VF_ATIMEI/ (LOOPS). */
+typedef struct cad_params {
+    double rho0, grz0, anu0, dtnow;
+    double scmvp, vvmax, flower, fupper, plower;
+    double cgpara, cgepsa, cgepsr, cgdiv;
+    int icgtyp, icgmax, ibsuw0, loops;
+    int precision;            /* CAD_PRECISION_* */
+} cad_params;
+
+/* Per-step outputs of the list-file line (vf_a1main.f:954-961). */
+typedef struct cad_step_info {
+    double cgbnrm, cgxnrm;    /* !B!, !R! */
+    double fsum, fcut;        /* FSUM, FCUT */
+    int icgitr;               /* ITR */
+    int nums;                 /* surface cells after flagging */
+    float ms_predictor, ms_poisson, ms_correction, ms_vof;  /* HIP-event phase timers */
+} cad_step_info;
+
+/* Library / context management ------------------------------------------ */
+const char *cad_last_error(void);
+int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out);
+void cad_destroy(cad_ctx *ctx);
+int cad_set_params(cad_ctx *ctx, const cad_params *prm);
+
+/* Set-up arrays built by the reference's VF_CSETUP/VF_CGLV/VF_CGLXYZ/VF_CINDX/
+ * VF_CINDB (vf_a1main.f:784) -- uploaded once. */
+int cad_upload_static(cad_ctx *ctx, const double *xx, const double *yy, const double *zz,
+                      const double *anu, const double *cd0,
+                      const double *ggv, const double *ggx, const double *ggy, const double *ggz,
+                      const double *glv, const double *glx, const double *gly, const double *glz,
+                      const int *indx, const int *indy, const int *indz, const int *indb,
+                      const double *bcvi);
+
+/* Time-level state: the arrays VF_OR1TRN writes to .rsl (vf_or1trn.f:105-200)
+ * plus the flags NF/INDC. */
+int cad_upload_state(cad_ctx *ctx, const double *uu, const double *vv, const double *ww,
+                     const double *pp, const double *ff, const double *bcu, const double *bcv,
+                     const double *bcw, const double *bcp, const double *bcf, const int *nf,
+                     const int *indc);
+int cad_download_state(cad_ctx *ctx, double *uu, double *vv, double *ww, double *pp, double *ff,
+                       double *bcu, double *bcv, double *bcw, double *bcp, double *bcf, int *nf,
+                       int *indc);
+
+/* Whole phases ------------------------------------------------------------ */
+/* VF_V1CAL (vf_v1cal.f:1-297): predictor, Poisson, correction, boundary values. */
+int cad_v1cal(cad_ctx *ctx, cad_step_info *info);
+/* VF_F1CAL (vf_f1cal.f:1-239): VOF advection, redistribution, flags, BCs. */
+int cad_f1cal(cad_ctx *ctx, cad_step_info *info);
+/* One SMAC step in the order of vf_a1main.f:1058-1104 (V1CAL then F1CAL). */
+int cad_step(cad_ctx *ctx, cad_step_info *info);
+/* VF_CDIV00 (vf_cdiv00.f:1-56): || sum of face fluxes ||_2 over active cells. */
+int cad_cdiv00(cad_ctx *ctx, double *dv);
+
+/* Finer-grained stages of VF_V1CAL, for stage-wise checking ---------------- */
+/* VF_VFLXDU/V/W + VF_VGENE + VF_VEULER + VF_BWUWN (vf_v1cal.f:126-251) */
+int cad_predictor(cad_ctx *ctx);
+/* VF_VPCOEF (vf_v1cal.f:261) into the context's matrix */
+int cad_vpcoef(cad_ctx *ctx);
+/* VF_VPSOL (vf_vpsol.f:55-76) on the context's matrix; PT in the context */
+int cad_vpsol(cad_ctx *ctx, cad_step_info *info);
+/* VF_VMODIF + VF_BSUWT/BSUWT2/BSUWN/BWUWT/BWPP (vf_v1cal.f:287-297) */
+int cad_correction(cad_ctx *ctx);
+int cad_download_matrix(cad_ctx *ctx, double *ad, double *ali, double *alj, double *alk,
+                        double *aui, double *auj, double *auk, double *bb);
+int cad_upload_matrix(cad_ctx *ctx, const double *ad, const double *ali, const double *alj,
+                      const double *alk, const double *aui, const double *auj, const double *auk,
+                      const double *bb, const int *indc);
+int cad_download_pt(cad_ctx *ctx, double *pt);
+
+/* Drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5) with the reference's by-reference
+ * argument list (trailing underscore: callable from Fortran as
+ * CALL CADMAS_HIP_M1BCGS(...)).  Host arrays in, host X0 out; the device
+ * context is created on first call and reused while the grid size is unchanged.
+ * DBUF and the eight work arrays DD..AE of the reference are not needed and
+ * are omitted. */
+void cadmas_hip_m1bcgs_(const double *ad, const double *ali, const double *alj, const double *alk,
+                        const double *aui, const double *auj, const double *auk, const double *bb,
+                        double *x0, const double *cgpara, const double *cgepsa,
+                        const double *cgepsr, const double *cgdiv, double *bnorm, double *xnorm,
+                        const int *indc, const int *mi, const int *mj, const int *mk,
+                        const int *ni, const int *nj, const int *nk, const int *icgtyp,
+                        const int *icgmax, int *iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -5,4 +5,5 @@ porous field layout (layout.py), its .rsl/.res per-step output (rsl.py), and the
 ctypes mirror (capi.py) of the extern "C" library built from csrc/ that runs the
 step's four phases as hand-written gfx950 HIP kernels.
 """
+from . import capi  # noqa: F401
 from . import configs  # noqa: F401

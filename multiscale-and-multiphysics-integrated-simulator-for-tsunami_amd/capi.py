@@ -1,0 +1,260 @@
+"""ctypes mirror of include/cadmas_hip.h (libcadmas_hip.so).
+
+The library is the product path: there is no CPU fallback.  Loading fails
+loudly when libcadmas_hip.so has not been built, and every call raises
+CadError with the library's message when it returns a negative code.
+
+Arrays follow the reference layout: numpy float64 / int32 in Fortran order,
+fields (NUMI, NUMJ, NUMK), metric tables (6, N), INDB (4, NUMB).
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcadmas_hip.so"
+
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+ERRORS = {-1: "ARG", -2: "HIP", -3: "STATE", -4: "UNSUPPORTED", -30: "NOT CONVERGED",
+          -31: "INSTABILITY", -32: "CAN NOT DECOMPOSITION", -40: "ORIENTATION"}
+
+
+class CadError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("rho0", C.c_double), ("grz0", C.c_double), ("anu0", C.c_double),
+                ("dtnow", C.c_double), ("scmvp", C.c_double), ("vvmax", C.c_double),
+                ("flower", C.c_double), ("fupper", C.c_double), ("plower", C.c_double),
+                ("cgpara", C.c_double), ("cgepsa", C.c_double), ("cgepsr", C.c_double),
+                ("cgdiv", C.c_double), ("icgtyp", C.c_int), ("icgmax", C.c_int),
+                ("ibsuw0", C.c_int), ("loops", C.c_int), ("precision", C.c_int)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("cgbnrm", C.c_double), ("cgxnrm", C.c_double), ("fsum", C.c_double),
+                ("fcut", C.c_double), ("icgitr", C.c_int), ("nums", C.c_int),
+                ("ms_predictor", C.c_float), ("ms_poisson", C.c_float),
+                ("ms_correction", C.c_float), ("ms_vof", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+DEFAULTS = dict(rho0=1.0e3, grz0=9.8, anu0=1.0e-6, dtnow=1.0e-3, scmvp=1.0, vvmax=30.0,
+                flower=1.0e-6, fupper=1.0 - 1.0e-6, plower=1.0e-4, cgpara=0.95, cgepsa=1.0e-15,
+                cgepsr=1.0e-12, cgdiv=1.0e30, icgtyp=1, icgmax=500, ibsuw0=0, loops=1,
+                precision=PRECISION_FP64)
+
+_lib = None
+
+
+def lib():
+    """Load libcadmas_hip.so (raises if it is missing: the HIP path is mandatory)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(str(LIB_PATH))
+    L.cad_last_error.restype = C.c_char_p
+    L.cad_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.cad_destroy.argtypes = [C.c_void_p]
+    L.cad_destroy.restype = None
+    L.cad_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+    L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
+    L.cad_upload_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
+    L.cad_download_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
+    for name in ("cad_v1cal", "cad_f1cal", "cad_step", "cad_vpsol"):
+        getattr(L, name).argtypes = [C.c_void_p, C.POINTER(StepInfo)]
+    for name in ("cad_predictor", "cad_vpcoef", "cad_correction"):
+        getattr(L, name).argtypes = [C.c_void_p]
+    L.cad_cdiv00.argtypes = [C.c_void_p, PD]
+    L.cad_download_matrix.argtypes = [C.c_void_p] + [PD] * 8
+    L.cad_upload_matrix.argtypes = [C.c_void_p] + [PD] * 8 + [PI]
+    L.cad_download_pt.argtypes = [C.c_void_p, PD]
+    _lib = L
+    return L
+
+
+EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad_upload_static",
+           "cad_upload_state", "cad_download_state", "cad_v1cal", "cad_f1cal", "cad_step",
+           "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
+           "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_")
+
+
+def _check(rc):
+    if rc != 0:
+        raise CadError(rc, lib().cad_last_error().decode(errors="replace"))
+
+
+def _f64(a, shape=None):
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _i32(a, shape=None):
+    a = np.asfortranarray(np.asarray(a, dtype=np.int32))
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _pd(a):
+    return a.ctypes.data_as(PD)
+
+
+def _pi(a):
+    return a.ctypes.data_as(PI)
+
+
+MATRIX = ("AD", "ALI", "ALJ", "ALK", "AUI", "AUJ", "AUK", "BB")
+STATE_F = ("UU", "VV", "WW", "PP", "FF")
+STATE_B = ("BCU", "BCV", "BCW", "BCP", "BCF")
+
+
+class Context:
+    """One grid resident in HBM (cad_ctx)."""
+
+    def __init__(self, numi, numj, numk, numb, device=0):
+        self.shape = (int(numi), int(numj), int(numk))
+        self.numb = int(numb)
+        h = C.c_void_p()
+        _check(lib().cad_create(numi, numj, numk, numb, device, C.byref(h)))
+        self._h = h
+        self.params = dict(DEFAULTS)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cad_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------------- set-up
+    def set_params(self, **kw):
+        p = dict(self.params)
+        for k, v in kw.items():
+            if k not in p:
+                raise KeyError(k)
+            p[k] = v
+        st = Params(**p)
+        _check(lib().cad_set_params(self._h, C.byref(st)))
+        self.params = p
+
+    def upload_static(self, a):
+        s, nb = self.shape, self.numb
+        ni, nj, nk = s
+        keep = [_f64(a["XX"], (6, ni)), _f64(a["YY"], (6, nj)), _f64(a["ZZ"], (6, nk))]
+        keep += [_f64(a.get("ANU", np.zeros(s)), s), _f64(a.get("CD0", np.zeros(s)), s)]
+        keep += [_f64(a[k], s) for k in ("GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ")]
+        ints = [_i32(a[k], s) for k in ("INDX", "INDY", "INDZ")]
+        indb = _i32(a["INDB"], (4, nb)) if nb else np.zeros((4, 1), np.int32, order="F")
+        bcvi = _f64(a.get("BCVI", np.zeros(max(nb, 1)))) if nb else np.zeros(1)
+        _check(lib().cad_upload_static(self._h, *[_pd(x) for x in keep], *[_pi(x) for x in ints],
+                                       _pi(indb), _pd(bcvi)))
+
+    def upload_state(self, a):
+        s, nb = self.shape, max(self.numb, 1)
+        fl = [_f64(a[k], s) for k in STATE_F]
+        bc = [_f64(a[k]) if self.numb else np.zeros(nb) for k in STATE_B]
+        ints = [_i32(a["NF"], s), _i32(a["INDC"], s)]
+        _check(lib().cad_upload_state(self._h, *[_pd(x) for x in fl + bc], *[_pi(x) for x in ints]))
+
+    def download_state(self):
+        s, nb = self.shape, max(self.numb, 1)
+        out = {k: np.zeros(s, order="F") for k in STATE_F}
+        out.update({k: np.zeros(nb) for k in STATE_B})
+        out["NF"] = np.zeros(s, dtype=np.int32, order="F")
+        out["INDC"] = np.zeros(s, dtype=np.int32, order="F")
+        _check(lib().cad_download_state(self._h, *[_pd(out[k]) for k in STATE_F + STATE_B],
+                                        _pi(out["NF"]), _pi(out["INDC"])))
+        if not self.numb:
+            for k in STATE_B:
+                out[k] = out[k][:0]
+        return out
+
+    # ---------------------------------------------------------------- phases
+    def _info_call(self, name):
+        info = StepInfo()
+        _check(getattr(lib(), name)(self._h, C.byref(info)))
+        return info.as_dict()
+
+    def step(self):
+        return self._info_call("cad_step")
+
+    def v1cal(self):
+        return self._info_call("cad_v1cal")
+
+    def f1cal(self):
+        return self._info_call("cad_f1cal")
+
+    def vpsol(self):
+        return self._info_call("cad_vpsol")
+
+    def predictor(self):
+        _check(lib().cad_predictor(self._h))
+
+    def vpcoef(self):
+        _check(lib().cad_vpcoef(self._h))
+
+    def correction(self):
+        _check(lib().cad_correction(self._h))
+
+    def cdiv00(self):
+        v = C.c_double()
+        _check(lib().cad_cdiv00(self._h, C.byref(v)))
+        return v.value
+
+    def download_matrix(self):
+        out = {k: np.zeros(self.shape, order="F") for k in MATRIX}
+        _check(lib().cad_download_matrix(self._h, *[_pd(out[k]) for k in MATRIX]))
+        return out
+
+    def upload_matrix(self, m, indc):
+        arrs = [_f64(m[k], self.shape) for k in MATRIX]
+        ic = _i32(indc, self.shape)
+        _check(lib().cad_upload_matrix(self._h, *[_pd(a) for a in arrs], _pi(ic)))
+
+    def download_pt(self):
+        pt = np.zeros(self.shape, order="F")
+        _check(lib().cad_download_pt(self._h, _pd(pt)))
+        return pt
+
+
+def m1bcgs(mat, indc, x0=None, cgpara=0.95, cgepsa=1.0e-15, cgepsr=1.0e-12, cgdiv=1.0e30,
+           icgtyp=1, icgmax=500):
+    """VF_M1BCGS through the Fortran-style drop-in entry point cadmas_hip_m1bcgs_."""
+    L = lib()
+    shape = mat["AD"].shape
+    ms = [_f64(mat[k], shape) for k in MATRIX]
+    x = _f64(np.zeros(shape) if x0 is None else x0, shape).copy(order="F")
+    ic = _i32(indc, shape)
+    d = lambda v: C.byref(C.c_double(v))  # noqa: E731
+    n = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    bn, xn, it = C.c_double(), C.c_double(), C.c_int()
+    L.cadmas_hip_m1bcgs_(*[_pd(a) for a in ms], _pd(x), d(cgpara), d(cgepsa), d(cgepsr), d(cgdiv),
+                         C.byref(bn), C.byref(xn), _pi(ic), n(shape[0]), n(shape[1]), n(shape[2]),
+                         n(shape[0]), n(shape[1]), n(shape[2]), n(icgtyp), n(icgmax), C.byref(it))
+    return x, bn.value, xn.value, it.value

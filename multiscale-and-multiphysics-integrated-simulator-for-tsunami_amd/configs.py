@@ -209,6 +209,28 @@ def buildings_tank(ni=512, nj=256, nk=128, d=0.02, h=0.8, H_over_h=0.15, nbuild=
                 ic=solitary_ic(h, H_over_h * h, 2.0))
 
 
+def corner3d(ni=24, nj=12, nk=16, d=0.01, steps=30, dt=1.0e-4, eps_r=1.0e-10, name=None):
+    """Fully three-dimensional closed-tank case: a water column in one corner
+    collapses around a solid block and through a porous block, so all three
+    velocity components carry flow."""
+    lines = [f"# corner column {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE"]
+    lines.append(f"OBST {ni * 2 // 3} {nj // 3} 1 {ni * 2 // 3 + 2} {nj // 3 + 3} {nk // 4}")
+    pi = ni // 2
+    lines.append(f"POROUS V {pi} 1 1 {pi + 1} {nj} {nk // 2} 0.5 0.0")
+    lines.append(f"POROUS X {pi} 1 1 {pi + 2} {nj} {nk // 2} 0.5 0.0")
+    lines.append(f"POROUS Y {pi} 1 1 {pi + 1} {nj + 1} {nk // 2} 0.5 0.0")
+    lines.append(f"POROUS Z {pi} 1 1 {pi + 1} {nj} {nk // 2 + 1} 0.5 0.0")
+    lines.append(f"POROUS CM {pi} 1 1 {pi + 1} {nj} {nk // 2} 1.0")
+    lines.append(f"POROUS CD {pi} 1 1 {pi + 1} {nj} {nk // 2} 0.8")
+    _common(lines, dt, steps, eps_r, wlevel=2.5 * d)
+    lines.append(f"DEBUG F-BOX 1 1 1 {ni // 3} {nj // 2} {nk * 5 // 8} 1.0")
+    return Case(name or f"corner_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
 def named(name):
     """The workloads by name (sizes as in the benchmark table)."""
     table = {
@@ -217,5 +239,6 @@ def named(name):
         "buildings_c": lambda: buildings_tank(),
         "tiny": lambda: dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5),
         "beach_small": lambda: beach_small(),
+        "corner3d": lambda: corner3d(),
     }
     return table[name]()

@@ -1,0 +1,148 @@
+// cad_common.h -- shared definitions of the gfx950 VOF-SMAC step library.
+//
+// Field layout in HBM is the reference's own: every field is one contiguous
+// (NUMI,NUMJ,NUMK) array with I fastest (Source code/vf_a1main.f:150-151),
+// FP64 for real fields and int32 for flags, so that a field moves between host
+// and device with one copy and a wavefront of 64 consecutive I reads 512
+// contiguous bytes.  Metric tables XX(6,NUMI), YY(6,NUMJ), ZZ(6,NUMK) keep the
+// reference's (6,N) shape (vf_cgrid.f:44-67).  Indices inside kernels are
+// 1-based like the reference so each expression can be checked against the
+// Fortran line it restates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/cadmas_hip.h"
+
+#define CAD_CHECK(x)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            cad_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x,                 \
+                          hipGetErrorString(e_));                                 \
+            return CAD_ERR_HIP;                                                   \
+        }                                                                         \
+    } while (0)
+
+void cad_set_error(const char *fmt, ...);
+
+// Grid geometry passed by value to every kernel.
+struct Dims {
+    int ni, nj, nk;      // NUMI, NUMJ, NUMK (cells + 2 ghosts)
+    long long nij;       // NUMI*NUMJ
+    long long nc;        // NUMI*NUMJ*NUMK
+    int numb;            // boundary faces
+};
+
+// Read-only set-up arrays (device pointers).
+struct Static {
+    const double *xx, *yy, *zz;
+    const double *anu, *cd0;
+    const double *ggv, *ggx, *ggy, *ggz;
+    const double *glv, *glx, *gly, *glz;
+    const int *indx, *indy, *indz;
+    const int *indb;     // INDB(4,NUMB)
+};
+
+__device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
+{
+    return (long long)(i - 1) + (long long)d.ni * (j - 1) + d.nij * (k - 1);
+}
+
+// metric entry m (1..6) of axis a at 1-based index i
+template <int A>
+__device__ __forceinline__ double met(const Static &s, int m, int i)
+{
+    const double *t = A == 0 ? s.xx : (A == 1 ? s.yy : s.zz);
+    return t[(long long)(i - 1) * 6 + (m - 1)];
+}
+
+__device__ __forceinline__ double metr(const Static &s, int a, int m, int i)
+{
+    const double *t = a == 0 ? s.xx : (a == 1 ? s.yy : s.zz);
+    return t[(long long)(i - 1) * 6 + (m - 1)];
+}
+
+__device__ __forceinline__ long long stride(const Dims &d, int a)
+{
+    return a == 0 ? 1LL : (a == 1 ? (long long)d.ni : d.nij);
+}
+
+__device__ __forceinline__ double dmaxd(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmind(double a, double b) { return a < b ? a : b; }
+
+// Thread -> cell mapping of the full-array kernels: x = I (64 wide), y = J, z = K.
+__device__ __forceinline__ bool cell_of_thread(const Dims &d, int &i, int &j, int &k)
+{
+    i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    k = blockIdx.z * blockDim.z + threadIdx.z + 1;
+    return i <= d.ni && j <= d.nj && k <= d.nk;
+}
+
+inline dim3 cell_block() { return dim3(64, 4, 1); }
+inline dim3 cell_grid(const Dims &d)
+{
+    return dim3((unsigned)((d.ni + 63) / 64), (unsigned)((d.nj + 3) / 4), (unsigned)d.nk);
+}
+
+// Deterministic FP64 reduction: fixed grid of REDUCE_BLOCKS blocks, each block
+// writes one partial, one block sums the partials in index order.
+constexpr int REDUCE_BLOCKS = 1024;
+constexpr int REDUCE_THREADS = 256;
+
+// Device-side context of one grid (all fields HBM resident).
+struct Fields {
+    double *uu, *vv, *ww, *pp, *ff;
+    double *bcu, *bcv, *bcw, *bcp, *bcf;
+    int *nf, *indc, *nlim;
+};
+
+struct cad_ctx {
+    int device;
+    hipStream_t stream;
+    Dims d;
+    cad_params prm;
+    // set-up arrays
+    double *xx, *yy, *zz, *anu, *cd0, *ggv, *ggx, *ggy, *ggz, *glv, *glx, *gly, *glz, *bcvi;
+    int *indx, *indy, *indz, *indb;
+    Fields f;
+    // spare velocity buffers for the out-of-place predictor
+    double *u2, *v2, *w2;
+    // Poisson matrix (AD..BB) and potential PT
+    double *ad, *ali, *alj, *alk, *aui, *auj, *auk, *bb, *pt;
+    // BiCGSTAB work vectors (vf_m1bcgs.f: DD, XW, YY, R0, RR, PP, AP, AE)
+    double *dd, *xw, *yv, *r0, *rr, *pv, *ap, *ae;
+    // VOF work: F fluxes
+    double *flf[3];
+    // scratch for reductions and counters
+    double *red_partial;    // REDUCE_BLOCKS * 4
+    double *red_out;        // 16 results
+    int *icount;            // 16 int counters
+    double *host_red;       // pinned mirror of red_out
+    int *host_cnt;          // pinned mirror of icount
+    // timers
+    hipEvent_t ev[16];
+    bool have_static, have_state;
+    // the surface-cell count of the last flagging pass
+    int nums;
+};
+
+// ---- launchers implemented in the phase files ---------------------------
+int cad_launch_predictor(cad_ctx *c);
+int cad_launch_bwuwn(cad_ctx *c);
+int cad_launch_vpcoef(cad_ctx *c);
+int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm, double *xnorm, int *iter);
+int cad_launch_vmodif(cad_ctx *c);
+int cad_launch_bsuwt(cad_ctx *c);
+int cad_launch_bsuwt2(cad_ctx *c);
+int cad_launch_bsuwn(cad_ctx *c);
+int cad_launch_bwuwt(cad_ctx *c);
+int cad_launch_bwpp(cad_ctx *c);
+int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut);
+int cad_reduce_dot(cad_ctx *c, const double *x, const double *y, const int *mask, double *out_dev);
+int cad_reduce_fetch(cad_ctx *c, int n, double *host_out);
+int cad_cdiv(cad_ctx *c, double *dv);
+Static cad_static(const cad_ctx *c);

@@ -1,0 +1,302 @@
+// cad_predictor.hip -- phase (a): explicit momentum predictor on gfx950.
+//
+// The reference computes nine flux arrays and three source arrays in separate
+// passes (VF_VFLXDU/V/W, VF_VGENE), stores them in WK01..WK12 and re-reads them
+// in VF_VEULER (Source code/vf_v1cal.f:126-227): about fifteen full-field
+// streams.  Here one kernel per velocity component evaluates, for each face it
+// owns, the six fluxes and the source that face's update needs and writes the
+// predicted velocity directly: the flux arrays never reach HBM.  A flux value
+// depends only on its own location's inputs, so re-evaluating it for each of
+// the two faces that difference it gives bit-identical results to the stored
+// array.  The kernel is out of place (old u, v, w read, new component written to
+// a spare buffer) because every update reads the old values of all three
+// components.
+#include "cad_common.h"
+
+namespace {
+
+// Normal flux of component A at cell n (FLUU/FLVV/FLWW):
+// vf_vflxdu.f:71-90, vf_vflxdv.f:71-90, vf_vflxdw.f:66-85.
+template <int A>
+__device__ __forceinline__ double flux_normal(const Static &s, const Dims &d, const double *q,
+                                              const double *gl, const double *gg, const int *indc,
+                                              double su, double sc, long long n, int ia)
+{
+    if (indc[n] == -1) return 0.0;
+    const long long sa = stride(d, A);
+    double fl = 0.0;
+    double win = (gl[n] * q[n] + gl[n + sa] * q[n + sa]) * 0.5;
+    double fg = win * (q[n] + q[n + sa]) * 0.5;
+    fl = fl - su * (dmaxd(win, 0.0) * q[n] - dmaxd(-win, 0.0) * q[n + sa]) - sc * fg;
+    fl = fl + (gg[n] + gg[n + sa]) * s.anu[n] * met<A>(s, 4, ia) * (q[n + sa] - q[n]);
+    return fl;
+}
+
+// Cross flux of component A through faces normal to B on the (A-face, B-face)
+// edge at n (FLUV, FLUW, FLVU, FLVW, FLWU, FLWV), including the wall /
+// free-surface branch: vf_vflxdu.f:94-347, vf_vflxdv.f:94-347,
+// vf_vflxdw.f:88-341 (the six are axis permutations of one formula).
+template <int A, int B>
+__device__ __forceinline__ double flux_cross(const Static &s, const Dims &d, const double *q,
+                                             const double *vb, const double *glb,
+                                             const double *ggb, const int *ind_a,
+                                             const int *ind_b, const double *bcq, const int *nf,
+                                             double su, double sc, long long n, int ia, int jb)
+{
+    const long long sa = stride(d, A), sb = stride(d, B);
+    const int ic = nf[n - sa - sb] * nf[n - sb] * nf[n - sa] * nf[n];
+    const int it = ind_a[n];
+    const int ib = ind_a[n - sb];
+    if (ic != 0 || (it != 0 && ib != 0)) return 0.0;
+    const double *anu = s.anu;
+    double fl = 0.0;
+    if (it == 0 && ib == 0) {
+        double win = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * glb[n] * vb[n]
+                                         + met<A>(s, 2, ia) * glb[n - sa] * vb[n - sa]);
+        double fg = win * met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * q[n] + met<B>(s, 2, jb) * q[n - sb]);
+        fl = fl - su * (dmaxd(win, 0.0) * q[n - sb] - dmaxd(-win, 0.0) * q[n]) - sc * fg;
+        double gg = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * ggb[n] + met<A>(s, 2, ia) * ggb[n - sa]);
+        double anr = met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * anu[n] + met<B>(s, 2, jb) * anu[n - sb]);
+        double anl = met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * anu[n - sa] + met<B>(s, 2, jb) * anu[n - sa - sb]);
+        double an = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * anr + met<A>(s, 2, ia) * anl);
+        double d1 = met<B>(s, 5, jb) * (q[n] - q[n - sb]);
+        double d2 = met<A>(s, 5, ia) * (vb[n] - vb[n - sa]);
+        fl = fl + gg * an * (d1 + d2);
+        return fl;
+    }
+    const int nsb = 2 * B + 1;
+    double br, bl;
+    int is = ind_b[n];
+    if (is >= 1) {
+        const int *e = s.indb + (long long)(is - 1) * 4;
+        if (e[2] == 1 || e[2] == 4) br = e[1] == nsb ? q[n] : q[n - sb];
+        else br = bcq[is - 1];
+    } else {
+        br = met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * (q[n] + q[n + sa])
+                                 + met<B>(s, 2, jb) * (q[n - sb] + q[n - sb + sa])) * 0.5;
+    }
+    is = ind_b[n - sa];
+    if (is >= 1) {
+        const int *e = s.indb + (long long)(is - 1) * 4;
+        if (e[2] == 1 || e[2] == 4) bl = e[1] == nsb ? q[n] : q[n - sb];
+        else bl = bcq[is - 1];
+    } else {
+        bl = met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * (q[n] + q[n - sa])
+                                 + met<B>(s, 2, jb) * (q[n - sb] + q[n - sb - sa])) * 0.5;
+    }
+    double win = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * glb[n] * vb[n]
+                                     + met<A>(s, 2, ia) * glb[n - sa] * vb[n - sa]);
+    double fg = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * glb[n] * vb[n] * br
+                                    + met<A>(s, 2, ia - 1) * glb[n - sa] * vb[n - sa] * bl);
+    if (it == 0 && win < 0.0)
+        fl = fl - su * win * q[n] - sc * fg;
+    else if (ib == 0 && win > 0.0)
+        fl = fl - su * win * q[n - sb] - sc * fg;
+    else
+        fl = fl - fg;
+    double gg = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * ggb[n] + met<A>(s, 2, ia) * ggb[n - sa]);
+    fg = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * br + met<A>(s, 2, ia - 1) * bl);
+    double anr, anl;
+    if (nf[n] == -1) anr = anu[n - sb];
+    else if (nf[n - sb] == -1) anr = anu[n];
+    else anr = met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * anu[n] + met<B>(s, 2, jb) * anu[n - sb]);
+    if (nf[n - sa] == -1) anl = anu[n - sa - sb];
+    else if (nf[n - sa - sb] == -1) anl = anu[n - sa];
+    else anl = met<B>(s, 6, jb) * (met<B>(s, 2, jb - 1) * anu[n - sa] + met<B>(s, 2, jb) * anu[n - sa - sb]);
+    double an = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * anr + met<A>(s, 2, ia) * anl);
+    double d1;
+    if (it == 0) d1 = met<B>(s, 4, jb) * (q[n] - fg) * 2.0;
+    else d1 = met<B>(s, 4, jb - 1) * (fg - q[n - sb]) * 2.0;
+    double d2 = met<A>(s, 5, ia) * (vb[n] - vb[n - sa]);
+    fl = fl + gg * an * (d1 + d2);
+    return fl;
+}
+
+template <int A> __device__ __forceinline__ const double *vel(const double *u, const double *v, const double *w)
+{
+    return A == 0 ? u : (A == 1 ? v : w);
+}
+template <int A> __device__ __forceinline__ const double *gl_of(const Static &s)
+{
+    return A == 0 ? s.glx : (A == 1 ? s.gly : s.glz);
+}
+template <int A> __device__ __forceinline__ const double *gg_of(const Static &s)
+{
+    return A == 0 ? s.ggx : (A == 1 ? s.ggy : s.ggz);
+}
+template <int A> __device__ __forceinline__ const int *ind_of(const Static &s)
+{
+    return A == 0 ? s.indx : (A == 1 ? s.indy : s.indz);
+}
+
+// Cross flux of component A through B-faces evaluated at the edge location n.
+template <int A, int B>
+__device__ __forceinline__ double cross_at(const Static &s, const Dims &d, const double *u,
+                                           const double *v, const double *w, const double *bcq,
+                                           const int *nf, double su, double sc, long long n,
+                                           const int idx[3])
+{
+    return flux_cross<A, B>(s, d, vel<A>(u, v, w), vel<B>(u, v, w), gl_of<B>(s), gg_of<B>(s),
+                            ind_of<A>(s), ind_of<B>(s), bcq, nf, su, sc, n, idx[A], idx[B]);
+}
+
+// One face of component A: VF_VGENE source (vf_vgene.f:84-330, IDRGN<=0) and
+// VF_VEULER update with the VVMAX clip (vf_veuler.f:116-249).
+template <int A>
+__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, const double *__restrict__ u,
+                                                 const double *__restrict__ v,
+                                                 const double *__restrict__ w,
+                                                 const double *__restrict__ p,
+                                                 const double *__restrict__ bcq,
+                                                 const int *__restrict__ nf,
+                                                 const int *__restrict__ indc,
+                                                 double *__restrict__ out, cad_params prm)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    const double *q = vel<A>(u, v, w);
+    const int idx[3] = {i, j, k};
+    const int lo[3] = {A == 0 ? 3 : 2, A == 1 ? 3 : 2, A == 2 ? 3 : 2};
+    const int hi[3] = {d.ni - 1, d.nj - 1, d.nk - 1};
+    const long long sa = stride(d, A);
+    const int *ind = ind_of<A>(s);
+    bool upd = idx[0] >= lo[0] && idx[0] <= hi[0] && idx[1] >= lo[1] && idx[1] <= hi[1] &&
+               idx[2] >= lo[2] && idx[2] <= hi[2];
+    if (upd) upd = ind[n] == 0 && nf[n - sa] * nf[n] == 0;
+    if (!upd) {
+        out[n] = q[n];
+        return;
+    }
+    const double su = prm.scmvp, sc = 1.0 - prm.scmvp;
+    const int ia = idx[A];
+    // fluxes differenced by this face, in the x, y, z order of VF_VEULER
+    double t[3];
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+        if (dd == A) {
+            double fn = flux_normal<A>(s, d, q, gl_of<A>(s), gg_of<A>(s), indc, su, sc, n, ia);
+            double fm = flux_normal<A>(s, d, q, gl_of<A>(s), gg_of<A>(s), indc, su, sc, n - sa, ia - 1);
+            t[dd] = met<A>(s, 5, ia) * (fn - fm);
+        } else {
+            const long long sd = stride(d, dd);
+            int idp[3] = {i, j, k};
+            idp[dd] += 1;
+            double fn, fp;
+            if (dd == 0) {
+                fn = cross_at<A, (A == 0 ? 1 : 0)>(s, d, u, v, w, bcq, nf, su, sc, n, idx);
+                fp = cross_at<A, (A == 0 ? 1 : 0)>(s, d, u, v, w, bcq, nf, su, sc, n + sd, idp);
+            } else if (dd == 1) {
+                fn = cross_at<A, (A == 1 ? 0 : 1)>(s, d, u, v, w, bcq, nf, su, sc, n, idx);
+                fp = cross_at<A, (A == 1 ? 0 : 1)>(s, d, u, v, w, bcq, nf, su, sc, n + sd, idp);
+            } else {
+                fn = cross_at<A, (A == 2 ? 0 : 2)>(s, d, u, v, w, bcq, nf, su, sc, n, idx);
+                fp = cross_at<A, (A == 2 ? 0 : 2)>(s, d, u, v, w, bcq, nf, su, sc, n + sd, idp);
+            }
+            t[dd] = metr(s, dd, 4, idx[dd]) * (fp - fn);
+        }
+    }
+    // source term (QU/QV/QW start from zero in VF_V1CAL)
+    const double ri = 1.0 / prm.rho0;
+    double qs = 0.0;
+    double gv = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * s.ggv[n] + met<A>(s, 2, ia - 1) * s.ggv[n - sa]);
+    double cd = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * s.cd0[n] + met<A>(s, 2, ia - 1) * s.cd0[n - sa]);
+    qs = qs + gv * ri * met<A>(s, 5, ia) * (p[n - sa] - p[n]);
+    double c[3];
+    const double *vv3[3] = {u, v, w};
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+        if (dd == A) {
+            c[dd] = q[n];
+        } else {
+            const double *vd = vv3[dd];
+            const long long sd = stride(d, dd);
+            c[dd] = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * (vd[n] + vd[n + sd])
+                                        + met<A>(s, 2, ia) * (vd[n - sa] + vd[n - sa + sd])) * 0.5;
+        }
+    }
+    double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    qs = qs - 0.5 * cd * met<A>(s, 5, ia) * (1.0 - gg_of<A>(s)[n]) * c[A] * uvw;
+    if (A == 2) {
+        qs = qs - gv * prm.grz0;
+        double dr = 0.0;
+        qs = qs - gv * prm.grz0 * dr / prm.rho0;
+    }
+    // Euler update; GV0 = GV for static porosity (vf_v1cal.f:218-227)
+    double g = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * s.glv[n] + met<A>(s, 2, ia - 1) * s.glv[n - sa]);
+    double g0 = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * s.glv[n] + met<A>(s, 2, ia - 1) * s.glv[n - sa]);
+    double du = (t[0] + t[1] + t[2] + qs);
+    double r = (g * q[n] + prm.dtnow * du) / g0;
+    if (r > prm.vvmax) r = prm.vvmax;
+    if (r < -prm.vvmax) r = -prm.vvmax;
+    out[n] = r;
+}
+
+// VF_BWUWN boundary-list pass (vf_bwuwn.f:84-254): normal velocity on wall and
+// free boundary faces.  Boundary type 5 (wave maker) is rejected on the host.
+__global__ void k_bwuwn(Dims d, const int *__restrict__ indb, double *u, double *v, double *w,
+                        double *bcu, double *bcv, double *bcw)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= d.numb) return;
+    const int *e = indb + (long long)l * 4;
+    const long long n = (long long)e[0] - 1;
+    const int ns = e[1], ib = e[2];
+    if (ib == 1 || ib == 6 || ib == 8) {
+        if (ns == 1 || ns == 2) { bcu[l] = 0.0; u[n] = 0.0; }
+        else if (ns == 3 || ns == 4) { bcv[l] = 0.0; v[n] = 0.0; }
+        else { bcw[l] = 0.0; w[n] = 0.0; }
+    } else if (ib == 2) {
+        bcu[l] = 0.0; bcv[l] = 0.0; bcw[l] = 0.0;
+        if (ns == 1 || ns == 2) u[n] = 0.0;
+        else if (ns == 3 || ns == 4) v[n] = 0.0;
+        else w[n] = 0.0;
+    } else if (ib == 3) {
+        if (ns == 1 || ns == 2) u[n] = bcu[l];
+        else if (ns == 3 || ns == 4) v[n] = bcv[l];
+        else w[n] = bcw[l];
+    } else if (ib == 4) {
+        double val;
+        switch (ns) {
+        case 1: val = u[n + 1]; bcu[l] = val; u[n] = val; break;
+        case 2: val = u[n - 1]; bcu[l] = val; u[n] = val; break;
+        case 3: val = v[n + d.ni]; bcv[l] = val; v[n] = val; break;
+        case 4: val = v[n - d.ni]; bcv[l] = val; v[n] = val; break;
+        case 5: val = w[n + d.nij]; bcw[l] = val; w[n] = val; break;
+        case 6: val = w[n - d.nij]; bcw[l] = val; w[n] = val; break;
+        default: break;
+        }
+    }
+}
+
+}  // namespace
+
+int cad_launch_predictor(cad_ctx *c)
+{
+    Static s = cad_static(c);
+    const Dims &d = c->d;
+    dim3 b = cell_block(), g = cell_grid(d);
+    Fields &f = c->f;
+    hipLaunchKernelGGL(k_predict<0>, g, b, 0, c->stream, d, s, f.uu, f.vv, f.ww, f.pp, f.bcu, f.nf,
+                       f.indc, c->u2, c->prm);
+    hipLaunchKernelGGL(k_predict<1>, g, b, 0, c->stream, d, s, f.uu, f.vv, f.ww, f.pp, f.bcv, f.nf,
+                       f.indc, c->v2, c->prm);
+    hipLaunchKernelGGL(k_predict<2>, g, b, 0, c->stream, d, s, f.uu, f.vv, f.ww, f.pp, f.bcw, f.nf,
+                       f.indc, c->w2, c->prm);
+    CAD_CHECK(hipGetLastError());
+    double *t;
+    t = f.uu; f.uu = c->u2; c->u2 = t;
+    t = f.vv; f.vv = c->v2; c->v2 = t;
+    t = f.ww; f.ww = c->w2; c->w2 = t;
+    return CAD_OK;
+}
+
+int cad_launch_bwuwn(cad_ctx *c)
+{
+    const Dims &d = c->d;
+    if (d.numb <= 0) return CAD_OK;
+    hipLaunchKernelGGL(k_bwuwn, dim3((d.numb + 255) / 256), dim3(256), 0, c->stream, d, c->indb,
+                       c->f.uu, c->f.vv, c->f.ww, c->f.bcu, c->f.bcv, c->f.bcw);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}

@@ -1,0 +1,715 @@
+// cad_vof.hip -- phase (c): VOF F-function advection, surface-cell
+// redistribution, clipping with the volume sums, and cell-type flagging
+// (Source code/vf_f1cal.f:124-239).
+//
+// Order-dependent list loops of the reference are restated as gathers:
+//  * VF_FMOD1 counts, per fluid / gas cell, the surface cells pointing at it
+//    (vf_fmod1.f:46-86); here each cell counts its pointing neighbours.
+//  * VF_FMOD1's overflow / underflow spill (vf_fmod1.f:152-191) adds into a
+//    fluid or gas cell from every surface cell pointing at it, in the order of
+//    the surface list, which is lexicographic (vf_fnfini.f:112-126); the gather
+//    visits the six neighbours in increasing linear index, reproducing the
+//    reference's summation order exactly.
+//  * VF_FNFINI's surface list is not materialised: NF = 7 marks exactly the
+//    listed cells at every stage of its iteration.
+#include "cad_common.h"
+
+namespace {
+
+// fluid-side offset axis/sign of surface orientation c (1..6): c = 2*axis+1
+// has fluid on the minus side, c = 2*axis+2 on the plus side
+__device__ __forceinline__ long long fluid_off(const Dims &d, int c)
+{
+    const int a = (c - 1) / 2;
+    return (c % 2 == 1) ? -stride(d, a) : stride(d, a);
+}
+
+// ------------------------------------------------------------------ FCONV
+// Donor-acceptor flux through faces normal to A, vf_fconv.f:56-200.
+template <int A>
+__global__ void __launch_bounds__(256) k_fconv(Dims d, Static s, const double *__restrict__ q,
+                                               const double *__restrict__ f,
+                                               const double *__restrict__ bcf,
+                                               const int *__restrict__ nf, double *__restrict__ flf,
+                                               int *nlim, double dtnow, double flower)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    const int idx[3] = {i, j, k};
+    const int hi[3] = {A == 0 ? d.ni : d.ni - 1, A == 1 ? d.nj : d.nj - 1, A == 2 ? d.nk : d.nk - 1};
+    const int *ind = A == 0 ? s.indx : (A == 1 ? s.indy : s.indz);
+    const double *gga = A == 0 ? s.ggx : (A == 1 ? s.ggy : s.ggz);
+    if (i < 2 || j < 2 || k < 2 || i > hi[0] || j > hi[1] || k > hi[2] || ind[n] == -1) {
+        flf[n] = 0.0;
+        return;
+    }
+    const long long sa = stride(d, A);
+    double c = dtnow * gga[n] * q[n];
+    double ac = fabs(c);
+    int lsn, ild;
+    long long la, ld, ldm;
+    if (c >= 0.0) { lsn = 1; la = n; ld = n - sa; ldm = n - 2 * sa; ild = idx[A] - 1; }
+    else { lsn = -1; la = n - sa; ld = n; ldm = n + sa; ild = idx[A]; }
+    const int nn = nf[ld];
+    double fd, fdmw, v, fa;
+    if (nn != -1) {
+        fd = f[ld];
+        fdmw = 1.0;
+        if (nf[ldm] != -1) fdmw = f[ldm];
+        v = met<A>(s, 2, ild) * s.ggv[ld];
+    } else {
+        fd = ind[n] >= 1 ? bcf[ind[n] - 1] : 0.0;  // a donor obstacle face is a boundary face
+        fdmw = 1.0;
+        v = met<A>(s, 2, ild);
+    }
+    if (nf[la] != -1) fa = f[la];
+    else fa = fd;
+    double fad = fa;
+    const bool tang = (nn == -1) || ((nn >= 1 && nn <= 6) && ((nn - 1) / 2 != A));
+    if (tang) {
+        if (fdmw != 0.0 && fa != 0.0) fad = fd;
+    }
+    double fdm = dmaxd(fdmw, fd);
+    if (fdmw < flower && fa < flower) fdm = dmaxd(fdm, 0.1);
+    double cfx = dmaxd((fdm - fad) * ac - (fdm - fd) * v, 0.0);
+    double fvx = fad * ac + cfx;
+    double fvm = fd * v;
+    flf[n] = 0.0 - (double)lsn * dmind(fvx, fvm);
+    if (fvx > fvm) nlim[ld] = 1;
+}
+
+// F update, vf_feuler.f:43-58 (static porosity, LOOPS=1, no source).
+__global__ void __launch_bounds__(256) k_feuler(Dims d, Static s, const double *__restrict__ flu,
+                                                const double *__restrict__ flv,
+                                                const double *__restrict__ flw,
+                                                const int *__restrict__ nf, double *f, double dtnow,
+                                                double loops)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] == -1) return;
+    const long long ni = d.ni, nij = d.nij;
+    double g0 = s.ggv[n] - (s.ggv[n] - s.ggv[n]) * loops;
+    double df = (met<0>(s, 4, i) * (flu[n + 1] - flu[n]) + met<1>(s, 4, j) * (flv[n + ni] - flv[n])
+                 + met<2>(s, 4, k) * (flw[n + nij] - flw[n]) + dtnow * 0.0);
+    f[n] = (g0 * f[n] + df) / s.ggv[n];
+}
+
+__device__ __forceinline__ double cell_vol(const Static &s, int i, int j, int k, double g)
+{
+    return met<0>(s, 2, i) * met<1>(s, 2, j) * met<2>(s, 2, k) * g;
+}
+
+// decode a linear index into 1-based (i,j,k)
+__device__ __forceinline__ void ijk_of(const Dims &d, long long n, int &i, int &j, int &k)
+{
+    k = (int)(n / d.nij) + 1;
+    long long r = n - (long long)(k - 1) * d.nij;
+    j = (int)(r / d.ni) + 1;
+    i = (int)(r - (long long)(j - 1) * d.ni) + 1;
+}
+
+// FMOD1 step 1: surface cells pointing at each cell, vf_fmod1.f:46-86.
+__global__ void __launch_bounds__(256) k_fmod_count(Dims d, const int *__restrict__ nf, int *nlim)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    int cnt = 0;
+    if (i >= 2 && i <= d.ni - 1 && j >= 2 && j <= d.nj - 1 && k >= 2 && k <= d.nk - 1) {
+        const long long nb[6] = {n - d.nij, n - d.ni, n - 1, n + 1, n + d.ni, n + d.nij};
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int c = nf[nb[q]];
+            if (c < 1 || c > 6) continue;
+            const long long fo = fluid_off(d, c);
+            // the neighbour's fluid cell (IFC) or gas cell (IEC) is this cell
+            if (nb[q] + fo == n) cnt++;
+            if (nb[q] - fo == n) cnt++;
+        }
+    }
+    nlim[n] = cnt;
+}
+
+// FMOD1 step 2: redistribution among the fluid-surface-gas triple,
+// vf_fmod1.f:85-142.  The targets with NLIM==1 belong to this cell alone.
+__global__ void __launch_bounds__(256) k_fmod_redist(Dims d, Static s, const int *__restrict__ nf,
+                                                     const int *__restrict__ nlim, double *f)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int c = nf[n];
+    if (c < 1 || c > 6) return;
+    const long long fo = fluid_off(d, c);
+    const long long ne = n - fo, nfc = n + fo;
+    int ie, je, ke, i2, j2, k2;
+    ijk_of(d, ne, ie, je, ke);
+    ijk_of(d, nfc, i2, j2, k2);
+    double ve = cell_vol(s, ie, je, ke, s.ggv[ne]);
+    double vs = cell_vol(s, i, j, k, s.ggv[n]);
+    double vf = cell_vol(s, i2, j2, k2, s.ggv[nfc]);
+    double ss = vs * f[n];
+    const bool le = nlim[ne] == 1, lf = nlim[nfc] == 1;
+    if (le) ss = ss + ve * f[ne];
+    if (lf) ss = ss + vf * f[nfc];
+    double ds;
+    if (lf) {
+        ds = dmind(vf, ss);
+        ss = ss - ds;
+        f[nfc] = ds / vf;
+    }
+    ds = dmind(vs, ss);
+    ss = ss - ds;
+    f[n] = ds / vs;
+    if (le) {
+        ds = dmind(ve, ss);
+        ss = ss - ds;
+        f[ne] = ds / ve;
+    }
+    if (ss > 0.0) {
+        if (le) f[ne] = f[ne] + ss / ve;
+        else f[n] = f[n] + ss / vs;
+    }
+}
+
+// FMOD1 step 3a: spill of surface over/underflow into the gas / fluid cell,
+// gathered per receiving cell in surface-list order, vf_fmod1.f:152-188.
+__global__ void __launch_bounds__(256) k_fmod_spill(Dims d, Static s, const int *__restrict__ nf,
+                                                    double *f)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int cx = nf[n];
+    if (cx >= 1 && cx <= 6) return;
+    const long long nb[6] = {n - d.nij, n - d.ni, n - 1, n + 1, n + d.ni, n + d.nij};
+    double fx = f[n];
+    bool touched = false;
+    double vx = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int c = nf[nb[q]];
+        if (c < 1 || c > 6) continue;
+        const double fs = f[nb[q]];
+        const long long fo = fluid_off(d, c);
+        const bool gas_side = (nb[q] - fo == n), fluid_side = (nb[q] + fo == n);
+        if (!(gas_side && fs > 1.0) && !(fluid_side && !(fs > 1.0) && fs < 0.0)) continue;
+        if (!touched) {
+            vx = cell_vol(s, i, j, k, s.ggv[n]);
+            touched = true;
+        }
+        int is, js, ks;
+        ijk_of(d, nb[q], is, js, ks);
+        const double vs = cell_vol(s, is, js, ks, s.ggv[nb[q]]);
+        if (gas_side && fs > 1.0) fx = fx + vs * (fs - 1.0) / vx;
+        else fx = fx + vs * fs / vx;
+    }
+    if (touched) f[n] = fx;
+}
+
+// FMOD1 step 3b: clamp the surface cells themselves.
+__global__ void __launch_bounds__(256) k_fmod_clamp(Dims d, const int *__restrict__ nf, double *f)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int c = nf[n];
+    if (c < 1 || c > 6) return;
+    if (f[n] > 1.0) f[n] = 1.0;
+    if (f[n] < 0.0) f[n] = 0.0;
+}
+
+// Clip F to {0,1} beyond FLOWER/FUPPER and partial sums of FSUM and FCUT,
+// vf_fcut01.f:34-55.
+__global__ void __launch_bounds__(REDUCE_THREADS) k_fcut(Dims d, Static s, const int *__restrict__ nf,
+                                                         double *f, double flower, double fupper,
+                                                         double *partial)
+{
+    __shared__ double sh[2][REDUCE_THREADS];
+    double fsum = 0.0, fcut = 0.0;
+    for (long long n = (long long)blockIdx.x * REDUCE_THREADS + threadIdx.x; n < d.nc;
+         n += (long long)REDUCE_BLOCKS * REDUCE_THREADS) {
+        int i, j, k;
+        ijk_of(d, n, i, j, k);
+        if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) continue;
+        if (nf[n] == -1) continue;
+        double v = cell_vol(s, i, j, k, s.ggv[n]);
+        double fv = f[n];
+        if (fv > fupper) {
+            fcut = fcut + (fv - 1.0) * v;
+            fv = 1.0;
+            f[n] = fv;
+        } else if (fv < flower) {
+            fcut = fcut + (fv - 0.0) * v;
+            fv = 0.0;
+            f[n] = fv;
+        }
+        fsum = fsum + fv * v;
+    }
+    sh[0][threadIdx.x] = fsum;
+    sh[1][threadIdx.x] = fcut;
+    __syncthreads();
+    for (int off = REDUCE_THREADS / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = sh[0][0];
+        partial[REDUCE_BLOCKS + blockIdx.x] = sh[1][0];
+    }
+}
+
+// Boundary F, vf_bwff.f:33-87 (types FIX and FREE).
+__global__ void k_bwff(Dims d, const int *__restrict__ indb, const double *__restrict__ f, double *bcf)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= d.numb) return;
+    const int *e = indb + (long long)l * 4;
+    if (e[3] != 2) return;
+    const long long n = (long long)e[0] - 1;
+    switch (e[1]) {
+    case 1: bcf[l] = f[n]; break;
+    case 2: bcf[l] = f[n - 1]; break;
+    case 3: bcf[l] = f[n]; break;
+    case 4: bcf[l] = f[n - d.ni]; break;
+    case 5: bcf[l] = f[n]; break;
+    case 6: bcf[l] = f[n - d.nij]; break;
+    default: break;
+    }
+}
+
+// Gas-side normal velocity of a surface cell from its tangential neighbours,
+// vf_bsuwn3.f:37-335.
+__global__ void __launch_bounds__(256) k_bsuwn3(Dims d, Static s, const int *__restrict__ nf, double *u,
+                                                double *v, double *w, int ibsuw0)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int nf0 = nf[n];
+    if (nf0 < 1 || nf0 > 6) return;
+    const int idx[3] = {i, j, k};
+    const int dn = (nf0 - 1) / 2;
+    const bool plus = (nf0 % 2 == 1);
+    const long long sd = stride(d, dn);
+    double *vel[3] = {u, v, w};
+    double *q = vel[dn];
+    const double d0 = metr(s, dn, 2, idx[dn]);
+    double v0 = plus ? q[n] : q[n + sd];
+    const long long l1 = plus ? n + sd : n - sd;
+    const long long wface = plus ? n + sd : n;
+    int is_[4] = {0, 0, 0, 0};
+    double dd[4] = {0.0, 0.0, 0.0, 0.0}, vv[4] = {0.0, 0.0, 0.0, 0.0};
+    int c = 0;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        if (t == dn) continue;
+        const long long st = stride(d, t);
+#pragma unroll
+        for (int pm = 0; pm < 2; pm++) {
+            const long long ls = pm ? n + st : n - st;
+            const long long ls1 = pm ? l1 + st : l1 - st;
+            if (nf[ls] == 0 || nf[ls1] == 0) {
+                if (nf[ls] != -1 && nf[ls1] != -1) {
+                    is_[c] = 1;
+                    dd[c] = pm ? metr(s, t, 3, idx[t] + 1) : metr(s, t, 3, idx[t]);
+                    vv[c] = pm ? q[wface + st] : q[wface - st];
+                }
+            }
+            c++;
+        }
+    }
+    const int iss = is_[0] + is_[1] + is_[2] + is_[3];
+    if (iss <= 0) {
+        // the fluid neighbour is interior, so the cell two steps away exists
+        if (plus) {
+            if (ibsuw0 == 0 && idx[dn] >= 3 && nf[n - 2 * sd] == 0)
+                v0 = v0 + (v0 - q[n - sd]) * metr(s, dn, 2, idx[dn]) / metr(s, dn, 2, idx[dn] - 1);
+        } else {
+            if (ibsuw0 == 0 && idx[dn] + 2 <= (dn == 0 ? d.ni : (dn == 1 ? d.nj : d.nk)) &&
+                nf[n + 2 * sd] == 0)
+                v0 = v0 + (v0 - q[n + 2 * sd]) * metr(s, dn, 2, idx[dn]) / metr(s, dn, 2, idx[dn] + 1);
+        }
+        q[wface] = v0;
+    } else {
+        const double ds = d0 + dd[0] + dd[1] + dd[2] + dd[3];
+        q[wface] = ((ds - d0) * v0 + (ds - dd[0]) * vv[0] + (ds - dd[1]) * vv[1] + (ds - dd[2]) * vv[2]
+                    + (ds - dd[3]) * vv[3]) / ds / (double)iss;
+    }
+}
+
+// VF_BSUWEM count: surface cells for which this cell is a tangential
+// neighbour, vf_bsuwem.f:36-61.
+__global__ void __launch_bounds__(256) k_bsuwem_count(Dims d, const int *__restrict__ nf, int *nlim)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    int cnt = 0;
+    if (i >= 2 && i <= d.ni - 1 && j >= 2 && j <= d.nj - 1 && k >= 2 && k <= d.nk - 1) {
+        const long long nb[6] = {n - d.nij, n - d.ni, n - 1, n + 1, n + d.ni, n + d.nij};
+        const int ax[6] = {2, 1, 0, 0, 1, 2};
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int c = nf[nb[q]];
+            if (c < 1 || c > 6) continue;
+            // normal axis of the neighbour: z for 5/6, y for 3/4, x otherwise
+            const int na = (c == 5 || c == 6) ? 2 : ((c == 3 || c == 4) ? 1 : 0);
+            if (ax[q] != na) cnt++;
+        }
+    }
+    nlim[n] = cnt;
+}
+
+// Velocities of gas cells enclosed by surface cells, vf_bsuwem.f:66-88.
+__global__ void __launch_bounds__(256) k_bsuwem(Dims d, const int *__restrict__ nf,
+                                                const int *__restrict__ nl, double *u, double *v,
+                                                double *w)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (!(nl[n] >= 2 && nf[n] == 8)) return;
+    const long long ni = d.ni, nij = d.nij;
+    if (nf[n - 1] == 8 && nl[n - 1] < 2) { if (nf[n + 1] != 8) u[n] = u[n + 1]; }
+    if (nf[n + 1] == 8 && nl[n + 1] < 2) { if (nf[n - 1] != 8) u[n + 1] = u[n]; }
+    if (nf[n - ni] == 8 && nl[n - ni] < 2) { if (nf[n + ni] != 8) v[n] = v[n + ni]; }
+    if (nf[n + ni] == 8 && nl[n + ni] < 2) { if (nf[n - ni] != 8) v[n + ni] = v[n]; }
+    if (nf[n - nij] == 8 && nl[n - nij] < 2) { if (nf[n + nij] != 8) w[n] = w[n + nij]; }
+    if (nf[n + nij] == 8 && nl[n + nij] < 2) { if (nf[n - nij] != 8) w[n + nij] = w[n]; }
+}
+
+__global__ void k_icopy(long long nc, const int *__restrict__ a, int *__restrict__ b)
+{
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
+         n += (long long)gridDim.x * blockDim.x)
+        b[n] = a[n];
+}
+
+__global__ void k_dcopy(long long nc, const double *__restrict__ a, double *__restrict__ b)
+{
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
+         n += (long long)gridDim.x * blockDim.x)
+        b[n] = a[n];
+}
+
+__global__ void k_iset(int *p, int v) { *p = v; }
+
+// --------------------------------------------------------------- FNFINI
+// vf_fnfini.f:36-42: every non-obstacle cell of layers 2..NK-1 becomes fluid.
+__global__ void __launch_bounds__(256) k_nf_reset(Dims d, int *nf)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] != -1) nf[n] = 0;
+}
+
+__device__ __forceinline__ double f_across(const double *f, const double *bcf, int ind, long long nb)
+{
+    return ind >= 1 ? bcf[ind - 1] : f[nb];
+}
+
+// vf_fnfini.f:45-102: empty cells with no full face neighbour become gas.
+__global__ void __launch_bounds__(256) k_nf_gas(Dims d, Static s, const double *__restrict__ f,
+                                                const double *__restrict__ bcf, int *nf,
+                                                double flower, double fupper)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] == -1 || !(f[n] < flower)) return;
+    const long long ni = d.ni, nij = d.nij;
+    const int ix1 = s.indx[n], ix2 = s.indx[n + 1];
+    const int iy1 = s.indy[n], iy2 = s.indy[n + ni];
+    const int iz1 = s.indz[n], iz2 = s.indz[n + nij];
+    const double fx1 = f_across(f, bcf, ix1, n - 1), fx2 = f_across(f, bcf, ix2, n + 1);
+    const double fy1 = f_across(f, bcf, iy1, n - ni), fy2 = f_across(f, bcf, iy2, n + ni);
+    const double fz1 = f_across(f, bcf, iz1, n - nij), fz2 = f_across(f, bcf, iz2, n + nij);
+    if (!(fz1 > fupper && iz1 == 0) && !(fy1 > fupper && iy1 == 0) && !(fy2 > fupper && iy2 == 0) &&
+        !(fx1 > fupper && ix1 == 0) && !(fx2 > fupper && ix2 == 0) && !(fz2 > fupper && iz2 == 0))
+        nf[n] = 8;
+}
+
+// vf_fnfini.f:112-126: non-gas cells next to gas become surface (7).
+__global__ void __launch_bounds__(256) k_nf_surface(Dims d, int *nf)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int c = nf[n];
+    if (c == -1 || c == 8) return;
+    const long long ni = d.ni, nij = d.nij;
+    if (nf[n + nij] == 8 || nf[n + ni] == 8 || nf[n - ni] == 8 || nf[n + 1] == 8 || nf[n - 1] == 8 ||
+        nf[n - nij] == 8)
+        nf[n] = 7;
+}
+
+// vf_fnfini.f:130-148: surface cells with no fluid neighbour become gas.
+__global__ void __launch_bounds__(256) k_nf_nofluid(Dims d, int *nf)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] != 7) return;
+    const long long ni = d.ni, nij = d.nij;
+    if (nf[n + nij] != 0 && nf[n + ni] != 0 && nf[n - ni] != 0 && nf[n + 1] != 0 && nf[n - 1] != 0 &&
+        nf[n - nij] != 0)
+        nf[n] = 8;
+}
+
+// vf_fnfini.f:152-172: surface cells not sandwiched between fluid and gas
+// along any axis are marked -7 and counted.
+__global__ void __launch_bounds__(256) k_nf_sandwich(Dims d, int *nf, int *nch)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] != 7) return;
+    const long long ni = d.ni, nij = d.nij;
+    if ((nf[n - nij] != 0 || nf[n + nij] != 8) && (nf[n - 1] != 0 || nf[n + 1] != 8) &&
+        (nf[n + 1] != 0 || nf[n - 1] != 8) && (nf[n - ni] != 0 || nf[n + ni] != 8) &&
+        (nf[n + ni] != 0 || nf[n - ni] != 8) && (nf[n + nij] != 0 || nf[n - nij] != 8)) {
+        nf[n] = -7;
+        atomicAdd(nch, 1);
+    }
+}
+
+// vf_fnfini.f:178-186
+__global__ void __launch_bounds__(256) k_nf_fix(Dims d, int *nf)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] == -7) nf[n] = 8;
+}
+
+// Surface orientation, vf_fnfprv.f:34-223; counts surface cells.
+__global__ void __launch_bounds__(256) k_nf_orient(Dims d, Static s, const double *__restrict__ f,
+                                                   const double *__restrict__ bcf, int *nf, int *cnt)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (nf[n] != 7) return;
+    const int *ind[3] = {s.indx, s.indy, s.indz};
+    const int order[6] = {5, 6, 3, 4, 1, 2};
+    double fm = -1.0;
+    int code = 7;
+#pragma unroll
+    for (int o = 0; o < 6; o++) {
+        const int cc = order[o];
+        const int a = (cc - 1) / 2;
+        const bool minus = (cc % 2 == 1);
+        const long long st = stride(d, a);
+        const long long nb = minus ? n - st : n + st;
+        const long long op = minus ? n + st : n - st;
+        if (!(nf[nb] == 0 && nf[op] == 8)) continue;
+        double fs = 2.0 * f[nb];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            if (t == a) continue;
+            const long long tt = stride(d, t);
+            fs = fs + f_across(f, bcf, ind[t][nb], nb - tt);
+            fs = fs + f_across(f, bcf, ind[t][nb + tt], nb + tt);
+        }
+        if (fs > fm) {
+            code = cc;
+            fm = fs;
+        }
+    }
+    // orientation written after all candidates: the tests read only ==0 / ==8
+    nf[n] = code;
+    if (code == 7) atomicAdd(cnt + 1, 1);
+    else atomicAdd(cnt, 1);
+}
+
+// vf_cindc.f:35-50
+__global__ void __launch_bounds__(256) k_cindc(Dims d, const int *__restrict__ nf, int *indc)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int v = nf[n];
+    indc[n] = (v == 8 || v == -1) ? -1 : 0;
+}
+
+// Pressure of cells that turned gas -> fluid: F-weighted mean of the fluid
+// pressures of the 27-neighbourhood, vf_bsppfl.f:40-62 (p0 is the copy).
+__global__ void __launch_bounds__(256) k_bsppfl(Dims d, const int *__restrict__ nf,
+                                                const int *__restrict__ nf0,
+                                                const double *__restrict__ f,
+                                                const double *__restrict__ p0, double *p)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    if (!(nf[n] == 0 && nf0[n] == 8)) return;
+    double sumf = 0.0, sump = 0.0;
+    for (int kp = -1; kp <= 1; kp++)
+        for (int jp = -1; jp <= 1; jp++)
+            for (int ip = -1; ip <= 1; ip++) {
+                const long long m = n + ip + (long long)jp * d.ni + (long long)kp * d.nij;
+                if (nf[m] == 0) {
+                    const double fv = f[m];
+                    sumf = sumf + fv;
+                    sump = sump + fv * p0[m];
+                }
+            }
+    if (sumf < 1.0e-20) p[n] = 0.0;
+    else p[n] = sump / sumf;
+}
+
+// Surface and gas pressures, vf_bspp.f:36-62 (no air-pressure model).
+__global__ void __launch_bounds__(256) k_bspp(Dims d, Static s, const int *__restrict__ nf,
+                                              const double *__restrict__ f, double *p)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
+    const long long n = cid(d, i, j, k);
+    const int v = nf[n];
+    const double pps = 0.0;
+    const long long ni = d.ni, nij = d.nij;
+    double psi;
+    switch (v) {
+    case 8: p[n] = 0.0; break;
+    case 5: psi = met<2>(s, 3, k) / (0.5 * met<2>(s, 2, k - 1) + f[n] * met<2>(s, 2, k)); p[n] = (1.0 - psi) * p[n - nij] + psi * pps; break;
+    case 6: psi = met<2>(s, 3, k + 1) / (0.5 * met<2>(s, 2, k + 1) + f[n] * met<2>(s, 2, k)); p[n] = (1.0 - psi) * p[n + nij] + psi * pps; break;
+    case 3: psi = met<1>(s, 3, j) / (0.5 * met<1>(s, 2, j - 1) + f[n] * met<1>(s, 2, j)); p[n] = (1.0 - psi) * p[n - ni] + psi * pps; break;
+    case 4: psi = met<1>(s, 3, j + 1) / (0.5 * met<1>(s, 2, j + 1) + f[n] * met<1>(s, 2, j)); p[n] = (1.0 - psi) * p[n + ni] + psi * pps; break;
+    case 1: psi = met<0>(s, 3, i) / (0.5 * met<0>(s, 2, i - 1) + f[n] * met<0>(s, 2, i)); p[n] = (1.0 - psi) * p[n - 1] + psi * pps; break;
+    case 2: psi = met<0>(s, 3, i + 1) / (0.5 * met<0>(s, 2, i + 1) + f[n] * met<0>(s, 2, i)); p[n] = (1.0 - psi) * p[n + 1] + psi * pps; break;
+    default: break;
+    }
+}
+
+inline dim3 sgrid(long long nc)
+{
+    long long b = (nc + 255) / 256;
+    if (b > 8192) b = 8192;
+    return dim3((unsigned)b);
+}
+
+}  // namespace
+
+#define RET_IF(x)              \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+// VF_FNFINI + VF_FNFPRV + VF_CINDC on the device; returns the surface count.
+static int flag_cells(cad_ctx *c, int *nums)
+{
+    Static s = cad_static(c);
+    const Dims &d = c->d;
+    Fields &f = c->f;
+    dim3 g = cell_grid(d), b = cell_block();
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_nf_reset, g, b, 0, st, d, f.nf);
+    hipLaunchKernelGGL(k_nf_gas, g, b, 0, st, d, s, f.ff, f.bcf, f.nf, c->prm.flower, c->prm.fupper);
+    for (int it = 0; it < 100000; it++) {
+        hipLaunchKernelGGL(k_nf_surface, g, b, 0, st, d, f.nf);
+        hipLaunchKernelGGL(k_nf_nofluid, g, b, 0, st, d, f.nf);
+        hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 2, 0);
+        hipLaunchKernelGGL(k_nf_sandwich, g, b, 0, st, d, f.nf, c->icount + 2);
+        CAD_CHECK(hipGetLastError());
+        CAD_CHECK(hipMemcpyAsync(c->host_cnt + 2, c->icount + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        CAD_CHECK(hipStreamSynchronize(st));
+        if (c->host_cnt[2] <= 0) break;
+        hipLaunchKernelGGL(k_nf_fix, g, b, 0, st, d, f.nf);
+    }
+    hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 3, 0);
+    hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 4, 0);
+    hipLaunchKernelGGL(k_nf_orient, g, b, 0, st, d, s, f.ff, f.bcf, f.nf, c->icount + 3);
+    hipLaunchKernelGGL(k_cindc, g, b, 0, st, d, f.nf, f.indc);
+    CAD_CHECK(hipGetLastError());
+    CAD_CHECK(hipMemcpyAsync(c->host_cnt + 3, c->icount + 3, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    CAD_CHECK(hipStreamSynchronize(st));
+    if (c->host_cnt[4] != 0) {
+        cad_set_error("VF_FNFPRV: %d surface cells without an orientation", c->host_cnt[4]);
+        return CAD_ERR_ORIENTATION;
+    }
+    *nums = c->host_cnt[3];
+    return CAD_OK;
+}
+
+// VF_F1CAL, vf_f1cal.f:124-239
+int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
+{
+    Static s = cad_static(c);
+    const Dims &d = c->d;
+    Fields &f = c->f;
+    dim3 g = cell_grid(d), b = cell_block();
+    hipStream_t st = c->stream;
+    const double dt = c->prm.dtnow;
+    // NLIM = 0 then FCONV (vf_f1cal.f:124-137)
+    CAD_CHECK(hipMemsetAsync(f.nlim, 0, (size_t)d.nc * sizeof(int), st));
+    hipLaunchKernelGGL(k_fconv<0>, g, b, 0, st, d, s, f.uu, f.ff, f.bcf, f.nf, c->flf[0], f.nlim, dt, c->prm.flower);
+    hipLaunchKernelGGL(k_fconv<1>, g, b, 0, st, d, s, f.vv, f.ff, f.bcf, f.nf, c->flf[1], f.nlim, dt, c->prm.flower);
+    hipLaunchKernelGGL(k_fconv<2>, g, b, 0, st, d, s, f.ww, f.ff, f.bcf, f.nf, c->flf[2], f.nlim, dt, c->prm.flower);
+    // VF_FEULER (vf_f1cal.f:160)
+    hipLaunchKernelGGL(k_feuler, g, b, 0, st, d, s, c->flf[0], c->flf[1], c->flf[2], f.nf, f.ff, dt,
+                       (double)c->prm.loops);
+    // VF_FMOD1 (vf_f1cal.f:185)
+    hipLaunchKernelGGL(k_fmod_count, g, b, 0, st, d, f.nf, f.nlim);
+    hipLaunchKernelGGL(k_fmod_redist, g, b, 0, st, d, s, f.nf, f.nlim, f.ff);
+    hipLaunchKernelGGL(k_fmod_spill, g, b, 0, st, d, s, f.nf, f.ff);
+    hipLaunchKernelGGL(k_fmod_clamp, g, b, 0, st, d, f.nf, f.ff);
+    // VF_FCUT01 (vf_f1cal.f:188)
+    hipLaunchKernelGGL(k_fcut, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, d, s, f.nf, f.ff,
+                       c->prm.flower, c->prm.fupper, c->red_partial);
+    CAD_CHECK(hipGetLastError());
+    double sums[2];
+    RET_IF(cad_reduce_fetch(c, 2, sums));
+    *fsum = sums[0];
+    *fcut = sums[1];
+    // VF_BWFF, VF_BSUWN3, VF_BSUWEM (vf_f1cal.f:192-194)
+    if (d.numb > 0)
+        hipLaunchKernelGGL(k_bwff, dim3((d.numb + 255) / 256), dim3(256), 0, st, d, c->indb, f.ff, f.bcf);
+    hipLaunchKernelGGL(k_bsuwn3, g, b, 0, st, d, s, f.nf, f.uu, f.vv, f.ww, c->prm.ibsuw0);
+    hipLaunchKernelGGL(k_bsuwem_count, g, b, 0, st, d, f.nf, f.nlim);
+    hipLaunchKernelGGL(k_bsuwem, g, b, 0, st, d, f.nf, f.nlim, f.uu, f.vv, f.ww);
+    // NLIM = old NF (vf_f1cal.f:197-203)
+    hipLaunchKernelGGL(k_icopy, sgrid(d.nc), dim3(256), 0, st, d.nc, f.nf, f.nlim);
+    CAD_CHECK(hipGetLastError());
+    int nums = 0;
+    RET_IF(flag_cells(c, &nums));
+    c->nums = nums;
+    // VF_BSPPFL with the old flags in NLIM (vf_f1cal.f:230)
+    hipLaunchKernelGGL(k_dcopy, sgrid(d.nc), dim3(256), 0, st, d.nc, f.pp, c->yv);
+    hipLaunchKernelGGL(k_bsppfl, g, b, 0, st, d, f.nf, f.nlim, f.ff, c->yv, f.pp);
+    CAD_CHECK(hipGetLastError());
+    // boundary values (vf_f1cal.f:234-239)
+    RET_IF(cad_launch_bsuwt(c));
+    RET_IF(cad_launch_bsuwt2(c));
+    hipLaunchKernelGGL(k_bsuwn3, g, b, 0, st, d, s, f.nf, f.uu, f.vv, f.ww, c->prm.ibsuw0);
+    hipLaunchKernelGGL(k_bspp, g, b, 0, st, d, s, f.nf, f.ff, f.pp);
+    CAD_CHECK(hipGetLastError());
+    RET_IF(cad_launch_bwuwt(c));
+    RET_IF(cad_launch_bwpp(c));
+    return CAD_OK;
+}
+
+int cad_flag_cells(cad_ctx *c, int *nums) { return flag_cells(c, nums); }

@@ -54,6 +54,7 @@ def _cases():
         "tiny": (lambda: cfg.dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5, name="tiny"),
                  [1, 2, 3, 4, 5]),
         "beach_small": (lambda: cfg.beach_small(steps=40, name="beach_small"), [1, 2, 3, 20, 40]),
+        "corner3d": (lambda: cfg.corner3d(steps=60, name="corner3d"), [1, 2, 10, 30, 60]),
         "dam_a": (lambda: cfg.dam_break(name="dam_a"), [1, 100, 500]),
     }
 
