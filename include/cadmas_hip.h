@@ -82,6 +82,19 @@ int cad_download_state(cad_ctx *ctx, double *uu, double *vv, double *ww, double 
                        double *bcu, double *bcv, double *bcw, double *bcp, double *bcf, int *nf,
                        int *indc);
 
+/* Device part of VF_CINIT (vf_cinit.f:190-409), run once after the state is
+ * uploaded with F at its initial value (W-LEVEL / DEBUG F-BOX or a restart):
+ * F clip and volume sums (VF_FCUT01), boundary F (VF_BWFF), flags and INDC
+ * (VF_FNFINI, VF_FNFPRV, VF_CINDC), then boundary values (VF_BWUWN, VF_BSUWT,
+ * VF_BSUWT2, VF_BSUWN3, VF_BSPP, VF_BWUWT, VF_BWPP).  flags bit 0: set the
+ * initial velocity (MATE I.C. V, vf_cinit.f:216-258); bit 1: set the pressure
+ * RHO0*GRZ0*(W-LEVEL - z) on active cells (vf_cinit.f:283-296).  With flags=0
+ * the uploaded velocity and pressure are kept, as after a restart. */
+#define CAD_INIT_VELOCITY 1
+#define CAD_INIT_PRESSURE 2
+int cad_init_state(cad_ctx *ctx, double wvlvl, double uini, double vini, double wini, int flags,
+                   cad_step_info *info);
+
 /* Whole phases ------------------------------------------------------------ */
 /* VF_V1CAL (vf_v1cal.f:1-297): predictor, Poisson, correction, boundary values. */
 int cad_v1cal(cad_ctx *ctx, cad_step_info *info);

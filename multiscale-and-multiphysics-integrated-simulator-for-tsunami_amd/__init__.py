@@ -7,3 +7,5 @@ step's four phases as hand-written gfx950 HIP kernels.
 """
 from . import capi  # noqa: F401
 from . import configs  # noqa: F401
+from . import deck  # noqa: F401
+from . import layout  # noqa: F401

@@ -79,6 +79,8 @@ def lib():
     for name in ("cad_predictor", "cad_vpcoef", "cad_correction"):
         getattr(L, name).argtypes = [C.c_void_p]
     L.cad_cdiv00.argtypes = [C.c_void_p, PD]
+    L.cad_init_state.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_int, C.POINTER(StepInfo)]
     L.cad_download_matrix.argtypes = [C.c_void_p] + [PD] * 8
     L.cad_upload_matrix.argtypes = [C.c_void_p] + [PD] * 8 + [PI]
     L.cad_download_pt.argtypes = [C.c_void_p, PD]
@@ -89,7 +91,11 @@ def lib():
 EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad_upload_static",
            "cad_upload_state", "cad_download_state", "cad_v1cal", "cad_f1cal", "cad_step",
            "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
-           "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_")
+           "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
+           "cad_init_state")
+
+INIT_VELOCITY = 1
+INIT_PRESSURE = 2
 
 
 def _check(rc):
@@ -199,6 +205,12 @@ class Context:
     def _info_call(self, name):
         info = StepInfo()
         _check(getattr(lib(), name)(self._h, C.byref(info)))
+        return info.as_dict()
+
+    def init_state(self, wvlvl=0.0, uvw=(0.0, 0.0, 0.0), flags=INIT_VELOCITY | INIT_PRESSURE):
+        """Device part of VF_CINIT on the uploaded state (see cad_init_state)."""
+        info = StepInfo()
+        _check(lib().cad_init_state(self._h, wvlvl, uvw[0], uvw[1], uvw[2], flags, C.byref(info)))
         return info.as_dict()
 
     def step(self):

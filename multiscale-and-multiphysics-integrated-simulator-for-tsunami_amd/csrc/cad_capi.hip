@@ -37,6 +37,7 @@ Static cad_static(const cad_ctx *c)
 }
 
 int cad_flag_cells(cad_ctx *c, int *nums);
+int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, double *fsum, double *fcut);
 
 namespace {
 
@@ -510,6 +511,22 @@ int cad_step(cad_ctx *c, cad_step_info *info)
     if (rc == CAD_OK) rc = cad_f1cal(c, &loc);
     if (info) *info = loc;
     return rc;
+}
+
+int cad_init_state(cad_ctx *c, double wvlvl, double uini, double vini, double wini, int flags,
+                   cad_step_info *info)
+{
+    RET_IF(ready(c));
+    const double uvw[3] = {uini, vini, wini};
+    double fsum = 0.0, fcut = 0.0;
+    RET_IF(cad_init_phase(c, wvlvl, uvw, flags, &fsum, &fcut));
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->fsum = fsum;
+        info->fcut = fcut;
+        info->nums = c->nums;
+    }
+    return CAD_OK;
 }
 
 int cad_cdiv00(cad_ctx *c, double *dv)

@@ -604,6 +604,45 @@ __global__ void __launch_bounds__(256) k_bspp(Dims d, Static s, const int *__res
     }
 }
 
+// Initial velocity of VF_CINIT (vf_cinit.f:216-258): UINI on faces with a
+// fluid side that are not boundary faces.
+template <int A>
+__global__ void __launch_bounds__(256) k_init_vel(Dims d, Static s, const int *__restrict__ nf,
+                                                  double *q, double val)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    const int idx[3] = {i, j, k};
+    double out = 0.0;
+    if (idx[A] >= 3 && i >= 2 && j >= 2 && k >= 2 && i <= d.ni - 1 && j <= d.nj - 1 && k <= d.nk - 1) {
+        const int *ind = A == 0 ? s.indx : (A == 1 ? s.indy : s.indz);
+        if (ind[n] == 0 && nf[n - stride(d, A)] * nf[n] == 0) out = val;
+    }
+    q[n] = out;
+}
+
+// Initial pressure of VF_CINIT (vf_cinit.f:283-296): RHO0*GRZ0*(W-LEVEL - z_c)
+// on active cells, zero elsewhere.
+__global__ void __launch_bounds__(256) k_init_p(Dims d, Static s, const int *__restrict__ indc,
+                                                double *p, double rho0, double grz0, double wvlvl)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    double out = 0.0;
+    if (i >= 2 && i <= d.ni - 1 && j >= 2 && j <= d.nj - 1 && k >= 2 && k <= d.nk - 1 && indc[n] != -1)
+        out = rho0 * grz0 * (wvlvl - (met<2>(s, 1, k) + met<2>(s, 1, k + 1)) * 0.5);
+    p[n] = out;
+}
+
+__global__ void k_ifill(long long nc, int *a, int v)
+{
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
+         n += (long long)gridDim.x * blockDim.x)
+        a[n] = v;
+}
+
 inline dim3 sgrid(long long nc)
 {
     long long b = (nc + 255) / 256;
@@ -713,3 +752,47 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
 }
 
 int cad_flag_cells(cad_ctx *c, int *nums) { return flag_cells(c, nums); }
+
+// Device part of VF_CINIT (vf_cinit.f:190-409) after F holds its initial
+// value: clip and volume sums, boundary F, flags, INDC, optional initial
+// velocity / pressure, and the boundary values.
+int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, double *fsum, double *fcut)
+{
+    Static s = cad_static(c);
+    const Dims &d = c->d;
+    Fields &f = c->f;
+    dim3 g = cell_grid(d), b = cell_block();
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_fcut, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, d, s, f.nf, f.ff,
+                       c->prm.flower, c->prm.fupper, c->red_partial);
+    CAD_CHECK(hipGetLastError());
+    double sums[2];
+    RET_IF(cad_reduce_fetch(c, 2, sums));
+    *fsum = sums[0];
+    *fcut = sums[1];
+    if (d.numb > 0)
+        hipLaunchKernelGGL(k_bwff, dim3((d.numb + 255) / 256), dim3(256), 0, st, d, c->indb, f.ff, f.bcf);
+    hipLaunchKernelGGL(k_ifill, sgrid(d.nc), dim3(256), 0, st, d.nc, f.indc, -1);
+    CAD_CHECK(hipGetLastError());
+    int nums = 0;
+    RET_IF(flag_cells(c, &nums));
+    c->nums = nums;
+    if (flags & 1) {
+        hipLaunchKernelGGL(k_init_vel<0>, g, b, 0, st, d, s, f.nf, f.uu, uvw[0]);
+        hipLaunchKernelGGL(k_init_vel<1>, g, b, 0, st, d, s, f.nf, f.vv, uvw[1]);
+        hipLaunchKernelGGL(k_init_vel<2>, g, b, 0, st, d, s, f.nf, f.ww, uvw[2]);
+    }
+    if (flags & 2)
+        hipLaunchKernelGGL(k_init_p, g, b, 0, st, d, s, f.indc, f.pp, c->prm.rho0, c->prm.grz0, wvlvl);
+    CAD_CHECK(hipGetLastError());
+    RET_IF(cad_launch_bwuwn(c));
+    RET_IF(cad_launch_bsuwt(c));
+    RET_IF(cad_launch_bsuwt2(c));
+    hipLaunchKernelGGL(k_bsuwn3, g, b, 0, st, d, s, f.nf, f.uu, f.vv, f.ww, c->prm.ibsuw0);
+    hipLaunchKernelGGL(k_bspp, g, b, 0, st, d, s, f.nf, f.ff, f.pp);
+    CAD_CHECK(hipGetLastError());
+    RET_IF(cad_launch_bwuwt(c));
+    RET_IF(cad_launch_bwpp(c));
+    CAD_CHECK(hipStreamSynchronize(st));
+    return CAD_OK;
+}

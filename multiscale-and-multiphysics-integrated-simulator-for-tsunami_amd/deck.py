@@ -1,0 +1,297 @@
+"""Reader for the CADMAS-SURF/3D input deck (<name>.in), the subset the
+single-phase VOF-SMAC path uses.
+
+Grammar: Simulators/CADMAS-SURF-3D/Source code/A11111.in; dispatch
+vf_ii1inp.f:174-413; tokenising vf_zgetln.f ('#' starts a comment, words are
+separated by blanks).  Keywords outside the implemented path raise
+UnsupportedDeck instead of being ignored, so a deck that would make the
+reference run physics this build does not have is refused, not mis-run.
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+
+class DeckError(ValueError):
+    pass
+
+
+class UnsupportedDeck(DeckError):
+    pass
+
+
+def _num(w):
+    try:
+        return float(w.replace("D", "E").replace("d", "e"))
+    except ValueError as e:
+        raise DeckError(f"not a number: {w!r}") from e
+
+
+def _int(w):
+    try:
+        return int(w)
+    except ValueError as e:
+        raise DeckError(f"not an integer: {w!r}") from e
+
+
+@dataclass
+class BoundaryLine:
+    """One B.C. line: kind D/X/Y/Z, a face box (1-based as in the deck, None
+    for D), physics 'VP' or 'F', type code and values (vf_iiboun.f:143-260)."""
+    kind: str
+    box: Optional[Tuple[int, int, int, int, int, int]]
+    phys: str
+    code: int
+    values: Tuple[float, ...]
+
+
+@dataclass
+class Deck:
+    grid: dict = field(default_factory=dict)              # 'X','Y','Z' -> list of coords
+    obst: List[Tuple[int, int, int, int, int, int]] = field(default_factory=list)
+    porous: List[tuple] = field(default_factory=list)     # (kind, box, values)
+    plower: float = 1.0e-4
+    bcs: List[BoundaryLine] = field(default_factory=list)
+    # TIME (vf_iitime.f); the default AUTO step is not implemented
+    dt_const: Optional[float] = None
+    nend: int = 0
+    tend: float = 0.0
+    # MATE (vf_iimate.f, defaults vf_a2dflt.f:161-164)
+    wvlvl: float = 0.0
+    rho0: float = 1.0e3
+    anu0: float = 1.0e-6
+    grz0: float = 9.8
+    uini: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    # COMP (vf_iicomp.f, defaults vf_a2dflt.f:37-60)
+    scmvp: float = 1.0
+    icgtyp: int = 1
+    cgpara: float = 0.95
+    icgmax: int = 500
+    cgepsa: float = 1.0e-15
+    cgepsr: float = 1.0e-12
+    # OPTION (vf_iiopt.f, defaults vf_a2dflt.f)
+    wbub: float = 0.2
+    idrop: int = 1
+    ibsuw0: int = 0
+    loops: int = 1
+    vvmax: float = 30.0
+    # DEBUG F-BOX (vf_iidbg.f; cell indices already +1 like IDBGF)
+    fbox: Optional[Tuple[int, int, int, int, int, int, float]] = None
+
+    @property
+    def ncells(self):
+        return tuple(len(self.grid[a]) - 1 for a in "XYZ")
+
+
+_IGNORED_FILE = {"L/P", "GRP", "RSL", "TRN", "OBS", "RES"}
+
+
+def parse(text):
+    d = Deck()
+    lines = []
+    for raw in text.splitlines():
+        words = raw.split("#", 1)[0].split()
+        if words:
+            lines.append(words)
+    i = 0
+    while i < len(lines):
+        w = lines[i]
+        head = w[0]
+        i += 1
+        if head == "GRID":
+            if len(w) < 2 or w[1] not in ("X", "Y", "Z"):
+                raise DeckError("GRID needs X, Y or Z")
+            axis = w[1]
+            if axis in d.grid:
+                raise DeckError(f"GRID {axis} given twice")
+            vals, words = [], w[2:]
+            while True:
+                done = False
+                for t in words:
+                    if t == "END":
+                        done = True
+                        break
+                    vals.append(_num(t))
+                if done:
+                    break
+                if i >= len(lines):
+                    raise DeckError("GRID: END not found")
+                words = lines[i]
+                i += 1
+            if len(vals) < 2:
+                raise DeckError("GRID needs at least two coordinates")
+            for a, b in zip(vals, vals[1:]):
+                if b - a < 1.0e-6:
+                    raise DeckError("GRID coordinates must increase by at least 1e-6")
+            d.grid[axis] = vals
+        elif head == "OBST":
+            if len(w) >= 2 and w[1] == "FILE":
+                raise UnsupportedDeck("OBST FILE is not supported")
+            if len(w) < 7:
+                raise DeckError("OBST needs 6 cell indices")
+            d.obst.append(tuple(_int(x) for x in w[1:7]))
+        elif head == "POROUS":
+            kind = w[1] if len(w) > 1 else ""
+            if kind == "LIM":
+                d.plower = _num(w[2])
+            elif kind in ("V", "X", "Y", "Z"):
+                if len(w) < 10:
+                    raise DeckError(f"POROUS {kind} needs 6 indices and 2 values")
+                d.porous.append((kind, tuple(_int(x) for x in w[2:8]), (_num(w[8]), _num(w[9]))))
+            elif kind in ("CM", "CD"):
+                if len(w) < 9:
+                    raise DeckError(f"POROUS {kind} needs 6 indices and a value")
+                d.porous.append((kind, tuple(_int(x) for x in w[2:8]), (_num(w[8]),)))
+            else:
+                raise UnsupportedDeck(f"POROUS {kind} is not supported")
+        elif head == "B.C.":
+            d.bcs.append(_parse_bc(w))
+        elif head == "TIME":
+            if w[1] == "CONST":
+                d.dt_const = _num(w[2])
+            elif w[1] == "END":
+                d.nend = _int(w[2])
+                d.tend = _num(w[3]) if len(w) > 3 else 0.0
+            else:
+                raise UnsupportedDeck(f"TIME {w[1]}: only a constant time step is implemented")
+        elif head == "MATE":
+            key = w[1]
+            if key == "W-LEVEL":
+                d.wvlvl = _num(w[2])
+            elif key == "DENSITY":
+                d.rho0 = _num(w[2])
+            elif key == "K-VISC":
+                d.anu0 = _num(w[2])
+            elif key == "GRAVITY":
+                d.grz0 = _num(w[2])
+            elif key == "I.C." and len(w) >= 6 and w[2] == "V":
+                d.uini = (_num(w[3]), _num(w[4]), _num(w[5]))
+            else:
+                raise UnsupportedDeck(f"MATE {' '.join(w[1:3])} is not supported")
+        elif head == "COMP":
+            _parse_comp(d, w)
+        elif head == "OPTION":
+            _parse_option(d, w)
+        elif head == "DEBUG":
+            if w[1] == "F-BOX":
+                b = tuple(_int(x) + 1 for x in w[2:8])
+                d.fbox = b + (_num(w[8]),)
+            else:
+                raise UnsupportedDeck(f"DEBUG {w[1]} is not supported")
+        elif head == "FILE":
+            if len(w) < 2 or w[1] not in _IGNORED_FILE:
+                raise UnsupportedDeck(f"FILE {w[1] if len(w) > 1 else ''} is not supported")
+            if w[1] == "RES":
+                raise UnsupportedDeck("restart input is taken through rsl.read, not FILE RES")
+        elif head == "EQUATION":
+            if len(w) >= 3 and w[2] == "NOCALC":
+                continue
+            if len(w) >= 3 and w[1] == "CONCENTRATION" and _int(w[2]) == 0:
+                continue
+            raise UnsupportedDeck("k-epsilon, temperature and concentration equations are not implemented")
+        elif head == "PARALLEL":
+            raise UnsupportedDeck("PARALLEL: one device runs the whole grid")
+        elif head == "MODEL":
+            raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (wave makers, damping) is not implemented")
+        else:
+            raise DeckError(f"unknown header {head}")
+    for a in "XYZ":
+        if a not in d.grid:
+            raise DeckError(f"GRID {a} missing")
+    _validate(d)
+    return d
+
+
+def _parse_bc(w):
+    kind = w[1]
+    if kind == "D":
+        box, ip = None, 2
+    elif kind in ("X", "Y", "Z"):
+        box, ip = tuple(_int(x) for x in w[2:8]), 8
+    else:
+        raise DeckError(f"B.C. {kind}")
+    phys = w[ip]
+    t = w[ip + 1] if len(w) > ip + 1 else ""
+    if phys == "VP":
+        if t == "SLIP":
+            return BoundaryLine(kind, box, "VP", 1, (0.0, 0.0, 0.0, 0.0, 0.0))
+        if t == "NON-S":
+            return BoundaryLine(kind, box, "VP", 2, (0.0, 0.0, 0.0, 0.0, 0.0))
+        if t == "FIX-V":
+            bu, bv, bw = (_num(x) for x in w[ip + 2:ip + 5])
+            return BoundaryLine(kind, box, "VP", 3, (bu, bv, bw, 0.0, 0.0))
+        if t == "FREE":
+            return BoundaryLine(kind, box, "VP", 4, (0.0, 0.0, 0.0, 0.0, 0.0))
+        raise UnsupportedDeck(f"B.C. VP {t} (log-law walls) is not implemented")
+    if phys == "F":
+        if t == "FIX":
+            bf = _num(w[ip + 2])
+            if not 0.0 <= bf <= 1.0:
+                raise DeckError("B.C. F FIX value outside [0,1]")
+            return BoundaryLine(kind, box, "F", 1, (bf,))
+        if t == "FREE":
+            return BoundaryLine(kind, box, "F", 2, (0.0,))
+        raise DeckError(f"B.C. F {t}")
+    if phys in ("K", "E", "T", "C"):
+        return BoundaryLine(kind, box, phys, 0, ())   # equations not solved: no effect
+    raise DeckError(f"B.C. {phys}")
+
+
+def _parse_comp(d, w):
+    if w[1] == "SCHM":
+        if w[2] == "VP-DONOR":
+            d.scmvp = _num(w[3])
+        elif w[2] == "FF-DN-AC":
+            pass
+        elif w[2] == "FF-SLOPE":
+            raise UnsupportedDeck("COMP SCHM FF-SLOPE is not implemented")
+        elif w[2] in ("KE-DONOR", "T-DONOR", "C-DONOR"):
+            pass
+        else:
+            raise DeckError(f"COMP SCHM {w[2]}")
+    elif w[1] == "MTRX":
+        if w[2] == "ILUBCGSTAB":
+            d.icgtyp = 0
+        elif w[2] == "M-ILUBCGSTAB":
+            d.icgtyp = 1
+            d.cgpara = _num(w[3])
+        elif w[2] == "MAX-ITR":
+            d.icgmax = _int(w[3])
+        elif w[2] == "A-ERROR":
+            d.cgepsa = _num(w[3])
+        elif w[2] == "R-ERROR":
+            d.cgepsr = _num(w[3])
+        else:
+            raise DeckError(f"COMP MTRX {w[2]}")
+    else:
+        raise DeckError(f"COMP {w[1]}")
+
+
+def _parse_option(d, w):
+    key = w[1]
+    if key == "T-DOOR":
+        if w[2] == "BUB":
+            d.wbub = 0.0 if w[3] == "OFF" else _num(w[3])
+        elif w[2] == "DROP":
+            d.idrop = 0 if w[3] == "OFF" else 1
+        else:
+            raise DeckError(f"OPTION T-DOOR {w[2]}")
+    elif key == "S-CELL-VEL":
+        d.ibsuw0 = 1 if w[2] == "DU=0" else 0
+    elif key == "SUB-LOOP":
+        d.loops = _int(w[2])
+    elif key == "MAX-VELOCITY":
+        d.vvmax = _num(w[2])
+    else:
+        raise UnsupportedDeck(f"OPTION {key} is not implemented")
+
+
+def _validate(d):
+    # Defaults that switch on models outside this path (vf_a2dflt.f:151,205,223)
+    if d.wbub >= 1.0e-20:
+        raise UnsupportedDeck("T-DOOR bubble model is on by default: the deck must say OPTION T-DOOR BUB OFF")
+    if d.idrop >= 1:
+        raise UnsupportedDeck("T-DOOR droplet model is on by default: the deck must say OPTION T-DOOR DROP OFF")
+    if d.dt_const is None:
+        raise UnsupportedDeck("TIME CONST is required (the automatic time step is not implemented)")
+    if d.loops != 1:
+        raise UnsupportedDeck("OPTION SUB-LOOP > 1 is not implemented")

@@ -1450,6 +1450,62 @@ double orc_cdiv00(const orc_state *s)
     return sqrt(val);
 }
 
+/* Device-side part of VF_CINIT restated: vf_cinit.f:190-409 after F holds its
+ * initial value.  flags bit 0: initial velocity uvw on non-boundary faces with
+ * a fluid side (vf_cinit.f:216-258); bit 1: pressure RHO0*GRZ0*(WVLVL - z) on
+ * active cells (vf_cinit.f:283-296). */
+int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double wini, int flags)
+{
+    DECL_DIMS(s);
+    size_t nc = nij * (size_t)nk;
+    fcut01(s);
+    int rc = bwff(s);
+    if (rc) return rc;
+    fnfini(s);
+    rc = fnfprv(s);
+    if (rc) return rc;
+    for (size_t n = 0; n < nc; n++) s->indc[n] = -1;
+    cindc(s);
+    if (flags & 1) {
+        const double ini[3] = {uini, vini, wini};
+        double *vel[3] = {s->uu, s->vv, s->ww};
+        const int *ind[3] = {s->indx, s->indy, s->indz};
+        const size_t st[3] = {1, (size_t)ni, nij};
+        for (int a = 0; a < 3; a++) {
+            memset(vel[a], 0, nc * sizeof(double));
+            for (int k = (a == 2 ? 3 : 2); k <= nk - 1; k++)
+                for (int j = (a == 1 ? 3 : 2); j <= nj - 1; j++)
+                    for (int i = (a == 0 ? 3 : 2); i <= ni - 1; i++) {
+                        size_t n = ID(i, j, k);
+                        if (ind[a][n] == 0 && s->nf[n - st[a]] * s->nf[n] == 0) vel[a][n] = ini[a];
+                    }
+        }
+    }
+    if (flags & 2) {
+        memset(s->pp, 0, nc * sizeof(double));
+        for (int k = 2; k <= nk - 1; k++) {
+            double val = s->rho0 * s->grz0 * (wvlvl - (ZZ(1, k) + ZZ(1, k + 1)) * 0.5);
+            for (int j = 2; j <= nj - 1; j++)
+                for (int i = 2; i <= ni - 1; i++) {
+                    size_t n = ID(i, j, k);
+                    if (s->indc[n] != -1) s->pp[n] = val;
+                }
+        }
+    }
+    rc = bwuwn(s);
+    if (rc) return rc;
+    bsuwt_axis(s, 0);
+    bsuwt_axis(s, 1);
+    bsuwt_axis(s, 2);
+    bsuwt2(s);
+    bsuwn3(s);
+    bspp(s);
+    rc = bwuwt(s);
+    if (rc) return rc;
+    bwpp(s);
+    return 0;
+}
+
 int orc_step(orc_state *s, double *pt)
 {
     int rc = orc_v1cal(s, pt);

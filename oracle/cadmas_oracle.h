@@ -78,6 +78,8 @@ int orc_v1cal(orc_state *s, double *pt);
 int orc_f1cal(orc_state *s);
 /* VF_CDIV00 */
 double orc_cdiv00(const orc_state *s);
+/* the part of VF_CINIT that follows the initial F (clip, flags, p, BCs) */
+int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double wini, int flags);
 /* one SMAC step: V1CAL then F1CAL */
 int orc_step(orc_state *s, double *pt);
 

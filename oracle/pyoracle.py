@@ -71,6 +71,8 @@ def lib():
         L.orc_cdiv00.argtypes = [C.POINTER(OrcState)]
         L.orc_cdiv00.restype = C.c_double
         L.orc_step.argtypes = [C.POINTER(OrcState), PD]
+        L.orc_init_state.argtypes = [C.POINTER(OrcState), C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_int]
         _LIB = L
     return _LIB
 
@@ -164,6 +166,9 @@ class Fields:
         pt = np.zeros(self.shape, order="F")
         rc = lib().orc_step(C.byref(self.st), _pd(pt))
         return rc, pt
+
+    def init_state(self, wvlvl=0.0, uvw=(0.0, 0.0, 0.0), flags=3):
+        return lib().orc_init_state(C.byref(self.st), wvlvl, uvw[0], uvw[1], uvw[2], flags)
 
     def cdiv00(self):
         return lib().orc_cdiv00(C.byref(self.st))

@@ -1,0 +1,70 @@
+"""Deck reader and host-side field layout against the reference's own set-up.
+
+The reference's set-up arrays were recorded from its VF_V1CAL arguments on the
+first step (tests/golden/*_static.npz); the host layout must reproduce them
+bit for bit, including the boundary-face numbering order of VF_CINDX.
+"""
+import numpy as np
+import pytest
+
+import fixtures as fx
+
+CASES = ["tiny", "beach_small", "corner3d", "dam_a"]
+STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
+          "INDX", "INDY", "INDZ", "INDB", "BCVI")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_layout_matches_reference_setup(pkg, case):
+    st = fx.static(case)
+    lay = pkg.layout.from_text(str(st["DECK"]))
+    a = lay.static_arrays()
+    assert lay.numb == st["INDB"].shape[1]
+    for k in STATIC:
+        assert a[k].shape == st[k].shape, k
+        assert np.array_equal(a[k], st[k]), k
+
+
+def test_metrics_closed_form(pkg):
+    """VF_CGRID on a stretched axis: dx, cx=(dx_i+dx_{i-1})/2, reciprocals, ghosts."""
+    x = [0.0, 0.1, 0.3, 0.6, 1.0]
+    m = pkg.layout.grid_metrics(x)
+    dx = np.diff(x)
+    assert m.shape == (6, 6)
+    assert np.allclose(m[0, 1:], x)
+    assert np.allclose(m[1, 1:5], dx)
+    assert m[1, 0] == dx[0] and m[1, 5] == dx[-1]
+    assert np.isclose(m[0, 0], -dx[0])
+    assert np.allclose(m[2, 2:5], (dx[1:] + dx[:-1]) / 2)
+    assert np.allclose(m[4, 1:], 1 / m[2, 1:])
+    assert np.allclose(m[5, 1:], 1 / (2 * m[2, 1:]))
+    assert m[2, 0] == m[4, 0] == m[5, 0] == 0.0
+
+
+@pytest.mark.parametrize("text,exc", [
+    ("GRID X 0 1 END\nGRID Y 0 1 END\nGRID Z 0 1 END\nTIME CONST 1e-3\n", "UnsupportedDeck"),
+    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
+     "OPTION T-DOOR DROP OFF\nTIME AUTO 1e-3 0.1\n", "UnsupportedDeck"),
+    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
+     "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nMODEL WAVE-BC X- FUNC STREAM 5\n", "UnsupportedDeck"),
+    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
+     "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nPARALLEL X 3\n", "UnsupportedDeck"),
+    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nFOO 1\n", "DeckError"),
+])
+def test_deck_rejects_outside_the_path(pkg, text, exc):
+    with pytest.raises(getattr(pkg.deck, exc)):
+        pkg.deck.parse(text)
+
+
+def test_deck_default_bubble_model_must_be_switched_off(pkg):
+    text = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
+    with pytest.raises(pkg.deck.UnsupportedDeck, match="T-DOOR"):
+        pkg.deck.parse(text)
+
+
+def test_workload_decks_parse(pkg):
+    """The benchmark workloads are valid decks with the documented sizes."""
+    for name, cells in (("runup_b", (512, 32, 96)), ("dam_a", (64, 16, 48))):
+        case = pkg.configs.named(name)
+        d = pkg.deck.parse(case.deck)
+        assert d.ncells == cells
