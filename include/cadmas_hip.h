@@ -121,6 +121,13 @@ int cad_upload_matrix(cad_ctx *ctx, const double *ad, const double *ali, const d
                       const double *bb, const int *indc);
 int cad_download_pt(cad_ctx *ctx, double *pt);
 
+/* Measurement helpers (no counterpart in the reference): average duration of
+ * one MILU preconditioner application M^-1 y (vf_mzminv.f) on the current
+ * factorisation, timed with HIP events on the context's stream; and the number
+ * of active cells (INDC != -1) that the solver's vector operations touch. */
+int cad_time_minv(cad_ctx *ctx, int reps, float *ms_avg);
+int cad_count_active(cad_ctx *ctx, long long *active);
+
 /* Drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5) with the reference's by-reference
  * argument list (trailing underscore: callable from Fortran as
  * CALL CADMAS_HIP_M1BCGS(...)).  Host arrays in, host X0 out; the device

@@ -84,6 +84,8 @@ def lib():
     L.cad_download_matrix.argtypes = [C.c_void_p] + [PD] * 8
     L.cad_upload_matrix.argtypes = [C.c_void_p] + [PD] * 8 + [PI]
     L.cad_download_pt.argtypes = [C.c_void_p, PD]
+    L.cad_time_minv.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+    L.cad_count_active.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -92,7 +94,7 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_upload_state", "cad_download_state", "cad_v1cal", "cad_f1cal", "cad_step",
            "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
            "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
-           "cad_init_state")
+           "cad_init_state", "cad_time_minv", "cad_count_active")
 
 INIT_VELOCITY = 1
 INIT_PRESSURE = 2
@@ -248,6 +250,16 @@ class Context:
         arrs = [_f64(m[k], self.shape) for k in MATRIX]
         ic = _i32(indc, self.shape)
         _check(lib().cad_upload_matrix(self._h, *[_pd(a) for a in arrs], _pi(ic)))
+
+    def time_minv(self, reps=5):
+        ms = C.c_float()
+        _check(lib().cad_time_minv(self._h, reps, C.byref(ms)))
+        return ms.value
+
+    def count_active(self):
+        n = C.c_longlong()
+        _check(lib().cad_count_active(self._h, C.byref(n)))
+        return n.value
 
     def download_pt(self):
         pt = np.zeros(self.shape, order="F")

@@ -131,15 +131,14 @@ def beach_small(ni=40, nj=4, nk=20, dx=0.02, dz=0.01, steps=50, dt=2.0e-4, eps_r
 def solitary_ic(h, H, x0, g=9.81, rho=1000.0):
     """Boussinesq solitary wave eta = H sech^2(sqrt(3H/4h^3)(x-x0)), u = sqrt(g/h) eta.
 
-    Returns a function (layout) -> None that overwrites F, U and P of a set-up field
-    layout: F per column from the free-surface height, U on x-faces of wet cells,
-    P hydrostatic under the local surface.
+    Returns a function apply(xx, zz, nf, state) that overwrites state["FF"],
+    state["UU"] and state["PP"] of a set-up field layout: F per column from the
+    free-surface height, U on x-faces below the surface, P hydrostatic under the
+    local surface.
     """
     kk = np.sqrt(3.0 * H / (4.0 * h ** 3))
 
-    def apply(lay):
-        xx, zz = lay.XX, lay.ZZ
-        nf = lay.NF
+    def apply(xx, zz, nf, state):
         numi, numj, numk = nf.shape
         xc = 0.5 * (xx[0, :-1] + xx[0, 1:])
         xc = np.append(xc, xc[-1] + xx[1, -1])
@@ -154,15 +153,17 @@ def solitary_ic(h, H, x0, g=9.81, rho=1000.0):
         F[0, :, :] = F[-1, :, :] = 0.0
         F[:, 0, :] = F[:, -1, :] = 0.0
         F[:, :, 0] = F[:, :, -1] = 0.0
-        lay.FF[...] = F
+        state["FF"][...] = F
         zc = 0.5 * (zb + zt)
         P = rho * g * (surf[:, None] - zc[None, :])
-        lay.PP[...] = np.where(lay.FF > 0.0, np.broadcast_to(P[:, None, :], (numi, numj, numk)), 0.0)
+        state["PP"][...] = np.where(F > 0.0, np.broadcast_to(P[:, None, :], (numi, numj, numk)), 0.0)
         u = np.sqrt(g / h) * eta_f
         wet = zc[None, :] < (h + eta_f)[:, None]
         U = np.where(wet, u[:, None], 0.0)
-        lay.UU[...] = np.broadcast_to(U[:, None, :], (numi, numj, numk))
-        lay.UU[nf == -1] = 0.0
+        uu = state["UU"]
+        uu[...] = np.broadcast_to(U[:, None, :], (numi, numj, numk))
+        uu[nf == -1] = 0.0
+        uu[0, :, :] = uu[1, :, :] = uu[-1, :, :] = 0.0
     return apply
 
 

@@ -38,6 +38,7 @@ Static cad_static(const cad_ctx *c)
 
 int cad_flag_cells(cad_ctx *c, int *nums);
 int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, double *fsum, double *fcut);
+int cad_time_minv_impl(cad_ctx *c, int reps, float *ms_avg);
 
 namespace {
 
@@ -526,6 +527,25 @@ int cad_init_state(cad_ctx *c, double wvlvl, double uini, double vini, double wi
         info->fcut = fcut;
         info->nums = c->nums;
     }
+    return CAD_OK;
+}
+
+int cad_time_minv(cad_ctx *c, int reps, float *ms_avg)
+{
+    RET_IF(ready(c));
+    if (!ms_avg) return CAD_ERR_ARG;
+    return cad_time_minv_impl(c, reps, ms_avg);
+}
+
+int cad_count_active(cad_ctx *c, long long *active)
+{
+    RET_IF(ready(c));
+    if (!active) return CAD_ERR_ARG;
+    std::vector<int> h((size_t)c->d.nc);
+    CAD_CHECK(hipMemcpy(h.data(), c->f.indc, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+    long long n = 0;
+    for (int v : h) n += (v != -1);
+    *active = n;
     return CAD_OK;
 }
 

@@ -566,3 +566,24 @@ int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_ou
     *iter_out = iter;
     return CAD_OK;
 }
+
+// Average duration of one M^-1 application (forward + backward MILU sweeps of
+// vf_mzminv.f) on the context's current factorisation, timed with HIP events on
+// the library's stream.  Used by the benchmark to report the dominant kernel.
+int cad_time_minv_impl(cad_ctx *c, int reps, float *ms_avg)
+{
+    if (reps < 1) reps = 1;
+    hipEvent_t a, b;
+    CAD_CHECK(hipEventCreate(&a));
+    CAD_CHECK(hipEventCreate(&b));
+    CAD_CHECK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; r++) RET_IF(apply_minv(c, c->ae, c->yv));
+    CAD_CHECK(hipEventRecord(b, c->stream));
+    CAD_CHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    CAD_CHECK(hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms_avg = ms / reps;
+    return CAD_OK;
+}
