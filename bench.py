@@ -133,6 +133,7 @@ def main():
         pred.append(info["ms_predictor"])
         corr.append(info["ms_correction"])
         vof.append(info["ms_vof"])
+        log(f"bench: step {s} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms")
     wall = time.perf_counter() - t0
     log(f"bench: timed {args.steps} steps in {wall:.3f}s ITR={itr}")
     ms_step = 1e3 * wall / args.steps

@@ -128,6 +128,16 @@ int cad_download_pt(cad_ctx *ctx, double *pt);
 int cad_time_minv(cad_ctx *ctx, int reps, float *ms_avg);
 int cad_count_active(cad_ctx *ctx, long long *active);
 
+/* Schedule of the MILU triangular sweeps (vf_mzdcmp.f, vf_mzfrwd.f,
+ * vf_mzbkwd.f).  Both give the serial sweep's results bit for bit:
+ *   CAD_SWEEP_PIPELINED  (default) one launch per sweep, wavefront tiles of
+ *                        8x8 lines walking I, handing values on point to point;
+ *   CAD_SWEEP_HYPERPLANE one launch per hyperplane I+J+K = const.
+ * No reference counterpart: the reference sweeps serially. */
+#define CAD_SWEEP_PIPELINED 0
+#define CAD_SWEEP_HYPERPLANE 1
+int cad_set_sweep_schedule(cad_ctx *ctx, int schedule);
+
 /* Drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5) with the reference's by-reference
  * argument list (trailing underscore: callable from Fortran as
  * CALL CADMAS_HIP_M1BCGS(...)).  Host arrays in, host X0 out; the device

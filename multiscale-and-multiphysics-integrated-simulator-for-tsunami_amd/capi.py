@@ -86,6 +86,7 @@ def lib():
     L.cad_download_pt.argtypes = [C.c_void_p, PD]
     L.cad_time_minv.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
     L.cad_count_active.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    L.cad_set_sweep_schedule.argtypes = [C.c_void_p, C.c_int]
     _lib = L
     return L
 
@@ -94,7 +95,11 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_upload_state", "cad_download_state", "cad_v1cal", "cad_f1cal", "cad_step",
            "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
            "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
-           "cad_init_state", "cad_time_minv", "cad_count_active")
+           "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule")
+
+# schedules of the MILU triangular sweeps (bit-identical results)
+SWEEP_PIPELINED = 0
+SWEEP_HYPERPLANE = 1
 
 INIT_VELOCITY = 1
 INIT_PRESSURE = 2
@@ -255,6 +260,9 @@ class Context:
         ms = C.c_float()
         _check(lib().cad_time_minv(self._h, reps, C.byref(ms)))
         return ms.value
+
+    def set_sweep_schedule(self, schedule):
+        _check(lib().cad_set_sweep_schedule(self._h, int(schedule)))
 
     def count_active(self):
         n = C.c_longlong()

@@ -155,7 +155,8 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
                       &c->anu, &c->cd0, &c->ggv, &c->ggx, &c->ggy, &c->ggz, &c->glv, &c->glx,
                       &c->gly, &c->glz, &c->ad, &c->ali, &c->alj, &c->alk, &c->aui, &c->auj,
                       &c->auk, &c->bb, &c->pt, &c->dd, &c->xw, &c->yv, &c->r0, &c->rr, &c->pv,
-                      &c->ap, &c->ae, &c->flf[0], &c->flf[1], &c->flf[2]};
+                      &c->ap, &c->ae, &c->flf[0], &c->flf[1], &c->flf[2], &c->lpi, &c->lpj,
+                      &c->lpk};
     for (double **p : f64) rc = rc ? rc : dalloc(p, nc);
     int **i32[] = {&c->f.nf, &c->f.indc, &c->f.nlim, &c->indx, &c->indy, &c->indz};
     for (int **p : i32) rc = rc ? rc : dalloc(p, nc);
@@ -168,6 +169,7 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
     rc = rc ? rc : dalloc(&c->red_partial, 4 * (size_t)REDUCE_BLOCKS);
     rc = rc ? rc : dalloc(&c->red_out, 16);
     rc = rc ? rc : dalloc(&c->icount, 16);
+    rc = rc ? rc : cad_sweep_alloc(c);
     if (rc) {
         cad_destroy(c);
         return rc;
@@ -206,9 +208,10 @@ void cad_destroy(cad_ctx *c)
                     c->r0, c->rr, c->pv, c->ap, c->ae, c->flf[0], c->flf[1], c->flf[2], c->f.nf,
                     c->f.indc, c->f.nlim, c->indx, c->indy, c->indz, c->f.bcu, c->f.bcv, c->f.bcw,
                     c->f.bcp, c->f.bcf, c->bcvi, c->indb, c->xx, c->yy, c->zz, c->red_partial,
-                    c->red_out, c->icount};
+                    c->red_out, c->icount, c->lpi, c->lpj, c->lpk};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    cad_sweep_free(c);
     if (c->host_red) hipHostFree(c->host_red);
     if (c->host_cnt) hipHostFree(c->host_cnt);
     for (auto &e : c->ev)
@@ -546,6 +549,16 @@ int cad_count_active(cad_ctx *c, long long *active)
     long long n = 0;
     for (int v : h) n += (v != -1);
     *active = n;
+    return CAD_OK;
+}
+
+int cad_set_sweep_schedule(cad_ctx *c, int schedule)
+{
+    if (!c || (schedule != CAD_SWEEP_PIPELINED && schedule != CAD_SWEEP_HYPERPLANE)) {
+        cad_set_error("cad_set_sweep_schedule: unknown schedule %d", schedule);
+        return CAD_ERR_ARG;
+    }
+    c->sweep_schedule = schedule;
     return CAD_OK;
 }
 

@@ -115,6 +115,17 @@ struct cad_ctx {
     double *ad, *ali, *alj, *alk, *aui, *auj, *auk, *bb, *pt;
     // BiCGSTAB work vectors (vf_m1bcgs.f: DD, XW, YY, R0, RR, PP, AP, AE)
     double *dd, *xw, *yv, *r0, *rr, *pv, *ap, *ae;
+    // MILU products ALI*DD(n-1), ALJ*DD(n-NI), ALK*DD(n-NIJ) of the pipelined sweeps
+    double *lpi, *lpj, *lpk;
+    // pipelined sweep schedule (cad_sweep.hip)
+    int sweep_schedule;                  // CAD_SWEEP_PIPELINED or CAD_SWEEP_HYPERPLANE
+    int sw_ntj, sw_ntk, sw_ntiles;
+    int *sw_order, *sw_range, *sw_err;
+    void *sw_mb;
+    unsigned long long sw_mb_bytes;
+    unsigned long long *sw_ticket;
+    unsigned long long sw_base;
+    unsigned sw_seq;
     // VOF work: F fluxes
     double *flf[3];
     // scratch for reductions and counters
@@ -145,4 +156,9 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut);
 int cad_reduce_dot(cad_ctx *c, const double *x, const double *y, const int *mask, double *out_dev);
 int cad_reduce_fetch(cad_ctx *c, int n, double *host_out);
 int cad_cdiv(cad_ctx *c, double *dv);
+int cad_sweep_alloc(cad_ctx *c);
+void cad_sweep_free(cad_ctx *c);
+int cad_sweep_dcmp(cad_ctx *c, double u, int *npos_host);
+int cad_sweep_minv(cad_ctx *c, double *x, const double *y);
+int cad_sweep_status(cad_ctx *c, int *code);
 Static cad_static(const cad_ctx *c);

@@ -1,0 +1,672 @@
+// cad_sweep.hip -- MILU triangular sweeps as one pipelined launch per sweep.
+//
+// The three recurrences of the MILU-preconditioned solver
+//   factorisation  DD(n) = 1/(AD - ALI*DD(n-1)*AU1 - ALJ*DD(n-NI)*AU2 - ALK*DD(n-NIJ)*AU3)
+//                                                         (vf_mzdcmp.f:53-64)
+//   forward        X(n)  = Y - ALI*DD(n-1)*X(n-1) - ALJ*DD(n-NI)*X(n-NI) - ALK*DD(n-NIJ)*X(n-NIJ)
+//                                                         (vf_mzfrwd.f:37-48)
+//   backward       X(n)  = (X - AUI*X(n+1) - AUJ*X(n+NI) - AUK*X(n+NIJ)) * DD(n)
+//                                                         (vf_mzbkwd.f:37-47)
+// each read the value just produced at the three upstream neighbours.  Every
+// cell here evaluates the reference's expression on the same operands in the
+// same order, so the results are the serial sweep's bits; only the schedule
+// differs.
+//
+// Schedule.  The (J,K) cross-section is cut into tiles of 8x8 lines along I.
+// One wavefront owns a tile; lane (jl,kl) owns line (J,K) and walks I with a
+// skew of jl+kl steps, so that at every step the J-1 and K-1 neighbours of its
+// cell were computed by lanes jl-1 / kl-1 one step earlier (a DPP row shift and
+// a ds_bpermute hand them over).  Tiles pass their last column / row to the
+// next tile through 16-byte mailbox entries {value, sequence number} written
+// and read with device-scope (sc1) accesses: one entry is one single-copy-atomic
+// 16-byte access, so a consumer that reads the current sequence number also
+// reads the value that goes with it, with no fence.  A tile obtains its place
+// in the dependency order from an atomic ticket, so a tile only ever waits on
+// tiles whose wavefronts are already running, and every wait has a bounded
+// spin count (exceeding it sets an error flag that the host reports).  Tiles
+// without an active cell exit at once; the others walk only the I range that
+// holds active cells.
+//
+// Memory.  The walk is cut into stages of S steps.  While stage b computes
+// from LDS, stage b+1's operands travel straight from HBM into the other LDS
+// buffer with buffer_load ... lds: each line's S-cell window (shifted by the
+// lane's skew) is one contiguous run, so one instruction moves 16*S contiguous
+// bytes of each of 128/S lines.  The group layout in LDS makes the per-step
+// reads of all 64 lanes bank-conflict free.  Mailbox entries for a stage
+// arrive the same way; results are staged in LDS and stored as contiguous
+// runs; mailbox feeds are stored every 8 steps.  The loads into LDS are not
+// tracked by the compiler's wait insertion, so the stage loop waits for them
+// explicitly.  Every access goes through a raw buffer resource sized to its
+// array: a load outside it returns 0 and a store outside it is dropped.
+//
+// Inactive cells.  Cells with INDC = -1 are not computed and keep the value
+// the output array holds, which for every array swept here is zero (the
+// solver clears its work vectors and DD at start and writes active cells
+// only), so an inactive neighbour contributes 0 exactly as the serial sweep
+// reads it from memory.  Activity is read from an operand stream instead of
+// INDC: the precomputed LP*/AU* arrays carry a marker value at inactive cells,
+// and DD is zero there and positive at active cells.
+#include "cad_common.h"
+
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+namespace {
+
+constexpr int TJ = 8, TK = 8;        // lines per tile in J and K (64 lanes)
+constexpr unsigned SPIN_LIMIT = 1u << 20;
+constexpr unsigned OOB = 0xFFFFFFF0u;
+constexpr int CPOL_DEVICE = 16;      // sc1: device-scope coherent access
+// inactive-cell marker: a signalling-NaN pattern that no arithmetic produces
+constexpr unsigned long long MARK = 0x7FF4DEADBEEF0001ull;
+
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef unsigned v3u __attribute__((ext_vector_type(3)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, unsigned long long bytes)
+{
+    const unsigned nrec = bytes > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (unsigned)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)nrec, 0x00020000);
+}
+
+// 16 bytes per lane from buffer offset off straight into LDS at lds + 16*lane
+template <int AUX>
+__device__ __forceinline__ void load_lds16(__amdgpu_buffer_rsrc_t r, void *lds, unsigned off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)lds, 16, off, 0, 0, AUX);
+}
+
+__device__ __forceinline__ bool is_mark(double x)
+{
+    return __builtin_bit_cast(unsigned long long, x) == MARK;
+}
+
+// {value, sequence number} of a mailbox entry: 12 bytes inside the 16-byte
+// entry that one store wrote
+__device__ __forceinline__ v3u mb_load(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, CPOL_DEVICE);
+}
+
+// Slow path of a mailbox read: polls the entry until it carries sequence
+// number seq, at most SPIN_LIMIT times (then flags err and returns what it
+// has).  Once any wait has given up, no wait of this schedule waits again, so
+// a failed launch still drains in about one spin limit.
+__device__ __attribute__((noinline)) v3u mb_wait(const void *mb, unsigned long long bytes,
+                                                 unsigned off, unsigned seq, int *err)
+{
+    const __amdgpu_buffer_rsrc_t r = rsrc(mb, bytes);
+    v3u e;
+    unsigned spins = 0;
+    do {
+        __builtin_amdgcn_s_sleep(1);
+        __asm__ __volatile__("" ::: "memory");
+        e = mb_load(r, off);
+        if ((++spins & 1023u) == 0u) {
+            if (spins > SPIN_LIMIT) atomicOr(err, 2);
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        }
+    } while (e.z != seq);
+    return e;
+}
+
+// value of lane-1 (row shift by one inside each row of 16 lanes; lanes with
+// jl == 0 take their J neighbour from the mailbox instead)
+__device__ __forceinline__ double from_lane_minus1(double x)
+{
+    v2u b = __builtin_bit_cast(v2u, x);
+    v2u r;
+    r.x = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b.x, 0x111, 0xF, 0xF, false);
+    r.y = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b.y, 0x111, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, r);
+}
+
+// value of lane-TJ
+__device__ __forceinline__ double from_lane_minus_tj(double x, int lane)
+{
+    v2u b = __builtin_bit_cast(v2u, x);
+    const int src = ((lane - TJ) & 63) << 2;
+    v2u r;
+    r.x = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)b.x);
+    r.y = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)b.y);
+    return __builtin_bit_cast(double, r);
+}
+
+struct SweepGeom {
+    int ni, nj, nk;
+    int ntj, ntk, ntiles;
+    long long nij, nc;
+};
+
+struct SweepCtl {
+    const int *order;               // tiles in dependency order
+    const int2 *range;              // per tile: first, last active I (0-based); x > y: none
+    v4u *mb;                        // mailboxes: J half then K half, each [tile][line 8][I]
+    unsigned long long mb_bytes;    // bytes of one half
+    unsigned long long *ticket;
+    unsigned long long base;
+    unsigned seq;
+    int *err;
+};
+
+// forward (L D^-1 + I) x = y with LPI = ALI*DD(n-1), LPJ = ALJ*DD(n-NI),
+// LPK = ALK*DD(n-NIJ) formed once per factorisation (the same products the
+// reference forms left to right before multiplying by X); LPI carries MARK at
+// inactive cells.
+struct FwdOp {
+    static constexpr int NS = 4;
+    static constexpr int S = 16;
+    const double *src[NS];          // Y, LPI, LPJ, LPK
+    double *out;                    // X
+    int *flag;
+    __device__ __forceinline__ static bool active(const double (&v)[NS]) { return !is_mark(v[1]); }
+    __device__ __forceinline__ static double eval(const double (&v)[NS], double xi, double xj, double xk)
+    {
+        return v[0] - v[1] * xi - v[2] * xj - v[3] * xk;
+    }
+    static constexpr bool CHECK_PIVOT = false;
+};
+
+// backward (D + U) x = x, in place; DD is 0 at inactive cells
+struct BwdOp {
+    static constexpr int NS = 5;
+    static constexpr int S = 16;
+    const double *src[NS];          // X, AUI, AUJ, AUK, DD
+    double *out;                    // X
+    int *flag;
+    __device__ __forceinline__ static bool active(const double (&v)[NS]) { return v[4] != 0.0; }
+    __device__ __forceinline__ static double eval(const double (&v)[NS], double xi, double xj, double xk)
+    {
+        return (v[0] - v[1] * xi - v[2] * xj - v[3] * xk) * v[4];
+    }
+    static constexpr bool CHECK_PIVOT = false;
+};
+
+// MILU diagonal; AU1..AU3 are the neighbour sums of vf_mzdcmp.f:57-59 (AU1
+// carries MARK at inactive cells)
+struct DcmpOp {
+    static constexpr int NS = 7;
+    static constexpr int S = 8;
+    const double *src[NS];          // AD, ALI, ALJ, ALK, AU1, AU2, AU3
+    double *out;                    // DD
+    int *flag;                      // cleared when a pivot is not positive
+    __device__ __forceinline__ static bool active(const double (&v)[NS]) { return !is_mark(v[4]); }
+    __device__ __forceinline__ static double eval(const double (&v)[NS], double xi, double xj, double xk)
+    {
+        return 1.0 / (v[0] - v[1] * xi * v[4] - v[2] * xj * v[5] - v[3] * xk * v[6]);
+    }
+    static constexpr bool CHECK_PIVOT = true;
+};
+
+// LDS layout of one operand stream of one stage: S window cells x 64 lines.
+// A load instruction q moves LPI lines (16 bytes = 2 cells per lane); lane l
+// takes line q*LPI + l%LPI, cells 2*(l/LPI) and 2*(l/LPI)+1, and lands at
+// gb(q) + 4*l dwords.  gb(q) mod 64 is 0, 2, 32, 34 for q = 0..3 (mod 4), so
+// the 32 lanes of a half-wave reading cell e of their own lines hit 32
+// distinct even banks.
+template <int S>
+struct Lay {
+    static constexpr int LPI = 128 / S;      // lines per load instruction
+    static constexpr int NG = 64 / LPI;      // load instructions per stream
+    static constexpr int GP = 320;           // group pitch (dwords)
+    static constexpr int WORDS = NG * GP;    // dwords per stream
+    __device__ __forceinline__ static int gb(int q)
+    {
+        return q * GP + 2 * (q & 1) + 32 * ((q >> 1) & 1);
+    }
+    __device__ __forceinline__ static int addr(int L, int e)
+    {
+        return gb(L / LPI) + (e >> 1) * (LPI * 4) + (L % LPI) * 4 + (e & 1) * 2;
+    }
+};
+
+template <class Op>
+struct __attribute__((aligned(16))) Smem {
+    static constexpr int S = Op::S, NS = Op::NS;
+    unsigned st[2][NS][Lay<S>::WORDS];       // operand streams, two stages
+    v4u halo[2][2][S * 8];                   // [stage][J,K][step*8 + line]
+    v4u feed[2][8 * 8];                      // [J,K][step%8 * 8 + line]
+    double xo[64][S + 1];                    // results of the stage, per line
+};
+
+template <class Op, int DIR>
+__global__ void __launch_bounds__(64) k_sweep(SweepGeom g, SweepCtl c, Op op)
+{
+    constexpr int NS = Op::NS, S = Op::S;
+    using LY = Lay<S>;
+    __shared__ Smem<Op> sm;
+    const int lane = threadIdx.x;
+    unsigned long long tk = 0;
+    if (lane == 0) tk = atomicAdd(c.ticket, 1ull);
+    const unsigned tlo = __builtin_amdgcn_readfirstlane((unsigned)tk);
+    const unsigned thi = __builtin_amdgcn_readfirstlane((unsigned)(tk >> 32));
+    const long long tix = (long long)((((unsigned long long)thi << 32) | tlo) - c.base);
+    if (tix < 0 || tix >= g.ntiles) {
+        if (lane == 0) atomicOr(c.err, 1);
+        return;
+    }
+    const int tile = c.order[DIR > 0 ? tix : g.ntiles - 1 - tix];
+    const int bj = tile % g.ntj, bk = tile / g.ntj;
+    const int2 rg = c.range[tile];
+    if (rg.x > rg.y) return;
+    const int len = rg.y - rg.x + 1;
+
+    // line geometry of logical line L = jl + TJ*kl
+    auto phys_j = [&] __device__(int jl) { return DIR > 0 ? jl : TJ - 1 - jl; };
+    auto phys_k = [&] __device__(int kl) { return DIR > 0 ? kl : TK - 1 - kl; };
+    auto line_ok = [&] __device__(int L) {
+        return 1 + bj * TJ + phys_j(L % TJ) <= g.nj - 2 && 1 + bk * TK + phys_k(L / TJ) <= g.nk - 2;
+    };
+    auto line_base = [&] __device__(int L) {
+        return (long long)(1 + bj * TJ + phys_j(L % TJ)) * g.ni +
+               (long long)(1 + bk * TK + phys_k(L / TJ)) * g.nij;
+    };
+    // first cell (ascending I) of line L's window in stage 0
+    auto win0 = [&] __device__(int L) {
+        const int sk = L % TJ + L / TJ;
+        return DIR > 0 ? rg.x - sk : rg.y - (S - 1) + sk;
+    };
+    auto cell_i = [&] __device__(int t, int sk) { return DIR > 0 ? rg.x + (t - sk) : rg.y - (t - sk); };
+
+    const int jl = lane % TJ, kl = lane / TJ;
+    const int skew = jl + kl;
+    const bool valid = line_ok(lane);
+
+    // per-lane byte offsets (stage 0) of the operand loads and result stores
+    unsigned ld_off[LY::NG];
+#pragma unroll
+    for (int q = 0; q < LY::NG; q++) {
+        const int L = q * LY::LPI + lane % LY::LPI;
+        const long long e = line_base(L) + win0(L) + 2 * (lane / LY::LPI);
+        ld_off[q] = line_ok(L) && e >= 0 ? (unsigned)(e * 8) : OOB;
+    }
+    constexpr int LPS = 64 / S;              // lines per result store
+    unsigned st_off[S];
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+        const int L = q * LPS + lane / S;
+        const long long e = line_base(L) + win0(L) + lane % S;
+        st_off[q] = line_ok(L) && e >= 0 ? (unsigned)(e * 8) : OOB;
+    }
+    const unsigned stage_step = (unsigned)(DIR * S * 8);   // byte step per stage (mod 2^32)
+
+    // mailboxes: producer tiles (uniform) and this tile's consumers
+    const int pbj = bj - DIR, pbk = bk - DIR;
+    const int ptj = pbj >= 0 && pbj < g.ntj ? pbj + bk * g.ntj : -1;
+    const int ptk = pbk >= 0 && pbk < g.ntk ? bj + pbk * g.ntj : -1;
+    const int2 prj = ptj >= 0 ? c.range[ptj] : make_int2(1, 0);
+    const int2 prk = ptk >= 0 ? c.range[ptk] : make_int2(1, 0);
+    const bool has_cj = bj + DIR >= 0 && bj + DIR < g.ntj;
+    const bool has_ck = bk + DIR >= 0 && bk + DIR < g.ntk;
+    const __amdgpu_buffer_rsrc_t rmb = rsrc(c.mb, 2 * c.mb_bytes);
+    const unsigned half_k = (unsigned)c.mb_bytes;
+    auto mb_off = [&] __device__(int tl, int l, int i) {
+        return (unsigned)((((long long)tl * 8 + l) * g.ni + i) * 16);
+    };
+    const unsigned seq = c.seq;
+
+    __amdgpu_buffer_rsrc_t rs[NS];
+#pragma unroll
+    for (int m = 0; m < NS; m++) rs[m] = rsrc(op.src[m], (unsigned long long)g.nc * 8);
+    const __amdgpu_buffer_rsrc_t rout = rsrc(op.out, (unsigned long long)g.nc * 8);
+
+    // issue the loads of stage b into LDS buffer b&1
+    auto load_stage = [&] __device__(int b) {
+        const int buf = b & 1;
+        const unsigned adv = stage_step * (unsigned)b;
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+#pragma unroll
+            for (int q = 0; q < LY::NG; q++) {
+                const unsigned o = ld_off[q] == OOB ? OOB : ld_off[q] + adv;
+                load_lds16<0>(rs[m], &sm.st[buf][m][LY::gb(q)], o);
+            }
+        }
+        // mailbox entries: lane l takes line l%8 at step 8h + l/8 of the stage
+#pragma unroll
+        for (int h = 0; h < S / 8; h++) {
+            const int l8 = lane % 8, t = b * S + 8 * h + lane / 8;
+            const int il = cell_i(t, l8);
+            const bool in_walk = (unsigned)(t - l8) < (unsigned)len;
+            // J halo of lane (0, l8)
+            const bool nj_ = ptj >= 0 && line_ok(l8 * TJ) && in_walk && il >= prj.x && il <= prj.y;
+            load_lds16<CPOL_DEVICE>(rmb, &sm.halo[buf][0][64 * h], nj_ ? mb_off(ptj, phys_k(l8), il) : OOB);
+            // K halo of lane (l8, 0)
+            const bool nk_ = ptk >= 0 && line_ok(l8) && in_walk && il >= prk.x && il <= prk.y;
+            load_lds16<CPOL_DEVICE>(rmb, &sm.halo[buf][1][64 * h], nk_ ? half_k + mb_off(ptk, phys_j(l8), il) : OOB);
+        }
+    };
+
+    // value of a mailbox entry staged in LDS; polls global memory (bounded)
+    // when the staged copy is older than this launch
+    auto halo = [&] __device__(v4u e, unsigned off) {
+        if (e.z != seq) {
+            v3u f = mb_wait(c.mb, 2 * c.mb_bytes, off, seq, c.err);
+            e.x = f.x;
+            e.y = f.y;
+        }
+        v2u b = {e.x, e.y};
+        return __builtin_bit_cast(double, b);
+    };
+
+    // feeds of steps t0..t0+7 to the downstream tiles: lane l stores the entry
+    // of line l%8 at step t0 + l/8
+    auto flush_feed = [&] __device__(int t0) {
+        const int l8 = lane % 8, t = t0 + lane / 8;
+        {   // J feed: column jl = TJ-1, row l8
+            const int L = (TJ - 1) + TJ * l8, sk = (TJ - 1) + l8;
+            const bool ok = has_cj && line_ok(L) && (unsigned)(t - sk) < (unsigned)len;
+            const v4u e = sm.feed[0][(lane / 8) * 8 + l8];
+            __builtin_amdgcn_raw_buffer_store_b128(e, rmb, ok ? mb_off(tile, phys_k(l8), cell_i(t, sk)) : OOB, 0, CPOL_DEVICE);
+        }
+        {   // K feed: row kl = TK-1, column l8
+            const int L = l8 + TJ * (TK - 1), sk = l8 + (TK - 1);
+            const bool ok = has_ck && line_ok(L) && (unsigned)(t - sk) < (unsigned)len;
+            const v4u e = sm.feed[1][(lane / 8) * 8 + l8];
+            __builtin_amdgcn_raw_buffer_store_b128(e, rmb, ok ? half_k + mb_off(tile, phys_j(l8), cell_i(t, sk)) : OOB, 0, CPOL_DEVICE);
+        }
+    };
+
+    double xcur = 0.0;
+    bool bad_pivot = false;
+    const int nsteps = len + (TJ - 1) + (TK - 1);
+    const int nstage = (nsteps + S - 1) / S;
+
+    load_stage(0);
+    __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
+    for (int b = 0; b < nstage; b++) {
+        const int buf = b & 1;
+        if (b + 1 < nstage) load_stage(b + 1);
+#pragma unroll
+        for (int p = 0; p < S; p++) {
+            const int t = b * S + p;
+            const bool lv = valid && (unsigned)(t - skew) < (unsigned)len;
+            const int i = cell_i(t, skew);
+            const int e = DIR > 0 ? p : S - 1 - p;
+            double v[NS];
+#pragma unroll
+            for (int m = 0; m < NS; m++) {
+                v[m] = *reinterpret_cast<const double *>(&sm.st[buf][m][LY::addr(lane, e)]);
+            }
+            const double xi = xcur;
+            const double sj = from_lane_minus1(xcur);
+            const double sk = from_lane_minus_tj(xcur, lane);
+            const bool act = lv && Op::active(v);
+            double xj = sj, xk = sk;
+            if (jl == 0) {
+                const bool need = act && ptj >= 0 && i >= prj.x && i <= prj.y;
+                xj = need ? halo(sm.halo[buf][0][p * 8 + kl], mb_off(ptj, phys_k(kl), i)) : 0.0;
+            }
+            if (kl == 0) {
+                const bool need = act && ptk >= 0 && i >= prk.x && i <= prk.y;
+                xk = need ? halo(sm.halo[buf][1][p * 8 + jl], half_k + mb_off(ptk, phys_j(jl), i)) : 0.0;
+            }
+            const double ev = Op::eval(v, xi, xj, xk);
+            const double r = act ? ev : 0.0;
+            if (Op::CHECK_PIVOT) bad_pivot = bad_pivot || (act && r <= 0.0);
+            xcur = r;
+            sm.xo[lane][e] = act ? r : __builtin_bit_cast(double, MARK);
+            const v2u rb = __builtin_bit_cast(v2u, r);
+            const v4u fe = {rb.x, rb.y, seq, seq};
+            if (jl == TJ - 1) sm.feed[0][(p % 8) * 8 + kl] = fe;
+            if (kl == TK - 1) sm.feed[1][(p % 8) * 8 + jl] = fe;
+            if (p % 8 == 7) flush_feed(t - 7);
+        }
+        // results of the stage: store q covers lines q*LPS .. q*LPS+LPS-1
+        const unsigned adv = stage_step * (unsigned)b;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            const int L = q * LPS + lane / S, e = lane % S;
+            const double x = sm.xo[L][e];
+            const unsigned o = st_off[q] == OOB || is_mark(x) ? OOB : st_off[q] + adv;
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), rout, o, 0, 0);
+        }
+        // the next stage's loads are older than the 2*S/8 feed stores and S
+        // result stores issued since
+        if (S == 16) __asm__ __volatile__("s_waitcnt vmcnt(20)" ::: "memory");
+        else __asm__ __volatile__("s_waitcnt vmcnt(10)" ::: "memory");
+    }
+    if (Op::CHECK_PIVOT && bad_pivot) __hip_atomic_store(op.flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Active I range of every tile (INDC != -1 on its lines).
+__global__ void __launch_bounds__(256) k_tile_range(SweepGeom g, const int *__restrict__ indc,
+                                                    int2 *__restrict__ range)
+{
+    const int tile = blockIdx.x;
+    const int bj = tile % g.ntj, bk = tile / g.ntj;
+    int lo = INT_MAX, hi = -1;
+    for (int l = 0; l < TJ * TK; l++) {
+        const int j = 1 + bj * TJ + l % TJ, k = 1 + bk * TK + l / TJ;
+        if (j > g.nj - 2 || k > g.nk - 2) continue;
+        const int *row = indc + (long long)j * g.ni + (long long)k * g.nij;
+        for (int i = 1 + threadIdx.x; i <= g.ni - 2; i += blockDim.x) {
+            if (row[i] != -1) {
+                lo = min(lo, i);
+                hi = max(hi, i);
+            }
+        }
+    }
+    __shared__ int slo[256], shi[256];
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + w]);
+            shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) range[tile] = shi[0] < 0 ? make_int2(1, 0) : make_int2(slo[0], shi[0]);
+}
+
+__device__ __forceinline__ bool interior(const Dims &d, int i, int j, int k)
+{
+    return i >= 2 && j >= 2 && k >= 2 && i <= d.ni - 1 && j <= d.nj - 1 && k <= d.nk - 1;
+}
+
+// AU1..AU3 of vf_mzdcmp.f:57-59 on active interior cells; AU1 = MARK on
+// inactive interior cells
+__global__ void __launch_bounds__(256) k_dcmp_pre(Dims d, const double *__restrict__ aui,
+                                                  const double *__restrict__ auj,
+                                                  const double *__restrict__ auk,
+                                                  const int *__restrict__ indc, double u,
+                                                  double *__restrict__ au1, double *__restrict__ au2,
+                                                  double *__restrict__ au3)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k) || !interior(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    if (indc[n] == -1) {
+        au1[n] = __builtin_bit_cast(double, MARK);
+        return;
+    }
+    const long long ni = d.ni, nij = d.nij;
+    au1[n] = aui[n - 1] + u * (auj[n - 1] + auk[n - 1]);
+    au2[n] = auj[n - ni] + u * (aui[n - ni] + auk[n - ni]);
+    au3[n] = auk[n - nij] + u * (aui[n - nij] + auj[n - nij]);
+}
+
+// LPI = ALI*DD(n-1), LPJ = ALJ*DD(n-NI), LPK = ALK*DD(n-NIJ); LPI = MARK on
+// inactive interior cells
+__global__ void __launch_bounds__(256) k_lprod(Dims d, const double *__restrict__ ali,
+                                               const double *__restrict__ alj,
+                                               const double *__restrict__ alk,
+                                               const double *__restrict__ dd,
+                                               const int *__restrict__ indc, double *__restrict__ lpi,
+                                               double *__restrict__ lpj, double *__restrict__ lpk)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k) || !interior(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    if (indc[n] == -1) {
+        lpi[n] = __builtin_bit_cast(double, MARK);
+        return;
+    }
+    lpi[n] = ali[n] * dd[n - 1];
+    lpj[n] = alj[n] * dd[n - d.ni];
+    lpk[n] = alk[n] * dd[n - d.nij];
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+SweepGeom geom(const cad_ctx *c)
+{
+    SweepGeom g;
+    g.ni = c->d.ni;
+    g.nj = c->d.nj;
+    g.nk = c->d.nk;
+    g.ntj = c->sw_ntj;
+    g.ntk = c->sw_ntk;
+    g.ntiles = c->sw_ntiles;
+    g.nij = c->d.nij;
+    g.nc = c->d.nc;
+    return g;
+}
+
+SweepCtl ctl(cad_ctx *c)
+{
+    SweepCtl s;
+    s.order = c->sw_order;
+    s.range = reinterpret_cast<const int2 *>(c->sw_range);
+    s.mb = reinterpret_cast<v4u *>(c->sw_mb);
+    s.mb_bytes = c->sw_mb_bytes;
+    s.ticket = c->sw_ticket;
+    s.base = c->sw_base;
+    s.seq = c->sw_seq;
+    s.err = c->sw_err;
+    return s;
+}
+
+template <class Op, int DIR>
+int launch(cad_ctx *c, const Op &op)
+{
+    SweepGeom g = geom(c);
+    SweepCtl s = ctl(c);
+    hipLaunchKernelGGL((k_sweep<Op, DIR>), dim3(g.ntiles), dim3(64), 0, c->stream, g, s, op);
+    CAD_CHECK(hipGetLastError());
+    c->sw_base += (unsigned long long)g.ntiles;
+    c->sw_seq += 1;
+    return CAD_OK;
+}
+
+}  // namespace
+
+int cad_sweep_alloc(cad_ctx *c)
+{
+    const Dims &d = c->d;
+    c->sw_ntj = (d.nj - 2 + TJ - 1) / TJ;
+    c->sw_ntk = (d.nk - 2 + TK - 1) / TK;
+    c->sw_ntiles = c->sw_ntj * c->sw_ntk;
+    const int nt = c->sw_ntiles;
+    std::vector<int> order(nt);
+    for (int t = 0; t < nt; t++) order[t] = t;
+    const int ntj = c->sw_ntj;
+    std::stable_sort(order.begin(), order.end(), [ntj](int a, int b) {
+        const int da = a % ntj + a / ntj, db = b % ntj + b / ntj;
+        if (da != db) return da < db;
+        return a % ntj < b % ntj;
+    });
+    c->sw_mb_bytes = (unsigned long long)nt * 8 * d.ni * 16;
+    if (2 * c->sw_mb_bytes > 0xFFFFFFF0ull || (unsigned long long)d.nc * 8 > 0xFFFFFFF0ull) {
+        cad_set_error("grid too large for 32-bit sweep offsets (%d x %d x %d)", d.ni, d.nj, d.nk);
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipMalloc((void **)&c->sw_order, nt * sizeof(int)));
+    CAD_CHECK(hipMalloc((void **)&c->sw_range, nt * 2 * sizeof(int)));
+    CAD_CHECK(hipMalloc(&c->sw_mb, 2 * c->sw_mb_bytes));
+    CAD_CHECK(hipMalloc((void **)&c->sw_ticket, sizeof(unsigned long long)));
+    CAD_CHECK(hipMalloc((void **)&c->sw_err, 2 * sizeof(int)));
+    CAD_CHECK(hipMemcpy(c->sw_order, order.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+    CAD_CHECK(hipMemset(c->sw_mb, 0, 2 * c->sw_mb_bytes));
+    CAD_CHECK(hipMemset(c->sw_ticket, 0, sizeof(unsigned long long)));
+    CAD_CHECK(hipMemset(c->sw_err, 0, 2 * sizeof(int)));
+    CAD_CHECK(hipDeviceSynchronize());
+    c->sw_base = 0;
+    c->sw_seq = 1;
+    return CAD_OK;
+}
+
+void cad_sweep_free(cad_ctx *c)
+{
+    void *p[] = {c->sw_order, c->sw_range, c->sw_mb, c->sw_ticket, c->sw_err};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+}
+
+// Tile ranges for the current INDC, the MILU diagonal DD and the products
+// LPI/LPJ/LPK.  *npos_host = 0 when a pivot is not positive (vf_mzdcmp.f:61).
+int cad_sweep_dcmp(cad_ctx *c, double u, int *npos_host)
+{
+    const Dims &d = c->d;
+    SweepGeom g = geom(c);
+    hipLaunchKernelGGL(k_tile_range, dim3(g.ntiles), dim3(256), 0, c->stream, g, c->f.indc,
+                       reinterpret_cast<int2 *>(c->sw_range));
+    hipLaunchKernelGGL(k_dcmp_pre, cell_grid(d), cell_block(), 0, c->stream, d, c->aui, c->auj,
+                       c->auk, c->f.indc, u, c->lpi, c->lpj, c->lpk);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->icount + 1, 1);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->sw_err, 0);
+    CAD_CHECK(hipGetLastError());
+    DcmpOp op;
+    op.src[0] = c->ad;
+    op.src[1] = c->ali;
+    op.src[2] = c->alj;
+    op.src[3] = c->alk;
+    op.src[4] = c->lpi;
+    op.src[5] = c->lpj;
+    op.src[6] = c->lpk;
+    op.out = c->dd;
+    op.flag = c->icount + 1;
+    if (int rc = launch<DcmpOp, 1>(c, op)) return rc;
+    hipLaunchKernelGGL(k_lprod, cell_grid(d), cell_block(), 0, c->stream, d, c->ali, c->alj,
+                       c->alk, c->dd, c->f.indc, c->lpi, c->lpj, c->lpk);
+    CAD_CHECK(hipGetLastError());
+    CAD_CHECK(hipMemcpyAsync(c->host_cnt + 1, c->icount + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipMemcpyAsync(c->host_cnt + 2, c->sw_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    if (c->host_cnt[2] != 0) {
+        cad_set_error("MILU sweep schedule failed (code %d)", c->host_cnt[2]);
+        return CAD_ERR_HIP;
+    }
+    *npos_host = c->host_cnt[1];
+    return CAD_OK;
+}
+
+// x = M^-1 y (vf_mzminv.f:38-41): forward then backward sweep, in place on x.
+int cad_sweep_minv(cad_ctx *c, double *x, const double *y)
+{
+    FwdOp f;
+    f.src[0] = y;
+    f.src[1] = c->lpi;
+    f.src[2] = c->lpj;
+    f.src[3] = c->lpk;
+    f.out = x;
+    f.flag = nullptr;
+    if (int rc = launch<FwdOp, 1>(c, f)) return rc;
+    BwdOp b;
+    b.src[0] = x;
+    b.src[1] = c->aui;
+    b.src[2] = c->auj;
+    b.src[3] = c->auk;
+    b.src[4] = c->dd;
+    b.out = x;
+    b.flag = nullptr;
+    return launch<BwdOp, -1>(c, b);
+}
+
+// Error flag of the sweeps since the last call (0 = none); clears it.
+int cad_sweep_status(cad_ctx *c, int *code)
+{
+    CAD_CHECK(hipMemcpyAsync(c->host_cnt + 2, c->sw_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    *code = c->host_cnt[2];
+    if (*code) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->sw_err, 0);
+        CAD_CHECK(hipGetLastError());
+    }
+    return CAD_OK;
+}
