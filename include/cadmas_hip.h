@@ -129,13 +129,20 @@ int cad_time_minv(cad_ctx *ctx, int reps, float *ms_avg);
 int cad_count_active(cad_ctx *ctx, long long *active);
 
 /* Schedule of the MILU triangular sweeps (vf_mzdcmp.f, vf_mzfrwd.f,
- * vf_mzbkwd.f).  Both give the serial sweep's results bit for bit:
- *   CAD_SWEEP_PIPELINED  (default) one launch per sweep, wavefront tiles of
- *                        8x8 lines walking I, handing values on point to point;
- *   CAD_SWEEP_HYPERPLANE one launch per hyperplane I+J+K = const.
+ * vf_mzbkwd.f).  All give the serial sweep's results bit for bit:
+ *   CAD_SWEEP_PIPELINED       (default) one launch per sweep, wavefront tiles
+ *                             of 64 lines walking I and handing values on
+ *                             point to point; the tile shape (8x8 or 16x4
+ *                             lines in J x K) is chosen per solve from the
+ *                             active region;
+ *   CAD_SWEEP_PIPELINED_8X8,
+ *   CAD_SWEEP_PIPELINED_16X4  the same with a fixed tile shape;
+ *   CAD_SWEEP_HYPERPLANE      one launch per hyperplane I+J+K = const.
  * No reference counterpart: the reference sweeps serially. */
 #define CAD_SWEEP_PIPELINED 0
 #define CAD_SWEEP_HYPERPLANE 1
+#define CAD_SWEEP_PIPELINED_8X8 2
+#define CAD_SWEEP_PIPELINED_16X4 3
 int cad_set_sweep_schedule(cad_ctx *ctx, int schedule);
 
 /* Drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5) with the reference's by-reference

@@ -100,6 +100,8 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
 SWEEP_HYPERPLANE = 1
+SWEEP_PIPELINED_8X8 = 2
+SWEEP_PIPELINED_16X4 = 3
 
 INIT_VELOCITY = 1
 INIT_PRESSURE = 2

@@ -554,7 +554,7 @@ int cad_count_active(cad_ctx *c, long long *active)
 
 int cad_set_sweep_schedule(cad_ctx *c, int schedule)
 {
-    if (!c || (schedule != CAD_SWEEP_PIPELINED && schedule != CAD_SWEEP_HYPERPLANE)) {
+    if (!c || schedule < CAD_SWEEP_PIPELINED || schedule > CAD_SWEEP_PIPELINED_16X4) {
         cad_set_error("cad_set_sweep_schedule: unknown schedule %d", schedule);
         return CAD_ERR_ARG;
     }

@@ -117,10 +117,11 @@ struct cad_ctx {
     double *dd, *xw, *yv, *r0, *rr, *pv, *ap, *ae;
     // MILU products ALI*DD(n-1), ALJ*DD(n-NI), ALK*DD(n-NIJ) of the pipelined sweeps
     double *lpi, *lpj, *lpk;
-    // pipelined sweep schedule (cad_sweep.hip)
+    // pipelined sweep schedule (cad_sweep.hip); tile shape 0 = 8x8 lines, 1 = 16x4
     int sweep_schedule;                  // CAD_SWEEP_PIPELINED or CAD_SWEEP_HYPERPLANE
-    int sw_ntj, sw_ntk, sw_ntiles;
-    int *sw_order, *sw_range, *sw_err;
+    int sw_shape;
+    int sw_ntj[2], sw_ntk[2], sw_ntiles[2];
+    int *sw_order[2], *sw_range[2], *sw_err;
     void *sw_mb;
     unsigned long long sw_mb_bytes;
     unsigned long long *sw_ticket;

@@ -412,7 +412,7 @@ static int hyper_launch_range(const Dims &d, int h, Plane &pl, unsigned &blocks)
 
 static int sweep_dcmp(cad_ctx *c, double u, int *npos_host)
 {
-    if (c->sweep_schedule == CAD_SWEEP_PIPELINED) return cad_sweep_dcmp(c, u, npos_host);
+    if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) return cad_sweep_dcmp(c, u, npos_host);
     const Dims &d = c->d;
     hipLaunchKernelGGL(k_iset1, dim3(1), dim3(1), 0, c->stream, c->icount + 1, 1);
     const int hmin = 6, hmax = (d.ni - 1) + (d.nj - 1) + (d.nk - 1);
@@ -434,7 +434,7 @@ static int sweep_dcmp(cad_ctx *c, double u, int *npos_host)
 // x = M^-1 y  (vf_mzminv.f:38-41)
 static int apply_minv(cad_ctx *c, double *x, const double *y)
 {
-    if (c->sweep_schedule == CAD_SWEEP_PIPELINED) return cad_sweep_minv(c, x, y);
+    if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) return cad_sweep_minv(c, x, y);
     const Dims &d = c->d;
     const int hmin = 6, hmax = (d.ni - 1) + (d.nj - 1) + (d.nk - 1);
     for (int h = hmin; h <= hmax; h++) {
@@ -563,7 +563,7 @@ int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_ou
     xnorm = std::sqrt(xnorm);
     if (iter >= 0) hipLaunchKernelGGL(k_copy_active, sg, sb, 0, st, nc, mask, c->xw, x0);
     CAD_CHECK(hipGetLastError());
-    if (c->sweep_schedule == CAD_SWEEP_PIPELINED) {
+    if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) {
         int code = 0;
         RET_IF(cad_sweep_status(c, &code));
         if (code != 0) {
@@ -590,7 +590,7 @@ int cad_time_minv_impl(cad_ctx *c, int reps, float *ms_avg)
     for (int r = 0; r < reps; r++) RET_IF(apply_minv(c, c->ae, c->yv));
     CAD_CHECK(hipEventRecord(b, c->stream));
     CAD_CHECK(hipEventSynchronize(b));
-    if (c->sweep_schedule == CAD_SWEEP_PIPELINED) {
+    if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) {
         int code = 0;
         RET_IF(cad_sweep_status(c, &code));
         if (code != 0) {

@@ -12,32 +12,39 @@
 // same order, so the results are the serial sweep's bits; only the schedule
 // differs.
 //
-// Schedule.  The (J,K) cross-section is cut into tiles of 8x8 lines along I.
-// One wavefront owns a tile; lane (jl,kl) owns line (J,K) and walks I with a
-// skew of jl+kl steps, so that at every step the J-1 and K-1 neighbours of its
-// cell were computed by lanes jl-1 / kl-1 one step earlier (a DPP row shift and
-// a ds_bpermute hand them over).  Tiles pass their last column / row to the
-// next tile through 16-byte mailbox entries {value, sequence number} written
-// and read with device-scope (sc1) accesses: one entry is one single-copy-atomic
-// 16-byte access, so a consumer that reads the current sequence number also
-// reads the value that goes with it, with no fence.  A tile obtains its place
-// in the dependency order from an atomic ticket, so a tile only ever waits on
-// tiles whose wavefronts are already running, and every wait has a bounded
-// spin count (exceeding it sets an error flag that the host reports).  Tiles
+// Schedule.  The (J,K) cross-section is cut into tiles of TJ x TK = 64 lines
+// along I (8x8 or 16x4, whichever gives the shorter chain of tiles over the
+// active region).  One wavefront owns a tile; lane (jl,kl) owns line (J,K) and
+// walks I with a skew of jl+kl steps, so that at every step the J-1 and K-1
+// neighbours of its cell were computed by lanes jl-1 / kl-1 one step earlier
+// (a DPP row shift and a ds_bpermute hand them over).  Tiles pass their last
+// column / row to the next tile through 16-byte mailbox entries {value,
+// sequence number}, stored every step and read a few steps ahead with
+// device-scope (sc1) accesses: one entry is one single-copy-atomic 16-byte
+// access, so a consumer that reads the current sequence number also reads the
+// value that goes with it, with no fence.  A tile obtains its place in the
+// dependency order from an atomic ticket, so a tile only ever waits on tiles
+// whose wavefronts are already running, and every wait has a bounded spin
+// count (exceeding it sets an error flag that the host reports).  Tiles
 // without an active cell exit at once; the others walk only the I range that
 // holds active cells.
 //
-// Memory.  The walk is cut into stages of S steps.  While stage b computes
-// from LDS, stage b+1's operands travel straight from HBM into the other LDS
-// buffer with buffer_load ... lds: each line's S-cell window (shifted by the
-// lane's skew) is one contiguous run, so one instruction moves 16*S contiguous
-// bytes of each of 128/S lines.  The group layout in LDS makes the per-step
-// reads of all 64 lanes bank-conflict free.  Mailbox entries for a stage
-// arrive the same way; results are staged in LDS and stored as contiguous
-// runs; mailbox feeds are stored every 8 steps.  The loads into LDS are not
-// tracked by the compiler's wait insertion, so the stage loop waits for them
-// explicitly.  Every access goes through a raw buffer resource sized to its
-// array: a load outside it returns 0 and a store outside it is dropped.
+// Memory.  The walk is cut into stages of S steps, and a workgroup is two
+// wavefronts: the compute wave walks the tile, the loader wave moves memory.
+// While stage b computes, the loader brings stage b+1's operands straight
+// from HBM into the other LDS buffer with buffer_load ... lds (each line's
+// S-cell window, shifted by the lane's skew, is one contiguous run, so one
+// instruction moves 16 contiguous bytes of each of 64 line-pieces) and stores
+// stage b-1's results, staged in LDS, as contiguous runs; one barrier per
+// stage hands the buffers over.  Keeping the bulk traffic in the other wave
+// keeps it out of the compute wave's in-order memory counter, so waiting for
+// a mailbox entry never waits for a prefetch.  At the start of a stage every
+// compute lane reads its S x NS operands from LDS into registers (the group
+// layout makes these reads bank-conflict free), so the steps themselves touch
+// no memory but the mailboxes.  The loads into LDS are not tracked by the
+// compiler's wait insertion, so the loader waits for them explicitly.  Every
+// access goes through a raw buffer resource sized to its array: a load
+// outside it returns 0 and a store outside it is dropped.
 //
 // Inactive cells.  Cells with INDC = -1 are not computed and keep the value
 // the output array holds, which for every array swept here is zero (the
@@ -54,10 +61,11 @@
 
 namespace {
 
-constexpr int TJ = 8, TK = 8;        // lines per tile in J and K (64 lanes)
 constexpr unsigned SPIN_LIMIT = 1u << 20;
 constexpr unsigned OOB = 0xFFFFFFF0u;
 constexpr int CPOL_DEVICE = 16;      // sc1: device-scope coherent access
+constexpr int MB_LINES = 16;         // mailbox lines per tile and side (max of TJ, TK)
+constexpr int HD = 4;                // mailbox read-ahead (steps)
 // inactive-cell marker: a signalling-NaN pattern that no arithmetic produces
 constexpr unsigned long long MARK = 0x7FF4DEADBEEF0001ull;
 
@@ -73,10 +81,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, unsigned l
 }
 
 // 16 bytes per lane from buffer offset off straight into LDS at lds + 16*lane
-template <int AUX>
 __device__ __forceinline__ void load_lds16(__amdgpu_buffer_rsrc_t r, void *lds, unsigned off)
 {
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)lds, 16, off, 0, 0, AUX);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)lds, 16, off, 0, 0, 0);
 }
 
 __device__ __forceinline__ bool is_mark(double x)
@@ -89,6 +96,13 @@ __device__ __forceinline__ bool is_mark(double x)
 __device__ __forceinline__ v3u mb_load(__amdgpu_buffer_rsrc_t r, unsigned off)
 {
     return __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, CPOL_DEVICE);
+}
+
+__device__ __forceinline__ void mb_store(__amdgpu_buffer_rsrc_t r, unsigned off, double x, unsigned seq)
+{
+    const v2u b = __builtin_bit_cast(v2u, x);
+    const v4u e = {b.x, b.y, seq, seq};
+    __builtin_amdgcn_raw_buffer_store_b128(e, r, off, 0, CPOL_DEVICE);
 }
 
 // Slow path of a mailbox read: polls the entry until it carries sequence
@@ -124,14 +138,35 @@ __device__ __forceinline__ double from_lane_minus1(double x)
     return __builtin_bit_cast(double, r);
 }
 
-// value of lane-TJ
-__device__ __forceinline__ double from_lane_minus_tj(double x, int lane)
+// value of lane src (any lane)
+__device__ __forceinline__ double from_lane(double x, int src)
 {
     v2u b = __builtin_bit_cast(v2u, x);
-    const int src = ((lane - TJ) & 63) << 2;
+    const int a = (src & 63) << 2;
     v2u r;
-    r.x = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)b.x);
-    r.y = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)b.y);
+    r.x = (unsigned)__builtin_amdgcn_ds_bpermute(a, (int)b.x);
+    r.y = (unsigned)__builtin_amdgcn_ds_bpermute(a, (int)b.y);
+    return __builtin_bit_cast(double, r);
+}
+
+// Barrier between the two waves of a workgroup that orders their LDS traffic
+// only: a full __syncthreads() would also make the compute wave wait for its
+// outstanding mailbox accesses every stage.  The compiler barriers keep LDS
+// accesses on their side of it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ __volatile__("" ::: "memory");
+}
+
+// value of lane l (uniform l)
+__device__ __forceinline__ double read_lane(double x, int l)
+{
+    v2u b = __builtin_bit_cast(v2u, x);
+    v2u r;
+    r.x = (unsigned)__builtin_amdgcn_readlane((int)b.x, l);
+    r.y = (unsigned)__builtin_amdgcn_readlane((int)b.y, l);
     return __builtin_bit_cast(double, r);
 }
 
@@ -144,8 +179,8 @@ struct SweepGeom {
 struct SweepCtl {
     const int *order;               // tiles in dependency order
     const int2 *range;              // per tile: first, last active I (0-based); x > y: none
-    v4u *mb;                        // mailboxes: J half then K half, each [tile][line 8][I]
-    unsigned long long mb_bytes;    // bytes of one half
+    v4u *mb;                        // mailboxes: J side then K side, each [tile][16 lines][I]
+    unsigned long long mb_bytes;    // bytes of one side
     unsigned long long *ticket;
     unsigned long long base;
     unsigned seq;
@@ -162,8 +197,8 @@ struct FwdOp {
     const double *src[NS];          // Y, LPI, LPJ, LPK
     double *out;                    // X
     int *flag;
-    __device__ __forceinline__ static bool active(const double (&v)[NS]) { return !is_mark(v[1]); }
-    __device__ __forceinline__ static double eval(const double (&v)[NS], double xi, double xj, double xk)
+    __device__ __forceinline__ static bool active(const double *v) { return !is_mark(v[1]); }
+    __device__ __forceinline__ static double eval(const double *v, double xi, double xj, double xk)
     {
         return v[0] - v[1] * xi - v[2] * xj - v[3] * xk;
     }
@@ -177,8 +212,8 @@ struct BwdOp {
     const double *src[NS];          // X, AUI, AUJ, AUK, DD
     double *out;                    // X
     int *flag;
-    __device__ __forceinline__ static bool active(const double (&v)[NS]) { return v[4] != 0.0; }
-    __device__ __forceinline__ static double eval(const double (&v)[NS], double xi, double xj, double xk)
+    __device__ __forceinline__ static bool active(const double *v) { return v[4] != 0.0; }
+    __device__ __forceinline__ static double eval(const double *v, double xi, double xj, double xk)
     {
         return (v[0] - v[1] * xi - v[2] * xj - v[3] * xk) * v[4];
     }
@@ -193,8 +228,8 @@ struct DcmpOp {
     const double *src[NS];          // AD, ALI, ALJ, ALK, AU1, AU2, AU3
     double *out;                    // DD
     int *flag;                      // cleared when a pivot is not positive
-    __device__ __forceinline__ static bool active(const double (&v)[NS]) { return !is_mark(v[4]); }
-    __device__ __forceinline__ static double eval(const double (&v)[NS], double xi, double xj, double xk)
+    __device__ __forceinline__ static bool active(const double *v) { return !is_mark(v[4]); }
+    __device__ __forceinline__ static double eval(const double *v, double xi, double xj, double xk)
     {
         return 1.0 / (v[0] - v[1] * xi * v[4] - v[2] * xj * v[5] - v[3] * xk * v[6]);
     }
@@ -227,34 +262,38 @@ template <class Op>
 struct __attribute__((aligned(16))) Smem {
     static constexpr int S = Op::S, NS = Op::NS;
     unsigned st[2][NS][Lay<S>::WORDS];       // operand streams, two stages
-    v4u halo[2][2][S * 8];                   // [stage][J,K][step*8 + line]
-    v4u feed[2][8 * 8];                      // [J,K][step%8 * 8 + line]
-    double xo[64][S + 1];                    // results of the stage, per line
+    double xo[2][64][S + 1];                 // results of two stages, per line
+    int tile;                                // this workgroup's tile
 };
 
-template <class Op, int DIR>
-__global__ void __launch_bounds__(64) k_sweep(SweepGeom g, SweepCtl c, Op op)
+template <class Op, int DIR, int TJ>
+__global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
 {
+    constexpr int TK = 64 / TJ;
     constexpr int NS = Op::NS, S = Op::S;
     using LY = Lay<S>;
     __shared__ Smem<Op> sm;
-    const int lane = threadIdx.x;
-    unsigned long long tk = 0;
-    if (lane == 0) tk = atomicAdd(c.ticket, 1ull);
-    const unsigned tlo = __builtin_amdgcn_readfirstlane((unsigned)tk);
-    const unsigned thi = __builtin_amdgcn_readfirstlane((unsigned)(tk >> 32));
-    const long long tix = (long long)((((unsigned long long)thi << 32) | tlo) - c.base);
-    if (tix < 0 || tix >= g.ntiles) {
-        if (lane == 0) atomicOr(c.err, 1);
-        return;
+    const int lane = threadIdx.x & 63;
+    const bool loader = threadIdx.x >= 64;   // wave 1 moves memory, wave 0 computes
+    if (threadIdx.x == 0) {
+        const unsigned long long tk = atomicAdd(c.ticket, 1ull);
+        const long long tix = (long long)(tk - c.base);
+        if (tix < 0 || tix >= g.ntiles) {
+            atomicOr(c.err, 1);
+            sm.tile = -1;
+        } else {
+            sm.tile = c.order[DIR > 0 ? tix : g.ntiles - 1 - tix];
+        }
     }
-    const int tile = c.order[DIR > 0 ? tix : g.ntiles - 1 - tix];
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(sm.tile);
+    if (tile < 0) return;
     const int bj = tile % g.ntj, bk = tile / g.ntj;
     const int2 rg = c.range[tile];
     if (rg.x > rg.y) return;
     const int len = rg.y - rg.x + 1;
 
-    // line geometry of logical line L = jl + TJ*kl
+    // geometry of logical line L = jl + TJ*kl
     auto phys_j = [&] __device__(int jl) { return DIR > 0 ? jl : TJ - 1 - jl; };
     auto phys_k = [&] __device__(int kl) { return DIR > 0 ? kl : TK - 1 - kl; };
     auto line_ok = [&] __device__(int L) {
@@ -293,20 +332,74 @@ __global__ void __launch_bounds__(64) k_sweep(SweepGeom g, SweepCtl c, Op op)
     }
     const unsigned stage_step = (unsigned)(DIR * S * 8);   // byte step per stage (mod 2^32)
 
-    // mailboxes: producer tiles (uniform) and this tile's consumers
-    const int pbj = bj - DIR, pbk = bk - DIR;
-    const int ptj = pbj >= 0 && pbj < g.ntj ? pbj + bk * g.ntj : -1;
-    const int ptk = pbk >= 0 && pbk < g.ntk ? bj + pbk * g.ntj : -1;
-    const int2 prj = ptj >= 0 ? c.range[ptj] : make_int2(1, 0);
-    const int2 prk = ptk >= 0 ? c.range[ptk] : make_int2(1, 0);
-    const bool has_cj = bj + DIR >= 0 && bj + DIR < g.ntj;
-    const bool has_ck = bk + DIR >= 0 && bk + DIR < g.ntk;
+    // mailboxes
     const __amdgpu_buffer_rsrc_t rmb = rsrc(c.mb, 2 * c.mb_bytes);
     const unsigned half_k = (unsigned)c.mb_bytes;
     auto mb_off = [&] __device__(int tl, int l, int i) {
-        return (unsigned)((((long long)tl * 8 + l) * g.ni + i) * 16);
+        return (unsigned)((((long long)tl * MB_LINES + l) * g.ni + i) * 16);
     };
     const unsigned seq = c.seq;
+    const int pbj = bj - DIR, pbk = bk - DIR;
+    const int ptj = pbj >= 0 && pbj < g.ntj ? pbj + bk * g.ntj : -1;
+    const int ptk = pbk >= 0 && pbk < g.ntk ? bj + pbk * g.ntj : -1;
+    const bool has_cj = bj + DIR >= 0 && bj + DIR < g.ntj;
+    const bool has_ck = bk + DIR >= 0 && bk + DIR < g.ntk;
+
+    // Mailbox reads.  A lane with jl == 0 reads the J-upstream tile's entry
+    // for its own line, a lane with kl == 0 (jl > 0) the K-upstream one; lane
+    // (1,1), which has no halo of its own, reads lane (0,0)'s K entry.
+    constexpr int HELPER = 1 + TJ;
+    int hsrc = -1, hline = 0, hsk = skew;
+    bool hval = valid;
+    unsigned hbase = 0;
+    int2 hr = make_int2(1, 0);
+    if (jl == 0) {
+        if (ptj >= 0) {
+            hsrc = ptj;
+            hr = c.range[ptj];
+        }
+        hline = phys_k(kl);
+    } else if (kl == 0 || lane == HELPER) {
+        if (ptk >= 0) {
+            hsrc = ptk;
+            hr = c.range[ptk];
+        }
+        hbase = half_k;
+        hline = phys_j(jl);
+        if (lane == HELPER) {
+            hline = phys_j(0);
+            hsk = 0;
+            hval = line_ok(0);
+        }
+    }
+    auto h_off = [&] __device__(int t) {
+        const int hi = cell_i(t, hsk);
+        const bool need = hsrc >= 0 && hval && (unsigned)(t - hsk) < (unsigned)len && hi >= hr.x && hi <= hr.y;
+        return need ? hbase + mb_off(hsrc, hline, hi) : OOB;
+    };
+    // Mailbox writes.  Lanes with jl == TJ-1 feed the J-downstream tile, lanes
+    // with kl == TK-1 (jl < TJ-1) the K-downstream one; lane FEED2, which has
+    // no feed of its own, writes lane 63's K entry.
+    constexpr int FEED2 = (TJ - 2) + TJ * (TK - 2);
+    bool fok = false;
+    int fline = 0, fsk = skew;
+    unsigned fbase = 0;
+    if (jl == TJ - 1) {
+        fok = valid && has_cj;
+        fline = phys_k(kl);
+    } else if (kl == TK - 1) {
+        fok = valid && has_ck;
+        fline = phys_j(jl);
+        fbase = half_k;
+    } else if (lane == FEED2) {
+        fok = line_ok(63) && has_ck;
+        fline = phys_j(TJ - 1);
+        fsk = (TJ - 1) + (TK - 1);
+        fbase = half_k;
+    }
+    auto f_off = [&] __device__(int t) {
+        return fok && (unsigned)(t - fsk) < (unsigned)len ? fbase + mb_off(tile, fline, cell_i(t, fsk)) : OOB;
+    };
 
     __amdgpu_buffer_rsrc_t rs[NS];
 #pragma unroll
@@ -315,131 +408,108 @@ __global__ void __launch_bounds__(64) k_sweep(SweepGeom g, SweepCtl c, Op op)
 
     // issue the loads of stage b into LDS buffer b&1
     auto load_stage = [&] __device__(int b) {
-        const int buf = b & 1;
         const unsigned adv = stage_step * (unsigned)b;
 #pragma unroll
         for (int m = 0; m < NS; m++) {
 #pragma unroll
-            for (int q = 0; q < LY::NG; q++) {
-                const unsigned o = ld_off[q] == OOB ? OOB : ld_off[q] + adv;
-                load_lds16<0>(rs[m], &sm.st[buf][m][LY::gb(q)], o);
-            }
-        }
-        // mailbox entries: lane l takes line l%8 at step 8h + l/8 of the stage
-#pragma unroll
-        for (int h = 0; h < S / 8; h++) {
-            const int l8 = lane % 8, t = b * S + 8 * h + lane / 8;
-            const int il = cell_i(t, l8);
-            const bool in_walk = (unsigned)(t - l8) < (unsigned)len;
-            // J halo of lane (0, l8)
-            const bool nj_ = ptj >= 0 && line_ok(l8 * TJ) && in_walk && il >= prj.x && il <= prj.y;
-            load_lds16<CPOL_DEVICE>(rmb, &sm.halo[buf][0][64 * h], nj_ ? mb_off(ptj, phys_k(l8), il) : OOB);
-            // K halo of lane (l8, 0)
-            const bool nk_ = ptk >= 0 && line_ok(l8) && in_walk && il >= prk.x && il <= prk.y;
-            load_lds16<CPOL_DEVICE>(rmb, &sm.halo[buf][1][64 * h], nk_ ? half_k + mb_off(ptk, phys_j(l8), il) : OOB);
+            for (int q = 0; q < LY::NG; q++)
+                load_lds16(rs[m], &sm.st[b & 1][m][LY::gb(q)], ld_off[q] == OOB ? OOB : ld_off[q] + adv);
         }
     };
 
-    // value of a mailbox entry staged in LDS; polls global memory (bounded)
-    // when the staged copy is older than this launch
-    auto halo = [&] __device__(v4u e, unsigned off) {
-        if (e.z != seq) {
-            v3u f = mb_wait(c.mb, 2 * c.mb_bytes, off, seq, c.err);
-            e.x = f.x;
-            e.y = f.y;
-        }
-        v2u b = {e.x, e.y};
-        return __builtin_bit_cast(double, b);
-    };
-
-    // feeds of steps t0..t0+7 to the downstream tiles: lane l stores the entry
-    // of line l%8 at step t0 + l/8
-    auto flush_feed = [&] __device__(int t0) {
-        const int l8 = lane % 8, t = t0 + lane / 8;
-        {   // J feed: column jl = TJ-1, row l8
-            const int L = (TJ - 1) + TJ * l8, sk = (TJ - 1) + l8;
-            const bool ok = has_cj && line_ok(L) && (unsigned)(t - sk) < (unsigned)len;
-            const v4u e = sm.feed[0][(lane / 8) * 8 + l8];
-            __builtin_amdgcn_raw_buffer_store_b128(e, rmb, ok ? mb_off(tile, phys_k(l8), cell_i(t, sk)) : OOB, 0, CPOL_DEVICE);
-        }
-        {   // K feed: row kl = TK-1, column l8
-            const int L = l8 + TJ * (TK - 1), sk = l8 + (TK - 1);
-            const bool ok = has_ck && line_ok(L) && (unsigned)(t - sk) < (unsigned)len;
-            const v4u e = sm.feed[1][(lane / 8) * 8 + l8];
-            __builtin_amdgcn_raw_buffer_store_b128(e, rmb, ok ? half_k + mb_off(tile, phys_j(l8), cell_i(t, sk)) : OOB, 0, CPOL_DEVICE);
-        }
-    };
-
-    double xcur = 0.0;
-    bool bad_pivot = false;
     const int nsteps = len + (TJ - 1) + (TK - 1);
     const int nstage = (nsteps + S - 1) / S;
 
-    load_stage(0);
-    __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
-    for (int b = 0; b < nstage; b++) {
-        const int buf = b & 1;
-        if (b + 1 < nstage) load_stage(b + 1);
-#pragma unroll
-        for (int p = 0; p < S; p++) {
-            const int t = b * S + p;
-            const bool lv = valid && (unsigned)(t - skew) < (unsigned)len;
-            const int i = cell_i(t, skew);
-            const int e = DIR > 0 ? p : S - 1 - p;
-            double v[NS];
-#pragma unroll
-            for (int m = 0; m < NS; m++) {
-                v[m] = *reinterpret_cast<const double *>(&sm.st[buf][m][LY::addr(lane, e)]);
-            }
-            const double xi = xcur;
-            const double sj = from_lane_minus1(xcur);
-            const double sk = from_lane_minus_tj(xcur, lane);
-            const bool act = lv && Op::active(v);
-            double xj = sj, xk = sk;
-            if (jl == 0) {
-                const bool need = act && ptj >= 0 && i >= prj.x && i <= prj.y;
-                xj = need ? halo(sm.halo[buf][0][p * 8 + kl], mb_off(ptj, phys_k(kl), i)) : 0.0;
-            }
-            if (kl == 0) {
-                const bool need = act && ptk >= 0 && i >= prk.x && i <= prk.y;
-                xk = need ? halo(sm.halo[buf][1][p * 8 + jl], half_k + mb_off(ptk, phys_j(jl), i)) : 0.0;
-            }
-            const double ev = Op::eval(v, xi, xj, xk);
-            const double r = act ? ev : 0.0;
-            if (Op::CHECK_PIVOT) bad_pivot = bad_pivot || (act && r <= 0.0);
-            xcur = r;
-            sm.xo[lane][e] = act ? r : __builtin_bit_cast(double, MARK);
-            const v2u rb = __builtin_bit_cast(v2u, r);
-            const v4u fe = {rb.x, rb.y, seq, seq};
-            if (jl == TJ - 1) sm.feed[0][(p % 8) * 8 + kl] = fe;
-            if (kl == TK - 1) sm.feed[1][(p % 8) * 8 + jl] = fe;
-            if (p % 8 == 7) flush_feed(t - 7);
-        }
-        // results of the stage: store q covers lines q*LPS .. q*LPS+LPS-1
+    // results of stage b (in xo[b&1]): store q covers lines q*LPS .. q*LPS+LPS-1
+    auto store_stage = [&] __device__(int b) {
         const unsigned adv = stage_step * (unsigned)b;
 #pragma unroll
         for (int q = 0; q < S; q++) {
             const int L = q * LPS + lane / S, e = lane % S;
-            const double x = sm.xo[L][e];
+            const double x = sm.xo[b & 1][L][e];
             const unsigned o = st_off[q] == OOB || is_mark(x) ? OOB : st_off[q] + adv;
             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), rout, o, 0, 0);
         }
-        // the next stage's loads are older than the 2*S/8 feed stores and S
-        // result stores issued since
-        if (S == 16) __asm__ __volatile__("s_waitcnt vmcnt(20)" ::: "memory");
-        else __asm__ __volatile__("s_waitcnt vmcnt(10)" ::: "memory");
+    };
+
+    if (loader) {
+        // stage b+1's operands in while stage b computes; stage b-1's results out
+        load_stage(0);
+        __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
+        lds_barrier();
+        for (int b = 0; b < nstage; b++) {
+            if (b + 1 < nstage) load_stage(b + 1);
+            if (b > 0) store_stage(b - 1);
+            __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
+            lds_barrier();
+        }
+        store_stage(nstage - 1);
+        return;
+    }
+
+    double xcur = 0.0;
+    bool bad_pivot = false;
+    v3u ring[HD];
+#pragma unroll
+    for (int d = 0; d < HD; d++) ring[d] = mb_load(rmb, h_off(d));
+    lds_barrier();
+    for (int b = 0; b < nstage; b++) {
+        // this stage's operands: LDS -> registers
+        double sv[S][NS];
+#pragma unroll
+        for (int p = 0; p < S; p++) {
+            const int e = DIR > 0 ? p : S - 1 - p;
+#pragma unroll
+            for (int m = 0; m < NS; m++)
+                sv[p][m] = *reinterpret_cast<const double *>(&sm.st[b & 1][m][LY::addr(lane, e)]);
+        }
+#pragma unroll
+        for (int p = 0; p < S; p++) {
+            const int t = b * S + p;
+            const bool lv = valid && (unsigned)(t - skew) < (unsigned)len;
+            const int e = DIR > 0 ? p : S - 1 - p;
+            const bool act = lv && Op::active(sv[p]);
+            const double xi = xcur;
+            const double sj = from_lane_minus1(xcur);
+            const double sk = from_lane(xcur, lane - TJ);
+            // mailbox entry of this step (read HD steps ago); a stale one is
+            // polled for, in a wave-uniform slow path
+            v3u he = ring[p % HD];
+            const unsigned ho = h_off(t);
+            const bool act0 = (__builtin_amdgcn_ballot_w64(act) & 1ull) != 0;
+            const bool stale = ho != OOB && (lane == HELPER ? act0 : act) && he.z != seq;
+            if (__builtin_amdgcn_ballot_w64(stale)) {
+                if (stale) he = mb_wait(c.mb, 2 * c.mb_bytes, ho, seq, c.err);
+            }
+            ring[p % HD] = mb_load(rmb, h_off(t + HD));
+            const double hv = __builtin_bit_cast(double, v2u{he.x, he.y});
+            const double hk0 = read_lane(hv, HELPER);
+            const double xj = jl == 0 ? hv : sj;
+            const double xk = kl == 0 ? (jl == 0 ? hk0 : hv) : sk;
+            const double ev = Op::eval(sv[p], xi, xj, xk);
+            const double r = act ? ev : 0.0;
+            if (Op::CHECK_PIVOT) bad_pivot = bad_pivot || (act && r <= 0.0);
+            xcur = r;
+            sm.xo[b & 1][lane][e] = act ? r : __builtin_bit_cast(double, MARK);
+            const double r63 = read_lane(r, 63);
+            mb_store(rmb, f_off(t), lane == FEED2 ? r63 : r, seq);
+        }
+        lds_barrier();
     }
     if (Op::CHECK_PIVOT && bad_pivot) __hip_atomic_store(op.flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Active I range of every tile (INDC != -1 on its lines).
+// Active I range of every tile (INDC != -1 on its lines) and the largest
+// active K (0-based) in *kmax.
+template <int TJ>
 __global__ void __launch_bounds__(256) k_tile_range(SweepGeom g, const int *__restrict__ indc,
-                                                    int2 *__restrict__ range)
+                                                    int2 *__restrict__ range, int *kmax)
 {
+    constexpr int TK = 64 / TJ;
     const int tile = blockIdx.x;
     const int bj = tile % g.ntj, bk = tile / g.ntj;
-    int lo = INT_MAX, hi = -1;
-    for (int l = 0; l < TJ * TK; l++) {
+    int lo = INT_MAX, hi = -1, kh = -1;
+    for (int l = 0; l < 64; l++) {
         const int j = 1 + bj * TJ + l % TJ, k = 1 + bk * TK + l / TJ;
         if (j > g.nj - 2 || k > g.nk - 2) continue;
         const int *row = indc + (long long)j * g.ni + (long long)k * g.nij;
@@ -447,21 +517,27 @@ __global__ void __launch_bounds__(256) k_tile_range(SweepGeom g, const int *__re
             if (row[i] != -1) {
                 lo = min(lo, i);
                 hi = max(hi, i);
+                kh = max(kh, k);
             }
         }
     }
-    __shared__ int slo[256], shi[256];
+    __shared__ int slo[256], shi[256], skh[256];
     slo[threadIdx.x] = lo;
     shi[threadIdx.x] = hi;
+    skh[threadIdx.x] = kh;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + w]);
             shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + w]);
+            skh[threadIdx.x] = max(skh[threadIdx.x], skh[threadIdx.x + w]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) range[tile] = shi[0] < 0 ? make_int2(1, 0) : make_int2(slo[0], shi[0]);
+    if (threadIdx.x == 0) {
+        range[tile] = shi[0] < 0 ? make_int2(1, 0) : make_int2(slo[0], shi[0]);
+        if (skh[0] >= 0) atomicMax(kmax, skh[0]);
+    }
 }
 
 __device__ __forceinline__ bool interior(const Dims &d, int i, int j, int k)
@@ -514,25 +590,27 @@ __global__ void __launch_bounds__(256) k_lprod(Dims d, const double *__restrict_
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
-SweepGeom geom(const cad_ctx *c)
+constexpr int SHAPE_TJ[2] = {8, 16};
+
+SweepGeom geom(const cad_ctx *c, int shape)
 {
     SweepGeom g;
     g.ni = c->d.ni;
     g.nj = c->d.nj;
     g.nk = c->d.nk;
-    g.ntj = c->sw_ntj;
-    g.ntk = c->sw_ntk;
-    g.ntiles = c->sw_ntiles;
+    g.ntj = c->sw_ntj[shape];
+    g.ntk = c->sw_ntk[shape];
+    g.ntiles = c->sw_ntiles[shape];
     g.nij = c->d.nij;
     g.nc = c->d.nc;
     return g;
 }
 
-SweepCtl ctl(cad_ctx *c)
+SweepCtl ctl(cad_ctx *c, int shape)
 {
     SweepCtl s;
-    s.order = c->sw_order;
-    s.range = reinterpret_cast<const int2 *>(c->sw_range);
+    s.order = c->sw_order[shape];
+    s.range = reinterpret_cast<const int2 *>(c->sw_range[shape]);
     s.mb = reinterpret_cast<v4u *>(c->sw_mb);
     s.mb_bytes = c->sw_mb_bytes;
     s.ticket = c->sw_ticket;
@@ -545,9 +623,13 @@ SweepCtl ctl(cad_ctx *c)
 template <class Op, int DIR>
 int launch(cad_ctx *c, const Op &op)
 {
-    SweepGeom g = geom(c);
-    SweepCtl s = ctl(c);
-    hipLaunchKernelGGL((k_sweep<Op, DIR>), dim3(g.ntiles), dim3(64), 0, c->stream, g, s, op);
+    const int shape = c->sw_shape;
+    SweepGeom g = geom(c, shape);
+    SweepCtl s = ctl(c, shape);
+    if (shape == 0)
+        hipLaunchKernelGGL((k_sweep<Op, DIR, 8>), dim3(g.ntiles), dim3(128), 0, c->stream, g, s, op);
+    else
+        hipLaunchKernelGGL((k_sweep<Op, DIR, 16>), dim3(g.ntiles), dim3(128), 0, c->stream, g, s, op);
     CAD_CHECK(hipGetLastError());
     c->sw_base += (unsigned long long)g.ntiles;
     c->sw_seq += 1;
@@ -559,53 +641,88 @@ int launch(cad_ctx *c, const Op &op)
 int cad_sweep_alloc(cad_ctx *c)
 {
     const Dims &d = c->d;
-    c->sw_ntj = (d.nj - 2 + TJ - 1) / TJ;
-    c->sw_ntk = (d.nk - 2 + TK - 1) / TK;
-    c->sw_ntiles = c->sw_ntj * c->sw_ntk;
-    const int nt = c->sw_ntiles;
-    std::vector<int> order(nt);
-    for (int t = 0; t < nt; t++) order[t] = t;
-    const int ntj = c->sw_ntj;
-    std::stable_sort(order.begin(), order.end(), [ntj](int a, int b) {
-        const int da = a % ntj + a / ntj, db = b % ntj + b / ntj;
-        if (da != db) return da < db;
-        return a % ntj < b % ntj;
-    });
-    c->sw_mb_bytes = (unsigned long long)nt * 8 * d.ni * 16;
+    unsigned long long lines = 0;
+    for (int sh = 0; sh < 2; sh++) {
+        const int tj = SHAPE_TJ[sh], tk = 64 / tj;
+        c->sw_ntj[sh] = (d.nj - 2 + tj - 1) / tj;
+        c->sw_ntk[sh] = (d.nk - 2 + tk - 1) / tk;
+        c->sw_ntiles[sh] = c->sw_ntj[sh] * c->sw_ntk[sh];
+        const int nt = c->sw_ntiles[sh], ntj = c->sw_ntj[sh];
+        std::vector<int> order(nt);
+        for (int t = 0; t < nt; t++) order[t] = t;
+        std::stable_sort(order.begin(), order.end(), [ntj](int a, int b) {
+            const int da = a % ntj + a / ntj, db = b % ntj + b / ntj;
+            if (da != db) return da < db;
+            return a % ntj < b % ntj;
+        });
+        CAD_CHECK(hipMalloc((void **)&c->sw_order[sh], nt * sizeof(int)));
+        CAD_CHECK(hipMalloc((void **)&c->sw_range[sh], nt * 2 * sizeof(int)));
+        CAD_CHECK(hipMemcpy(c->sw_order[sh], order.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+        lines = std::max(lines, (unsigned long long)nt * MB_LINES);
+    }
+    c->sw_mb_bytes = lines * d.ni * 16;
     if (2 * c->sw_mb_bytes > 0xFFFFFFF0ull || (unsigned long long)d.nc * 8 > 0xFFFFFFF0ull) {
         cad_set_error("grid too large for 32-bit sweep offsets (%d x %d x %d)", d.ni, d.nj, d.nk);
         return CAD_ERR_ARG;
     }
-    CAD_CHECK(hipMalloc((void **)&c->sw_order, nt * sizeof(int)));
-    CAD_CHECK(hipMalloc((void **)&c->sw_range, nt * 2 * sizeof(int)));
     CAD_CHECK(hipMalloc(&c->sw_mb, 2 * c->sw_mb_bytes));
     CAD_CHECK(hipMalloc((void **)&c->sw_ticket, sizeof(unsigned long long)));
     CAD_CHECK(hipMalloc((void **)&c->sw_err, 2 * sizeof(int)));
-    CAD_CHECK(hipMemcpy(c->sw_order, order.data(), nt * sizeof(int), hipMemcpyHostToDevice));
     CAD_CHECK(hipMemset(c->sw_mb, 0, 2 * c->sw_mb_bytes));
     CAD_CHECK(hipMemset(c->sw_ticket, 0, sizeof(unsigned long long)));
     CAD_CHECK(hipMemset(c->sw_err, 0, 2 * sizeof(int)));
     CAD_CHECK(hipDeviceSynchronize());
     c->sw_base = 0;
     c->sw_seq = 1;
+    c->sw_shape = 0;
     return CAD_OK;
 }
 
 void cad_sweep_free(cad_ctx *c)
 {
-    void *p[] = {c->sw_order, c->sw_range, c->sw_mb, c->sw_ticket, c->sw_err};
+    void *p[] = {c->sw_order[0], c->sw_order[1], c->sw_range[0], c->sw_range[1], c->sw_mb,
+                 c->sw_ticket, c->sw_err};
     for (void *q : p)
         if (q) (void)hipFree(q);
 }
 
-// Tile ranges for the current INDC, the MILU diagonal DD and the products
-// LPI/LPJ/LPK.  *npos_host = 0 when a pivot is not positive (vf_mzdcmp.f:61).
+// Tile ranges for the current INDC, choice of the tile shape, the MILU
+// diagonal DD and the products LPI/LPJ/LPK.  *npos_host = 0 when a pivot is
+// not positive (vf_mzdcmp.f:61).
 int cad_sweep_dcmp(cad_ctx *c, double u, int *npos_host)
 {
     const Dims &d = c->d;
-    SweepGeom g = geom(c);
-    hipLaunchKernelGGL(k_tile_range, dim3(g.ntiles), dim3(256), 0, c->stream, g, c->f.indc,
-                       reinterpret_cast<int2 *>(c->sw_range));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->icount + 3, -1);
+    for (int sh = 0; sh < 2; sh++) {
+        SweepGeom g = geom(c, sh);
+        if (sh == 0)
+            hipLaunchKernelGGL(k_tile_range<8>, dim3(g.ntiles), dim3(256), 0, c->stream, g, c->f.indc,
+                               reinterpret_cast<int2 *>(c->sw_range[sh]), c->icount + 3);
+        else
+            hipLaunchKernelGGL(k_tile_range<16>, dim3(g.ntiles), dim3(256), 0, c->stream, g, c->f.indc,
+                               reinterpret_cast<int2 *>(c->sw_range[sh]), c->icount + 3);
+    }
+    CAD_CHECK(hipGetLastError());
+    CAD_CHECK(hipMemcpyAsync(c->host_cnt + 3, c->icount + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    // tile shape: the one with the shorter chain of tiles over the active K range
+    const int kmax = c->host_cnt[3];
+    if (c->sweep_schedule == CAD_SWEEP_PIPELINED_8X8) {
+        c->sw_shape = 0;
+    } else if (c->sweep_schedule == CAD_SWEEP_PIPELINED_16X4) {
+        c->sw_shape = 1;
+    } else {
+        int best = 0, best_hops = INT_MAX;
+        for (int sh = 0; sh < 2; sh++) {
+            const int tk = 64 / SHAPE_TJ[sh];
+            const int hops = c->sw_ntj[sh] + (kmax > 0 ? (kmax + tk - 1) / tk : 1);
+            if (hops < best_hops) {
+                best_hops = hops;
+                best = sh;
+            }
+        }
+        c->sw_shape = best;
+    }
     hipLaunchKernelGGL(k_dcmp_pre, cell_grid(d), cell_block(), 0, c->stream, d, c->aui, c->auj,
                        c->auk, c->f.indc, u, c->lpi, c->lpj, c->lpk);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->icount + 1, 1);

@@ -48,13 +48,17 @@ def _solve(pkg, st, r, schedule):
         return info, c.download_pt()
 
 
+PIPELINED = ["SWEEP_PIPELINED", "SWEEP_PIPELINED_8X8", "SWEEP_PIPELINED_16X4"]
+
+
+@pytest.mark.parametrize("sched", PIPELINED)
 @pytest.mark.parametrize("case", CASES)
-def test_schedules_agree_on_reference_systems(pkg, case):
+def test_schedules_agree_on_reference_systems(pkg, case, sched):
     st = fx.static(case)
     for s in fx.steps(case):
         r = fx.step(case, s)
-        print(f"case {case} step {s}", flush=True)
-        ip, pp = _solve(pkg, st, r, pkg.capi.SWEEP_PIPELINED)
+        print(f"case {case} step {s} {sched}", flush=True)
+        ip, pp = _solve(pkg, st, r, getattr(pkg.capi, sched))
         ih, ph = _solve(pkg, st, r, pkg.capi.SWEEP_HYPERPLANE)
         for k in ("cgbnrm", "cgxnrm", "icgitr"):
             assert ip[k] == ih[k], (k, ip[k], ih[k])
@@ -76,16 +80,17 @@ def _run_steps(pkg, lay, state, flags, schedule, nsteps):
         return infos, c.download_state(), c.download_pt()
 
 
+@pytest.mark.parametrize("sched", PIPELINED)
 @pytest.mark.parametrize("shape", [(48, 40, 36), (70, 19, 45)])
-def test_schedules_agree_over_steps(pkg, shape):
+def test_schedules_agree_over_steps(pkg, shape, sched):
     ni, nj, nk = shape
     case = pkg.configs.buildings_tank(ni, nj, nk, nbuild=6, seed=3, eps_r=1.0e-10,
                                       name=f"buildings_{ni}x{nj}x{nk}")
     lay = pkg.layout.from_text(case.deck)
     state = lay.raw_state()
     case.ic(lay.XX, lay.ZZ, lay.NF0, state)
-    print(f"buildings {shape}", flush=True)
-    ip, sp, pp = _run_steps(pkg, lay, dict(state), 0, pkg.capi.SWEEP_PIPELINED, 4)
+    print(f"buildings {shape} {sched}", flush=True)
+    ip, sp, pp = _run_steps(pkg, lay, dict(state), 0, getattr(pkg.capi, sched), 4)
     ih, sh, ph = _run_steps(pkg, lay, dict(state), 0, pkg.capi.SWEEP_HYPERPLANE, 4)
     for k in ("icgitr", "cgbnrm", "cgxnrm", "fsum", "fcut", "nums"):
         assert [i[k] for i in ip] == [i[k] for i in ih], k
