@@ -1,0 +1,147 @@
+! cadmas_hip.f90 -- Fortran 2003 interface (ISO_C_BINDING) to the C ABI in
+! cadmas_hip.h, for the reference's own Fortran driver.
+!
+! The reference passes explicit-shape REAL*8 (NUMI,NUMJ,NUMK) arrays, INTEGER*4
+! flags and INDB(4,NUMB) by reference (Source code/vf_a1main.f:412-697); every
+! array argument below is declared assumed-size so that those arrays are passed
+! as they are, without copies.  The context handle is an opaque TYPE(C_PTR).
+!
+! Build: flang -c cadmas_hip.f90 (module cadmas_hip), link with
+! -L<package dir> -lcadmas_hip.
+module cadmas_hip
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: CAD_OK = 0
+  integer(c_int), parameter :: CAD_ERR_NOT_CONVERGED = -30
+  integer(c_int), parameter :: CAD_ERR_INSTABILITY = -31
+  integer(c_int), parameter :: CAD_ERR_DECOMPOSITION = -32
+  integer(c_int), parameter :: CAD_ERR_ORIENTATION = -40
+  integer(c_int), parameter :: CAD_PRECISION_FP64 = 0
+  integer(c_int), parameter :: CAD_INIT_VELOCITY = 1
+  integer(c_int), parameter :: CAD_INIT_PRESSURE = 2
+
+  ! COMMON /VF_APHYSR/, /VF_ACOMPR/, /VF_ACOMPI/, /VF_ATIMER/ scalars
+  type, bind(c) :: cad_params
+    real(c_double) :: rho0, grz0, anu0, dtnow
+    real(c_double) :: scmvp, vvmax, flower, fupper, plower
+    real(c_double) :: cgpara, cgepsa, cgepsr, cgdiv
+    integer(c_int) :: icgtyp, icgmax, ibsuw0, loops
+    integer(c_int) :: precision
+  end type cad_params
+
+  ! list-file line of one step (vf_a1main.f:954-961)
+  type, bind(c) :: cad_step_info
+    real(c_double) :: cgbnrm, cgxnrm, fsum, fcut
+    integer(c_int) :: icgitr, nums
+    real(c_float) :: ms_predictor, ms_poisson, ms_correction, ms_vof
+  end type cad_step_info
+
+  interface
+    function cad_last_error() bind(c, name='cad_last_error')
+      import :: c_ptr
+      type(c_ptr) :: cad_last_error
+    end function
+
+    function cad_create(numi, numj, numk, numb, device, ctx) bind(c, name='cad_create')
+      import :: c_int, c_ptr
+      integer(c_int), value :: numi, numj, numk, numb, device
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: cad_create
+    end function
+
+    subroutine cad_destroy(ctx) bind(c, name='cad_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+
+    function cad_set_params(ctx, prm) bind(c, name='cad_set_params')
+      import :: c_int, c_ptr, cad_params
+      type(c_ptr), value :: ctx
+      type(cad_params), intent(in) :: prm
+      integer(c_int) :: cad_set_params
+    end function
+
+    function cad_upload_static(ctx, xx, yy, zz, anu, cd0, ggv, ggx, ggy, ggz, glv, glx, gly, glz, &
+                               indx, indy, indz, indb, bcvi) bind(c, name='cad_upload_static')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: xx(*), yy(*), zz(*), anu(*), cd0(*), ggv(*), ggx(*), ggy(*), &
+                                    ggz(*), glv(*), glx(*), gly(*), glz(*), bcvi(*)
+      integer(c_int), intent(in) :: indx(*), indy(*), indz(*), indb(*)
+      integer(c_int) :: cad_upload_static
+    end function
+
+    function cad_upload_state(ctx, uu, vv, ww, pp, ff, bcu, bcv, bcw, bcp, bcf, nf, indc) &
+        bind(c, name='cad_upload_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uu(*), vv(*), ww(*), pp(*), ff(*), bcu(*), bcv(*), bcw(*), &
+                                    bcp(*), bcf(*)
+      integer(c_int), intent(in) :: nf(*), indc(*)
+      integer(c_int) :: cad_upload_state
+    end function
+
+    function cad_download_state(ctx, uu, vv, ww, pp, ff, bcu, bcv, bcw, bcp, bcf, nf, indc) &
+        bind(c, name='cad_download_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: uu(*), vv(*), ww(*), pp(*), ff(*), bcu(*), bcv(*), bcw(*), &
+                                     bcp(*), bcf(*)
+      integer(c_int), intent(out) :: nf(*), indc(*)
+      integer(c_int) :: cad_download_state
+    end function
+
+    function cad_init_state(ctx, wvlvl, uini, vini, wini, flags, info) bind(c, name='cad_init_state')
+      import :: c_int, c_ptr, c_double, cad_step_info
+      type(c_ptr), value :: ctx
+      real(c_double), value :: wvlvl, uini, vini, wini
+      integer(c_int), value :: flags
+      type(cad_step_info), intent(out) :: info
+      integer(c_int) :: cad_init_state
+    end function
+
+    function cad_v1cal(ctx, info) bind(c, name='cad_v1cal')
+      import :: c_int, c_ptr, cad_step_info
+      type(c_ptr), value :: ctx
+      type(cad_step_info), intent(out) :: info
+      integer(c_int) :: cad_v1cal
+    end function
+
+    function cad_f1cal(ctx, info) bind(c, name='cad_f1cal')
+      import :: c_int, c_ptr, cad_step_info
+      type(c_ptr), value :: ctx
+      type(cad_step_info), intent(out) :: info
+      integer(c_int) :: cad_f1cal
+    end function
+
+    function cad_step(ctx, info) bind(c, name='cad_step')
+      import :: c_int, c_ptr, cad_step_info
+      type(c_ptr), value :: ctx
+      type(cad_step_info), intent(out) :: info
+      integer(c_int) :: cad_step
+    end function
+
+    function cad_cdiv00(ctx, dv) bind(c, name='cad_cdiv00')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dv
+      integer(c_int) :: cad_cdiv00
+    end function
+
+    ! drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5): the same by-reference
+    ! arguments minus DBUF and the work arrays DD..AE
+    subroutine cadmas_hip_m1bcgs(ad, ali, alj, alk, aui, auj, auk, bb, x0, cgpara, cgepsa, cgepsr, &
+                                 cgdiv, bnorm, xnorm, indc, mi, mj, mk, ni, nj, nk, icgtyp, icgmax, &
+                                 iter) bind(c, name='cadmas_hip_m1bcgs_')
+      import :: c_int, c_double
+      real(c_double), intent(in) :: ad(*), ali(*), alj(*), alk(*), aui(*), auj(*), auk(*), bb(*)
+      real(c_double), intent(inout) :: x0(*)
+      real(c_double), intent(in) :: cgpara, cgepsa, cgepsr, cgdiv
+      real(c_double), intent(out) :: bnorm, xnorm
+      integer(c_int), intent(in) :: indc(*)
+      integer(c_int), intent(in) :: mi, mj, mk, ni, nj, nk, icgtyp, icgmax
+      integer(c_int), intent(out) :: iter
+    end subroutine
+  end interface
+end module cadmas_hip

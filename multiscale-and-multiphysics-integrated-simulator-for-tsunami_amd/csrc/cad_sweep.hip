@@ -19,7 +19,7 @@
 // neighbours of its cell were computed by lanes jl-1 / kl-1 one step earlier
 // (a DPP row shift and a ds_bpermute hand them over).  Tiles pass their last
 // column / row to the next tile through 16-byte mailbox entries {value,
-// sequence number}, stored every step and read a few steps ahead with
+// sequence number}, stored every step and read 16 steps ahead with
 // device-scope (sc1) accesses: one entry is one single-copy-atomic 16-byte
 // access, so a consumer that reads the current sequence number also reads the
 // value that goes with it, with no fence.  A tile obtains its place in the
@@ -65,7 +65,7 @@ constexpr unsigned SPIN_LIMIT = 1u << 20;
 constexpr unsigned OOB = 0xFFFFFFF0u;
 constexpr int CPOL_DEVICE = 16;      // sc1: device-scope coherent access
 constexpr int MB_LINES = 16;         // mailbox lines per tile and side (max of TJ, TK)
-constexpr int HD = 4;                // mailbox read-ahead (steps)
+constexpr int HD = 16;               // mailbox read-ahead (steps)
 // inactive-cell marker: a signalling-NaN pattern that no arithmetic produces
 constexpr unsigned long long MARK = 0x7FF4DEADBEEF0001ull;
 

@@ -28,3 +28,22 @@ def test_library_exports_every_declared_symbol(pkg):
 
 def test_python_mirror_covers_the_header(pkg):
     assert set(_declared()) <= set(pkg.capi.EXPORTS)
+
+
+def test_fortran_module_binds_exported_symbols(pkg):
+    """Every bind(c, name='...') of the Fortran interface module names a symbol
+    the library exports, and the module compiles where flang is present."""
+    import shutil
+    import subprocess
+    import tempfile
+    src = ROOT / "include" / "cadmas_hip.f90"
+    names = re.findall(r"bind\(c,\s*name='([a-z0-9_]+)'\)", src.read_text())
+    assert "cadmas_hip_m1bcgs_" in names and "cad_step" in names
+    lib = ctypes.CDLL(str(pkg.capi.LIB_PATH))
+    assert [n for n in names if not hasattr(lib, n)] == []
+    flang = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    if Path(flang).exists():
+        with tempfile.TemporaryDirectory() as d:
+            r = subprocess.run([flang, "-fsyntax-only", "-module-dir", d, str(src)],
+                               capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr
