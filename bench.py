@@ -13,9 +13,14 @@ roofline = the MILU preconditioner application M^-1 (forward + backward
            active cell (DD, ALI, ALJ, ALK, y, x forward; DD, AUI, AUJ, AUK, x,
            x backward) = 96 B per active cell per application, divided by its
            average duration timed with HIP events on the library's stream.
-cpu_baseline = the C restatement (oracle/) on one host core: one SMAC step of
-           the 512x64x128 slab of the same tank (same beach, wave and cell size,
-           10 buildings), a bounded sample of about 15 s, as cell-updates/s.
+cpu_baseline = the C restatement (oracle/) on one host core: three SMAC steps
+           of the 512x64x128 slab of the same tank (same beach, wave and cell
+           size, 10 buildings), a bounded sample of about 15 s, as
+           cell-updates/s.
+
+--calibrate-copy runs one device-to-device copy of 1 GiB (torch) and nothing
+else: its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc scale the counters of
+the sweep kernels to bytes.
 """
 import argparse
 import importlib.util
@@ -97,12 +102,24 @@ def cpu_baseline(pkg, lay, state, flags, steps=1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=os.environ.get("CADMAS_BENCH_WORKLOAD", "buildings_c"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--minv-reps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 27                                    # 2^27 doubles = 1 GiB
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": n * 8, "bytes_written": n * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs one grid on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not an implementation).", file=sys.stderr)
@@ -172,11 +189,11 @@ def main():
     if not args.no_cpu_baseline:
         sample = workload(pkg, "buildings_c_slab") if args.workload == "buildings_c" else case
         slay, sstate, sflags = prepare(pkg, sample)
-        v, dt, it = cpu_baseline(pkg, slay, sstate, sflags, steps=1)
+        v, dt, it = cpu_baseline(pkg, slay, sstate, sflags, steps=args.cpu_steps)
         scells = int(np.prod(slay.deck.ncells))
         out["cpu_baseline"] = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1,
                                "kind": "port",
-                               "sample": f"1 SMAC step of {sample.name} {list(slay.deck.ncells)} "
+                               "sample": f"{args.cpu_steps} SMAC steps of {sample.name} {list(slay.deck.ncells)} "
                                          f"({scells} cells, {dt:.1f} s, ITR={it}) with the C restatement in "
                                          "oracle/, one thread"}
     print(json.dumps(out), flush=True)

@@ -145,12 +145,22 @@ int cad_count_active(cad_ctx *ctx, long long *active);
 #define CAD_SWEEP_PIPELINED_16X4 3
 int cad_set_sweep_schedule(cad_ctx *ctx, int schedule);
 
+/* The MILU preconditioner on its own, for checking the sweeps: factor the
+ * uploaded matrix (vf_mzdcmp.f:49-67; *npos = 0 when a pivot is not positive)
+ * and apply x = M^-1 y (vf_mzminv.f:38-41) to a host vector (x is 0 on
+ * inactive cells).  The reference calls these only inside VF_M1BCGS, so they
+ * have no entry-point counterpart there. */
+int cad_milu_factor(cad_ctx *ctx, int *npos);
+int cad_milu_apply(cad_ctx *ctx, const double *y, double *x);
+
 /* Drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5) with the reference's by-reference
  * argument list (trailing underscore: callable from Fortran as
  * CALL CADMAS_HIP_M1BCGS(...)).  Host arrays in, host X0 out; the device
  * context is created on first call and reused while the grid size is unchanged.
  * DBUF and the eight work arrays DD..AE of the reference are not needed and
- * are omitted. */
+ * are omitted.  MI/MJ/MK must equal NI/NJ/NK (VF_VPSOL passes NUMI, NUMJ,
+ * NUMK for both, vf_vpsol.f:61-69).  ITER keeps VF_M1BCGS's meaning; -3 means
+ * the library itself failed (message in cad_last_error()). */
 void cadmas_hip_m1bcgs_(const double *ad, const double *ali, const double *alj, const double *alk,
                         const double *aui, const double *auj, const double *auk, const double *bb,
                         double *x0, const double *cgpara, const double *cgepsa,

@@ -87,6 +87,8 @@ def lib():
     L.cad_time_minv.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
     L.cad_count_active.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.cad_set_sweep_schedule.argtypes = [C.c_void_p, C.c_int]
+    L.cad_milu_factor.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.cad_milu_apply.argtypes = [C.c_void_p, PD, PD]
     _lib = L
     return L
 
@@ -95,7 +97,8 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_upload_state", "cad_download_state", "cad_v1cal", "cad_f1cal", "cad_step",
            "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
            "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
-           "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule")
+           "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
+           "cad_milu_factor", "cad_milu_apply")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -265,6 +268,19 @@ class Context:
 
     def set_sweep_schedule(self, schedule):
         _check(lib().cad_set_sweep_schedule(self._h, int(schedule)))
+
+    def milu_factor(self):
+        """MILU factorisation of the uploaded matrix; returns npos (0: a pivot <= 0)."""
+        n = C.c_int()
+        _check(lib().cad_milu_factor(self._h, C.byref(n)))
+        return n.value
+
+    def milu_apply(self, y):
+        """x = M^-1 y on the current factorisation (host arrays)."""
+        yy = _f64(y, self.shape)
+        x = np.zeros(self.shape, order="F")
+        _check(lib().cad_milu_apply(self._h, _pd(yy), _pd(x)))
+        return x
 
     def count_active(self):
         n = C.c_longlong()

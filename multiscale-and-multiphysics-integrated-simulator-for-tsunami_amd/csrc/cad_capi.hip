@@ -168,6 +168,8 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
     rc = rc ? rc : dalloc(&c->zz, 6 * (size_t)numk);
     rc = rc ? rc : dalloc(&c->red_partial, 4 * (size_t)REDUCE_BLOCKS);
     rc = rc ? rc : dalloc(&c->red_out, 16);
+    rc = rc ? rc : dalloc(&c->ksc, 32);
+    rc = rc ? rc : dalloc(&c->act_rng, 2);
     rc = rc ? rc : dalloc(&c->icount, 16);
     rc = rc ? rc : cad_sweep_alloc(c);
     if (rc) {
@@ -208,7 +210,7 @@ void cad_destroy(cad_ctx *c)
                     c->r0, c->rr, c->pv, c->ap, c->ae, c->flf[0], c->flf[1], c->flf[2], c->f.nf,
                     c->f.indc, c->f.nlim, c->indx, c->indy, c->indz, c->f.bcu, c->f.bcv, c->f.bcw,
                     c->f.bcp, c->f.bcf, c->bcvi, c->indb, c->xx, c->yy, c->zz, c->red_partial,
-                    c->red_out, c->icount, c->lpi, c->lpj, c->lpk};
+                    c->red_out, c->icount, c->lpi, c->lpj, c->lpk, c->ksc, c->act_rng};
     for (void *p : ptrs)
         if (p) hipFree(p);
     cad_sweep_free(c);
@@ -560,6 +562,20 @@ int cad_set_sweep_schedule(cad_ctx *c, int schedule)
     }
     c->sweep_schedule = schedule;
     return CAD_OK;
+}
+
+int cad_milu_factor(cad_ctx *c, int *npos)
+{
+    if (!c || !npos) return CAD_ERR_ARG;
+    CAD_CHECK(hipSetDevice(c->device));
+    return cad_milu_factor_impl(c, npos);
+}
+
+int cad_milu_apply(cad_ctx *c, const double *y, double *x)
+{
+    if (!c || !y || !x) return CAD_ERR_ARG;
+    CAD_CHECK(hipSetDevice(c->device));
+    return cad_milu_apply_impl(c, y, x);
 }
 
 int cad_cdiv00(cad_ctx *c, double *dv)

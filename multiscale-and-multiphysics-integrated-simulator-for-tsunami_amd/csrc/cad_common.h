@@ -132,6 +132,8 @@ struct cad_ctx {
     // scratch for reductions and counters
     double *red_partial;    // REDUCE_BLOCKS * 4
     double *red_out;        // 16 results
+    double *ksc;            // BiCGSTAB scalars on the device (32 slots)
+    unsigned long long *act_rng;   // [n0, n1) of the active cells of the current solve
     int *icount;            // 16 int counters
     double *host_red;       // pinned mirror of red_out
     int *host_cnt;          // pinned mirror of icount
@@ -162,4 +164,6 @@ void cad_sweep_free(cad_ctx *c);
 int cad_sweep_dcmp(cad_ctx *c, double u, int *npos_host);
 int cad_sweep_minv(cad_ctx *c, double *x, const double *y);
 int cad_sweep_status(cad_ctx *c, int *code);
+int cad_milu_factor_impl(cad_ctx *c, int *npos);
+int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x);
 Static cad_static(const cad_ctx *c);

@@ -188,8 +188,60 @@ __global__ void k_bkwd_plane(Dims d, Plane pl, const double *__restrict__ dd,
     x[n] = (x[n] - aui[n] * x[n + 1] - auj[n] * x[n + ni] - auk[n] * x[n + nij]) * dd[n];
 }
 
-// y = A x on active interior cells, vf_mzax.f:42-56
-__global__ void __launch_bounds__(256) k_mzax(Dims d, const double *__restrict__ ad,
+// ------------------------------------------------------- Krylov kernels
+// The vector kernels of the solver run over the flat index range [n0, n1)
+// that holds the active cells (found once per solve, k_active_range), in a
+// fixed grid-stride order; cells outside it are inactive everywhere.  The
+// BiCGSTAB scalars live in a small device array (slots SC_*), so the only
+// value the host reads per iteration is (R,R) for the stop test.
+enum { SC_S = 0, SC_W1, SC_ALPHA, SC_W1B, SC_W2, SC_OMEGA, SC_XN, SC_W1C, SC_BETA, SC_BN, SC_N };
+enum { RED_PLAIN = 0, RED_ALPHA, RED_OMEGA, RED_BETA };
+constexpr int VEC_BLOCKS = 2048;
+
+// [n0, n1) of the active cells; rng must hold {nc, 0} on entry
+__global__ void __launch_bounds__(256) k_active_range(long long nc, const int *__restrict__ indc,
+                                                      unsigned long long *rng)
+{
+    unsigned long long lo = ~0ull, hi = 0;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (indc[n] != -1) {
+            lo = min(lo, (unsigned long long)n);
+            hi = max(hi, (unsigned long long)n + 1);
+        }
+    }
+    __shared__ unsigned long long slo[256], shi[256];
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + w]);
+            shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && shi[0] > 0) {
+        atomicMin(&rng[0], slo[0]);
+        atomicMax(&rng[1], shi[0]);
+    }
+}
+
+__global__ void k_range_init(unsigned long long *rng, long long nc)
+{
+    rng[0] = (unsigned long long)nc;
+    rng[1] = 0;
+}
+
+#define ACTIVE_LOOP(n, rng, nblocks)                                                   \
+    const long long n0_ = (long long)(rng)[0], n1_ = (long long)(rng)[1];           \
+    for (long long n = n0_ + (long long)blockIdx.x * blockDim.x + threadIdx.x; n < n1_; \
+         n += (long long)(nblocks) * blockDim.x)
+
+// y = A x on active cells, vf_mzax.f:42-56 (active cells are interior, so all
+// seven neighbours exist)
+__global__ void __launch_bounds__(256) k_mzax(Dims d, const unsigned long long *rng,
+                                              const double *__restrict__ ad,
                                               const double *__restrict__ ali,
                                               const double *__restrict__ alj,
                                               const double *__restrict__ alk,
@@ -199,19 +251,16 @@ __global__ void __launch_bounds__(256) k_mzax(Dims d, const double *__restrict__
                                               const int *__restrict__ indc,
                                               const double *__restrict__ x, double *__restrict__ y)
 {
-    int i, j, k;
-    if (!cell_of_thread(d, i, j, k)) return;
-    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
-    const long long n = cid(d, i, j, k);
-    if (indc[n] == -1) return;
     const long long ni = d.ni, nij = d.nij;
-    y[n] = ad[n] * x[n] + ali[n] * x[n - 1] + alj[n] * x[n - ni] + alk[n] * x[n - nij]
-         + aui[n] * x[n + 1] + auj[n] * x[n + ni] + auk[n] * x[n + nij];
+    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
+        if (indc[n] == -1) continue;
+        y[n] = ad[n] * x[n] + ali[n] * x[n - 1] + alj[n] * x[n - ni] + alk[n] * x[n - nij]
+             + aui[n] * x[n + 1] + auj[n] * x[n + ni] + auk[n] * x[n + nij];
+    }
 }
 
-// ------------------------------------------------------------- reductions
-// Partial sums of up to three products over active cells (INDC != -1) in a
-// fixed block-strided order; see k_reduce_final.
+// Partial sums of up to three products over active cells in a fixed
+// block-strided order; k_reduce_final adds the partials in block order.
 template <int NPROD>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[3], double *partial)
 {
@@ -232,28 +281,29 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[3], double *par
     }
 }
 
-__global__ void __launch_bounds__(REDUCE_THREADS) k_dot(long long nc, const int *__restrict__ mask,
+// (x, y) over active cells, vf_mzip.f:32-43
+__global__ void __launch_bounds__(REDUCE_THREADS) k_dot(const unsigned long long *rng,
+                                                        const int *__restrict__ mask,
                                                         const double *__restrict__ x,
                                                         const double *__restrict__ y,
                                                         double *partial)
 {
     double acc[3] = {0.0, 0.0, 0.0};
-    for (long long n = (long long)blockIdx.x * REDUCE_THREADS + threadIdx.x; n < nc;
-         n += (long long)REDUCE_BLOCKS * REDUCE_THREADS) {
+    ACTIVE_LOOP(n, rng, REDUCE_BLOCKS) {
         if (mask[n] != -1) acc[0] += x[n] * y[n];
     }
     block_reduce_store<1>(acc, partial);
 }
 
 // omega numerator and denominator: (r, ae) and (ae, ae), vf_m1bcgs.f:199-203
-__global__ void __launch_bounds__(REDUCE_THREADS) k_dot2(long long nc, const int *__restrict__ mask,
+__global__ void __launch_bounds__(REDUCE_THREADS) k_dot2(const unsigned long long *rng,
+                                                         const int *__restrict__ mask,
                                                          const double *__restrict__ r,
                                                          const double *__restrict__ ae,
                                                          double *partial)
 {
     double acc[3] = {0.0, 0.0, 0.0};
-    for (long long n = (long long)blockIdx.x * REDUCE_THREADS + threadIdx.x; n < nc;
-         n += (long long)REDUCE_BLOCKS * REDUCE_THREADS) {
+    ACTIVE_LOOP(n, rng, REDUCE_BLOCKS) {
         if (mask[n] != -1) {
             acc[0] += r[n] * ae[n];
             acc[1] += ae[n] * ae[n];
@@ -264,18 +314,19 @@ __global__ void __launch_bounds__(REDUCE_THREADS) k_dot2(long long nc, const int
 
 // x += alpha p + omega r ; r -= omega ae ; partials of (r, r) and (r0, r),
 // vf_m1bcgs.f:205-227
-__global__ void __launch_bounds__(REDUCE_THREADS) k_update_xr(long long nc, const int *__restrict__ mask,
+__global__ void __launch_bounds__(REDUCE_THREADS) k_update_xr(const unsigned long long *rng,
+                                                              const int *__restrict__ mask,
                                                               double *__restrict__ x,
                                                               double *__restrict__ r,
                                                               const double *__restrict__ p,
                                                               const double *__restrict__ ae,
                                                               const double *__restrict__ r0,
-                                                              double alpha, double omega,
+                                                              const double *__restrict__ sc,
                                                               double *partial)
 {
+    const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
     double acc[3] = {0.0, 0.0, 0.0};
-    for (long long n = (long long)blockIdx.x * REDUCE_THREADS + threadIdx.x; n < nc;
-         n += (long long)REDUCE_BLOCKS * REDUCE_THREADS) {
+    ACTIVE_LOOP(n, rng, REDUCE_BLOCKS) {
         if (mask[n] != -1) {
             x[n] = x[n] + (alpha * p[n] + omega * r[n]);
             double rn = r[n] - omega * ae[n];
@@ -287,8 +338,13 @@ __global__ void __launch_bounds__(REDUCE_THREADS) k_update_xr(long long nc, cons
     block_reduce_store<2>(acc, partial);
 }
 
+// Adds the partials of nprod products in block order into sc[slot..], then
+// forms the BiCGSTAB scalar the next kernel needs (vf_m1bcgs.f:179, 204, 229):
+//   RED_ALPHA: alpha = s / w1          RED_OMEGA: omega = w1 / w2
+//   RED_BETA:  beta = w1 alpha / (s omega), then s = w1
 __global__ void __launch_bounds__(REDUCE_THREADS) k_reduce_final(const double *__restrict__ partial,
-                                                                 int nprod, double *out)
+                                                                 int nprod, double *sc, int slot,
+                                                                 int mode)
 {
     __shared__ double sh[REDUCE_THREADS];
     for (int q = 0; q < nprod; q++) {
@@ -300,8 +356,18 @@ __global__ void __launch_bounds__(REDUCE_THREADS) k_reduce_final(const double *_
             if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[q] = sh[0];
+        if (threadIdx.x == 0) sc[slot + q] = sh[0];
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (mode == RED_ALPHA) {
+            sc[SC_ALPHA] = sc[SC_S] / sc[SC_W1];
+        } else if (mode == RED_OMEGA) {
+            sc[SC_OMEGA] = sc[SC_W1B] / sc[SC_W2];
+        } else if (mode == RED_BETA) {
+            sc[SC_BETA] = sc[SC_W1C] * sc[SC_ALPHA] / (sc[SC_S] * sc[SC_OMEGA]);
+            sc[SC_S] = sc[SC_W1C];
+        }
     }
 }
 
@@ -321,48 +387,56 @@ __global__ void k_copy(long long nc, const double *__restrict__ a, double *__res
 }
 
 // y = b - y (active), vf_m1bcgs.f:122-130
-__global__ void k_bminus(long long nc, const int *__restrict__ mask, const double *__restrict__ b,
-                         double *y)
+__global__ void __launch_bounds__(256) k_bminus(const unsigned long long *rng, const int *__restrict__ mask,
+                                                const double *__restrict__ b, double *y)
 {
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
-         n += (long long)gridDim.x * blockDim.x)
+    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
         if (mask[n] != -1) y[n] = b[n] - y[n];
+    }
 }
 
 // rr = r0, pp = r0 (active), vf_m1bcgs.f:147-155
-__global__ void k_init_rp(long long nc, const int *__restrict__ mask, const double *__restrict__ r0,
-                          double *rr, double *pp)
+__global__ void __launch_bounds__(256) k_init_rp(const unsigned long long *rng, const int *__restrict__ mask,
+                                                 const double *__restrict__ r0, double *rr, double *pp)
 {
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
-         n += (long long)gridDim.x * blockDim.x)
-        if (mask[n] != -1) { rr[n] = r0[n]; pp[n] = r0[n]; }
+    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
+        if (mask[n] != -1) {
+            rr[n] = r0[n];
+            pp[n] = r0[n];
+        }
+    }
 }
 
 // r -= alpha ap (active), vf_m1bcgs.f:181-189
-__global__ void k_axpy_neg(long long nc, const int *__restrict__ mask, double alpha,
-                           const double *__restrict__ ap, double *r)
+__global__ void __launch_bounds__(256) k_axpy_neg(const unsigned long long *rng, const int *__restrict__ mask,
+                                                  const double *__restrict__ sc,
+                                                  const double *__restrict__ ap, double *r)
 {
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
-         n += (long long)gridDim.x * blockDim.x)
+    const double alpha = sc[SC_ALPHA];
+    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
         if (mask[n] != -1) r[n] = r[n] - alpha * ap[n];
+    }
 }
 
 // p = r + beta (p - omega ap) (active), vf_m1bcgs.f:230-238
-__global__ void k_update_p(long long nc, const int *__restrict__ mask, double beta, double omega,
-                           const double *__restrict__ r, const double *__restrict__ ap, double *p)
+__global__ void __launch_bounds__(256) k_update_p(const unsigned long long *rng, const int *__restrict__ mask,
+                                                  const double *__restrict__ sc,
+                                                  const double *__restrict__ r,
+                                                  const double *__restrict__ ap, double *p)
 {
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
-         n += (long long)gridDim.x * blockDim.x)
+    const double beta = sc[SC_BETA], omega = sc[SC_OMEGA];
+    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
         if (mask[n] != -1) p[n] = r[n] + beta * (p[n] - omega * ap[n]);
+    }
 }
 
 // x0 = xw (active), vf_m1bcgs.f:248-257
-__global__ void k_copy_active(long long nc, const int *__restrict__ mask,
-                              const double *__restrict__ a, double *b)
+__global__ void __launch_bounds__(256) k_copy_active(const unsigned long long *rng, const int *__restrict__ mask,
+                                                     const double *__restrict__ a, double *b)
 {
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nc;
-         n += (long long)gridDim.x * blockDim.x)
+    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
         if (mask[n] != -1) b[n] = a[n];
+    }
 }
 
 __global__ void k_iset1(int *p, int v) { *p = v; }
@@ -455,58 +529,78 @@ static int apply_minv(cad_ctx *c, double *x, const double *y)
     return CAD_OK;
 }
 
-static int apply_a(cad_ctx *c, const double *x, double *y)
-{
-    hipLaunchKernelGGL(k_mzax, cell_grid(c->d), cell_block(), 0, c->stream, c->d, c->ad, c->ali,
-                       c->alj, c->alk, c->aui, c->auj, c->auk, c->f.indc, x, y);
-    CAD_CHECK(hipGetLastError());
-    return CAD_OK;
-}
-
-int cad_reduce_fetch(cad_ctx *c, int n, double *host_out)
-{
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(REDUCE_THREADS), 0, c->stream, c->red_partial,
-                       n, c->red_out);
-    CAD_CHECK(hipGetLastError());
-    CAD_CHECK(hipMemcpyAsync(c->host_red, c->red_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CAD_CHECK(hipStreamSynchronize(c->stream));
-    for (int q = 0; q < n; q++) host_out[q] = c->host_red[q];
-    return CAD_OK;
-}
-
-static int dot(cad_ctx *c, const double *x, const double *y, double *out)
-{
-    hipLaunchKernelGGL(k_dot, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, c->stream, c->d.nc,
-                       c->f.indc, x, y, c->red_partial);
-    CAD_CHECK(hipGetLastError());
-    return cad_reduce_fetch(c, 1, out);
-}
-
-#define RET_IF(x)              \
+#define RET_IF_(x)             \
     do {                       \
         int rc_ = (x);         \
         if (rc_) return rc_;   \
     } while (0)
 
+static int apply_a(cad_ctx *c, const double *x, double *y)
+{
+    hipLaunchKernelGGL(k_mzax, dim3(VEC_BLOCKS), dim3(256), 0, c->stream, c->d, c->act_rng, c->ad,
+                       c->ali, c->alj, c->alk, c->aui, c->auj, c->auk, c->f.indc, x, y);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+// partials of nprod products in red_partial -> sc[slot..] (+ scalar mode)
+static int reduce_to(cad_ctx *c, int nprod, int slot, int mode)
+{
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(REDUCE_THREADS), 0, c->stream, c->red_partial,
+                       nprod, c->ksc, slot, mode);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+static int dot_to(cad_ctx *c, const double *x, const double *y, int slot, int mode)
+{
+    hipLaunchKernelGGL(k_dot, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, c->stream, c->act_rng,
+                       c->f.indc, x, y, c->red_partial);
+    CAD_CHECK(hipGetLastError());
+    return reduce_to(c, 1, slot, mode);
+}
+
+// n scalars sc[slot..] -> host
+static int fetch_sc(cad_ctx *c, int slot, int n, double *out)
+{
+    CAD_CHECK(hipMemcpyAsync(c->host_red, c->ksc + slot, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < n; q++) out[q] = c->host_red[q];
+    return CAD_OK;
+}
+
+// Generic reduction used outside the solver (VOF sums, CDIV00): partials in
+// red_partial -> n values on the host.
+int cad_reduce_fetch(cad_ctx *c, int n, double *host_out)
+{
+    RET_IF_(reduce_to(c, n, 0, RED_PLAIN));
+    return fetch_sc(c, 0, n, host_out);
+}
+
 // VF_M1BCGS on the context's matrix with solution vector x0 (device),
-// vf_m1bcgs.f:78-262.
+// vf_m1bcgs.f:78-262.  Same algebra, scalar recurrences and stop test as the
+// reference; the dot products are deterministic two-stage sums.
 int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out, int *iter_out)
 {
     const Dims &d = c->d;
     const long long nc = d.nc;
     const int *mask = c->f.indc;
+    const unsigned long long *rng = c->act_rng;
     hipStream_t st = c->stream;
-    const dim3 sg = stream_grid(nc), sb(256);
+    const dim3 sg = stream_grid(nc), sb(256), vg(VEC_BLOCKS);
+    const dim3 rgd(REDUCE_BLOCKS), rbd(REDUCE_THREADS);
     // XW = X0; DD, YY, R0, RR, PP, AP, AE = 0  (vf_m1bcgs.f:82-99)
     hipLaunchKernelGGL(k_copy, sg, sb, 0, st, nc, x0, c->xw);
     double *zero_vecs[7] = {c->dd, c->yv, c->r0, c->rr, c->pv, c->ap, c->ae};
     for (double *z : zero_vecs) hipLaunchKernelGGL(k_fill, sg, sb, 0, st, nc, z, 0.0);
+    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, st, c->act_rng, nc);
+    hipLaunchKernelGGL(k_active_range, dim3(1024), dim3(256), 0, st, nc, mask, c->act_rng);
     CAD_CHECK(hipGetLastError());
     int iter = -1;
-    double bnorm = 0.0, xnorm = 0.0, s = 0.0;
+    double bnorm = 0.0, xnorm = 0.0;
     const double u = c->prm.icgtyp != 0 ? c->prm.cgpara : 0.0;
     int npos = 1;
-    RET_IF(sweep_dcmp(c, u, &npos));
+    RET_IF_(sweep_dcmp(c, u, &npos));
     if (npos == 0) {
         *bnorm_out = 0.0;
         *xnorm_out = 0.0;
@@ -516,56 +610,51 @@ int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_ou
     const double epsa2 = c->prm.cgepsa * c->prm.cgepsa;
     const double epsr2 = c->prm.cgepsr * c->prm.cgepsr;
     const double div2 = c->prm.cgdiv * c->prm.cgdiv;
-    RET_IF(apply_minv(c, c->yv, c->bb));
-    RET_IF(dot(c, c->yv, c->yv, &bnorm));
+    RET_IF_(apply_minv(c, c->yv, c->bb));
+    RET_IF_(dot_to(c, c->yv, c->yv, SC_BN, RED_PLAIN));
     iter = 0;
-    RET_IF(apply_a(c, c->xw, c->yv));
-    hipLaunchKernelGGL(k_bminus, sg, sb, 0, st, nc, mask, c->bb, c->yv);
-    RET_IF(apply_minv(c, c->r0, c->yv));
-    RET_IF(dot(c, c->r0, c->r0, &s));
-    xnorm = s;
+    RET_IF_(apply_a(c, c->xw, c->yv));
+    hipLaunchKernelGGL(k_bminus, vg, sb, 0, st, rng, mask, c->bb, c->yv);
+    RET_IF_(apply_minv(c, c->r0, c->yv));
+    RET_IF_(dot_to(c, c->r0, c->r0, SC_S, RED_PLAIN));
+    double init[2];
+    RET_IF_(fetch_sc(c, SC_S, 1, &init[0]));
+    RET_IF_(fetch_sc(c, SC_BN, 1, &init[1]));
+    xnorm = init[0];
+    bnorm = init[1];
     if (!(xnorm <= bnorm * epsr2 + epsa2)) {
-        hipLaunchKernelGGL(k_init_rp, sg, sb, 0, st, nc, mask, c->r0, c->rr, c->pv);
+        hipLaunchKernelGGL(k_init_rp, vg, sb, 0, st, rng, mask, c->r0, c->rr, c->pv);
         for (;;) {
             iter = iter + 1;
             if (iter > c->prm.icgmax) break;
-            RET_IF(apply_a(c, c->pv, c->yv));
-            RET_IF(apply_minv(c, c->ap, c->yv));
-            double w1;
-            RET_IF(dot(c, c->r0, c->ap, &w1));
-            const double alpha = s / w1;
-            hipLaunchKernelGGL(k_axpy_neg, sg, sb, 0, st, nc, mask, alpha, c->ap, c->rr);
-            RET_IF(apply_a(c, c->rr, c->yv));
-            RET_IF(apply_minv(c, c->ae, c->yv));
-            hipLaunchKernelGGL(k_dot2, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, nc, mask,
-                               c->rr, c->ae, c->red_partial);
-            double w12[2];
-            RET_IF(cad_reduce_fetch(c, 2, w12));
-            const double omega = w12[0] / w12[1];
-            hipLaunchKernelGGL(k_update_xr, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, nc,
-                               mask, c->xw, c->rr, c->pv, c->ae, c->r0, alpha, omega,
-                               c->red_partial);
-            double rr2[2];
-            RET_IF(cad_reduce_fetch(c, 2, rr2));
-            xnorm = rr2[0];
+            RET_IF_(apply_a(c, c->pv, c->yv));
+            RET_IF_(apply_minv(c, c->ap, c->yv));
+            RET_IF_(dot_to(c, c->r0, c->ap, SC_W1, RED_ALPHA));
+            hipLaunchKernelGGL(k_axpy_neg, vg, sb, 0, st, rng, mask, c->ksc, c->ap, c->rr);
+            RET_IF_(apply_a(c, c->rr, c->yv));
+            RET_IF_(apply_minv(c, c->ae, c->yv));
+            hipLaunchKernelGGL(k_dot2, rgd, rbd, 0, st, rng, mask, c->rr, c->ae, c->red_partial);
+            RET_IF_(reduce_to(c, 2, SC_W1B, RED_OMEGA));
+            hipLaunchKernelGGL(k_update_xr, rgd, rbd, 0, st, rng, mask, c->xw, c->rr, c->pv, c->ae,
+                               c->r0, c->ksc, c->red_partial);
+            // (R,R) -> SC_XN, (R0,R) -> SC_W1C; beta and the new s on the device
+            RET_IF_(reduce_to(c, 2, SC_XN, RED_BETA));
+            RET_IF_(fetch_sc(c, SC_XN, 1, &xnorm));
             if (xnorm <= bnorm * epsr2 + epsa2) break;
             if (xnorm > div2) {
                 iter = -2;
                 break;
             }
-            w1 = rr2[1];
-            const double beta = w1 * alpha / (s * omega);
-            hipLaunchKernelGGL(k_update_p, sg, sb, 0, st, nc, mask, beta, omega, c->rr, c->ap, c->pv);
-            s = w1;
+            hipLaunchKernelGGL(k_update_p, vg, sb, 0, st, rng, mask, c->ksc, c->rr, c->ap, c->pv);
         }
     }
     bnorm = std::sqrt(bnorm);
     xnorm = std::sqrt(xnorm);
-    if (iter >= 0) hipLaunchKernelGGL(k_copy_active, sg, sb, 0, st, nc, mask, c->xw, x0);
+    if (iter >= 0) hipLaunchKernelGGL(k_copy_active, vg, sb, 0, st, rng, mask, c->xw, x0);
     CAD_CHECK(hipGetLastError());
     if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) {
         int code = 0;
-        RET_IF(cad_sweep_status(c, &code));
+        RET_IF_(cad_sweep_status(c, &code));
         if (code != 0) {
             cad_set_error("MILU sweep schedule failed (code %d)", code);
             return CAD_ERR_HIP;
@@ -574,6 +663,37 @@ int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_ou
     *bnorm_out = bnorm;
     *xnorm_out = xnorm;
     *iter_out = iter;
+    return CAD_OK;
+}
+
+// MILU factorisation of the context's matrix on its own (cad_milu_factor).
+int cad_milu_factor_impl(cad_ctx *c, int *npos)
+{
+    const long long nc = c->d.nc;
+    hipLaunchKernelGGL(k_fill, stream_grid(nc), dim3(256), 0, c->stream, nc, c->dd, 0.0);
+    CAD_CHECK(hipGetLastError());
+    const double u = c->prm.icgtyp != 0 ? c->prm.cgpara : 0.0;
+    return sweep_dcmp(c, u, npos);
+}
+
+// x = M^-1 y for a host y (cad_milu_apply); uses YV and AE as device buffers.
+int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x)
+{
+    const long long nc = c->d.nc;
+    CAD_CHECK(hipMemcpyAsync(c->yv, y, nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_fill, stream_grid(nc), dim3(256), 0, c->stream, nc, c->ae, 0.0);
+    CAD_CHECK(hipGetLastError());
+    RET_IF_(apply_minv(c, c->ae, c->yv));
+    CAD_CHECK(hipMemcpyAsync(x, c->ae, nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) {
+        int code = 0;
+        RET_IF_(cad_sweep_status(c, &code));
+        if (code != 0) {
+            cad_set_error("MILU sweep schedule failed (code %d)", code);
+            return CAD_ERR_HIP;
+        }
+    }
     return CAD_OK;
 }
 
@@ -587,12 +707,12 @@ int cad_time_minv_impl(cad_ctx *c, int reps, float *ms_avg)
     CAD_CHECK(hipEventCreate(&a));
     CAD_CHECK(hipEventCreate(&b));
     CAD_CHECK(hipEventRecord(a, c->stream));
-    for (int r = 0; r < reps; r++) RET_IF(apply_minv(c, c->ae, c->yv));
+    for (int r = 0; r < reps; r++) RET_IF_(apply_minv(c, c->ae, c->yv));
     CAD_CHECK(hipEventRecord(b, c->stream));
     CAD_CHECK(hipEventSynchronize(b));
     if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) {
         int code = 0;
-        RET_IF(cad_sweep_status(c, &code));
+        RET_IF_(cad_sweep_status(c, &code));
         if (code != 0) {
             cad_set_error("MILU sweep schedule failed (code %d)", code);
             return CAD_ERR_HIP;

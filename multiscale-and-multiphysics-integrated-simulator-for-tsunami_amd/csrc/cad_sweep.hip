@@ -19,7 +19,7 @@
 // neighbours of its cell were computed by lanes jl-1 / kl-1 one step earlier
 // (a DPP row shift and a ds_bpermute hand them over).  Tiles pass their last
 // column / row to the next tile through 16-byte mailbox entries {value,
-// sequence number}, stored every step and read 16 steps ahead with
+// sequence number}, stored every step and read four steps ahead with
 // device-scope (sc1) accesses: one entry is one single-copy-atomic 16-byte
 // access, so a consumer that reads the current sequence number also reads the
 // value that goes with it, with no fence.  A tile obtains its place in the
@@ -65,7 +65,6 @@ constexpr unsigned SPIN_LIMIT = 1u << 20;
 constexpr unsigned OOB = 0xFFFFFFF0u;
 constexpr int CPOL_DEVICE = 16;      // sc1: device-scope coherent access
 constexpr int MB_LINES = 16;         // mailbox lines per tile and side (max of TJ, TK)
-constexpr int HD = 16;               // mailbox read-ahead (steps)
 // inactive-cell marker: a signalling-NaN pattern that no arithmetic produces
 constexpr unsigned long long MARK = 0x7FF4DEADBEEF0001ull;
 
@@ -271,6 +270,10 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
 {
     constexpr int TK = 64 / TJ;
     constexpr int NS = Op::NS, S = Op::S;
+    // mailbox read-ahead (steps); the ring is indexed by p % HD, so HD must
+    // divide the stage length
+    constexpr int HD = 4;
+    static_assert(S % HD == 0, "ring depth must divide the stage length");
     using LY = Lay<S>;
     __shared__ Smem<Op> sm;
     const int lane = threadIdx.x & 63;
@@ -359,6 +362,12 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
             hr = c.range[ptj];
         }
         hline = phys_k(kl);
+        if (kl == TK - 1) {
+            // the J-upstream corner cell (its lane 63) is published on its K
+            // side only
+            hbase = half_k;
+            hline = phys_j(TJ - 1);
+        }
     } else if (kl == 0 || lane == HELPER) {
         if (ptk >= 0) {
             hsrc = ptk;
@@ -372,33 +381,43 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
             hval = line_ok(0);
         }
     }
+    // the watched entry of step t is at h0 + DIR*16*t for t in [hlo, hlo+hn)
+    // (the steps where the watched cell is walked and lies in the producer's
+    // range), else not needed
+    int hlo = 0, hn = 0;             // window [hlo, hlo+hn)
+    unsigned h0 = 0;
+    if (hsrc >= 0 && hval) {
+        // cell_i(t, hsk) in [hr.x, hr.y] and 0 <= t - hsk < len
+        const int ta = DIR > 0 ? hr.x - rg.x + hsk : rg.y - hr.y + hsk;
+        const int tb = DIR > 0 ? hr.y - rg.x + hsk : rg.y - hr.x + hsk;
+        hlo = max(ta, hsk);
+        hn = max(min(tb, hsk + len - 1) - hlo + 1, 0);
+        h0 = hbase + mb_off(hsrc, hline, 0) + (unsigned)((DIR > 0 ? rg.x - hsk : rg.y + hsk) * 16);
+    }
     auto h_off = [&] __device__(int t) {
-        const int hi = cell_i(t, hsk);
-        const bool need = hsrc >= 0 && hval && (unsigned)(t - hsk) < (unsigned)len && hi >= hr.x && hi <= hr.y;
-        return need ? hbase + mb_off(hsrc, hline, hi) : OOB;
+        return (unsigned)(t - hlo) < (unsigned)hn ? h0 + (unsigned)(DIR * 16 * t) : OOB;
     };
-    // Mailbox writes.  Lanes with jl == TJ-1 feed the J-downstream tile, lanes
-    // with kl == TK-1 (jl < TJ-1) the K-downstream one; lane FEED2, which has
-    // no feed of its own, writes lane 63's K entry.
-    constexpr int FEED2 = (TJ - 2) + TJ * (TK - 2);
+    // Mailbox writes, one per lane and step.  Lanes with kl == TK-1 feed the
+    // K-downstream tile; lanes with jl == TJ-1 (kl < TK-1) the J-downstream
+    // one.  The corner lane 63 writes its K entry only, which the J-downstream
+    // tile reads too, so it writes it when either downstream tile exists.
     bool fok = false;
-    int fline = 0, fsk = skew;
+    int fline = 0;
+    const int fsk = skew;
     unsigned fbase = 0;
-    if (jl == TJ - 1) {
-        fok = valid && has_cj;
-        fline = phys_k(kl);
-    } else if (kl == TK - 1) {
-        fok = valid && has_ck;
+    if (kl == TK - 1) {
+        fok = valid && (has_ck || (jl == TJ - 1 && has_cj));
         fline = phys_j(jl);
         fbase = half_k;
-    } else if (lane == FEED2) {
-        fok = line_ok(63) && has_ck;
-        fline = phys_j(TJ - 1);
-        fsk = (TJ - 1) + (TK - 1);
-        fbase = half_k;
+    } else if (jl == TJ - 1) {
+        fok = valid && has_cj;
+        fline = phys_k(kl);
     }
+    // the fed entry of step t is at f0 + DIR*16*t for t in [fsk, fsk+fn)
+    const unsigned f0 = fbase + mb_off(tile, fline, 0) + (unsigned)((DIR > 0 ? rg.x - fsk : rg.y + fsk) * 16);
+    const int fn = fok ? len : 0;
     auto f_off = [&] __device__(int t) {
-        return fok && (unsigned)(t - fsk) < (unsigned)len ? fbase + mb_off(tile, fline, cell_i(t, fsk)) : OOB;
+        return (unsigned)(t - fsk) < (unsigned)fn ? f0 + (unsigned)(DIR * 16 * t) : OOB;
     };
 
     __amdgpu_buffer_rsrc_t rs[NS];
@@ -491,8 +510,7 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
             if (Op::CHECK_PIVOT) bad_pivot = bad_pivot || (act && r <= 0.0);
             xcur = r;
             sm.xo[b & 1][lane][e] = act ? r : __builtin_bit_cast(double, MARK);
-            const double r63 = read_lane(r, 63);
-            mb_store(rmb, f_off(t), lane == FEED2 ? r63 : r, seq);
+            mb_store(rmb, f_off(t), r, seq);
         }
         lds_barrier();
     }

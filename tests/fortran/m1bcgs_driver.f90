@@ -30,7 +30,7 @@ program m1bcgs_driver
     end do
   end do
   call cadmas_hip_m1bcgs(ad, ali, alj, alk, aui, auj, auk, bb, x, 0.95d0, 1.0d-15, 1.0d-12, 1.0d30, &
-                         bn, xn, indc, 1, 1, 1, ni, nj, nk, 1, 500, iter)
+                         bn, xn, indc, ni, nj, nk, ni, nj, nk, 1, 500, iter)
   rn = 0; b2 = 0
   do k = 2, nk - 1
     do j = 2, nj - 1
