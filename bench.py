@@ -76,6 +76,21 @@ def prepare(pkg, case):
     return lay, state, flags
 
 
+def measured_traffic(workload_name):
+    """HBM bytes of one M^-1 application measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) for the default
+    workload, from the committed profile summary; None for other workloads."""
+    if workload_name != "buildings_c":
+        return None
+    f = ROOT / "profiles" / "sweep_traffic_buildings_512x256x128.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_minv:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -161,7 +176,8 @@ def main():
     minv_bytes = 96.0 * active
     achieved = minv_bytes / (minv_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(args.workload),
                 "kernel": "MILU M^-1 sweeps (forward+backward), 96 B/active cell",
                 "avg_ms": round(minv_ms, 4), "active_cells": active}
     mean_itr = float(np.mean(itr))
