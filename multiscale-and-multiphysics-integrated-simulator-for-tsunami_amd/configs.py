@@ -48,7 +48,11 @@ class Case:
         return self.ni * self.nj * self.nk
 
 
-def _common(lines, dt, steps, eps_r=1.0e-10, rho=1000.0, nu=1.0e-6, g=9.81, wlevel=None):
+def _common(lines, dt, steps, eps_r=1.0e-10, rho=1000.0, nu=1.0e-6, g=9.81, wlevel=None,
+            tdoor=("OFF", "OFF")):
+    """Time, material and solver lines.  tdoor = (BUB word, DROP word) of
+    OPTION T-DOOR (Source code/vf_iiopt.f:57-77); None leaves a model at the
+    reference default (bubbles rise at 0.2 m/s, droplets FREE-RUNDOWN)."""
     lines += [
         f"TIME CONST {dt:.6e}",
         f"TIME END {steps} 1.0D9",
@@ -57,9 +61,10 @@ def _common(lines, dt, steps, eps_r=1.0e-10, rho=1000.0, nu=1.0e-6, g=9.81, wlev
         f"MATE GRAVITY {g}",
         "COMP MTRX M-ILUBCGSTAB 0.95",
         f"COMP MTRX R-ERROR {eps_r:.6e}",
-        "OPTION T-DOOR BUB OFF",
-        "OPTION T-DOOR DROP OFF",
     ]
+    for word, val in zip(("BUB", "DROP"), tdoor):
+        if val is not None:
+            lines.append(f"OPTION T-DOOR {word} {val}")
     if wlevel is not None:
         lines.append(f"MATE W-LEVEL {wlevel:.6f}")
 
@@ -232,6 +237,44 @@ def corner3d(ni=24, nj=12, nk=16, d=0.01, steps=30, dt=1.0e-4, eps_r=1.0e-10, na
     return Case(name or f"corner_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
 
 
+def tdoor_drops(ni=20, nj=5, nk=16, d=0.02, steps=300, dt=1.0e-3, eps_r=1.0e-10, wbub="2.0",
+                name=None):
+    """TimerDoor bubble and droplet models (Source code/vf_fbubup.f, vf_fdropf.f).
+
+    A one-cell sheet of F = 0.5 hangs in the air (DEBUG F-BOX): every cell of
+    it is a droplet (gas-flagged with water in it).  The sheet falls onto two
+    blocks of different heights (droplets that land on the lower block next to
+    the taller one slide sideways, and so do those next to the y walls) and
+    into still water (MATE W-LEVEL), where merged cells feed the bubble model.
+    Bubbles rise at WBUB m/s (2.0 fires every 10 steps here); the droplet model
+    is left at the reference default.
+    """
+    lines = [f"# timer door {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE"]
+    lines.append(f"OBST 12 1 1 14 {nj} 6")
+    lines.append(f"OBST 15 1 1 17 {nj} 8")
+    _common(lines, dt, steps, eps_r, wlevel=3 * d, tdoor=(wbub, None))
+    lines.append(f"DEBUG F-BOX 10 1 12 17 {nj} 12 0.5")
+    return Case(name or f"tdoor_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
+def tdoor_dam(ni=32, nj=4, nk=24, d=0.02, steps=900, dt=5.0e-4, eps_r=1.0e-10, name=None):
+    """Dam break with both TimerDoor models at the reference defaults (no
+    OPTION T-DOOR line): the bore runs up the far wall and throws droplets
+    upward, and the bubble model fires every ZZ(2,K)/0.2 s."""
+    lines = [f"# dam break, timer door defaults {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE"]
+    _common(lines, dt, steps, eps_r, tdoor=(None, None))
+    lines.append(f"DEBUG F-BOX 1 1 1 {ni * 3 // 8} {nj} {nk * 5 // 8} 1.0")
+    return Case(name or f"tdoor_dam_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
 def named(name):
     """The workloads by name (sizes as in the benchmark table)."""
     table = {
@@ -241,5 +284,7 @@ def named(name):
         "tiny": lambda: dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5),
         "beach_small": lambda: beach_small(),
         "corner3d": lambda: corner3d(),
+        "tdoor": lambda: tdoor_drops(),
+        "tdoor_dam": lambda: tdoor_dam(),
     }
     return table[name]()

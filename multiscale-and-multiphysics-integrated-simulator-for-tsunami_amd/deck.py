@@ -269,10 +269,20 @@ def _parse_comp(d, w):
 def _parse_option(d, w):
     key = w[1]
     if key == "T-DOOR":
+        # Source code/vf_iiopt.f:57-77
+        if len(w) < 4:
+            raise DeckError("OPTION T-DOOR: SYNTAX ERROR.")
         if w[2] == "BUB":
             d.wbub = 0.0 if w[3] == "OFF" else _num(w[3])
+            if d.wbub < 1.0e-20 and w[3] != "OFF":
+                raise DeckError("OPTION T-DOOR BUB: INVALID VALUE.")
         elif w[2] == "DROP":
-            d.idrop = 0 if w[3] == "OFF" else 1
+            if w[3] == "OFF":
+                d.idrop = 0
+            elif w[3] == "FREE-RUNDOWN":
+                d.idrop = 1
+            else:
+                raise DeckError(f"OPTION T-DOOR DROP {w[3]}: UNKNOWN WORD.")
         else:
             raise DeckError(f"OPTION T-DOOR {w[2]}")
     elif key == "S-CELL-VEL":
@@ -286,11 +296,6 @@ def _parse_option(d, w):
 
 
 def _validate(d):
-    # Defaults that switch on models outside this path (vf_a2dflt.f:151,205,223)
-    if d.wbub >= 1.0e-20:
-        raise UnsupportedDeck("T-DOOR bubble model is on by default: the deck must say OPTION T-DOOR BUB OFF")
-    if d.idrop >= 1:
-        raise UnsupportedDeck("T-DOOR droplet model is on by default: the deck must say OPTION T-DOOR DROP OFF")
     if d.dt_const is None:
         raise UnsupportedDeck("TIME CONST is required (the automatic time step is not implemented)")
     if d.loops != 1:

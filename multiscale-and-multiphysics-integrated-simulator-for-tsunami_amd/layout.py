@@ -267,7 +267,7 @@ class Layout:
                     scmvp=d.scmvp, vvmax=d.vvmax, flower=1.0e-6, fupper=1.0 - 1.0e-6,
                     plower=d.plower, cgpara=d.cgpara, cgepsa=d.cgepsa, cgepsr=d.cgepsr,
                     cgdiv=1.0e30, icgtyp=d.icgtyp, icgmax=d.icgmax, ibsuw0=d.ibsuw0,
-                    loops=d.loops)
+                    loops=d.loops, wbub=d.wbub, idrop=d.idrop)
 
 
 def from_text(text):

@@ -14,7 +14,8 @@
  *   F FIX / FREE (INDB(4,*) = 1,2), static porosity (IPRNT<=1), donor scheme
  *   (ISCMVP=0, ISCMFF=0), drag by CD (IDRGN=0), no sponge (IDAMP<0), no wave
  *   maker / wave source / open boundary, no air-pressure model (PVCP0<ZERO),
- *   T-DOOR bubble and drop models off, LOOPS=1.
+ *   LOOPS=1.  The TimerDoor bubble and droplet models (OPTION T-DOOR) are
+ *   restated (orc_fbubup, orc_fdropf).
  */
 #include "cadmas_oracle.h"
 
@@ -1395,6 +1396,272 @@ static void bspp(orc_state *s)
 }
 
 /* VF_F1CAL (vf_f1cal.f:124-239) for the restated feature set. */
+/* ======================================================================
+ * TimerDoor corrections
+ * ====================================================================== */
+
+/* Bubble rise: in every level K whose timer has run out, water moves down
+ * from K+1 into fluid cells of K that are not full.  vf_fbubup.f:35-51. */
+void orc_fbubup(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int k = nk - 2; k >= 2; k--) {
+        const double dtbub = ZZ(2, k) / s->wbub;
+        if (s->tnow - s->tbub[k - 1] > dtbub) {
+            const int k1 = k + 1;
+            for (int j = 2; j <= nj - 1; j++)
+                for (int i = 2; i <= ni - 1; i++) {
+                    const size_t n = ID(i, j, k), n1 = ID(i, j, k1);
+                    if (s->ff[n] < s->fupper && s->nf[n] == 0 && s->nf[n1] != -1) {
+                        const double v0 = ZZ(2, k) * s->ggv[n];
+                        const double v1 = ZZ(2, k1) * s->ggv[n1];
+                        const double fmove = dmin((1.0 - s->ff[n]) * v0, s->ff[n1] * v1);
+                        s->ff[n1] = s->ff[n1] - fmove / v1;
+                        s->ff[n] = s->ff[n] + fmove / v0;
+                    }
+                }
+            s->tbub[k - 1] = s->tbub[k - 1] + dtbub;
+        }
+    }
+}
+
+/* droplet state of a cell that holds no droplet (vf_fdropf.f:299-306) */
+static void drop_clear(orc_state *s, size_t n)
+{
+    s->droptx[n] = -1.0e10;
+    s->dropty[n] = 0.0;
+    s->droptz[n] = 0.0;
+    s->dropuu[n] = 0.0;
+    s->dropvv[n] = 0.0;
+    s->dropww[n] = 0.0;
+}
+
+/* cell-centre velocity of cell c added to (up,vp,wp) (vf_fdropf.f:82-84) */
+static void drop_addvel(const orc_state *s, size_t c, double *up, double *vp, double *wp)
+{
+    const size_t ni = (size_t)s->numi, nij = ni * (size_t)s->numj;
+    *up = *up + 0.5 * (s->uu[c + 1] + s->uu[c]);
+    *vp = *vp + 0.5 * (s->vv[c + ni] + s->vv[c]);
+    *wp = *wp + 0.5 * (s->ww[c + nij] + s->ww[c]);
+}
+
+/* weighted droplet values moved to (or kept in) cell m, from droplet n, with
+ * weight q; the moving axis a has its time advanced by dt and, along z, its
+ * velocity reduced by dt*g (vf_fdropf.f:359-367, 371-380, 599-609) */
+typedef struct { double *tx, *ty, *tz, *uu, *vv, *ww, *ff; } drop_wk;
+
+static void drop_acc(const orc_state *s, const drop_wk *w, size_t m, size_t n, double q, int a,
+                     double dt)
+{
+    w->tx[m] = w->tx[m] + q * (a == 0 ? s->droptx[n] + dt : s->droptx[n]);
+    w->ty[m] = w->ty[m] + q * (a == 1 ? s->dropty[n] + dt : s->dropty[n]);
+    w->tz[m] = w->tz[m] + q * (a == 2 ? s->droptz[n] + dt : s->droptz[n]);
+    w->uu[m] = w->uu[m] + q * s->dropuu[n];
+    w->vv[m] = w->vv[m] + q * s->dropvv[n];
+    w->ww[m] = w->ww[m] + q * (a == 2 ? s->dropww[n] - dt * s->grz0 : s->dropww[n]);
+    w->ff[m] = w->ff[m] + q;
+}
+
+static int is_drop(const orc_state *s, size_t n) { return s->ff[n] > s->flower && s->nf[n] == 8; }
+
+/* New droplets take the mean velocity of their fluid (else surface)
+ * neighbours; old droplets resting on an obstacle slide down the local
+ * obstacle/porosity slope.  vf_fdropf.f:66-307. */
+static void fdropf_start(orc_state *s)
+{
+    DECL_DIMS(s);
+    /* edge neighbours in the reference's order: offsets (di,dj,dk) */
+    static const int edge[12][3] = {
+        {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
+        {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0},
+        {0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1}};
+    static const int face[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, 0, -1}, {0, 0, 1}, {0, -1, 0}, {0, 1, 0}};
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                const size_t n = ID(i, j, k);
+                if (!is_drop(s, n)) {
+                    drop_clear(s, n);
+                    continue;
+                }
+                if (s->droptx[n] < 0.0) {
+                    s->droptx[n] = s->tnow;
+                    s->dropty[n] = s->tnow;
+                    s->droptz[n] = s->tnow;
+                    int ip = 0;
+                    double up = 0.0, vp = 0.0, wp = 0.0;
+                    for (int e = 0; e < 12; e++) {
+                        const int *o = edge[e];
+                        const size_t c = ID(i + o[0], j + o[1], k + o[2]);
+                        /* the two face cells between n and c: first along the
+                         * lower axis of the pair, then along the higher one */
+                        size_t fa, fb;
+                        if (o[0] != 0) {
+                            fa = ID(i + o[0], j, k);
+                            fb = o[1] != 0 ? ID(i, j + o[1], k) : ID(i, j, k + o[2]);
+                        } else {
+                            fa = ID(i, j + o[1], k);
+                            fb = ID(i, j, k + o[2]);
+                        }
+                        if (s->nf[c] == 0 && (s->nf[fa] != -1 || s->nf[fb] != -1)) {
+                            ip++;
+                            drop_addvel(s, c, &up, &vp, &wp);
+                        }
+                    }
+                    if (ip == 0)
+                        for (int e = 0; e < 6; e++) {
+                            const size_t c = ID(i + face[e][0], j + face[e][1], k + face[e][2]);
+                            if (s->nf[c] != -1 && s->nf[c] != 8) {
+                                ip++;
+                                drop_addvel(s, c, &up, &vp, &wp);
+                            }
+                        }
+                    if (ip >= 1) {
+                        const double cp = 1.0 / (double)ip;
+                        s->dropuu[n] = up * cp;
+                        s->dropvv[n] = vp * cp;
+                        s->dropww[n] = wp * cp;
+                    } else {
+                        s->dropuu[n] = 0.5 * (s->uu[n] + s->uu[n + 1]);
+                        s->dropvv[n] = 0.5 * (s->vv[n] + s->vv[n + ni]);
+                        s->dropww[n] = 0.5 * (s->ww[n] + s->ww[n + nij]);
+                    }
+                    if (s->nf[n - nij] == -1) {
+                        s->dropuu[n] = 0.0;
+                        s->dropvv[n] = 0.0;
+                        s->dropww[n] = 0.0;
+                    }
+                } else if (s->nf[n - nij] == -1) {
+                    double h1 = s->nf[n - 1] == -1 ? 1.0 : 1.0 - s->ggv[n - 1];
+                    double h2 = s->nf[n + 1] == -1 ? 1.0 : 1.0 - s->ggv[n + 1];
+                    double hx = 0.5 * (h2 - h1) * ZZ(2, k);
+                    h1 = s->nf[n - ni] == -1 ? 1.0 : 1.0 - s->ggv[n - ni];
+                    h2 = s->nf[n + ni] == -1 ? 1.0 : 1.0 - s->ggv[n + ni];
+                    double hy = 0.5 * (h2 - h1) * ZZ(2, k);
+                    hx = hx / XX(2, i);
+                    hy = hy / YY(2, j);
+                    const double dn = 1.0 / (hx * hx + hy * hy + 1.0);
+                    hx = -s->grz0 * hx * dn;
+                    hy = -s->grz0 * hy * dn;
+                    s->dropuu[n] = s->dropuu[n] + s->dtnow * hx;
+                    s->dropvv[n] = s->dropvv[n] + s->dtnow * hy;
+                }
+            }
+}
+
+/* One pass of droplet moves along axis a (0 x, 1 y, 2 z) towards +a (l = 1)
+ * or -a (l = 2), then the weighted means.  vf_fdropf.f:317-398 (x),
+ * :409-490 (y), :501-601 (z). */
+static void fdropf_pass(orc_state *s, const drop_wk *w, int a, int l)
+{
+    DECL_DIMS(s);
+    const int nn[3] = {ni, nj, nk};
+    const size_t st[3] = {1, (size_t)ni, nij};
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                const size_t n = ID(i, j, k);
+                w->tx[n] = 0.0; w->ty[n] = 0.0; w->tz[n] = 0.0;
+                w->uu[n] = 0.0; w->vv[n] = 0.0; w->ww[n] = 0.0; w->ff[n] = 0.0;
+            }
+    /* sweep range of the moving index: interior, against the move */
+    const int dl = l == 1 ? -1 : 1;
+    const int p0 = l == 1 ? nn[a] - 1 : 2, p1 = l == 1 ? 2 : nn[a] - 1;
+    const double sg = l == 1 ? 1.0 : -1.0;
+    for (int p = p0; dl > 0 ? p <= p1 : p >= p1; p += dl) {
+        /* the reference's nests: x (I; K; J), y (J; K; I), z (K; J; I) */
+        const double xz = a == 2 ? sqrt(2.0 * s->grz0 * ZZ(2, p)) : 0.0;
+        for (int o = 2; o <= (a == 2 ? nj : nk) - 1; o++)
+            for (int q = 2; q <= (a == 0 ? nj : ni) - 1; q++) {
+                int i, j, k;
+                if (a == 0) { i = p; k = o; j = q; }
+                else if (a == 1) { j = p; k = o; i = q; }
+                else { k = p; j = o; i = q; }
+                const size_t n = ID(i, j, k);
+                const size_t n1 = dl > 0 ? n - st[a] : n + st[a];
+                double dt = 0.0;
+                if (is_drop(s, n) && s->nf[n1] != -1) {
+                    int go = 0;
+                    double tlast;
+                    if (a == 2) {
+                        const double wv = s->dropww[n];
+                        if (l == 1) {
+                            if (wv > xz) dt = (wv - sqrt(wv * wv - xz * xz)) / s->grz0;
+                        } else if (wv <= xz) {
+                            if (wv <= 0.0)
+                                dt = (wv + sqrt(wv * wv + xz * xz)) / s->grz0;
+                            else
+                                dt = 2.0 * wv / s->grz0;
+                        }
+                        tlast = s->droptz[n];
+                        go = dt > 0.0 && s->tnow - tlast > dt;
+                    } else {
+                        const double vel = a == 0 ? s->dropuu[n] : s->dropvv[n];
+                        if (sg * vel > 0.0) {
+                            dt = (a == 0 ? XX(2, i) : YY(2, j)) / fabs(vel);
+                            tlast = a == 0 ? s->droptx[n] : s->dropty[n];
+                            go = s->tnow - tlast > dt;
+                            if (!go) dt = 0.0;
+                        }
+                    }
+                    if (a == 2 && !go) dt = 0.0;
+                    if (go) {
+                        int i1 = i, j1 = j, k1 = k;
+                        if (a == 0) i1 = i - dl; else if (a == 1) j1 = j - dl; else k1 = k - dl;
+                        const double v0 = XX(2, i) * YY(2, j) * ZZ(2, k) * s->ggv[n];
+                        const double v1 = XX(2, i1) * YY(2, j1) * ZZ(2, k1) * s->ggv[n1];
+                        const double fmove = dmin((1.0 - s->ff[n1]) * v1, s->ff[n] * v0);
+                        s->ff[n] = s->ff[n] - fmove / v0;
+                        s->ff[n1] = s->ff[n1] + fmove / v1;
+                        if (s->nf[n1] == 8) drop_acc(s, w, n1, n, fmove, a, dt);
+                    }
+                }
+                if (is_drop(s, n)) {
+                    const double fold = s->ff[n] * XX(2, i) * YY(2, j) * ZZ(2, k) * s->ggv[n];
+                    drop_acc(s, w, n, n, fold, a, dt);
+                } else {
+                    drop_clear(s, n);
+                }
+            }
+    }
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                const size_t n = ID(i, j, k);
+                if (!is_drop(s, n)) continue;
+                const double wq = 1.0 / w->ff[n];
+                s->droptx[n] = w->tx[n] * wq;
+                s->dropty[n] = w->ty[n] * wq;
+                s->droptz[n] = w->tz[n] * wq;
+                s->dropuu[n] = w->uu[n] * wq;
+                s->dropvv[n] = w->vv[n] * wq;
+                s->dropww[n] = w->ww[n] * wq;
+                if (a == 2 && l == 1) {
+                    const double xz = sqrt(2.0 * s->grz0 * ZZ(2, k));
+                    if (s->nf[n + nij] == -1 && s->dropww[n] > xz) {
+                        s->dropuu[n] = 0.0;
+                        s->dropvv[n] = 0.0;
+                        s->dropww[n] = 0.0;
+                    }
+                }
+            }
+}
+
+/* Droplet free fall and run-down.  vf_fdropf.f:1-678. */
+int orc_fdropf(orc_state *s)
+{
+    DECL_DIMS(s);
+    const size_t nc = nij * (size_t)nk;
+    double *buf = (double *)malloc(7 * nc * sizeof(double));
+    if (!buf) return -1;
+    drop_wk w = {buf, buf + nc, buf + 2 * nc, buf + 3 * nc, buf + 4 * nc, buf + 5 * nc, buf + 6 * nc};
+    fdropf_start(s);
+    for (int a = 0; a < 3; a++)
+        for (int l = 1; l <= 2; l++) fdropf_pass(s, &w, a, l);
+    free(buf);
+    return 0;
+}
+
 int orc_f1cal(orc_state *s)
 {
     DECL_DIMS(s);
@@ -1416,6 +1683,15 @@ int orc_f1cal(orc_state *s)
     bsuwem(s);
     memcpy(s->nlim, s->nf, nc * sizeof(int));
     fnfini(s);
+    if (s->wbub >= ZERO) {
+        orc_fbubup(s);
+        fnfini(s);
+    }
+    if (s->idrop >= 1) {
+        rc = orc_fdropf(s);
+        if (rc) return rc;
+        fnfini(s);
+    }
     rc = fnfprv(s);
     if (rc) return rc;
     cindc(s);
@@ -1503,11 +1779,19 @@ int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double 
     rc = bwuwt(s);
     if (rc) return rc;
     bwpp(s);
+    /* TimerDoor timers (vf_cinit.f:412-421) */
+    if (s->tbub)
+        for (int k = 0; k < nk; k++) s->tbub[k] = 0.0;
+    if (s->droptx)
+        for (size_t n = 0; n < nc; n++) drop_clear(s, n);
     return 0;
 }
 
+/* one pass of the time loop body: TNOW += DTNOW (vf_a1main.f:1047), then
+ * VF_V1CAL and VF_F1CAL */
 int orc_step(orc_state *s, double *pt)
 {
+    s->tnow = s->tnow + s->dtnow;
     int rc = orc_v1cal(s, pt);
     if (rc) return rc;
     return orc_f1cal(s);

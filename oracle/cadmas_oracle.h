@@ -47,6 +47,15 @@ typedef struct orc_state {
     /* per-step outputs */
     double cgbnrm, cgxnrm, fsum, fcut;
     int icgitr;
+    /* TimerDoor models (vf_f1cal.f:208-217): bubbles rise when
+     * WBUB >= 1e-20, droplets fall when IDROP >= 1.  TNOW is the time of
+     * COMMON /VF_ATIMER/ (orc_step advances it by DTNOW first, as the time
+     * loop does at vf_a1main.f:1047); tbub holds TBUB(NUMK), the drop*
+     * fields DROPTX..DROPWW (NUMI,NUMJ,NUMK).  Unused when both are off. */
+    double tnow, wbub;
+    int idrop;
+    double *tbub;
+    double *droptx, *dropty, *droptz, *dropuu, *dropvv, *dropww;
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */
@@ -76,6 +85,9 @@ int orc_post_bc(orc_state *s);
 int orc_v1cal(orc_state *s, double *pt);
 /* VF_F1CAL: phase (c) */
 int orc_f1cal(orc_state *s);
+/* TimerDoor corrections on their own (vf_fbubup.f, vf_fdropf.f) */
+void orc_fbubup(orc_state *s);
+int orc_fdropf(orc_state *s);
 /* VF_CDIV00 */
 double orc_cdiv00(const orc_state *s);
 /* the part of VF_CINIT that follows the initial F (clip, flags, p, BCs) */

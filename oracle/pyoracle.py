@@ -39,6 +39,9 @@ class OrcState(C.Structure):
         ("icgtyp", C.c_int), ("icgmax", C.c_int), ("ibsuw0", C.c_int), ("loops", C.c_int),
         ("cgbnrm", C.c_double), ("cgxnrm", C.c_double), ("fsum", C.c_double),
         ("fcut", C.c_double), ("icgitr", C.c_int),
+        ("tnow", C.c_double), ("wbub", C.c_double), ("idrop", C.c_int),
+        ("tbub", PD), ("droptx", PD), ("dropty", PD), ("droptz", PD),
+        ("dropuu", PD), ("dropvv", PD), ("dropww", PD),
     ]
 
 
@@ -68,6 +71,9 @@ def lib():
         L.orc_post_bc.argtypes = [C.POINTER(OrcState)]
         L.orc_v1cal.argtypes = [C.POINTER(OrcState), PD]
         L.orc_f1cal.argtypes = [C.POINTER(OrcState)]
+        L.orc_fbubup.argtypes = [C.POINTER(OrcState)]
+        L.orc_fbubup.restype = None
+        L.orc_fdropf.argtypes = [C.POINTER(OrcState)]
         L.orc_cdiv00.argtypes = [C.POINTER(OrcState)]
         L.orc_cdiv00.restype = C.c_double
         L.orc_step.argtypes = [C.POINTER(OrcState), PD]
@@ -88,13 +94,15 @@ def _pi(a):
 
 
 F64 = ("XX", "YY", "ZZ", "UU", "VV", "WW", "PP", "FF", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ",
-       "GLV", "GLX", "GLY", "GLZ", "BCU", "BCV", "BCW", "BCP", "BCF", "BCVI")
+       "GLV", "GLX", "GLY", "GLZ", "BCU", "BCV", "BCW", "BCP", "BCF", "BCVI",
+       "TBUB", "DROPTX", "DROPTY", "DROPTZ", "DROPUU", "DROPVV", "DROPWW")
+DROP = ("DROPTX", "DROPTY", "DROPTZ", "DROPUU", "DROPVV", "DROPWW")
 I32 = ("NF", "INDX", "INDY", "INDZ", "INDC", "INDB")
 
 DEFAULT_PARAMS = dict(rho0=1000.0, grz0=9.8, anu0=1.0e-6, dtnow=1.0e-3, scmvp=1.0, vvmax=30.0,
                       flower=1.0e-6, fupper=1.0 - 1.0e-6, plower=1.0e-4, cgpara=0.95,
                       cgepsa=1.0e-15, cgepsr=1.0e-12, cgdiv=1.0e30, icgtyp=1, icgmax=500,
-                      ibsuw0=0, loops=1)
+                      ibsuw0=0, loops=1, tnow=0.0, wbub=0.2, idrop=1)
 
 
 class Fields:
@@ -117,6 +125,12 @@ class Fields:
             self.a["BCVI"] = np.zeros(self.numb, order="F")
         if "CD0" not in self.a:
             self.a["CD0"] = np.zeros(self.shape, order="F")
+        # TimerDoor state at its VF_CINIT value unless given (vf_cinit.f:412-421)
+        if "TBUB" not in self.a:
+            self.a["TBUB"] = np.zeros(numk)
+        for k in DROP:
+            if k not in self.a:
+                self.a[k] = np.full(self.shape, -1.0e10 if k == "DROPTX" else 0.0, order="F")
         self.a["INDS"] = np.zeros(numi * numj * numk, dtype=np.int32)
         nums = 0
         if inds is not None:
@@ -161,6 +175,12 @@ class Fields:
 
     def f1cal(self):
         return lib().orc_f1cal(C.byref(self.st))
+
+    def fbubup(self):
+        lib().orc_fbubup(C.byref(self.st))
+
+    def fdropf(self):
+        return lib().orc_fdropf(C.byref(self.st))
 
     def step(self):
         pt = np.zeros(self.shape, order="F")

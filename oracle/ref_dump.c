@@ -11,7 +11,8 @@
  * files, and call the original.  Argument lists follow the reference:
  *   VF_V1CAL   Source code/vf_v1cal.f:1-9     (called vf_a1main.f:1058)
  *   VF_VPSOL   Source code/vf_vpsol.f:1-3     (called vf_v1cal.f:278)
- *   VF_F1CAL   Source code/vf_f1cal.f:1-12    (called vf_a1main.f:1104)
+ *   VF_F1CAL   Source code/vf_f1cal.f:1-12    (called vf_a1main.f:1104);
+ *              its records also hold the TimerDoor state TBUB, DROPTX..DROPWW
  *   VF_CDIV00  Source code/vf_cdiv00.f:1      (called vf_a1main.f:954)
  * COMMON block layouts follow Source code/VF_ANUMBI.h, VF_ACOMPR.h,
  * VF_ACOMPI.h, VF_ATIMER.h, VF_ATIMEI.h (leading members only).
@@ -130,6 +131,16 @@ static void inds(const I4 *a)
     put("INDS", 'i', 1, d, vf_anumbi_.nums > 0 ? (const void *)a : (const void *)&zero);
 }
 
+/* TimerDoor state (vf_f1cal.f:59-65): TBUB(NUMK) and the droplet fields */
+static void tdoor(const R8 *tbub, const R8 *dtx, const R8 *dty, const R8 *dtz,
+                  const R8 *duu, const R8 *dvv, const R8 *dww)
+{
+    int d[1] = {vf_anumbi_.numk};
+    put("TBUB", 'd', 1, d, tbub);
+    f3("DROPTX", dtx); f3("DROPTY", dty); f3("DROPTZ", dtz);
+    f3("DROPUU", duu); f3("DROPVV", dvv); f3("DROPWW", dww);
+}
+
 /* ------------------------------------------------------------------ */
 #define P R8 *
 void vf_v1cal_orig_(I4 *iloop, P xx, P yy, P zz, P uu, P vv, P ww, P pp, P ff, P anu, P cm0, P cd0,
@@ -233,6 +244,7 @@ void vf_f1cal_(P xx, P yy, P zz, P uu, P vv, P ww, P pp, P ff, P anu, P ggv, P g
         f3("UU", uu); f3("VV", vv); f3("WW", ww); f3("PP", pp); f3("FF", ff);
         fb("BCU", bcu); fb("BCV", bcv); fb("BCW", bcw); fb("BCP", bcp); fb("BCF", bcf);
         i3("NF", nf); i3("INDC", indc); inds(inds_);
+        tdoor(tbub, dtx, dty, dtz, duu, dvv, dww);
         close_dump();
     }
     vf_f1cal_orig_(xx, yy, zz, uu, vv, ww, pp, ff, anu, ggv, ggx, ggy, ggz, bcu, bcv, bcw, bcp, bcf,
@@ -246,6 +258,7 @@ void vf_f1cal_(P xx, P yy, P zz, P uu, P vv, P ww, P pp, P ff, P anu, P ggv, P g
         fb("BCU", bcu); fb("BCV", bcv); fb("BCW", bcw); fb("BCP", bcp); fb("BCF", bcf);
         i3("NF", nf); i3("INDC", indc); inds(inds_);
         i3("NLIM", nlim);
+        tdoor(tbub, dtx, dty, dtz, duu, dvv, dww);
         close_dump();
     }
 }

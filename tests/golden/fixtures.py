@@ -35,9 +35,14 @@ def trajectory(case):
 
 def deck_params(deck_text):
     """Physics / solver parameters stated in a deck (defaults of vf_a2dflt.f otherwise)."""
-    p = dict(rho0=1000.0, grz0=9.8, anu0=1.0e-6, cgepsr=1.0e-12, cgpara=0.95)
+    p = dict(rho0=1000.0, grz0=9.8, anu0=1.0e-6, cgepsr=1.0e-12, cgpara=0.95, wbub=0.2, idrop=1)
     for line in str(deck_text).splitlines():
         w = line.split("#", 1)[0].split()
+        if len(w) >= 4 and w[0] == "OPTION" and w[1] == "T-DOOR":
+            if w[2] == "BUB":
+                p["wbub"] = 0.0 if w[3] == "OFF" else float(w[3].replace("D", "E"))
+            elif w[2] == "DROP":
+                p["idrop"] = 0 if w[3] == "OFF" else 1
         if len(w) >= 3 and w[0] == "MATE":
             v = float(w[2].replace("D", "E"))
             if w[1] == "DENSITY":
@@ -59,7 +64,8 @@ def state_arrays(st, rec):
     keys = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY",
             "GLZ", "INDX", "INDY", "INDZ", "INDB", "BCVI")
     arrays = {k: st[k] for k in keys}
-    for k in ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC"):
+    for k in ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC",
+              "TBUB", "DROPTX", "DROPTY", "DROPTZ", "DROPUU", "DROPVV", "DROPWW"):
         if k in rec:
             arrays[k] = rec[k]
     return arrays

@@ -56,12 +56,20 @@ def _cases():
         "beach_small": (lambda: cfg.beach_small(steps=40, name="beach_small"), [1, 2, 3, 20, 40]),
         "corner3d": (lambda: cfg.corner3d(steps=60, name="corner3d"), [1, 2, 10, 30, 60]),
         "dam_a": (lambda: cfg.dam_break(name="dam_a"), [1, 100, 500]),
+        # TimerDoor bubble / droplet models (vf_fbubup.f, vf_fdropf.f)
+        "tdoor": (lambda: cfg.tdoor_drops(name="tdoor"),
+                  [1, 2, 100, 101, 145, 170, 171, 180, 195, 200, 210, 220, 230, 245, 300]),
+        "tdoor_dam": (lambda: cfg.tdoor_dam(name="tdoor_dam"),
+                      [1, 2, 26, 200, 201, 400, 533, 571, 600, 601, 700, 900]),
     }
 
 
 # keep only these records for large cases (tag -> keys or None for all)
+_F1_ONLY = {"f1_in": None, "f1_out": None}
 _KEEP = {
     "dam_a": {1: {"v1_in": None, "f1_in": ("BCF",)}, 100: {"f1_out": None}, 500: {"f1_out": None}},
+    "tdoor": {s: _F1_ONLY for s in (2, 100, 101, 145, 170, 171, 180, 195, 200, 210, 220, 230, 245, 300)},
+    "tdoor_dam": {s: _F1_ONLY for s in (2, 26, 200, 201, 400, 533, 571, 600, 601, 700, 900)},
 }
 
 

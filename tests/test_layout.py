@@ -42,7 +42,10 @@ def test_metrics_closed_form(pkg):
 
 
 @pytest.mark.parametrize("text,exc", [
-    ("GRID X 0 1 END\nGRID Y 0 1 END\nGRID Z 0 1 END\nTIME CONST 1e-3\n", "UnsupportedDeck"),
+    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR DROP FALL\n"
+     "TIME CONST 1e-3\n", "DeckError"),
+    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB\n"
+     "TIME CONST 1e-3\n", "DeckError"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
      "OPTION T-DOOR DROP OFF\nTIME AUTO 1e-3 0.1\n", "UnsupportedDeck"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
@@ -56,10 +59,18 @@ def test_deck_rejects_outside_the_path(pkg, text, exc):
         pkg.deck.parse(text)
 
 
-def test_deck_default_bubble_model_must_be_switched_off(pkg):
-    text = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
-    with pytest.raises(pkg.deck.UnsupportedDeck, match="T-DOOR"):
-        pkg.deck.parse(text)
+def test_deck_timer_door_words(pkg):
+    """OPTION T-DOOR as Source code/vf_iiopt.f:57-77 reads it, with the
+    defaults of vf_a2dflt.f:151,205 (both models on)."""
+    base = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
+    d = pkg.deck.parse(base)
+    assert (d.wbub, d.idrop) == (0.2, 1)
+    d = pkg.deck.parse(base + "OPTION T-DOOR BUB 0.5\nOPTION T-DOOR DROP OFF\n")
+    assert (d.wbub, d.idrop) == (0.5, 0)
+    d = pkg.deck.parse(base + "OPTION T-DOOR BUB OFF\nOPTION T-DOOR DROP FREE-RUNDOWN\n")
+    assert (d.wbub, d.idrop) == (0.0, 1)
+    lay = pkg.layout.from_text(base + "B.C. D VP SLIP\nB.C. D F FREE\n")
+    assert (lay.params()["wbub"], lay.params()["idrop"]) == (0.2, 1)
 
 
 def test_workload_decks_parse(pkg):

@@ -97,3 +97,54 @@ def test_dam_break_500_steps_bitexact():
             ref = fx.step(case, s)["f1_out"]
             for k in ("UU", "VV", "WW", "PP", "FF", "NF"):
                 assert np.array_equal(F[k], ref[k]), (s, k)
+
+
+TDOOR = ["tdoor", "tdoor_dam"]
+DROP = ("DROPTX", "DROPTY", "DROPTZ", "DROPUU", "DROPVV", "DROPWW")
+
+
+@pytest.mark.parametrize("case", TDOOR)
+def test_f1cal_timer_door_bitexact(case):
+    """VF_F1CAL with the bubble (VF_FBUBUP) and droplet (VF_FDROPF) models on:
+    F, flags, the droplet fields and the bubble timers equal the reference's
+    at every recorded step (droplets born, falling, rising, sliding off
+    obstacles, merging, landing; bubble levels firing)."""
+    st = fx.static(case)
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        fin, ref = r["f1_in"], r["f1_out"]
+        p = _params(st, fin)
+        p["tnow"] = float(fin["TNOW"][0])
+        F = _fields(st, fin, p)
+        assert F.f1cal() == 0
+        for k in ("FF", "UU", "VV", "WW", "PP", "BCF", "NF", "INDC", "NLIM", "TBUB") + DROP:
+            assert np.array_equal(F[k], ref[k]), (case, s, k)
+        assert F.nums == int(ref["NUMS"][0])
+        assert F.st.fsum == ref["FSUM"][0] and F.st.fcut == ref["FCUT"][0]
+
+
+@pytest.mark.parametrize("case", TDOOR)
+def test_timer_door_free_run_bitexact(case):
+    """Every step of the run from the reference's initial state: ITR and
+    ||div|| each step, and all fields at the recorded steps, equal the
+    reference's (TNOW advanced by the step as the time loop does)."""
+    st = fx.static(case)
+    r1 = fx.step(case, 1)
+    p = _params(st, r1["v1_in"])
+    arr = fx.state_arrays(st, r1["v1_in"])
+    arr["BCF"] = r1["f1_in"]["BCF"]
+    for k in ("TBUB",) + DROP:
+        arr[k] = r1["f1_in"][k]
+    F = po.Fields(arr, p, inds=r1["v1_in"]["INDS"][: int(r1["v1_in"]["NUMS"][0])])
+    tr = fx.trajectory(case)
+    kept = set(fx.steps(case))
+    for s in range(1, int(tr["STEP"][-1]) + 1):
+        rc, _ = F.step()
+        assert rc == 0, s
+        assert F.st.icgitr == tr["ITR"][s], s
+        assert F.cdiv00() == tr["DV_FULL"][s], s
+        if s in kept:
+            ref = fx.step(case, s)["f1_out"]
+            assert F.st.tnow == ref["TNOW"][0], s
+            for k in ("UU", "VV", "WW", "PP", "FF", "NF", "TBUB") + DROP:
+                assert np.array_equal(F[k], ref[k]), (s, k)
