@@ -21,13 +21,17 @@ module cadmas_hip
   integer(c_int), parameter :: CAD_INIT_VELOCITY = 1
   integer(c_int), parameter :: CAD_INIT_PRESSURE = 2
 
-  ! COMMON /VF_APHYSR/, /VF_ACOMPR/, /VF_ACOMPI/, /VF_ATIMER/ scalars
+  ! COMMON /VF_APHYSR/, /VF_ACOMPR/, /VF_ACOMPI/, /VF_ATIMER/ scalars;
+  ! IDROP / WBUB are OPTION T-DOOR DROP / BUB (vf_iiopt.f:57-77), TNOW the
+  ! time that CAD_STEP advances by DTNOW before each step (vf_a1main.f:1047)
   type, bind(c) :: cad_params
     real(c_double) :: rho0, grz0, anu0, dtnow
     real(c_double) :: scmvp, vvmax, flower, fupper, plower
     real(c_double) :: cgpara, cgepsa, cgepsr, cgdiv
     integer(c_int) :: icgtyp, icgmax, ibsuw0, loops
     integer(c_int) :: precision
+    integer(c_int) :: idrop
+    real(c_double) :: wbub, tnow
   end type cad_params
 
   ! list-file line of one step (vf_a1main.f:954-961)
@@ -60,6 +64,32 @@ module cadmas_hip
       type(c_ptr), value :: ctx
       type(cad_params), intent(in) :: prm
       integer(c_int) :: cad_set_params
+    end function
+
+    function cad_get_params(ctx, prm) bind(c, name='cad_get_params')
+      import :: c_int, c_ptr, cad_params
+      type(c_ptr), value :: ctx
+      type(cad_params), intent(out) :: prm
+      integer(c_int) :: cad_get_params
+    end function
+
+    ! TimerDoor state: TBUB(NUMK), DROPTX..DROPWW(NUMI,NUMJ,NUMK) (vf_f1cal.f:2-3)
+    function cad_upload_tdoor(ctx, tbub, droptx, dropty, droptz, dropuu, dropvv, dropww) &
+        bind(c, name='cad_upload_tdoor')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tbub(*), droptx(*), dropty(*), droptz(*), dropuu(*), &
+                                    dropvv(*), dropww(*)
+      integer(c_int) :: cad_upload_tdoor
+    end function
+
+    function cad_download_tdoor(ctx, tbub, droptx, dropty, droptz, dropuu, dropvv, dropww) &
+        bind(c, name='cad_download_tdoor')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: tbub(*), droptx(*), dropty(*), droptz(*), dropuu(*), &
+                                     dropvv(*), dropww(*)
+      integer(c_int) :: cad_download_tdoor
     end function
 
     function cad_upload_static(ctx, xx, yy, zz, anu, cd0, ggv, ggx, ggy, ggz, glv, glx, gly, glz, &

@@ -38,14 +38,20 @@ extern "C" {
 typedef struct cad_ctx cad_ctx;
 
 /* Scalars the reference keeps in COMMON /VF_APHYSR/ (RHO0, GRZ0, ANU0, VVMAX),
- * /VF_ACOMPR/ (SCMVP, FLOWER, FUPPER, PLOWER, CGPARA, CGEPSA, CGEPSR, CGDIV),
- * /VF_ACOMPI/ (ICGTYP, ICGMAX, IBSUW0), /VF_ATIMER/ (DTNOW), /VF_ATIMEI/ (LOOPS). */
+ * /VF_ACOMPR/ (SCMVP, FLOWER, FUPPER, PLOWER, CGPARA, CGEPSA, CGEPSR, CGDIV,
+ * WBUB), /VF_ACOMPI/ (ICGTYP, ICGMAX, IBSUW0, IDROP), /VF_ATIMER/ (DTNOW,
+ * TNOW), /VF_ATIMEI/ (LOOPS).  cad_create fills the defaults of
+ * vf_a2dflt.f, which include both TimerDoor models on (IDROP=1 at :151,
+ * WBUB=0.2 at :205). */
 typedef struct cad_params {
     double rho0, grz0, anu0, dtnow;
     double scmvp, vvmax, flower, fupper, plower;
     double cgpara, cgepsa, cgepsr, cgdiv;
     int icgtyp, icgmax, ibsuw0, loops;
     int precision;            /* CAD_PRECISION_* */
+    int idrop;                /* OPTION T-DOOR DROP: 1 FREE-RUNDOWN, 0 OFF (vf_iiopt.f:68-75) */
+    double wbub;              /* OPTION T-DOOR BUB rise speed [m/s]; < 1e-20 is OFF (vf_f1cal.f:208) */
+    double tnow;              /* TNOW; cad_step advances it by DTNOW first (vf_a1main.f:1047) */
 } cad_params;
 
 /* Per-step outputs of the list-file line (vf_a1main.f:954-961). */
@@ -62,6 +68,8 @@ const char *cad_last_error(void);
 int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out);
 void cad_destroy(cad_ctx *ctx);
 int cad_set_params(cad_ctx *ctx, const cad_params *prm);
+/* The context's current scalars (TNOW included). */
+int cad_get_params(cad_ctx *ctx, cad_params *prm);
 
 /* Set-up arrays built by the reference's VF_CSETUP/VF_CGLV/VF_CGLXYZ/VF_CINDX/
  * VF_CINDB (vf_a1main.f:784) -- uploaded once. */
@@ -95,12 +103,25 @@ int cad_download_state(cad_ctx *ctx, double *uu, double *vv, double *ww, double 
 int cad_init_state(cad_ctx *ctx, double wvlvl, double uini, double vini, double wini, int flags,
                    cad_step_info *info);
 
+/* TimerDoor state carried between steps, the arguments TBUB(NUMK) and
+ * DROPTX, DROPTY, DROPTZ, DROPUU, DROPVV, DROPWW (NUMI,NUMJ,NUMK) of VF_F1CAL
+ * (vf_f1cal.f:2-3, 59-65): the bubble-level timers and the droplet times and
+ * velocities of VF_FBUBUP / VF_FDROPF.  cad_init_state sets their VF_CINIT
+ * values (vf_cinit.f:412-421); upload them with the rest of a restart. */
+int cad_upload_tdoor(cad_ctx *ctx, const double *tbub, const double *droptx, const double *dropty,
+                     const double *droptz, const double *dropuu, const double *dropvv,
+                     const double *dropww);
+int cad_download_tdoor(cad_ctx *ctx, double *tbub, double *droptx, double *dropty, double *droptz,
+                       double *dropuu, double *dropvv, double *dropww);
+
 /* Whole phases ------------------------------------------------------------ */
 /* VF_V1CAL (vf_v1cal.f:1-297): predictor, Poisson, correction, boundary values. */
 int cad_v1cal(cad_ctx *ctx, cad_step_info *info);
-/* VF_F1CAL (vf_f1cal.f:1-239): VOF advection, redistribution, flags, BCs. */
+/* VF_F1CAL (vf_f1cal.f:1-239): VOF advection, redistribution, flags with the
+ * TimerDoor bubble / droplet corrections (at the context's TNOW), BCs. */
 int cad_f1cal(cad_ctx *ctx, cad_step_info *info);
-/* One SMAC step in the order of vf_a1main.f:1058-1104 (V1CAL then F1CAL). */
+/* One pass of the time loop body (vf_a1main.f:1047-1104): TNOW += DTNOW,
+ * then V1CAL and F1CAL. */
 int cad_step(cad_ctx *ctx, cad_step_info *info);
 /* VF_CDIV00 (vf_cdiv00.f:1-56): || sum of face fluxes ||_2 over active cells. */
 int cad_cdiv00(cad_ctx *ctx, double *dv);

@@ -37,7 +37,11 @@ class Params(C.Structure):
                 ("flower", C.c_double), ("fupper", C.c_double), ("plower", C.c_double),
                 ("cgpara", C.c_double), ("cgepsa", C.c_double), ("cgepsr", C.c_double),
                 ("cgdiv", C.c_double), ("icgtyp", C.c_int), ("icgmax", C.c_int),
-                ("ibsuw0", C.c_int), ("loops", C.c_int), ("precision", C.c_int)]
+                ("ibsuw0", C.c_int), ("loops", C.c_int), ("precision", C.c_int),
+                ("idrop", C.c_int), ("wbub", C.c_double), ("tnow", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class StepInfo(C.Structure):
@@ -53,7 +57,7 @@ class StepInfo(C.Structure):
 DEFAULTS = dict(rho0=1.0e3, grz0=9.8, anu0=1.0e-6, dtnow=1.0e-3, scmvp=1.0, vvmax=30.0,
                 flower=1.0e-6, fupper=1.0 - 1.0e-6, plower=1.0e-4, cgpara=0.95, cgepsa=1.0e-15,
                 cgepsr=1.0e-12, cgdiv=1.0e30, icgtyp=1, icgmax=500, ibsuw0=0, loops=1,
-                precision=PRECISION_FP64)
+                precision=PRECISION_FP64, idrop=1, wbub=0.2, tnow=0.0)
 
 _lib = None
 
@@ -71,6 +75,9 @@ def lib():
     L.cad_destroy.argtypes = [C.c_void_p]
     L.cad_destroy.restype = None
     L.cad_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+    L.cad_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+    L.cad_upload_tdoor.argtypes = [C.c_void_p] + [PD] * 7
+    L.cad_download_tdoor.argtypes = [C.c_void_p] + [PD] * 7
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
     L.cad_upload_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
     L.cad_download_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
@@ -98,7 +105,8 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
            "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
-           "cad_milu_factor", "cad_milu_apply")
+           "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
+           "cad_download_tdoor")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -140,6 +148,7 @@ def _pi(a):
 MATRIX = ("AD", "ALI", "ALJ", "ALK", "AUI", "AUJ", "AUK", "BB")
 STATE_F = ("UU", "VV", "WW", "PP", "FF")
 STATE_B = ("BCU", "BCV", "BCW", "BCP", "BCF")
+DROP = ("DROPTX", "DROPTY", "DROPTZ", "DROPUU", "DROPVV", "DROPWW")
 
 
 class Context:
@@ -172,7 +181,9 @@ class Context:
 
     # ---------------------------------------------------------------- set-up
     def set_params(self, **kw):
-        p = dict(self.params)
+        """Change scalars; the others keep the context's current values
+        (TNOW as cad_step left it)."""
+        p = self.get_params()
         for k, v in kw.items():
             if k not in p:
                 raise KeyError(k)
@@ -180,6 +191,23 @@ class Context:
         st = Params(**p)
         _check(lib().cad_set_params(self._h, C.byref(st)))
         self.params = p
+
+    def get_params(self):
+        st = Params()
+        _check(lib().cad_get_params(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def upload_tdoor(self, a):
+        """TimerDoor state: TBUB (NUMK) and DROPTX..DROPWW fields."""
+        tb = _f64(a["TBUB"], (self.shape[2],))
+        fl = [_f64(a[k], self.shape) for k in DROP]
+        _check(lib().cad_upload_tdoor(self._h, _pd(tb), *[_pd(x) for x in fl]))
+
+    def download_tdoor(self):
+        out = {"TBUB": np.zeros(self.shape[2])}
+        out.update({k: np.zeros(self.shape, order="F") for k in DROP})
+        _check(lib().cad_download_tdoor(self._h, _pd(out["TBUB"]), *[_pd(out[k]) for k in DROP]))
+        return out
 
     def upload_static(self, a):
         s, nb = self.shape, self.numb

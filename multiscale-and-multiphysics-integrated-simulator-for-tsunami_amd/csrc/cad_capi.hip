@@ -176,6 +176,7 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
         cad_destroy(c);
         return rc;
     }
+    c->zz_host = new double[6 * (size_t)numk]();
     if (hipHostMalloc((void **)&c->host_red, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->host_cnt, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         cad_set_error("hipHostMalloc failed");
@@ -194,6 +195,7 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
     p.scmvp = 1.0; p.vvmax = 30.0; p.flower = 1.0e-6; p.fupper = 1.0 - 1.0e-6; p.plower = 1.0e-4;
     p.cgpara = 0.95; p.cgepsa = 1.0e-15; p.cgepsr = 1.0e-12; p.cgdiv = 1.0e30;
     p.icgtyp = 1; p.icgmax = 500; p.ibsuw0 = 0; p.loops = 1; p.precision = CAD_PRECISION_FP64;
+    p.idrop = 1; p.wbub = 0.2; p.tnow = 0.0;
     c->prm = p;
     *out = c;
     return CAD_OK;
@@ -214,6 +216,8 @@ void cad_destroy(cad_ctx *c)
     for (void *p : ptrs)
         if (p) hipFree(p);
     cad_sweep_free(c);
+    cad_tdoor_free(c);
+    delete[] c->zz_host;
     if (c->host_red) hipHostFree(c->host_red);
     if (c->host_cnt) hipHostFree(c->host_cnt);
     for (auto &e : c->ev)
@@ -240,7 +244,62 @@ int cad_set_params(cad_ctx *c, const cad_params *p)
         cad_set_error("cad_set_params: DT and RHO0 must be positive");
         return CAD_ERR_ARG;
     }
+    if (p->wbub < 0.0 || p->idrop < 0 || p->idrop > 1) {
+        cad_set_error("cad_set_params: WBUB %g / IDROP %d invalid (vf_iiopt.f:60-75)", p->wbub, p->idrop);
+        return CAD_ERR_ARG;
+    }
     c->prm = *p;
+    if (p->wbub >= 1.0e-20 || p->idrop >= 1) {
+        CAD_CHECK(hipSetDevice(c->device));
+        RET_IF(cad_tdoor_alloc(c));
+    }
+    return CAD_OK;
+}
+
+int cad_get_params(cad_ctx *c, cad_params *p)
+{
+    if (!c || !p) {
+        cad_set_error("cad_get_params: null argument");
+        return CAD_ERR_ARG;
+    }
+    *p = c->prm;
+    return CAD_OK;
+}
+
+int cad_upload_tdoor(cad_ctx *c, const double *tbub, const double *droptx, const double *dropty,
+                     const double *droptz, const double *dropuu, const double *dropvv,
+                     const double *dropww)
+{
+    if (!c || !tbub || !droptx || !dropty || !droptz || !dropuu || !dropvv || !dropww) {
+        cad_set_error("cad_upload_tdoor: null argument");
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipSetDevice(c->device));
+    RET_IF(cad_tdoor_alloc(c));
+    const size_t nc = (size_t)c->d.nc;
+    const double *src[6] = {droptx, dropty, droptz, dropuu, dropvv, dropww};
+    for (int a = 0; a < 6; a++) RET_IF(h2d(c, c->drop[a], src[a], nc));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->d.nk; k++) c->tbub[k] = tbub[k];
+    // the next droplet pass clears every cell without a droplet itself
+    c->td_full = true;
+    return CAD_OK;
+}
+
+int cad_download_tdoor(cad_ctx *c, double *tbub, double *droptx, double *dropty, double *droptz,
+                       double *dropuu, double *dropvv, double *dropww)
+{
+    if (!c || !tbub || !droptx || !dropty || !droptz || !dropuu || !dropvv || !dropww) {
+        cad_set_error("cad_download_tdoor: null argument");
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipSetDevice(c->device));
+    RET_IF(cad_tdoor_alloc(c));
+    const size_t nc = (size_t)c->d.nc;
+    double *dst[6] = {droptx, dropty, droptz, dropuu, dropvv, dropww};
+    for (int a = 0; a < 6; a++) RET_IF(d2h(c, dst[a], c->drop[a], nc));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->d.nk; k++) tbub[k] = c->tbub[k];
     return CAD_OK;
 }
 
@@ -286,6 +345,7 @@ int cad_upload_static(cad_ctx *c, const double *xx, const double *yy, const doub
     RET_IF(h2d(c, c->xx, xx, 6 * (size_t)c->d.ni));
     RET_IF(h2d(c, c->yy, yy, 6 * (size_t)c->d.nj));
     RET_IF(h2d(c, c->zz, zz, 6 * (size_t)c->d.nk));
+    std::memcpy(c->zz_host, zz, 6 * (size_t)c->d.nk * sizeof(double));
     RET_IF(h2d(c, c->anu, anu, nc));
     RET_IF(h2d(c, c->cd0, cd0, nc));
     RET_IF(h2d(c, c->ggv, ggv, nc));
@@ -513,6 +573,9 @@ int cad_step(cad_ctx *c, cad_step_info *info)
 {
     cad_step_info loc;
     std::memset(&loc, 0, sizeof loc);
+    RET_IF(ready(c));
+    // TNOW = TNOW + DTNOW before VF_V1CAL (vf_a1main.f:1047)
+    c->prm.tnow = c->prm.tnow + c->prm.dtnow;
     int rc = cad_v1cal(c, &loc);
     if (rc == CAD_OK) rc = cad_f1cal(c, &loc);
     if (info) *info = loc;
@@ -526,6 +589,8 @@ int cad_init_state(cad_ctx *c, double wvlvl, double uini, double vini, double wi
     const double uvw[3] = {uini, vini, wini};
     double fsum = 0.0, fcut = 0.0;
     RET_IF(cad_init_phase(c, wvlvl, uvw, flags, &fsum, &fcut));
+    // TimerDoor state (vf_cinit.f:412-421)
+    if (c->td_alloc) RET_IF(cad_tdoor_init(c));
     if (info) {
         std::memset(info, 0, sizeof *info);
         info->fsum = fsum;

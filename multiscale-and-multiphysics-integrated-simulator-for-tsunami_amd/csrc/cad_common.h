@@ -142,6 +142,15 @@ struct cad_ctx {
     bool have_static, have_state;
     // the surface-cell count of the last flagging pass
     int nums;
+    // TimerDoor models (cad_tdoor.hip), allocated on first use: droplet
+    // fields DROPTX..DROPWW, per-line droplet flags, the level mask of the
+    // bubble timers, counters and their pinned mirror; TBUB(NUMK) and the
+    // ZZ table on the host
+    double *drop[6];
+    int *td_flag, *td_cnt, *td_host;
+    unsigned char *td_lvl;
+    double *tbub, *zz_host;
+    bool td_alloc, td_full;
 };
 
 // ---- launchers implemented in the phase files ---------------------------
@@ -167,3 +176,8 @@ int cad_sweep_status(cad_ctx *c, int *code);
 int cad_milu_factor_impl(cad_ctx *c, int *npos);
 int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x);
 Static cad_static(const cad_ctx *c);
+int cad_fnfini(cad_ctx *c);
+int cad_tdoor_alloc(cad_ctx *c);
+void cad_tdoor_free(cad_ctx *c);
+int cad_tdoor_init(cad_ctx *c);
+int cad_tdoor_phase(cad_ctx *c);

@@ -538,8 +538,13 @@ __global__ void __launch_bounds__(256) k_nf_orient(Dims d, Static s, const doubl
     }
     // orientation written after all candidates: the tests read only ==0 / ==8
     nf[n] = code;
-    if (code == 7) atomicAdd(cnt + 1, 1);
-    else atomicAdd(cnt, 1);
+    // counts aggregated per wavefront: one atomic per wave, not per cell
+    const unsigned long long all = __ballot(1), bad = __ballot(code == 7);
+    if ((int)__lane_id() == __builtin_ctzll(all)) {
+        const int nb = __popcll(bad), ng = __popcll(all) - nb;
+        if (ng) atomicAdd(cnt, ng);
+        if (nb) atomicAdd(cnt + 1, nb);
+    }
 }
 
 // vf_cindc.f:35-50
@@ -658,8 +663,8 @@ inline dim3 sgrid(long long nc)
         if (rc_) return rc_;   \
     } while (0)
 
-// VF_FNFINI + VF_FNFPRV + VF_CINDC on the device; returns the surface count.
-static int flag_cells(cad_ctx *c, int *nums)
+// VF_FNFINI (vf_fnfini.f:36-197) on the device: fluid / gas / surface flags.
+int cad_fnfini(cad_ctx *c)
 {
     Static s = cad_static(c);
     const Dims &d = c->d;
@@ -679,6 +684,18 @@ static int flag_cells(cad_ctx *c, int *nums)
         if (c->host_cnt[2] <= 0) break;
         hipLaunchKernelGGL(k_nf_fix, g, b, 0, st, d, f.nf);
     }
+    return CAD_OK;
+}
+
+// VF_FNFPRV + VF_CINDC (vf_fnfprv.f:34-223, vf_cindc.f:35-50); returns the
+// surface count.
+static int orient_cells(cad_ctx *c, int *nums)
+{
+    Static s = cad_static(c);
+    const Dims &d = c->d;
+    Fields &f = c->f;
+    dim3 g = cell_grid(d), b = cell_block();
+    hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 3, 0);
     hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 4, 0);
     hipLaunchKernelGGL(k_nf_orient, g, b, 0, st, d, s, f.ff, f.bcf, f.nf, c->icount + 3);
@@ -692,6 +709,13 @@ static int flag_cells(cad_ctx *c, int *nums)
     }
     *nums = c->host_cnt[3];
     return CAD_OK;
+}
+
+// VF_FNFINI + VF_FNFPRV + VF_CINDC
+static int flag_cells(cad_ctx *c, int *nums)
+{
+    RET_IF(cad_fnfini(c));
+    return orient_cells(c, nums);
 }
 
 // VF_F1CAL, vf_f1cal.f:124-239
@@ -733,8 +757,12 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
     // NLIM = old NF (vf_f1cal.f:197-203)
     hipLaunchKernelGGL(k_icopy, sgrid(d.nc), dim3(256), 0, st, d.nc, f.nf, f.nlim);
     CAD_CHECK(hipGetLastError());
+    // VF_FNFINI, the TimerDoor corrections each followed by VF_FNFINI when
+    // they moved water, then VF_FNFPRV and VF_CINDC (vf_f1cal.f:206-219)
     int nums = 0;
-    RET_IF(flag_cells(c, &nums));
+    RET_IF(cad_fnfini(c));
+    RET_IF(cad_tdoor_phase(c));
+    RET_IF(orient_cells(c, &nums));
     c->nums = nums;
     // VF_BSPPFL with the old flags in NLIM (vf_f1cal.f:230)
     hipLaunchKernelGGL(k_dcopy, sgrid(d.nc), dim3(256), 0, st, d.nc, f.pp, c->yv);

@@ -47,3 +47,37 @@ def test_fortran_module_binds_exported_symbols(pkg):
             r = subprocess.run([flang, "-fsyntax-only", "-module-dir", d, str(src)],
                                capture_output=True, text=True)
             assert r.returncode == 0, r.stderr
+
+
+def test_params_struct_layout_agrees(pkg):
+    """cad_params has one layout in C (the header), in the ctypes mirror and
+    in the Fortran bind(c) type: same size and the same offsets of the fields
+    appended last (IDROP, WBUB, TNOW)."""
+    import shutil
+    import subprocess
+    import tempfile
+    P = pkg.capi.Params
+    fields = ("precision", "idrop", "wbub", "tnow")
+    want = [ctypes.sizeof(P)] + [getattr(P, f).offset for f in fields]
+    with tempfile.TemporaryDirectory() as d:
+        csrc = Path(d) / "p.c"
+        csrc.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "cadmas_hip.h"\n'
+                        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(cad_params),"
+                        + ",".join(f"offsetof(cad_params,{f})" for f in fields) + ");return 0;}\n")
+        exe = Path(d) / "p"
+        r = subprocess.run(["gcc", "-I", str(ROOT / "include"), str(csrc), "-o", str(exe)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+        assert got == want, (got, want)
+        flang = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+        if Path(flang).exists():
+            fsrc = Path(d) / "p.f90"
+            fsrc.write_text("program p\n  use cadmas_hip\n  type(cad_params) :: q\n"
+                            "  print '(i0)', c_sizeof(q)\nend program p\n")
+            fexe = Path(d) / "pf"
+            r = subprocess.run([flang, "-module-dir", d, str(ROOT / "include" / "cadmas_hip.f90"),
+                                str(fsrc), "-o", str(fexe)], capture_output=True, text=True, cwd=d)
+            assert r.returncode == 0, r.stderr
+            out = subprocess.run([str(fexe)], capture_output=True, text=True).stdout.split()
+            assert int(out[0]) == want[0]

@@ -32,7 +32,7 @@ def _params(st, rec):
 
 def _capi_params(p):
     keys = ("rho0", "grz0", "anu0", "dtnow", "scmvp", "flower", "fupper", "plower", "cgpara",
-            "cgepsa", "cgepsr", "cgdiv", "icgtyp", "icgmax")
+            "cgepsa", "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
     return {k: p[k] for k in keys if k in p}
 
 

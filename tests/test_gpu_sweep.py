@@ -27,7 +27,7 @@ def _params(st, rec):
     p.update(fx.solver_params(st))
     p["dtnow"] = float(rec["DTNOW"][0])
     keys = ("rho0", "grz0", "anu0", "dtnow", "scmvp", "flower", "fupper", "plower", "cgpara",
-            "cgepsa", "cgepsr", "cgdiv", "icgtyp", "icgmax")
+            "cgepsa", "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
     return {k: p[k] for k in keys if k in p}
 
 
