@@ -152,6 +152,16 @@ module cadmas_hip
       integer(c_int) :: cad_step
     end function
 
+    ! VF_CDTCAL: the next step size (TIME AUTO DTINIT DTSAFE, TIME LIMIT DTMIN DTMAX)
+    function cad_cdtcal(ctx, nnow, dtinit, dtsafe, dtmin, dtmax, dt) bind(c, name='cad_cdtcal')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nnow
+      real(c_double), value :: dtinit, dtsafe, dtmin, dtmax
+      real(c_double), intent(out) :: dt
+      integer(c_int) :: cad_cdtcal
+    end function
+
     function cad_cdiv00(ctx, dv) bind(c, name='cad_cdiv00')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -120,9 +120,18 @@ int cad_v1cal(cad_ctx *ctx, cad_step_info *info);
 /* VF_F1CAL (vf_f1cal.f:1-239): VOF advection, redistribution, flags with the
  * TimerDoor bubble / droplet corrections (at the context's TNOW), BCs. */
 int cad_f1cal(cad_ctx *ctx, cad_step_info *info);
-/* One pass of the time loop body (vf_a1main.f:1047-1104): TNOW += DTNOW,
- * then V1CAL and F1CAL. */
+/* One pass of the time loop body (vf_a1main.f:1029-1104) with DT = DTNOW:
+ * LOOPS passes of TNOW += DT/LOOPS and V1CAL at DTNOW = DT/LOOPS (OPTION
+ * SUB-LOOP; passes 2.. first refresh the boundary values as
+ * vf_v1cal.f:115-118 does), then DTNOW = DT and F1CAL. */
 int cad_step(cad_ctx *ctx, cad_step_info *info);
+/* VF_CDTCAL (vf_cdtcal.f, called at vf_a1main.f:940): the next step size
+ * from the resident fields for TIME AUTO dtinit dtsafe / TIME LIMIT dtmin
+ * dtmax, with W = DTNOW (dtinit when nnow = 0).  The cell limits are
+ * combined by an exact minimum; the reference folds them in serially, which
+ * can differ by the rounding of V/(V/DT) per cell (a few ulps). */
+int cad_cdtcal(cad_ctx *ctx, int nnow, double dtinit, double dtsafe, double dtmin, double dtmax,
+               double *dt);
 /* VF_CDIV00 (vf_cdiv00.f:1-56): || sum of face fluxes ||_2 over active cells. */
 int cad_cdiv00(cad_ctx *ctx, double *dv);
 

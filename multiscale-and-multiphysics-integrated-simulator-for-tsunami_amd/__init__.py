@@ -9,3 +9,4 @@ from . import capi  # noqa: F401
 from . import configs  # noqa: F401
 from . import deck  # noqa: F401
 from . import layout  # noqa: F401
+from . import timeloop  # noqa: F401

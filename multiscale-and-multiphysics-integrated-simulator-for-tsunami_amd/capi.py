@@ -77,6 +77,7 @@ def lib():
     L.cad_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
     L.cad_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
     L.cad_upload_tdoor.argtypes = [C.c_void_p] + [PD] * 7
+    L.cad_cdtcal.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, PD]
     L.cad_download_tdoor.argtypes = [C.c_void_p] + [PD] * 7
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
     L.cad_upload_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
@@ -106,7 +107,7 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
-           "cad_download_tdoor")
+           "cad_download_tdoor", "cad_cdtcal")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -255,6 +256,12 @@ class Context:
 
     def step(self):
         return self._info_call("cad_step")
+
+    def cdtcal(self, nnow, dtinit, dtsafe, dtmin, dtmax):
+        """VF_CDTCAL: the next step size from the resident fields."""
+        dt = C.c_double()
+        _check(lib().cad_cdtcal(self._h, int(nnow), dtinit, dtsafe, dtmin, dtmax, C.byref(dt)))
+        return dt.value
 
     def v1cal(self):
         return self._info_call("cad_v1cal")

@@ -49,12 +49,14 @@ class Case:
 
 
 def _common(lines, dt, steps, eps_r=1.0e-10, rho=1000.0, nu=1.0e-6, g=9.81, wlevel=None,
-            tdoor=("OFF", "OFF")):
+            tdoor=("OFF", "OFF"), time=None, loops=1):
     """Time, material and solver lines.  tdoor = (BUB word, DROP word) of
     OPTION T-DOOR (Source code/vf_iiopt.f:57-77); None leaves a model at the
     reference default (bubbles rise at 0.2 m/s, droplets FREE-RUNDOWN)."""
-    lines += [
-        f"TIME CONST {dt:.6e}",
+    # time=None: TIME CONST dt; otherwise the words after TIME of the
+    # automatic step lines, e.g. ["AUTO 1e-4 0.2", "LIMIT 1e-6 2e-3"]
+    tl = [f"TIME CONST {dt:.6e}"] if time is None else [f"TIME {t}" for t in time]
+    lines += tl + [
         f"TIME END {steps} 1.0D9",
         f"MATE DENSITY {rho}",
         f"MATE K-VISC {nu:.6e}",
@@ -65,6 +67,8 @@ def _common(lines, dt, steps, eps_r=1.0e-10, rho=1000.0, nu=1.0e-6, g=9.81, wlev
     for word, val in zip(("BUB", "DROP"), tdoor):
         if val is not None:
             lines.append(f"OPTION T-DOOR {word} {val}")
+    if loops != 1:
+        lines.append(f"OPTION SUB-LOOP {loops}")
     if wlevel is not None:
         lines.append(f"MATE W-LEVEL {wlevel:.6f}")
 
@@ -275,6 +279,30 @@ def tdoor_dam(ni=32, nj=4, nk=24, d=0.02, steps=900, dt=5.0e-4, eps_r=1.0e-10, n
     return Case(name or f"tdoor_dam_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
 
 
+def auto_dam(ni=32, nj=4, nk=24, d=0.02, steps=400, eps_r=1.0e-10, name=None):
+    """Dam break with the automatic time step (TIME AUTO DTINIT DTSAFE and
+    TIME LIMIT DTMIN DTMAX, Source code/vf_iitime.f:37-52): every step size
+    comes from VF_CDTCAL (vf_cdtcal.f), growing from 1e-4 s by at most 1.2x
+    per step up to the flow's limit."""
+    lines = [f"# dam break, automatic time step {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE"]
+    _common(lines, None, steps, eps_r, time=["AUTO 1.0e-4 0.3", "LIMIT 1.0e-6 1.0e-2"])
+    lines.append(f"DEBUG F-BOX 1 1 1 {ni * 3 // 8} {nj} {nk * 5 // 8} 1.0")
+    return Case(name or f"auto_dam_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, None)
+
+
+def subloop(loops=3, steps=40, name=None):
+    """The three-dimensional corner case with OPTION SUB-LOOP: each step runs
+    VF_V1CAL LOOPS times at DT/LOOPS (boundary values refreshed before the
+    second and later passes, vf_v1cal.f:115-118), then VF_F1CAL at DT."""
+    c = corner3d(steps=steps, dt=3.0e-4, name=name or f"subloop{loops}")
+    deck = c.deck.replace("OPTION T-DOOR BUB OFF", f"OPTION SUB-LOOP {loops}\nOPTION T-DOOR BUB OFF")
+    return Case(c.name, deck, c.ni, c.nj, c.nk, steps, c.dt)
+
+
 def named(name):
     """The workloads by name (sizes as in the benchmark table)."""
     table = {
@@ -286,5 +314,7 @@ def named(name):
         "corner3d": lambda: corner3d(),
         "tdoor": lambda: tdoor_drops(),
         "tdoor_dam": lambda: tdoor_dam(),
+        "auto_dam": lambda: auto_dam(),
+        "subloop": lambda: subloop(),
     }
     return table[name]()

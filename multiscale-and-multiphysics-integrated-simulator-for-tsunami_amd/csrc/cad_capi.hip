@@ -39,6 +39,9 @@ Static cad_static(const cad_ctx *c)
 int cad_flag_cells(cad_ctx *c, int *nums);
 int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, double *fsum, double *fcut);
 int cad_time_minv_impl(cad_ctx *c, int reps, float *ms_avg);
+int cad_cdtcal_impl(cad_ctx *c, int nnow, double dtinit, double dtsafe, double dtmin, double dtmax,
+                    double *dt);
+int cad_launch_bsuwn3(cad_ctx *c);
 
 namespace {
 
@@ -232,9 +235,9 @@ int cad_set_params(cad_ctx *c, const cad_params *p)
         cad_set_error("cad_set_params: null argument");
         return CAD_ERR_ARG;
     }
-    if (p->loops != 1) {
-        cad_set_error("OPTION SUB-LOOP %d: only LOOPS=1 is implemented", p->loops);
-        return CAD_ERR_UNSUPPORTED;
+    if (p->loops < 1) {
+        cad_set_error("OPTION SUB-LOOP %d: LOOPS must be >= 1 (vf_iiopt.f:44)", p->loops);
+        return CAD_ERR_ARG;
     }
     if (p->precision != CAD_PRECISION_FP64) {
         cad_set_error("precision mode %d is not available in this build", p->precision);
@@ -574,12 +577,36 @@ int cad_step(cad_ctx *c, cad_step_info *info)
     cad_step_info loc;
     std::memset(&loc, 0, sizeof loc);
     RET_IF(ready(c));
-    // TNOW = TNOW + DTNOW before VF_V1CAL (vf_a1main.f:1047)
-    c->prm.tnow = c->prm.tnow + c->prm.dtnow;
-    int rc = cad_v1cal(c, &loc);
+    // DT = DTNOW: LOOPS passes of TNOW += DT/LOOPS and VF_V1CAL at
+    // DTNOW = DT/LOOPS (passes 2.. first refresh the boundary values,
+    // vf_v1cal.f:115-118), then DTNOW = DT and VF_F1CAL
+    // (vf_a1main.f:1029-1104)
+    const double dt = c->prm.dtnow;
+    c->prm.dtnow = dt / (double)c->prm.loops;
+    int rc = CAD_OK;
+    for (int l = 1; l <= c->prm.loops && rc == CAD_OK; l++) {
+        c->prm.tnow = c->prm.tnow + c->prm.dtnow;
+        if (l >= 2) {
+            rc = cad_launch_bsuwn3(c);
+            if (rc == CAD_OK) rc = cad_launch_bwuwt(c);
+        }
+        if (rc == CAD_OK) rc = cad_v1cal(c, &loc);
+    }
+    c->prm.dtnow = dt;
     if (rc == CAD_OK) rc = cad_f1cal(c, &loc);
     if (info) *info = loc;
     return rc;
+}
+
+int cad_cdtcal(cad_ctx *c, int nnow, double dtinit, double dtsafe, double dtmin, double dtmax,
+               double *dt)
+{
+    RET_IF(ready(c));
+    if (!dt || !(dtsafe > 0.0) || !(dtinit >= 0.0) || !(dtmin >= 0.0) || !(dtmax >= dtmin)) {
+        cad_set_error("cad_cdtcal: invalid TIME AUTO / LIMIT values (vf_iitime.f:37-52)");
+        return CAD_ERR_ARG;
+    }
+    return cad_cdtcal_impl(c, nnow, dtinit, dtsafe, dtmin, dtmax, dt);
 }
 
 int cad_init_state(cad_ctx *c, double wvlvl, double uini, double vini, double wini, int flags,

@@ -781,6 +781,15 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
 
 int cad_flag_cells(cad_ctx *c, int *nums) { return flag_cells(c, nums); }
 
+// VF_BSUWN3 on its own (the boundary refresh of sub-loop passes 2..)
+int cad_launch_bsuwn3(cad_ctx *c)
+{
+    hipLaunchKernelGGL(k_bsuwn3, cell_grid(c->d), cell_block(), 0, c->stream, c->d, cad_static(c), c->f.nf,
+                       c->f.uu, c->f.vv, c->f.ww, c->prm.ibsuw0);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
 // Device part of VF_CINIT (vf_cinit.f:190-409) after F holds its initial
 // value: clip and volume sums, boundary F, flags, INDC, optional initial
 // velocity / pressure, and the boundary values.

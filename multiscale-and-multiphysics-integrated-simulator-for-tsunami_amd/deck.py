@@ -51,8 +51,14 @@ class Deck:
     porous: List[tuple] = field(default_factory=list)     # (kind, box, values)
     plower: float = 1.0e-4
     bcs: List[BoundaryLine] = field(default_factory=list)
-    # TIME (vf_iitime.f); the default AUTO step is not implemented
+    # TIME (vf_iitime.f, defaults vf_a2dflt.f:223-234): IDTTYP 0 = CONST,
+    # 1 = AUTO (the default) with DTINIT, DTSAFE and LIMIT DTMIN, DTMAX
     dt_const: Optional[float] = None
+    idttyp: int = 1
+    dtinit: float = 1.0e-6
+    dtsafe: float = 1.0e-1
+    dtmin: float = 1.0e-20
+    dtmax: float = 1.0e20
     nend: int = 0
     tend: float = 0.0
     # MATE (vf_iimate.f, defaults vf_a2dflt.f:161-164)
@@ -146,13 +152,7 @@ def parse(text):
         elif head == "B.C.":
             d.bcs.append(_parse_bc(w))
         elif head == "TIME":
-            if w[1] == "CONST":
-                d.dt_const = _num(w[2])
-            elif w[1] == "END":
-                d.nend = _int(w[2])
-                d.tend = _num(w[3]) if len(w) > 3 else 0.0
-            else:
-                raise UnsupportedDeck(f"TIME {w[1]}: only a constant time step is implemented")
+            _parse_time(d, w)
         elif head == "MATE":
             key = w[1]
             if key == "W-LEVEL":
@@ -266,6 +266,39 @@ def _parse_comp(d, w):
         raise DeckError(f"COMP {w[1]}")
 
 
+def _parse_time(d, w):
+    """TIME CONST / AUTO / LIMIT / END, Source code/vf_iitime.f:28-60."""
+    if len(w) < 2:
+        raise DeckError("TIME: SYNTAX ERROR.")
+    if w[1] == "CONST":
+        if len(w) < 3:
+            raise DeckError("TIME CONST: SYNTAX ERROR.")
+        d.dt_const = _num(w[2])
+        if d.dt_const < 1.0e-20:
+            raise DeckError("TIME CONST: INVALID VALUE.")
+        d.idttyp = 0
+    elif w[1] == "AUTO":
+        if len(w) < 4:
+            raise DeckError("TIME AUTO: SYNTAX ERROR.")
+        d.dtinit, d.dtsafe = _num(w[2]), _num(w[3])
+        if d.dtinit < 1.0e-20 or d.dtsafe < 1.0e-20:
+            raise DeckError("TIME AUTO: INVALID VALUE.")
+        d.idttyp = 1
+    elif w[1] == "LIMIT":
+        if len(w) < 4:
+            raise DeckError("TIME LIMIT: SYNTAX ERROR.")
+        d.dtmin, d.dtmax = _num(w[2]), _num(w[3])
+        if d.dtmin < 1.0e-20 or d.dtmin > d.dtmax:
+            raise DeckError("TIME LIMIT: INVALID VALUE.")
+    elif w[1] == "END":
+        if len(w) < 4:
+            raise DeckError("TIME END: SYNTAX.")
+        d.nend = _int(w[2])
+        d.tend = _num(w[3])
+    else:
+        raise DeckError(f"TIME {w[1]}: UNKNOWN WORD.")
+
+
 def _parse_option(d, w):
     key = w[1]
     if key == "T-DOOR":
@@ -296,7 +329,5 @@ def _parse_option(d, w):
 
 
 def _validate(d):
-    if d.dt_const is None:
-        raise UnsupportedDeck("TIME CONST is required (the automatic time step is not implemented)")
-    if d.loops != 1:
-        raise UnsupportedDeck("OPTION SUB-LOOP > 1 is not implemented")
+    if d.loops < 1:
+        raise DeckError("OPTION SUB-LOOP: INVALID VALUE.")

@@ -263,7 +263,9 @@ class Layout:
 
     def params(self, dt=None):
         d = self.deck
-        return dict(rho0=d.rho0, grz0=d.grz0, anu0=d.anu0, dtnow=d.dt_const if dt is None else dt,
+        if dt is None:
+            dt = d.dt_const if d.idttyp == 0 else d.dtinit
+        return dict(rho0=d.rho0, grz0=d.grz0, anu0=d.anu0, dtnow=dt,
                     scmvp=d.scmvp, vvmax=d.vvmax, flower=1.0e-6, fupper=1.0 - 1.0e-6,
                     plower=d.plower, cgpara=d.cgpara, cgepsa=d.cgepsa, cgepsr=d.cgepsr,
                     cgdiv=1.0e30, icgtyp=d.icgtyp, icgmax=d.icgmax, ibsuw0=d.ibsuw0,

@@ -898,10 +898,19 @@ int orc_post_bc(orc_state *s)
 }
 
 /* VF_V1CAL (vf_v1cal.f:111-297) for ILOOP=1, static porosity. */
-int orc_v1cal(orc_state *s, double *pt)
+static void bsuwn3(orc_state *s);
+
+/* VF_V1CAL pass iloop of a step; passes 2.. of OPTION SUB-LOOP first refresh
+ * the boundary values (vf_v1cal.f:115-118). */
+int orc_v1cal_loop(orc_state *s, double *pt, int iloop)
 {
     DECL_DIMS(s);
     size_t nc = nij * (size_t)nk;
+    if (iloop >= 2) {
+        bsuwn3(s);
+        int rc0 = bwuwt(s);
+        if (rc0) return rc0;
+    }
     int rc = orc_predictor(s);
     if (rc) return rc;
     double *w = (double *)calloc(8 * nc, sizeof(double));
@@ -921,6 +930,8 @@ int orc_v1cal(orc_state *s, double *pt)
     if (rc) return rc;
     return orc_post_bc(s);
 }
+
+int orc_v1cal(orc_state *s, double *pt) { return orc_v1cal_loop(s, pt, 1); }
 
 /* ======================================================================
  * Phase (c): VOF
@@ -1787,12 +1798,60 @@ int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double 
     return 0;
 }
 
-/* one pass of the time loop body: TNOW += DTNOW (vf_a1main.f:1047), then
- * VF_V1CAL and VF_F1CAL */
+/* One pass of the time loop body with DT = DTNOW (vf_a1main.f:1029-1104):
+ * LOOPS passes of TNOW += DT/LOOPS and VF_V1CAL at DTNOW = DT/LOOPS, then
+ * DTNOW = DT and VF_F1CAL. */
 int orc_step(orc_state *s, double *pt)
 {
-    s->tnow = s->tnow + s->dtnow;
-    int rc = orc_v1cal(s, pt);
-    if (rc) return rc;
+    const double dt = s->dtnow;
+    s->dtnow = dt / (double)s->loops;
+    for (int l = 1; l <= s->loops; l++) {
+        s->tnow = s->tnow + s->dtnow;
+        int rc = orc_v1cal_loop(s, pt, l);
+        if (rc) return rc;
+    }
+    s->dtnow = dt;
     return orc_f1cal(s);
+}
+
+/* Step size of the next step (VF_CDTCAL, vf_cdtcal.f, one rank so that
+ * VF_P0MIND is the identity).  The limit is folded cell by cell into DT in
+ * the reference's loop order; each cell's flux limit V/max|face flux| and
+ * diffusion limit (0.5/sum 1/dx^2)/nu enter through DT = V/MAX(..., V/DT).
+ * nnow = 0 takes DTINIT as the previous step. */
+double orc_cdtcal(const orc_state *s, int nnow, double dtinit, double dtsafe, double dtmin,
+                  double dtmax)
+{
+    DECL_DIMS(s);
+    double dt = dtmax / dtsafe;
+    for (int k = 2; k <= nk - 1; k++)
+        for (int j = 2; j <= nj - 1; j++)
+            for (int i = 2; i <= ni - 1; i++) {
+                const size_t n = ID(i, j, k);
+                if (s->nf[n] == -1) continue;
+                const double dx = XX(2, i), dy = YY(2, j), dz = ZZ(2, k);
+                const double dxy = dx * dy, dxz = dx * dz, dyz = dy * dz;
+                const double fc = s->ff[n];
+                double v = s->ggv[n] * dx * dy * dz;
+                double fl[6];
+                const size_t face[6] = {n, n + 1, n, n + ni, n, n + nij};
+                const double *g[3] = {s->ggx, s->ggy, s->ggz};
+                const double *q[3] = {s->uu, s->vv, s->ww};
+                const int *ind[3] = {s->indx, s->indy, s->indz};
+                const double ar[3] = {dyz, dxz, dxy};
+                for (int e = 0; e < 6; e++) {
+                    const int a = e / 2;
+                    const size_t f = face[e];
+                    fl[e] = fabs(ar[a] * g[a][f] * q[a][f]);
+                    const int b = ind[a][f];
+                    if (b >= 1 && dmax(fc, s->bcf[b - 1]) <= 0.0) fl[e] = 0.0;
+                }
+                double m = fl[0];
+                for (int e = 1; e < 6; e++) m = dmax(m, fl[e]);
+                dt = v / dmax(m, v / dt);
+                v = 0.5 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz));
+                dt = v / dmax(s->anu[n], v / dt);
+            }
+    const double w = nnow == 0 ? dtinit : s->dtnow;
+    return dmax(dmin(dt * dtsafe, 1.20 * w), dtmin);
 }

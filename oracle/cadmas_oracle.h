@@ -92,8 +92,13 @@ int orc_fdropf(orc_state *s);
 double orc_cdiv00(const orc_state *s);
 /* the part of VF_CINIT that follows the initial F (clip, flags, p, BCs) */
 int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double wini, int flags);
-/* one SMAC step: V1CAL then F1CAL */
+/* one SMAC step at DT = dtnow: LOOPS passes of V1CAL at DT/LOOPS, F1CAL at DT */
 int orc_step(orc_state *s, double *pt);
+/* VF_V1CAL pass iloop of OPTION SUB-LOOP (iloop >= 2 refreshes boundary values) */
+int orc_v1cal_loop(orc_state *s, double *pt, int iloop);
+/* VF_CDTCAL: the next step size (TIME AUTO / LIMIT values as arguments) */
+double orc_cdtcal(const orc_state *s, int nnow, double dtinit, double dtsafe, double dtmin,
+                  double dtmax);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,10 @@ def lib():
         L.orc_cdiv00.argtypes = [C.POINTER(OrcState)]
         L.orc_cdiv00.restype = C.c_double
         L.orc_step.argtypes = [C.POINTER(OrcState), PD]
+        L.orc_v1cal_loop.argtypes = [C.POINTER(OrcState), PD, C.c_int]
+        L.orc_cdtcal.argtypes = [C.POINTER(OrcState), C.c_int, C.c_double, C.c_double, C.c_double,
+                                 C.c_double]
+        L.orc_cdtcal.restype = C.c_double
         L.orc_init_state.argtypes = [C.POINTER(OrcState), C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_int]
         _LIB = L
@@ -181,6 +185,14 @@ class Fields:
 
     def fdropf(self):
         return lib().orc_fdropf(C.byref(self.st))
+
+    def v1cal_loop(self, iloop):
+        pt = np.zeros(self.shape, order="F")
+        rc = lib().orc_v1cal_loop(C.byref(self.st), _pd(pt), iloop)
+        return rc, pt
+
+    def cdtcal(self, nnow, dtinit, dtsafe, dtmin, dtmax):
+        return lib().orc_cdtcal(C.byref(self.st), nnow, dtinit, dtsafe, dtmin, dtmax)
 
     def step(self):
         pt = np.zeros(self.shape, order="F")

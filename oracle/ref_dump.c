@@ -40,6 +40,7 @@ extern struct { I4 nend, nnow, idttyp, loops; } vf_atimei_;
 
 /* ------------------------------------------------------------------ */
 static FILE *g_fp = NULL;
+static int g_iloop = 1;   /* sub-loop pass of the current VF_V1CAL call */
 static int g_static_done = 0;
 static R8 *g_u = NULL, *g_v = NULL, *g_w = NULL, *g_p = NULL, *g_f = NULL;
 static I4 *g_nf = NULL;
@@ -69,6 +70,9 @@ static int open_dump(const char *tag)
     char path[4096];
     if (strcmp(tag, "static") == 0)
         snprintf(path, sizeof path, "%s/static.bin", dir);
+    else if (g_iloop > 1 && strncmp(tag, "f1", 2) != 0 && strcmp(tag, "dv") != 0)
+        /* the second and later VF_V1CAL passes of OPTION SUB-LOOP */
+        snprintf(path, sizeof path, "%s/s%05d_%s_l%d.bin", dir, vf_atimei_.nnow, tag, g_iloop);
     else
         snprintf(path, sizeof path, "%s/s%05d_%s.bin", dir, vf_atimei_.nnow, tag);
     g_fp = fopen(path, "wb");
@@ -160,6 +164,7 @@ void vf_v1cal_(I4 *iloop, P xx, P yy, P zz, P uu, P vv, P ww, P pp, P ff, P anu,
                I4 *nf, I4 *indx, I4 *indy, I4 *indz, I4 *indc, I4 *indb, I4 *inds_)
 {
     g_u = uu; g_v = vv; g_w = ww; g_p = pp; g_f = ff; g_nf = nf;
+    g_iloop = *iloop;
     if (!g_static_done && open_dump("static")) {
         int dx[2] = {6, vf_anumbi_.numi}, dy[2] = {6, vf_anumbi_.numj}, dz[2] = {6, vf_anumbi_.numk};
         put("XX", 'd', 2, dx, xx); put("YY", 'd', 2, dy, yy); put("ZZ", 'd', 2, dz, zz);
@@ -198,6 +203,7 @@ void vf_v1cal_(I4 *iloop, P xx, P yy, P zz, P uu, P vv, P ww, P pp, P ff, P anu,
         f3("PT", wk09);
         close_dump();
     }
+    g_iloop = 1;
 }
 
 void vf_vpsol_orig_(P dbuf, P ad, P ali, P alj, P alk, P aui, P auj, P auk, P bb, P pt,
@@ -271,6 +277,9 @@ void vf_cdiv00_(P val, P xx, P yy, P zz, P uu, P vv, P ww, P ggx, P ggy, P ggz, 
     if (open_dump("dv")) {
         sd("DV", *val);
         si("NNOW", vf_atimei_.nnow);
+        /* the step size and time of step NNOW, full precision */
+        sd("DTNOW", vf_atimer_.dtnow);
+        sd("TNOW", vf_atimer_.tnow);
         close_dump();
     }
 }

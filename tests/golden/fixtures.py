@@ -38,6 +38,8 @@ def deck_params(deck_text):
     p = dict(rho0=1000.0, grz0=9.8, anu0=1.0e-6, cgepsr=1.0e-12, cgpara=0.95, wbub=0.2, idrop=1)
     for line in str(deck_text).splitlines():
         w = line.split("#", 1)[0].split()
+        if len(w) >= 3 and w[0] == "OPTION" and w[1] == "SUB-LOOP":
+            p["loops"] = int(w[2])
         if len(w) >= 4 and w[0] == "OPTION" and w[1] == "T-DOOR":
             if w[2] == "BUB":
                 p["wbub"] = 0.0 if w[3] == "OFF" else float(w[3].replace("D", "E"))
@@ -57,6 +59,21 @@ def deck_params(deck_text):
             elif w[2] == "M-ILUBCGSTAB":
                 p["cgpara"] = float(w[3].replace("D", "E"))
     return p
+
+
+def time_params(deck_text):
+    """TIME AUTO / LIMIT of a deck (vf_iitime.f, defaults vf_a2dflt.f:223-234)."""
+    t = dict(idttyp=1, dtinit=1.0e-6, dtsafe=0.1, dtmin=1.0e-20, dtmax=1.0e20)
+    for line in str(deck_text).splitlines():
+        w = line.split("#", 1)[0].split()
+        if len(w) >= 3 and w[0] == "TIME":
+            if w[1] == "CONST":
+                t["idttyp"] = 0
+            elif w[1] == "AUTO":
+                t.update(idttyp=1, dtinit=float(w[2]), dtsafe=float(w[3]))
+            elif w[1] == "LIMIT":
+                t.update(dtmin=float(w[2]), dtmax=float(w[3]))
+    return t
 
 
 def state_arrays(st, rec):

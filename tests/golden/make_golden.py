@@ -61,6 +61,9 @@ def _cases():
                   [1, 2, 100, 101, 145, 170, 171, 180, 195, 200, 210, 220, 230, 245, 300]),
         "tdoor_dam": (lambda: cfg.tdoor_dam(name="tdoor_dam"),
                       [1, 2, 26, 200, 201, 400, 533, 571, 600, 601, 700, 900]),
+        # automatic time step (vf_cdtcal.f) and OPTION SUB-LOOP
+        "auto_dam": (lambda: cfg.auto_dam(name="auto_dam"), [1, 2, 100, 250, 400]),
+        "subloop": (lambda: cfg.subloop(name="subloop"), [1, 2, 20, 40]),
     }
 
 
@@ -70,6 +73,7 @@ _KEEP = {
     "dam_a": {1: {"v1_in": None, "f1_in": ("BCF",)}, 100: {"f1_out": None}, 500: {"f1_out": None}},
     "tdoor": {s: _F1_ONLY for s in (2, 100, 101, 145, 170, 171, 180, 195, 200, 210, 220, 230, 245, 300)},
     "tdoor_dam": {s: _F1_ONLY for s in (2, 26, 200, 201, 400, 533, 571, 600, 601, 700, 900)},
+    "auto_dam": {s: _F1_ONLY for s in (100, 250, 400)},
 }
 
 
@@ -99,20 +103,32 @@ def run_case(name, factory, keep_steps):
                 traj[key].append(float(v))
             traj["ITR"].append(int(vals[8]))
         # full-precision per-step scalars from the recorder
-        dv = {}
+        dv, dtf, tnf = {}, {}, {}
         for f in sorted((td / "dump").glob("s*_dv.bin")):
             d = read_dump(f)
-            dv[int(d["NNOW"][0])] = float(d["DV"][0])
+            s = int(d["NNOW"][0])
+            dv[s] = float(d["DV"][0])
+            if "DTNOW" in d:
+                dtf[s] = float(d["DTNOW"][0])
+                tnf[s] = float(d["TNOW"][0])
+        steps_dv = sorted(dv)
+        extra = {}
+        if len(dtf) == len(dv):
+            extra = dict(DT_FULL=np.array([dtf[s] for s in steps_dv]),
+                         TNOW_FULL=np.array([tnf[s] for s in steps_dv]))
         np.savez_compressed(HERE / f"{name}_trajectory.npz",
                             **{k: np.array(v) for k, v in traj.items()},
-                            DV_STEPS=np.array(sorted(dv)), DV_FULL=np.array([dv[s] for s in sorted(dv)]),
-                            DECK=np.array(case.deck))
+                            DV_STEPS=np.array(steps_dv), DV_FULL=np.array([dv[s] for s in steps_dv]),
+                            DECK=np.array(case.deck), **extra)
         static = read_dump(td / "dump" / "static.bin")
         np.savez_compressed(HERE / f"{name}_static.npz", **static, DECK=np.array(case.deck))
         for s in keep_steps:
             rec = {}
             keep = _KEEP.get(name, {}).get(s)
-            for tag in ("v1_in", "ps_in", "ps_out", "v1_out", "f1_in", "f1_out"):
+            tags = ["v1_in", "ps_in", "ps_out", "v1_out", "f1_in", "f1_out"]
+            # sub-loop passes 2.. of VF_V1CAL (OPTION SUB-LOOP)
+            tags += [f"{t}_l{l}" for l in range(2, 5) for t in ("v1_in", "ps_in", "ps_out", "v1_out")]
+            for tag in tags:
                 p = td / "dump" / f"s{s:05d}_{tag}.bin"
                 if keep is not None and tag not in keep:
                     continue

@@ -47,8 +47,6 @@ def test_metrics_closed_form(pkg):
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB\n"
      "TIME CONST 1e-3\n", "DeckError"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
-     "OPTION T-DOOR DROP OFF\nTIME AUTO 1e-3 0.1\n", "UnsupportedDeck"),
-    ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
      "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nMODEL WAVE-BC X- FUNC STREAM 5\n", "UnsupportedDeck"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
      "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nPARALLEL X 3\n", "UnsupportedDeck"),
@@ -79,3 +77,56 @@ def test_workload_decks_parse(pkg):
         case = pkg.configs.named(name)
         d = pkg.deck.parse(case.deck)
         assert d.ncells == cells
+
+
+def test_deck_time_words(pkg):
+    """TIME CONST / AUTO / LIMIT / END and OPTION SUB-LOOP as vf_iitime.f:28-60
+    and vf_iiopt.f:41-45 read them (defaults vf_a2dflt.f:223-234)."""
+    base = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\n"
+    d = pkg.deck.parse(base)
+    assert (d.idttyp, d.dtinit, d.dtsafe, d.dtmin, d.dtmax, d.loops) == (1, 1e-6, 0.1, 1e-20, 1e20, 1)
+    d = pkg.deck.parse(base + "TIME AUTO 1e-4 0.3\nTIME LIMIT 1e-6 1e-2\nTIME END 7 2.5\nOPTION SUB-LOOP 3\n")
+    assert (d.idttyp, d.dtinit, d.dtsafe, d.dtmin, d.dtmax, d.nend, d.tend, d.loops) == \
+        (1, 1e-4, 0.3, 1e-6, 1e-2, 7, 2.5, 3)
+    d = pkg.deck.parse(base + "TIME AUTO 1e-4 0.3\nTIME CONST 2e-3\n")
+    assert (d.idttyp, d.dt_const) == (0, 2e-3)
+    for bad in ("TIME LIMIT 1e-2 1e-3\n", "TIME AUTO 1e-4\n", "TIME FOO 1\n", "OPTION SUB-LOOP 0\n"):
+        with pytest.raises(pkg.deck.DeckError):
+            pkg.deck.parse(base + bad)
+
+
+class _FakeCtx:
+    """Stands in for capi.Context: a step advances TNOW by DTNOW."""
+
+    def __init__(self, dts):
+        self.p = {"tnow": 0.0, "dtnow": 0.0}
+        self.dts = list(dts)
+        self.calls = []
+
+    def get_params(self):
+        return dict(self.p)
+
+    def set_params(self, **kw):
+        self.p.update(kw)
+
+    def cdtcal(self, nnow, *limits):
+        self.calls.append(nnow)
+        return self.dts[nnow]
+
+    def step(self):
+        self.p["tnow"] = self.p["tnow"] + self.p["dtnow"]
+        return {"icgitr": 1}
+
+
+def test_time_loop_end_rule(pkg):
+    """vf_a1main.f:937-947: the step size is computed first, then the loop ends
+    when NNOW >= NEND or TNOW + DT/2 >= TEND."""
+    base = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\n"
+    d = pkg.deck.parse(base + "TIME AUTO 1e-4 0.3\nTIME END 100 0.35\n")
+    ctx = _FakeCtx([0.1] * 10)
+    recs = list(pkg.timeloop.run(ctx, d))
+    # t = 0.1, 0.2, 0.3; at t = 0.3 the next step would pass 0.35 - 0.05
+    assert [r.nnow for r in recs] == [1, 2, 3] and ctx.calls == [0, 1, 2, 3]
+    d = pkg.deck.parse(base + "TIME CONST 0.1\nTIME END 2 10.0\n")
+    recs = list(pkg.timeloop.run(_FakeCtx([]), d))
+    assert [r.nnow for r in recs] == [1, 2] and recs[-1].dtnow == 0.1

@@ -148,3 +148,126 @@ def test_timer_door_free_run_bitexact(case):
             assert F.st.tnow == ref["TNOW"][0], s
             for k in ("UU", "VV", "WW", "PP", "FF", "NF", "TBUB") + DROP:
                 assert np.array_equal(F[k], ref[k]), (s, k)
+
+
+def _free_run_fields(case):
+    st = fx.static(case)
+    r1 = fx.step(case, 1)
+    p = _params(st, r1["v1_in"])
+    arr = fx.state_arrays(st, r1["v1_in"])
+    arr["BCF"] = r1["f1_in"]["BCF"]
+    return st, p, po.Fields(arr, p, inds=r1["v1_in"]["INDS"][: int(r1["v1_in"]["NUMS"][0])])
+
+
+def test_auto_time_step_free_run_bitexact():
+    """TIME AUTO (VF_CDTCAL before every step): every step size equals the
+    reference's to the last bit, and so do ITR, ||div|| and the fields."""
+    case = "auto_dam"
+    st, p, F = _free_run_fields(case)
+    tp = fx.time_params(st["DECK"])
+    tr = fx.trajectory(case)
+    kept = set(fx.steps(case))
+    for s in range(1, int(tr["STEP"][-1]) + 1):
+        dt = F.cdtcal(s - 1, tp["dtinit"], tp["dtsafe"], tp["dtmin"], tp["dtmax"])
+        assert dt == tr["DT_FULL"][s], (s, dt, tr["DT_FULL"][s])
+        F.st.dtnow = dt
+        rc, _ = F.step()
+        assert rc == 0, s
+        assert F.st.tnow == tr["TNOW_FULL"][s], s
+        assert F.st.icgitr == tr["ITR"][s], s
+        assert F.cdiv00() == tr["DV_FULL"][s], s
+        if s in kept:
+            ref = fx.step(case, s)["f1_out"]
+            for k in ("UU", "VV", "WW", "PP", "FF", "NF"):
+                assert np.array_equal(F[k], ref[k]), (s, k)
+
+
+def test_sub_loop_free_run_bitexact():
+    """OPTION SUB-LOOP 3: three VF_V1CAL passes at DT/3 per step (boundary
+    values refreshed before passes 2 and 3), then VF_F1CAL at DT."""
+    case = "subloop"
+    st, p, F = _free_run_fields(case)
+    assert p["loops"] == 3
+    tr = fx.trajectory(case)
+    kept = set(fx.steps(case))
+    F.st.dtnow = tr["DT_FULL"][1]     # the step's DT (V1CAL's record holds DT/3)
+    for s in range(1, int(tr["STEP"][-1]) + 1):
+        rc, _ = F.step()
+        assert rc == 0, s
+        assert F.st.dtnow == tr["DT_FULL"][s] and F.st.tnow == tr["TNOW_FULL"][s], s
+        assert F.st.icgitr == tr["ITR"][s], s
+        assert F.cdiv00() == tr["DV_FULL"][s], s
+        if s in kept:
+            ref = fx.step(case, s)["f1_out"]
+            for k in ("UU", "VV", "WW", "PP", "FF", "NF"):
+                assert np.array_equal(F[k], ref[k]), (s, k)
+
+
+def test_sub_loop_passes_bitexact():
+    """Each recorded V1CAL pass of a sub-looped step from its own input."""
+    case = "subloop"
+    st = fx.static(case)
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        for l, sfx in ((1, ""), (2, "_l2"), (3, "_l3")):
+            rin, rout = r["v1_in" + sfx], r["v1_out" + sfx]
+            p = _params(st, rin)
+            F = _fields(st, rin, p, r["f1_in"]["BCF"])
+            rc, pt = F.v1cal_loop(l)
+            assert rc == 0
+            for k in ("UU", "VV", "WW", "PP", "BCU", "BCV", "BCW", "BCP"):
+                assert np.array_equal(F[k], rout[k]), (s, l, k)
+            assert np.array_equal(pt, rout["PT"]), (s, l)
+
+
+def _cell_limits(st, rec):
+    """Per-cell step limits of VF_CDTCAL evaluated independently (numpy):
+    V / max|face flux| and (0.5 / sum 1/dx^2) / nu over non-obstacle cells."""
+    ff, nf = rec["FF"], rec["NF"]
+    xx, yy, zz = st["XX"], st["YY"], st["ZZ"]
+    dx = xx[1][1:-1, None, None]
+    dy = yy[1][None, 1:-1, None]
+    dz = zz[1][None, None, 1:-1]
+    I = (slice(1, -1),) * 3
+    def fl(area, g, q, ind, sh):
+        lo = np.abs(area * g[I] * q[I])
+        up = np.abs(area * np.roll(g, -1, sh)[I] * np.roll(q, -1, sh)[I])
+        for a, f in ((lo, ind[I]), (up, np.roll(ind, -1, sh)[I])):
+            b = f >= 1
+            bcf = np.where(b, rec["BCF"][np.maximum(f, 1) - 1], 0.0)
+            a[b & (np.maximum(ff[I], bcf) <= 0.0)] = 0.0
+        return lo, up
+    fls = []
+    for area, g, q, ind, sh in ((dy * dz, st["GGX"], rec["UU"], st["INDX"], 0),
+                                (dx * dz, st["GGY"], rec["VV"], st["INDY"], 1),
+                                (dx * dy, st["GGZ"], rec["WW"], st["INDZ"], 2)):
+        fls += list(fl(area, g, q, ind, sh))
+    m = np.max(fls, axis=0)
+    v = st["GGV"][I] * dx * dy * dz
+    live = nf[I] != -1
+    with np.errstate(divide="ignore"):
+        c1 = np.where(live & (m > 0), v / np.where(m > 0, m, 1.0), np.inf)
+        vd = 0.5 / (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz))
+        anu = st["ANU"][I]
+        c2 = np.where(live & (anu > 0), vd / np.where(anu > 0, anu, 1.0), np.inf)
+    return min(c1.min(), c2.min())
+
+
+def test_cdtcal_is_the_cell_minimum():
+    """What the recurrence computes: the smallest per-cell limit (times
+    DTSAFE, capped by 1.2x the last step and DTMIN/DTMAX), up to the
+    rounding that V/(V/DT) adds cell by cell: within 1e-12 relative."""
+    case = "auto_dam"
+    st = fx.static(case)
+    tp = fx.time_params(st["DECK"])
+    tr = fx.trajectory(case)
+    worst = 0.0
+    for s in fx.steps(case):
+        if s + 1 > int(tr["STEP"][-1]):
+            continue
+        rec = fx.step(case, s)["f1_out"]
+        lim = min(_cell_limits(st, rec), tp["dtmax"] / tp["dtsafe"])
+        dt = max(min(lim * tp["dtsafe"], 1.2 * tr["DT_FULL"][s]), tp["dtmin"])
+        ref = tr["DT_FULL"][s + 1]
+        worst = max(worst, abs(dt - ref) / ref)
+    assert worst <= 1e-12, worst
