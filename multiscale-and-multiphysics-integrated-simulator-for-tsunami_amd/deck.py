@@ -80,6 +80,10 @@ class Deck:
     ibsuw0: int = 0
     loops: int = 1
     vvmax: float = 30.0
+    # FILE RSL STEP n1 n2 dn / TIME t1 t2 dt (vf_iifile.f:209-238) and
+    # FILE RES step (:241-244)
+    rsl: Optional[tuple] = None
+    res_step: Optional[int] = None
     # DEBUG F-BOX (vf_iidbg.f; cell indices already +1 like IDBGF)
     fbox: Optional[Tuple[int, int, int, int, int, int, float]] = None
 
@@ -88,7 +92,9 @@ class Deck:
         return tuple(len(self.grid[a]) - 1 for a in "XYZ")
 
 
-_IGNORED_FILE = {"L/P", "GRP", "RSL", "TRN", "OBS", "RES"}
+# output controls read elsewhere or not produced here: list-file echo
+# (L/P), graphics (GRP), time series (TRN), observation points (OBS)
+_IGNORED_FILE = {"L/P", "GRP", "TRN", "OBS"}
 
 
 def parse(text):
@@ -178,10 +184,7 @@ def parse(text):
             else:
                 raise UnsupportedDeck(f"DEBUG {w[1]} is not supported")
         elif head == "FILE":
-            if len(w) < 2 or w[1] not in _IGNORED_FILE:
-                raise UnsupportedDeck(f"FILE {w[1] if len(w) > 1 else ''} is not supported")
-            if w[1] == "RES":
-                raise UnsupportedDeck("restart input is taken through rsl.read, not FILE RES")
+            _parse_file(d, w)
         elif head == "EQUATION":
             if len(w) >= 3 and w[2] == "NOCALC":
                 continue
@@ -264,6 +267,39 @@ def _parse_comp(d, w):
             raise DeckError(f"COMP MTRX {w[2]}")
     else:
         raise DeckError(f"COMP {w[1]}")
+
+
+def _parse_file(d, w):
+    """FILE RSL (result file) and FILE RES (restart), Source code/vf_iifile.f:209-244."""
+    if len(w) < 2:
+        raise DeckError("FILE: SYNTAX ERROR.")
+    if w[1] == "RSL":
+        if len(w) < 3:
+            raise DeckError("FILE RSL: SYNTAX ERROR.")
+        if w[2] == "STEP":
+            if len(w) < 6:
+                raise DeckError("FILE RSL STEP: SYNTAX ERROR.")
+            a, b, c = _int(w[3]), _int(w[4]), _int(w[5])
+            if a > b or c <= 0:
+                raise DeckError("FILE RSL STEP: INVALID VALUE.")
+            d.rsl = ("STEP", a, b, c)
+        elif w[2] == "TIME":
+            if len(w) < 6:
+                raise DeckError("FILE RSL TIME: SYNTAX ERROR.")
+            a, b, c = _num(w[3]), _num(w[4]), _num(w[5])
+            if a > b or c <= 0.0:
+                raise DeckError("FILE RSL TIME: INVALID VALUE.")
+            d.rsl = ("TIME", a, b, c)
+        elif w[2] == "ELAPSE":
+            pass    # wall-clock stop with a final result record: not produced
+        else:
+            raise DeckError(f"FILE RSL {w[2]}: UNKNOWN WORD.")
+    elif w[1] == "RES":
+        if len(w) < 3 or _int(w[2]) < 0:
+            raise DeckError("FILE RES: INVALID VALUE.")
+        d.res_step = _int(w[2])
+    elif w[1] not in _IGNORED_FILE:
+        raise UnsupportedDeck(f"FILE {w[1]} is not supported")
 
 
 def _parse_time(d, w):

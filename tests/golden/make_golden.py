@@ -77,6 +77,10 @@ _KEEP = {
 }
 
 
+# cases whose reference run also writes its result file (FILE RSL, Source
+# code/vf_iifile.f:209-238); the file's bytes are kept as <case>_rsl.npz
+_RSL = {"tiny": "FILE RSL STEP 0 5 1", "tdoor": "FILE RSL STEP 100 300 100"}
+
 _STEP_RE = re.compile(
     r"STEP=\s*(\d+)\s*:\s*TIME=\s*(\S+)\s*:\s*DT\s*=\s*(\S+)\s*:\s*FSUM=\s*(\S+)\s*:\s*FCUT=\s*(\S+)"
     r"\s*:\s*!VD!=\s*(\S+)\s*\n\s*:\s*!B!\s*=\s*(\S+)\s*:\s*!R!\s*=\s*(\S+)\s*:\s*ITR\s*=\s*(\S+)")
@@ -84,6 +88,8 @@ _STEP_RE = re.compile(
 
 def run_case(name, factory, keep_steps):
     case = factory()
+    if name in _RSL:
+        case.deck = case.deck + _RSL[name] + "\n"
     with tempfile.TemporaryDirectory(prefix=f"golden_{name}_") as td:
         td = Path(td)
         (td / "dump").mkdir()
@@ -95,6 +101,11 @@ def run_case(name, factory, keep_steps):
         if r.returncode != 0:
             raise RuntimeError(f"reference run {name} failed rc={r.returncode}\n{r.stdout[-2000:]}")
         listing = (td / f"{name}.list").read_text()
+        lines = listing.splitlines()
+        list_lines = [a + "\n" + b for a, b in zip(lines, lines[1:]) if a.startswith(" STEP=")]
+        if name in _RSL:
+            raw = (td / f"{name}.rsl").read_bytes()
+            np.savez_compressed(HERE / f"{name}_rsl.npz", RSL=np.frombuffer(raw, dtype=np.uint8))
         traj = {k: [] for k in ("STEP", "TIME", "DT", "FSUM", "FCUT", "DV", "BNORM", "RNORM", "ITR")}
         for m in _STEP_RE.finditer(listing):
             vals = m.groups()
@@ -119,7 +130,7 @@ def run_case(name, factory, keep_steps):
         np.savez_compressed(HERE / f"{name}_trajectory.npz",
                             **{k: np.array(v) for k, v in traj.items()},
                             DV_STEPS=np.array(steps_dv), DV_FULL=np.array([dv[s] for s in steps_dv]),
-                            DECK=np.array(case.deck), **extra)
+                            DECK=np.array(case.deck), LIST=np.array(list_lines), **extra)
         static = read_dump(td / "dump" / "static.bin")
         np.savez_compressed(HERE / f"{name}_static.npz", **static, DECK=np.array(case.deck))
         for s in keep_steps:

@@ -1,0 +1,73 @@
+"""Whole runs from the reference's input deck through driver.run on the GPU:
+the result file and list-file lines it writes, and a restart from the
+reference's own result file (FILE RES)."""
+import numpy as np
+import pytest
+
+import fixtures as fx
+from helpers import assert_close, assert_same_int, assert_vel_close
+
+pytestmark = pytest.mark.gpu
+
+
+def _ref_rsl(tmp_path, case, name):
+    with np.load(fx.HERE / f"{case}_rsl.npz") as z:
+        raw = z["RSL"].tobytes()
+    p = tmp_path / name
+    p.write_bytes(raw)
+    return p
+
+
+def test_run_writes_the_reference_outputs(pkg, tmp_path):
+    """tiny (FILE RSL STEP 0 5 1): the same records in the .rsl, fields to the
+    plain-form bars, flags exactly; list lines with the same step, ITR and
+    numbers to print precision."""
+    case = "tiny"
+    deck = str(fx.static(case)["DECK"])
+    res = pkg.driver.run(deck, "run", tmp_path)
+    assert res["nnow"] == 5
+    ref = list(pkg.rsl.read(_ref_rsl(tmp_path, case, "ref.rsl")))
+    mine = list(pkg.rsl.read(tmp_path / "run.rsl"))
+    assert len(mine) == len(ref) == 7
+    assert np.array_equal(mine[0]["GGV"], ref[0]["GGV"])
+    for a, b in zip(mine[1:], ref[1:]):
+        s = b["NNOW"]
+        assert a["NNOW"] == s and a["ICGITR"] == b["ICGITR"]
+        assert a["TNOW"] == b["TNOW"] and a["DTNOW"] == b["DTNOW"]
+        assert_vel_close(f"step {s} velocity", a, b, 1e-10)
+        for k in ("PP", "FF"):
+            assert_close(f"step {s} {k}", a[k], b[k], 1e-10)
+        assert_same_int(f"step {s} NF", a["NF"], b["NF"])
+    lines = (tmp_path / "run.list").read_text().splitlines()
+    ref_lines = [ln for x in fx.trajectory(case)["LIST"] for ln in str(x).splitlines()]
+    assert len(lines) == len(ref_lines)
+    for x, y in zip(lines, ref_lines):
+        assert x.split(":")[0] == y.split(":")[0]
+        fx_ = [w for w in x.replace("=", " ").split() if w[0].isdigit() or w[0] == "-"]
+        fy_ = [w for w in y.replace("=", " ").split() if w[0].isdigit() or w[0] == "-"]
+        assert len(fx_) == len(fy_)
+        for u, v in zip(fx_, fy_):
+            assert abs(float(u) - float(v)) <= 2e-5 * max(abs(float(v)), 1e-30), (x, y)
+
+
+def test_restart_from_reference_result_file(pkg, tmp_path):
+    """FILE RES 200 from the droplet case's .rsl: ten steps later the state
+    matches the reference's step 210."""
+    case = "tdoor"
+    _ref_rsl(tmp_path, case, "rst.res")
+    deck = str(fx.static(case)["DECK"])
+    deck = "\n".join(ln for ln in deck.splitlines() if not ln.startswith("FILE RSL"))
+    deck += "\nFILE RES 200\nFILE RSL STEP 210 210 1\n"
+    res = pkg.driver.run(deck, "rst", tmp_path, nmax=10)
+    assert res["nnow"] == 210
+    ref = fx.step(case, 210)["f1_out"]
+    assert res["tnow"] == pytest.approx(float(ref["TNOW"][0]), rel=1e-14)
+    assert res["info"]["icgitr"] == int(ref["ICGITR"][0])
+    recs = list(pkg.rsl.read(tmp_path / "rst.rsl"))
+    assert len(recs) == 2 and recs[1]["NNOW"] == 210
+    a = recs[1]
+    assert_vel_close("step 210 velocity", a, ref, 1e-8)
+    for k in ("PP", "FF"):
+        assert_close(f"step 210 {k}", a[k], ref[k], 1e-8)
+    assert_same_int("step 210 NF", a["NF"], ref["NF"])
+    assert np.array_equal(a["TBUB"], ref["TBUB"])
