@@ -10,6 +10,7 @@ from . import capi  # noqa: F401
 from . import configs  # noqa: F401
 from . import deck  # noqa: F401
 from . import driver  # noqa: F401
+from . import grp  # noqa: F401
 from . import layout  # noqa: F401
 from . import rsl  # noqa: F401
 from . import timeloop  # noqa: F401

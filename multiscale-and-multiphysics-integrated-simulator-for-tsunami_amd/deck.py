@@ -84,6 +84,9 @@ class Deck:
     # FILE RES step (:241-244)
     rsl: Optional[tuple] = None
     res_step: Optional[int] = None
+    # FILE GRP STEP/TIME and AREA (cell indices +1 like IGRARA)
+    grp: Optional[tuple] = None
+    grp_area: Optional[tuple] = None
     # DEBUG F-BOX (vf_iidbg.f; cell indices already +1 like IDBGF)
     fbox: Optional[Tuple[int, int, int, int, int, int, float]] = None
 
@@ -92,9 +95,9 @@ class Deck:
         return tuple(len(self.grid[a]) - 1 for a in "XYZ")
 
 
-# output controls read elsewhere or not produced here: list-file echo
-# (L/P), graphics (GRP), time series (TRN), observation points (OBS)
-_IGNORED_FILE = {"L/P", "GRP", "TRN", "OBS"}
+# output controls not produced here: list-file echo (L/P), time series
+# (TRN), observation points (OBS)
+_IGNORED_FILE = {"L/P", "TRN", "OBS"}
 
 
 def parse(text):
@@ -298,6 +301,27 @@ def _parse_file(d, w):
         if len(w) < 3 or _int(w[2]) < 0:
             raise DeckError("FILE RES: INVALID VALUE.")
         d.res_step = _int(w[2])
+    elif w[1] == "GRP":
+        # vf_iifile.f:148-207
+        if len(w) < 3:
+            raise DeckError("FILE GRP: SYNTAX ERROR.")
+        if w[2] in ("STEP", "TIME"):
+            if len(w) < 6:
+                raise DeckError(f"FILE GRP {w[2]}: SYNTAX ERROR.")
+            conv = _int if w[2] == "STEP" else _num
+            a, b, c = conv(w[3]), conv(w[4]), conv(w[5])
+            if a > b or c <= 0:
+                raise DeckError(f"FILE GRP {w[2]}: INVALID VALUE.")
+            d.grp = (w[2], a, b, c)
+        elif w[2] == "AREA":
+            if len(w) < 9:
+                raise DeckError("FILE GRP AREA: SYNTAX ERROR.")
+            d.grp_area = tuple(_int(x) + 1 for x in w[3:9])
+        elif w[2] in ("ON", "OFF"):
+            if len(w) >= 4 and w[3] == "VORT" and w[2] == "ON":
+                raise UnsupportedDeck("FILE GRP ON VORT: vorticity output is not written")
+        else:
+            raise DeckError(f"FILE GRP {w[2]}: UNKNOWN WORD.")
     elif w[1] not in _IGNORED_FILE:
         raise UnsupportedDeck(f"FILE {w[1]} is not supported")
 

@@ -9,8 +9,9 @@ per iteration:
   - the next step size (TIME CONST, or VF_CDTCAL on the device);
   - the end test (NNOW >= NEND or TNOW + DT/2 >= TEND);
   - the outputs of the state so far: the list-file lines (VF_CDIV00 for
-    !VD!) and, where FILE RSL selects the step, a record in <name>.rsl
-    (VF_OR1TRN's step / time rule);
+    !VD!) and, where FILE RSL / FILE GRP select the step, a record in
+    <name>.rsl (VF_OR1TRN) and <name>.grp (VF_OG1TRN), both by the step /
+    time rule of vf_or1trn.f:62-78;
   - the step (cad_step).
 """
 from pathlib import Path
@@ -18,14 +19,15 @@ from pathlib import Path
 import numpy as np
 
 from . import capi
-from . import deck as deckmod
+from . import grp as grpmod
 from . import layout as layoutmod
 from . import rsl as rslmod
 from . import timeloop
 
 
 class _RslRule:
-    """Which steps VF_OR1TRN writes (vf_or1trn.f:62-78)."""
+    """Which steps VF_OR1TRN / VF_OG1TRN write (vf_or1trn.f:62-78,
+    vf_og1trn.f:54-70)."""
 
     def __init__(self, spec, restart):
         self.spec = spec
@@ -75,10 +77,14 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
             ctx.set_params(tnow=0.0)
             nnow = 0
         rule = _RslRule(d.rsl, restart)
-        writer = None
+        grule = _RslRule(d.grp, restart)
+        writer = gwriter = None
         if d.rsl is not None:
             writer = rslmod.Writer(workdir / f"{name}.rsl", lay.shape, lay.numb, static["GGV"],
                                    idrop=d.idrop)
+        if d.grp is not None:
+            gwriter = grpmod.Writer(workdir / f"{name}.grp", static["XX"], static["YY"], static["ZZ"],
+                                    static["INDB"], static["GGV"], d.grp_area)
         out = open(workdir / f"{name}.list", "w")
         done = 0
         try:
@@ -91,8 +97,12 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
                 a, b = rslmod.list_lines(nnow, tnow, dtnow, info["fsum"], info["fcut"], dv,
                                          info["cgbnrm"], info["cgxnrm"], info["icgitr"])
                 out.write(a + "\n" + b + "\n")
-                if writer is not None and rule(nnow, tnow, dt):
+                st = None
+                if gwriter is not None and grule(nnow, tnow, dt):
                     st = ctx.download_state()
+                    gwriter.write_step(nnow, tnow, st)
+                if writer is not None and rule(nnow, tnow, dt):
+                    st = st if st is not None else ctx.download_state()
                     td = ctx.download_tdoor() if d.idrop >= 1 or d.wbub >= 1e-20 else None
                     writer.write_step(nnow, tnow, dtnow, info, st, td)
                 if log:
@@ -105,8 +115,9 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
                 done += 1
         finally:
             out.close()
-            if writer is not None:
-                writer.close()
+            for w in (writer, gwriter):
+                if w is not None:
+                    w.close()
         par = ctx.get_params()
         return {"nnow": nnow, "tnow": par["tnow"], "dtnow": par["dtnow"], "info": info}
     finally:

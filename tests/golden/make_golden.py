@@ -77,9 +77,11 @@ _KEEP = {
 }
 
 
-# cases whose reference run also writes its result file (FILE RSL, Source
-# code/vf_iifile.f:209-238); the file's bytes are kept as <case>_rsl.npz
-_RSL = {"tiny": "FILE RSL STEP 0 5 1", "tdoor": "FILE RSL STEP 100 300 100"}
+# cases whose reference run also writes its result file (FILE RSL) and its
+# graphics file (FILE GRP, Source code/vf_iifile.f:148-238); the files' bytes
+# are kept as <case>_rsl.npz / <case>_grp.npz
+_RSL = {"tiny": "FILE RSL STEP 0 5 1\nFILE GRP STEP 0 5 1",
+        "tdoor": "FILE RSL STEP 100 300 100\nFILE GRP STEP 100 300 100\nFILE GRP AREA 4 2 3 18 4 14"}
 
 _STEP_RE = re.compile(
     r"STEP=\s*(\d+)\s*:\s*TIME=\s*(\S+)\s*:\s*DT\s*=\s*(\S+)\s*:\s*FSUM=\s*(\S+)\s*:\s*FCUT=\s*(\S+)"
@@ -104,8 +106,10 @@ def run_case(name, factory, keep_steps):
         lines = listing.splitlines()
         list_lines = [a + "\n" + b for a, b in zip(lines, lines[1:]) if a.startswith(" STEP=")]
         if name in _RSL:
-            raw = (td / f"{name}.rsl").read_bytes()
-            np.savez_compressed(HERE / f"{name}_rsl.npz", RSL=np.frombuffer(raw, dtype=np.uint8))
+            for ext in ("rsl", "grp"):
+                raw = (td / f"{name}.{ext}").read_bytes()
+                np.savez_compressed(HERE / f"{name}_{ext}.npz",
+                                    **{ext.upper(): np.frombuffer(raw, dtype=np.uint8)})
         traj = {k: [] for k in ("STEP", "TIME", "DT", "FSUM", "FCUT", "DV", "BNORM", "RNORM", "ITR")}
         for m in _STEP_RE.finditer(listing):
             vals = m.groups()

@@ -38,6 +38,10 @@ def test_run_writes_the_reference_outputs(pkg, tmp_path):
         for k in ("PP", "FF"):
             assert_close(f"step {s} {k}", a[k], b[k], 1e-10)
         assert_same_int(f"step {s} NF", a["NF"], b["NF"])
+    with np.load(fx.HERE / f"{case}_grp.npz") as z:
+        graw = z["GRP"].tobytes()
+    gmine = (tmp_path / "run.grp").read_bytes()
+    assert len(gmine) == len(graw)
     lines = (tmp_path / "run.list").read_text().splitlines()
     ref_lines = [ln for x in fx.trajectory(case)["LIST"] for ln in str(x).splitlines()]
     assert len(lines) == len(ref_lines)
