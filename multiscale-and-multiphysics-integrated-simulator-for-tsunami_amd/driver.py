@@ -16,8 +16,6 @@ per iteration:
 """
 from pathlib import Path
 
-import numpy as np
-
 from . import capi
 from . import grp as grpmod
 from . import layout as layoutmod
