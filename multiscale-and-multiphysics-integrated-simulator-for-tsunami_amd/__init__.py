@@ -14,3 +14,4 @@ from . import grp  # noqa: F401
 from . import layout  # noqa: F401
 from . import rsl  # noqa: F401
 from . import timeloop  # noqa: F401
+from . import tran  # noqa: F401

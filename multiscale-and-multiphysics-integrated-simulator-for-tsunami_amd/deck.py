@@ -87,6 +87,9 @@ class Deck:
     # FILE GRP STEP/TIME and AREA (cell indices +1 like IGRARA)
     grp: Optional[tuple] = None
     grp_area: Optional[tuple] = None
+    # FILE TRN STEP/TIME and its probe lines (words after FILE TRN)
+    trn: Optional[tuple] = None
+    trn_probes: List[tuple] = field(default_factory=list)
     # DEBUG F-BOX (vf_iidbg.f; cell indices already +1 like IDBGF)
     fbox: Optional[Tuple[int, int, int, int, int, int, float]] = None
 
@@ -95,9 +98,9 @@ class Deck:
         return tuple(len(self.grid[a]) - 1 for a in "XYZ")
 
 
-# output controls not produced here: list-file echo (L/P), time series
-# (TRN), observation points (OBS)
-_IGNORED_FILE = {"L/P", "TRN", "OBS"}
+# output controls not produced here: list-file echo (L/P), observation
+# points (OBS)
+_IGNORED_FILE = {"L/P", "OBS"}
 
 
 def parse(text):
@@ -301,6 +304,21 @@ def _parse_file(d, w):
         if len(w) < 3 or _int(w[2]) < 0:
             raise DeckError("FILE RES: INVALID VALUE.")
         d.res_step = _int(w[2])
+    elif w[1] == "TRN":
+        # vf_iifile.f:247-441: STEP / TIME, or one probe (parsed by tran.py
+        # once the grid size is known)
+        if len(w) < 3:
+            raise DeckError("FILE TRN: SYNTAX ERROR.")
+        if w[2] in ("STEP", "TIME"):
+            if len(w) < 6:
+                raise DeckError(f"FILE TRN {w[2]}: SYNTAX ERROR.")
+            conv = _int if w[2] == "STEP" else _num
+            a, b, c = conv(w[3]), conv(w[4]), conv(w[5])
+            if a > b or c <= 0:
+                raise DeckError(f"FILE TRN {w[2]}: INVALID VALUE.")
+            d.trn = (w[2], a, b, c)
+        else:
+            d.trn_probes.append(tuple(w[2:]))
     elif w[1] == "GRP":
         # vf_iifile.f:148-207
         if len(w) < 3:

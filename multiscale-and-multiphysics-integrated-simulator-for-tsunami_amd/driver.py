@@ -9,9 +9,10 @@ per iteration:
   - the next step size (TIME CONST, or VF_CDTCAL on the device);
   - the end test (NNOW >= NEND or TNOW + DT/2 >= TEND);
   - the outputs of the state so far: the list-file lines (VF_CDIV00 for
-    !VD!) and, where FILE RSL / FILE GRP select the step, a record in
-    <name>.rsl (VF_OR1TRN) and <name>.grp (VF_OG1TRN), both by the step /
-    time rule of vf_or1trn.f:62-78;
+    !VD!) and, where FILE RSL / GRP / TRN select the step, a record in
+    <name>.rsl (VF_OR1TRN), <name>.grp (VF_OG1TRN) and a line of
+    <name>.tran (VF_OT1TRN), all by the step / time rule of
+    vf_or1trn.f:62-78;
   - the step (cad_step).
 """
 from pathlib import Path
@@ -21,6 +22,7 @@ from . import grp as grpmod
 from . import layout as layoutmod
 from . import rsl as rslmod
 from . import timeloop
+from . import tran as tranmod
 
 
 class _RslRule:
@@ -76,13 +78,17 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
             nnow = 0
         rule = _RslRule(d.rsl, restart)
         grule = _RslRule(d.grp, restart)
-        writer = gwriter = None
+        trule = _RslRule(d.trn, restart)
+        writer = gwriter = twriter = None
         if d.rsl is not None:
             writer = rslmod.Writer(workdir / f"{name}.rsl", lay.shape, lay.numb, static["GGV"],
                                    idrop=d.idrop)
         if d.grp is not None:
             gwriter = grpmod.Writer(workdir / f"{name}.grp", static["XX"], static["YY"], static["ZZ"],
                                     static["INDB"], static["GGV"], d.grp_area)
+        if d.trn is not None and d.trn_probes:
+            probes = [tranmod.parse(w, lay.shape) for w in d.trn_probes]
+            twriter = tranmod.Writer(workdir / f"{name}.tran", probes, static, d.wvlvl)
         out = open(workdir / f"{name}.list", "w")
         done = 0
         try:
@@ -96,8 +102,11 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
                                          info["cgbnrm"], info["cgxnrm"], info["icgitr"])
                 out.write(a + "\n" + b + "\n")
                 st = None
-                if gwriter is not None and grule(nnow, tnow, dt):
+                if twriter is not None and trule(nnow, tnow, dt):
                     st = ctx.download_state()
+                    twriter.write_step(tnow, st)
+                if gwriter is not None and grule(nnow, tnow, dt):
+                    st = st if st is not None else ctx.download_state()
                     gwriter.write_step(nnow, tnow, st)
                 if writer is not None and rule(nnow, tnow, dt):
                     st = st if st is not None else ctx.download_state()
@@ -113,7 +122,7 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
                 done += 1
         finally:
             out.close()
-            for w in (writer, gwriter):
+            for w in (writer, gwriter, twriter):
                 if w is not None:
                     w.close()
         par = ctx.get_params()

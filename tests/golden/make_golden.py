@@ -81,7 +81,16 @@ _KEEP = {
 # graphics file (FILE GRP, Source code/vf_iifile.f:148-238); the files' bytes
 # are kept as <case>_rsl.npz / <case>_grp.npz
 _RSL = {"tiny": "FILE RSL STEP 0 5 1\nFILE GRP STEP 0 5 1",
-        "tdoor": "FILE RSL STEP 100 300 100\nFILE GRP STEP 100 300 100\nFILE GRP AREA 4 2 3 18 4 14"}
+        "tdoor": "FILE RSL STEP 100 300 100\nFILE GRP STEP 100 300 100\nFILE GRP AREA 4 2 3 18 4 14\n"
+                 + "\n".join("FILE TRN " + t for t in (
+                     "STEP 0 300 10", "W-LEVEL 5 3", "W-LEVEL 11 2", "POINT U 5 3 2", "POINT V 3 2 2",
+                     "POINT W 3 3 2", "POINT P 8 3 2", "POINT F 11 3 3", "MAX U 1 1 1 20 5 16",
+                     "MIN P 1 1 1 20 5 16", "AV F 1 1 1 11 5 8", "INT P 1 1 1 11 5 8",
+                     "AV W 1 1 1 20 5 16", "INT U 2 1 1 10 5 6", "MIN F 1 1 1 20 5 16",
+                     "FORCE X- 1 1 1 20 5 16", "FORCE X+ 1 1 1 20 5 16", "FORCE Z- 1 1 1 20 5 16",
+                     "FORCE Z+ 1 1 1 20 5 16", "FORCE Y- 1 1 1 20 5 16", "FORCE Y+ 1 1 1 20 5 16"))}
+# result files kept from those runs (the .tran only where FILE TRN is set)
+_OUT = {"tiny": ("rsl", "grp"), "tdoor": ("rsl", "grp", "tran")}
 
 _STEP_RE = re.compile(
     r"STEP=\s*(\d+)\s*:\s*TIME=\s*(\S+)\s*:\s*DT\s*=\s*(\S+)\s*:\s*FSUM=\s*(\S+)\s*:\s*FCUT=\s*(\S+)"
@@ -106,7 +115,7 @@ def run_case(name, factory, keep_steps):
         lines = listing.splitlines()
         list_lines = [a + "\n" + b for a, b in zip(lines, lines[1:]) if a.startswith(" STEP=")]
         if name in _RSL:
-            for ext in ("rsl", "grp"):
+            for ext in _OUT[name]:
                 raw = (td / f"{name}.{ext}").read_bytes()
                 np.savez_compressed(HERE / f"{name}_{ext}.npz",
                                     **{ext.upper(): np.frombuffer(raw, dtype=np.uint8)})

@@ -75,3 +75,24 @@ def test_restart_from_reference_result_file(pkg, tmp_path):
         assert_close(f"step 210 {k}", a[k], ref[k], 1e-8)
     assert_same_int("step 210 NF", a["NF"], ref["NF"])
     assert np.array_equal(a["TBUB"], ref["TBUB"])
+
+
+def test_run_time_series_against_reference(pkg, tmp_path):
+    """The droplet case (300 steps, bubbles and droplets on) run from its deck:
+    the .tran it writes has the reference's probe list and, every 10 steps,
+    probe values within 1e-8 of each probe's scale."""
+    case = "tdoor"
+    deck = str(fx.static(case)["DECK"])
+    res = pkg.driver.run(deck, "td", tmp_path)
+    assert res["nnow"] == 300
+    with np.load(fx.HERE / f"{case}_tran.npz") as z:
+        ref = z["TRAN"].tobytes().decode().splitlines()
+    mine = (tmp_path / "td.tran").read_text().splitlines()
+    nprobe = sum(1 for ln in ref if ln.split() and ln.split()[0].isdigit())
+    assert mine[: nprobe + 1] == ref[: nprobe + 1]
+    a = np.array([[float(x) for x in ln.split()] for ln in mine[nprobe + 1:]])
+    b = np.array([[float(x) for x in ln.split()] for ln in ref[nprobe + 1:]])
+    assert a.shape == b.shape == (31, nprobe + 1)
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-30)
+    err = (np.abs(a - b) / scale).max()
+    assert err <= 1e-8, err

@@ -131,3 +131,32 @@ def test_graphics_file_matches_reference(pkg, case, steps):
         mine = out.read_bytes()
     assert len(mine) == len(raw)
     assert mine == raw
+
+
+def test_time_series_matches_reference(pkg):
+    """tran.py from the recorded states reproduces the reference's .tran:
+    the probe list, the column line and the value lines of every recorded
+    step that FILE TRN STEP 0 300 10 selects (water levels, point values,
+    box min/max/mean/integral, pressure forces)."""
+    case = "tdoor"
+    with np.load(fx.HERE / f"{case}_tran.npz") as z:
+        text = z["TRAN"].tobytes().decode()
+    ref_lines = text.splitlines(keepends=True)
+    st = fx.static(case)
+    d = pkg.deck.parse(str(st["DECK"]))
+    shape = st["GGV"].shape
+    probes = [pkg.tran.parse(w, shape) for w in d.trn_probes]
+    hdr = pkg.tran.header(probes)
+    assert hdr == "".join(ref_lines[: len(probes) + 1])
+    body = ref_lines[len(probes) + 1:]
+    tr = fx.trajectory(case)
+    checked = 0
+    for s in fx.steps(case):
+        if s % 10:
+            continue
+        rec = _after_step(case, s)
+        vals = pkg.tran.values(probes, st, rec, d.wvlvl)
+        got = pkg.tran.line(float(tr["TNOW_FULL"][s]), vals)
+        assert got == body[s // 10], (s, got, body[s // 10])
+        checked += 1
+    assert checked >= 6
