@@ -94,5 +94,15 @@ def test_run_time_series_against_reference(pkg, tmp_path):
     b = np.array([[float(x) for x in ln.split()] for ln in ref[nprobe + 1:]])
     assert a.shape == b.shape == (31, nprobe + 1)
     scale = np.maximum(np.abs(b).max(axis=0), 1e-30)
-    err = (np.abs(a - b) / scale).max()
-    assert err <= 1e-8, err
+    # velocity-valued probes (point, min, max, mean of U/V/W) against the
+    # velocity scale: a component that is round-off in this y-symmetric
+    # flow is judged against the flow
+    d = pkg.deck.parse(deck)
+    vel = [n + 1 for n, w in enumerate(d.trn_probes)
+           if w[0] in ("POINT", "MIN", "MAX", "AV") and w[1] in ("U", "V", "W")]
+    scale[vel] = np.abs(b[:, vel]).max()
+    col = (np.abs(a - b) / scale).max(axis=0)
+    worst = int(np.argmax(col))
+    row = int(np.argmax(np.abs(a[:, worst] - b[:, worst])))
+    assert col.max() <= 1e-8, (list(np.round(col, 12)), worst, row, a[row, worst], b[row, worst],
+                               mine[nprobe + 1 + row][:40])
