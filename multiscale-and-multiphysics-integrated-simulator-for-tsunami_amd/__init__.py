@@ -12,6 +12,7 @@ from . import deck  # noqa: F401
 from . import driver  # noqa: F401
 from . import grp  # noqa: F401
 from . import layout  # noqa: F401
+from . import mtb  # noqa: F401
 from . import rsl  # noqa: F401
 from . import timeloop  # noqa: F401
 from . import tran  # noqa: F401

@@ -303,6 +303,49 @@ def subloop(loops=3, steps=40, name=None):
     return Case(c.name, deck, c.ni, c.nj, c.nk, steps, c.dt)
 
 
+def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81):
+    """A LEVEL-ON wave-maker table (.mtb, Source code/vf_im1inp.f) of one
+    solitary wave.  The reference walks the table phase as frac(T0 - t/T),
+    so with T0 = 0 the phase falls from 1 and time t reads phase 1 - t/T.
+    Level eta = H sech^2(sqrt(3H/4h^3) c (t - t_crest)), c = sqrt(g(h+H));
+    the velocity normal to the boundary is sqrt(g/h) eta below the surface
+    and 0 above it, the tangential (vertical) velocity 0."""
+    kk = np.sqrt(3.0 * H / (4.0 * h ** 3))
+    c = np.sqrt(g * (h + H))
+    ztop = H * 1.5 if ztop is None else ztop
+    zs = np.linspace(-h, ztop, nz)
+    # the reader takes at most 256 characters of a line (MAXCHR): one value
+    # or one (normal, tangential) pair per line
+    lines = ["LEVEL-ON", str(nz), str(nt), "0.0"] + [f"{z:.8e}" for z in zs]
+    for i in range(nt):
+        ph = i / (nt - 1)
+        t = (1.0 - ph) * period
+        eta = H / np.cosh(kk * c * (t - t_crest)) ** 2
+        u = np.sqrt(g / h) * eta
+        lines.append(f"T {ph:.8e} {eta:.8e}")
+        lines += [f"{(u if z <= eta else 0.0):.8e} 0.0" for z in zs]
+    return "\n".join(lines) + "\n"
+
+
+def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400, dt=1.0e-3,
+               eps_r=1.0e-10, name=None):
+    """A flume with a table wave-maker on its x- side (MODEL WAVE-BC X- FUNC
+    MATRIX with <name>.mtb, vf_bwuwn.f:257-480): a solitary wave enters the
+    still water (MATE W-LEVEL) and runs to the far wall.  The table goes to
+    meta["files"]["mtb"]."""
+    lines = [f"# wave flume {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE",
+              "MODEL WAVE-BC X- FUNC MATRIX", f"MODEL WAVE-BC X- DEPTH {h}",
+              f"MODEL WAVE-BC X- PERIOD {period}"]
+    _common(lines, dt, steps, eps_r, wlevel=h)
+    case = Case(name or f"wave_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+    case.meta["files"] = {"mtb": solitary_mtb(h, H, period, t_crest=0.2)}
+    return case
+
+
 def named(name):
     """The workloads by name (sizes as in the benchmark table)."""
     table = {
@@ -316,5 +359,6 @@ def named(name):
         "tdoor_dam": lambda: tdoor_dam(),
         "auto_dam": lambda: auto_dam(),
         "subloop": lambda: subloop(),
+        "wave_flume": lambda: wave_flume(),
     }
     return table[name]()

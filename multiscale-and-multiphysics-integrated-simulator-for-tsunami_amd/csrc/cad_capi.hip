@@ -318,9 +318,9 @@ int cad_upload_static(cad_ctx *c, const double *xx, const double *yy, const doub
     // boundary types this build implements (vf_bwuwn.f, vf_bwuwt.f, vf_bwff.f)
     for (size_t l = 0; l < nb; l++) {
         const int tv = indb[4 * l + 2], tf = indb[4 * l + 3];
-        if (!(tv == 1 || tv == 2 || tv == 3 || tv == 4 || tv == 7) || !(tf == 1 || tf == 2)) {
+        if (!(tv == 1 || tv == 2 || tv == 3 || tv == 4) || !(tf == 1 || tf == 2)) {
             cad_set_error("boundary face %zu: velocity type %d / F type %d not implemented "
-                          "(wave-maker, log-law boundaries)", l + 1, tv, tf);
+                          "(wave-maker, log-law, radiation boundaries)", l + 1, tv, tf);
             return CAD_ERR_UNSUPPORTED;
         }
         if (indb[4 * l] < 1 || (long long)indb[4 * l] > c->d.nc || indb[4 * l + 1] < 1 ||

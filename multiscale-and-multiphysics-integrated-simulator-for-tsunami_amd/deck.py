@@ -19,11 +19,63 @@ class UnsupportedDeck(DeckError):
     pass
 
 
+def zstoi(text):
+    """VF_ZSTOI (Source code/vf_zstoi.f): optional sign, then decimal digits
+    up to the first other character."""
+    t = text.lstrip(" ")
+    sg = 1
+    if t[:1] == "+":
+        t = t[1:]
+    elif t[:1] == "-":
+        sg, t = -1, t[1:]
+    v = 0
+    for ch in t:
+        if not ch.isdigit():
+            break
+        v = 10 * v + int(ch)
+    return sg * v
+
+
+def zstor(text):
+    """VF_ZSTOR (Source code/vf_zstor.f), the reference's reading of a real
+    number, restated operation by operation: the digits accumulate as
+    VAL = 10*VAL + d, the decimals count into PW, VAL = SG*(VAL/PW), then an
+    exponent multiplies by 10.0D0**I.  It is not the correctly rounded
+    decimal conversion: "2.30101890e-01" is read as fl(2.3010189)*fl(0.1),
+    so the deck and table values must be read this way for the reference's
+    bits."""
+    t = text.lstrip(" ")
+    if not t:
+        return 0.0
+    val, sg, pw = 0.0, 1.0, 1.0
+    if t[0] == "+":
+        t = t[1:]
+    elif t[0] == "-":
+        sg, t = -1.0, t[1:]
+    point = False
+    expo = None
+    for n, ch in enumerate(t):
+        if ch.isdigit():
+            val = 10.0 * val + float(ch)
+            if point:
+                pw = pw * 10.0
+        elif ch == "." and not point:
+            point = True
+        elif ch in "DdEe":
+            expo = t[n + 1:]
+            break
+        else:
+            break
+    val = sg * (val / pw)
+    if expo:
+        val = val * (10.0 ** zstoi(expo))
+    return val
+
+
 def _num(w):
-    try:
-        return float(w.replace("D", "E").replace("d", "e"))
-    except ValueError as e:
-        raise DeckError(f"not a number: {w!r}") from e
+    if not w or not (w.lstrip("+-")[:1].isdigit() or w.lstrip("+-")[:1] == "."):
+        raise DeckError(f"not a number: {w!r}")
+    return zstor(w)
 
 
 def _int(w):
@@ -87,6 +139,9 @@ class Deck:
     # FILE GRP STEP/TIME and AREA (cell indices +1 like IGRARA)
     grp: Optional[tuple] = None
     grp_area: Optional[tuple] = None
+    # MODEL WAVE-BC per side (1 X-, 2 X+, 3 Y-, 4 Y+): func, depth, height,
+    # period, ampl, area (+1 like IBCTYP(3..4)), angle
+    waves: dict = field(default_factory=dict)
     # FILE TRN STEP/TIME and its probe lines (words after FILE TRN)
     trn: Optional[tuple] = None
     trn_probes: List[tuple] = field(default_factory=list)
@@ -200,7 +255,7 @@ def parse(text):
         elif head == "PARALLEL":
             raise UnsupportedDeck("PARALLEL: one device runs the whole grid")
         elif head == "MODEL":
-            raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (wave makers, damping) is not implemented")
+            _parse_model(d, w)
         else:
             raise DeckError(f"unknown header {head}")
     for a in "XYZ":
@@ -273,6 +328,48 @@ def _parse_comp(d, w):
             raise DeckError(f"COMP MTRX {w[2]}")
     else:
         raise DeckError(f"COMP {w[1]}")
+
+
+_WAVE_SIDE = {"X-": 1, "X+": 2, "Y-": 3, "Y+": 4}
+
+
+def _parse_model(d, w):
+    """MODEL WAVE-BC (Source code/vf_iimdl.f:33-115).  Only the matrix-table
+    wave-maker (FUNC MATRIX with a <name>.mtb table) is produced here."""
+    if len(w) < 2 or w[1] != "WAVE-BC":
+        raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (open boundaries, damping, "
+                              "wave sources) is not implemented")
+    if len(w) < 4:
+        raise DeckError("MODEL WAVE-BC: SYNTAX ERROR.")
+    if w[2] not in _WAVE_SIDE:
+        raise DeckError(f"MODEL WAVE-BC {w[2]}: UNKNOWN WORD.")
+    wv = d.waves.setdefault(_WAVE_SIDE[w[2]], dict(func=None, depth=0.0, height=0.0, period=0.0,
+                                                   ampl=0.0, area=None, angle=0.0))
+    key = w[3]
+    if key == "FUNC":
+        if len(w) < 5:
+            raise DeckError("MODEL WAVE-BC FUNC: SYNTAX ERROR.")
+        if w[4] != "MATRIX":
+            raise UnsupportedDeck(f"MODEL WAVE-BC FUNC {w[4]}: only the matrix table (MATRIX) "
+                                  "is implemented")
+        wv["func"] = "MATRIX"
+    elif key in ("DEPTH", "HEIGHT", "PERIOD", "AMPL"):
+        if len(w) < 5:
+            raise DeckError(f"MODEL WAVE-BC {key}: SYNTAX ERROR.")
+        v = _num(w[4])
+        if key != "AMPL" and v < 1.0e-20:
+            raise DeckError(f"MODEL WAVE-BC {key}: INVALID VALUE.")
+        wv[key.lower()] = v
+    elif key == "AREA":
+        if len(w) < 6:
+            raise DeckError("MODEL WAVE-BC AREA: SYNTAX ERROR.")
+        wv["area"] = (_int(w[4]) + 1, _int(w[5]) + 1)
+    elif key == "ANGLE":
+        if len(w) < 7:
+            raise DeckError("MODEL WAVE-BC ANGLE: SYNTAX ERROR.")
+        wv["angle"] = _num(w[4])
+    else:
+        raise DeckError(f"MODEL WAVE-BC {key}: UNKNOWN WORD.")
 
 
 def _parse_file(d, w):
@@ -409,3 +506,13 @@ def _parse_option(d, w):
 def _validate(d):
     if d.loops < 1:
         raise DeckError("OPTION SUB-LOOP: INVALID VALUE.")
+    # vf_csetup.f:37-95: a table wave-maker needs DEPTH and PERIOD, no ANGLE
+    for jd, wv in d.waves.items():
+        if wv["func"] != "MATRIX":
+            raise DeckError("MODEL WAVE-BC: FUNC MATRIX missing")
+        if wv["depth"] < 1.0e-20:
+            raise DeckError("MODEL WAVE-BC: (WAVE-BC DEPTH) NOT FOUND.")
+        if wv["period"] < 1.0e-20:
+            raise DeckError("MODEL WAVE-BC: (WAVE-BC PERIOD) NOT FOUND.")
+        if abs(wv["angle"]) > 1.0e-20:
+            raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")

@@ -268,6 +268,161 @@ static void veuler(orc_state *s, int a, double *const fl[3], const double *qsrc)
 }
 
 /* Normal velocity on boundary faces.  vf_bwuwn.f:84-254 (types 1-4, 6-8). */
+/* the reference's PI (Source code/VF_A0PRM.h:23) */
+static const double PI_REF = 3.141592653589794;
+
+/* VF_CWMTB1 (vf_cwmtb1.f): time weights of the table at phase wvt, searching
+ * down from MTBNOW first, then down from the top; the level there. */
+static int mtb_time(orc_state *s, double wvt, double *w1o, double *w2o, double *wvz)
+{
+    const int nt = s->mtbtt;
+    const double *tt = s->mtb_tt - 1;   /* 1-based */
+    double w1 = 0.0, w2 = 0.0;
+    if (s->mtbnow <= 0) s->mtbnow = nt - 1;
+    if (wvt <= tt[1]) {
+        s->mtbnow = 1;
+        w1 = 1.0;
+        w2 = 0.0;
+    } else if (wvt >= tt[nt]) {
+        s->mtbnow = nt - 1;
+        w1 = 0.0;
+        w2 = 1.0;
+    } else {
+        int ip = 0;
+        for (int i = s->mtbnow; i >= 1 && !ip; i--)
+            if (tt[i] <= wvt && wvt <= tt[i + 1]) {
+                ip = i;
+                w2 = 1.0 / (tt[i + 1] - tt[i]);
+                w1 = (tt[i + 1] - wvt) * w2;
+                w2 = (wvt - tt[i]) * w2;
+            }
+        for (int i = nt - 1; i >= s->mtbnow + 1 && !ip; i--)
+            if (tt[i] <= wvt && wvt <= tt[i + 1]) {
+                ip = i;
+                w2 = 1.0 / (tt[i + 1] - tt[i]);
+                w1 = (tt[i + 1] - wvt) * w2;
+                w2 = (wvt - tt[i]) * w2;
+            }
+        if (!ip) return -13;
+        s->mtbnow = ip;
+    }
+    *wvz = w1 * s->mtb_hh[s->mtbnow - 1] + w2 * s->mtb_hh[s->mtbnow];
+    *w1o = w1;
+    *w2o = w2;
+    return 0;
+}
+
+/* VF_CWMTB2 (vf_cwmtb2.f): normal and tangential velocity at height zc,
+ * searching up from km, between the two phases of the time weights. */
+static int mtb_depth(const orc_state *s, int *km, double wt1, double wt2, double zc, double *un,
+                     double *ut)
+{
+    const int nz = s->mtbzz;
+    const double *zz = s->mtb_zz - 1;
+    double w1 = 0.0, w2 = 0.0;
+    if (zc <= zz[1]) {
+        *km = 1;
+        w1 = 1.0;
+        w2 = 0.0;
+    } else if (zc >= zz[nz]) {
+        *km = nz - 1;
+        w1 = 0.0;
+        w2 = 1.0;
+    } else {
+        int kp = 0;
+        for (int k = *km; k <= nz - 1 && !kp; k++)
+            if (zz[k] <= zc && zc <= zz[k + 1]) {
+                kp = k;
+                w2 = 1.0 / (zz[k + 1] - zz[k]);
+                w1 = (zz[k + 1] - zc) * w2;
+                w2 = (zc - zz[k]) * w2;
+            }
+        if (!kp) return -14;
+        *km = kp;
+    }
+    const int km1 = *km + 1, im = s->mtbnow, im1 = s->mtbnow + 1;
+#define TAB(a, k, i) (a)[(size_t)((k) - 1) + (size_t)nz * (size_t)((i) - 1)]
+    double p1 = w1 * TAB(s->mtb_un, *km, im) + w2 * TAB(s->mtb_un, km1, im);
+    double p2 = w1 * TAB(s->mtb_un, *km, im1) + w2 * TAB(s->mtb_un, km1, im1);
+    *un = wt1 * p1 + wt2 * p2;
+    p1 = w1 * TAB(s->mtb_ut, *km, im) + w2 * TAB(s->mtb_ut, km1, im);
+    p2 = w1 * TAB(s->mtb_ut, *km, im1) + w2 * TAB(s->mtb_ut, km1, im1);
+    *ut = wt1 * p1 + wt2 * p2;
+#undef TAB
+    return 0;
+}
+
+/* The table wave-maker sides of VF_BWUWN (vf_bwuwn.f:257-480, FUNC MATRIX
+ * with a LEVEL-ON table, normal incidence): for every boundary column the
+ * table profile, scaled by (level + depth) / (measured level + depth), gives
+ * the normal and tangential velocity of each face. */
+static int wave_bc(orc_state *s)
+{
+    DECL_DIMS(s);
+    for (int jd = 1; jd <= 4; jd++) {
+        if (!s->wv_on[jd - 1]) continue;
+        const double d = s->wv_d[jd - 1], t = s->wv_t[jd - 1], t0 = s->wv_t0[jd - 1];
+        double a = s->wv_a[jd - 1];
+        /* the table phase in single precision, as the reference keeps it */
+        float r4tn = (float)(t0 - s->tnow / t);
+        r4tn = r4tn - (float)(int)r4tn;
+        if (r4tn < 0.0f) r4tn = r4tn + 1.0f;
+        double aw = 1.0;
+        if (a >= ZERO) {
+            a = s->tnow / t / a;
+            if (a < 1.0) aw = 0.5 * sin(PI_REF * (a - 0.5)) + 0.5;
+        }
+        const double wvt = (double)r4tn;
+        double wmt1, wmt2, wvz;
+        int rc = mtb_time(s, wvt, &wmt1, &wmt2, &wvz);
+        if (rc) return rc;
+        s->wv_level[jd - 1] = wvz * aw;
+        const double sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
+        const double sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
+        const int xside = jd <= 2;
+        const int pos = xside ? (jd == 1 ? 2 : ni) : (jd == 3 ? 2 : nj);
+        const int cpos = xside ? (jd == 1 ? 2 : ni - 1) : (jd == 3 ? 2 : nj - 1);
+        const int lim = xside ? nj : ni;
+        for (int q = s->wv_l1[jd - 1]; q <= s->wv_l2[jd - 1]; q++) {
+            if (q < 2 || q > lim - 1) continue;
+            double val = 0.0;
+            for (int k = 2; k <= nk - 1; k++) {
+                const size_t c = xside ? ID(cpos, q, k) : ID(q, cpos, k);
+                const int nw = s->nf[c];
+                if (nw == -1) {
+                    val = val + ZZ(2, k) * 1.0;
+                } else if (nw == 0) {
+                    val = val + ZZ(2, k) * s->ff[c];
+                } else if (nw != 8) {
+                    val = val + ZZ(2, k) * s->ff[c];
+                    break;
+                }
+            }
+            val = val - (s->wvlvl - ZZ(1, 2));
+            const double vws = (wvz + d) / (val + d);
+            int kmt = 1;
+            for (int k = 2; k <= nk - 1; k++) {
+                const size_t f = xside ? ID(pos, q, k) : ID(q, pos, k);
+                const int l = xside ? s->indx[f] : s->indy[f];
+                if (l < 1) continue;
+                double zc = (ZZ(1, k) + ZZ(1, k + 1)) * 0.5 - s->wvlvl;
+                zc = vws * (zc + d) - d;
+                double un, ut;
+                rc = mtb_depth(s, &kmt, wmt1, wmt2, zc, &un, &ut);
+                if (rc) return rc;
+                s->bcu[l - 1] = un * vws * aw * sa;
+                s->bcv[l - 1] = un * vws * aw * sb;
+                s->bcw[l - 1] = ut * vws * aw;
+                if (xside)
+                    s->uu[f] = s->bcu[l - 1];
+                else
+                    s->vv[f] = s->bcv[l - 1];
+            }
+        }
+    }
+    return 0;
+}
+
 static int bwuwn(orc_state *s)
 {
     DECL_DIMS(s);
@@ -299,12 +454,14 @@ static int bwuwn(orc_state *s)
             case 5: val = s->ww[n + nij]; s->bcw[l - 1] = val; s->ww[n] = val; break;
             case 6: val = s->ww[n - nij]; s->bcw[l - 1] = val; s->ww[n] = val; break;
             }
+        } else if (ib == 5) {
+            /* a LEVEL-ON table sets these faces below (vf_bwuwn.f:171-223) */
         } else if (ib == 7) {
         } else {
-            return -10; /* wave-maker boundaries are outside the restated path */
+            return -10;
         }
     }
-    return 0;
+    return wave_bc(s);
 }
 
 int orc_predictor(orc_state *s)
@@ -837,7 +994,9 @@ static void bsuwn(orc_state *s)
 }
 
 /* Tangential wall velocities on slip/free boundaries.  vf_bwuwt.f:42-176
- * (types 1,4; 2,3 leave them; 7 radiation does nothing). */
+ * (types 1,4; 2,3 leave them; 7 radiation does nothing; 5, a wave-maker
+ * face, sets them only for LEVEL-ONLY tables, vf_bwuwt.f:80-116, and the
+ * tables here are LEVEL-ON). */
 static int bwuwt(orc_state *s)
 {
     DECL_DIMS(s);
@@ -856,7 +1015,7 @@ static int bwuwt(orc_state *s)
             case 5: s->bcu[l - 1] = (u[n] + u[n + 1]) * 0.5; s->bcv[l - 1] = (v[n] + v[n + ni]) * 0.5; break;
             case 6: s->bcu[l - 1] = (u[n - nij] + u[n - nij + 1]) * 0.5; s->bcv[l - 1] = (v[n - nij] + v[n - nij + ni]) * 0.5; break;
             }
-        } else if (ib == 2 || ib == 3 || ib == 7) {
+        } else if (ib == 2 || ib == 3 || ib == 5 || ib == 7) {
         } else {
             return -11; /* log-law and wave boundaries are outside the restated path */
         }
@@ -1122,7 +1281,8 @@ static int bwff(orc_state *s)
         int ns = INDB(2, l), ib = INDB(4, l);
         size_t n = ID(i, j, k);
         if (ib == 1) {
-        } else if (ib == 2) {
+        } else if (ib == 2 || (ib == 5 && ns <= 4)) {
+            /* free, and the wave-maker faces (vf_bwff.f:41-76) */
             switch (ns) {
             case 1: s->bcf[l - 1] = s->ff[n]; break;
             case 2: s->bcf[l - 1] = s->ff[n - 1]; break;
@@ -1744,6 +1904,7 @@ double orc_cdiv00(const orc_state *s)
 int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double wini, int flags)
 {
     DECL_DIMS(s);
+    s->wvlvl = wvlvl;
     size_t nc = nij * (size_t)nk;
     fcut01(s);
     int rc = bwff(s);

@@ -56,6 +56,18 @@ typedef struct orc_state {
     int idrop;
     double *tbub;
     double *droptx, *dropty, *droptz, *dropuu, *dropvv, *dropww;
+    /* Table wave-makers, MODEL WAVE-BC FUNC MATRIX with a LEVEL-ON table
+     * (vf_bwuwn.f:257-480): per side JD = 1..4 (x-, x+, y-, y+) whether it
+     * is on, its range L1..L2 (IBCTYP(3..4)), DEPTH, PERIOD, AMPL and the
+     * initial phase T0; the table DMTBZZ(MTBZZ), DMTBTT/DMTBHH(MTBTT),
+     * DMTBUN/DMTBUT(MTBZZ,MTBTT); MTBNOW, the search position the table
+     * lookup keeps between calls; wv_level the side's level (BCTYP(7)).
+     * wvlvl is MATE W-LEVEL. */
+    double wvlvl;
+    int wv_on[4], wv_l1[4], wv_l2[4];
+    double wv_d[4], wv_t[4], wv_a[4], wv_t0[4], wv_level[4];
+    int mtbzz, mtbtt, mtbnow;
+    const double *mtb_zz, *mtb_tt, *mtb_hh, *mtb_un, *mtb_ut;
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */

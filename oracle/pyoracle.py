@@ -42,6 +42,12 @@ class OrcState(C.Structure):
         ("tnow", C.c_double), ("wbub", C.c_double), ("idrop", C.c_int),
         ("tbub", PD), ("droptx", PD), ("dropty", PD), ("droptz", PD),
         ("dropuu", PD), ("dropvv", PD), ("dropww", PD),
+        ("wvlvl", C.c_double),
+        ("wv_on", C.c_int * 4), ("wv_l1", C.c_int * 4), ("wv_l2", C.c_int * 4),
+        ("wv_d", C.c_double * 4), ("wv_t", C.c_double * 4), ("wv_a", C.c_double * 4),
+        ("wv_t0", C.c_double * 4), ("wv_level", C.c_double * 4),
+        ("mtbzz", C.c_int), ("mtbtt", C.c_int), ("mtbnow", C.c_int),
+        ("mtb_zz", PD), ("mtb_tt", PD), ("mtb_hh", PD), ("mtb_un", PD), ("mtb_ut", PD),
     ]
 
 
@@ -156,6 +162,26 @@ class Fields:
         st.nums = nums
         for k, v in self.params.items():
             setattr(st, k, v)
+
+    def set_waves(self, waves, table, wvlvl):
+        """Table wave-makers: waves = deck.waves (side -> DEPTH, PERIOD, AMPL,
+        AREA), table = mtb.Table (LEVEL-ON)."""
+        st = self.st
+        st.wvlvl = wvlvl
+        numi, numj, _ = self.shape
+        for jd, wv in waves.items():
+            q = jd - 1
+            st.wv_on[q] = 1
+            lim = numj if jd <= 2 else numi
+            l1, l2 = wv["area"] if wv["area"] else (2, lim - 1)
+            st.wv_l1[q], st.wv_l2[q] = l1, l2
+            st.wv_d[q], st.wv_t[q], st.wv_a[q] = wv["depth"], wv["period"], wv["ampl"]
+            st.wv_t0[q] = table.t0
+        self.tab = {k: np.asfortranarray(np.array(getattr(table, k), dtype=np.float64))
+                    for k in ("zz", "tt", "hh", "un", "ut")}
+        st.mtbzz, st.mtbtt, st.mtbnow = len(table.zz), len(table.tt), 0
+        st.mtb_zz, st.mtb_tt, st.mtb_hh = (_pd(self.tab[k]) for k in ("zz", "tt", "hh"))
+        st.mtb_un, st.mtb_ut = _pd(self.tab["un"]), _pd(self.tab["ut"])
 
     def __getitem__(self, k):
         return self.a[k]

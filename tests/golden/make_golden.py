@@ -64,6 +64,8 @@ def _cases():
         # automatic time step (vf_cdtcal.f) and OPTION SUB-LOOP
         "auto_dam": (lambda: cfg.auto_dam(name="auto_dam"), [1, 2, 100, 250, 400]),
         "subloop": (lambda: cfg.subloop(name="subloop"), [1, 2, 20, 40]),
+        # table wave-maker (MODEL WAVE-BC FUNC MATRIX, vf_bwuwn.f:257-480)
+        "wave_flume": (lambda: cfg.wave_flume(name="wave_flume"), [1, 2, 100, 200, 300, 400]),
     }
 
 
@@ -74,6 +76,8 @@ _KEEP = {
     "tdoor": {s: _F1_ONLY for s in (2, 100, 101, 145, 170, 171, 180, 195, 200, 210, 220, 230, 245, 300)},
     "tdoor_dam": {s: _F1_ONLY for s in (2, 26, 200, 201, 400, 533, 571, 600, 601, 700, 900)},
     "auto_dam": {s: _F1_ONLY for s in (100, 250, 400)},
+    "wave_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                       "f1_out": None} for s in (100, 200, 300, 400)},
 }
 
 
@@ -105,6 +109,8 @@ def run_case(name, factory, keep_steps):
         td = Path(td)
         (td / "dump").mkdir()
         (td / f"{name}.in").write_text(case.deck)
+        for ext, text in case.meta.get("files", {}).items():
+            (td / f"{name}.{ext}").write_text(text)
         (td / "data.env").write_text(f"1\n{name} 1 0\n")
         env = dict(os.environ, CADMAS_DUMP_DIR=str(td / "dump"),
                    CADMAS_DUMP_STEPS=",".join(str(s) for s in keep_steps))
@@ -145,7 +151,9 @@ def run_case(name, factory, keep_steps):
                             DV_STEPS=np.array(steps_dv), DV_FULL=np.array([dv[s] for s in steps_dv]),
                             DECK=np.array(case.deck), LIST=np.array(list_lines), **extra)
         static = read_dump(td / "dump" / "static.bin")
-        np.savez_compressed(HERE / f"{name}_static.npz", **static, DECK=np.array(case.deck))
+        extra_files = {f"FILE_{k.upper()}": np.array(v) for k, v in case.meta.get("files", {}).items()}
+        np.savez_compressed(HERE / f"{name}_static.npz", **static, DECK=np.array(case.deck),
+                            **extra_files)
         for s in keep_steps:
             rec = {}
             keep = _KEEP.get(name, {}).get(s)

@@ -130,3 +130,24 @@ def test_time_loop_end_rule(pkg):
     d = pkg.deck.parse(base + "TIME CONST 0.1\nTIME END 2 10.0\n")
     recs = list(pkg.timeloop.run(_FakeCtx([]), d))
     assert [r.nnow for r in recs] == [1, 2] and recs[-1].dtnow == 0.1
+
+
+def test_number_reading_follows_the_reference(pkg):
+    """deck.zstor reads numbers the reference's way (vf_zstor.f): digits
+    accumulated in binary, divided by the power of ten of the decimals, then
+    scaled by 10.0**exponent -- not the correctly rounded conversion.  The
+    expected values are what those operations give, worked out here by hand."""
+    z = pkg.deck.zstor
+    # 3*10**-4: fl(3 * fl(1e-4)) is one ulp above the nearest double to 0.0003
+    assert z("3.000000e-04") == 3.0 * 1.0e-4 and z("3.000000e-04") != 0.0003
+    # 0.1 read as 1/10 is the nearest double; 0.15 as 15/100 likewise
+    assert z("0.1") == 0.1 and z("-0.15") == -0.15
+    # digits / 10**decimals, then the exponent (D and E both mark it)
+    assert z("12.5D2") == 1250.0 and z("+7e2") == 700.0
+    assert z("7E-1") == 7.0 * 0.1 and z("7E-1") != 0.7
+    assert z("1.25e-3") == 125.0 / 100.0 * 1.0e-3
+    # a second point or any other character ends the number; "1.0e" has no exponent
+    assert z("1.5.3") == 1.5 and z("2x4") == 2.0 and z("1.0e") == 1.0
+    # leading blanks, an empty field
+    assert z("   4") == 4.0 and z("") == 0.0
+    assert pkg.deck.zstoi("-12x") == -12 and pkg.deck.zstoi("+3") == 3

@@ -271,3 +271,55 @@ def test_cdtcal_is_the_cell_minimum():
         ref = tr["DT_FULL"][s + 1]
         worst = max(worst, abs(dt - ref) / ref)
     assert worst <= 1e-12, worst
+
+
+def _wave_fields(pkg, case, rec, bcf):
+    st = fx.static(case)
+    p = _params(st, rec)
+    F = _fields(st, rec, p, bcf)
+    d = pkg.deck.parse(str(st["DECK"]))
+    F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    return st, p, F
+
+
+def test_wave_maker_phases_bitexact(pkg):
+    """MODEL WAVE-BC X- FUNC MATRIX: V1CAL (the wave-maker sets the boundary
+    velocities inside it) and F1CAL at every recorded step equal the
+    reference's bit for bit."""
+    case = "wave_flume"
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        st, p, F = _wave_fields(pkg, case, r["v1_in"], r["f1_in"]["BCF"])
+        F.st.tnow = float(r["v1_in"]["TNOW"][0])
+        assert F.predictor() == 0
+        for k, kk in (("UU", "US"), ("VV", "VS"), ("WW", "WS")):
+            assert np.array_equal(F[k], r["ps_in"][kk]), (s, k)
+        st, p, F = _wave_fields(pkg, case, r["v1_in"], r["f1_in"]["BCF"])
+        F.st.tnow = float(r["v1_in"]["TNOW"][0])
+        rc, pt = F.v1cal()
+        assert rc == 0
+        for k in ("UU", "VV", "WW", "PP", "BCU", "BCV", "BCW", "BCP"):
+            assert np.array_equal(F[k], r["v1_out"][k]), (s, k)
+        st, p, F = _wave_fields(pkg, case, r["f1_in"], None)
+        assert F.f1cal() == 0
+        for k in ("FF", "NF", "BCF", "UU"):
+            assert np.array_equal(F[k], r["f1_out"][k]), (s, k)
+
+
+def test_wave_maker_free_run_bitexact(pkg):
+    """400 steps of the wave flume from the reference's initial state: ITR and
+    ||div|| every step and the fields at the recorded steps, bit for bit."""
+    case = "wave_flume"
+    r1 = fx.step(case, 1)
+    st, p, F = _wave_fields(pkg, case, r1["v1_in"], r1["f1_in"]["BCF"])
+    tr = fx.trajectory(case)
+    kept = set(fx.steps(case))
+    for s in range(1, int(tr["STEP"][-1]) + 1):
+        rc, _ = F.step()
+        assert rc == 0, s
+        assert F.st.icgitr == tr["ITR"][s], s
+        assert F.cdiv00() == tr["DV_FULL"][s], s
+        if s in kept:
+            ref = fx.step(case, s)["f1_out"]
+            for k in ("UU", "VV", "WW", "PP", "FF", "NF", "BCU", "BCW"):
+                assert np.array_equal(F[k], ref[k]), (s, k)
