@@ -17,6 +17,7 @@ module cadmas_hip
   integer(c_int), parameter :: CAD_ERR_INSTABILITY = -31
   integer(c_int), parameter :: CAD_ERR_DECOMPOSITION = -32
   integer(c_int), parameter :: CAD_ERR_ORIENTATION = -40
+  integer(c_int), parameter :: CAD_ERR_WAVE = -50
   integer(c_int), parameter :: CAD_PRECISION_FP64 = 0
   integer(c_int), parameter :: CAD_INIT_VELOCITY = 1
   integer(c_int), parameter :: CAD_INIT_PRESSURE = 2
@@ -160,6 +161,37 @@ module cadmas_hip
       real(c_double), value :: dtinit, dtsafe, dtmin, dtmax
       real(c_double), intent(out) :: dt
       integer(c_int) :: cad_cdtcal
+    end function
+
+    ! the LEVEL-ON table of <name>.mtb (VF_IM1INP): ZZ(NZ), TT(NT), HH(NT),
+    ! UN(NZ,NT), UT(NZ,NT) as the reference's DMTB* arrays, and DMTBT0
+    function cad_set_wave_table(ctx, nz, nt, zz, tt, hh, un, ut, t0) &
+        bind(c, name='cad_set_wave_table')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nz, nt
+      real(c_double), intent(in) :: zz(*), tt(*), hh(*), un(*), ut(*)
+      real(c_double), value :: t0
+      integer(c_int) :: cad_set_wave_table
+    end function
+
+    ! MODEL WAVE-BC side (1..4 = X-, X+, Y-, Y+) FUNC MATRIX: columns l1..l2
+    ! (IBCTYP(3..4,side)), DEPTH, PERIOD, AMPL, and W-LEVEL
+    function cad_set_wave(ctx, side, l1, l2, depth, period, ampl, wvlvl) &
+        bind(c, name='cad_set_wave')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: side, l1, l2
+      real(c_double), value :: depth, period, ampl, wvlvl
+      integer(c_int) :: cad_set_wave
+    end function
+
+    function cad_get_wave_level(ctx, side, level) bind(c, name='cad_get_wave_level')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: side
+      real(c_double), intent(out) :: level
+      integer(c_int) :: cad_get_wave_level
     end function
 
     function cad_cdiv00(ctx, dv) bind(c, name='cad_cdiv00')

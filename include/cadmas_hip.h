@@ -31,6 +31,7 @@ extern "C" {
 #define CAD_ERR_INSTABILITY (-31)     /* VF_VPSOL 'INSTABILITY.'    vf_vpsol.f:73 */
 #define CAD_ERR_DECOMPOSITION (-32)   /* VF_VPSOL 'CAN NOT DECOMPOSITION.' :75 */
 #define CAD_ERR_ORIENTATION (-40)     /* VF_FNFPRV 'P.G ERROR.'     vf_fnfprv.f:223 */
+#define CAD_ERR_WAVE (-50)            /* VF_CWMTB1/2 'P.G ERROR.'   vf_cwmtb1.f, vf_cwmtb2.f */
 
 #define CAD_PRECISION_FP64 0          /* plain form: all FP64 */
 #define CAD_PRECISION_MIXED 1         /* FP32 stencils/MILU/momentum, FP64 dots */
@@ -113,6 +114,28 @@ int cad_upload_tdoor(cad_ctx *ctx, const double *tbub, const double *droptx, con
                      const double *dropww);
 int cad_download_tdoor(cad_ctx *ctx, double *tbub, double *droptx, double *dropty, double *droptz,
                        double *dropuu, double *dropvv, double *dropww);
+
+/* Matrix-table wave-maker (MODEL WAVE-BC <side> FUNC MATRIX, vf_iimdl.f:33-115,
+ * applied in VF_BWUWN, vf_bwuwn.f:257-480) -------------------------------- */
+/* The LEVEL-ON table of <name>.mtb as VF_IM1INP reads it (vf_im1inp.f): NZ
+ * increasing heights ZZ (relative to the still-water level), NT increasing
+ * times TT (fractions of a period), the level HH(NT), the normal and
+ * tangential velocities UN(NZ,NT), UT(NZ,NT) in Fortran order, and the phase
+ * origin T0.  Resets the time search start (MTBNOW) and clears the sides
+ * set before (cad_set_wave comes after the table). */
+int cad_set_wave_table(cad_ctx *ctx, int nz, int nt, const double *zz, const double *tt,
+                       const double *hh, const double *un, const double *ut, double t0);
+/* One wave-maker side: 1..4 = X-, X+, Y-, Y+; columns l1..l2 along it
+ * (1-based with ghosts, within 2..NUMJ-1 or 2..NUMI-1), water depth, period,
+ * the ramp length in periods (AMPL; below 1e-20 no ramp) and the still-water
+ * level (W-LEVEL).  The side's faces carry boundary type 5 in INDB (velocity
+ * and F); every VF_BWUWN from then on (the predictor, the initial state, the
+ * sub-loop passes) sets their velocities at the context's TNOW. */
+int cad_set_wave(cad_ctx *ctx, int side, int l1, int l2, double depth, double period,
+                 double ampl, double wvlvl);
+/* The wave-maker level of a side at the last VF_BWUWN (table level times the
+ * ramp; the W-LEVEL ANS probe of FILE TRN, vf_ot1trn.f). */
+int cad_get_wave_level(cad_ctx *ctx, int side, double *level);
 
 /* Whole phases ------------------------------------------------------------ */
 /* VF_V1CAL (vf_v1cal.f:1-297): predictor, Poisson, correction, boundary values. */

@@ -79,6 +79,9 @@ def lib():
     L.cad_upload_tdoor.argtypes = [C.c_void_p] + [PD] * 7
     L.cad_cdtcal.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, PD]
     L.cad_download_tdoor.argtypes = [C.c_void_p] + [PD] * 7
+    L.cad_set_wave_table.argtypes = [C.c_void_p, C.c_int, C.c_int] + [PD] * 5 + [C.c_double]
+    L.cad_set_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
+    L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
     L.cad_upload_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
     L.cad_download_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
@@ -107,7 +110,8 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
-           "cad_download_tdoor", "cad_cdtcal")
+           "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
+           "cad_get_wave_level")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -209,6 +213,27 @@ class Context:
         out.update({k: np.zeros(self.shape, order="F") for k in DROP})
         _check(lib().cad_download_tdoor(self._h, _pd(out["TBUB"]), *[_pd(out[k]) for k in DROP]))
         return out
+
+    def set_waves(self, waves, table, wvlvl):
+        """Matrix-table wave-makers: waves = deck.waves (side -> depth,
+        period, ampl, area), table = mtb.Table (LEVEL-ON)."""
+        zz, tt, hh = (_f64(getattr(table, k)) for k in ("zz", "tt", "hh"))
+        nz, nt = len(zz), len(tt)
+        un, ut = _f64(table.un, (nz, nt)), _f64(table.ut, (nz, nt))
+        _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), _pd(hh), _pd(un),
+                                        _pd(ut), float(table.t0)))
+        ni, nj, _ = self.shape
+        for side, wv in sorted(waves.items()):
+            lim = nj if side <= 2 else ni
+            l1, l2 = wv["area"] if wv["area"] else (2, lim - 1)
+            _check(lib().cad_set_wave(self._h, int(side), int(l1), int(l2), float(wv["depth"]),
+                                      float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
+
+    def wave_level(self, side):
+        """The wave-maker level of a side at the last boundary pass."""
+        v = C.c_double()
+        _check(lib().cad_get_wave_level(self._h, int(side), C.byref(v)))
+        return v.value
 
     def upload_static(self, a):
         s, nb = self.shape, self.numb

@@ -220,6 +220,7 @@ void cad_destroy(cad_ctx *c)
         if (p) hipFree(p);
     cad_sweep_free(c);
     cad_tdoor_free(c);
+    cad_wave_free(c);
     delete[] c->zz_host;
     if (c->host_red) hipHostFree(c->host_red);
     if (c->host_cnt) hipHostFree(c->host_cnt);
@@ -318,9 +319,11 @@ int cad_upload_static(cad_ctx *c, const double *xx, const double *yy, const doub
     // boundary types this build implements (vf_bwuwn.f, vf_bwuwt.f, vf_bwff.f)
     for (size_t l = 0; l < nb; l++) {
         const int tv = indb[4 * l + 2], tf = indb[4 * l + 3];
-        if (!(tv == 1 || tv == 2 || tv == 3 || tv == 4) || !(tf == 1 || tf == 2)) {
+        const bool wave = tv == 5 && tf == 5 && indb[4 * l + 1] <= 4;
+        if (!wave && (!(tv == 1 || tv == 2 || tv == 3 || tv == 4) || !(tf == 1 || tf == 2))) {
             cad_set_error("boundary face %zu: velocity type %d / F type %d not implemented "
-                          "(wave-maker, log-law, radiation boundaries)", l + 1, tv, tf);
+                          "(log-law, radiation boundaries; wave-maker faces are type 5 for "
+                          "both on an x or y side)", l + 1, tv, tf);
             return CAD_ERR_UNSUPPORTED;
         }
         if (indb[4 * l] < 1 || (long long)indb[4 * l] > c->d.nc || indb[4 * l + 1] < 1 ||
@@ -345,6 +348,8 @@ int cad_upload_static(cad_ctx *c, const double *xx, const double *yy, const doub
             }
         }
     }
+    c->n_wave_faces = 0;
+    for (size_t l = 0; l < nb; l++) c->n_wave_faces += indb[4 * l + 2] == 5;
     RET_IF(h2d(c, c->xx, xx, 6 * (size_t)c->d.ni));
     RET_IF(h2d(c, c->yy, yy, 6 * (size_t)c->d.nj));
     RET_IF(h2d(c, c->zz, zz, 6 * (size_t)c->d.nk));
@@ -455,6 +460,11 @@ static int ready(cad_ctx *c)
         cad_set_error("set-up arrays and state must be uploaded first");
         return CAD_ERR_STATE;
     }
+    if (c->n_wave_faces > 0 && !(c->wv_on[0] || c->wv_on[1] || c->wv_on[2] || c->wv_on[3])) {
+        cad_set_error("%d wave-maker faces (type 5) but no wave-maker side: call "
+                      "cad_set_wave_table and cad_set_wave", c->n_wave_faces);
+        return CAD_ERR_STATE;
+    }
     CAD_CHECK(hipSetDevice(c->device));
     return CAD_OK;
 }
@@ -465,7 +475,7 @@ int cad_predictor(cad_ctx *c)
     RET_IF(cad_launch_predictor(c));
     RET_IF(cad_launch_bwuwn(c));
     CAD_CHECK(hipStreamSynchronize(c->stream));
-    return CAD_OK;
+    return cad_wave_check(c);
 }
 
 int cad_vpcoef(cad_ctx *c)
@@ -541,6 +551,7 @@ int cad_v1cal(cad_ctx *c, cad_step_info *info)
     RET_IF(cad_launch_bwpp(c));
     hipEventRecord(c->ev[3], c->stream);
     CAD_CHECK(hipStreamSynchronize(c->stream));
+    RET_IF(cad_wave_check(c));
     hipEventElapsedTime(&loc.ms_predictor, c->ev[0], c->ev[1]);
     hipEventElapsedTime(&loc.ms_poisson, c->ev[1], c->ev[2]);
     hipEventElapsedTime(&loc.ms_correction, c->ev[2], c->ev[3]);
@@ -616,6 +627,7 @@ int cad_init_state(cad_ctx *c, double wvlvl, double uini, double vini, double wi
     const double uvw[3] = {uini, vini, wini};
     double fsum = 0.0, fcut = 0.0;
     RET_IF(cad_init_phase(c, wvlvl, uvw, flags, &fsum, &fcut));
+    RET_IF(cad_wave_check(c));
     // TimerDoor state (vf_cinit.f:412-421)
     if (c->td_alloc) RET_IF(cad_tdoor_init(c));
     if (info) {
@@ -624,6 +636,34 @@ int cad_init_state(cad_ctx *c, double wvlvl, double uini, double vini, double wi
         info->fcut = fcut;
         info->nums = c->nums;
     }
+    return CAD_OK;
+}
+
+int cad_set_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
+                       const double *hh, const double *un, const double *ut, double t0)
+{
+    if (!c) return CAD_ERR_ARG;
+    return cad_wave_table(c, nz, nt, zz, tt, hh, un, ut, t0);
+}
+
+int cad_set_wave(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
+                 double wvlvl)
+{
+    if (!c) return CAD_ERR_ARG;
+    return cad_wave_side(c, side, l1, l2, depth, period, ampl, wvlvl);
+}
+
+int cad_get_wave_level(cad_ctx *c, int side, double *level)
+{
+    if (!c || !level || side < 1 || side > 4) {
+        cad_set_error("cad_get_wave_level: side 1..4 and a non-null result");
+        return CAD_ERR_ARG;
+    }
+    if (!c->wv_on[side - 1]) {
+        cad_set_error("cad_get_wave_level: side %d has no wave-maker", side);
+        return CAD_ERR_STATE;
+    }
+    *level = c->wv_level[side - 1];
     return CAD_OK;
 }
 

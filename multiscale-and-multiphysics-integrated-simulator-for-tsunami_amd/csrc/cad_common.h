@@ -151,6 +151,16 @@ struct cad_ctx {
     unsigned char *td_lvl;
     double *tbub, *zz_host;
     bool td_alloc, td_full;
+    // matrix-table wave-maker (cad_wave.hip): the table's heights and
+    // velocities on the device (ZZ, UN, UT), its times and levels on the
+    // host, the time search start MTBNOW, the sides and an error flag
+    double *wv_tab, *wv_tt, *wv_hh;
+    int wv_nz, wv_nt, wv_mtbnow;
+    double wv_t0, wv_wvlvl;
+    int wv_on[4], wv_l1[4], wv_l2[4];
+    double wv_d[4], wv_t[4], wv_a[4], wv_level[4];
+    int *wv_err, *wv_err_host;
+    int n_wave_faces;   // boundary faces of type 5 in INDB
 };
 
 // ---- launchers implemented in the phase files ---------------------------
@@ -181,3 +191,10 @@ int cad_tdoor_alloc(cad_ctx *c);
 void cad_tdoor_free(cad_ctx *c);
 int cad_tdoor_init(cad_ctx *c);
 int cad_tdoor_phase(cad_ctx *c);
+int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
+                   const double *hh, const double *un, const double *ut, double t0);
+int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
+                  double wvlvl);
+int cad_wave_apply(cad_ctx *c);
+int cad_wave_check(cad_ctx *c);
+void cad_wave_free(cad_ctx *c);

@@ -233,7 +233,8 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, const double 
 }
 
 // VF_BWUWN boundary-list pass (vf_bwuwn.f:84-254): normal velocity on wall and
-// free boundary faces.  Boundary type 5 (wave maker) is rejected on the host.
+// free boundary faces.  Type 5 (wave-maker) faces are set afterwards by the
+// table wave-maker (cad_wave.hip).
 __global__ void k_bwuwn(Dims d, const int *__restrict__ indb, double *u, double *v, double *w,
                         double *bcu, double *bcv, double *bcw)
 {
@@ -298,5 +299,6 @@ int cad_launch_bwuwn(cad_ctx *c)
     hipLaunchKernelGGL(k_bwuwn, dim3((d.numb + 255) / 256), dim3(256), 0, c->stream, d, c->indb,
                        c->f.uu, c->f.vv, c->f.ww, c->f.bcu, c->f.bcv, c->f.bcw);
     CAD_CHECK(hipGetLastError());
-    return CAD_OK;
+    // the wave-maker sides (vf_bwuwn.f:257-480)
+    return cad_wave_apply(c);
 }

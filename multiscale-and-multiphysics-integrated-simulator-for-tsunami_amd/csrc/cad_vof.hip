@@ -269,13 +269,14 @@ __global__ void __launch_bounds__(REDUCE_THREADS) k_fcut(Dims d, Static s, const
     }
 }
 
-// Boundary F, vf_bwff.f:33-87 (types FIX and FREE).
+// Boundary F, vf_bwff.f:33-87 (types FIX and FREE; a wave-maker face, type
+// 5 on the sides 1-4, takes F like FREE, vf_bwff.f:41-76).
 __global__ void k_bwff(Dims d, const int *__restrict__ indb, const double *__restrict__ f, double *bcf)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= d.numb) return;
     const int *e = indb + (long long)l * 4;
-    if (e[3] != 2) return;
+    if (!(e[3] == 2 || (e[3] == 5 && e[1] <= 4))) return;
     const long long n = (long long)e[0] - 1;
     switch (e[1]) {
     case 1: bcf[l] = f[n]; break;

@@ -1,0 +1,321 @@
+// cad_wave.hip -- the matrix-table wave-maker sides of VF_BWUWN.
+//
+// MODEL WAVE-BC <side> FUNC MATRIX with a LEVEL-ON table (<name>.mtb, read by
+// VF_IM1INP, Source code/vf_im1inp.f) sets, at every call of VF_BWUWN
+// (vf_bwuwn.f:257-480), the velocity of each boundary face of the side from
+// the table.  The table's phase and the ramp are scalars of the call and are
+// worked out on the host in the reference's precision (the phase in single
+// precision, the ramp with the C library's sin, the time weights of VF_CWMTB1
+// with its persistent search start MTBNOW).  The device then takes one thread
+// per boundary column, which measures the column's water level (the
+// F-weighted cell heights up to the first surface cell), scales the table's
+// depth axis by (table level + depth) / (measured level + depth), and walks up
+// the column's faces interpolating the normal and tangential velocity
+// (VF_CWMTB2), in the reference's order of operations.  Faces of type 5
+// outside a configured side are left as they are, as in the reference's
+// boundary-list pass.
+#include "cad_common.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace {
+
+// the reference's PI (VF_A0PRM.h): one ulp above M_PI
+constexpr double PI_REF = 3.141592653589794;
+constexpr double ZERO_REF = 1.0e-20;
+
+struct WaveLaunch {
+    int xside;           // faces normal to x (sides 1, 2) or to y (3, 4)
+    int pos;             // face index of the side (2, or NUMI / NUMJ)
+    int cpos;            // the cell column next to it (2, or NUMI-1 / NUMJ-1)
+    int q0, nq;          // first column along the side and column count
+    int nz;              // table depths
+    int im;              // MTBNOW: the table's time interval for this call
+    double d, wvz, aw, wmt1, wmt2, sa, sb, wvlvl;
+};
+
+__global__ void __launch_bounds__(64) k_wave_maker(Dims d, const double *__restrict__ zz,
+                                                   const int *__restrict__ nf,
+                                                   const double *__restrict__ ff,
+                                                   const int *__restrict__ indf,
+                                                   const double *__restrict__ tz,
+                                                   const double *__restrict__ tun,
+                                                   const double *__restrict__ tut, double *vel,
+                                                   double *bcu, double *bcv, double *bcw,
+                                                   WaveLaunch w, int *err)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w.nq) return;
+    const int q = w.q0 + t;
+    // water level of the column next to the side (vf_bwuwn.f:300-330)
+    double val = 0.0;
+    for (int k = 2; k <= d.nk - 1; k++) {
+        const long long c = w.xside ? cid(d, w.cpos, q, k) : cid(d, q, w.cpos, k);
+        const int nw = nf[c];
+        const double dz = zz[(long long)(k - 1) * 6 + 1];
+        if (nw == -1) {
+            val = val + dz * 1.0;
+        } else if (nw == 0) {
+            val = val + dz * ff[c];
+        } else if (nw != 8) {
+            val = val + dz * ff[c];
+            break;
+        }
+    }
+    val = val - (w.wvlvl - zz[6]);
+    const double vws = (w.wvz + w.d) / (val + w.d);
+    // the column's faces bottom up (vf_bwuwn.f:331-380); VF_CWMTB2's depth
+    // search starts where the face below found its interval
+    const int nz = w.nz;
+    const long long c0 = (long long)(w.im - 1) * nz, c1 = (long long)w.im * nz;
+    int km = 1;
+    for (int k = 2; k <= d.nk - 1; k++) {
+        const long long f = w.xside ? cid(d, w.pos, q, k) : cid(d, q, w.pos, k);
+        const int l = indf[f];
+        if (l < 1) continue;
+        double zc = (zz[(long long)(k - 1) * 6] + zz[(long long)k * 6]) * 0.5 - w.wvlvl;
+        zc = vws * (zc + w.d) - w.d;
+        double w1 = 0.0, w2 = 0.0;
+        if (zc <= tz[0]) {
+            km = 1;
+            w1 = 1.0;
+            w2 = 0.0;
+        } else if (zc >= tz[nz - 1]) {
+            km = nz - 1;
+            w1 = 0.0;
+            w2 = 1.0;
+        } else {
+            int kp = 0;
+            for (int m = km; m <= nz - 1 && !kp; m++) {
+                if (tz[m - 1] <= zc && zc <= tz[m]) {
+                    kp = m;
+                    w2 = 1.0 / (tz[m] - tz[m - 1]);
+                    w1 = (tz[m] - zc) * w2;
+                    w2 = (zc - tz[m - 1]) * w2;
+                }
+            }
+            if (!kp) {
+                // VF_CWMTB2: P.G ERROR. (no interval above the face below)
+                err[0] = 1;
+                return;
+            }
+            km = kp;
+        }
+        const long long a0 = km - 1, a1 = km;
+        double p1 = w1 * tun[a0 + c0] + w2 * tun[a1 + c0];
+        double p2 = w1 * tun[a0 + c1] + w2 * tun[a1 + c1];
+        const double un = w.wmt1 * p1 + w.wmt2 * p2;
+        p1 = w1 * tut[a0 + c0] + w2 * tut[a1 + c0];
+        p2 = w1 * tut[a0 + c1] + w2 * tut[a1 + c1];
+        const double ut = w.wmt1 * p1 + w.wmt2 * p2;
+        const double u = un * vws * w.aw * w.sa;
+        const double v = un * vws * w.aw * w.sb;
+        bcu[l - 1] = u;
+        bcv[l - 1] = v;
+        bcw[l - 1] = ut * vws * w.aw;
+        vel[f] = w.xside ? u : v;
+    }
+}
+
+// VF_CWMTB1 (vf_cwmtb1.f): the time weights of the table at phase wvt and the
+// table level there; searches down from MTBNOW, then down from the top.
+int mtb_time(cad_ctx *c, double wvt, double *w1o, double *w2o, double *wvz)
+{
+    const int nt = c->wv_nt;
+    const double *tt = c->wv_tt - 1;  // 1-based
+    double w1 = 0.0, w2 = 0.0;
+    if (c->wv_mtbnow <= 0) c->wv_mtbnow = nt - 1;
+    if (wvt <= tt[1]) {
+        c->wv_mtbnow = 1;
+        w1 = 1.0;
+        w2 = 0.0;
+    } else if (wvt >= tt[nt]) {
+        c->wv_mtbnow = nt - 1;
+        w1 = 0.0;
+        w2 = 1.0;
+    } else {
+        int ip = 0;
+        for (int i = c->wv_mtbnow; i >= 1 && !ip; i--)
+            if (tt[i] <= wvt && wvt <= tt[i + 1]) {
+                ip = i;
+                w2 = 1.0 / (tt[i + 1] - tt[i]);
+                w1 = (tt[i + 1] - wvt) * w2;
+                w2 = (wvt - tt[i]) * w2;
+            }
+        for (int i = nt - 1; i >= c->wv_mtbnow + 1 && !ip; i--)
+            if (tt[i] <= wvt && wvt <= tt[i + 1]) {
+                ip = i;
+                w2 = 1.0 / (tt[i + 1] - tt[i]);
+                w1 = (tt[i + 1] - wvt) * w2;
+                w2 = (wvt - tt[i]) * w2;
+            }
+        if (!ip) {
+            cad_set_error("VF_CWMTB1: P.G ERROR. (phase %.17g outside the table's times)", wvt);
+            return CAD_ERR_WAVE;
+        }
+        c->wv_mtbnow = ip;
+    }
+    *wvz = w1 * c->wv_hh[c->wv_mtbnow - 1] + w2 * c->wv_hh[c->wv_mtbnow];
+    *w1o = w1;
+    *w2o = w2;
+    return CAD_OK;
+}
+
+}  // namespace
+
+void cad_wave_free(cad_ctx *c)
+{
+    if (c->wv_tab) hipFree(c->wv_tab);
+    if (c->wv_err) hipFree(c->wv_err);
+    if (c->wv_err_host) hipHostFree(c->wv_err_host);
+    std::free(c->wv_tt);
+    std::free(c->wv_hh);
+    c->wv_tab = nullptr;
+    c->wv_err = nullptr;
+    c->wv_err_host = nullptr;
+    c->wv_tt = c->wv_hh = nullptr;
+    c->wv_nz = c->wv_nt = 0;
+    // the sides refer to the table: a new table needs them set again
+    for (int q = 0; q < 4; q++) c->wv_on[q] = 0;
+}
+
+int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
+                   const double *hh, const double *un, const double *ut, double t0)
+{
+    if (nz < 2 || nt < 2 || !zz || !tt || !hh || !un || !ut) {
+        cad_set_error("cad_set_wave_table: need at least 2 depths and 2 times, non-null arrays");
+        return CAD_ERR_ARG;
+    }
+    for (int k = 1; k < nz; k++)
+        if (!(zz[k] > zz[k - 1])) {
+            cad_set_error("cad_set_wave_table: depths must increase (ZZ(%d) <= ZZ(%d))", k + 1, k);
+            return CAD_ERR_ARG;
+        }
+    for (int i = 1; i < nt; i++)
+        if (!(tt[i] > tt[i - 1])) {
+            cad_set_error("cad_set_wave_table: times must increase (TT(%d) <= TT(%d))", i + 1, i);
+            return CAD_ERR_ARG;
+        }
+    CAD_CHECK(hipSetDevice(c->device));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    cad_wave_free(c);
+    const size_t tab = (size_t)nz * (size_t)nt;
+    CAD_CHECK(hipMalloc((void **)&c->wv_tab, (nz + 2 * tab) * sizeof(double)));
+    CAD_CHECK(hipMalloc((void **)&c->wv_err, 64));
+    CAD_CHECK(hipHostMalloc((void **)&c->wv_err_host, 64, hipHostMallocDefault));
+    c->wv_tt = (double *)std::malloc(nt * sizeof(double));
+    c->wv_hh = (double *)std::malloc(nt * sizeof(double));
+    if (!c->wv_tt || !c->wv_hh) {
+        cad_set_error("cad_set_wave_table: out of host memory");
+        return CAD_ERR_HIP;
+    }
+    std::memcpy(c->wv_tt, tt, nt * sizeof(double));
+    std::memcpy(c->wv_hh, hh, nt * sizeof(double));
+    CAD_CHECK(hipMemcpy(c->wv_tab, zz, nz * sizeof(double), hipMemcpyHostToDevice));
+    CAD_CHECK(hipMemcpy(c->wv_tab + nz, un, tab * sizeof(double), hipMemcpyHostToDevice));
+    CAD_CHECK(hipMemcpy(c->wv_tab + nz + tab, ut, tab * sizeof(double), hipMemcpyHostToDevice));
+    CAD_CHECK(hipMemset(c->wv_err, 0, 64));
+    c->wv_err_host[0] = 0;
+    c->wv_nz = nz;
+    c->wv_nt = nt;
+    c->wv_t0 = t0;
+    c->wv_mtbnow = 0;
+    return CAD_OK;
+}
+
+int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
+                  double wvlvl)
+{
+    if (side < 1 || side > 4) {
+        cad_set_error("cad_set_wave: side %d (1..4 = X-, X+, Y-, Y+)", side);
+        return CAD_ERR_ARG;
+    }
+    if (!c->wv_tab) {
+        cad_set_error("cad_set_wave: set the table first (cad_set_wave_table)");
+        return CAD_ERR_STATE;
+    }
+    if (!(depth > 0.0) || !(period > 0.0)) {
+        cad_set_error("cad_set_wave: depth and period must be positive (vf_iimdl.f)");
+        return CAD_ERR_ARG;
+    }
+    const int lim = side <= 2 ? c->d.nj : c->d.ni;
+    if (l1 < 2 || l2 > lim - 1 || l1 > l2) {
+        cad_set_error("cad_set_wave: columns %d..%d outside 2..%d", l1, l2, lim - 1);
+        return CAD_ERR_ARG;
+    }
+    const int q = side - 1;
+    c->wv_on[q] = 1;
+    c->wv_l1[q] = l1;
+    c->wv_l2[q] = l2;
+    c->wv_d[q] = depth;
+    c->wv_t[q] = period;
+    c->wv_a[q] = ampl;
+    c->wv_wvlvl = wvlvl;
+    c->wv_level[q] = 0.0;
+    return CAD_OK;
+}
+
+int cad_wave_apply(cad_ctx *c)
+{
+    const Dims &d = c->d;
+    bool any = false;
+    for (int jd = 1; jd <= 4; jd++) {
+        const int q = jd - 1;
+        if (!c->wv_on[q]) continue;
+        any = true;
+        const double dep = c->wv_d[q], t = c->wv_t[q], t0 = c->wv_t0;
+        double a = c->wv_a[q];
+        // the table's phase in single precision (vf_bwuwn.f:270-276)
+        float r4tn = (float)(t0 - c->prm.tnow / t);
+        r4tn = r4tn - (float)(int)r4tn;
+        if (r4tn < 0.0f) r4tn = r4tn + 1.0f;
+        // ramp over the first AMPL periods (vf_bwuwn.f:277-283)
+        double aw = 1.0;
+        if (a >= ZERO_REF) {
+            a = c->prm.tnow / t / a;
+            if (a < 1.0) aw = 0.5 * std::sin(PI_REF * (a - 0.5)) + 0.5;
+        }
+        double wmt1, wmt2, wvz;
+        const int rc = mtb_time(c, (double)r4tn, &wmt1, &wmt2, &wvz);
+        if (rc) return rc;
+        c->wv_level[q] = wvz * aw;
+        WaveLaunch w;
+        w.xside = jd <= 2;
+        w.pos = w.xside ? (jd == 1 ? 2 : d.ni) : (jd == 3 ? 2 : d.nj);
+        w.cpos = w.xside ? (jd == 1 ? 2 : d.ni - 1) : (jd == 3 ? 2 : d.nj - 1);
+        w.q0 = c->wv_l1[q];
+        w.nq = c->wv_l2[q] - c->wv_l1[q] + 1;
+        w.nz = c->wv_nz;
+        w.im = c->wv_mtbnow;
+        w.d = dep;
+        w.wvz = wvz;
+        w.aw = aw;
+        w.wmt1 = wmt1;
+        w.wmt2 = wmt2;
+        w.sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
+        w.sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
+        w.wvlvl = c->wv_wvlvl;
+        const size_t tab = (size_t)c->wv_nz * (size_t)c->wv_nt;
+        const double *tz = c->wv_tab, *tun = c->wv_tab + c->wv_nz, *tut = c->wv_tab + c->wv_nz + tab;
+        hipLaunchKernelGGL(k_wave_maker, dim3((w.nq + 63) / 64), dim3(64), 0, c->stream, d, c->zz,
+                           c->f.nf, c->f.ff, w.xside ? c->indx : c->indy, tz, tun, tut,
+                           w.xside ? c->f.uu : c->f.vv, c->f.bcu, c->f.bcv, c->f.bcw, w, c->wv_err);
+        CAD_CHECK(hipGetLastError());
+    }
+    if (any)
+        CAD_CHECK(hipMemcpyAsync(c->wv_err_host, c->wv_err, sizeof(int), hipMemcpyDeviceToHost,
+                                 c->stream));
+    return CAD_OK;
+}
+
+int cad_wave_check(cad_ctx *c)
+{
+    if (c->wv_err_host && c->wv_err_host[0]) {
+        cad_set_error("VF_CWMTB2: P.G ERROR. (a face height has no table interval above the "
+                      "face below it)");
+        return CAD_ERR_WAVE;
+    }
+    return CAD_OK;
+}

@@ -14,12 +14,15 @@ per iteration:
     <name>.tran (VF_OT1TRN), all by the step / time rule of
     vf_or1trn.f:62-78;
   - the step (cad_step).
+With MODEL WAVE-BC ... FUNC MATRIX the table is read from <name>.mtb in the
+working directory (VF_IM1INP, vf_im1inp.f:44-46) before the initial state.
 """
 from pathlib import Path
 
 from . import capi
 from . import grp as grpmod
 from . import layout as layoutmod
+from . import mtb as mtbmod
 from . import rsl as rslmod
 from . import timeloop
 from . import tran as tranmod
@@ -62,6 +65,9 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
         ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
         static = lay.static_arrays()
         ctx.upload_static(static)
+        if d.waves:
+            table = mtbmod.parse((workdir / f"{name}.mtb").read_text())
+            ctx.set_waves(d.waves, table, d.wvlvl)
         restart = d.res_step is not None
         if restart:
             r = rslmod.restart(workdir / f"{name}.res", d.res_step)

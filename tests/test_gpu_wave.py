@@ -1,0 +1,129 @@
+"""The matrix-table wave-maker (MODEL WAVE-BC X- FUNC MATRIX with a LEVEL-ON
+<name>.mtb table) on the GPU.
+
+Fixture "wave_flume" (tests/golden/make_golden.py): a 60x4x20 flume at rest
+at depth 0.2 whose X- side follows a solitary-wave table; the reference ran
+400 steps and recorded the phases of steps 1, 2, 100, 200, 300 and 400.  The
+predictor and the boundary pass that sets the wave faces have no reduction, so
+they are compared bit for bit, as is F1CAL (the wave faces take F like FREE);
+whole V1CAL and free-run steps go through the Poisson solve and are compared
+to the plain-form bars.
+"""
+import numpy as np
+import pytest
+
+import fixtures as fx
+from helpers import assert_close, assert_same_int, assert_vel_close, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASE = "wave_flume"
+STATE = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")
+KEYS = ("rho0", "grz0", "anu0", "scmvp", "flower", "fupper", "plower", "cgpara", "cgepsa",
+        "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
+
+
+def _ctx(pkg, st, dtnow, tnow):
+    numi, numj, numk = st["GGV"].shape
+    p = fx.deck_params(st["DECK"])
+    p.update(fx.solver_params(st))
+    c = pkg.capi.Context(numi, numj, numk, st["INDB"].shape[1])
+    c.set_params(**{k: p[k] for k in KEYS}, dtnow=dtnow, tnow=tnow)
+    c.upload_static(st)
+    d = pkg.deck.parse(str(st["DECK"]))
+    c.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    return c, d
+
+
+def _v1_state(r):
+    a = {k: r["v1_in"][k] for k in STATE if k != "BCF"}
+    a["BCF"] = r["f1_in"]["BCF"]
+    return a
+
+
+def _wave_faces(st):
+    return np.asarray(st["INDB"])[2] == 5
+
+
+def test_predictor_with_wave_faces_bitexact(pkg):
+    """VF_VFLXDU..VF_VEULER then VF_BWUWN with the wave-maker at the step's
+    TNOW: every velocity and the wave faces' boundary values bit for bit."""
+    st = fx.static(CASE)
+    wave = _wave_faces(st)
+    for s in fx.steps(CASE):
+        r = fx.step(CASE, s)
+        print(f"{CASE} step {s} predictor", flush=True)
+        c, d = _ctx(pkg, st, float(r["v1_in"]["DTNOW"][0]), float(r["v1_in"]["TNOW"][0]))
+        with c:
+            c.upload_state(_v1_state(r))
+            c.predictor()
+            out = c.download_state()
+            level = c.wave_level(1)
+        for k, kk in (("UU", "US"), ("VV", "VS"), ("WW", "WS")):
+            assert np.array_equal(out[k], r["ps_in"][kk]), f"step {s} {k}: {rel_err(out[k], r['ps_in'][kk]):.3e}"
+        for k in ("BCU", "BCV", "BCW"):
+            assert np.array_equal(out[k][wave], r["v1_out"][k][wave]), f"step {s} {k}"
+        assert np.isfinite(level) and abs(level) < d.waves[1]["depth"]
+
+
+def test_v1cal_with_wave_faces(pkg):
+    """Whole VF_V1CAL: the fields to the plain-form bars, the wave faces'
+    boundary values (set before the solve) bit for bit."""
+    st = fx.static(CASE)
+    wave = _wave_faces(st)
+    for s in fx.steps(CASE):
+        r = fx.step(CASE, s)
+        print(f"{CASE} step {s} v1cal", flush=True)
+        c, _ = _ctx(pkg, st, float(r["v1_in"]["DTNOW"][0]), float(r["v1_in"]["TNOW"][0]))
+        with c:
+            c.upload_state(_v1_state(r))
+            info = c.v1cal()
+            out = c.download_state()
+        ref = r["v1_out"]
+        assert info["icgitr"] == int(ref["ICGITR"][0]), f"step {s}"
+        assert_vel_close(f"step {s} velocity", out, ref, 1e-10)
+        assert_close(f"step {s} PP", out["PP"], ref["PP"], 1e-10)
+        for k in ("BCU", "BCV", "BCW"):
+            assert np.array_equal(out[k][wave], ref[k][wave]), f"step {s} {k}"
+
+
+def test_f1cal_with_wave_faces_bitexact(pkg):
+    """VF_F1CAL with the wave faces' F taken like FREE (vf_bwff.f:41-76)."""
+    st = fx.static(CASE)
+    for s in fx.steps(CASE):
+        r = fx.step(CASE, s)
+        fin, ref = r["f1_in"], r["f1_out"]
+        print(f"{CASE} step {s} f1cal", flush=True)
+        c, _ = _ctx(pkg, st, float(fin["DTNOW"][0]), float(fin["TNOW"][0]))
+        with c:
+            c.upload_state({k: fin[k] for k in STATE})
+            c.f1cal()
+            out = c.download_state()
+        for k in ("FF", "BCF", "UU", "VV", "WW"):
+            assert np.array_equal(out[k], ref[k]), f"step {s} {k}: {rel_err(out[k], ref[k]):.3e}"
+        assert_same_int(f"step {s} NF", out["NF"], ref["NF"])
+
+
+def test_wave_flume_free_run(pkg, tmp_path):
+    """The deck through driver.run (the table read from <name>.mtb, the
+    initial state by VF_CINIT's device part) for 400 steps: ITR every step as
+    the reference's list file has it, the fields at the recorded steps from
+    the .rsl to 1e-8."""
+    st = fx.static(CASE)
+    tr = fx.trajectory(CASE)
+    deck = str(st["DECK"]) + "\nFILE RSL STEP 100 400 100\n"
+    (tmp_path / "flume.mtb").write_text(str(st["FILE_MTB"]))
+    res = pkg.driver.run(deck, "flume", tmp_path)
+    assert res["nnow"] == 400
+    lines = (tmp_path / "flume.list").read_text().splitlines()
+    itr = [int(ln.split("ITR =")[1]) for ln in lines if "ITR =" in ln]
+    assert itr == [int(x) for x in tr["ITR"][: len(itr)]] and len(itr) == 401
+    recs = [r for r in pkg.rsl.read(tmp_path / "flume.rsl")][1:]
+    assert [r["NNOW"] for r in recs] == [100, 200, 300, 400]
+    for rec in recs:
+        s = rec["NNOW"]
+        ref = fx.step(CASE, s)["f1_out"]
+        assert_vel_close(f"step {s} velocity", rec, ref, 1e-8)
+        for k in ("PP", "FF"):
+            assert_close(f"step {s} {k}", rec[k], ref[k], 1e-8)
+        assert_same_int(f"step {s} NF", rec["NF"], ref["NF"])
