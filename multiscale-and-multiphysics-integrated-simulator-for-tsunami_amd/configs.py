@@ -328,10 +328,12 @@ def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81):
 
 
 def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400, dt=1.0e-3,
-               eps_r=1.0e-10, name=None):
+               eps_r=1.0e-10, open_end=False, name=None):
     """A flume with a table wave-maker on its x- side (MODEL WAVE-BC X- FUNC
     MATRIX with <name>.mtb, vf_bwuwn.f:257-480): a solitary wave enters the
-    still water (MATE W-LEVEL) and runs to the far wall.  The table goes to
+    still water (MATE W-LEVEL) and runs to the far wall, or with open_end
+    leaves through a radiation boundary on the x+ side (MODEL OPEN-BC X+
+    FUNC TYPE1, vf_bwuwn.f:483-620).  The table goes to
     meta["files"]["mtb"]."""
     lines = [f"# wave flume {ni}x{nj}x{nk}"]
     lines += _grid_lines("X", _coords(ni, d))
@@ -340,6 +342,9 @@ def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400,
     lines += ["B.C. D VP SLIP", "B.C. D F FREE",
               "MODEL WAVE-BC X- FUNC MATRIX", f"MODEL WAVE-BC X- DEPTH {h}",
               f"MODEL WAVE-BC X- PERIOD {period}"]
+    if open_end:
+        lines += ["MODEL OPEN-BC X+ FUNC TYPE1", f"MODEL OPEN-BC X+ DEPTH {h}",
+                  f"MODEL OPEN-BC X+ PERIOD {period}"]
     _common(lines, dt, steps, eps_r, wlevel=h)
     case = Case(name or f"wave_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
     case.meta["files"] = {"mtb": solitary_mtb(h, H, period, t_crest=0.2)}
@@ -360,5 +365,6 @@ def named(name):
         "auto_dam": lambda: auto_dam(),
         "subloop": lambda: subloop(),
         "wave_flume": lambda: wave_flume(),
+        "wave_open": lambda: wave_flume(ni=40, steps=1000, open_end=True),
     }
     return table[name]()

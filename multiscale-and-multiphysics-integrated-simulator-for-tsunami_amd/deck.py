@@ -142,6 +142,9 @@ class Deck:
     # MODEL WAVE-BC per side (1 X-, 2 X+, 3 Y-, 4 Y+): func, depth, height,
     # period, ampl, area (+1 like IBCTYP(3..4)), angle
     waves: dict = field(default_factory=dict)
+    # MODEL OPEN-BC per side: radiation boundary (FUNC TYPE1) with the
+    # depth and period that give its celerity
+    opens: dict = field(default_factory=dict)
     # FILE TRN STEP/TIME and its probe lines (words after FILE TRN)
     trn: Optional[tuple] = None
     trn_probes: List[tuple] = field(default_factory=list)
@@ -333,16 +336,50 @@ def _parse_comp(d, w):
 _WAVE_SIDE = {"X-": 1, "X+": 2, "Y-": 3, "Y+": 4}
 
 
+def _parse_open(d, w):
+    """MODEL OPEN-BC side FUNC TYPE1 / DEPTH d / PERIOD t
+    (Source code/vf_iimdl.f:174-210): a radiation boundary on the side."""
+    if len(w) < 3:
+        raise DeckError("MODEL OPEN-BC: SYNTAX ERROR.")
+    if w[2] not in _WAVE_SIDE:
+        raise DeckError(f"MODEL OPEN-BC {w[2]}: UNKNOWN WORD.")
+    jd = _WAVE_SIDE[w[2]]
+    if jd in d.waves:
+        raise DeckError("MODEL OPEN-BC: DEFINED DOUBLY.")
+    op = d.opens.setdefault(jd, dict(func="TYPE1", depth=-1.0, period=-1.0))
+    if len(w) < 4:
+        raise DeckError("MODEL OPEN-BC: SYNTAX ERROR.")
+    key = w[3]
+    if key == "FUNC":
+        if len(w) < 5 or w[4] != "TYPE1":
+            raise DeckError("MODEL OPEN-BC FUNC: UNKNOWN WORD.")
+    elif key in ("DEPTH", "PERIOD"):
+        if len(w) < 5:
+            raise DeckError(f"MODEL OPEN-BC {key}: SYNTAX ERROR.")
+        v = _num(w[4])
+        if v < 1.0e-20:
+            raise DeckError(f"MODEL OPEN-BC {key}: INVALID VALUE.")
+        op[key.lower()] = v
+    else:
+        raise DeckError(f"MODEL OPEN-BC {key}: UNKNOWN WORD.")
+
+
 def _parse_model(d, w):
-    """MODEL WAVE-BC (Source code/vf_iimdl.f:33-115).  Only the matrix-table
-    wave-maker (FUNC MATRIX with a <name>.mtb table) is produced here."""
+    """MODEL WAVE-BC (Source code/vf_iimdl.f:33-115) and MODEL OPEN-BC
+    (:174-210).  Only the matrix-table wave-maker (FUNC MATRIX with a
+    <name>.mtb table) and the radiation boundary are produced here."""
+    if len(w) >= 2 and w[1] == "OPEN-BC":
+        _parse_open(d, w)
+        return
     if len(w) < 2 or w[1] != "WAVE-BC":
-        raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (open boundaries, damping, "
-                              "wave sources) is not implemented")
+        raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (damping, wave sources, "
+                              "turbulence) is not implemented")
     if len(w) < 4:
         raise DeckError("MODEL WAVE-BC: SYNTAX ERROR.")
     if w[2] not in _WAVE_SIDE:
         raise DeckError(f"MODEL WAVE-BC {w[2]}: UNKNOWN WORD.")
+    if _WAVE_SIDE[w[2]] in d.opens:
+        raise DeckError("MODEL WAVE-BC: DEFINED DOUBLY.")
     wv = d.waves.setdefault(_WAVE_SIDE[w[2]], dict(func=None, depth=0.0, height=0.0, period=0.0,
                                                    ampl=0.0, area=None, angle=0.0))
     key = w[3]
@@ -516,3 +553,9 @@ def _validate(d):
             raise DeckError("MODEL WAVE-BC: (WAVE-BC PERIOD) NOT FOUND.")
         if abs(wv["angle"]) > 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
+    # vf_csetup.f:128-137: a radiation boundary needs DEPTH and PERIOD
+    for jd, op in d.opens.items():
+        if op["depth"] < 1.0e-20:
+            raise DeckError("VF_CSETUP: (OPEN-BC DEPTH) NOT FOUND.")
+        if op["period"] < 1.0e-20:
+            raise DeckError("VF_CSETUP: (OPEN-BC PERIOD) NOT FOUND.")

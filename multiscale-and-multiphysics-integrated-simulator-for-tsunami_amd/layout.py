@@ -157,21 +157,24 @@ class Layout:
                 else:
                     self.INDB[3, ids] = bc.code
                     self.BCF[ids] = bc.values[0]
-        # wave-maker sides: boundary type 5 for velocity and F on every face
-        # of the side (in AREA), values cleared (vf_csetup.f:150-205)
-        for jd, wv in self.deck.waves.items():
+        # wave-maker sides (type 5, in AREA) and radiation sides (type 7, the
+        # whole side): the type for velocity and F on every face of the side,
+        # values cleared (vf_csetup.f:146-205)
+        special = [(jd, 5, wv["area"]) for jd, wv in self.deck.waves.items()]
+        special += [(jd, 7, None) for jd in self.deck.opens]
+        for jd, code, area in sorted(special):
             a, ind = (0, self.INDX) if jd in (1, 2) else (1, self.INDY)
             pos = 2 if jd in (1, 3) else self.shape[a]
             other = 1 - a
-            l1, l2 = wv["area"] if wv["area"] else (2, self.shape[other] - 1)
+            l1, l2 = area if area else (2, self.shape[other] - 1)
             sl = [slice(None)] * 3
             sl[a] = slice(pos - 1, pos)
             sl[other] = slice(l1 - 1, l2)
             sl[2] = slice(1, self.shape[2] - 1)
             ids = ind[tuple(sl)].ravel(order="F")
             ids = ids[ids >= 1] - 1
-            self.INDB[2, ids] = 5
-            self.INDB[3, ids] = 5
+            self.INDB[2, ids] = code
+            self.INDB[3, ids] = code
             for arr in (self.BCU, self.BCV, self.BCW, self.BCP, self.BCF, self.BCVI):
                 arr[ids] = 0.0
         self.INDB = np.asfortranarray(self.INDB)

@@ -423,7 +423,104 @@ static int wave_bc(orc_state *s)
     return 0;
 }
 
-static int bwuwn(orc_state *s)
+/* VF_COMAK0 (vf_comak0.f): the wave length of a small-amplitude wave of
+ * period t in depth d, by Newton's method on the dispersion relation
+ * (at most 10 iterations, stop at |dX/X| <= 1e-10). */
+static double comak0(double d, double t, double gg)
+{
+    const double dd = 4.0 * PI_REF * PI_REF * d / gg / t / t;
+    double x = dd >= 1.0 ? dd : sqrt(dd);
+    for (int itr = 1; itr <= 10; itr++) {
+        const double ex = exp(x), exi = 1.0 / ex;
+        const double coth = (ex + exi) / (ex - exi);
+        const double dx = (x - dd * coth) / (1.0 + dd * (coth * coth - 1.0));
+        x = x - dx;
+        if (fabs(dx / x) <= 1.0e-10) break;
+    }
+    return 2.0 * PI_REF * d / x;
+}
+
+/* The radiation sides of VF_BWUWN (vf_bwuwn.f:483-620): on each face of the
+ * side next to a fluid cell the boundary values relax towards the inside
+ * velocities of the start of the step (UB, VB, WB) with the weights
+ * ALN = min(C*DTNOW/dx, 1) (normal) and ALT = min(2*ALN, 1) (tangential),
+ * C = wave length / period; both are 1 in the initial pass (NNOW = 0).
+ * Other faces get 0; then faces next to a surface cell copy the face below
+ * (NF = 5) or above (NF = 6) it. */
+static void open_bc(orc_state *s, const double *ub, const double *vb, const double *wb, int first)
+{
+    DECL_DIMS(s);
+    for (int jd = 1; jd <= 4; jd++) {
+        if (!s->op_on[jd - 1]) continue;
+        const double c = comak0(s->op_d[jd - 1], s->op_t[jd - 1], s->grz0) / s->op_t[jd - 1];
+        const int xside = jd <= 2;
+        const int pos = xside ? (jd == 1 ? 2 : ni) : (jd == 3 ? 2 : nj);
+        const int ic = xside ? (jd == 1 ? 2 : ni - 1) : (jd == 3 ? 2 : nj - 1);
+        const int i2 = jd == 1 || jd == 3 ? 3 : ic;
+        double aln = 1.0, alt = 1.0;
+        if (!first) {
+            aln = c * s->dtnow / (xside ? XX(2, ic) : YY(2, ic));
+            if (aln > 1.0) aln = 1.0;
+            alt = aln * 2.0;
+            if (alt > 1.0) alt = 1.0;
+        }
+        const int lim = xside ? nj : ni;
+        const int *ind = xside ? s->indx : s->indy;
+        double *vel = xside ? s->uu : s->vv;
+        /* faces next to fluid cells, and the rest (vf_bwuwn.f:505-527) */
+        for (int k = 2; k <= nk - 1; k++)
+            for (int q = 2; q <= lim - 1; q++) {
+                const size_t f = xside ? ID(pos, q, k) : ID(q, pos, k);
+                const int l = ind[f];
+                if (l < 1) continue;
+                const size_t cc = xside ? ID(ic, q, k) : ID(q, ic, k);
+                if (s->nf[cc] == 0) {
+                    if (xside) {
+                        const double vc = (vb[cc] + vb[cc + ni]) * 0.5;
+                        const double wc = (wb[cc] + wb[cc + nij]) * 0.5;
+                        s->bcu[l - 1] = (1.0 - aln) * s->bcu[l - 1] + aln * ub[ID(i2, q, k)];
+                        s->bcv[l - 1] = (1.0 - alt) * s->bcv[l - 1] + alt * vc;
+                        s->bcw[l - 1] = (1.0 - alt) * s->bcw[l - 1] + alt * wc;
+                        vel[f] = s->bcu[l - 1];
+                    } else {
+                        const double uc = (ub[cc] + ub[cc + 1]) * 0.5;
+                        const double wc = (wb[cc] + wb[cc + nij]) * 0.5;
+                        s->bcu[l - 1] = (1.0 - alt) * s->bcu[l - 1] + alt * uc;
+                        s->bcv[l - 1] = (1.0 - aln) * s->bcv[l - 1] + aln * vb[ID(q, i2, k)];
+                        s->bcw[l - 1] = (1.0 - alt) * s->bcw[l - 1] + alt * wc;
+                        vel[f] = s->bcv[l - 1];
+                    }
+                } else {
+                    s->bcu[l - 1] = 0.0;
+                    s->bcv[l - 1] = 0.0;
+                    s->bcw[l - 1] = 0.0;
+                    vel[f] = 0.0;
+                }
+            }
+        /* faces next to surface cells (vf_bwuwn.f:528-552); on an x side a
+         * face under a surface cell with water above gets U = 0, on a y side
+         * V = BCV (vf_bwuwn.f:546, :614) */
+        for (int k = 2; k <= nk - 1; k++)
+            for (int q = 2; q <= lim - 1; q++) {
+                const size_t f = xside ? ID(pos, q, k) : ID(q, pos, k);
+                const int l = ind[f];
+                if (l < 1) continue;
+                const int nw = s->nf[xside ? ID(ic, q, k) : ID(q, ic, k)];
+                if (nw != 5 && nw != 6) continue;
+                const int l2 = ind[nw == 5 ? f - nij : f + nij];
+                if (l2 < 1) continue;   /* no face there: out of the reference's reach */
+                s->bcu[l - 1] = s->bcu[l2 - 1];
+                s->bcv[l - 1] = s->bcv[l2 - 1];
+                s->bcw[l - 1] = s->bcw[l2 - 1];
+                if (nw == 5)
+                    vel[f] = xside ? s->bcu[l - 1] : s->bcv[l - 1];
+                else
+                    vel[f] = xside ? 0.0 : s->bcv[l - 1];
+            }
+    }
+}
+
+static int bwuwn(orc_state *s, const double *ub, const double *vb, const double *wb, int first)
 {
     DECL_DIMS(s);
     for (int l = 1; l <= s->numb; l++) {
@@ -457,20 +554,29 @@ static int bwuwn(orc_state *s)
         } else if (ib == 5) {
             /* a LEVEL-ON table sets these faces below (vf_bwuwn.f:171-223) */
         } else if (ib == 7) {
+            /* a radiation side, set below (vf_bwuwn.f:483-620) */
         } else {
             return -10;
         }
     }
-    return wave_bc(s);
+    int rc = wave_bc(s);
+    if (rc) return rc;
+    open_bc(s, ub, vb, wb, first);
+    return 0;
 }
 
 int orc_predictor(orc_state *s)
 {
     DECL_DIMS(s);
     size_t nc = nij * (size_t)nk;
-    double *w = (double *)calloc(12 * nc, sizeof(double));
+    double *w = (double *)calloc(15 * nc, sizeof(double));
     if (!w) return -1;
-    /* WK01..WK12 of vf_v1cal.f:126-146 */
+    /* WK01..WK12 of vf_v1cal.f:126-146, and WK13..WK15, the velocities of
+     * the start of the step that the radiation sides read */
+    double *ub = w + 12 * nc, *vb = w + 13 * nc, *wb = w + 14 * nc;
+    memcpy(ub, s->uu, nc * sizeof(double));
+    memcpy(vb, s->vv, nc * sizeof(double));
+    memcpy(wb, s->ww, nc * sizeof(double));
     double *flu[3] = {w + 0 * nc, w + 1 * nc, w + 2 * nc};
     double *qu = w + 3 * nc;
     double *flv[3] = {w + 4 * nc, w + 5 * nc, w + 6 * nc};
@@ -504,9 +610,10 @@ int orc_predictor(orc_state *s)
     veuler(s, 0, flu, qu);
     veuler(s, 1, flv, qv);
     veuler(s, 2, flw, qw);
-    free(w);
     /* VF_BWUWN (vf_v1cal.f:251) */
-    return bwuwn(s);
+    int rc = bwuwn(s, ub, vb, wb, 0);
+    free(w);
+    return rc;
 }
 
 /* ======================================================================
@@ -1281,8 +1388,8 @@ static int bwff(orc_state *s)
         int ns = INDB(2, l), ib = INDB(4, l);
         size_t n = ID(i, j, k);
         if (ib == 1) {
-        } else if (ib == 2 || (ib == 5 && ns <= 4)) {
-            /* free, and the wave-maker faces (vf_bwff.f:41-76) */
+        } else if (ib == 2 || ((ib == 5 || ib == 7) && ns <= 4)) {
+            /* free, and the wave-maker and radiation faces (vf_bwff.f:41-87) */
             switch (ns) {
             case 1: s->bcf[l - 1] = s->ff[n]; break;
             case 2: s->bcf[l - 1] = s->ff[n - 1]; break;
@@ -1940,7 +2047,14 @@ int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double 
                 }
         }
     }
-    rc = bwuwn(s);
+    /* VF_BWUWN with UB = a copy of the velocities, NNOW = 0 (vf_cinit.f:376-390) */
+    double *bk = (double *)malloc(3 * nc * sizeof(double));
+    if (!bk) return -1;
+    memcpy(bk, s->uu, nc * sizeof(double));
+    memcpy(bk + nc, s->vv, nc * sizeof(double));
+    memcpy(bk + 2 * nc, s->ww, nc * sizeof(double));
+    rc = bwuwn(s, bk, bk + nc, bk + 2 * nc, 1);
+    free(bk);
     if (rc) return rc;
     bsuwt_axis(s, 0);
     bsuwt_axis(s, 1);

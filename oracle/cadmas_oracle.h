@@ -68,6 +68,11 @@ typedef struct orc_state {
     double wv_d[4], wv_t[4], wv_a[4], wv_t0[4], wv_level[4];
     int mtbzz, mtbtt, mtbnow;
     const double *mtb_zz, *mtb_tt, *mtb_hh, *mtb_un, *mtb_ut;
+    /* Radiation boundaries, MODEL OPEN-BC side FUNC TYPE1 (vf_bwuwn.f:483-620):
+     * per side whether it is on, with the DEPTH and PERIOD that give its
+     * celerity (VF_COMAK0). */
+    int op_on[4];
+    double op_d[4], op_t[4];
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */

@@ -48,6 +48,7 @@ class OrcState(C.Structure):
         ("wv_t0", C.c_double * 4), ("wv_level", C.c_double * 4),
         ("mtbzz", C.c_int), ("mtbtt", C.c_int), ("mtbnow", C.c_int),
         ("mtb_zz", PD), ("mtb_tt", PD), ("mtb_hh", PD), ("mtb_un", PD), ("mtb_ut", PD),
+        ("op_on", C.c_int * 4), ("op_d", C.c_double * 4), ("op_t", C.c_double * 4),
     ]
 
 
@@ -182,6 +183,13 @@ class Fields:
         st.mtbzz, st.mtbtt, st.mtbnow = len(table.zz), len(table.tt), 0
         st.mtb_zz, st.mtb_tt, st.mtb_hh = (_pd(self.tab[k]) for k in ("zz", "tt", "hh"))
         st.mtb_un, st.mtb_ut = _pd(self.tab["un"]), _pd(self.tab["ut"])
+
+    def set_opens(self, opens):
+        """Radiation sides: opens = deck.opens (side -> depth, period)."""
+        for jd, op in opens.items():
+            q = jd - 1
+            self.st.op_on[q] = 1
+            self.st.op_d[q], self.st.op_t[q] = op["depth"], op["period"]
 
     def __getitem__(self, k):
         return self.a[k]

@@ -66,6 +66,9 @@ def _cases():
         "subloop": (lambda: cfg.subloop(name="subloop"), [1, 2, 20, 40]),
         # table wave-maker (MODEL WAVE-BC FUNC MATRIX, vf_bwuwn.f:257-480)
         "wave_flume": (lambda: cfg.wave_flume(name="wave_flume"), [1, 2, 100, 200, 300, 400]),
+        # ... with a radiation boundary at the far end (MODEL OPEN-BC X+ FUNC TYPE1)
+        "wave_open": (lambda: cfg.wave_flume(ni=40, steps=1000, open_end=True, name="wave_open"),
+                      [1, 2, 250, 500, 750, 1000]),
     }
 
 
@@ -78,6 +81,8 @@ _KEEP = {
     "auto_dam": {s: _F1_ONLY for s in (100, 250, 400)},
     "wave_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                        "f1_out": None} for s in (100, 200, 300, 400)},
+    "wave_open": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                      "f1_out": None} for s in (250, 500, 750, 1000)},
 }
 
 

@@ -44,6 +44,28 @@ def test_init_state_matches_reference(pkg, case):
     assert np.array_equal(F.inds, ref["INDS"][: int(ref["NUMS"][0])])
 
 
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open"])
+def test_init_state_with_special_sides(pkg, case):
+    """VF_CINIT's boundary pass with a table wave-maker (at TNOW = 0) and a
+    radiation side (the initial pass, ALN = ALT = 1) == the reference's
+    initial state."""
+    st = fx.static(case)
+    lay = pkg.layout.from_text(str(st["DECK"]))
+    arr = dict(lay.static_arrays())
+    arr.update(lay.raw_state())
+    F = po.Fields(arr, lay.params())
+    d = lay.deck
+    F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    F.set_opens(d.opens)
+    F.st.tnow = 0.0
+    assert F.init_state(d.wvlvl, d.uini, 3) == 0
+    r1 = fx.step(case, 1)
+    ref = r1["v1_in"]
+    for k in ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "NF", "INDC"):
+        assert np.array_equal(F[k], ref[k]), k
+    assert np.array_equal(F["BCF"], r1["f1_in"]["BCF"])
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_phases_bitexact(case):
     st = fx.static(case)
@@ -279,14 +301,16 @@ def _wave_fields(pkg, case, rec, bcf):
     F = _fields(st, rec, p, bcf)
     d = pkg.deck.parse(str(st["DECK"]))
     F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    F.set_opens(d.opens)
     return st, p, F
 
 
-def test_wave_maker_phases_bitexact(pkg):
-    """MODEL WAVE-BC X- FUNC MATRIX: V1CAL (the wave-maker sets the boundary
-    velocities inside it) and F1CAL at every recorded step equal the
-    reference's bit for bit."""
-    case = "wave_flume"
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open"])
+def test_wave_maker_phases_bitexact(pkg, case):
+    """MODEL WAVE-BC X- FUNC MATRIX, and in wave_open a radiation boundary
+    (MODEL OPEN-BC X+ FUNC TYPE1) at the far end: V1CAL (the wave-maker and
+    the radiation side set the boundary velocities inside it) and F1CAL at
+    every recorded step equal the reference's bit for bit."""
     for s in fx.steps(case):
         r = fx.step(case, s)
         st, p, F = _wave_fields(pkg, case, r["v1_in"], r["f1_in"]["BCF"])
@@ -306,10 +330,11 @@ def test_wave_maker_phases_bitexact(pkg):
             assert np.array_equal(F[k], r["f1_out"][k]), (s, k)
 
 
-def test_wave_maker_free_run_bitexact(pkg):
-    """400 steps of the wave flume from the reference's initial state: ITR and
-    ||div|| every step and the fields at the recorded steps, bit for bit."""
-    case = "wave_flume"
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open"])
+def test_wave_maker_free_run_bitexact(pkg, case):
+    """The wave flume (400 steps) and the flume with the radiation end (1000
+    steps) from the reference's initial state: ITR and ||div|| every step and
+    the fields at the recorded steps, bit for bit."""
     r1 = fx.step(case, 1)
     st, p, F = _wave_fields(pkg, case, r1["v1_in"], r1["f1_in"]["BCF"])
     tr = fx.trajectory(case)
