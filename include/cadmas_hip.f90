@@ -186,6 +186,15 @@ module cadmas_hip
       integer(c_int) :: cad_set_wave
     end function
 
+    ! MODEL OPEN-BC side (1..4) FUNC TYPE1 with DEPTH and PERIOD
+    function cad_set_open(ctx, side, depth, period) bind(c, name='cad_set_open')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: side
+      real(c_double), value :: depth, period
+      integer(c_int) :: cad_set_open
+    end function
+
     function cad_get_wave_level(ctx, side, level) bind(c, name='cad_get_wave_level')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -133,6 +133,13 @@ int cad_set_wave_table(cad_ctx *ctx, int nz, int nt, const double *zz, const dou
  * sub-loop passes) sets their velocities at the context's TNOW. */
 int cad_set_wave(cad_ctx *ctx, int side, int l1, int l2, double depth, double period,
                  double ampl, double wvlvl);
+/* A radiation (open) side, MODEL OPEN-BC <side> FUNC TYPE1 (vf_iimdl.f:174-210,
+ * applied in VF_BWUWN, vf_bwuwn.f:483-620): side 1..4 = X-, X+, Y-, Y+, the
+ * DEPTH and PERIOD whose small-amplitude wave (VF_COMAK0) gives the
+ * celerity.  The side's faces carry boundary type 7 in INDB (velocity and
+ * F); every VF_BWUWN relaxes their values towards the inside velocities of
+ * the start of the step. */
+int cad_set_open(cad_ctx *ctx, int side, double depth, double period);
 /* The wave-maker level of a side at the last VF_BWUWN (table level times the
  * ramp; the W-LEVEL ANS probe of FILE TRN, vf_ot1trn.f). */
 int cad_get_wave_level(cad_ctx *ctx, int side, double *level);

@@ -82,6 +82,7 @@ def lib():
     L.cad_set_wave_table.argtypes = [C.c_void_p, C.c_int, C.c_int] + [PD] * 5 + [C.c_double]
     L.cad_set_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
+    L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
     L.cad_upload_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
     L.cad_download_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
@@ -111,7 +112,7 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
-           "cad_get_wave_level")
+           "cad_get_wave_level", "cad_set_open")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -228,6 +229,11 @@ class Context:
             l1, l2 = wv["area"] if wv["area"] else (2, lim - 1)
             _check(lib().cad_set_wave(self._h, int(side), int(l1), int(l2), float(wv["depth"]),
                                       float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
+
+    def set_opens(self, opens):
+        """Radiation sides: opens = deck.opens (side -> depth, period)."""
+        for side, op in sorted(opens.items()):
+            _check(lib().cad_set_open(self._h, int(side), float(op["depth"]), float(op["period"])))
 
     def wave_level(self, side):
         """The wave-maker level of a side at the last boundary pass."""

@@ -180,6 +180,8 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
         return rc;
     }
     c->zz_host = new double[6 * (size_t)numk]();
+    c->xx_host = new double[6 * (size_t)numi]();
+    c->yy_host = new double[6 * (size_t)numj]();
     if (hipHostMalloc((void **)&c->host_red, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->host_cnt, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         cad_set_error("hipHostMalloc failed");
@@ -222,6 +224,8 @@ void cad_destroy(cad_ctx *c)
     cad_tdoor_free(c);
     cad_wave_free(c);
     delete[] c->zz_host;
+    delete[] c->xx_host;
+    delete[] c->yy_host;
     if (c->host_red) hipHostFree(c->host_red);
     if (c->host_cnt) hipHostFree(c->host_cnt);
     for (auto &e : c->ev)
@@ -319,11 +323,13 @@ int cad_upload_static(cad_ctx *c, const double *xx, const double *yy, const doub
     // boundary types this build implements (vf_bwuwn.f, vf_bwuwt.f, vf_bwff.f)
     for (size_t l = 0; l < nb; l++) {
         const int tv = indb[4 * l + 2], tf = indb[4 * l + 3];
-        const bool wave = tv == 5 && tf == 5 && indb[4 * l + 1] <= 4;
-        if (!wave && (!(tv == 1 || tv == 2 || tv == 3 || tv == 4) || !(tf == 1 || tf == 2))) {
+        // wave-maker (5) and radiation (7) faces carry the type for both on
+        // an x or y side (vf_csetup.f:146-205)
+        const bool side = (tv == 5 || tv == 7) && tf == tv && indb[4 * l + 1] <= 4;
+        if (!side && (!(tv == 1 || tv == 2 || tv == 3 || tv == 4) || !(tf == 1 || tf == 2))) {
             cad_set_error("boundary face %zu: velocity type %d / F type %d not implemented "
-                          "(log-law, radiation boundaries; wave-maker faces are type 5 for "
-                          "both on an x or y side)", l + 1, tv, tf);
+                          "(log-law boundaries; wave-maker and radiation faces are type 5 / 7 "
+                          "for both on an x or y side)", l + 1, tv, tf);
             return CAD_ERR_UNSUPPORTED;
         }
         if (indb[4 * l] < 1 || (long long)indb[4 * l] > c->d.nc || indb[4 * l + 1] < 1 ||
@@ -348,12 +354,17 @@ int cad_upload_static(cad_ctx *c, const double *xx, const double *yy, const doub
             }
         }
     }
-    c->n_wave_faces = 0;
-    for (size_t l = 0; l < nb; l++) c->n_wave_faces += indb[4 * l + 2] == 5;
+    c->n_wave_faces = c->n_open_faces = 0;
+    for (size_t l = 0; l < nb; l++) {
+        c->n_wave_faces += indb[4 * l + 2] == 5;
+        c->n_open_faces += indb[4 * l + 2] == 7;
+    }
     RET_IF(h2d(c, c->xx, xx, 6 * (size_t)c->d.ni));
     RET_IF(h2d(c, c->yy, yy, 6 * (size_t)c->d.nj));
     RET_IF(h2d(c, c->zz, zz, 6 * (size_t)c->d.nk));
     std::memcpy(c->zz_host, zz, 6 * (size_t)c->d.nk * sizeof(double));
+    std::memcpy(c->xx_host, xx, 6 * (size_t)c->d.ni * sizeof(double));
+    std::memcpy(c->yy_host, yy, 6 * (size_t)c->d.nj * sizeof(double));
     RET_IF(h2d(c, c->anu, anu, nc));
     RET_IF(h2d(c, c->cd0, cd0, nc));
     RET_IF(h2d(c, c->ggv, ggv, nc));
@@ -463,6 +474,11 @@ static int ready(cad_ctx *c)
     if (c->n_wave_faces > 0 && !(c->wv_on[0] || c->wv_on[1] || c->wv_on[2] || c->wv_on[3])) {
         cad_set_error("%d wave-maker faces (type 5) but no wave-maker side: call "
                       "cad_set_wave_table and cad_set_wave", c->n_wave_faces);
+        return CAD_ERR_STATE;
+    }
+    if (c->n_open_faces > 0 && !(c->op_on[0] || c->op_on[1] || c->op_on[2] || c->op_on[3])) {
+        cad_set_error("%d radiation faces (type 7) but no radiation side: call cad_set_open",
+                      c->n_open_faces);
         return CAD_ERR_STATE;
     }
     CAD_CHECK(hipSetDevice(c->device));
@@ -651,6 +667,12 @@ int cad_set_wave(cad_ctx *c, int side, int l1, int l2, double depth, double peri
 {
     if (!c) return CAD_ERR_ARG;
     return cad_wave_side(c, side, l1, l2, depth, period, ampl, wvlvl);
+}
+
+int cad_set_open(cad_ctx *c, int side, double depth, double period)
+{
+    if (!c) return CAD_ERR_ARG;
+    return cad_open_side(c, side, depth, period);
 }
 
 int cad_get_wave_level(cad_ctx *c, int side, double *level)

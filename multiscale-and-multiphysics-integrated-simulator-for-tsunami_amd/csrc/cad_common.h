@@ -161,11 +161,18 @@ struct cad_ctx {
     double wv_d[4], wv_t[4], wv_a[4], wv_level[4];
     int *wv_err, *wv_err_host;
     int n_wave_faces;   // boundary faces of type 5 in INDB
+    // radiation sides (cad_wave.hip): on, depth, period; the x and y metric
+    // tables on the host for their cell widths
+    int op_on[4];
+    double op_d[4], op_t[4];
+    int n_open_faces;   // boundary faces of type 7 in INDB
+    double *xx_host, *yy_host;
 };
 
 // ---- launchers implemented in the phase files ---------------------------
 int cad_launch_predictor(cad_ctx *c);
-int cad_launch_bwuwn(cad_ctx *c);
+// initial: VF_CINIT's pass (NNOW = 0), else the pass after the predictor
+int cad_launch_bwuwn(cad_ctx *c, bool initial = false);
 int cad_launch_vpcoef(cad_ctx *c);
 int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm, double *xnorm, int *iter);
 int cad_launch_vmodif(cad_ctx *c);
@@ -195,6 +202,7 @@ int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *t
                    const double *hh, const double *un, const double *ut, double t0);
 int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
                   double wvlvl);
-int cad_wave_apply(cad_ctx *c);
+int cad_wave_apply(cad_ctx *c, bool initial);
+int cad_open_side(cad_ctx *c, int side, double depth, double period);
 int cad_wave_check(cad_ctx *c);
 void cad_wave_free(cad_ctx *c);

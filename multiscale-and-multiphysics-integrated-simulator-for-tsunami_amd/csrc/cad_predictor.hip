@@ -233,8 +233,8 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, const double 
 }
 
 // VF_BWUWN boundary-list pass (vf_bwuwn.f:84-254): normal velocity on wall and
-// free boundary faces.  Type 5 (wave-maker) faces are set afterwards by the
-// table wave-maker (cad_wave.hip).
+// free boundary faces.  Type 5 (wave-maker) and type 7 (radiation) faces are
+// set afterwards by the special sides (cad_wave.hip).
 __global__ void k_bwuwn(Dims d, const int *__restrict__ indb, double *u, double *v, double *w,
                         double *bcu, double *bcv, double *bcw)
 {
@@ -292,13 +292,13 @@ int cad_launch_predictor(cad_ctx *c)
     return CAD_OK;
 }
 
-int cad_launch_bwuwn(cad_ctx *c)
+int cad_launch_bwuwn(cad_ctx *c, bool initial)
 {
     const Dims &d = c->d;
     if (d.numb <= 0) return CAD_OK;
     hipLaunchKernelGGL(k_bwuwn, dim3((d.numb + 255) / 256), dim3(256), 0, c->stream, d, c->indb,
                        c->f.uu, c->f.vv, c->f.ww, c->f.bcu, c->f.bcv, c->f.bcw);
     CAD_CHECK(hipGetLastError());
-    // the wave-maker sides (vf_bwuwn.f:257-480)
-    return cad_wave_apply(c);
+    // the wave-maker and radiation sides (vf_bwuwn.f:257-620)
+    return cad_wave_apply(c, initial);
 }

@@ -269,14 +269,14 @@ __global__ void __launch_bounds__(REDUCE_THREADS) k_fcut(Dims d, Static s, const
     }
 }
 
-// Boundary F, vf_bwff.f:33-87 (types FIX and FREE; a wave-maker face, type
-// 5 on the sides 1-4, takes F like FREE, vf_bwff.f:41-76).
+// Boundary F, vf_bwff.f:33-87 (types FIX and FREE; wave-maker and radiation
+// faces, types 5 and 7 on the sides 1-4, take F like FREE).
 __global__ void k_bwff(Dims d, const int *__restrict__ indb, const double *__restrict__ f, double *bcf)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= d.numb) return;
     const int *e = indb + (long long)l * 4;
-    if (!(e[3] == 2 || (e[3] == 5 && e[1] <= 4))) return;
+    if (!(e[3] == 2 || ((e[3] == 5 || e[3] == 7) && e[1] <= 4))) return;
     const long long n = (long long)e[0] - 1;
     switch (e[1]) {
     case 1: bcf[l] = f[n]; break;
@@ -823,7 +823,7 @@ int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, doubl
     if (flags & 2)
         hipLaunchKernelGGL(k_init_p, g, b, 0, st, d, s, f.indc, f.pp, c->prm.rho0, c->prm.grz0, wvlvl);
     CAD_CHECK(hipGetLastError());
-    RET_IF(cad_launch_bwuwn(c));
+    RET_IF(cad_launch_bwuwn(c, true));
     RET_IF(cad_launch_bsuwt(c));
     RET_IF(cad_launch_bsuwt2(c));
     hipLaunchKernelGGL(k_bsuwn3, g, b, 0, st, d, s, f.nf, f.uu, f.vv, f.ww, c->prm.ibsuw0);

@@ -1,4 +1,5 @@
-// cad_wave.hip -- the matrix-table wave-maker sides of VF_BWUWN.
+// cad_wave.hip -- the special sides of VF_BWUWN: the matrix-table
+// wave-maker and the radiation (open) boundary.
 //
 // MODEL WAVE-BC <side> FUNC MATRIX with a LEVEL-ON table (<name>.mtb, read by
 // VF_IM1INP, Source code/vf_im1inp.f) sets, at every call of VF_BWUWN
@@ -14,6 +15,13 @@
 // (VF_CWMTB2), in the reference's order of operations.  Faces of type 5
 // outside a configured side are left as they are, as in the reference's
 // boundary-list pass.
+//
+// MODEL OPEN-BC <side> FUNC TYPE1 (vf_bwuwn.f:483-620) relaxes the boundary
+// values of the side's faces towards the inside velocities of the start of
+// the step with the weights ALN = min(C*DTNOW/dx, 1) and ALT = min(2*ALN, 1),
+// C the celerity of VF_COMAK0's wave length over the period (both weights 1
+// in VF_CINIT's pass); faces next to a surface cell then copy the face below
+// or above.  One thread per column does both passes in the reference's order.
 #include "cad_common.h"
 
 #include <cmath>
@@ -117,6 +125,93 @@ __global__ void __launch_bounds__(64) k_wave_maker(Dims d, const double *__restr
         bcw[l - 1] = ut * vws * w.aw;
         vel[f] = w.xside ? u : v;
     }
+}
+
+struct OpenLaunch {
+    int xside, pos, ic, i2, nq;
+    double aln, alt;
+};
+
+__global__ void __launch_bounds__(64) k_open_side(Dims d, const int *__restrict__ nf,
+                                                  const int *__restrict__ indf,
+                                                  const double *__restrict__ ub,
+                                                  const double *__restrict__ vb,
+                                                  const double *__restrict__ wb, double *vel,
+                                                  double *bcu, double *bcv, double *bcw,
+                                                  OpenLaunch o)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= o.nq) return;
+    const int q = 2 + t;
+    const double aln = o.aln, alt = o.alt;
+    // faces next to fluid cells, and the rest (vf_bwuwn.f:505-527)
+    for (int k = 2; k <= d.nk - 1; k++) {
+        const long long f = o.xside ? cid(d, o.pos, q, k) : cid(d, q, o.pos, k);
+        const int l = indf[f];
+        if (l < 1) continue;
+        const long long cc = o.xside ? cid(d, o.ic, q, k) : cid(d, q, o.ic, k);
+        if (nf[cc] == 0) {
+            if (o.xside) {
+                const double vc = (vb[cc] + vb[cc + d.ni]) * 0.5;
+                const double wc = (wb[cc] + wb[cc + d.nij]) * 0.5;
+                const double u = (1.0 - aln) * bcu[l - 1] + aln * ub[cid(d, o.i2, q, k)];
+                bcu[l - 1] = u;
+                bcv[l - 1] = (1.0 - alt) * bcv[l - 1] + alt * vc;
+                bcw[l - 1] = (1.0 - alt) * bcw[l - 1] + alt * wc;
+                vel[f] = u;
+            } else {
+                const double uc = (ub[cc] + ub[cc + 1]) * 0.5;
+                const double wc = (wb[cc] + wb[cc + d.nij]) * 0.5;
+                bcu[l - 1] = (1.0 - alt) * bcu[l - 1] + alt * uc;
+                const double v = (1.0 - aln) * bcv[l - 1] + aln * vb[cid(d, q, o.i2, k)];
+                bcv[l - 1] = v;
+                bcw[l - 1] = (1.0 - alt) * bcw[l - 1] + alt * wc;
+                vel[f] = v;
+            }
+        } else {
+            bcu[l - 1] = 0.0;
+            bcv[l - 1] = 0.0;
+            bcw[l - 1] = 0.0;
+            vel[f] = 0.0;
+        }
+    }
+    // faces next to surface cells copy the face below (NF = 5) or above
+    // (NF = 6); an x side then has U = 0 under water above, a y side V = BCV
+    // (vf_bwuwn.f:528-552, :596-617)
+    for (int k = 2; k <= d.nk - 1; k++) {
+        const long long f = o.xside ? cid(d, o.pos, q, k) : cid(d, q, o.pos, k);
+        const int l = indf[f];
+        if (l < 1) continue;
+        const int nw = nf[o.xside ? cid(d, o.ic, q, k) : cid(d, q, o.ic, k)];
+        if (nw != 5 && nw != 6) continue;
+        const int l2 = indf[nw == 5 ? f - d.nij : f + d.nij];
+        if (l2 < 1) continue;
+        const double a = bcu[l2 - 1], b = bcv[l2 - 1], cw = bcw[l2 - 1];
+        bcu[l - 1] = a;
+        bcv[l - 1] = b;
+        bcw[l - 1] = cw;
+        if (nw == 5)
+            vel[f] = o.xside ? a : b;
+        else
+            vel[f] = o.xside ? 0.0 : b;
+    }
+}
+
+// VF_COMAK0 (vf_comak0.f): the wave length of a small-amplitude wave of
+// period t in depth dep, Newton's method on the dispersion relation (at most
+// 10 iterations, stop at |dX/X| <= 1e-10).
+double comak0(double dep, double t, double gg)
+{
+    const double dd = 4.0 * PI_REF * PI_REF * dep / gg / t / t;
+    double x = dd >= 1.0 ? dd : std::sqrt(dd);
+    for (int itr = 1; itr <= 10; itr++) {
+        const double ex = std::exp(x), exi = 1.0 / ex;
+        const double coth = (ex + exi) / (ex - exi);
+        const double dx = (x - dd * coth) / (1.0 + dd * (coth * coth - 1.0));
+        x = x - dx;
+        if (std::fabs(dx / x) <= 1.0e-10) break;
+    }
+    return 2.0 * PI_REF * dep / x;
 }
 
 // VF_CWMTB1 (vf_cwmtb1.f): the time weights of the table at phase wvt and the
@@ -225,11 +320,35 @@ int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *t
     return CAD_OK;
 }
 
+int cad_open_side(cad_ctx *c, int side, double depth, double period)
+{
+    if (side < 1 || side > 4) {
+        cad_set_error("cad_set_open: side %d (1..4 = X-, X+, Y-, Y+)", side);
+        return CAD_ERR_ARG;
+    }
+    if (!(depth > 0.0) || !(period > 0.0)) {
+        cad_set_error("cad_set_open: (OPEN-BC DEPTH / PERIOD) NOT FOUND. (vf_csetup.f:128-137)");
+        return CAD_ERR_ARG;
+    }
+    if (c->wv_on[side - 1]) {
+        cad_set_error("cad_set_open: side %d is a wave-maker (DEFINED DOUBLY.)", side);
+        return CAD_ERR_ARG;
+    }
+    c->op_on[side - 1] = 1;
+    c->op_d[side - 1] = depth;
+    c->op_t[side - 1] = period;
+    return CAD_OK;
+}
+
 int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
                   double wvlvl)
 {
     if (side < 1 || side > 4) {
         cad_set_error("cad_set_wave: side %d (1..4 = X-, X+, Y-, Y+)", side);
+        return CAD_ERR_ARG;
+    }
+    if (c->op_on[side - 1]) {
+        cad_set_error("cad_set_wave: side %d is a radiation side (DEFINED DOUBLY.)", side);
         return CAD_ERR_ARG;
     }
     if (!c->wv_tab) {
@@ -257,7 +376,7 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
     return CAD_OK;
 }
 
-int cad_wave_apply(cad_ctx *c)
+int cad_wave_apply(cad_ctx *c, bool initial)
 {
     const Dims &d = c->d;
     bool any = false;
@@ -307,6 +426,43 @@ int cad_wave_apply(cad_ctx *c)
     if (any)
         CAD_CHECK(hipMemcpyAsync(c->wv_err_host, c->wv_err, sizeof(int), hipMemcpyDeviceToHost,
                                  c->stream));
+    // radiation sides (vf_bwuwn.f:483-620); the inside velocities of the
+    // start of the step are the predictor's spare buffers (the pre-step
+    // velocities after its swap), or in VF_CINIT's pass a copy taken here
+    // (vf_cinit.f:376-384)
+    if (!(c->op_on[0] || c->op_on[1] || c->op_on[2] || c->op_on[3])) return CAD_OK;
+    const size_t bytes = (size_t)d.nc * sizeof(double);
+    if (initial) {
+        CAD_CHECK(hipMemcpyAsync(c->u2, c->f.uu, bytes, hipMemcpyDeviceToDevice, c->stream));
+        CAD_CHECK(hipMemcpyAsync(c->v2, c->f.vv, bytes, hipMemcpyDeviceToDevice, c->stream));
+        CAD_CHECK(hipMemcpyAsync(c->w2, c->f.ww, bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int jd = 1; jd <= 4; jd++) {
+        const int q = jd - 1;
+        if (!c->op_on[q]) continue;
+        const double cel = comak0(c->op_d[q], c->op_t[q], c->prm.grz0) / c->op_t[q];
+        OpenLaunch o;
+        o.xside = jd <= 2;
+        o.pos = o.xside ? (jd == 1 ? 2 : d.ni) : (jd == 3 ? 2 : d.nj);
+        o.ic = o.xside ? (jd == 1 ? 2 : d.ni - 1) : (jd == 3 ? 2 : d.nj - 1);
+        o.i2 = (jd == 1 || jd == 3) ? 3 : o.ic;
+        o.nq = (o.xside ? d.nj : d.ni) - 2;
+        o.aln = 1.0;
+        o.alt = 1.0;
+        if (!initial) {
+            // XX(2,IC) or YY(2,IC): the width of the cell next to the side
+            const double *m = o.xside ? c->xx_host : c->yy_host;
+            o.aln = cel * c->prm.dtnow / m[(size_t)(o.ic - 1) * 6 + 1];
+            if (o.aln > 1.0) o.aln = 1.0;
+            o.alt = o.aln * 2.0;
+            if (o.alt > 1.0) o.alt = 1.0;
+        }
+        if (o.nq <= 0) continue;
+        hipLaunchKernelGGL(k_open_side, dim3((o.nq + 63) / 64), dim3(64), 0, c->stream, d, c->f.nf,
+                           o.xside ? c->indx : c->indy, c->u2, c->v2, c->w2,
+                           o.xside ? c->f.uu : c->f.vv, c->f.bcu, c->f.bcv, c->f.bcw, o);
+        CAD_CHECK(hipGetLastError());
+    }
     return CAD_OK;
 }
 

@@ -1,13 +1,15 @@
 """The matrix-table wave-maker (MODEL WAVE-BC X- FUNC MATRIX with a LEVEL-ON
-<name>.mtb table) on the GPU.
+<name>.mtb table) and the radiation boundary (MODEL OPEN-BC X+ FUNC TYPE1) on
+the GPU.
 
-Fixture "wave_flume" (tests/golden/make_golden.py): a 60x4x20 flume at rest
-at depth 0.2 whose X- side follows a solitary-wave table; the reference ran
-400 steps and recorded the phases of steps 1, 2, 100, 200, 300 and 400.  The
-predictor and the boundary pass that sets the wave faces have no reduction, so
-they are compared bit for bit, as is F1CAL (the wave faces take F like FREE);
-whole V1CAL and free-run steps go through the Poisson solve and are compared
-to the plain-form bars.
+Fixtures (tests/golden/make_golden.py): "wave_flume", a 60x4x20 flume at rest
+at depth 0.2 whose X- side follows a solitary-wave table, 400 steps recorded
+at 1, 2, 100, 200, 300, 400; "wave_open", the same on 40x4x20 with a
+radiation boundary at the far end that the wave leaves through, 1000 steps
+recorded at 1, 2, 250, 500, 750, 1000.  The predictor and the boundary pass
+that sets the special faces have no reduction, so they are compared bit for
+bit, as is F1CAL (those faces take F like FREE); whole V1CAL and free-run
+steps go through the Poisson solve and are compared to the plain-form bars.
 """
 import numpy as np
 import pytest
@@ -17,7 +19,8 @@ from helpers import assert_close, assert_same_int, assert_vel_close, rel_err
 
 pytestmark = pytest.mark.gpu
 
-CASE = "wave_flume"
+CASES = ["wave_flume", "wave_open"]
+RSL = {"wave_flume": (100, 400, 100), "wave_open": (250, 1000, 250)}
 STATE = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")
 KEYS = ("rho0", "grz0", "anu0", "scmvp", "flower", "fupper", "plower", "cgpara", "cgepsa",
         "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
@@ -32,6 +35,7 @@ def _ctx(pkg, st, dtnow, tnow):
     c.upload_static(st)
     d = pkg.deck.parse(str(st["DECK"]))
     c.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    c.set_opens(d.opens)
     return c, d
 
 
@@ -42,17 +46,37 @@ def _v1_state(r):
 
 
 def _wave_faces(st):
-    return np.asarray(st["INDB"])[2] == 5
+    """The wave-maker and radiation faces."""
+    t = np.asarray(st["INDB"])[2]
+    return (t == 5) | (t == 7)
 
 
-def test_predictor_with_wave_faces_bitexact(pkg):
+@pytest.mark.parametrize("case", CASES)
+def test_init_with_special_sides(pkg, case):
+    """cad_init_state with the wave-maker at TNOW = 0 and the radiation side's
+    initial pass: the reference's initial state bit for bit."""
+    st = fx.static(case)
+    lay = pkg.layout.from_text(str(st["DECK"]))
+    c, d = _ctx(pkg, st, lay.params()["dtnow"], 0.0)
+    with c:
+        c.upload_state(lay.raw_state())
+        c.init_state(d.wvlvl, d.uini)
+        out = c.download_state()
+    r1 = fx.step(case, 1)
+    for k in ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "NF", "INDC"):
+        assert np.array_equal(out[k], r1["v1_in"][k]), k
+    assert np.array_equal(out["BCF"], r1["f1_in"]["BCF"])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_predictor_with_wave_faces_bitexact(pkg, case):
     """VF_VFLXDU..VF_VEULER then VF_BWUWN with the wave-maker at the step's
     TNOW: every velocity and the wave faces' boundary values bit for bit."""
-    st = fx.static(CASE)
+    st = fx.static(case)
     wave = _wave_faces(st)
-    for s in fx.steps(CASE):
-        r = fx.step(CASE, s)
-        print(f"{CASE} step {s} predictor", flush=True)
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        print(f"{case} step {s} predictor", flush=True)
         c, d = _ctx(pkg, st, float(r["v1_in"]["DTNOW"][0]), float(r["v1_in"]["TNOW"][0]))
         with c:
             c.upload_state(_v1_state(r))
@@ -66,14 +90,15 @@ def test_predictor_with_wave_faces_bitexact(pkg):
         assert np.isfinite(level) and abs(level) < d.waves[1]["depth"]
 
 
-def test_v1cal_with_wave_faces(pkg):
+@pytest.mark.parametrize("case", CASES)
+def test_v1cal_with_wave_faces(pkg, case):
     """Whole VF_V1CAL: the fields to the plain-form bars, the wave faces'
     boundary values (set before the solve) bit for bit."""
-    st = fx.static(CASE)
+    st = fx.static(case)
     wave = _wave_faces(st)
-    for s in fx.steps(CASE):
-        r = fx.step(CASE, s)
-        print(f"{CASE} step {s} v1cal", flush=True)
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        print(f"{case} step {s} v1cal", flush=True)
         c, _ = _ctx(pkg, st, float(r["v1_in"]["DTNOW"][0]), float(r["v1_in"]["TNOW"][0]))
         with c:
             c.upload_state(_v1_state(r))
@@ -87,13 +112,14 @@ def test_v1cal_with_wave_faces(pkg):
             assert np.array_equal(out[k][wave], ref[k][wave]), f"step {s} {k}"
 
 
-def test_f1cal_with_wave_faces_bitexact(pkg):
+@pytest.mark.parametrize("case", CASES)
+def test_f1cal_with_wave_faces_bitexact(pkg, case):
     """VF_F1CAL with the wave faces' F taken like FREE (vf_bwff.f:41-76)."""
-    st = fx.static(CASE)
-    for s in fx.steps(CASE):
-        r = fx.step(CASE, s)
+    st = fx.static(case)
+    for s in fx.steps(case):
+        r = fx.step(case, s)
         fin, ref = r["f1_in"], r["f1_out"]
-        print(f"{CASE} step {s} f1cal", flush=True)
+        print(f"{case} step {s} f1cal", flush=True)
         c, _ = _ctx(pkg, st, float(fin["DTNOW"][0]), float(fin["TNOW"][0]))
         with c:
             c.upload_state({k: fin[k] for k in STATE})
@@ -104,25 +130,28 @@ def test_f1cal_with_wave_faces_bitexact(pkg):
         assert_same_int(f"step {s} NF", out["NF"], ref["NF"])
 
 
-def test_wave_flume_free_run(pkg, tmp_path):
+@pytest.mark.parametrize("case", CASES)
+def test_wave_flume_free_run(pkg, tmp_path, case):
     """The deck through driver.run (the table read from <name>.mtb, the
-    initial state by VF_CINIT's device part) for 400 steps: ITR every step as
-    the reference's list file has it, the fields at the recorded steps from
-    the .rsl to 1e-8."""
-    st = fx.static(CASE)
-    tr = fx.trajectory(CASE)
-    deck = str(st["DECK"]) + "\nFILE RSL STEP 100 400 100\n"
+    initial state by VF_CINIT's device part) to its last step: ITR every step
+    as the reference's list file has it, the fields at the recorded steps
+    from the .rsl to 1e-8."""
+    st = fx.static(case)
+    tr = fx.trajectory(case)
+    a, b, n = RSL[case]
+    deck = str(st["DECK"]) + f"\nFILE RSL STEP {a} {b} {n}\n"
     (tmp_path / "flume.mtb").write_text(str(st["FILE_MTB"]))
     res = pkg.driver.run(deck, "flume", tmp_path)
-    assert res["nnow"] == 400
+    last = int(tr["STEP"][-1])
+    assert res["nnow"] == last
     lines = (tmp_path / "flume.list").read_text().splitlines()
     itr = [int(ln.split("ITR =")[1]) for ln in lines if "ITR =" in ln]
-    assert itr == [int(x) for x in tr["ITR"][: len(itr)]] and len(itr) == 401
+    assert itr == [int(x) for x in tr["ITR"][: len(itr)]] and len(itr) == last + 1
     recs = [r for r in pkg.rsl.read(tmp_path / "flume.rsl")][1:]
-    assert [r["NNOW"] for r in recs] == [100, 200, 300, 400]
+    assert [r["NNOW"] for r in recs] == list(range(a, b + 1, n))
     for rec in recs:
         s = rec["NNOW"]
-        ref = fx.step(CASE, s)["f1_out"]
+        ref = fx.step(case, s)["f1_out"]
         assert_vel_close(f"step {s} velocity", rec, ref, 1e-8)
         for k in ("PP", "FF"):
             assert_close(f"step {s} {k}", rec[k], ref[k], 1e-8)
