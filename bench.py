@@ -124,6 +124,7 @@ def main():
     ap.add_argument("--minv-reps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--form", choices=("plain", "mixed"), default="plain")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -149,13 +150,16 @@ def main():
     ctx = pkg.capi.Context(ni, nj, nk, lay.numb)
     p = lay.params()
     ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
+    if args.form == "mixed":
+        ctx.set_params(precision=pkg.capi.PRECISION_MIXED)
     ctx.upload_static(lay.static_arrays())
     ctx.upload_state(state)
     ctx.init_state(lay.deck.wvlvl, lay.deck.uini, flags)
-    log("bench: state resident on device")
+    log(f"bench: state resident on device, form {args.form}")
     for w in range(args.warmup):
         info = ctx.step()
-        log(f"bench: warmup {w} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms")
+        log(f"bench: warmup {w} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms "
+            f"true residual {ctx.poisson_residual():.3e}")
     itr, poisson, pred, corr, vof = [], [], [], [], []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -167,7 +171,8 @@ def main():
         vof.append(info["ms_vof"])
         log(f"bench: step {s} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms")
     wall = time.perf_counter() - t0
-    log(f"bench: timed {args.steps} steps in {wall:.3f}s ITR={itr}")
+    true_res = ctx.poisson_residual()
+    log(f"bench: timed {args.steps} steps in {wall:.3f}s ITR={itr} true residual {true_res:.3e}")
     ms_step = 1e3 * wall / args.steps
     value = cells * args.steps / wall
     # dominant kernel: M^-1 application on the last step's factorisation
@@ -193,6 +198,7 @@ def main():
                    "grid": list(lay.deck.ncells), "form": "plain FP64",
                    "parallelism": "single GPU"},
         "poisson_ms_per_step": round(float(np.mean(poisson)), 3),
+        "poisson_true_residual": float(f"{true_res:.3e}"),
         "itr_per_step": mean_itr,
         "phase_ms": {"predictor": round(float(np.mean(pred)), 3),
                      "poisson": round(float(np.mean(poisson)), 3),

@@ -19,6 +19,7 @@ module cadmas_hip
   integer(c_int), parameter :: CAD_ERR_ORIENTATION = -40
   integer(c_int), parameter :: CAD_ERR_WAVE = -50
   integer(c_int), parameter :: CAD_PRECISION_FP64 = 0
+  integer(c_int), parameter :: CAD_PRECISION_MIXED = 1
   integer(c_int), parameter :: CAD_INIT_VELOCITY = 1
   integer(c_int), parameter :: CAD_INIT_PRESSURE = 2
 
@@ -201,6 +202,14 @@ module cadmas_hip
       integer(c_int), value :: side
       real(c_double), intent(out) :: level
       integer(c_int) :: cad_get_wave_level
+    end function
+
+    ! ||BB - A*PT|| / ||BB|| of the last pressure solve (FP64)
+    function cad_poisson_residual(ctx, rel) bind(c, name='cad_poisson_residual')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: rel
+      integer(c_int) :: cad_poisson_residual
     end function
 
     function cad_cdiv00(ctx, dv) bind(c, name='cad_cdiv00')

@@ -34,7 +34,8 @@ extern "C" {
 #define CAD_ERR_WAVE (-50)            /* VF_CWMTB1/2 'P.G ERROR.'   vf_cwmtb1.f, vf_cwmtb2.f */
 
 #define CAD_PRECISION_FP64 0          /* plain form: all FP64 */
-#define CAD_PRECISION_MIXED 1         /* FP32 stencils/MILU/momentum, FP64 dots */
+#define CAD_PRECISION_MIXED 1         /* FP64 BiCGSTAB with an FP32 multigrid preconditioner,
+                                         stopped on the true residual (cad_vpsol) */
 
 typedef struct cad_ctx cad_ctx;
 
@@ -170,7 +171,12 @@ int cad_cdiv00(cad_ctx *ctx, double *dv);
 int cad_predictor(cad_ctx *ctx);
 /* VF_VPCOEF (vf_v1cal.f:261) into the context's matrix */
 int cad_vpcoef(cad_ctx *ctx);
-/* VF_VPSOL (vf_vpsol.f:55-76) on the context's matrix; PT in the context */
+/* VF_VPSOL (vf_vpsol.f:55-76) on the context's matrix; PT in the context.
+ * With precision CAD_PRECISION_MIXED the solve is the mixed form's: FP64
+ * BiCGSTAB right-preconditioned by an FP32 aggregation multigrid V-cycle,
+ * stopped when ||BB - A PT|| <= CGEPSR ||BB|| + CGEPSA (the true residual,
+ * rechecked explicitly); info gets ||BB||, the final true relative residual
+ * and the iteration count. */
 int cad_vpsol(cad_ctx *ctx, cad_step_info *info);
 /* VF_VMODIF + VF_BSUWT/BSUWT2/BSUWN/BWUWT/BWPP (vf_v1cal.f:287-297) */
 int cad_correction(cad_ctx *ctx);
@@ -181,6 +187,11 @@ int cad_upload_matrix(cad_ctx *ctx, const double *ad, const double *ali, const d
                       const double *bb, const int *indc);
 int cad_download_pt(cad_ctx *ctx, double *pt);
 
+/* The true relative residual ||BB - A*PT||_2 / ||BB||_2 over the active cells,
+ * in FP64, of the last pressure solve (its matrix and potential).  No
+ * counterpart in the reference (VF_M1BCGS stops on the preconditioned
+ * residual); it is what the two forms' solves are compared on. */
+int cad_poisson_residual(cad_ctx *ctx, double *rel);
 /* Measurement helpers (no counterpart in the reference): average duration of
  * one MILU preconditioner application M^-1 y (vf_mzminv.f) on the current
  * factorisation, timed with HIP events on the context's stream; and the number

@@ -223,6 +223,7 @@ void cad_destroy(cad_ctx *c)
     cad_sweep_free(c);
     cad_tdoor_free(c);
     cad_wave_free(c);
+    cad_mg_free(c);
     delete[] c->zz_host;
     delete[] c->xx_host;
     delete[] c->yy_host;
@@ -244,7 +245,7 @@ int cad_set_params(cad_ctx *c, const cad_params *p)
         cad_set_error("OPTION SUB-LOOP %d: LOOPS must be >= 1 (vf_iiopt.f:44)", p->loops);
         return CAD_ERR_ARG;
     }
-    if (p->precision != CAD_PRECISION_FP64) {
+    if (p->precision != CAD_PRECISION_FP64 && p->precision != CAD_PRECISION_MIXED) {
         cad_set_error("precision mode %d is not available in this build", p->precision);
         return CAD_ERR_UNSUPPORTED;
     }
@@ -508,7 +509,10 @@ int cad_vpsol(cad_ctx *c, cad_step_info *info)
     CAD_CHECK(hipGetLastError());
     double bn = 0.0, xn = 0.0;
     int it = 0;
-    RET_IF(cad_solve_m1bcgs(c, c->pt, &bn, &xn, &it));
+    if (c->prm.precision == CAD_PRECISION_MIXED)
+        RET_IF(cad_solve_mixed(c, c->pt, &bn, &xn, &it));
+    else
+        RET_IF(cad_solve_m1bcgs(c, c->pt, &bn, &xn, &it));
     CAD_CHECK(hipStreamSynchronize(c->stream));
     if (info) {
         info->cgbnrm = bn;
@@ -687,6 +691,13 @@ int cad_get_wave_level(cad_ctx *c, int side, double *level)
     }
     *level = c->wv_level[side - 1];
     return CAD_OK;
+}
+
+int cad_poisson_residual(cad_ctx *c, double *rel)
+{
+    RET_IF(ready(c));
+    if (!rel) return CAD_ERR_ARG;
+    return cad_poisson_residual_impl(c, rel);
 }
 
 int cad_time_minv(cad_ctx *c, int reps, float *ms_avg)

@@ -93,6 +93,17 @@ inline dim3 cell_grid(const Dims &d)
 constexpr int REDUCE_BLOCKS = 1024;
 constexpr int REDUCE_THREADS = 256;
 
+// One level of the mixed form's multigrid (cad_mg.hip): a grid with one ghost
+// layer, its FP32 7-point rows (inactive rows AD = 1, no couplings), the
+// level's unknowns and right-hand side, and the row activity.
+constexpr int CAD_MG_MAX = 12;
+struct MgLevel {
+    Dims d;
+    float *ad, *ali, *alj, *alk, *aui, *auj, *auk;
+    float *x, *b;
+    unsigned char *act;
+};
+
 // Device-side context of one grid (all fields HBM resident).
 struct Fields {
     double *uu, *vv, *ww, *pp, *ff;
@@ -167,6 +178,9 @@ struct cad_ctx {
     double op_d[4], op_t[4];
     int n_open_faces;   // boundary faces of type 7 in INDB
     double *xx_host, *yy_host;
+    // the mixed form's multigrid levels (allocated on first use)
+    MgLevel mg[CAD_MG_MAX];
+    int mg_n;
 };
 
 // ---- launchers implemented in the phase files ---------------------------
@@ -191,6 +205,12 @@ int cad_sweep_dcmp(cad_ctx *c, double u, int *npos_host);
 int cad_sweep_minv(cad_ctx *c, double *x, const double *y);
 int cad_sweep_status(cad_ctx *c, int *code);
 int cad_milu_factor_impl(cad_ctx *c, int *npos);
+int cad_poisson_residual_impl(cad_ctx *c, double *rel);
+int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm, double *xnorm, int *iter);
+int cad_mg_alloc(cad_ctx *c);
+void cad_mg_free(cad_ctx *c);
+int cad_mg_setup(cad_ctx *c);
+int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1);
 int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x);
 Static cad_static(const cad_ctx *c);
 int cad_fnfini(cad_ctx *c);
