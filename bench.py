@@ -7,6 +7,14 @@ preconditioned BiCGSTAB stopped at a relative (preconditioned) residual of
 1e-8.  A "step" is one SMAC time step (predictor, Poisson solve, correction,
 VOF advection and flagging) over the whole grid; fields stay resident in HBM.
 
+Both forms run on the same workload from the same initial state (--form
+selects one): the plain FP64 form (the reference's MILU-BiCGSTAB, stopped on
+its preconditioned residual) gives value, roofline and the step figures; the
+mixed form (FP64 BiCGSTAB with an FP32 aggregation-multigrid preconditioner,
+stopped on the true residual ||BB - A PT|| <= 1e-8 ||BB||) is reported under
+"mixed" with its speed-up and its fields' differences from the plain form's
+after the same steps.
+
 value  = interior cells x timed steps / wall time of the timed steps.
 roofline = the MILU preconditioner application M^-1 (forward + backward
            triangular sweeps, the solver's dominant kernel): 12 FP64 streams per
@@ -114,6 +122,81 @@ def cpu_baseline(pkg, lay, state, flags, steps=1):
     return cells * steps / dt, dt, F.st.icgitr
 
 
+def run_form(pkg, lay, state, flags, form, args):
+    """Warm-up then timed steps of one form on a fresh context; the figures of
+    the bench line for that form."""
+    ni, nj, nk = lay.shape
+    cells = int(np.prod(lay.deck.ncells))
+    ctx = pkg.capi.Context(ni, nj, nk, lay.numb)
+    p = lay.params()
+    ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
+    if form == "mixed":
+        ctx.set_params(precision=pkg.capi.PRECISION_MIXED)
+    ctx.upload_static(lay.static_arrays())
+    ctx.upload_state(state)
+    ctx.init_state(lay.deck.wvlvl, lay.deck.uini, flags)
+    log(f"bench: state resident on device, form {form}")
+    for w in range(args.warmup):
+        info = ctx.step()
+        log(f"bench: {form} warmup {w} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms "
+            f"true residual {ctx.poisson_residual():.3e}")
+    itr, poisson, pred, corr, vof = [], [], [], [], []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        info = ctx.step()
+        itr.append(info["icgitr"])
+        poisson.append(info["ms_poisson"])
+        pred.append(info["ms_predictor"])
+        corr.append(info["ms_correction"])
+        vof.append(info["ms_vof"])
+        log(f"bench: {form} step {s} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms")
+    wall = time.perf_counter() - t0
+    true_res = ctx.poisson_residual()
+    log(f"bench: {form} timed {args.steps} steps in {wall:.3f}s ITR={itr} true residual {true_res:.3e}")
+    end_state = ctx.download_state() if args.form == "both" else None
+    r = {"form": "plain FP64" if form == "plain" else "mixed FP64/FP32",
+         "form_text": ("plain FP64 form, MILU-BiCGSTAB to 1e-8 relative (preconditioned) residual"
+                       if form == "plain" else
+                       "mixed form, FP64 BiCGSTAB with an FP32 aggregation-multigrid preconditioner "
+                       "to 1e-8 relative true residual"),
+         "value": round(cells * args.steps / wall, 1), "ms_per_step": round(1e3 * wall / args.steps, 3),
+         "poisson_ms_per_step": round(float(np.mean(poisson)), 3),
+         "poisson_true_residual": float(f"{true_res:.3e}"),
+         "itr_per_step": float(np.mean(itr)),
+         "phase_ms": {"predictor": round(float(np.mean(pred)), 3),
+                      "poisson": round(float(np.mean(poisson)), 3),
+                      "correction": round(float(np.mean(corr)), 3),
+                      "vof": round(float(np.mean(vof)), 3)}}
+    if form == "plain":
+        # dominant kernel: M^-1 application on the last step's factorisation
+        minv_ms = ctx.time_minv(args.minv_reps)
+        active = ctx.count_active()
+        minv_bytes = 96.0 * active
+        achieved = minv_bytes / (minv_ms * 1e-3) / 1e9
+        r["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                         "traffic": measured_traffic(args.workload),
+                         "kernel": "MILU M^-1 sweeps (forward+backward), 96 B/active cell",
+                         "avg_ms": round(minv_ms, 4), "active_cells": active}
+    ctx.close()
+    r["state"] = end_state
+    return r
+
+
+def field_errors(a, b):
+    """Relative differences of state a against b after the same steps: the
+    velocity components against b's velocity scale, P and F against their own
+    scale, and the number of cells whose flag differs."""
+    vs = max(float(np.max(np.abs(b[k]))) for k in ("UU", "VV", "WW")) or 1.0
+    e = {"velocity": max(float(np.max(np.abs(a[k] - b[k]))) for k in ("UU", "VV", "WW")) / vs}
+    for k, n in (("PP", "pressure"), ("FF", "F")):
+        sc = float(np.max(np.abs(b[k]))) or 1.0
+        e[n] = float(np.max(np.abs(a[k] - b[k]))) / sc
+    out = {k: float(f"{v:.3e}") for k, v in e.items()}
+    out["flags_differing"] = int(np.count_nonzero(a["NF"] != b["NF"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +207,7 @@ def main():
     ap.add_argument("--minv-reps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--form", choices=("plain", "mixed"), default="plain")
+    ap.add_argument("--form", choices=("plain", "mixed", "both"), default="both")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -145,69 +228,41 @@ def main():
     t0 = time.perf_counter()
     lay, state, flags = prepare(pkg, case)
     log(f"bench: {case.name} {lay.deck.ncells} layout {time.perf_counter() - t0:.1f}s")
-    ni, nj, nk = lay.shape
     cells = int(np.prod(lay.deck.ncells))
-    ctx = pkg.capi.Context(ni, nj, nk, lay.numb)
-    p = lay.params()
-    ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
-    if args.form == "mixed":
-        ctx.set_params(precision=pkg.capi.PRECISION_MIXED)
-    ctx.upload_static(lay.static_arrays())
-    ctx.upload_state(state)
-    ctx.init_state(lay.deck.wvlvl, lay.deck.uini, flags)
-    log(f"bench: state resident on device, form {args.form}")
-    for w in range(args.warmup):
-        info = ctx.step()
-        log(f"bench: warmup {w} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms "
-            f"true residual {ctx.poisson_residual():.3e}")
-    itr, poisson, pred, corr, vof = [], [], [], [], []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        info = ctx.step()
-        itr.append(info["icgitr"])
-        poisson.append(info["ms_poisson"])
-        pred.append(info["ms_predictor"])
-        corr.append(info["ms_correction"])
-        vof.append(info["ms_vof"])
-        log(f"bench: step {s} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms")
-    wall = time.perf_counter() - t0
-    true_res = ctx.poisson_residual()
-    log(f"bench: timed {args.steps} steps in {wall:.3f}s ITR={itr} true residual {true_res:.3e}")
-    ms_step = 1e3 * wall / args.steps
-    value = cells * args.steps / wall
-    # dominant kernel: M^-1 application on the last step's factorisation
-    minv_ms = ctx.time_minv(args.minv_reps)
-    active = ctx.count_active()
-    minv_bytes = 96.0 * active
-    achieved = minv_bytes / (minv_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(args.workload),
-                "kernel": "MILU M^-1 sweeps (forward+backward), 96 B/active cell",
-                "avg_ms": round(minv_ms, 4), "active_cells": active}
-    mean_itr = float(np.mean(itr))
-    # step-level algorithmic traffic: 1112 + 496*ITR bytes per cell
-    step_bytes = (1112.0 + 496.0 * mean_itr) * cells
+    forms = ("plain", "mixed") if args.form == "both" else (args.form,)
+    res = {f: run_form(pkg, lay, state, flags, f, args) for f in forms}
+    head = res["plain"] if "plain" in res else res[forms[0]]
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": head["value"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head is res.get("plain") else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"{case.name} plain FP64 form, MILU-BiCGSTAB to 1e-8 relative "
-                               "(preconditioned) residual", "cells": cells,
-                   "grid": list(lay.deck.ncells), "form": "plain FP64",
+        "config": {"workload": f"{case.name}: {head['form_text']}", "cells": cells,
+                   "grid": list(lay.deck.ncells), "form": head["form"],
                    "parallelism": "single GPU"},
-        "poisson_ms_per_step": round(float(np.mean(poisson)), 3),
-        "poisson_true_residual": float(f"{true_res:.3e}"),
-        "itr_per_step": mean_itr,
-        "phase_ms": {"predictor": round(float(np.mean(pred)), 3),
-                     "poisson": round(float(np.mean(poisson)), 3),
-                     "correction": round(float(np.mean(corr)), 3),
-                     "vof": round(float(np.mean(vof)), 3)},
-        "step_algorithmic_GBps": round(step_bytes / (ms_step * 1e-3) / 1e9, 2),
-        "roofline": roofline,
+        "poisson_ms_per_step": head["poisson_ms_per_step"],
+        "poisson_true_residual": head["poisson_true_residual"],
+        "itr_per_step": head["itr_per_step"],
+        "phase_ms": head["phase_ms"],
     }
-    ctx.close()
+    if "plain" in res:
+        pl = res["plain"]
+        step_bytes = (1112.0 + 496.0 * pl["itr_per_step"]) * cells
+        out["step_algorithmic_GBps"] = round(step_bytes / (pl["ms_per_step"] * 1e-3) / 1e9, 2)
+        out["roofline"] = pl["roofline"]
+    if "mixed" in res:
+        mx = res["mixed"]
+        out["mixed"] = {k: mx[k] for k in ("value", "ms_per_step", "poisson_ms_per_step",
+                                           "poisson_true_residual", "itr_per_step", "phase_ms",
+                                           "form_text")}
+        if "plain" in res:
+            out["mixed"]["field_error_vs_plain"] = field_errors(mx["state"], res["plain"]["state"])
+            out["mixed"]["speedup_vs_plain"] = round(mx["value"] / res["plain"]["value"], 3)
+            out["mixed"]["poisson_speedup_vs_plain"] = round(
+                res["plain"]["poisson_ms_per_step"] / mx["poisson_ms_per_step"], 3)
+        if "roofline" not in out:
+            out["roofline"] = None
     if not args.no_cpu_baseline:
         sample = workload(pkg, "buildings_c_slab") if args.workload == "buildings_c" else case
         slay, sstate, sflags = prepare(pkg, sample)
