@@ -181,6 +181,7 @@ struct cad_ctx {
     // the mixed form's multigrid levels (allocated on first use)
     MgLevel mg[CAD_MG_MAX];
     int mg_n;
+    unsigned long long mg_span;   // cells in the active index range of the current solve
 };
 
 // ---- launchers implemented in the phase files ---------------------------

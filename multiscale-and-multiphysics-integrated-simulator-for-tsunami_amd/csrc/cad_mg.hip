@@ -24,7 +24,10 @@
 // cells is summed into the coarse right-hand side in one kernel; the coarse
 // correction is added scaled by MG_SCALE (piecewise-constant prolongation
 // under-corrects smooth errors); the coarsest level is smoothed MG_COARSE
-// times by one workgroup.
+// times.  Every level starts from x = 0, so the first (even) half of its
+// first sweep is x = b/AD, which the kernel that sets b writes at once.  The
+// levels of at most MG_TAIL_CELLS cells run as one workgroup (k_mg_tail), so
+// the small grids cost one launch instead of five per level.
 #include "cad_common.h"
 
 #define RET_IF(x)              \
@@ -37,7 +40,8 @@
 namespace {
 
 constexpr float MG_SCALE = 1.6f;
-constexpr int MG_COARSE = 40;
+constexpr int MG_COARSE = 12;
+constexpr long long MG_TAIL_CELLS = 4096;
 
 __device__ __forceinline__ long long mid(const Dims &d, int i, int j, int k)
 {
@@ -156,8 +160,10 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C)
         s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n);
     }
     const long long m = mid(C.d, I, J, K);
-    C.b[m] = C.act[m] ? s : 0.0f;
-    C.x[m] = 0.0f;
+    const float bc = C.act[m] ? s : 0.0f;
+    C.b[m] = bc;
+    // from x = 0 the even half of the first sweep is x = b/AD
+    C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : 0.0f;
 }
 
 // x += MG_SCALE * coarse x of the block (active fine cells)
@@ -173,44 +179,93 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MgLevel F, MgLevel C, int k0
     F.x[n] += MG_SCALE * C.x[m];
 }
 
-// the coarsest level: MG_COARSE Gauss-Seidel sweeps from x = 0 by one workgroup
-__global__ void __launch_bounds__(1024) k_mg_coarsest(MgLevel L)
+// level-0 right-hand side from the FP64 vector over the active index range
+// [rng[0], rng[1]) (rows outside it are inactive: their x stays 0); from
+// x = 0 the even half of the first sweep is x = b/AD
+__global__ void __launch_bounds__(256) k_mg_in(const unsigned long long *rng, const double *__restrict__ y,
+                                               MgLevel L)
+{
+    const long long n = (long long)rng[0] + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (long long)rng[1]) return;
+    const bool a = L.act[n] != 0;
+    const float b = a ? (float)y[n] : 0.0f;
+    L.b[n] = b;
+    const long long ni = L.d.ni;
+    const long long i = n % ni, j = (n / ni) % L.d.nj, k = n / L.d.nij;
+    L.x[n] = a && ((i + j + k + 3) & 1) == 0 ? b / L.ad[n] : 0.0f;
+}
+
+// FP64 result from level 0 over the active index range (0 on inactive rows)
+__global__ void __launch_bounds__(256) k_mg_out(const unsigned long long *rng, const MgLevel L,
+                                                double *__restrict__ x)
+{
+    const long long n = (long long)rng[0] + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (long long)rng[1]) return;
+    x[n] = L.act[n] ? (double)L.x[n] : 0.0;
+}
+
+// The V-cycle below level t0 inside one workgroup: for each level down to the
+// coarsest the odd half-sweep (the even half was set with b), the coarse
+// right-hand side; MG_COARSE sweeps on the coarsest; then up: correction and
+// one sweep.  Levels are a kernel argument (at most CAD_MG_MAX).
+struct MgTail {
+    MgLevel L[CAD_MG_MAX];
+};
+
+__device__ void tail_half(const MgLevel &L, int color)
 {
     const int ni = L.d.ni - 2, nj = L.d.nj - 2, nk = L.d.nk - 2;
     const int cells = ni * nj * nk;
     for (int q = threadIdx.x; q < cells; q += blockDim.x) {
         const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
-        L.x[mid(L.d, i, j, k)] = 0.0f;
+        if (((i + j + k) & 1) != color) continue;
+        const long long n = mid(L.d, i, j, k);
+        if (L.act[n]) L.x[n] = (L.b[n] - row_off(L, n)) / L.ad[n];
     }
     __syncthreads();
-    for (int s = 0; s < MG_COARSE; s++) {
-        for (int color = 0; color < 2; color++) {
-            for (int q = threadIdx.x; q < cells; q += blockDim.x) {
-                const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
-                if (((i + j + k) & 1) != color) continue;
-                const long long n = mid(L.d, i, j, k);
-                if (L.act[n]) L.x[n] = (L.b[n] - row_off(L, n)) / L.ad[n];
+}
+
+__global__ void __launch_bounds__(1024) k_mg_tail(MgTail T, int t0, int nl)
+{
+    for (int l = t0; l < nl - 1; l++) {
+        const MgLevel &F = T.L[l], &C = T.L[l + 1];
+        tail_half(F, 1);
+        const int ni = C.d.ni - 2, nj = C.d.nj - 2, nk = C.d.nk - 2;
+        for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
+            const int I = q % ni + 2, J = (q / ni) % nj + 2, K = q / (ni * nj) + 2;
+            float s = 0.0f;
+            for (int c = 0; c < 8; c++) {
+                const int i = 2 * I - 2 + (c & 1), j = 2 * J - 2 + ((c >> 1) & 1), k = 2 * K - 2 + (c >> 2);
+                if (i > F.d.ni - 1 || j > F.d.nj - 1 || k > F.d.nk - 1) continue;
+                const long long n = mid(F.d, i, j, k);
+                if (!F.act[n]) continue;
+                s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n);
             }
-            __syncthreads();
+            const long long m = mid(C.d, I, J, K);
+            const float bc = C.act[m] ? s : 0.0f;
+            C.b[m] = bc;
+            C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : 0.0f;
         }
+        __syncthreads();
     }
-}
-
-// level-0 right-hand side from the FP64 vector; x = 0
-__global__ void __launch_bounds__(256) k_mg_in(long long nc, const double *__restrict__ y, MgLevel L)
-{
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nc) return;
-    L.b[n] = L.act[n] ? (float)y[n] : 0.0f;
-    L.x[n] = 0.0f;
-}
-
-// FP64 result from level 0 (0 outside the active rows)
-__global__ void __launch_bounds__(256) k_mg_out(long long nc, const MgLevel L, double *__restrict__ x)
-{
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nc) return;
-    x[n] = L.act[n] ? (double)L.x[n] : 0.0;
+    const MgLevel &B = T.L[nl - 1];
+    tail_half(B, 1);
+    for (int s = 1; s < MG_COARSE; s++) {
+        tail_half(B, 0);
+        tail_half(B, 1);
+    }
+    for (int l = nl - 2; l >= t0; l--) {
+        const MgLevel &F = T.L[l], &C = T.L[l + 1];
+        const int ni = F.d.ni - 2, nj = F.d.nj - 2, nk = F.d.nk - 2;
+        for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
+            const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
+            const long long n = mid(F.d, i, j, k);
+            if (F.act[n]) F.x[n] += MG_SCALE * C.x[mid(C.d, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)];
+        }
+        __syncthreads();
+        tail_half(F, 0);
+        tail_half(F, 1);
+    }
 }
 
 dim3 lvl_grid(const Dims &d, int k0, int k1)
@@ -224,11 +279,12 @@ dim3 half_grid(const Dims &d, int k0, int k1)
     return dim3((unsigned)((half + 63) / 64), (unsigned)((d.nj - 2 + 3) / 4), (unsigned)(k1 - k0 + 1));
 }
 
-int smooth(cad_ctx *c, const MgLevel &L, int k0, int k1)
+// half-sweeps of colours c0..1 on K planes k0..k1
+int smooth(cad_ctx *c, const MgLevel &L, int k0, int k1, int c0)
 {
     const dim3 b(64, 4, 1), g = half_grid(L.d, k0, k1);
-    hipLaunchKernelGGL(k_mg_smooth, g, b, 0, c->stream, L, 0, k0);
-    hipLaunchKernelGGL(k_mg_smooth, g, b, 0, c->stream, L, 1, k0);
+    for (int color = c0; color <= 1; color++)
+        hipLaunchKernelGGL(k_mg_smooth, g, b, 0, c->stream, L, color, k0);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
@@ -308,31 +364,38 @@ int cad_mg_setup(cad_ctx *c)
 // that hold active cells
 int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1)
 {
-    const Dims &d = c->d;
     const dim3 b(64, 4, 1);
-    const unsigned cells = (unsigned)((d.nc + 255) / 256);
-    hipLaunchKernelGGL(k_mg_in, dim3(cells), dim3(256), 0, c->stream, d.nc, y, c->mg[0]);
-    CAD_CHECK(hipGetLastError());
     const int nl = c->mg_n;
-    // down: smooth, then the coarse right-hand side
-    for (int l = 0; l < nl - 1; l++) {
+    const unsigned long long span = c->mg_span;
+    const unsigned vblocks = (unsigned)((span + 255) / 256);
+    hipLaunchKernelGGL(k_mg_in, dim3(vblocks), dim3(256), 0, c->stream, c->act_rng, y, c->mg[0]);
+    CAD_CHECK(hipGetLastError());
+    // the first level small enough for one workgroup
+    int t0 = nl - 1;
+    while (t0 > 0 && (long long)(c->mg[t0 - 1].d.ni - 2) * (c->mg[t0 - 1].d.nj - 2) *
+                             (c->mg[t0 - 1].d.nk - 2) <= MG_TAIL_CELLS)
+        t0--;
+    // down: the odd half-sweep (the even half came with b), the coarse b
+    for (int l = 0; l < t0; l++) {
         const MgLevel &F = c->mg[l], &C = c->mg[l + 1];
         const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
-        RET_IF(smooth(c, F, a0, a1));
+        RET_IF(smooth(c, F, a0, a1, 1));
         hipLaunchKernelGGL(k_mg_restrict, lvl_grid(C.d, 2, C.d.nk - 1), b, 0, c->stream, F, C);
         CAD_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(1024), 0, c->stream, c->mg[nl - 1]);
+    MgTail T;
+    for (int l = 0; l < nl; l++) T.L[l] = c->mg[l];
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, c->stream, T, t0, nl);
     CAD_CHECK(hipGetLastError());
-    // up: correct, then smooth
-    for (int l = nl - 2; l >= 0; l--) {
+    // up: correct, then one sweep
+    for (int l = t0 - 1; l >= 0; l--) {
         const MgLevel &F = c->mg[l], &C = c->mg[l + 1];
         const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
         hipLaunchKernelGGL(k_mg_prolong, lvl_grid(F.d, a0, a1), b, 0, c->stream, F, C, a0);
         CAD_CHECK(hipGetLastError());
-        RET_IF(smooth(c, F, a0, a1));
+        RET_IF(smooth(c, F, a0, a1, 0));
     }
-    hipLaunchKernelGGL(k_mg_out, dim3(cells), dim3(256), 0, c->stream, d.nc, c->mg[0], x);
+    hipLaunchKernelGGL(k_mg_out, dim3(vblocks), dim3(256), 0, c->stream, c->act_rng, c->mg[0], x);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }

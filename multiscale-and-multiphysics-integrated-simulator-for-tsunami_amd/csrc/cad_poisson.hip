@@ -759,6 +759,7 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
         *iter_out = 0;
         return CAD_OK;
     }
+    c->mg_span = hr[1] - hr[0];
     const int k0 = (int)(hr[0] / (unsigned long long)d.nij) + 1;
     const int k1 = (int)((hr[1] - 1) / (unsigned long long)d.nij) + 1;
     RET_IF_(cad_mg_setup(c));
