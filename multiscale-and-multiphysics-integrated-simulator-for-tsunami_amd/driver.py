@@ -111,7 +111,7 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
                 st = None
                 if twriter is not None and trule(nnow, tnow, dt):
                     st = ctx.download_state()
-                    twriter.write_step(tnow, st)
+                    twriter.write_step(tnow, st, {side: ctx.wave_level(side) for side in d.waves})
                 if gwriter is not None and grule(nnow, tnow, dt):
                     st = st if st is not None else ctx.download_state()
                     gwriter.write_step(nnow, tnow, st)

@@ -5,14 +5,16 @@ probes; VF_OT1INI writes their list and the column line
 (vf_ot1ini.f), VF_OT1TRN one line per selected step with TNOW and every
 probe's value (vf_ot1trn.f, FORMAT 9510).  Probes:
   W-LEVEL i j            water level of column (i,j) above W-LEVEL (VF_CWLVL)
+  W-LEVEL ANS side       the wave-maker's level on a side (BCTYP(7,side): the
+                         table level times the ramp at the last boundary pass)
   POINT phy i j k        U, V, W (face values), P or F of a cell
   MIN|MAX|AV|INT phy box minimum, maximum, F-weighted mean and integral over
                          the fluid cells of a box (VF_CAREAP)
   FORCE side box         pressure force on the obstacle faces of a box
                          normal to x, y or z, on its - or + side (VF_CFORCE)
 Indices in the deck are 1-based interior cell numbers.  Single rank; no
-k-epsilon, temperature, scalar or vorticity probes, and W-LEVEL ANS (the
-wave-maker's level) is not produced.
+k-epsilon, temperature, scalar or vorticity probes, and no W-LEVEL ANS SRC
+(wave sources are not modelled).
 """
 from dataclasses import dataclass
 from typing import List
@@ -25,6 +27,7 @@ _KIND = {"POINT": 1, "MIN": 2, "MAX": 3, "AV": 4, "INT": 5, "FORCE": 11}
 _KIND_TEXT = {0: "W-LEVEL", 1: "POINT", 2: "MIN", 3: "MAX", 4: "AV", 5: "INT", 11: "FORCE"}
 _PHY = {"U": 1, "V": 2, "W": 3, "P": 4, "F": 5}
 _SIDE = {"X-": 11, "X+": 12, "Y-": 13, "Y+": 14, "Z-": 15, "Z+": 16}
+_ANS_SIDE = {"X-": 1, "X+": 2, "Y-": 3, "Y+": 4}
 _PHY_TEXT = {0: "------", 1: "U", 2: "V", 3: "W", 4: "P", 5: "F", 11: "X-", 12: "X+", 13: "Y-",
              14: "Y+", 15: "Z-", 16: "Z+"}
 
@@ -50,7 +53,10 @@ def parse(words, shape) -> Probe:
         if len(w) < 3:
             raise TrnError("FILE TRN W-LEVEL: SYNTAX ERROR.")
         if w[1] == "ANS":
-            raise TrnError("FILE TRN W-LEVEL ANS: the wave-maker level is not produced")
+            if w[2] not in _ANS_SIDE:
+                raise TrnError(f"FILE TRN W-LEVEL ANS {w[2]}: not produced here (vf_iifile.f:275-288)")
+            # ITRPRM(3) = -side, the rest 0 (vf_iifile.f:275-289)
+            return Probe(0, 0, (-_ANS_SIDE[w[2]], 0, 0, 0, 0, 0))
         i1, j1 = int(w[1]), int(w[2])
         if not (1 <= i1 <= numi - 2 and 1 <= j1 <= numj - 2):
             raise TrnError("FILE TRN W-LEVEL: INVALID VALUE.")
@@ -94,8 +100,14 @@ def _e15_8(x: float) -> str:
 def header(probes: List[Probe]) -> str:
     """The probe list and column line of VF_OT1INI (FORMAT 9510, 9530)."""
     lines = []
+    sides = {v: k for k, v in _ANS_SIDE.items()}
     for n, p in enumerate(probes, 1):
         t1 = _KIND_TEXT[p.kind].ljust(12)[:10]
+        if p.kind == 0 and p.idx[0] < 0:
+            # FORMAT 9520: the side in A6 in place of the first index
+            nums = "".join(f"{v - 1:6d}" for v in p.idx[1:])
+            lines.append(f" {n:6d} {t1} {'ANS'.ljust(12)} {sides[-p.idx[0]].rjust(6)}{nums}")
+            continue
         t2 = _PHY_TEXT[p.phy].ljust(12)
         nums = "".join(f"{v - 1:6d}" for v in p.idx)
         lines.append(f" {n:6d} {t1} {t2} {nums}")
@@ -225,10 +237,17 @@ def force(probe, st, fields):
     return fm if probe.phy % 2 == 1 else fp
 
 
-def values(probes, st, fields, wvlvl):
+def values(probes, st, fields, wvlvl, wave_levels=None):
+    """The probe values of one step; wave_levels maps a wave-maker side to
+    its level (BCTYP(7,side)) for W-LEVEL ANS."""
     out = []
     for p in probes:
-        if p.kind == 0:
+        if p.kind == 0 and p.idx[0] < 0:
+            side = -p.idx[0]
+            if not wave_levels or side not in wave_levels:
+                raise TrnError(f"W-LEVEL ANS: side {side} has no wave-maker")
+            out.append(float(wave_levels[side]))
+        elif p.kind == 0:
             out.append(water_level(p, st, fields["FF"], fields["NF"], wvlvl))
         elif p.kind == 1:
             arr = {1: "UU", 2: "VV", 3: "WW", 4: "PP", 5: "FF"}[p.phy]
@@ -254,8 +273,8 @@ class Writer:
         self.f = open(path, "w")
         self.f.write(header(self.probes))
 
-    def write_step(self, tnow, fields):
-        self.f.write(line(tnow, values(self.probes, self.static, fields, self.wvlvl)))
+    def write_step(self, tnow, fields, wave_levels=None):
+        self.f.write(line(tnow, values(self.probes, self.static, fields, self.wvlvl, wave_levels)))
 
     def close(self):
         if self.f:

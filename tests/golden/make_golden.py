@@ -99,7 +99,10 @@ _RSL = {"tiny": "FILE RSL STEP 0 5 1\nFILE GRP STEP 0 5 1",
                      "FORCE X- 1 1 1 20 5 16", "FORCE X+ 1 1 1 20 5 16", "FORCE Z- 1 1 1 20 5 16",
                      "FORCE Z+ 1 1 1 20 5 16", "FORCE Y- 1 1 1 20 5 16", "FORCE Y+ 1 1 1 20 5 16"))}
 # result files kept from those runs (the .tran only where FILE TRN is set)
-_OUT = {"tiny": ("rsl", "grp"), "tdoor": ("rsl", "grp", "tran")}
+_RSL["wave_flume"] = "\n".join("FILE TRN " + t for t in (
+    "STEP 0 400 20", "W-LEVEL ANS X-", "W-LEVEL 2 2", "W-LEVEL 30 3", "POINT U 1 2 5", "POINT U 31 2 8",
+    "POINT P 10 2 5", "AV F 1 1 1 60 4 20"))
+_OUT = {"tiny": ("rsl", "grp"), "tdoor": ("rsl", "grp", "tran"), "wave_flume": ("tran",)}
 
 _STEP_RE = re.compile(
     r"STEP=\s*(\d+)\s*:\s*TIME=\s*(\S+)\s*:\s*DT\s*=\s*(\S+)\s*:\s*FSUM=\s*(\S+)\s*:\s*FCUT=\s*(\S+)"

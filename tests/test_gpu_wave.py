@@ -149,6 +149,21 @@ def test_wave_flume_free_run(pkg, tmp_path, case):
     assert itr == [int(x) for x in tr["ITR"][: len(itr)]] and len(itr) == last + 1
     recs = [r for r in pkg.rsl.read(tmp_path / "flume.rsl")][1:]
     assert [r["NNOW"] for r in recs] == list(range(a, b + 1, n))
+    if case == "wave_flume":
+        # the .tran (FILE TRN in the deck): the probe list exactly, the values
+        # every 20 steps within 1e-8 of each probe's scale; the wave-maker's
+        # level (W-LEVEL ANS X-) depends on TNOW only, so to print precision
+        with np.load(fx.HERE / f"{case}_tran.npz") as z:
+            ref = z["TRAN"].tobytes().decode().splitlines()
+        mine = (tmp_path / "flume.tran").read_text().splitlines()
+        nprobe = sum(1 for ln in ref if ln.split() and ln.split()[0].isdigit())
+        assert mine[: nprobe + 1] == ref[: nprobe + 1]
+        ta = np.array([[float(x) for x in ln.split()] for ln in mine[nprobe + 1:]])
+        tb = np.array([[float(x) for x in ln.split()] for ln in ref[nprobe + 1:]])
+        assert ta.shape == tb.shape == (21, nprobe + 1)
+        assert np.array_equal(ta[:, :2], tb[:, :2])
+        scale = np.maximum(np.abs(tb).max(axis=0), 1e-30)
+        assert (np.abs(ta - tb) / scale).max() <= 1e-8
     for rec in recs:
         s = rec["NNOW"]
         ref = fx.step(case, s)["f1_out"]

@@ -160,3 +160,51 @@ def test_time_series_matches_reference(pkg):
         assert got == body[s // 10], (s, got, body[s // 10])
         checked += 1
     assert checked >= 6
+
+
+def test_wave_flume_time_series_matches_reference(pkg):
+    """The wave flume's .tran (FILE TRN STEP 0 400 20 with W-LEVEL ANS X-,
+    water levels, points and an F mean): the probe list and column line, and
+    the value lines of step 0 and the recorded steps.  The wave-maker level
+    (BCTYP(7,1)) of a line is the oracle's boundary pass at that step's TNOW,
+    which pins the oracle's table level and ramp against the reference's file."""
+    import pyoracle as po
+    case = "wave_flume"
+    with np.load(fx.HERE / f"{case}_tran.npz") as z:
+        text = z["TRAN"].tobytes().decode()
+    ref_lines = text.splitlines(keepends=True)
+    st = fx.static(case)
+    d = pkg.deck.parse(str(st["DECK"]))
+    shape = st["GGV"].shape
+    probes = [pkg.tran.parse(w, shape) for w in d.trn_probes]
+    assert pkg.tran.header(probes) == "".join(ref_lines[: len(probes) + 1])
+    body = ref_lines[len(probes) + 1:]
+    tr = fx.trajectory(case)
+    table = pkg.mtb.parse(str(st["FILE_MTB"]))
+
+    def level(rec, tnow):
+        p = fx.deck_params(st["DECK"])
+        p.update(fx.solver_params(st))
+        p["dtnow"] = float(rec["DTNOW"][0])
+        F = po.Fields(fx.state_arrays(st, rec), p)
+        F.set_waves(d.waves, table, d.wvlvl)
+        F.st.tnow = tnow
+        assert F.predictor() == 0
+        return F.st.wv_level[0]
+
+    checked = 0
+    for s in [0] + [s for s in fx.steps(case) if s % 20 == 0]:
+        if s == 0:
+            r1 = fx.step(case, 1)
+            rec = dict(r1["v1_in"])
+            rec["BCF"] = r1["f1_in"]["BCF"]
+            lv = level(r1["v1_in"], 0.0)
+        else:
+            r = fx.step(case, s)
+            rec = r["f1_out"]
+            lv = level(r["v1_in"], float(r["v1_in"]["TNOW"][0]))
+        vals = pkg.tran.values(probes, st, rec, d.wvlvl, {1: lv})
+        got = pkg.tran.line(float(tr["TNOW_FULL"][s]), vals)
+        assert got == body[s // 20], (s, got, body[s // 20])
+        checked += 1
+    assert checked == 5
