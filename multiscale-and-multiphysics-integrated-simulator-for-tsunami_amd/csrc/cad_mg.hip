@@ -39,7 +39,7 @@
 
 namespace {
 
-constexpr float MG_SCALE = 1.6f;
+constexpr float MG_SCALE = 1.8f;
 constexpr int MG_COARSE = 12;
 constexpr long long MG_TAIL_CELLS = 4096;
 
@@ -312,6 +312,7 @@ int cad_mg_alloc(cad_ctx *c)
         MgLevel &L = c->mg[n];
         L = MgLevel();
         L.d = d;
+        c->mg_n = n + 1;   // counted before its arrays, so a failure frees what was allocated
         const size_t nc = (size_t)d.nc;
         float **f[] = {&L.ad, &L.ali, &L.alj, &L.alk, &L.aui, &L.auj, &L.auk, &L.x, &L.b};
         for (float **p : f) {
@@ -321,7 +322,6 @@ int cad_mg_alloc(cad_ctx *c)
         CAD_CHECK(hipMalloc((void **)&L.act, nc));
         CAD_CHECK(hipMemset(L.act, 0, nc));
         n++;
-        c->mg_n = n;
         // coarsen while every interior extent is at least 4
         const int ei = d.ni - 2, ej = d.nj - 2, ek = d.nk - 2;
         if (ei < 4 || ej < 4 || ek < 4) break;
