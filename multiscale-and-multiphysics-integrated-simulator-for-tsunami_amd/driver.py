@@ -52,9 +52,10 @@ class _RslRule:
         return False
 
 
-def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
+def run(deck_text, name, workdir=".", nmax=None, device=0, log=None, precision=None):
     """Run the deck; writes <name>.list (step lines) and, with FILE RSL,
-    <name>.rsl into workdir.  Returns a summary of the last step."""
+    <name>.rsl into workdir.  precision = capi.PRECISION_MIXED runs the
+    pressure solves in the mixed form.  Returns a summary of the last step."""
     workdir = Path(workdir)
     lay = layoutmod.from_text(deck_text)
     d = lay.deck
@@ -63,6 +64,8 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None):
     try:
         p = lay.params()
         ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
+        if precision is not None:
+            ctx.set_params(precision=precision)
         static = lay.static_arrays()
         ctx.upload_static(static)
         if d.waves:

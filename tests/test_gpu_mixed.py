@@ -87,3 +87,26 @@ def test_mixed_free_run_dam_break(pkg):
     assert_vel_close("step 100 velocity", out, ref, 1e-6)
     for k in ("PP", "FF"):
         assert_close(f"step 100 {k}", out[k], ref[k], 1e-6)
+
+
+@pytest.mark.parametrize("case", ["wave_open", "tdoor"])
+def test_mixed_form_with_boundary_and_timer_door_models(pkg, tmp_path, case):
+    """The mixed form under the models around the step: the wave flume with
+    a radiation end (1000 steps) and the droplet deck with both TimerDoor
+    models (300 steps), run from the deck through driver.run: flags equal and
+    fields within 1e-6 of the reference's last step."""
+    st = fx.static(case)
+    tr = fx.trajectory(case)
+    last = int(tr["STEP"][-1])
+    deck = str(st["DECK"]) + f"\nFILE RSL STEP {last} {last} 1\n"
+    if "FILE_MTB" in st:
+        (tmp_path / "run.mtb").write_text(str(st["FILE_MTB"]))
+    res = pkg.driver.run(deck, "run", tmp_path, precision=pkg.capi.PRECISION_MIXED)
+    assert res["nnow"] == last
+    rec = [r for r in pkg.rsl.read(tmp_path / "run.rsl")][1:][-1]
+    ref = fx.step(case, last)["f1_out"]
+    print(f"{case}: mixed run to step {last}", flush=True)
+    assert_same_int(f"step {last} NF", rec["NF"], ref["NF"])
+    assert_vel_close(f"step {last} velocity", rec, ref, 1e-6)
+    for k in ("PP", "FF"):
+        assert_close(f"step {last} {k}", rec[k], ref[k], 1e-6)
