@@ -469,8 +469,20 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
     double xcur = 0.0;
     bool bad_pivot = false;
     v3u ring[HD];
+    // Each step issues one mailbox load (HD steps ahead) and one mailbox
+    // store.  The loads ahead of the first step are interleaved with stores
+    // that fall outside the buffer (dropped), so that the vector-memory
+    // instructions in flight when a stage starts follow the same pattern as
+    // in the middle of a stage: the wait for a ring entry then counts the
+    // HD-1 loads and HD stores issued after it, and the first steps of a
+    // stage do not also wait for the loads and stores of the last few steps.
 #pragma unroll
-    for (int d = 0; d < HD; d++) ring[d] = mb_load(rmb, h_off(d));
+    for (int d = 0; d < HD; d++) {
+        ring[d] = mb_load(rmb, h_off(d));
+        mb_store(rmb, OOB, 0.0, seq);
+        // the stores have one address: keep each (no store merging)
+        __asm__ __volatile__("" ::: "memory");
+    }
     lds_barrier();
     for (int b = 0; b < nstage; b++) {
         // this stage's operands: LDS -> registers
