@@ -22,7 +22,7 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
 ERRORS = {-1: "ARG", -2: "HIP", -3: "STATE", -4: "UNSUPPORTED", -30: "NOT CONVERGED",
-          -31: "INSTABILITY", -32: "CAN NOT DECOMPOSITION", -40: "ORIENTATION"}
+          -31: "INSTABILITY", -32: "CAN NOT DECOMPOSITION", -40: "ORIENTATION", -50: "WAVE TABLE"}
 
 
 class CadError(RuntimeError):

@@ -171,3 +171,46 @@ def test_wave_flume_free_run(pkg, tmp_path, case):
         for k in ("PP", "FF"):
             assert_close(f"step {s} {k}", rec[k], ref[k], 1e-8)
         assert_same_int(f"step {s} NF", rec["NF"], ref["NF"])
+
+
+def test_special_side_errors(pkg):
+    """The wave-maker and radiation entry points refuse what the reference
+    would stop on or what would leave faces unset: stepping a context whose
+    INDB has wave or radiation faces before their sides are set, a side
+    before its table, a side that is taken, bad side numbers and values, a
+    table whose depths do not increase."""
+    st = fx.static("wave_open")
+    d = pkg.deck.parse(str(st["DECK"]))
+    table = pkg.mtb.parse(str(st["FILE_MTB"]))
+    E = pkg.capi.CadError
+    lay = pkg.layout.from_text(str(st["DECK"]))
+    numi, numj, numk = st["GGV"].shape
+    with pkg.capi.Context(numi, numj, numk, st["INDB"].shape[1]) as c:
+        p = lay.params()
+        c.set_params(**{k: v for k, v in p.items() if k in c.params})
+        c.upload_static(st)
+        c.upload_state(lay.raw_state())
+        with pytest.raises(E) as e:
+            c.init_state(d.wvlvl, d.uini)          # type-5 faces, no wave side yet
+        assert e.value.code == -3
+        lib = pkg.capi.lib()
+        assert lib.cad_set_wave(c._h, 1, 2, numj - 1, 0.2, 2.0, 0.0, 0.2) == -3   # no table yet
+        with pytest.raises(E):
+            bad = pkg.mtb.Table(table.kind, table.t0, table.zz[::-1].copy(), table.tt, table.hh,
+                                table.un, table.ut)
+            c.set_waves(d.waves, bad, d.wvlvl)
+        c.set_waves(d.waves, table, d.wvlvl)
+        with pytest.raises(E) as e:
+            c.init_state(d.wvlvl, d.uini)          # type-7 faces, no radiation side yet
+        assert e.value.code == -3
+        assert lib.cad_set_open(c._h, 1, 0.2, 2.0) == -1      # the wave-maker's side
+        assert lib.cad_set_open(c._h, 5, 0.2, 2.0) == -1      # no such side
+        assert lib.cad_set_open(c._h, 2, 0.0, 2.0) == -1      # no depth
+        assert lib.cad_set_wave(c._h, 2, 2, numj, 0.2, 2.0, 0.0, 0.2) == -1   # columns past the side
+        c.set_opens(d.opens)
+        with pytest.raises(E):
+            c.wave_level(2)                        # a radiation side has no wave level
+        c.init_state(d.wvlvl, d.uini)
+        # the level of VF_CINIT's pass: the table's level at its phase origin
+        # (the same table as the wave flume's, whose .tran prints 3.61819724E-06)
+        assert abs(c.wave_level(1) - 3.61819724e-06) <= 1e-14

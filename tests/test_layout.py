@@ -151,3 +151,33 @@ def test_number_reading_follows_the_reference(pkg):
     # leading blanks, an empty field
     assert z("   4") == 4.0 and z("") == 0.0
     assert pkg.deck.zstoi("-12x") == -12 and pkg.deck.zstoi("+3") == 3
+
+
+def test_deck_special_side_words(pkg):
+    """MODEL WAVE-BC ... FUNC MATRIX and MODEL OPEN-BC ... FUNC TYPE1 as
+    Source code/vf_iimdl.f:33-210 and vf_csetup.f:37-205 read them: the
+    numbers a side needs, one model per side, and the boundary types the
+    layout gives the side's faces (5: wave-maker, 7: radiation)."""
+    base = ("GRID X 0 1 2 3 4 END\nGRID Y 0 1 2 3 END\nGRID Z 0 1 2 3 END\nTIME CONST 1e-3\n"
+            "B.C. D VP SLIP\nB.C. D F FREE\nMATE W-LEVEL 1.5\n")
+    wave = "MODEL WAVE-BC X- FUNC MATRIX\nMODEL WAVE-BC X- DEPTH 1.5\nMODEL WAVE-BC X- PERIOD 2.0\n"
+    opn = "MODEL OPEN-BC X+ FUNC TYPE1\nMODEL OPEN-BC X+ DEPTH 1.5\nMODEL OPEN-BC X+ PERIOD 2.0\n"
+    d = pkg.deck.parse(base + wave + opn)
+    assert d.opens == {2: {"func": "TYPE1", "depth": 1.5, "period": 2.0}}
+    assert d.waves[1]["depth"] == 1.5 and d.waves[1]["period"] == 2.0
+    lay = pkg.layout.from_text(base + wave + opn)
+    indb, indx = lay.static_arrays()["INDB"], lay.static_arrays()["INDX"]
+    lx = indx[1, 1:-1, 1:-1].ravel() - 1          # faces of the x- side
+    rx = indx[-1, 1:-1, 1:-1].ravel() - 1         # faces of the x+ side
+    assert np.all(indb[2, lx] == 5) and np.all(indb[3, lx] == 5)
+    assert np.all(indb[2, rx] == 7) and np.all(indb[3, rx] == 7)
+    assert np.count_nonzero(indb[2] == 5) == 3 * 3 and np.count_nonzero(indb[2] == 7) == 3 * 3
+    # a side takes one model; a radiation side needs its depth and period
+    with pytest.raises(pkg.deck.DeckError):
+        pkg.deck.parse(base + wave + "MODEL OPEN-BC X- FUNC TYPE1\n")
+    with pytest.raises(pkg.deck.DeckError):
+        pkg.deck.parse(base + "MODEL OPEN-BC Y- FUNC TYPE1\nMODEL OPEN-BC Y- DEPTH 1.0\n")
+    with pytest.raises(pkg.deck.DeckError):
+        pkg.deck.parse(base + "MODEL OPEN-BC Y- FUNC TYPE2\n")
+    with pytest.raises(pkg.deck.DeckError):
+        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC MATRIX\nMODEL WAVE-BC X- DEPTH 1.5\n")
