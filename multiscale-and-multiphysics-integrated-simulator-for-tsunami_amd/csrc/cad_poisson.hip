@@ -441,6 +441,35 @@ __global__ void __launch_bounds__(256) k_copy_active(const unsigned long long *r
 
 __global__ void k_iset1(int *p, int v) { *p = v; }
 
+// y = A x fused with the partials of (w, y) and, with NPROD = 2, (y, y)
+// (active cells); the mixed form's matvecs, whose next step needs those dots
+template <int NPROD>
+__global__ void __launch_bounds__(REDUCE_THREADS) k_mzax_dot(Dims d, const unsigned long long *rng,
+                                                            const double *__restrict__ ad,
+                                                            const double *__restrict__ ali,
+                                                            const double *__restrict__ alj,
+                                                            const double *__restrict__ alk,
+                                                            const double *__restrict__ aui,
+                                                            const double *__restrict__ auj,
+                                                            const double *__restrict__ auk,
+                                                            const int *__restrict__ indc,
+                                                            const double *__restrict__ x,
+                                                            const double *__restrict__ w,
+                                                            double *__restrict__ y, double *partial)
+{
+    const long long ni = d.ni, nij = d.nij;
+    double acc[3] = {0.0, 0.0, 0.0};
+    ACTIVE_LOOP(n, rng, REDUCE_BLOCKS) {
+        if (indc[n] == -1) continue;
+        const double yn = ad[n] * x[n] + ali[n] * x[n - 1] + alj[n] * x[n - ni] + alk[n] * x[n - nij]
+                        + aui[n] * x[n + 1] + auj[n] * x[n + ni] + auk[n] * x[n + nij];
+        y[n] = yn;
+        acc[0] += w[n] * yn;
+        if (NPROD == 2) acc[1] += yn * yn;
+    }
+    block_reduce_store<NPROD>(acc, partial);
+}
+
 // vector updates of the mixed form's BiCGSTAB (active cells)
 __global__ void __launch_bounds__(256) k_bicg_p(const unsigned long long *rng, const int *__restrict__ mask,
                                                 double beta, double om, const double *__restrict__ r,
@@ -787,7 +816,7 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
     int restarts = 0;
     bool fail = false;
     while (rnorm > tol && !fail) {
-        double rho = 1.0, alpha = 1.0, om = 1.0;
+        double rho = 1.0, alpha = 1.0, om = 1.0, rho1 = 0.0;
         hipLaunchKernelGGL(k_fill, sg, sb, 0, st, nc, p, 0.0);
         hipLaunchKernelGGL(k_fill, sg, sb, 0, st, nc, v, 0.0);
         bool converged = false;
@@ -797,23 +826,28 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
                 break;
             }
             iter++;
-            double rho1 = 0.0;
-            RET_IF_(dot_to(c, r0, r, SC_W1, RED_PLAIN));
-            RET_IF_(fetch_sc(c, SC_W1, 1, &rho1));
+            // (r0, r): explicit after a (re)start, else from the last update
+            if (rho1 == 0.0) {
+                RET_IF_(dot_to(c, r0, r, SC_W1, RED_PLAIN));
+                RET_IF_(fetch_sc(c, SC_W1, 1, &rho1));
+            }
             if (!(std::fabs(rho1) > 0.0) || !std::isfinite(rho1)) break;   // breakdown: restart
             const double beta = (rho1 / rho) * (alpha / om);
             hipLaunchKernelGGL(k_bicg_p, vg, sb, 0, st, rng, mask, beta, om, r, v, p);
             RET_IF_(cad_mg_apply(c, p, ph, k0, k1));
-            RET_IF_(apply_a(c, ph, v));
+            // v = A ph with (r0, v)
+            hipLaunchKernelGGL(k_mzax_dot<1>, rgd, rbd, 0, st, d, rng, c->ad, c->ali, c->alj, c->alk,
+                               c->aui, c->auj, c->auk, mask, ph, r0, v, c->red_partial);
+            RET_IF_(reduce_to(c, 1, SC_W2, RED_PLAIN));
             double rv = 0.0;
-            RET_IF_(dot_to(c, r0, v, SC_W2, RED_PLAIN));
             RET_IF_(fetch_sc(c, SC_W2, 1, &rv));
             if (!(std::fabs(rv) > 0.0) || !std::isfinite(rv)) break;
             alpha = rho1 / rv;
             hipLaunchKernelGGL(k_bicg_s, vg, sb, 0, st, rng, mask, alpha, r, v, s);
             RET_IF_(cad_mg_apply(c, s, sh, k0, k1));
-            RET_IF_(apply_a(c, sh, t));
-            hipLaunchKernelGGL(k_dot2, rgd, rbd, 0, st, rng, mask, s, t, c->red_partial);
+            // t = A sh with (s, t) and (t, t)
+            hipLaunchKernelGGL(k_mzax_dot<2>, rgd, rbd, 0, st, d, rng, c->ad, c->ali, c->alj, c->alk,
+                               c->aui, c->auj, c->auk, mask, sh, s, t, c->red_partial);
             RET_IF_(reduce_to(c, 2, SC_W1B, RED_PLAIN));
             double ts[2] = {0.0, 0.0};
             RET_IF_(fetch_sc(c, SC_W1B, 2, ts));
@@ -822,13 +856,14 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
                                c->red_partial);
             RET_IF_(reduce_to(c, 2, SC_XN, RED_PLAIN));
             double rn[2] = {0.0, 0.0};
-            RET_IF_(fetch_sc(c, SC_XN, 1, rn));
+            RET_IF_(fetch_sc(c, SC_XN, 2, rn));
             rnorm = std::sqrt(rn[0]);
             if (!std::isfinite(rnorm)) {
                 fail = true;
                 break;
             }
             rho = rho1;
+            rho1 = rn[1];           // (r0, r) of the new residual
             if (!(om != 0.0)) break;
             converged = rnorm <= tol;
         }
