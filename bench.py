@@ -16,11 +16,13 @@ stopped on the true residual ||BB - A PT|| <= 1e-8 ||BB||) is reported under
 after the same steps.
 
 value  = interior cells x timed steps / wall time of the timed steps.
-roofline = the MILU preconditioner application M^-1 (forward + backward
+roofline = (plain form) the MILU preconditioner application M^-1 (forward + backward
            triangular sweeps, the solver's dominant kernel): 12 FP64 streams per
            active cell (DD, ALI, ALJ, ALK, y, x forward; DD, AUI, AUJ, AUK, x,
            x backward) = 96 B per active cell per application, divided by its
            average duration timed with HIP events on the library's stream.
+mixed.roofline = one FP32 multigrid V-cycle, timed the same way, against its
+           algorithmic bytes from the level sizes (cad_time_precond).
 cpu_baseline = the C restatement (oracle/) on one host core: three SMAC steps
            of the 512x64x128 slab of the same tank (same beach, wave and cell
            size, 10 buildings), a bounded sample of about 15 s, as
@@ -178,6 +180,14 @@ def run_form(pkg, lay, state, flags, form, args):
                          "traffic": measured_traffic(args.workload),
                          "kernel": "MILU M^-1 sweeps (forward+backward), 96 B/active cell",
                          "avg_ms": round(minv_ms, 4), "active_cells": active}
+    else:
+        # dominant kernel of the mixed form: one multigrid V-cycle
+        vc_ms, vc_bytes = ctx.time_precond(args.minv_reps)
+        achieved = vc_bytes / (vc_ms * 1e-3) / 1e9
+        r["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                         "kernel": "FP32 multigrid V-cycle (the mixed form's preconditioner)",
+                         "avg_ms": round(vc_ms, 4), "bytes": round(vc_bytes)}
     ctx.close()
     r["state"] = end_state
     return r
@@ -255,7 +265,7 @@ def main():
         mx = res["mixed"]
         out["mixed"] = {k: mx[k] for k in ("value", "ms_per_step", "poisson_ms_per_step",
                                            "poisson_true_residual", "itr_per_step", "phase_ms",
-                                           "form_text")}
+                                           "form_text", "roofline")}
         if "plain" in res:
             out["mixed"]["field_error_vs_plain"] = field_errors(mx["state"], res["plain"]["state"])
             out["mixed"]["speedup_vs_plain"] = round(mx["value"] / res["plain"]["value"], 3)

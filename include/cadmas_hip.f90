@@ -204,6 +204,16 @@ module cadmas_hip
       integer(c_int) :: cad_get_wave_level
     end function
 
+    ! the preconditioner application of the context's form, timed after a solve
+    function cad_time_precond(ctx, reps, ms_avg, bytes) bind(c, name='cad_time_precond')
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      real(c_float), intent(out) :: ms_avg
+      real(c_double), intent(out) :: bytes
+      integer(c_int) :: cad_time_precond
+    end function
+
     ! ||BB - A*PT|| / ||BB|| of the last pressure solve (FP64)
     function cad_poisson_residual(ctx, rel) bind(c, name='cad_poisson_residual')
       import :: c_int, c_ptr, c_double

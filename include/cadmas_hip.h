@@ -192,6 +192,13 @@ int cad_download_pt(cad_ctx *ctx, double *pt);
  * counterpart in the reference (VF_M1BCGS stops on the preconditioned
  * residual); it is what the two forms' solves are compared on. */
 int cad_poisson_residual(cad_ctx *ctx, double *rel);
+/* The preconditioner application of the context's form, timed with HIP
+ * events on its stream (average of reps) after a pressure solve: the MILU
+ * M^-1 y (as cad_time_minv; 96 B per active cell) in the plain form, one
+ * multigrid V-cycle in the mixed form (its algorithmic bytes from the level
+ * sizes).  No counterpart in the reference; it is the benchmark's dominant
+ * kernel. */
+int cad_time_precond(cad_ctx *ctx, int reps, float *ms_avg, double *bytes);
 /* Measurement helpers (no counterpart in the reference): average duration of
  * one MILU preconditioner application M^-1 y (vf_mzminv.f) on the current
  * factorisation, timed with HIP events on the context's stream; and the number

@@ -84,6 +84,7 @@ def lib():
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     L.cad_poisson_residual.argtypes = [C.c_void_p, PD]
+    L.cad_time_precond.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
     L.cad_upload_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
     L.cad_download_state.argtypes = [C.c_void_p] + [PD] * 10 + [PI] * 2
@@ -113,7 +114,7 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
-           "cad_get_wave_level", "cad_set_open", "cad_poisson_residual")
+           "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -235,6 +236,13 @@ class Context:
         """Radiation sides: opens = deck.opens (side -> depth, period)."""
         for side, op in sorted(opens.items()):
             _check(lib().cad_set_open(self._h, int(side), float(op["depth"]), float(op["period"])))
+
+    def time_precond(self, reps=5):
+        """(ms, algorithmic bytes) of one preconditioner application of the
+        context's form after a solve."""
+        ms, by = C.c_float(), C.c_double()
+        _check(lib().cad_time_precond(self._h, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def poisson_residual(self):
         """||BB - A PT|| / ||BB|| of the last pressure solve (FP64)."""

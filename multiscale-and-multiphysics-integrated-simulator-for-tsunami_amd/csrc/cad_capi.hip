@@ -693,6 +693,20 @@ int cad_get_wave_level(cad_ctx *c, int side, double *level)
     return CAD_OK;
 }
 
+int cad_time_precond(cad_ctx *c, int reps, float *ms_avg, double *bytes)
+{
+    RET_IF(ready(c));
+    if (!ms_avg || !bytes) return CAD_ERR_ARG;
+    if (c->prm.precision != CAD_PRECISION_MIXED) {
+        RET_IF(cad_time_minv_impl(c, reps, ms_avg));
+        long long active = 0;
+        RET_IF(cad_count_active(c, &active));
+        *bytes = 96.0 * (double)active;
+        return CAD_OK;
+    }
+    return cad_mg_time(c, reps, ms_avg, bytes);
+}
+
 int cad_poisson_residual(cad_ctx *c, double *rel)
 {
     RET_IF(ready(c));

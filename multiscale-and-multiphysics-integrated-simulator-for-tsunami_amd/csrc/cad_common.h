@@ -182,6 +182,7 @@ struct cad_ctx {
     MgLevel mg[CAD_MG_MAX];
     int mg_n;
     unsigned long long mg_span;   // cells in the active index range of the current solve
+    int mg_k0, mg_k1;             // K planes of level 0's passes in the current solve
 };
 
 // ---- launchers implemented in the phase files ---------------------------
@@ -212,6 +213,7 @@ int cad_mg_alloc(cad_ctx *c);
 void cad_mg_free(cad_ctx *c);
 int cad_mg_setup(cad_ctx *c);
 int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1);
+int cad_mg_time(cad_ctx *c, int reps, float *ms_avg, double *bytes);
 int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x);
 Static cad_static(const cad_ctx *c);
 int cad_fnfini(cad_ctx *c);

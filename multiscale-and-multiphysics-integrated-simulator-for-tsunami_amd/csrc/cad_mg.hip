@@ -399,3 +399,41 @@ int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1)
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
+
+// One V-cycle on the hierarchy and K range of the last mixed solve, timed
+// with HIP events on the library's stream (input BB, output AE), and its
+// algorithmic bytes: per cell of a level below the coarsest, three
+// half-sweeps of 20.5 B (one colour's 7 coefficients, b, activity and x
+// written, the other colour's x read), the residual-restriction 38 B (7
+// coefficients, b, x, activity; the coarse b and x) and the prolongation
+// 9.5 B; the coarsest level's sweeps 41 B each; and per cell of the active
+// index range the FP64 -> FP32 input 21 B and the FP32 -> FP64 output 13 B.
+int cad_mg_time(cad_ctx *c, int reps, float *ms_avg, double *bytes)
+{
+    if (c->mg_n <= 0 || c->mg_span == 0) {
+        cad_set_error("cad_time_precond: no mixed solve has run on this context");
+        return CAD_ERR_STATE;
+    }
+    if (reps < 1) reps = 1;
+    hipEvent_t a, b;
+    CAD_CHECK(hipEventCreate(&a));
+    CAD_CHECK(hipEventCreate(&b));
+    CAD_CHECK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; r++) RET_IF(cad_mg_apply(c, c->bb, c->ae, c->mg_k0, c->mg_k1));
+    CAD_CHECK(hipEventRecord(b, c->stream));
+    CAD_CHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    CAD_CHECK(hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms_avg = ms / reps;
+    double by = 34.0 * (double)c->mg_span;
+    for (int l = 0; l < c->mg_n; l++) {
+        const Dims &d = c->mg[l].d;
+        const int planes = l == 0 ? c->mg_k1 - c->mg_k0 + 1 : d.nk - 2;
+        const double cells = (double)(d.ni - 2) * (double)(d.nj - 2) * (double)planes;
+        by += l == c->mg_n - 1 ? 41.0 * MG_COARSE * cells : (3.0 * 20.5 + 38.0 + 9.5) * cells;
+    }
+    *bytes = by;
+    return CAD_OK;
+}

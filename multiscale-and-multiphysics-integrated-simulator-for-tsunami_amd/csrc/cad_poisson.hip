@@ -791,6 +791,8 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
     c->mg_span = hr[1] - hr[0];
     const int k0 = (int)(hr[0] / (unsigned long long)d.nij) + 1;
     const int k1 = (int)((hr[1] - 1) / (unsigned long long)d.nij) + 1;
+    c->mg_k0 = k0;
+    c->mg_k1 = k1;
     RET_IF_(cad_mg_setup(c));
     RET_IF_(dot_to(c, c->bb, c->bb, SC_BN, RED_PLAIN));
     double bn2 = 0.0;
