@@ -23,10 +23,13 @@ roofline = (plain form) the MILU preconditioner application M^-1 (forward + back
            average duration timed with HIP events on the library's stream.
 mixed.roofline = one FP32 multigrid V-cycle, timed the same way, against its
            algorithmic bytes from the level sizes (cad_time_precond).
-cpu_baseline = the C restatement (oracle/) on one host core: three SMAC steps
-           of the 512x64x128 slab of the same tank (same beach, wave and cell
-           size, 10 buildings), a bounded sample of about 15 s, as
-           cell-updates/s.
+cpu_baseline = the reference itself (oracle/_ref/cadmas_ref.x, compiled from
+           its Fortran sources) on one host core: one SMAC step of the
+           512x64x128 slab of the same tank (same beach, wave and cell size, 10
+           buildings) restarted from the slab's initial state, about 15 s, as
+           cell-updates/s; cpu_port = the C restatement (oracle/) on the same
+           slab, three steps.  Without the executable, cpu_baseline is the C
+           restatement's figure.
 
 --calibrate-copy runs one device-to-device copy of 1 GiB (torch) and nothing
 else: its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc scale the counters of
@@ -34,6 +37,7 @@ the sweep kernels to bytes.
 """
 import argparse
 import importlib.util
+import re
 import json
 import os
 import sys
@@ -207,6 +211,62 @@ def field_errors(a, b):
     return out
 
 
+REF_EXE = ROOT / "oracle" / "_ref" / "cadmas_ref.x"
+
+
+def cpu_baseline_reference(pkg, case, lay, state, flags, steps=1, timeout=600):
+    """The reference itself (compiled from its Fortran sources by
+    oracle/build_ref.sh; one rank, one core) on the sample: it restarts from
+    the sample's initial state, written as <name>.res by rsl.Writer from the
+    C restatement's VF_CINIT, and runs `steps` steps; a run of 0 steps from
+    the same file is subtracted, so reading the deck and the restart file are
+    not timed.  None when the executable is not there or fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    if not REF_EXE.exists():
+        return None
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    arr = dict(lay.static_arrays())
+    arr.update(state)
+    F = po.Fields(arr, lay.params())
+    if F.init_state(lay.deck.wvlvl, lay.deck.uini, flags) != 0:
+        return None
+    name = "sample"
+    keys = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF")
+    info = {"fsum": 0.0, "fcut": 0.0, "cgbnrm": 0.0, "cgxnrm": 0.0, "icgitr": 0}
+    body = "\n".join(ln for ln in case.deck.splitlines() if not ln.startswith(("TIME END", "FILE ")))
+    with tempfile.TemporaryDirectory(prefix="cadmas_ref_") as top:
+        top = Path(top)
+        with pkg.rsl.Writer(top / f"{name}.res", lay.shape, lay.numb, lay.static_arrays()["GGV"],
+                            idrop=lay.deck.idrop) as w:
+            w.write_step(0, 0.0, lay.params()["dtnow"], info, {k: F[k] for k in keys},
+                         {"TBUB": np.zeros(lay.shape[2])})
+        times, itr = {}, None
+        for n in (0, steps):
+            run = top / f"run{n}"
+            run.mkdir()
+            shutil.copy(top / f"{name}.res", run / f"{name}.res")
+            (run / f"{name}.in").write_text(body + f"\nTIME END {n} 1.0D9\nFILE RES 0\n")
+            (run / "data.env").write_text(f"1\n{name} 1 0\n")
+            env = {k: v for k, v in os.environ.items() if not k.startswith("CADMAS_DUMP")}
+            env["OMP_NUM_THREADS"] = "1"
+            t0 = time.perf_counter()
+            r = subprocess.run([str(REF_EXE)], cwd=run, env=env, capture_output=True, text=True,
+                               timeout=timeout)
+            times[n] = time.perf_counter() - t0
+            if r.returncode != 0 or "NORMAL END" not in r.stdout:
+                log(f"bench: reference run of {n} steps failed (rc {r.returncode})")
+                return None
+            if n:
+                lst = (run / f"{name}.list").read_text()
+                itr = [int(x) for x in re.findall(r"ITR\s*=\s*(\d+)", lst)][1:]
+    dt = times[steps] - times[0]
+    cells = int(np.prod(lay.deck.ncells))
+    return cells * steps / dt, dt, itr
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,13 +336,24 @@ def main():
     if not args.no_cpu_baseline:
         sample = workload(pkg, "buildings_c_slab") if args.workload == "buildings_c" else case
         slay, sstate, sflags = prepare(pkg, sample)
-        v, dt, it = cpu_baseline(pkg, slay, sstate, sflags, steps=args.cpu_steps)
         scells = int(np.prod(slay.deck.ncells))
-        out["cpu_baseline"] = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1,
-                               "kind": "port",
-                               "sample": f"{args.cpu_steps} SMAC steps of {sample.name} {list(slay.deck.ncells)} "
-                                         f"({scells} cells, {dt:.1f} s, ITR={it}) with the C restatement in "
-                                         "oracle/, one thread"}
+        v, dt, it = cpu_baseline(pkg, slay, sstate, sflags, steps=args.cpu_steps)
+        port = {"value": round(v, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                "sample": f"{args.cpu_steps} SMAC steps of {sample.name} {list(slay.deck.ncells)} "
+                          f"({scells} cells, {dt:.1f} s, ITR={it}) with the C restatement in "
+                          "oracle/, one thread"}
+        ref = cpu_baseline_reference(pkg, sample, slay, sstate, sflags)
+        if ref is None:
+            out["cpu_baseline"] = port
+        else:
+            rv, rdt, ritr = ref
+            out["cpu_baseline"] = {
+                "value": round(rv, 1), "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"1 SMAC step of {sample.name} {list(slay.deck.ncells)} ({scells} cells, "
+                          f"{rdt:.1f} s, ITR={ritr}) by the reference's own Fortran (oracle/_ref, "
+                          "flang -O2, one MPI rank), restarted from the same initial state; the "
+                          "time of a 0-step run from the same restart file is subtracted"}
+            out["cpu_port"] = port
     print(json.dumps(out), flush=True)
 
 
