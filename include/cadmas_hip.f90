@@ -204,6 +204,16 @@ module cadmas_hip
       integer(c_int) :: cad_get_wave_level
     end function
 
+    ! OPTION DRAG-DF yu n with its n rows dr, ap, bt (n = 0: the CD drag form)
+    function cad_set_drag_df(ctx, yu, n, dr, ap, bt) bind(c, name='cad_set_drag_df')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: yu
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: dr(*), ap(*), bt(*)
+      integer(c_int) :: cad_set_drag_df
+    end function
+
     ! the preconditioner application of the context's form, timed after a solve
     function cad_time_precond(ctx, reps, ms_avg, bytes) bind(c, name='cad_time_precond')
       import :: c_int, c_ptr, c_float, c_double

@@ -145,6 +145,16 @@ int cad_set_open(cad_ctx *ctx, int side, double depth, double period);
  * ramp; the W-LEVEL ANS probe of FILE TRN, vf_ot1trn.f). */
 int cad_get_wave_level(cad_ctx *ctx, int side, double *level);
 
+/* Dupuit-Forchheimer porous drag, OPTION DRAG-DF yu n with n lines 'dr ap bt'
+ * (vf_iiopt.f:86-108, applied in VF_VGENE, vf_vgene.f:111-146, 184-222,
+ * 260-298): YU > 0 and n = 1..10 rows of a diameter DR > 0 and coefficients
+ * AP, BT >= 0.  The predictor then takes the drag U(alpha + beta|U|)GV from
+ * the face porosity GV and the face value of POROUS CD read as a diameter,
+ * in place of the CD form.  n = 0 restores the CD form (the default, IDRGN=0
+ * in vf_a2dflt.f:155). */
+int cad_set_drag_df(cad_ctx *ctx, double yu, int n, const double *dr, const double *ap,
+                    const double *bt);
+
 /* Whole phases ------------------------------------------------------------ */
 /* VF_V1CAL (vf_v1cal.f:1-297): predictor, Poisson, correction, boundary values. */
 int cad_v1cal(cad_ctx *ctx, cad_step_info *info);

@@ -83,6 +83,7 @@ def lib():
     L.cad_set_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+    L.cad_set_drag_df.argtypes = [C.c_void_p, C.c_double, C.c_int, PD, PD, PD]
     L.cad_poisson_residual.argtypes = [C.c_void_p, PD]
     L.cad_time_precond.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
@@ -114,7 +115,8 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
-           "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond")
+           "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
+           "cad_set_drag_df")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -236,6 +238,17 @@ class Context:
         """Radiation sides: opens = deck.opens (side -> depth, period)."""
         for side, op in sorted(opens.items()):
             _check(lib().cad_set_open(self._h, int(side), float(op["depth"]), float(op["period"])))
+
+    def set_drag_df(self, drag):
+        """OPTION DRAG-DF: drag = (YU, [(DR, AP, BT), ...]) as deck.drag_df,
+        or None for the CD drag form."""
+        if drag is None:
+            _check(lib().cad_set_drag_df(self._h, 1.0, 0, None, None, None))
+            return
+        yu, rows = drag
+        t = np.ascontiguousarray(np.array(rows, dtype=np.float64).T)
+        n = t.shape[1]
+        _check(lib().cad_set_drag_df(self._h, float(yu), n, _pd(t[0]), _pd(t[1]), _pd(t[2])))
 
     def time_precond(self, reps=5):
         """(ms, algorithmic bytes) of one preconditioner application of the

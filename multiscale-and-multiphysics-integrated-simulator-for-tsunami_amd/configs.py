@@ -137,6 +137,38 @@ def beach_small(ni=40, nj=4, nk=20, dx=0.02, dz=0.01, steps=50, dt=2.0e-4, eps_r
     return Case(name or f"beach_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
 
 
+def porous_df(ni=40, nj=4, nk=20, dx=0.02, dz=0.01, steps=60, dt=2.0e-4, eps_r=1.0e-10,
+              name=None):
+    """The run-up case of beach_small with the Dupuit-Forchheimer drag of
+    OPTION DRAG-DF (Source code/vf_iiopt.f:86-108, vf_vgene.f:111-146).
+
+    The rubble block is two stone sizes, POROUS CD read as the diameter: 3 cm
+    in its front half, 20 cm in its back half.  The three-row table then
+    gives the first row on the block's faces to open water (diameter 1.5 cm),
+    the second row inside the 3 cm half and the last row past the table's end
+    in the 20 cm half; open water (CD = 0) lies below the first diameter and
+    has no drag.
+    """
+    xs = _coords(ni, dx)
+    lines = [f"# porous DF {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", xs)
+    lines += _grid_lines("Y", _coords(nj, dx))
+    lines += _grid_lines("Z", _coords(nk, dz))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE",
+              f"B.C. X 1 1 1 1 {nj} {nk} VP FREE"]
+    _beach_obst(lines, xs, dz, ni, nj, x0=0.55 * ni * dx, slope=0.25, zcap=0.6 * nk * dz)
+    pi1 = int(0.35 * ni) + 1
+    _porous_block(lines, pi1, pi1 + 3, nj, nk, 0.4, 1.2, 0.03)
+    lines.append(f"POROUS CD {pi1 + 2} 1 1 {pi1 + 3} {nj} {nk} 0.2")
+    _common(lines, dt, steps, eps_r, wlevel=0.40 * nk * dz)
+    lines += ["OPTION DRAG-DF 1.0E-6 3",
+              "  0.01  1000.0  1.1",
+              "  0.02  1200.0  1.5",
+              "  0.10  1500.0  2.5"]
+    lines.append(f"DEBUG F-BOX 1 1 1 {int(0.2 * ni)} {nj} {int(0.7 * nk)} 1.0")
+    return Case(name or f"porous_df_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
 def solitary_ic(h, H, x0, g=9.81, rho=1000.0):
     """Boussinesq solitary wave eta = H sech^2(sqrt(3H/4h^3)(x-x0)), u = sqrt(g/h) eta.
 
@@ -366,5 +398,6 @@ def named(name):
         "subloop": lambda: subloop(),
         "wave_flume": lambda: wave_flume(),
         "wave_open": lambda: wave_flume(ni=40, steps=1000, open_end=True),
+        "porous_df": lambda: porous_df(),
     }
     return table[name]()

@@ -679,6 +679,41 @@ int cad_set_open(cad_ctx *c, int side, double depth, double period)
     return cad_open_side(c, side, depth, period);
 }
 
+int cad_set_drag_df(cad_ctx *c, double yu, int n, const double *dr, const double *ap,
+                    const double *bt)
+{
+    if (!c) return CAD_ERR_ARG;
+    if (n == 0) {
+        c->drag.n = 0;
+        return CAD_OK;
+    }
+    // the checks of VF_IIOPT (vf_iiopt.f:89-106)
+    if (n < 0 || n > CAD_DRAG_ROWS || !dr || !ap || !bt) {
+        cad_set_error("cad_set_drag_df: 0..%d rows and non-null tables", CAD_DRAG_ROWS);
+        return CAD_ERR_ARG;
+    }
+    if (!(yu > 0.0)) {
+        cad_set_error("cad_set_drag_df: YU must be positive");
+        return CAD_ERR_ARG;
+    }
+    for (int l = 0; l < n; l++) {
+        if (!(dr[l] > 0.0) || !(ap[l] >= 0.0) || !(bt[l] >= 0.0)) {
+            cad_set_error("cad_set_drag_df: row %d needs DR > 0, AP >= 0, BT >= 0", l + 1);
+            return CAD_ERR_ARG;
+        }
+    }
+    DragDF t = {};
+    t.n = n;
+    t.yu = yu;
+    for (int l = 0; l < n; l++) {
+        t.dr[l] = dr[l];
+        t.ap[l] = ap[l];
+        t.bt[l] = bt[l];
+    }
+    c->drag = t;
+    return CAD_OK;
+}
+
 int cad_get_wave_level(cad_ctx *c, int side, double *level)
 {
     if (!c || !level || side < 1 || side > 4) {

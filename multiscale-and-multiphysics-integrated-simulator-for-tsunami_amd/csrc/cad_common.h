@@ -36,6 +36,16 @@ struct Dims {
     int numb;            // boundary faces
 };
 
+// Dupuit-Forchheimer drag table of OPTION DRAG-DF (vf_iiopt.f:86-108),
+// passed by value to the predictor: n rows (0 = the CD drag form), YU and
+// per row the diameter DR and the coefficients AP, BT.
+#define CAD_DRAG_ROWS 10
+struct DragDF {
+    int n;
+    double yu;
+    double dr[CAD_DRAG_ROWS], ap[CAD_DRAG_ROWS], bt[CAD_DRAG_ROWS];
+};
+
 // Read-only set-up arrays (device pointers).
 struct Static {
     const double *xx, *yy, *zz;
@@ -178,6 +188,7 @@ struct cad_ctx {
     double op_d[4], op_t[4];
     int n_open_faces;   // boundary faces of type 7 in INDB
     double *xx_host, *yy_host;
+    DragDF drag;        // OPTION DRAG-DF (n = 0: the CD form)
     // the mixed form's multigrid levels (allocated on first use)
     MgLevel mg[CAD_MG_MAX];
     int mg_n;

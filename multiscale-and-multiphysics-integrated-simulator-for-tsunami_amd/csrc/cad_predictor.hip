@@ -140,10 +140,12 @@ __device__ __forceinline__ double cross_at(const Static &s, const Dims &d, const
                             ind_of<A>(s), ind_of<B>(s), bcq, nf, su, sc, n, idx[A], idx[B]);
 }
 
-// One face of component A: VF_VGENE source (vf_vgene.f:84-330, IDRGN<=0) and
-// VF_VEULER update with the VVMAX clip (vf_veuler.f:116-249).
-template <int A>
-__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, const double *__restrict__ u,
+// One face of component A: VF_VGENE source (vf_vgene.f:84-330) and
+// VF_VEULER update with the VVMAX clip (vf_veuler.f:116-249).  DF selects
+// the Dupuit-Forchheimer drag of OPTION DRAG-DF (vf_vgene.f:111-146) in
+// place of the CD form (:102-110).
+template <int A, bool DF>
+__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, const double *__restrict__ u,
                                                  const double *__restrict__ v,
                                                  const double *__restrict__ w,
                                                  const double *__restrict__ p,
@@ -204,19 +206,63 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, const double 
     qs = qs + gv * ri * met<A>(s, 5, ia) * (p[n - sa] - p[n]);
     double c[3];
     const double *vv3[3] = {u, v, w};
+    if (!DF) {
 #pragma unroll
-    for (int dd = 0; dd < 3; dd++) {
-        if (dd == A) {
-            c[dd] = q[n];
-        } else {
-            const double *vd = vv3[dd];
-            const long long sd = stride(d, dd);
-            c[dd] = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * (vd[n] + vd[n + sd])
-                                        + met<A>(s, 2, ia) * (vd[n - sa] + vd[n - sa + sd])) * 0.5;
+        for (int dd = 0; dd < 3; dd++) {
+            if (dd == A) {
+                c[dd] = q[n];
+            } else {
+                const double *vd = vv3[dd];
+                const long long sd = stride(d, dd);
+                c[dd] = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * (vd[n] + vd[n + sd])
+                                            + met<A>(s, 2, ia) * (vd[n - sa] + vd[n - sa + sd])) * 0.5;
+            }
         }
+        double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        qs = qs - 0.5 * cd * met<A>(s, 5, ia) * (1.0 - gg_of<A>(s)[n]) * c[A] * uvw;
+    } else {
+        // table row: off below PLOWER or the first diameter, else the row
+        // before the first DR(L) >= CD, the last row past the end
+        // (vf_vgene.f:112-130)
+        double rn, dr, ap, bt;
+        if (gv <= prm.plower || cd <= df.dr[0]) {
+            rn = prm.plower;
+            dr = df.dr[0];
+            ap = 0.0;
+            bt = 0.0;
+        } else {
+            rn = gv;
+            dr = cd;
+            ap = df.ap[df.n - 1];
+            bt = df.bt[df.n - 1];
+            for (int l = 1; l < df.n; l++)
+                if (dr <= df.dr[l]) {
+                    ap = df.ap[l - 1];
+                    bt = df.bt[l - 1];
+                    break;
+                }
+        }
+        const double rn1 = 1.0 - rn;
+        const double alpha = ap * rn1 * rn1 * rn1 * df.yu / (rn * rn * dr * dr);
+        const double beta = bt * rn1 / (rn * rn * rn * dr);
+        // velocities weighted by their face porosities (vf_vgene.f:133-141)
+        const double *gd3[3] = {s.ggx, s.ggy, s.ggz};
+#pragma unroll
+        for (int dd = 0; dd < 3; dd++) {
+            const double *gd = gd3[dd];
+            if (dd == A) {
+                c[dd] = q[n] * gd[n];
+            } else {
+                const double *vd = vv3[dd];
+                const long long sd = stride(d, dd);
+                c[dd] = met<A>(s, 6, ia) * (met<A>(s, 2, ia - 1) * (vd[n] * gd[n] + vd[n + sd] * gd[n + sd])
+                                            + met<A>(s, 2, ia) * (vd[n - sa] * gd[n - sa]
+                                                                  + vd[n - sa + sd] * gd[n - sa + sd])) * 0.5;
+            }
+        }
+        double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        qs = qs - c[A] * (alpha + beta * uvw) * gv;
     }
-    double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    qs = qs - 0.5 * cd * met<A>(s, 5, ia) * (1.0 - gg_of<A>(s)[n]) * c[A] * uvw;
     if (A == 2) {
         qs = qs - gv * prm.grz0;
         double dr = 0.0;
@@ -278,12 +324,22 @@ int cad_launch_predictor(cad_ctx *c)
     const Dims &d = c->d;
     dim3 b = cell_block(), g = cell_grid(d);
     Fields &f = c->f;
-    hipLaunchKernelGGL(k_predict<0>, g, b, 0, c->stream, d, s, f.uu, f.vv, f.ww, f.pp, f.bcu, f.nf,
-                       f.indc, c->u2, c->prm);
-    hipLaunchKernelGGL(k_predict<1>, g, b, 0, c->stream, d, s, f.uu, f.vv, f.ww, f.pp, f.bcv, f.nf,
-                       f.indc, c->v2, c->prm);
-    hipLaunchKernelGGL(k_predict<2>, g, b, 0, c->stream, d, s, f.uu, f.vv, f.ww, f.pp, f.bcw, f.nf,
-                       f.indc, c->w2, c->prm);
+    const DragDF &df = c->drag;
+    if (df.n > 0) {
+        hipLaunchKernelGGL((k_predict<0, true>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
+                           f.bcu, f.nf, f.indc, c->u2, c->prm);
+        hipLaunchKernelGGL((k_predict<1, true>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
+                           f.bcv, f.nf, f.indc, c->v2, c->prm);
+        hipLaunchKernelGGL((k_predict<2, true>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
+                           f.bcw, f.nf, f.indc, c->w2, c->prm);
+    } else {
+        hipLaunchKernelGGL((k_predict<0, false>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
+                           f.bcu, f.nf, f.indc, c->u2, c->prm);
+        hipLaunchKernelGGL((k_predict<1, false>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
+                           f.bcv, f.nf, f.indc, c->v2, c->prm);
+        hipLaunchKernelGGL((k_predict<2, false>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
+                           f.bcw, f.nf, f.indc, c->w2, c->prm);
+    }
     CAD_CHECK(hipGetLastError());
     double *t;
     t = f.uu; f.uu = c->u2; c->u2 = t;

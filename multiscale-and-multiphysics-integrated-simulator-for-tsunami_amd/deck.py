@@ -145,6 +145,9 @@ class Deck:
     # MODEL OPEN-BC per side: radiation boundary (FUNC TYPE1) with the
     # depth and period that give its celerity
     opens: dict = field(default_factory=dict)
+    # OPTION DRAG-DF (vf_iiopt.f:86-108): (DRGYU, [(DRGDR, DRGAP, DRGBT),
+    # ...]) with at most 10 rows; None keeps the CD drag form
+    drag_df: Optional[tuple] = None
     # FILE TRN STEP/TIME and its probe lines (words after FILE TRN)
     trn: Optional[tuple] = None
     trn_probes: List[tuple] = field(default_factory=list)
@@ -239,6 +242,8 @@ def parse(text):
                 raise UnsupportedDeck(f"MATE {' '.join(w[1:3])} is not supported")
         elif head == "COMP":
             _parse_comp(d, w)
+        elif head == "OPTION" and len(w) >= 2 and w[1] == "DRAG-DF":
+            i = _parse_drag_df(d, w, lines, i)
         elif head == "OPTION":
             _parse_option(d, w)
         elif head == "DEBUG":
@@ -509,6 +514,35 @@ def _parse_time(d, w):
         d.tend = _num(w[3])
     else:
         raise DeckError(f"TIME {w[1]}: UNKNOWN WORD.")
+
+
+def _parse_drag_df(d, w, lines, i):
+    """OPTION DRAG-DF yu n, then n lines 'dr ap bt' (Source code/vf_iiopt.f:86-108);
+    returns the index of the line after the table."""
+    if len(w) < 4:
+        raise DeckError("OPTION DRAG-DF: SYNTAX ERROR.")
+    yu = _num(w[2])
+    if yu <= 0.0:
+        raise DeckError("OPTION DRAG-DF: INVALID VALUE.")
+    n = _int(w[3])
+    if n <= 0:
+        raise DeckError("OPTION DRAG-DF: INVALID VALUE.")
+    if n > 10:
+        raise DeckError("OPTION DRAG-DF: AREA IS FULL.")
+    rows = []
+    for _ in range(n):
+        if i >= len(lines):
+            raise DeckError("OPTION DRAG-DF: SYNTAX ERROR.")
+        r = lines[i]
+        i += 1
+        if len(r) != 3:
+            raise DeckError("OPTION DRAG-DF: SYNTAX ERROR.")
+        dr, ap, bt = (_num(x) for x in r)
+        if dr <= 0.0 or ap < 0.0 or bt < 0.0:
+            raise DeckError("OPTION DRAG-DF: INVALID VALUE.")
+        rows.append((dr, ap, bt))
+    d.drag_df = (yu, rows)
+    return i
 
 
 def _parse_option(d, w):

@@ -72,6 +72,7 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None, precision=N
             table = mtbmod.parse((workdir / f"{name}.mtb").read_text())
             ctx.set_waves(d.waves, table, d.wvlvl)
         ctx.set_opens(d.opens)
+        ctx.set_drag_df(d.drag_df)
         restart = d.res_step is not None
         if restart:
             r = rslmod.restart(workdir / f"{name}.res", d.res_step)

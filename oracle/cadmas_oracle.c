@@ -185,7 +185,11 @@ static void flux_cross(orc_state *s, int a, int b, const double *q, const double
 }
 
 /* Source terms of component a: pressure gradient, porous drag, gravity.
- * vf_vgene.f:84-163 (u), :166-241 (v), :244-330 (w); IDRGN<=0 branch. */
+ * vf_vgene.f:84-163 (u), :166-241 (v), :244-330 (w).  The drag is the
+ * CD form (IDRGN<=0, :102-110) or the Dupuit-Forchheimer form of OPTION
+ * DRAG-DF (:111-146): alpha and beta from the porosity GV and the face
+ * diameter CD through the DRGDR table, applied to the porosity-weighted
+ * velocities. */
 static void vgene(orc_state *s, int a, double *qsrc)
 {
     DECL_DIMS(s);
@@ -209,18 +213,59 @@ static void vgene(orc_state *s, int a, double *qsrc)
                 double cd = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * s->cd0[n] + MET(s, a, 2, ia - 1) * s->cd0[n - sa]);
                 qsrc[n] = qsrc[n] + gv * ri * MET(s, a, 5, ia) * (s->pp[n - sa] - s->pp[n]);
                 double c[3];
-                for (int d = 0; d < 3; d++) {
-                    if (d == a) {
-                        c[d] = vel[a][n];
-                    } else {
-                        const double *vd = vel[d];
-                        size_t sd = st[d];
-                        c[d] = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * (vd[n] + vd[n + sd])
-                                                   + MET(s, a, 2, ia) * (vd[n - sa] + vd[n - sa + sd])) * 0.5;
+                if (s->idrgn <= 0) {
+                    for (int d = 0; d < 3; d++) {
+                        if (d == a) {
+                            c[d] = vel[a][n];
+                        } else {
+                            const double *vd = vel[d];
+                            size_t sd = st[d];
+                            c[d] = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * (vd[n] + vd[n + sd])
+                                                       + MET(s, a, 2, ia) * (vd[n - sa] + vd[n - sa + sd])) * 0.5;
+                        }
                     }
+                    double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                    qsrc[n] = qsrc[n] - 0.5 * cd * MET(s, a, 5, ia) * (1.0 - gga[n]) * c[a] * uvw;
+                } else {
+                    /* vf_vgene.f:112-130: below PLOWER or the first diameter
+                     * the drag is off; otherwise the row below the first
+                     * diameter DRGDR(L) >= CD, the last row past the end */
+                    double rn, dr, ap, bt;
+                    if (gv <= s->plower || cd <= s->drgdr[0]) {
+                        rn = s->plower;
+                        dr = s->drgdr[0];
+                        ap = 0.0;
+                        bt = 0.0;
+                    } else {
+                        rn = gv;
+                        dr = cd;
+                        ap = s->drgap[s->idrgn - 1];
+                        bt = s->drgbt[s->idrgn - 1];
+                        for (int l = 1; l < s->idrgn; l++)
+                            if (dr <= s->drgdr[l]) {
+                                ap = s->drgap[l - 1];
+                                bt = s->drgbt[l - 1];
+                                break;
+                            }
+                    }
+                    double rn1 = 1.0 - rn;
+                    double alpha = ap * rn1 * rn1 * rn1 * s->drgyu / (rn * rn * dr * dr);
+                    double beta = bt * rn1 / (rn * rn * rn * dr);
+                    const double *ggd[3] = {s->ggx, s->ggy, s->ggz};
+                    for (int d = 0; d < 3; d++) {
+                        if (d == a) {
+                            c[d] = vel[a][n] * gga[n];
+                        } else {
+                            const double *vd = vel[d], *gd = ggd[d];
+                            size_t sd = st[d];
+                            c[d] = MET(s, a, 6, ia) * (MET(s, a, 2, ia - 1) * (vd[n] * gd[n] + vd[n + sd] * gd[n + sd])
+                                                       + MET(s, a, 2, ia) * (vd[n - sa] * gd[n - sa]
+                                                                             + vd[n - sa + sd] * gd[n - sa + sd])) * 0.5;
+                        }
+                    }
+                    double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                    qsrc[n] = qsrc[n] - c[a] * (alpha + beta * uvw) * gv;
                 }
-                double uvw = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-                qsrc[n] = qsrc[n] - 0.5 * cd * MET(s, a, 5, ia) * (1.0 - gga[n]) * c[a] * uvw;
                 if (a == 2) {
                     qsrc[n] = qsrc[n] - gv * s->grz0;
                     double dr = 0.0;

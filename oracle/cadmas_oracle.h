@@ -73,6 +73,11 @@ typedef struct orc_state {
      * celerity (VF_COMAK0). */
     int op_on[4];
     double op_d[4], op_t[4];
+    /* Dupuit-Forchheimer drag, OPTION DRAG-DF (vf_iiopt.f:86-108): IDRGN
+     * table rows (0 = off, the CD form), DRGYU and per row the diameter
+     * DRGDR and the coefficients DRGAP, DRGBT (MAXDRG = 10 rows). */
+    int idrgn;
+    double drgyu, drgdr[10], drgap[10], drgbt[10];
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */

@@ -49,6 +49,8 @@ class OrcState(C.Structure):
         ("mtbzz", C.c_int), ("mtbtt", C.c_int), ("mtbnow", C.c_int),
         ("mtb_zz", PD), ("mtb_tt", PD), ("mtb_hh", PD), ("mtb_un", PD), ("mtb_ut", PD),
         ("op_on", C.c_int * 4), ("op_d", C.c_double * 4), ("op_t", C.c_double * 4),
+        ("idrgn", C.c_int), ("drgyu", C.c_double), ("drgdr", C.c_double * 10),
+        ("drgap", C.c_double * 10), ("drgbt", C.c_double * 10),
     ]
 
 
@@ -190,6 +192,18 @@ class Fields:
             q = jd - 1
             self.st.op_on[q] = 1
             self.st.op_d[q], self.st.op_t[q] = op["depth"], op["period"]
+
+    def set_drag_df(self, drag):
+        """OPTION DRAG-DF: drag = (DRGYU, [(DRGDR, DRGAP, DRGBT), ...]) or
+        None for the CD form."""
+        st = self.st
+        if drag is None:
+            st.idrgn = 0
+            return
+        yu, rows = drag
+        st.idrgn, st.drgyu = len(rows), yu
+        for l, (dr, ap, bt) in enumerate(rows):
+            st.drgdr[l], st.drgap[l], st.drgbt[l] = dr, ap, bt
 
     def __getitem__(self, k):
         return self.a[k]

@@ -69,6 +69,8 @@ def _cases():
         # ... with a radiation boundary at the far end (MODEL OPEN-BC X+ FUNC TYPE1)
         "wave_open": (lambda: cfg.wave_flume(ni=40, steps=1000, open_end=True, name="wave_open"),
                       [1, 2, 250, 500, 750, 1000]),
+        # Dupuit-Forchheimer drag (OPTION DRAG-DF, vf_vgene.f:111-146)
+        "porous_df": (lambda: cfg.porous_df(name="porous_df"), [1, 2, 3, 30, 60]),
     }
 
 

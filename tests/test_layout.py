@@ -9,7 +9,8 @@ import pytest
 
 import fixtures as fx
 
-CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume"]
+CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume",
+         "porous_df"]
 STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
           "INDX", "INDY", "INDZ", "INDB", "BCVI")
 
@@ -69,6 +70,31 @@ def test_deck_timer_door_words(pkg):
     assert (d.wbub, d.idrop) == (0.0, 1)
     lay = pkg.layout.from_text(base + "B.C. D VP SLIP\nB.C. D F FREE\n")
     assert (lay.params()["wbub"], lay.params()["idrop"]) == (0.2, 1)
+
+
+def test_deck_drag_df_words(pkg):
+    """OPTION DRAG-DF yu n and its n table lines as Source code/vf_iiopt.f:86-108
+    reads them: the table lines are consumed by the option (a deck line after
+    it is read as usual), and each check of the reference is an error."""
+    base = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
+    assert pkg.deck.parse(base).drag_df is None
+    d = pkg.deck.parse(base + "OPTION DRAG-DF 1.0E-6 2\n 0.01 1000 1.1\n"
+                       "# a comment between rows\n 0.05 1500.0 2.0\nOPTION SUB-LOOP 2\n")
+    assert d.drag_df == (pkg.deck.zstor("1.0E-6"), [(pkg.deck.zstor("0.01"), 1000.0, 1.1),
+                                                    (pkg.deck.zstor("0.05"), 1500.0, 2.0)])
+    assert d.loops == 2
+    bad = ["OPTION DRAG-DF 1.0E-6\n",                     # no row count
+           "OPTION DRAG-DF 0.0 1\n0.01 1000 1.1\n",      # YU <= 0
+           "OPTION DRAG-DF 1.0E-6 0\n",                   # no rows
+           "OPTION DRAG-DF 1.0E-6 11\n",                  # more than MAXDRG = 10
+           "OPTION DRAG-DF 1.0E-6 1\n0.01 1000\n",       # two words in a row
+           "OPTION DRAG-DF 1.0E-6 2\n0.01 1000 1.1\n",   # deck ends inside the table
+           "OPTION DRAG-DF 1.0E-6 1\n0.0 1000 1.1\n",    # DR <= 0
+           "OPTION DRAG-DF 1.0E-6 1\n0.01 -1 1.1\n",     # AP < 0
+           "OPTION DRAG-DF 1.0E-6 1\n0.01 1000 -0.1\n"]  # BT < 0
+    for t in bad:
+        with pytest.raises(pkg.deck.DeckError):
+            pkg.deck.parse(base + t)
 
 
 def test_workload_decks_parse(pkg):

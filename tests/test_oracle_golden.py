@@ -348,3 +348,48 @@ def test_wave_maker_free_run_bitexact(pkg, case):
             ref = fx.step(case, s)["f1_out"]
             for k in ("UU", "VV", "WW", "PP", "FF", "NF", "BCU", "BCW"):
                 assert np.array_equal(F[k], ref[k]), (s, k)
+
+
+def test_drag_df_predictor_bitexact(pkg):
+    """VF_VGENE's Dupuit-Forchheimer branch (OPTION DRAG-DF, vf_vgene.f:111-146):
+    the predictor's US/VS/WS equal the reference's at every recorded step of
+    the two-stone rubble block, whose faces reach every row of the table
+    (none below the first diameter, a middle row, the last row), while the
+    CD form of the same arrays differs inside the block."""
+    case = "porous_df"
+    st = fx.static(case)
+    drag = pkg.deck.parse(str(st["DECK"])).drag_df
+    assert drag is not None and len(drag[1]) == 3
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        p = _params(st, r["v1_in"])
+        F = _fields(st, r["v1_in"], p, r["f1_in"]["BCF"])
+        F.set_drag_df(drag)
+        assert F.predictor() == 0
+        for k, kk in (("UU", "US"), ("VV", "VS"), ("WW", "WS")):
+            assert np.array_equal(F[k], r["ps_in"][kk]), (s, k)
+        if s > 1:
+            G = _fields(st, r["v1_in"], p, r["f1_in"]["BCF"])
+            assert G.predictor() == 0
+            assert not np.array_equal(G["UU"], r["ps_in"]["US"]), s
+
+
+def test_drag_df_free_run_bitexact(pkg):
+    """The 60 steps of the run with the Dupuit-Forchheimer drag from the
+    reference's initial state: ITR each step and every field at the recorded
+    steps equal the reference's."""
+    case = "porous_df"
+    st = fx.static(case)
+    r1 = fx.step(case, 1)
+    p = _params(st, r1["v1_in"])
+    F = _fields(st, r1["v1_in"], p, r1["f1_in"]["BCF"])
+    F.set_drag_df(pkg.deck.parse(str(st["DECK"])).drag_df)
+    tr = fx.trajectory(case)
+    for s in range(1, int(tr["STEP"][-1]) + 1):
+        rc, _ = F.step()
+        assert rc == 0
+        assert F.st.icgitr == tr["ITR"][s], s
+        if s in fx.steps(case):
+            ref = fx.step(case, s)["f1_out"]
+            for k in ("UU", "VV", "WW", "PP", "FF", "NF"):
+                assert np.array_equal(F[k], ref[k]), (s, k)
