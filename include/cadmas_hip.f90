@@ -214,6 +214,16 @@ module cadmas_hip
       integer(c_int) :: cad_set_drag_df
     end function
 
+    ! MODEL DAMP side (1..4) with DEGREE (< 0: off), PARAM-XY, PARAM-Z, WIDTH, DEPTH
+    function cad_set_damp(ctx, side, degree, param_xy, param_z, width, depth) &
+        bind(c, name='cad_set_damp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: side, degree
+      real(c_double), value :: param_xy, param_z, width, depth
+      integer(c_int) :: cad_set_damp
+    end function
+
     ! the preconditioner application of the context's form, timed after a solve
     function cad_time_precond(ctx, reps, ms_avg, bytes) bind(c, name='cad_time_precond')
       import :: c_int, c_ptr, c_float, c_double

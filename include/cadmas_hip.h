@@ -155,6 +155,17 @@ int cad_get_wave_level(cad_ctx *ctx, int side, double *level);
 int cad_set_drag_df(cad_ctx *ctx, double yu, int n, const double *dr, const double *ap,
                     const double *bt);
 
+/* A damping zone, MODEL DAMP <side> DEGREE/PARAM-XY/PARAM-Z/WIDTH/DEPTH
+ * (vf_iimdl.f:216-258; IDAMP(side) and DAMP(1..4,side)), applied implicitly
+ * after the predictor's Euler update (VF_VEULER, vf_veuler.f:75-166, 187-221,
+ * 243-277): side 1..4 = X-, X+, Y-, Y+.  Within WIDTH of that side of the grid
+ * a face's velocity is divided by 1 + DTNOW G DMAX/GV0 with DMAX the largest
+ * (DEGREE+1) PARAM sqrt(GRZ0/DEPTH) (1 - dist/WIDTH)**DEGREE over the zones
+ * (PARAM-XY for u and v, PARAM-Z for w).  degree < 0 removes the side's zone
+ * (IDAMP = -1, the default of vf_a2dflt.f:150). */
+int cad_set_damp(cad_ctx *ctx, int side, int degree, double param_xy, double param_z,
+                 double width, double depth);
+
 /* Whole phases ------------------------------------------------------------ */
 /* VF_V1CAL (vf_v1cal.f:1-297): predictor, Poisson, correction, boundary values. */
 int cad_v1cal(cad_ctx *ctx, cad_step_info *info);

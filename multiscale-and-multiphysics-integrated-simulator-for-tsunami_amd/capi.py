@@ -84,6 +84,7 @@ def lib():
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     L.cad_set_drag_df.argtypes = [C.c_void_p, C.c_double, C.c_int, PD, PD, PD]
+    L.cad_set_damp.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_poisson_residual.argtypes = [C.c_void_p, PD]
     L.cad_time_precond.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_upload_static.argtypes = [C.c_void_p] + [PD] * 13 + [PI] * 4 + [PD]
@@ -116,7 +117,7 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
            "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
-           "cad_set_drag_df")
+           "cad_set_drag_df", "cad_set_damp")
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -249,6 +250,17 @@ class Context:
         t = np.ascontiguousarray(np.array(rows, dtype=np.float64).T)
         n = t.shape[1]
         _check(lib().cad_set_drag_df(self._h, float(yu), n, _pd(t[0]), _pd(t[1]), _pd(t[2])))
+
+    def set_damps(self, damps):
+        """Damping zones: damps = deck.damps (side -> degree, pxy, pz, width,
+        depth); sides not named have none."""
+        for side in range(1, 5):
+            dm = damps.get(side)
+            if dm is None:
+                _check(lib().cad_set_damp(self._h, side, -1, 0.0, 0.0, 0.0, 0.0))
+            else:
+                _check(lib().cad_set_damp(self._h, side, int(dm["degree"]), float(dm["pxy"]),
+                                          float(dm["pz"]), float(dm["width"]), float(dm["depth"])))
 
     def time_precond(self, reps=5):
         """(ms, algorithmic bytes) of one preconditioner application of the

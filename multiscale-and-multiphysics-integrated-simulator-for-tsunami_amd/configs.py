@@ -383,6 +383,31 @@ def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400,
     return case
 
 
+def wave_damp(ni=40, steps=600, name=None):
+    """The 40-cell wave flume with a damping zone (MODEL DAMP X+, the
+    implicit sponge of VF_VEULER, Source code/vf_veuler.f:131-166) over its
+    last 0.3 m in place of a wall reflection: DEGREE 4, so the profile
+    (1 - d/W)**4 is a product whose order matters, at the default PARAM-XY
+    and PARAM-Z 0.6."""
+    c = wave_flume(ni=ni, steps=steps, name=name or f"wave_damp_{ni}")
+    c.deck += ("MODEL DAMP X+ DEGREE 4\nMODEL DAMP X+ WIDTH 0.3\n"
+               "MODEL DAMP X+ DEPTH 0.2\n")
+    return c
+
+
+def damp3d(steps=60, name=None):
+    """The three-dimensional corner collapse with two overlapping damping
+    zones on the column's sides (MODEL DAMP X- DEGREE 1 over 6 cm and Y-
+    DEGREE 3 over 4 cm), PARAM-XY and PARAM-Z different, so the corner takes
+    the larger of the two zones and w its own coefficient."""
+    c = corner3d(steps=steps, name=name or "damp3d")
+    c.deck += ("MODEL DAMP X- DEGREE 1\nMODEL DAMP X- WIDTH 0.06\nMODEL DAMP X- DEPTH 0.05\n"
+               "MODEL DAMP X- PARAM-XY 1.5\nMODEL DAMP X- PARAM-Z 0.4\n"
+               "MODEL DAMP Y- DEGREE 3\nMODEL DAMP Y- WIDTH 0.04\nMODEL DAMP Y- DEPTH 0.05\n"
+               "MODEL DAMP Y- PARAM-XY 2.0\n")
+    return c
+
+
 def named(name):
     """The workloads by name (sizes as in the benchmark table)."""
     table = {
@@ -399,5 +424,7 @@ def named(name):
         "wave_flume": lambda: wave_flume(),
         "wave_open": lambda: wave_flume(ni=40, steps=1000, open_end=True),
         "porous_df": lambda: porous_df(),
+        "wave_damp": lambda: wave_damp(),
+        "damp3d": lambda: damp3d(),
     }
     return table[name]()

@@ -714,6 +714,34 @@ int cad_set_drag_df(cad_ctx *c, double yu, int n, const double *dr, const double
     return CAD_OK;
 }
 
+int cad_set_damp(cad_ctx *c, int side, int degree, double param_xy, double param_z, double width,
+                 double depth)
+{
+    if (!c) return CAD_ERR_ARG;
+    if (side < 1 || side > 4) {
+        cad_set_error("cad_set_damp: side must be 1..4 (X-, X+, Y-, Y+), got %d", side);
+        return CAD_ERR_ARG;
+    }
+    const int q = side - 1;
+    if (degree < 0) {
+        c->dm_on[q] = 0;
+        return CAD_OK;
+    }
+    // the checks of VF_IIMDL (vf_iimdl.f:238-257) and VF_CSETUP (vf_csetup.f:221-234)
+    if (!(param_xy >= 1.0e-20) || !(param_z >= 1.0e-20) || !(width >= 1.0e-20) ||
+        !(depth >= 1.0e-20)) {
+        cad_set_error("cad_set_damp: PARAM-XY, PARAM-Z, WIDTH and DEPTH must be positive");
+        return CAD_ERR_ARG;
+    }
+    c->dm_on[q] = 1;
+    c->dm_deg[q] = degree;
+    c->dm_par[q][0] = param_xy;
+    c->dm_par[q][1] = param_z;
+    c->dm_par[q][2] = width;
+    c->dm_par[q][3] = depth;
+    return CAD_OK;
+}
+
 int cad_get_wave_level(cad_ctx *c, int side, double *level)
 {
     if (!c || !level || side < 1 || side > 4) {

@@ -46,6 +46,17 @@ struct DragDF {
     double dr[CAD_DRAG_ROWS], ap[CAD_DRAG_ROWS], bt[CAD_DRAG_ROWS];
 };
 
+// Damping zones of MODEL DAMP (vf_iimdl.f:216-258) as VF_VEULER uses them
+// (vf_veuler.f:75-113), passed by value to the predictor: per side x-, x+,
+// y-, y+ the degree (-1 = off), the width and the coefficients
+// P = (degree+1) PARAM sqrt(GRZ0/DEPTH) for u, v (PARAM-XY) and w (PARAM-Z);
+// the grid's extent (vf_iigrid.f:124-131).
+struct Damp {
+    int deg[4];
+    double w[4], px[4], pz[4];
+    double xmin, xmax, ymin, ymax;
+};
+
 // Read-only set-up arrays (device pointers).
 struct Static {
     const double *xx, *yy, *zz;
@@ -189,6 +200,9 @@ struct cad_ctx {
     int n_open_faces;   // boundary faces of type 7 in INDB
     double *xx_host, *yy_host;
     DragDF drag;        // OPTION DRAG-DF (n = 0: the CD form)
+    // MODEL DAMP per side: on, DEGREE, PARAM-XY, PARAM-Z, WIDTH, DEPTH
+    int dm_on[4], dm_deg[4];
+    double dm_par[4][4];
     // the mixed form's multigrid levels (allocated on first use)
     MgLevel mg[CAD_MG_MAX];
     int mg_n;

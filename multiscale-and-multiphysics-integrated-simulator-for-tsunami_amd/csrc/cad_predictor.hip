@@ -13,6 +13,8 @@
 // components.
 #include "cad_common.h"
 
+#include <cmath>
+
 namespace {
 
 // Normal flux of component A at cell n (FLUU/FLVV/FLWW):
@@ -140,12 +142,27 @@ __device__ __forceinline__ double cross_at(const Static &s, const Dims &d, const
                             ind_of<A>(s), ind_of<B>(s), bcq, nf, su, sc, n, idx[A], idx[B]);
 }
 
+// x**n for n >= 0 by squaring, the lowest bit first (Fortran's REAL**INTEGER
+// as the reference's compiler evaluates it)
+__device__ __forceinline__ double ipow(double x, int n)
+{
+    double r = 1.0;
+    while (n > 0) {
+        if (n & 1) r *= x;
+        n >>= 1;
+        if (n) x *= x;
+    }
+    return r;
+}
+
 // One face of component A: VF_VGENE source (vf_vgene.f:84-330) and
 // VF_VEULER update with the VVMAX clip (vf_veuler.f:116-249).  DF selects
 // the Dupuit-Forchheimer drag of OPTION DRAG-DF (vf_vgene.f:111-146) in
-// place of the CD form (:102-110).
-template <int A, bool DF>
-__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, const double *__restrict__ u,
+// place of the CD form (:102-110); SP the implicit damping of MODEL DAMP
+// after the clip (vf_veuler.f:131-166, 187-221, 243-277).
+template <int A, bool DF, bool SP>
+__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Damp dz,
+                                                 const double *__restrict__ u,
                                                  const double *__restrict__ v,
                                                  const double *__restrict__ w,
                                                  const double *__restrict__ p,
@@ -275,6 +292,25 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, co
     double r = (g * q[n] + prm.dtnow * du) / g0;
     if (r > prm.vvmax) r = prm.vvmax;
     if (r < -prm.vvmax) r = -prm.vvmax;
+    if (SP) {
+        // the largest P (1 - dist/WIDTH)**n over the zones within WIDTH of
+        // the face's (x, y): x at the face for u, at the cell centre else
+        const double x = A == 0 ? metr(s, 0, 1, i) : (metr(s, 0, 1, i) + metr(s, 0, 1, i + 1)) * 0.5;
+        const double y = A == 1 ? metr(s, 1, 1, j) : (metr(s, 1, 1, j) + metr(s, 1, 1, j + 1)) * 0.5;
+        const double dist[4] = {x - dz.xmin, dz.xmax - x, y - dz.ymin, dz.ymax - y};
+        double dmx = 0.0;
+#pragma unroll
+        for (int jd = 0; jd < 4; jd++) {
+            if (dz.deg[jd] < 0) continue;
+            double e = dist[jd];
+            if (e <= dz.w[jd]) {
+                e = (A == 2 ? dz.pz[jd] : dz.px[jd]) * ipow(1.0 - e / dz.w[jd], dz.deg[jd]);
+                if (e > dmx) dmx = e;
+            }
+        }
+        const double gg = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * s.ggv[n] + met<A>(s, 2, ia - 1) * s.ggv[n - sa]);
+        r = r * (g0 / (g0 + prm.dtnow * gg * dmx));
+    }
     out[n] = r;
 }
 
@@ -316,30 +352,58 @@ __global__ void k_bwuwn(Dims d, const int *__restrict__ indb, double *u, double 
     }
 }
 
+// the three components of the predictor, out of place into u2, v2, w2
+template <bool DF, bool SP>
+void launch_predict(cad_ctx *c, const Static &s, const DragDF &df, const Damp &dz)
+{
+    const Dims &d = c->d;
+    dim3 b = cell_block(), g = cell_grid(d);
+    Fields &f = c->f;
+    hipLaunchKernelGGL((k_predict<0, DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww,
+                       f.pp, f.bcu, f.nf, f.indc, c->u2, c->prm);
+    hipLaunchKernelGGL((k_predict<1, DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww,
+                       f.pp, f.bcv, f.nf, f.indc, c->v2, c->prm);
+    hipLaunchKernelGGL((k_predict<2, DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww,
+                       f.pp, f.bcw, f.nf, f.indc, c->w2, c->prm);
+}
+
+// The damping zones at the context's GRZ0, as VF_VEULER derives them on each
+// call (vf_veuler.f:75-113); false when no side has one.
+bool damp_zones(const cad_ctx *c, Damp *dz)
+{
+    int ipa = 0;
+    for (int jd = 0; jd < 4; jd++) {
+        dz->deg[jd] = -1;
+        dz->w[jd] = dz->px[jd] = dz->pz[jd] = 0.0;
+        if (!c->dm_on[jd]) continue;
+        ipa++;
+        const double e = std::sqrt(c->prm.grz0 / c->dm_par[jd][3]);
+        dz->deg[jd] = c->dm_deg[jd];
+        dz->w[jd] = c->dm_par[jd][2];
+        dz->px[jd] = (double)(c->dm_deg[jd] + 1) * c->dm_par[jd][0] * e;
+        dz->pz[jd] = (double)(c->dm_deg[jd] + 1) * c->dm_par[jd][1] * e;
+    }
+    // GLXMIN..GLYMAX: the first and last grid coordinates, XX(1,2), XX(1,NUMI)
+    dz->xmin = c->xx_host[6 * 1];
+    dz->xmax = c->xx_host[6 * (size_t)(c->d.ni - 1)];
+    dz->ymin = c->yy_host[6 * 1];
+    dz->ymax = c->yy_host[6 * (size_t)(c->d.nj - 1)];
+    return ipa > 0;
+}
+
 }  // namespace
 
 int cad_launch_predictor(cad_ctx *c)
 {
     Static s = cad_static(c);
-    const Dims &d = c->d;
-    dim3 b = cell_block(), g = cell_grid(d);
     Fields &f = c->f;
     const DragDF &df = c->drag;
-    if (df.n > 0) {
-        hipLaunchKernelGGL((k_predict<0, true>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
-                           f.bcu, f.nf, f.indc, c->u2, c->prm);
-        hipLaunchKernelGGL((k_predict<1, true>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
-                           f.bcv, f.nf, f.indc, c->v2, c->prm);
-        hipLaunchKernelGGL((k_predict<2, true>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
-                           f.bcw, f.nf, f.indc, c->w2, c->prm);
-    } else {
-        hipLaunchKernelGGL((k_predict<0, false>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
-                           f.bcu, f.nf, f.indc, c->u2, c->prm);
-        hipLaunchKernelGGL((k_predict<1, false>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
-                           f.bcv, f.nf, f.indc, c->v2, c->prm);
-        hipLaunchKernelGGL((k_predict<2, false>), g, b, 0, c->stream, d, s, df, f.uu, f.vv, f.ww, f.pp,
-                           f.bcw, f.nf, f.indc, c->w2, c->prm);
-    }
+    Damp dz;
+    const bool sp = damp_zones(c, &dz);
+    if (df.n > 0 && sp) launch_predict<true, true>(c, s, df, dz);
+    else if (df.n > 0) launch_predict<true, false>(c, s, df, dz);
+    else if (sp) launch_predict<false, true>(c, s, df, dz);
+    else launch_predict<false, false>(c, s, df, dz);
     CAD_CHECK(hipGetLastError());
     double *t;
     t = f.uu; f.uu = c->u2; c->u2 = t;

@@ -148,6 +148,10 @@ class Deck:
     # OPTION DRAG-DF (vf_iiopt.f:86-108): (DRGYU, [(DRGDR, DRGAP, DRGBT),
     # ...]) with at most 10 rows; None keeps the CD drag form
     drag_df: Optional[tuple] = None
+    # MODEL DAMP per side (1 X-, 2 X+, 3 Y-, 4 Y+): the implicit sponge of
+    # VF_VEULER with DEGREE (IDAMP), PARAM-XY, PARAM-Z, WIDTH and DEPTH
+    # (DAMP(1..4,side)); defaults of vf_a2dflt.f:150,189-204
+    damps: dict = field(default_factory=dict)
     # FILE TRN STEP/TIME and its probe lines (words after FILE TRN)
     trn: Optional[tuple] = None
     trn_probes: List[tuple] = field(default_factory=list)
@@ -376,8 +380,11 @@ def _parse_model(d, w):
     if len(w) >= 2 and w[1] == "OPEN-BC":
         _parse_open(d, w)
         return
+    if len(w) >= 2 and w[1] == "DAMP":
+        _parse_damp(d, w)
+        return
     if len(w) < 2 or w[1] != "WAVE-BC":
-        raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (damping, wave sources, "
+        raise UnsupportedDeck(f"MODEL {w[1] if len(w) > 1 else ''} (wave sources, "
                               "turbulence) is not implemented")
     if len(w) < 4:
         raise DeckError("MODEL WAVE-BC: SYNTAX ERROR.")
@@ -412,6 +419,34 @@ def _parse_model(d, w):
         wv["angle"] = _num(w[4])
     else:
         raise DeckError(f"MODEL WAVE-BC {key}: UNKNOWN WORD.")
+
+
+def _parse_damp(d, w):
+    """MODEL DAMP side DEGREE n | PARAM-XY v | PARAM-Z v | WIDTH v | DEPTH v
+    (Source code/vf_iimdl.f:216-258).  A side named here without DEGREE keeps
+    degree -2, which the set-up check rejects."""
+    if len(w) < 3:
+        raise DeckError("MODEL DAMP: SYNTAX ERROR.")
+    if w[2] not in _WAVE_SIDE:
+        raise DeckError(f"MODEL DAMP {w[2]}: UNKNOWN WORD.")
+    dm = d.damps.setdefault(_WAVE_SIDE[w[2]], dict(degree=-2, pxy=0.6, pz=0.6, width=0.0, depth=0.0))
+    if len(w) < 4:
+        raise DeckError("MODEL DAMP: SYNTAX ERROR.")
+    key = w[3]
+    if key not in ("DEGREE", "PARAM-XY", "PARAM-Z", "WIDTH", "DEPTH"):
+        raise DeckError(f"MODEL DAMP {key}: UNKNOWN WORD.")
+    if len(w) < 5:
+        raise DeckError(f"MODEL DAMP {key}: SYNTAX ERROR.")
+    if key == "DEGREE":
+        n = _int(w[4])
+        if n < 0:
+            raise DeckError("MODEL DAMP DEGREE: INVALID VALUE.")
+        dm["degree"] = n
+    else:
+        v = _num(w[4])
+        if v < 1.0e-20:
+            raise DeckError(f"MODEL DAMP {key}: INVALID VALUE.")
+        dm[{"PARAM-XY": "pxy", "PARAM-Z": "pz", "WIDTH": "width", "DEPTH": "depth"}[key]] = v
 
 
 def _parse_file(d, w):
@@ -587,6 +622,14 @@ def _validate(d):
             raise DeckError("MODEL WAVE-BC: (WAVE-BC PERIOD) NOT FOUND.")
         if abs(wv["angle"]) > 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
+    # vf_csetup.f:221-234: a damping side needs DEGREE, WIDTH and DEPTH
+    for jd, dm in d.damps.items():
+        if dm["degree"] < 0:
+            raise DeckError("VF_CSETUP: (DAMP DEGREE) NOT FOUND.")
+        if dm["width"] < 1.0e-20:
+            raise DeckError("VF_CSETUP: (DAMP WIDTH) NOT FOUND.")
+        if dm["depth"] < 1.0e-20:
+            raise DeckError("VF_CSETUP: (DAMP DEPTH) NOT FOUND.")
     # vf_csetup.f:128-137: a radiation boundary needs DEPTH and PERIOD
     for jd, op in d.opens.items():
         if op["depth"] < 1.0e-20:

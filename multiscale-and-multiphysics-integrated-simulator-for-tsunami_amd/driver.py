@@ -73,6 +73,7 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None, precision=N
             ctx.set_waves(d.waves, table, d.wvlvl)
         ctx.set_opens(d.opens)
         ctx.set_drag_df(d.drag_df)
+        ctx.set_damps(d.damps)
         restart = d.res_step is not None
         if restart:
             r = rslmod.restart(workdir / f"{name}.res", d.res_step)

@@ -274,6 +274,19 @@ static void vgene(orc_state *s, int a, double *qsrc)
             }
 }
 
+/* x**n for a non-negative integer n by squaring, the lowest bit first
+ * (Fortran's REAL**INTEGER as the reference's compiler evaluates it). */
+static double ipow(double x, int n)
+{
+    double r = 1.0;
+    while (n > 0) {
+        if (n & 1) r *= x;
+        n >>= 1;
+        if (n) x *= x;
+    }
+    return r;
+}
+
 /* Explicit update of component a and the VVMAX clip.
  * vf_veuler.f:116-158 (u), :161-203 (v), :206-249 (w); GV0 = GV for static
  * porosity (vf_v1cal.f:218-227 passes GLV twice). */
@@ -288,6 +301,17 @@ static void veuler(orc_state *s, int a, double *const fl[3], const double *qsrc)
     if (a == 0) i0 = 3;
     if (a == 1) j0 = 3;
     if (a == 2) k0 = 3;
+    /* damping parameters, vf_veuler.f:75-113: P = (n+1) PARAM sqrt(g/DEPTH),
+     * PARAM-XY for u and v, PARAM-Z for w */
+    int ipa = 0;
+    double pw[4], pp[4];
+    for (int jd = 0; jd < 4; jd++) {
+        if (!s->dm_on[jd]) continue;
+        ipa++;
+        double d = sqrt(s->grz0 / s->dm_par[jd][3]);
+        pw[jd] = s->dm_par[jd][2];
+        pp[jd] = (double)(s->dm_deg[jd] + 1) * s->dm_par[jd][a == 2 ? 1 : 0] * d;
+    }
     for (int k = k0; k <= nk - 1; k++)
         for (int j = j0; j <= nj - 1; j++)
             for (int i = i0; i <= ni - 1; i++) {
@@ -309,6 +333,26 @@ static void veuler(orc_state *s, int a, double *const fl[3], const double *qsrc)
                 q[n] = (gv * q[n] + s->dtnow * du) / gv0;
                 if (q[n] > s->vvmax) q[n] = s->vvmax;
                 if (q[n] < -s->vvmax) q[n] = -s->vvmax;
+                /* implicit damping, vf_veuler.f:131-166 (u), :187-221 (v),
+                 * :243-277 (w): the largest P(1 - dist/WIDTH)**n over the
+                 * zones within WIDTH of the face's (x, y) */
+                if (ipa != 0) {
+                    double x = a == 0 ? MET(s, 0, 1, i) : (MET(s, 0, 1, i) + MET(s, 0, 1, i + 1)) * 0.5;
+                    double y = a == 1 ? MET(s, 1, 1, j) : (MET(s, 1, 1, j) + MET(s, 1, 1, j + 1)) * 0.5;
+                    const double dist[4] = {x - s->glxmin, s->glxmax - x, y - s->glymin, s->glymax - y};
+                    double dm = 0.0;
+                    for (int jd = 0; jd < 4; jd++) {
+                        if (!s->dm_on[jd]) continue;
+                        double d = dist[jd];
+                        if (d <= pw[jd]) {
+                            d = pp[jd] * ipow(1.0 - d / pw[jd], s->dm_deg[jd]);
+                            if (d > dm) dm = d;
+                        }
+                    }
+                    double g = MET(s, a, 6, ia) * (MET(s, a, 2, ia) * s->ggv[n] + MET(s, a, 2, ia - 1) * s->ggv[n - sa]);
+                    double d = gv0 / (gv0 + s->dtnow * g * dm);
+                    q[n] = q[n] * d;
+                }
             }
 }
 

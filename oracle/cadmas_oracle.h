@@ -78,6 +78,13 @@ typedef struct orc_state {
      * DRGDR and the coefficients DRGAP, DRGBT (MAXDRG = 10 rows). */
     int idrgn;
     double drgyu, drgdr[10], drgap[10], drgbt[10];
+    /* Damping zones, MODEL DAMP side (vf_iimdl.f:216-258): per side JD =
+     * 1..4 (x-, x+, y-, y+) whether it is on (IDAMP(JD) >= 0), its DEGREE
+     * IDAMP and DAMP(1..4,JD) = PARAM-XY, PARAM-Z, WIDTH, DEPTH; the grid's
+     * extent GLXMIN, GLXMAX, GLYMIN, GLYMAX (vf_iigrid.f:124-131). */
+    int dm_on[4], dm_deg[4];
+    double dm_par[4][4];
+    double glxmin, glxmax, glymin, glymax;
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */

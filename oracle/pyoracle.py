@@ -51,6 +51,9 @@ class OrcState(C.Structure):
         ("op_on", C.c_int * 4), ("op_d", C.c_double * 4), ("op_t", C.c_double * 4),
         ("idrgn", C.c_int), ("drgyu", C.c_double), ("drgdr", C.c_double * 10),
         ("drgap", C.c_double * 10), ("drgbt", C.c_double * 10),
+        ("dm_on", C.c_int * 4), ("dm_deg", C.c_int * 4), ("dm_par", (C.c_double * 4) * 4),
+        ("glxmin", C.c_double), ("glxmax", C.c_double), ("glymin", C.c_double),
+        ("glymax", C.c_double),
     ]
 
 
@@ -204,6 +207,20 @@ class Fields:
         st.idrgn, st.drgyu = len(rows), yu
         for l, (dr, ap, bt) in enumerate(rows):
             st.drgdr[l], st.drgap[l], st.drgbt[l] = dr, ap, bt
+
+    def set_damps(self, damps):
+        """Damping zones: damps = deck.damps (side -> degree, pxy, pz, width,
+        depth); the grid's extent from the metric tables (the first and last
+        coordinates, XX(1,2) and XX(1,NUMI))."""
+        st = self.st
+        xx, yy = self.a["XX"], self.a["YY"]
+        st.glxmin, st.glxmax = xx[0, 1], xx[0, -1]
+        st.glymin, st.glymax = yy[0, 1], yy[0, -1]
+        for jd, dm in damps.items():
+            q = jd - 1
+            st.dm_on[q], st.dm_deg[q] = 1, dm["degree"]
+            for m, key in enumerate(("pxy", "pz", "width", "depth")):
+                st.dm_par[q][m] = dm[key]
 
     def __getitem__(self, k):
         return self.a[k]

@@ -71,6 +71,9 @@ def _cases():
                       [1, 2, 250, 500, 750, 1000]),
         # Dupuit-Forchheimer drag (OPTION DRAG-DF, vf_vgene.f:111-146)
         "porous_df": (lambda: cfg.porous_df(name="porous_df"), [1, 2, 3, 30, 60]),
+        # implicit damping zones (MODEL DAMP, vf_veuler.f:131-166)
+        "wave_damp": (lambda: cfg.wave_damp(name="wave_damp"), [1, 2, 300, 450, 600]),
+        "damp3d": (lambda: cfg.damp3d(name="damp3d"), [1, 2, 10, 30, 60]),
     }
 
 
@@ -85,6 +88,8 @@ _KEEP = {
                        "f1_out": None} for s in (100, 200, 300, 400)},
     "wave_open": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                       "f1_out": None} for s in (250, 500, 750, 1000)},
+    "wave_damp": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                      "f1_out": None} for s in (300, 450, 600)},
 }
 
 

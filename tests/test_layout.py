@@ -10,7 +10,7 @@ import pytest
 import fixtures as fx
 
 CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume",
-         "porous_df"]
+         "porous_df", "wave_damp", "damp3d"]
 STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
           "INDX", "INDY", "INDZ", "INDB", "BCVI")
 
@@ -92,6 +92,27 @@ def test_deck_drag_df_words(pkg):
            "OPTION DRAG-DF 1.0E-6 1\n0.0 1000 1.1\n",    # DR <= 0
            "OPTION DRAG-DF 1.0E-6 1\n0.01 -1 1.1\n",     # AP < 0
            "OPTION DRAG-DF 1.0E-6 1\n0.01 1000 -0.1\n"]  # BT < 0
+    for t in bad:
+        with pytest.raises(pkg.deck.DeckError):
+            pkg.deck.parse(base + t)
+
+
+def test_deck_damp_words(pkg):
+    """MODEL DAMP as Source code/vf_iimdl.f:216-258 reads it, with the
+    defaults of vf_a2dflt.f:150,189-204 and the set-up checks of
+    vf_csetup.f:221-234 (a named side needs DEGREE, WIDTH and DEPTH)."""
+    base = "GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
+    assert pkg.deck.parse(base).damps == {}
+    d = pkg.deck.parse(base + "MODEL DAMP Y+ DEGREE 2\nMODEL DAMP Y+ WIDTH 0.5\n"
+                       "MODEL DAMP Y+ DEPTH 1.0\nMODEL DAMP Y+ PARAM-Z 0.3\n")
+    assert d.damps == {4: dict(degree=2, pxy=0.6, pz=pkg.deck.zstor("0.3"),
+                               width=pkg.deck.zstor("0.5"), depth=1.0)}
+    bad = ["MODEL DAMP\n", "MODEL DAMP Z- DEGREE 1\n", "MODEL DAMP X- DEGREE\n",
+           "MODEL DAMP X- DEGREE -1\n", "MODEL DAMP X- WIDTH 0.0\n", "MODEL DAMP X- COLOR 1\n",
+           # named without DEGREE, WIDTH or DEPTH
+           "MODEL DAMP X- WIDTH 0.5\nMODEL DAMP X- DEPTH 1.0\n",
+           "MODEL DAMP X- DEGREE 1\nMODEL DAMP X- DEPTH 1.0\n",
+           "MODEL DAMP X- DEGREE 1\nMODEL DAMP X- WIDTH 0.5\n"]
     for t in bad:
         with pytest.raises(pkg.deck.DeckError):
             pkg.deck.parse(base + t)

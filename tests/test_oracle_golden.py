@@ -295,13 +295,19 @@ def test_cdtcal_is_the_cell_minimum():
     assert worst <= 1e-12, worst
 
 
-def _wave_fields(pkg, case, rec, bcf):
+def _wave_fields(pkg, case, rec, bcf, damps=True):
+    """Oracle fields with the deck's models around the step: wave-maker and
+    radiation sides, the drag table and the damping zones."""
     st = fx.static(case)
     p = _params(st, rec)
     F = _fields(st, rec, p, bcf)
     d = pkg.deck.parse(str(st["DECK"]))
-    F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    if d.waves:
+        F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
     F.set_opens(d.opens)
+    F.set_drag_df(d.drag_df)
+    if damps:
+        F.set_damps(d.damps)
     return st, p, F
 
 
@@ -390,6 +396,50 @@ def test_drag_df_free_run_bitexact(pkg):
         assert rc == 0
         assert F.st.icgitr == tr["ITR"][s], s
         if s in fx.steps(case):
+            ref = fx.step(case, s)["f1_out"]
+            for k in ("UU", "VV", "WW", "PP", "FF", "NF"):
+                assert np.array_equal(F[k], ref[k]), (s, k)
+
+
+DAMP = ["wave_damp", "damp3d"]
+
+
+@pytest.mark.parametrize("case", DAMP)
+def test_damping_predictor_bitexact(pkg, case):
+    """VF_VEULER's implicit damping (MODEL DAMP, vf_veuler.f:131-166, 187-221,
+    243-277): the predictor's US/VS/WS equal the reference's at every recorded
+    step -- a degree-4 zone at the flume's end (the power evaluated by
+    squaring), and two overlapping zones of degrees 1 and 3 with their own
+    PARAM-Z in the 3-D tank -- while the same step without the zones
+    differs."""
+    for s in fx.steps(case):
+        r = fx.step(case, s)
+        for damps in (True, False):
+            st, p, F = _wave_fields(pkg, case, r["v1_in"], r["f1_in"]["BCF"], damps)
+            F.st.tnow = float(r["v1_in"]["TNOW"][0])
+            assert F.predictor() == 0
+            same = [np.array_equal(F[k], r["ps_in"][kk]) for k, kk in (("UU", "US"), ("VV", "VS"), ("WW", "WS"))]
+            if damps:
+                assert all(same), (s, same)
+            elif s > 1:
+                assert not all(same), s
+
+
+@pytest.mark.parametrize("case", DAMP)
+def test_damping_free_run_bitexact(pkg, case):
+    """Both damping decks from the reference's initial state to their last
+    step (600 and 60): ITR and ||div|| every step and the fields at the
+    recorded steps, bit for bit."""
+    r1 = fx.step(case, 1)
+    st, p, F = _wave_fields(pkg, case, r1["v1_in"], r1["f1_in"]["BCF"])
+    tr = fx.trajectory(case)
+    kept = set(fx.steps(case))
+    for s in range(1, int(tr["STEP"][-1]) + 1):
+        rc, _ = F.step()
+        assert rc == 0, s
+        assert F.st.icgitr == tr["ITR"][s], s
+        assert F.cdiv00() == tr["DV_FULL"][s], s
+        if s in kept:
             ref = fx.step(case, s)["f1_out"]
             for k in ("UU", "VV", "WW", "PP", "FF", "NF"):
                 assert np.array_equal(F[k], ref[k]), (s, k)
