@@ -134,6 +134,26 @@ int cad_set_wave_table(cad_ctx *ctx, int nz, int nt, const double *zz, const dou
  * sub-loop passes) sets their velocities at the context's TNOW. */
 int cad_set_wave(cad_ctx *ctx, int side, int l1, int l2, double depth, double period,
                  double ampl, double wvlvl);
+/* A theory wave-maker side, MODEL WAVE-BC <side> FUNC STOKES / CNOIDAL / STK-CND
+ * (vf_iimdl.f:51-70; VF_CWMAK0/1/2, vf_wstk0.f, vf_wcnd0.f): func -2 fifth-order
+ * Stokes, -1 third-order cnoidal, 0 one of the two by the Ursell number
+ * g H T^2 / d^2 (Stokes up to 25, vf_csetup.f:101-111).  Columns l1..l2 as in
+ * cad_set_wave; DEPTH, HEIGHT, PERIOD, AMPL and W-LEVEL.  The phase origin is
+ * where the wave's level crosses zero (VF_CWZERO).  Every VF_BWUWN from then on
+ * sets the side's face velocities from the wave, scaled by the columns'
+ * measured levels, as for the table. */
+int cad_set_wave_theory(cad_ctx *ctx, int side, int func, int l1, int l2, double depth,
+                        double height, double period, double ampl, double wvlvl);
+/* The theory wave itself, on the host and without a context: one VF_CWMAK0 of
+ * the wave (func -2 Stokes, -1 cnoidal; depth, height in m, period in s), then
+ * optionally its phase origin (VF_CWZERO, t0), its level at phase wvt
+ * (VF_CWMAK1, level) and the velocities at nz heights zc from the still water
+ * level (VF_CWMAK2, un, ut).  stk_state is VF_STK0's secant step carried from
+ * call to call (0 before the first); cad_set_wave_theory and every boundary
+ * pass advance the context's own copy the same way. */
+int cad_wave_theory_eval(int func, double depth, double height, double period, float *stk_state,
+                         double wvt, int nz, const double *zc, double *un, double *ut,
+                         double *level, double *t0);
 /* A radiation (open) side, MODEL OPEN-BC <side> FUNC TYPE1 (vf_iimdl.f:174-210,
  * applied in VF_BWUWN, vf_bwuwn.f:483-620): side 1..4 = X-, X+, Y-, Y+, the
  * DEPTH and PERIOD whose small-amplitude wave (VF_COMAK0) gives the

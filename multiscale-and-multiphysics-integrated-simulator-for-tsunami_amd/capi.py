@@ -83,6 +83,9 @@ def lib():
     L.cad_set_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+    L.cad_set_wave_theory.argtypes = [C.c_void_p] + [C.c_int] * 4 + [C.c_double] * 5
+    L.cad_wave_theory_eval.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.POINTER(C.c_float), C.c_double, C.c_int, PD, PD, PD, PD, PD]
     L.cad_set_drag_df.argtypes = [C.c_void_p, C.c_double, C.c_int, PD, PD, PD]
     L.cad_set_damp.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_poisson_residual.argtypes = [C.c_void_p, PD]
@@ -117,7 +120,31 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
            "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
-           "cad_set_drag_df", "cad_set_damp")
+           "cad_set_drag_df", "cad_set_damp", "cad_set_wave_theory", "cad_wave_theory_eval")
+
+class TheoryWave:
+    """A theory wave evaluated on the host (cad_wave_theory_eval), with
+    VF_STK0's carried state: each call() is one VF_CWMAK0 of the wave."""
+
+    def __init__(self, func, depth, height, period):
+        self.func = WAVE_FUNC[func] if isinstance(func, str) else int(func)
+        self.depth, self.height, self.period = float(depth), float(height), float(period)
+        self.state = C.c_float(0.0)
+
+    def call(self, wvt=0.0, zc=(), origin=False):
+        """(level at wvt, un, ut at heights zc, phase origin or None)."""
+        z = np.ascontiguousarray(np.asarray(zc, dtype=np.float64).reshape(-1))
+        un, ut = np.zeros_like(z), np.zeros_like(z)
+        lv, t0 = C.c_double(), C.c_double()
+        _check(lib().cad_wave_theory_eval(self.func, self.depth, self.height, self.period,
+                                          C.byref(self.state), float(wvt), len(z), _pd(z), _pd(un),
+                                          _pd(ut), C.byref(lv), C.byref(t0) if origin else None))
+        return lv.value, un, ut, (t0.value if origin else None)
+
+
+# MODEL WAVE-BC FUNC words of the theory waves -> cad_set_wave_theory's func
+# (IBCTYP(2,side), Source code/vf_iimdl.f:58-63)
+WAVE_FUNC = {"STK-CND": 0, "CNOIDAL": -1, "STOKES": -2}
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -221,19 +248,27 @@ class Context:
         return out
 
     def set_waves(self, waves, table, wvlvl):
-        """Matrix-table wave-makers: waves = deck.waves (side -> depth,
-        period, ampl, area), table = mtb.Table (LEVEL-ON)."""
-        zz, tt, hh = (_f64(getattr(table, k)) for k in ("zz", "tt", "hh"))
-        nz, nt = len(zz), len(tt)
-        un, ut = _f64(table.un, (nz, nt)), _f64(table.ut, (nz, nt))
-        _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), _pd(hh), _pd(un),
-                                        _pd(ut), float(table.t0)))
+        """Wave-makers: waves = deck.waves (side -> func, depth, height,
+        period, ampl, area); table = mtb.Table (LEVEL-ON) for the sides with
+        FUNC MATRIX, None when there are none.  Sides are set in side order,
+        as the reference's set-up visits them."""
+        if any(wv["func"] == "MATRIX" for wv in waves.values()):
+            zz, tt, hh = (_f64(getattr(table, k)) for k in ("zz", "tt", "hh"))
+            nz, nt = len(zz), len(tt)
+            un, ut = _f64(table.un, (nz, nt)), _f64(table.ut, (nz, nt))
+            _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), _pd(hh), _pd(un),
+                                            _pd(ut), float(table.t0)))
         ni, nj, _ = self.shape
         for side, wv in sorted(waves.items()):
             lim = nj if side <= 2 else ni
             l1, l2 = wv["area"] if wv["area"] else (2, lim - 1)
-            _check(lib().cad_set_wave(self._h, int(side), int(l1), int(l2), float(wv["depth"]),
-                                      float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
+            if wv["func"] == "MATRIX":
+                _check(lib().cad_set_wave(self._h, int(side), int(l1), int(l2), float(wv["depth"]),
+                                          float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
+            else:
+                _check(lib().cad_set_wave_theory(
+                    self._h, int(side), WAVE_FUNC[wv["func"]], int(l1), int(l2), float(wv["depth"]),
+                    float(wv["height"]), float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
 
     def set_opens(self, opens):
         """Radiation sides: opens = deck.opens (side -> depth, period)."""

@@ -383,6 +383,23 @@ def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400,
     return case
 
 
+def theory_flume(func="STOKES", ni=40, nj=4, nk=20, d=0.02, h=0.2, H=0.02, period=1.0, ampl=1.0,
+                 steps=600, dt=1.0e-3, eps_r=1.0e-10, name=None):
+    """A flume driven by a theory wave on its x- side (MODEL WAVE-BC X- FUNC
+    STOKES, CNOIDAL or STK-CND; Source code/vf_cwmak0.f, vf_wstk0.f,
+    vf_wcnd0.f), ramped in over AMPL periods, running into the far wall."""
+    lines = [f"# {func.lower()} flume {ni}x{nj}x{nk}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE",
+              f"MODEL WAVE-BC X- FUNC {func}", f"MODEL WAVE-BC X- DEPTH {h}",
+              f"MODEL WAVE-BC X- HEIGHT {H}", f"MODEL WAVE-BC X- PERIOD {period}",
+              f"MODEL WAVE-BC X- AMPL {ampl}"]
+    _common(lines, dt, steps, eps_r, wlevel=h)
+    return Case(name or f"{func.lower()}_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
 def wave_damp(ni=40, steps=600, name=None):
     """The 40-cell wave flume with a damping zone (MODEL DAMP X+, the
     implicit sponge of VF_VEULER, Source code/vf_veuler.f:131-166) over its
@@ -425,6 +442,8 @@ def named(name):
         "wave_open": lambda: wave_flume(ni=40, steps=1000, open_end=True),
         "porous_df": lambda: porous_df(),
         "wave_damp": lambda: wave_damp(),
+        "stokes_flume": lambda: theory_flume("STOKES"),
+        "cnoidal_flume": lambda: theory_flume("STK-CND", H=0.03, period=2.0),
         "damp3d": lambda: damp3d(),
     }
     return table[name]()

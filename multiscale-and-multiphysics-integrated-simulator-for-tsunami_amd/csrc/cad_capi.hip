@@ -223,6 +223,7 @@ void cad_destroy(cad_ctx *c)
     cad_sweep_free(c);
     cad_tdoor_free(c);
     cad_wave_free(c);
+    cad_wave_theory_free(c);
     cad_mg_free(c);
     delete[] c->zz_host;
     delete[] c->xx_host;
@@ -671,6 +672,13 @@ int cad_set_wave(cad_ctx *c, int side, int l1, int l2, double depth, double peri
 {
     if (!c) return CAD_ERR_ARG;
     return cad_wave_side(c, side, l1, l2, depth, period, ampl, wvlvl);
+}
+
+int cad_set_wave_theory(cad_ctx *c, int side, int func, int l1, int l2, double depth,
+                        double height, double period, double ampl, double wvlvl)
+{
+    if (!c) return CAD_ERR_ARG;
+    return cad_wave_theory_side(c, side, func, l1, l2, depth, height, period, ampl, wvlvl);
 }
 
 int cad_set_open(cad_ctx *c, int side, double depth, double period)

@@ -193,6 +193,15 @@ struct cad_ctx {
     double wv_d[4], wv_t[4], wv_a[4], wv_level[4];
     int *wv_err, *wv_err_host;
     int n_wave_faces;   // boundary faces of type 5 in INDB
+    // the theory sides (cad_wave_theory.hip): per side the function
+    // (-3 the table, -2 Stokes, -1 cnoidal), height, phase origin and the
+    // persistent secant step of VF_STK0; per side the columns' level scales
+    // and face velocities (device and pinned host)
+    int wv_kind[4];
+    double wv_h[4], wv_t0s[4];
+    float wv_stk_dddk;
+    double *wv_vws, *wv_vws_host, *wv_uv, *wv_uv_host;
+    size_t wv_cols;     // columns per side in the buffers (max(NUMI, NUMJ))
     // radiation sides (cad_wave.hip): on, depth, period; the x and y metric
     // tables on the host for their cell widths
     int op_on[4];
@@ -253,4 +262,8 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
 int cad_wave_apply(cad_ctx *c, bool initial);
 int cad_open_side(cad_ctx *c, int side, double depth, double period);
 int cad_wave_check(cad_ctx *c);
+int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double depth, double height,
+                         double period, double ampl, double wvlvl);
+int cad_wave_theory_apply(cad_ctx *c, int jd);
+void cad_wave_theory_free(cad_ctx *c);
 void cad_wave_free(cad_ctx *c);

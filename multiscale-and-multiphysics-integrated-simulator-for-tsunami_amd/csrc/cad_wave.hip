@@ -1,5 +1,6 @@
 // cad_wave.hip -- the special sides of VF_BWUWN: the matrix-table
-// wave-maker and the radiation (open) boundary.
+// wave-maker and the radiation (open) boundary (the theory wave-makers are
+// in cad_wave_theory.hip).
 //
 // MODEL WAVE-BC <side> FUNC MATRIX with a LEVEL-ON table (<name>.mtb, read by
 // VF_IM1INP, Source code/vf_im1inp.f) sets, at every call of VF_BWUWN
@@ -272,8 +273,9 @@ void cad_wave_free(cad_ctx *c)
     c->wv_err_host = nullptr;
     c->wv_tt = c->wv_hh = nullptr;
     c->wv_nz = c->wv_nt = 0;
-    // the sides refer to the table: a new table needs them set again
-    for (int q = 0; q < 4; q++) c->wv_on[q] = 0;
+    // the table's sides refer to it: a new table needs them set again
+    for (int q = 0; q < 4; q++)
+        if (c->wv_kind[q] == -3) c->wv_on[q] = 0;
 }
 
 int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
@@ -366,6 +368,7 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
     }
     const int q = side - 1;
     c->wv_on[q] = 1;
+    c->wv_kind[q] = -3;
     c->wv_l1[q] = l1;
     c->wv_l2[q] = l2;
     c->wv_d[q] = depth;
@@ -384,6 +387,12 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         const int q = jd - 1;
         if (!c->wv_on[q]) continue;
         any = true;
+        if (c->wv_kind[q] != -3) {
+            // a theory side: STOKES or CNOIDAL (cad_wave_theory.hip)
+            const int rc = cad_wave_theory_apply(c, jd);
+            if (rc) return rc;
+            continue;
+        }
         const double dep = c->wv_d[q], t = c->wv_t[q], t0 = c->wv_t0;
         double a = c->wv_a[q];
         // the table's phase in single precision (vf_bwuwn.f:270-276)
@@ -423,7 +432,7 @@ int cad_wave_apply(cad_ctx *c, bool initial)
                            w.xside ? c->f.uu : c->f.vv, c->f.bcu, c->f.bcv, c->f.bcw, w, c->wv_err);
         CAD_CHECK(hipGetLastError());
     }
-    if (any)
+    if (any && c->wv_err)
         CAD_CHECK(hipMemcpyAsync(c->wv_err_host, c->wv_err, sizeof(int), hipMemcpyDeviceToHost,
                                  c->stream));
     // radiation sides (vf_bwuwn.f:483-620); the inside velocities of the

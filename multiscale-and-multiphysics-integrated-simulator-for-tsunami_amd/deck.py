@@ -398,10 +398,12 @@ def _parse_model(d, w):
     if key == "FUNC":
         if len(w) < 5:
             raise DeckError("MODEL WAVE-BC FUNC: SYNTAX ERROR.")
-        if w[4] != "MATRIX":
-            raise UnsupportedDeck(f"MODEL WAVE-BC FUNC {w[4]}: only the matrix table (MATRIX) "
-                                  "is implemented")
-        wv["func"] = "MATRIX"
+        if w[4] not in ("MATRIX", "STOKES", "CNOIDAL", "STK-CND"):
+            if w[4] in ("STREAM", "MATRIX2"):
+                raise UnsupportedDeck(f"MODEL WAVE-BC FUNC {w[4]}: the stream-function wave and the "
+                                      "second table are not implemented")
+            raise DeckError(f"MODEL WAVE-BC FUNC {w[4]}: UNKNOWN WORD.")
+        wv["func"] = w[4]
     elif key in ("DEPTH", "HEIGHT", "PERIOD", "AMPL"):
         if len(w) < 5:
             raise DeckError(f"MODEL WAVE-BC {key}: SYNTAX ERROR.")
@@ -613,15 +615,21 @@ def _validate(d):
     if d.loops < 1:
         raise DeckError("OPTION SUB-LOOP: INVALID VALUE.")
     # vf_csetup.f:37-95: a table wave-maker needs DEPTH and PERIOD, no ANGLE
+    # vf_csetup.f:60-100: a theory wave also needs HEIGHT; ANGLE is refused
+    # for the table and not produced here for a theory wave
     for jd, wv in d.waves.items():
-        if wv["func"] != "MATRIX":
-            raise DeckError("MODEL WAVE-BC: FUNC MATRIX missing")
+        if wv["func"] is None:
+            raise DeckError("MODEL WAVE-BC: FUNC missing")
         if wv["depth"] < 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC DEPTH) NOT FOUND.")
+        if wv["func"] != "MATRIX" and wv["height"] < 1.0e-20:
+            raise DeckError("MODEL WAVE-BC: (WAVE-BC HEIGHT) NOT FOUND.")
         if wv["period"] < 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC PERIOD) NOT FOUND.")
         if abs(wv["angle"]) > 1.0e-20:
-            raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
+            if wv["func"] == "MATRIX":
+                raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
+            raise UnsupportedDeck("MODEL WAVE-BC ANGLE: oblique theory waves are not implemented")
     # vf_csetup.f:221-234: a damping side needs DEGREE, WIDTH and DEPTH
     for jd, dm in d.damps.items():
         if dm["degree"] < 0:

@@ -69,7 +69,9 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None, precision=N
         static = lay.static_arrays()
         ctx.upload_static(static)
         if d.waves:
-            table = mtbmod.parse((workdir / f"{name}.mtb").read_text())
+            table = None
+            if any(wv["func"] == "MATRIX" for wv in d.waves.values()):
+                table = mtbmod.parse((workdir / f"{name}.mtb").read_text())
             ctx.set_waves(d.waves, table, d.wvlvl)
         ctx.set_opens(d.opens)
         ctx.set_drag_df(d.drag_df)

@@ -74,6 +74,12 @@ def _cases():
         # implicit damping zones (MODEL DAMP, vf_veuler.f:131-166)
         "wave_damp": (lambda: cfg.wave_damp(name="wave_damp"), [1, 2, 300, 450, 600]),
         "damp3d": (lambda: cfg.damp3d(name="damp3d"), [1, 2, 10, 30, 60]),
+        # theory wave-makers (VF_CWMAK0/1/2): fifth-order Stokes, and STK-CND
+        # whose Ursell number (29) picks the third-order cnoidal wave
+        "stokes_flume": (lambda: cfg.theory_flume("STOKES", name="stokes_flume"),
+                         [1, 2, 100, 300, 600]),
+        "cnoidal_flume": (lambda: cfg.theory_flume("STK-CND", H=0.03, period=2.0,
+                                                   name="cnoidal_flume"), [1, 2, 100, 300, 600]),
     }
 
 
@@ -90,6 +96,10 @@ _KEEP = {
                       "f1_out": None} for s in (250, 500, 750, 1000)},
     "wave_damp": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                       "f1_out": None} for s in (300, 450, 600)},
+    "stokes_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                         "f1_out": None} for s in (100, 300, 600)},
+    "cnoidal_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                          "f1_out": None} for s in (100, 300, 600)},
 }
 
 

@@ -1,12 +1,15 @@
-"""The matrix-table wave-maker (MODEL WAVE-BC X- FUNC MATRIX with a LEVEL-ON
-<name>.mtb table) and the radiation boundary (MODEL OPEN-BC X+ FUNC TYPE1) on
-the GPU.
+"""The wave-makers (MODEL WAVE-BC X- FUNC MATRIX with a LEVEL-ON <name>.mtb
+table; FUNC STOKES and STK-CND, the theory waves) and the radiation boundary
+(MODEL OPEN-BC X+ FUNC TYPE1) on the GPU.
 
 Fixtures (tests/golden/make_golden.py): "wave_flume", a 60x4x20 flume at rest
 at depth 0.2 whose X- side follows a solitary-wave table, 400 steps recorded
 at 1, 2, 100, 200, 300, 400; "wave_open", the same on 40x4x20 with a
 radiation boundary at the far end that the wave leaves through, 1000 steps
-recorded at 1, 2, 250, 500, 750, 1000.  The predictor and the boundary pass
+recorded at 1, 2, 250, 500, 750, 1000; "stokes_flume" and "cnoidal_flume",
+40x4x20 flumes driven by a fifth-order Stokes and (by the Ursell number) a
+third-order cnoidal wave, 600 steps recorded at 1, 2, 100, 300, 600.  The
+predictor and the boundary pass
 that sets the special faces have no reduction, so they are compared bit for
 bit, as is F1CAL (those faces take F like FREE); whole V1CAL and free-run
 steps go through the Poisson solve and are compared to the plain-form bars.
@@ -19,8 +22,9 @@ from helpers import assert_close, assert_same_int, assert_vel_close, rel_err
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["wave_flume", "wave_open"]
-RSL = {"wave_flume": (100, 400, 100), "wave_open": (250, 1000, 250)}
+CASES = ["wave_flume", "wave_open", "stokes_flume", "cnoidal_flume"]
+RSL = {"wave_flume": (100, 400, 100), "wave_open": (250, 1000, 250),
+       "stokes_flume": (300, 600, 300), "cnoidal_flume": (300, 600, 300)}
 STATE = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")
 KEYS = ("rho0", "grz0", "anu0", "scmvp", "flower", "fupper", "plower", "cgpara", "cgepsa",
         "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
@@ -34,7 +38,8 @@ def _ctx(pkg, st, dtnow, tnow):
     c.set_params(**{k: p[k] for k in KEYS}, dtnow=dtnow, tnow=tnow)
     c.upload_static(st)
     d = pkg.deck.parse(str(st["DECK"]))
-    c.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    table = pkg.mtb.parse(str(st["FILE_MTB"])) if "FILE_MTB" in st else None
+    c.set_waves(d.waves, table, d.wvlvl)
     c.set_opens(d.opens)
     return c, d
 
@@ -140,7 +145,8 @@ def test_wave_flume_free_run(pkg, tmp_path, case):
     tr = fx.trajectory(case)
     a, b, n = RSL[case]
     deck = str(st["DECK"]) + f"\nFILE RSL STEP {a} {b} {n}\n"
-    (tmp_path / "flume.mtb").write_text(str(st["FILE_MTB"]))
+    if "FILE_MTB" in st:
+        (tmp_path / "flume.mtb").write_text(str(st["FILE_MTB"]))
     res = pkg.driver.run(deck, "flume", tmp_path)
     last = int(tr["STEP"][-1])
     assert res["nnow"] == last

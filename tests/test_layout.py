@@ -10,7 +10,7 @@ import pytest
 import fixtures as fx
 
 CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume",
-         "porous_df", "wave_damp", "damp3d"]
+         "porous_df", "wave_damp", "damp3d", "stokes_flume", "cnoidal_flume"]
 STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
           "INDX", "INDY", "INDZ", "INDB", "BCVI")
 
@@ -95,6 +95,30 @@ def test_deck_drag_df_words(pkg):
     for t in bad:
         with pytest.raises(pkg.deck.DeckError):
             pkg.deck.parse(base + t)
+
+
+def test_deck_theory_wave_words(pkg):
+    """MODEL WAVE-BC FUNC STOKES / CNOIDAL / STK-CND (Source code/vf_iimdl.f:51-70)
+    with the set-up checks of vf_csetup.f:86-94: a theory wave needs DEPTH,
+    HEIGHT and PERIOD; STREAM and MATRIX2 are outside what is produced here."""
+    base = ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
+            "B.C. D VP SLIP\nB.C. D F FREE\n")
+    full = ("MODEL WAVE-BC X- DEPTH 0.2\nMODEL WAVE-BC X- HEIGHT 0.02\n"
+            "MODEL WAVE-BC X- PERIOD 1.0\n")
+    for func in ("STOKES", "CNOIDAL", "STK-CND"):
+        d = pkg.deck.parse(base + f"MODEL WAVE-BC X- FUNC {func}\n" + full)
+        assert d.waves[1]["func"] == func and d.waves[1]["height"] == pkg.deck.zstor("0.02")
+    with pytest.raises(pkg.deck.DeckError):      # no HEIGHT
+        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STOKES\nMODEL WAVE-BC X- DEPTH 0.2\n"
+                       "MODEL WAVE-BC X- PERIOD 1.0\n")
+    with pytest.raises(pkg.deck.DeckError):
+        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC AIRY\n" + full)
+    for func in ("STREAM 5", "MATRIX2"):
+        with pytest.raises(pkg.deck.UnsupportedDeck):
+            pkg.deck.parse(base + f"MODEL WAVE-BC X- FUNC {func}\n" + full)
+    with pytest.raises(pkg.deck.UnsupportedDeck):  # oblique theory wave
+        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STOKES\n" + full +
+                       "MODEL WAVE-BC X- ANGLE 10.0 0.0 0.0\n")
 
 
 def test_deck_damp_words(pkg):
