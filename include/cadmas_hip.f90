@@ -188,24 +188,24 @@ module cadmas_hip
     end function
 
     ! MODEL WAVE-BC side (1..4) FUNC STOKES (-2), CNOIDAL (-1) or STK-CND (0)
-    function cad_set_wave_theory(ctx, side, func, l1, l2, depth, height, period, ampl, wvlvl) &
-        bind(c, name='cad_set_wave_theory')
+    function cad_set_wave_theory(ctx, side, func, l1, l2, depth, height, period, ampl, wvlvl, &
+        angle, ax, ay) bind(c, name='cad_set_wave_theory')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: side, func, l1, l2
-      real(c_double), value :: depth, height, period, ampl, wvlvl
+      real(c_double), value :: depth, height, period, ampl, wvlvl, angle, ax, ay
       integer(c_int) :: cad_set_wave_theory
     end function
 
     ! the theory wave on the host: level, velocities at nz heights, phase origin
     function cad_wave_theory_eval(func, depth, height, period, stk_state, wvt, nz, zc, un, ut, &
-        level, t0) bind(c, name='cad_wave_theory_eval')
+        level, t0, length) bind(c, name='cad_wave_theory_eval')
       import :: c_int, c_double, c_float
       integer(c_int), value :: func, nz
       real(c_double), value :: depth, height, period, wvt
       real(c_float), intent(inout) :: stk_state
       real(c_double), intent(in) :: zc(*)
-      real(c_double), intent(out) :: un(*), ut(*), level, t0
+      real(c_double), intent(out) :: un(*), ut(*), level, t0, length
       integer(c_int) :: cad_wave_theory_eval
     end function
 

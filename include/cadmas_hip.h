@@ -141,19 +141,23 @@ int cad_set_wave(cad_ctx *ctx, int side, int l1, int l2, double depth, double pe
  * cad_set_wave; DEPTH, HEIGHT, PERIOD, AMPL and W-LEVEL.  The phase origin is
  * where the wave's level crosses zero (VF_CWZERO).  Every VF_BWUWN from then on
  * sets the side's face velocities from the wave, scaled by the columns'
- * measured levels, as for the table. */
+ * measured levels, as for the table.  angle (degrees, ANGLE; 0 = normal
+ * incidence) turns the wave about the side's normal with the phase shifted
+ * along the ray through (ax, ay) by the set-up wave length (vf_bwuwn.f:297-325,
+ * BCTYP(9..11)). */
 int cad_set_wave_theory(cad_ctx *ctx, int side, int func, int l1, int l2, double depth,
-                        double height, double period, double ampl, double wvlvl);
+                        double height, double period, double ampl, double wvlvl, double angle,
+                        double ax, double ay);
 /* The theory wave itself, on the host and without a context: one VF_CWMAK0 of
  * the wave (func -2 Stokes, -1 cnoidal; depth, height in m, period in s), then
- * optionally its phase origin (VF_CWZERO, t0), its level at phase wvt
- * (VF_CWMAK1, level) and the velocities at nz heights zc from the still water
- * level (VF_CWMAK2, un, ut).  stk_state is VF_STK0's secant step carried from
+ * optionally its wave length (length), its phase origin (VF_CWZERO, t0), its
+ * level at phase wvt (VF_CWMAK1, level) and the velocities at nz heights zc
+ * from the still water level (VF_CWMAK2, un, ut).  stk_state is VF_STK0's secant step carried from
  * call to call (0 before the first); cad_set_wave_theory and every boundary
  * pass advance the context's own copy the same way. */
 int cad_wave_theory_eval(int func, double depth, double height, double period, float *stk_state,
                          double wvt, int nz, const double *zc, double *un, double *ut,
-                         double *level, double *t0);
+                         double *level, double *t0, double *length);
 /* A radiation (open) side, MODEL OPEN-BC <side> FUNC TYPE1 (vf_iimdl.f:174-210,
  * applied in VF_BWUWN, vf_bwuwn.f:483-620): side 1..4 = X-, X+, Y-, Y+, the
  * DEPTH and PERIOD whose small-amplitude wave (VF_COMAK0) gives the

@@ -83,9 +83,10 @@ def lib():
     L.cad_set_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
-    L.cad_set_wave_theory.argtypes = [C.c_void_p] + [C.c_int] * 4 + [C.c_double] * 5
+    L.cad_set_wave_theory.argtypes = [C.c_void_p] + [C.c_int] * 4 + [C.c_double] * 8
     L.cad_wave_theory_eval.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
-                                       C.POINTER(C.c_float), C.c_double, C.c_int, PD, PD, PD, PD, PD]
+                                       C.POINTER(C.c_float), C.c_double, C.c_int, PD, PD, PD, PD, PD,
+                                       PD]
     L.cad_set_drag_df.argtypes = [C.c_void_p, C.c_double, C.c_int, PD, PD, PD]
     L.cad_set_damp.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_poisson_residual.argtypes = [C.c_void_p, PD]
@@ -132,13 +133,16 @@ class TheoryWave:
         self.state = C.c_float(0.0)
 
     def call(self, wvt=0.0, zc=(), origin=False):
-        """(level at wvt, un, ut at heights zc, phase origin or None)."""
+        """(level at wvt, un, ut at heights zc, phase origin or None); the
+        wave length of the call is kept as .length."""
         z = np.ascontiguousarray(np.asarray(zc, dtype=np.float64).reshape(-1))
         un, ut = np.zeros_like(z), np.zeros_like(z)
-        lv, t0 = C.c_double(), C.c_double()
+        lv, t0, wl = C.c_double(), C.c_double(), C.c_double()
         _check(lib().cad_wave_theory_eval(self.func, self.depth, self.height, self.period,
                                           C.byref(self.state), float(wvt), len(z), _pd(z), _pd(un),
-                                          _pd(ut), C.byref(lv), C.byref(t0) if origin else None))
+                                          _pd(ut), C.byref(lv), C.byref(t0) if origin else None,
+                                          C.byref(wl)))
+        self.length = wl.value
         return lv.value, un, ut, (t0.value if origin else None)
 
 
@@ -268,7 +272,8 @@ class Context:
             else:
                 _check(lib().cad_set_wave_theory(
                     self._h, int(side), WAVE_FUNC[wv["func"]], int(l1), int(l2), float(wv["depth"]),
-                    float(wv["height"]), float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
+                    float(wv["height"]), float(wv["period"]), float(wv["ampl"]), float(wvlvl),
+                    float(wv["angle"]), float(wv.get("ax", 0.0)), float(wv.get("ay", 0.0))))
 
     def set_opens(self, opens):
         """Radiation sides: opens = deck.opens (side -> depth, period)."""

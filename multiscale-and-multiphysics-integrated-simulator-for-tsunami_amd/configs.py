@@ -384,10 +384,11 @@ def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400,
 
 
 def theory_flume(func="STOKES", ni=40, nj=4, nk=20, d=0.02, h=0.2, H=0.02, period=1.0, ampl=1.0,
-                 steps=600, dt=1.0e-3, eps_r=1.0e-10, name=None):
+                 steps=600, dt=1.0e-3, eps_r=1.0e-10, angle=None, name=None):
     """A flume driven by a theory wave on its x- side (MODEL WAVE-BC X- FUNC
     STOKES, CNOIDAL or STK-CND; Source code/vf_cwmak0.f, vf_wstk0.f,
-    vf_wcnd0.f), ramped in over AMPL periods, running into the far wall."""
+    vf_wcnd0.f), ramped in over AMPL periods, running into the far wall;
+    angle = (degrees, SX, SY) turns it (MODEL WAVE-BC X- ANGLE)."""
     lines = [f"# {func.lower()} flume {ni}x{nj}x{nk}"]
     lines += _grid_lines("X", _coords(ni, d))
     lines += _grid_lines("Y", _coords(nj, d))
@@ -396,6 +397,8 @@ def theory_flume(func="STOKES", ni=40, nj=4, nk=20, d=0.02, h=0.2, H=0.02, perio
               f"MODEL WAVE-BC X- FUNC {func}", f"MODEL WAVE-BC X- DEPTH {h}",
               f"MODEL WAVE-BC X- HEIGHT {H}", f"MODEL WAVE-BC X- PERIOD {period}",
               f"MODEL WAVE-BC X- AMPL {ampl}"]
+    if angle is not None:
+        lines.append("MODEL WAVE-BC X- ANGLE {} {} {}".format(*angle))
     _common(lines, dt, steps, eps_r, wlevel=h)
     return Case(name or f"{func.lower()}_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
 
@@ -444,6 +447,8 @@ def named(name):
         "wave_damp": lambda: wave_damp(),
         "stokes_flume": lambda: theory_flume("STOKES"),
         "cnoidal_flume": lambda: theory_flume("STK-CND", H=0.03, period=2.0),
+        "oblique_basin": lambda: theory_flume("STOKES", ni=30, nj=16, nk=16, h=0.16, period=0.8,
+                                              steps=300, angle=(25.0, 0.0, 0.1)),
         "damp3d": lambda: damp3d(),
     }
     return table[name]()

@@ -675,10 +675,12 @@ int cad_set_wave(cad_ctx *c, int side, int l1, int l2, double depth, double peri
 }
 
 int cad_set_wave_theory(cad_ctx *c, int side, int func, int l1, int l2, double depth,
-                        double height, double period, double ampl, double wvlvl)
+                        double height, double period, double ampl, double wvlvl, double angle,
+                        double ax, double ay)
 {
     if (!c) return CAD_ERR_ARG;
-    return cad_wave_theory_side(c, side, func, l1, l2, depth, height, period, ampl, wvlvl);
+    return cad_wave_theory_side(c, side, func, l1, l2, depth, height, period, ampl, wvlvl, angle,
+                                ax, ay);
 }
 
 int cad_set_open(cad_ctx *c, int side, double depth, double period)

@@ -199,8 +199,10 @@ struct cad_ctx {
     // and face velocities (device and pinned host)
     int wv_kind[4];
     double wv_h[4], wv_t0s[4];
+    double wv_wl[4];        // wave length of VF_CWMAK0 at set-up (BCTYP(4,side))
+    double wv_ang[4][3];    // ANGLE: degrees and the ray's origin SX, SY (BCTYP(9..11,side))
     float wv_stk_dddk;
-    double *wv_vws, *wv_vws_host, *wv_uv, *wv_uv_host;
+    double *wv_vws, *wv_vws_host, *wv_uv, *wv_uv_host, *wv_ph_host;
     size_t wv_cols;     // columns per side in the buffers (max(NUMI, NUMJ))
     // radiation sides (cad_wave.hip): on, depth, period; the x and y metric
     // tables on the host for their cell widths
@@ -263,7 +265,7 @@ int cad_wave_apply(cad_ctx *c, bool initial);
 int cad_open_side(cad_ctx *c, int side, double depth, double period);
 int cad_wave_check(cad_ctx *c);
 int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double depth, double height,
-                         double period, double ampl, double wvlvl);
+                         double period, double ampl, double wvlvl, double angle, double ax, double ay);
 int cad_wave_theory_apply(cad_ctx *c, int jd);
 void cad_wave_theory_free(cad_ctx *c);
 void cad_wave_free(cad_ctx *c);

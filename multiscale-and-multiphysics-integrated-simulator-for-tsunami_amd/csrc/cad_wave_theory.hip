@@ -58,7 +58,7 @@ inline float p4(float x) { return x * x * x * x; }
 
 struct Stokes {
     int n;
-    float rk, dk, cc, c0, b0;
+    float rk, dk, cc, c0, b0, rl;
     float a[5], b[5];
 };
 
@@ -128,6 +128,7 @@ void stk0(Stokes &s, float &dddk, int n, float d, float t, float h)
     }
     if (!done) dk = dk - dddk;
     const float rk = dk / d;
+    s.rl = 2.0f * PI_S / rk;
     const float e = dk * hd;
     const float c0 = std::sqrt(G_CGS * th / rk);
     s.rk = rk;
@@ -222,7 +223,7 @@ void stk2(const Stokes &s, float x, float z, float *u, float *w)
 
 struct Cnoidal {
     int n;
-    float d, ql;
+    float d, ql, rl;
     float a[4], b[3][4], bb[3][4];   // b[j][i]: B(J,I)
 };
 
@@ -412,6 +413,7 @@ void cnd0(Cnoidal &s, int n, float d, float t, float h)
     c = c + e * (1.0f + 2.0f * r - 3.0f * o) / 2.0f + 1.0f;
     const float cc = c * c0;
     const float rl = t * cc;
+    s.rl = rl;
     const float dl = d / rl;
     const float sk = 2.0f * std::sqrt(t3) * qql * dl;
     for (int i = 1; i <= nn; i++) {
@@ -481,6 +483,12 @@ void cwmak0(cad_ctx *c, Theory &w, int kind, double d, double t, double h)
         cnd0(w.cnd, 3, r4d, r4t, r4h);
 }
 
+// the wave length (m) VF_CWMAK0 returns (WVLENG)
+double cwlength(const Theory &w)
+{
+    return (double)(w.kind == -2 ? w.stk.rl : w.cnd.rl) * 0.01;
+}
+
 // VF_CWMAK1: the level (m) at phase wvt
 double cwmak1(const Theory &w, double wvt)
 {
@@ -543,8 +551,7 @@ double cwzero(const Theory &w)
 __global__ void __launch_bounds__(64) k_wave_scale(Dims d, const double *__restrict__ zz,
                                                    const int *__restrict__ nf,
                                                    const double *__restrict__ ff, int xside, int cpos,
-                                                   int q0, int nq, double dep, double wvz, double wvlvl,
-                                                   double *vws)
+                                                   int q0, int nq, double dep, double wvlvl, double *vws)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nq) return;
@@ -564,7 +571,8 @@ __global__ void __launch_bounds__(64) k_wave_scale(Dims d, const double *__restr
         }
     }
     val = val - (wvlvl - zz[6]);
-    vws[t] = (wvz + dep) / (val + dep);
+    // vws holds the column's expected level on entry
+    vws[t] = (vws[t] + dep) / (val + dep);
 }
 
 // the side's boundary faces from the host's velocities (vf_bwuwn.f:331-380):
@@ -599,7 +607,7 @@ __global__ void __launch_bounds__(64) k_wave_theory_faces(Dims d, const int *__r
 
 int cad_wave_theory_eval(int func, double depth, double height, double period, float *stk_state,
                          double wvt, int nz, const double *zc, double *un, double *ut,
-                         double *level, double *t0)
+                         double *level, double *t0, double *length)
 {
     if ((func != -2 && func != -1) || !stk_state || (nz > 0 && (!zc || !un || !ut)) || nz < 0) {
         cad_set_error("cad_wave_theory_eval: func -2 (Stokes) or -1 (cnoidal), the state, and "
@@ -617,6 +625,7 @@ int cad_wave_theory_eval(int func, double depth, double height, double period, f
         stk0(w.stk, *stk_state, 5, r4d, r4t, r4h);
     else
         cnd0(w.cnd, 3, r4d, r4t, r4h);
+    if (length) *length = cwlength(w);
     if (t0) *t0 = cwzero(w);
     if (level) *level = cwmak1(w, wvt);
     for (int k = 0; k < nz; k++) cwmak2(w, wvt, zc[k], &un[k], &ut[k]);
@@ -629,12 +638,13 @@ void cad_wave_theory_free(cad_ctx *c)
     if (c->wv_uv) hipFree(c->wv_uv);
     if (c->wv_vws_host) hipHostFree(c->wv_vws_host);
     if (c->wv_uv_host) hipHostFree(c->wv_uv_host);
-    c->wv_vws = c->wv_uv = c->wv_vws_host = c->wv_uv_host = nullptr;
+    std::free(c->wv_ph_host);
+    c->wv_vws = c->wv_uv = c->wv_vws_host = c->wv_uv_host = c->wv_ph_host = nullptr;
     c->wv_cols = 0;
 }
 
 int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double depth, double height,
-                         double period, double ampl, double wvlvl)
+                         double period, double ampl, double wvlvl, double angle, double ax, double ay)
 {
     if (side < 1 || side > 4) {
         cad_set_error("cad_set_wave_theory: side %d (1..4 = X-, X+, Y-, Y+)", side);
@@ -658,6 +668,10 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
         cad_set_error("cad_set_wave_theory: columns %d..%d outside 2..%d", l1, l2, lim - 1);
         return CAD_ERR_ARG;
     }
+    if (!std::isfinite(angle) || !std::isfinite(ax) || !std::isfinite(ay)) {
+        cad_set_error("cad_set_wave_theory: the angle and its origin must be finite");
+        return CAD_ERR_ARG;
+    }
     // STK-CND: Stokes up to an Ursell number of 25 (vf_csetup.f:101-111)
     int kind = func;
     if (kind == 0) {
@@ -672,6 +686,11 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
         CAD_CHECK(hipMalloc((void **)&c->wv_uv, nuv * sizeof(double)));
         CAD_CHECK(hipHostMalloc((void **)&c->wv_vws_host, 4 * cols * sizeof(double), hipHostMallocDefault));
         CAD_CHECK(hipHostMalloc((void **)&c->wv_uv_host, nuv * sizeof(double), hipHostMallocDefault));
+        c->wv_ph_host = (double *)std::malloc(4 * cols * sizeof(double));
+        if (!c->wv_ph_host) {
+            cad_set_error("cad_set_wave_theory: out of host memory");
+            return CAD_ERR_HIP;
+        }
         c->wv_cols = cols;
     }
     if (!c->wv_err) {
@@ -680,8 +699,9 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
         CAD_CHECK(hipMemset(c->wv_err, 0, 64));
         c->wv_err_host[0] = 0;
     }
-    // VF_CSETUP: the wave once (VF_STK0 carries its secant step on), and the
-    // phase origin where the level crosses zero (vf_csetup.f:113-121)
+    // VF_CSETUP: the wave once (VF_STK0 carries its secant step on), its
+    // length, and the phase origin where the level crosses zero
+    // (vf_csetup.f:113-125)
     Theory w;
     cwmak0(c, w, kind, depth, period, height);
     const int q = side - 1;
@@ -693,6 +713,10 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
     c->wv_h[q] = height;
     c->wv_t[q] = period;
     c->wv_a[q] = ampl;
+    c->wv_wl[q] = cwlength(w);
+    c->wv_ang[q][0] = angle;
+    c->wv_ang[q][1] = ax;
+    c->wv_ang[q][2] = ay;
     c->wv_t0s[q] = cwzero(w);
     c->wv_wvlvl = wvlvl;
     c->wv_level[q] = 0.0;
@@ -719,6 +743,24 @@ int cad_wave_theory_apply(cad_ctx *c, int jd)
     const double wvt = (double)r4tn;
     const double wvz = cwmak1(w, wvt);
     c->wv_level[q] = wvz * aw;
+    // direction of the wave (vf_bwuwn.f:297-325): normal to the side, or at
+    // ANGLE degrees from it with the phase shifted along the ray through
+    // (SX, SY)
+    double sa, sb, sc = 0.0;
+    const int isa = std::fabs(c->wv_ang[q][0]) > ZERO_REF;
+    if (!isa) {
+        sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
+        sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
+    } else {
+        double ang = c->wv_ang[q][0];
+        if (jd == 2) ang = ang + 180.0;
+        else if (jd == 3) ang = ang + 90.0;
+        else if (jd == 4) ang = ang + 270.0;
+        ang = ang * PI_REF / 180.0;
+        sb = std::sin(ang);
+        sa = std::cos(ang);
+        sc = -(sa * c->wv_ang[q][1] + sb * c->wv_ang[q][2]);
+    }
     const int xside = jd <= 2;
     const int pos = xside ? (jd == 1 ? 2 : d.ni) : (jd == 3 ? 2 : d.nj);
     const int cpos = xside ? (jd == 1 ? 2 : d.ni - 1) : (jd == 3 ? 2 : d.nj - 1);
@@ -734,8 +776,35 @@ int cad_wave_theory_apply(cad_ctx *c, int jd)
     const size_t per = 2 * (size_t)d.nk;
     double *uv = c->wv_uv + (size_t)q * c->wv_cols * per;
     double *uv_h = c->wv_uv_host + (size_t)q * c->wv_cols * per;
+    // each column's phase and expected level; the scale kernel turns the
+    // level into the column's VWS in place
+    double *ph = c->wv_ph_host + (size_t)q * c->wv_cols;
+    const double *xh = c->xx_host, *yh = c->yy_host;
+    for (int tq = 0; tq < nq; tq++) {
+        double wq = wvt, zq = wvz;
+        if (isa) {
+            const int qq = q0 + tq;
+            double x0, y0;
+            if (xside) {
+                x0 = xh[(size_t)(pos - 1) * 6];
+                y0 = (yh[(size_t)(qq - 1) * 6] + yh[(size_t)qq * 6]) * 0.5;
+            } else {
+                y0 = yh[(size_t)(pos - 1) * 6];
+                x0 = (xh[(size_t)(qq - 1) * 6] + xh[(size_t)qq * 6]) * 0.5;
+            }
+            const double rr = (sa * x0 + sb * y0 + sc) / std::sqrt(sa * sa + sb * sb);
+            float r4 = (float)(t0 - (c->prm.tnow / t - rr / c->wv_wl[q]));
+            r4 = r4 - (float)(int)r4;
+            if (r4 < 0.0f) r4 = r4 + 1.0f;
+            wq = (double)r4;
+            zq = cwmak1(w, wq);
+        }
+        ph[tq] = wq;
+        vws_h[tq] = zq;
+    }
+    CAD_CHECK(hipMemcpyAsync(vws, vws_h, nq * sizeof(double), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_wave_scale, dim3((nq + 63) / 64), dim3(64), 0, c->stream, d, c->zz, c->f.nf,
-                       c->f.ff, xside, cpos, q0, nq, dep, wvz, wvlvl, vws);
+                       c->f.ff, xside, cpos, q0, nq, dep, wvlvl, vws);
     CAD_CHECK(hipGetLastError());
     CAD_CHECK(hipMemcpyAsync(vws_h, vws, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CAD_CHECK(hipStreamSynchronize(c->stream));
@@ -748,13 +817,11 @@ int cad_wave_theory_apply(cad_ctx *c, int jd)
         for (int k = 2; k <= d.nk - 1; k++) {
             double zc = (z[(size_t)(k - 1) * 6] + z[(size_t)k * 6]) * 0.5 - wvlvl;
             zc = s * (zc + dep) - dep;
-            cwmak2(w, wvt, zc, &col[k - 1], &col[d.nk + k - 1]);
+            cwmak2(w, ph[tq], zc, &col[k - 1], &col[d.nk + k - 1]);
         }
     }
     CAD_CHECK(hipMemcpyAsync(uv, uv_h, (size_t)nq * per * sizeof(double), hipMemcpyHostToDevice,
                              c->stream));
-    const double sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
-    const double sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
     hipLaunchKernelGGL(k_wave_theory_faces, dim3((nq + 63) / 64), dim3(64), 0, c->stream, d,
                        xside ? c->indx : c->indy, vws, uv, xside, pos, q0, nq, aw, sa, sb,
                        xside ? c->f.uu : c->f.vv, c->f.bcu, c->f.bcv, c->f.bcw);

@@ -393,7 +393,7 @@ def _parse_model(d, w):
     if _WAVE_SIDE[w[2]] in d.opens:
         raise DeckError("MODEL WAVE-BC: DEFINED DOUBLY.")
     wv = d.waves.setdefault(_WAVE_SIDE[w[2]], dict(func=None, depth=0.0, height=0.0, period=0.0,
-                                                   ampl=0.0, area=None, angle=0.0))
+                                                   ampl=0.0, area=None, angle=0.0, ax=0.0, ay=0.0))
     key = w[3]
     if key == "FUNC":
         if len(w) < 5:
@@ -418,7 +418,7 @@ def _parse_model(d, w):
     elif key == "ANGLE":
         if len(w) < 7:
             raise DeckError("MODEL WAVE-BC ANGLE: SYNTAX ERROR.")
-        wv["angle"] = _num(w[4])
+        wv["angle"], wv["ax"], wv["ay"] = _num(w[4]), _num(w[5]), _num(w[6])
     else:
         raise DeckError(f"MODEL WAVE-BC {key}: UNKNOWN WORD.")
 
@@ -626,10 +626,8 @@ def _validate(d):
             raise DeckError("MODEL WAVE-BC: (WAVE-BC HEIGHT) NOT FOUND.")
         if wv["period"] < 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC PERIOD) NOT FOUND.")
-        if abs(wv["angle"]) > 1.0e-20:
-            if wv["func"] == "MATRIX":
-                raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
-            raise UnsupportedDeck("MODEL WAVE-BC ANGLE: oblique theory waves are not implemented")
+        if abs(wv["angle"]) > 1.0e-20 and wv["func"] == "MATRIX":
+            raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
     # vf_csetup.f:221-234: a damping side needs DEGREE, WIDTH and DEPTH
     for jd, dm in d.damps.items():
         if dm["degree"] < 0:

@@ -80,6 +80,10 @@ def _cases():
                          [1, 2, 100, 300, 600]),
         "cnoidal_flume": (lambda: cfg.theory_flume("STK-CND", H=0.03, period=2.0,
                                                    name="cnoidal_flume"), [1, 2, 100, 300, 600]),
+        # the Stokes wave at 25 degrees to the x- side's normal (MODEL WAVE-BC ANGLE)
+        "oblique_basin": (lambda: cfg.theory_flume("STOKES", ni=30, nj=16, nk=16, h=0.16, period=0.8,
+                                                   steps=300, angle=(25.0, 0.0, 0.1),
+                                                   name="oblique_basin"), [1, 2, 100, 300]),
     }
 
 

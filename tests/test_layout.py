@@ -10,7 +10,8 @@ import pytest
 import fixtures as fx
 
 CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume",
-         "porous_df", "wave_damp", "damp3d", "stokes_flume", "cnoidal_flume"]
+         "porous_df", "wave_damp", "damp3d", "stokes_flume", "cnoidal_flume",
+         "oblique_basin"]
 STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
           "INDX", "INDY", "INDZ", "INDB", "BCVI")
 
@@ -98,9 +99,10 @@ def test_deck_drag_df_words(pkg):
 
 
 def test_deck_theory_wave_words(pkg):
-    """MODEL WAVE-BC FUNC STOKES / CNOIDAL / STK-CND (Source code/vf_iimdl.f:51-70)
-    with the set-up checks of vf_csetup.f:86-94: a theory wave needs DEPTH,
-    HEIGHT and PERIOD; STREAM and MATRIX2 are outside what is produced here."""
+    """MODEL WAVE-BC FUNC STOKES / CNOIDAL / STK-CND and ANGLE (Source
+    code/vf_iimdl.f:51-70, 107-111) with the set-up checks of
+    vf_csetup.f:86-100: a theory wave needs DEPTH, HEIGHT and PERIOD, the table
+    takes no ANGLE; STREAM and MATRIX2 are outside what is produced here."""
     base = ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
             "B.C. D VP SLIP\nB.C. D F FREE\n")
     full = ("MODEL WAVE-BC X- DEPTH 0.2\nMODEL WAVE-BC X- HEIGHT 0.02\n"
@@ -116,9 +118,12 @@ def test_deck_theory_wave_words(pkg):
     for func in ("STREAM 5", "MATRIX2"):
         with pytest.raises(pkg.deck.UnsupportedDeck):
             pkg.deck.parse(base + f"MODEL WAVE-BC X- FUNC {func}\n" + full)
-    with pytest.raises(pkg.deck.UnsupportedDeck):  # oblique theory wave
-        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STOKES\n" + full +
-                       "MODEL WAVE-BC X- ANGLE 10.0 0.0 0.0\n")
+    d = pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STOKES\n" + full +
+                       "MODEL WAVE-BC X- ANGLE 10.0 0.5 -0.25\n")
+    assert (d.waves[1]["angle"], d.waves[1]["ax"], d.waves[1]["ay"]) == (10.0, 0.5, -0.25)
+    with pytest.raises(pkg.deck.DeckError):        # the table refuses an angle
+        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC MATRIX\nMODEL WAVE-BC X- DEPTH 0.2\n"
+                       "MODEL WAVE-BC X- PERIOD 1.0\nMODEL WAVE-BC X- ANGLE 10.0 0.0 0.0\n")
 
 
 def test_deck_damp_words(pkg):

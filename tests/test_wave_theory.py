@@ -19,7 +19,7 @@ import pytest
 
 import fixtures as fx
 
-CASES = ["stokes_flume", "cnoidal_flume"]
+CASES = ["stokes_flume", "cnoidal_flume", "oblique_basin"]
 
 
 def _kind(pkg, wv, grz0):
@@ -34,6 +34,15 @@ def _kind(pkg, wv, grz0):
 def _phase(t0, tnow, t):
     """The single-precision phase of vf_bwuwn.f:270-276."""
     r = np.float32(t0 - tnow / t)
+    r = np.float32(r - np.float32(int(r)))
+    if r < np.float32(0.0):
+        r = np.float32(r + np.float32(1.0))
+    return float(r)
+
+
+def _phase_ray(t0, tnow, t, rr, dl):
+    """The phase of a column at distance rr along an oblique wave's ray."""
+    r = np.float32(t0 - (tnow / t - rr / dl))
     r = np.float32(r - np.float32(int(r)))
     if r < np.float32(0.0):
         r = np.float32(r + np.float32(1.0))
@@ -75,9 +84,17 @@ def test_theory_wave_faces_match_reference(pkg, case):
     wv = d.waves[1]
     grz0 = fx.deck_params(st["DECK"])["grz0"]
     kind = _kind(pkg, wv, grz0)
-    assert kind == (-2 if case == "stokes_flume" else -1)
+    assert kind == (-1 if case == "cnoidal_flume" else -2)
     dep, t, wvlvl = wv["depth"], wv["period"], d.wvlvl
-    zz, indx = st["ZZ"], st["INDX"]
+    zz, indx, xx, yy = st["ZZ"], st["INDX"], st["XX"], st["YY"]
+    # oblique incidence (vf_bwuwn.f:297-325): the wave's direction, and each
+    # column's phase shifted along the ray by the set-up wave length
+    isa = abs(wv["angle"]) > 1.0e-20
+    sa, sb, sc = 1.0, 0.0, 0.0
+    if isa:
+        ang = wv["angle"] * 3.141592653589794 / 180.0
+        sb, sa = np.sin(ang), np.cos(ang)
+        sc = -(sa * wv["ax"] + sb * wv["ay"])
     numk = zz.shape[1]
     for s in fx.steps(case):
         r = fx.step(case, s)
@@ -85,6 +102,7 @@ def test_theory_wave_faces_match_reference(pkg, case):
             continue
         w = pkg.capi.TheoryWave(kind, dep, wv["height"], t)
         _, _, _, t0 = w.call(origin=True)            # VF_CSETUP
+        dl = w.length
         for _ in range(s):                           # VF_CINIT, steps 1 .. s-1
             w.call()
         tnow = float(r["v1_in"]["TNOW"][0])
@@ -95,16 +113,26 @@ def test_theory_wave_faces_match_reference(pkg, case):
         w.state.value = keep                          # one VF_CWMAK0 for the step
         bcu, bcw = r["v1_out"]["BCU"], r["v1_out"]["BCW"]
         checked = 0
+        bcv = r["v1_out"]["BCV"]
         for j in range(2, indx.shape[1]):
-            vws = _scale(st, r["v1_in"], j, wvz, dep, wvlvl)
+            wq, zq = wvt, wvz
+            if isa:
+                x0, y0 = xx[0, 1], (yy[0, j - 1] + yy[0, j]) * 0.5
+                rr = (sa * x0 + sb * y0 + sc) / np.sqrt(sa * sa + sb * sb)
+                wq = _phase_ray(t0, tnow, t, rr, dl)
+                keep = w.state.value
+                zq = w.call(wq)[0]
+                w.state.value = keep
+            vws = _scale(st, r["v1_in"], j, zq, dep, wvlvl)
             ks = [k for k in range(2, numk) if indx[1, j - 1, k - 1] >= 1]
             zc = [vws * ((zz[0, k - 1] + zz[0, k]) * 0.5 - wvlvl + dep) - dep for k in ks]
             keep = w.state.value
-            _, un, ut, _ = w.call(wvt, zc)
+            _, un, ut, _ = w.call(wq, zc)
             w.state.value = keep
             for k, a, b in zip(ks, un, ut):
                 ll = int(indx[1, j - 1, k - 1]) - 1
-                assert a * vws * aw * 1.0 == bcu[ll], (s, j, k)
+                assert a * vws * aw * sa == bcu[ll], (s, j, k)
+                assert a * vws * aw * sb == bcv[ll], (s, j, k)
                 assert b * vws * aw == bcw[ll], (s, j, k)
                 checked += 1
         assert checked > 0
