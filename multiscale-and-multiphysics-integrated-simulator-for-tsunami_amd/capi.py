@@ -260,7 +260,9 @@ class Context:
             zz, tt, hh = (_f64(getattr(table, k)) for k in ("zz", "tt", "hh"))
             nz, nt = len(zz), len(tt)
             un, ut = _f64(table.un, (nz, nt)), _f64(table.ut, (nz, nt))
-            _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), _pd(hh), _pd(un),
+            # a LEVEL-OFF table goes without its level (a null level array)
+            hp = None if table.kind == "LEVEL-OFF" else _pd(hh)
+            _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), hp, _pd(un),
                                             _pd(ut), float(table.t0)))
         ni, nj, _ = self.shape
         for side, wv in sorted(waves.items()):

@@ -335,8 +335,9 @@ def subloop(loops=3, steps=40, name=None):
     return Case(c.name, deck, c.ni, c.nj, c.nk, steps, c.dt)
 
 
-def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81):
-    """A LEVEL-ON wave-maker table (.mtb, Source code/vf_im1inp.f) of one
+def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81, kind="LEVEL-ON"):
+    """A LEVEL-ON (or, without its level column, LEVEL-OFF) wave-maker table
+    (.mtb, Source code/vf_im1inp.f) of one
     solitary wave.  The reference walks the table phase as frac(T0 - t/T),
     so with T0 = 0 the phase falls from 1 and time t reads phase 1 - t/T.
     Level eta = H sech^2(sqrt(3H/4h^3) c (t - t_crest)), c = sqrt(g(h+H));
@@ -348,19 +349,19 @@ def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81):
     zs = np.linspace(-h, ztop, nz)
     # the reader takes at most 256 characters of a line (MAXCHR): one value
     # or one (normal, tangential) pair per line
-    lines = ["LEVEL-ON", str(nz), str(nt), "0.0"] + [f"{z:.8e}" for z in zs]
+    lines = [kind, str(nz), str(nt), "0.0"] + [f"{z:.8e}" for z in zs]
     for i in range(nt):
         ph = i / (nt - 1)
         t = (1.0 - ph) * period
         eta = H / np.cosh(kk * c * (t - t_crest)) ** 2
         u = np.sqrt(g / h) * eta
-        lines.append(f"T {ph:.8e} {eta:.8e}")
+        lines.append(f"T {ph:.8e} {eta:.8e}" if kind == "LEVEL-ON" else f"T {ph:.8e}")
         lines += [f"{(u if z <= eta else 0.0):.8e} 0.0" for z in zs]
     return "\n".join(lines) + "\n"
 
 
 def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400, dt=1.0e-3,
-               eps_r=1.0e-10, open_end=False, name=None):
+               eps_r=1.0e-10, open_end=False, table="LEVEL-ON", name=None):
     """A flume with a table wave-maker on its x- side (MODEL WAVE-BC X- FUNC
     MATRIX with <name>.mtb, vf_bwuwn.f:257-480): a solitary wave enters the
     still water (MATE W-LEVEL) and runs to the far wall, or with open_end
@@ -379,7 +380,7 @@ def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400,
                   f"MODEL OPEN-BC X+ PERIOD {period}"]
     _common(lines, dt, steps, eps_r, wlevel=h)
     case = Case(name or f"wave_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
-    case.meta["files"] = {"mtb": solitary_mtb(h, H, period, t_crest=0.2)}
+    case.meta["files"] = {"mtb": solitary_mtb(h, H, period, t_crest=0.2, kind=table)}
     return case
 
 
@@ -445,6 +446,7 @@ def named(name):
         "wave_open": lambda: wave_flume(ni=40, steps=1000, open_end=True),
         "porous_df": lambda: porous_df(),
         "wave_damp": lambda: wave_damp(),
+        "wave_off": lambda: wave_flume(ni=40, steps=400, table="LEVEL-OFF"),
         "stokes_flume": lambda: theory_flume("STOKES"),
         "cnoidal_flume": lambda: theory_flume("STK-CND", H=0.03, period=2.0),
         "oblique_basin": lambda: theory_flume("STOKES", ni=30, nj=16, nk=16, h=0.16, period=0.8,

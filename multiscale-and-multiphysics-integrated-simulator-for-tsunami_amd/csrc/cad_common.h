@@ -197,6 +197,7 @@ struct cad_ctx {
     // (-3 the table, -2 Stokes, -1 cnoidal), height, phase origin and the
     // persistent secant step of VF_STK0; per side the columns' level scales
     // and face velocities (device and pinned host)
+    int wv_off;         // the table is LEVEL-OFF
     int wv_kind[4];
     double wv_h[4], wv_t0s[4];
     double wv_wl[4];        // wave length of VF_CWMAK0 at set-up (BCTYP(4,side))

@@ -42,6 +42,7 @@ struct WaveLaunch {
     int q0, nq;          // first column along the side and column count
     int nz;              // table depths
     int im;              // MTBNOW: the table's time interval for this call
+    int off;             // LEVEL-OFF: profiles unscaled
     double d, wvz, aw, wmt1, wmt2, sa, sb, wvlvl;
 };
 
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__(64) k_wave_maker(Dims d, const double *__restr
         }
     }
     val = val - (w.wvlvl - zz[6]);
-    const double vws = (w.wvz + w.d) / (val + w.d);
+    const double vws = w.off ? 1.0 : (w.wvz + w.d) / (val + w.d);
     // the column's faces bottom up (vf_bwuwn.f:331-380); VF_CWMTB2's depth
     // search starts where the face below found its interval
     const int nz = w.nz;
@@ -281,7 +282,7 @@ void cad_wave_free(cad_ctx *c)
 int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
                    const double *hh, const double *un, const double *ut, double t0)
 {
-    if (nz < 2 || nt < 2 || !zz || !tt || !hh || !un || !ut) {
+    if (nz < 2 || nt < 2 || !zz || !tt || !un || !ut) {
         cad_set_error("cad_set_wave_table: need at least 2 depths and 2 times, non-null arrays");
         return CAD_ERR_ARG;
     }
@@ -309,7 +310,13 @@ int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *t
         return CAD_ERR_HIP;
     }
     std::memcpy(c->wv_tt, tt, nt * sizeof(double));
-    std::memcpy(c->wv_hh, hh, nt * sizeof(double));
+    // LEVEL-OFF (no level column): the level is 0 and the profiles are not
+    // scaled (vf_im1inp.f:148-155, vf_bwuwn.f:326-329)
+    c->wv_off = hh == nullptr;
+    if (hh)
+        std::memcpy(c->wv_hh, hh, nt * sizeof(double));
+    else
+        std::memset(c->wv_hh, 0, nt * sizeof(double));
     CAD_CHECK(hipMemcpy(c->wv_tab, zz, nz * sizeof(double), hipMemcpyHostToDevice));
     CAD_CHECK(hipMemcpy(c->wv_tab + nz, un, tab * sizeof(double), hipMemcpyHostToDevice));
     CAD_CHECK(hipMemcpy(c->wv_tab + nz + tab, ut, tab * sizeof(double), hipMemcpyHostToDevice));
@@ -417,6 +424,7 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         w.nq = c->wv_l2[q] - c->wv_l1[q] + 1;
         w.nz = c->wv_nz;
         w.im = c->wv_mtbnow;
+        w.off = c->wv_off;
         w.d = dep;
         w.wvz = wvz;
         w.aw = aw;

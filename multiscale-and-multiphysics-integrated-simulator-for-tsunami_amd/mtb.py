@@ -6,8 +6,9 @@ LEVEL-OFF: profiles only; LEVEL-ONLY: level only), the number of z levels
 MTBZZ (not for LEVEL-ONLY), the number of phases MTBTT, the initial phase in
 [0,1], the z levels (mean water level at 0, increasing), then per phase
 'T', the phase (increasing in [0,1]), the level (not for LEVEL-OFF) and for
-every z level the normal and tangential velocity.  Only LEVEL-ON is used by
-the device (the other kinds change the boundary treatment and are rejected).
+every z level the normal and tangential velocity.  LEVEL-ON and LEVEL-OFF are
+produced (a LEVEL-OFF table's level is 0 and its profiles are not scaled by the
+measured level); LEVEL-ONLY changes the boundary treatment and is rejected.
 """
 from dataclasses import dataclass
 
@@ -22,7 +23,7 @@ class MtbError(ValueError):
 
 @dataclass
 class Table:
-    kind: str          # LEVEL-ON
+    kind: str          # LEVEL-ON or LEVEL-OFF
     t0: float          # DMTBT0
     zz: np.ndarray     # DMTBZZ (MTBZZ)
     tt: np.ndarray     # DMTBTT (MTBTT)
@@ -52,8 +53,8 @@ def parse(text) -> Table:
     kind = nxt()
     if kind not in ("LEVEL-ON", "LEVEL-OFF", "LEVEL-ONLY"):
         raise MtbError(f"VF_IM1INP: UNKNOWN WORD {kind}.")
-    if kind != "LEVEL-ON":
-        raise MtbError(f"{kind} tables are not implemented (only LEVEL-ON)")
+    if kind == "LEVEL-ONLY":
+        raise MtbError("LEVEL-ONLY tables are not implemented (LEVEL-ON and LEVEL-OFF are)")
     nz = zstoi(nxt())
     nt = zstoi(nxt())
     if nz <= 0 or nt <= 0:
@@ -76,7 +77,8 @@ def parse(text) -> Table:
         tt[i] = num()
         if not 0.0 <= tt[i] <= 1.0 or (i > 0 and tt[i] <= tt[i - 1]):
             raise MtbError("VF_IM1INP: INVALID VALUE (phase).")
-        hh[i] = num()
+        if kind != "LEVEL-OFF":
+            hh[i] = num()
         for k in range(nz):
             un[k, i] = num()
             ut[k, i] = num()

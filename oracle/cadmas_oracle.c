@@ -488,7 +488,9 @@ static int wave_bc(orc_state *s)
                 }
             }
             val = val - (s->wvlvl - ZZ(1, 2));
-            const double vws = (wvz + d) / (val + d);
+            /* a LEVEL-OFF table (MTBTYP=2) takes its profiles unscaled
+             * (vf_bwuwn.f:326-329) */
+            const double vws = s->mtb_off ? 1.0 : (wvz + d) / (val + d);
             int kmt = 1;
             for (int k = 2; k <= nk - 1; k++) {
                 const size_t f = xside ? ID(pos, q, k) : ID(q, pos, k);

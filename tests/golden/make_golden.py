@@ -69,6 +69,9 @@ def _cases():
         # ... with a radiation boundary at the far end (MODEL OPEN-BC X+ FUNC TYPE1)
         "wave_open": (lambda: cfg.wave_flume(ni=40, steps=1000, open_end=True, name="wave_open"),
                       [1, 2, 250, 500, 750, 1000]),
+        # ... driven by a LEVEL-OFF table (profiles not scaled by the level)
+        "wave_off": (lambda: cfg.wave_flume(ni=40, steps=400, table="LEVEL-OFF", name="wave_off"),
+                     [1, 2, 100, 200, 400]),
         # Dupuit-Forchheimer drag (OPTION DRAG-DF, vf_vgene.f:111-146)
         "porous_df": (lambda: cfg.porous_df(name="porous_df"), [1, 2, 3, 30, 60]),
         # implicit damping zones (MODEL DAMP, vf_veuler.f:131-166)
@@ -98,6 +101,8 @@ _KEEP = {
                        "f1_out": None} for s in (100, 200, 300, 400)},
     "wave_open": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                       "f1_out": None} for s in (250, 500, 750, 1000)},
+    "wave_off": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                     "f1_out": None} for s in (100, 200, 400)},
     "wave_damp": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                       "f1_out": None} for s in (300, 450, 600)},
     "stokes_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,

@@ -9,7 +9,9 @@ radiation boundary at the far end that the wave leaves through, 1000 steps
 recorded at 1, 2, 250, 500, 750, 1000; "stokes_flume" and "cnoidal_flume",
 40x4x20 flumes driven by a fifth-order Stokes and (by the Ursell number) a
 third-order cnoidal wave, 600 steps recorded at 1, 2, 100, 300, 600;
-"oblique_basin", a 30x16x16 basin with the Stokes wave at 25 degrees to the
+"wave_off", the 40-cell flume driven by the same wave as a LEVEL-OFF table
+(no level column: profiles unscaled), 400 steps recorded at 1, 2, 100, 200,
+400; "oblique_basin", a 30x16x16 basin with the Stokes wave at 25 degrees to the
 side's normal (MODEL WAVE-BC ANGLE), 300 steps recorded at 1, 2, 100, 300.  The
 predictor and the boundary pass
 that sets the special faces have no reduction, so they are compared bit for
@@ -24,10 +26,10 @@ from helpers import assert_close, assert_same_int, assert_vel_close, rel_err
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["wave_flume", "wave_open", "stokes_flume", "cnoidal_flume", "oblique_basin"]
+CASES = ["wave_flume", "wave_open", "wave_off", "stokes_flume", "cnoidal_flume", "oblique_basin"]
 RSL = {"wave_flume": (100, 400, 100), "wave_open": (250, 1000, 250),
        "stokes_flume": (300, 600, 300), "cnoidal_flume": (300, 600, 300),
-       "oblique_basin": (100, 300, 200)}
+       "oblique_basin": (100, 300, 200), "wave_off": (200, 400, 200)}
 STATE = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")
 KEYS = ("rho0", "grz0", "anu0", "scmvp", "flower", "fupper", "plower", "cgpara", "cgepsa",
         "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
