@@ -124,8 +124,12 @@ int cad_download_tdoor(cad_ctx *ctx, double *tbub, double *droptx, double *dropt
  * tangential velocities UN(NZ,NT), UT(NZ,NT) in Fortran order, and the phase
  * origin T0.  A LEVEL-OFF table (no level column) is passed with hh = NULL:
  * its level is 0 and its profiles are not scaled by the measured level
- * (vf_bwuwn.f:326-329).  Resets the time search start (MTBNOW) and clears the
- * table sides set before (cad_set_wave comes after the table). */
+ * (vf_bwuwn.f:326-329).  A LEVEL-ONLY table (a fixed water level) is passed
+ * with nz = 0 and zz, un, ut NULL: its sides' faces are then taken as free faces
+ * (the list pass of VF_BWUWN, VF_VPCOEF, VF_VMODIF, VF_BWUWT) and the cells next
+ * to them take F from the level (VF_BWFFSF, in VF_F1CAL and the initial state).
+ * Resets the time search start (MTBNOW) and clears the table sides set before
+ * (cad_set_wave comes after the table). */
 int cad_set_wave_table(cad_ctx *ctx, int nz, int nt, const double *zz, const double *tt,
                        const double *hh, const double *un, const double *ut, double t0);
 /* One wave-maker side: 1..4 = X-, X+, Y-, Y+; columns l1..l2 along it

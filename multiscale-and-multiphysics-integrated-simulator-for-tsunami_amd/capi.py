@@ -260,10 +260,15 @@ class Context:
             zz, tt, hh = (_f64(getattr(table, k)) for k in ("zz", "tt", "hh"))
             nz, nt = len(zz), len(tt)
             un, ut = _f64(table.un, (nz, nt)), _f64(table.ut, (nz, nt))
-            # a LEVEL-OFF table goes without its level (a null level array)
+            # a LEVEL-OFF table goes without its level (a null level array), a
+            # LEVEL-ONLY table without depths and profiles (nz = 0)
             hp = None if table.kind == "LEVEL-OFF" else _pd(hh)
-            _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), hp, _pd(un),
-                                            _pd(ut), float(table.t0)))
+            if table.kind == "LEVEL-ONLY":
+                _check(lib().cad_set_wave_table(self._h, 0, nt, None, _pd(tt), hp, None, None,
+                                                float(table.t0)))
+            else:
+                _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), hp, _pd(un),
+                                                _pd(ut), float(table.t0)))
         ni, nj, _ = self.shape
         for side, wv in sorted(waves.items()):
             lim = nj if side <= 2 else ni

@@ -336,7 +336,8 @@ def subloop(loops=3, steps=40, name=None):
 
 
 def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81, kind="LEVEL-ON"):
-    """A LEVEL-ON (or, without its level column, LEVEL-OFF) wave-maker table
+    """A LEVEL-ON wave-maker table (LEVEL-OFF: without its level column;
+    LEVEL-ONLY: the level alone)
     (.mtb, Source code/vf_im1inp.f) of one
     solitary wave.  The reference walks the table phase as frac(T0 - t/T),
     so with T0 = 0 the phase falls from 1 and time t reads phase 1 - t/T.
@@ -349,14 +350,18 @@ def solitary_mtb(h, H, period, t_crest, nz=27, nt=201, ztop=None, g=9.81, kind="
     zs = np.linspace(-h, ztop, nz)
     # the reader takes at most 256 characters of a line (MAXCHR): one value
     # or one (normal, tangential) pair per line
-    lines = [kind, str(nz), str(nt), "0.0"] + [f"{z:.8e}" for z in zs]
+    if kind == "LEVEL-ONLY":
+        lines = [kind, str(nt), "0.0"]
+    else:
+        lines = [kind, str(nz), str(nt), "0.0"] + [f"{z:.8e}" for z in zs]
     for i in range(nt):
         ph = i / (nt - 1)
         t = (1.0 - ph) * period
         eta = H / np.cosh(kk * c * (t - t_crest)) ** 2
         u = np.sqrt(g / h) * eta
-        lines.append(f"T {ph:.8e} {eta:.8e}" if kind == "LEVEL-ON" else f"T {ph:.8e}")
-        lines += [f"{(u if z <= eta else 0.0):.8e} 0.0" for z in zs]
+        lines.append(f"T {ph:.8e}" if kind == "LEVEL-OFF" else f"T {ph:.8e} {eta:.8e}")
+        if kind != "LEVEL-ONLY":
+            lines += [f"{(u if z <= eta else 0.0):.8e} 0.0" for z in zs]
     return "\n".join(lines) + "\n"
 
 
@@ -447,6 +452,7 @@ def named(name):
         "porous_df": lambda: porous_df(),
         "wave_damp": lambda: wave_damp(),
         "wave_off": lambda: wave_flume(ni=40, steps=400, table="LEVEL-OFF"),
+        "wave_only": lambda: wave_flume(ni=40, steps=400, table="LEVEL-ONLY"),
         "stokes_flume": lambda: theory_flume("STOKES"),
         "cnoidal_flume": lambda: theory_flume("STK-CND", H=0.03, period=2.0),
         "oblique_basin": lambda: theory_flume("STOKES", ni=30, nj=16, nk=16, h=0.16, period=0.8,

@@ -33,6 +33,7 @@ Static cad_static(const cad_ctx *c)
     s.glv = c->glv; s.glx = c->glx; s.gly = c->gly; s.glz = c->glz;
     s.indx = c->indx; s.indy = c->indy; s.indz = c->indz;
     s.indb = c->indb;
+    s.lvo = cad_level_sides(c);
     return s;
 }
 

@@ -65,7 +65,19 @@ struct Static {
     const double *glv, *glx, *gly, *glz;
     const int *indx, *indy, *indz;
     const int *indb;     // INDB(4,NUMB)
+    int lvo;             // sides (bits 0..3 = X-, X+, Y-, Y+) driven by a LEVEL-ONLY table
 };
+
+// The boundary type of face is (1-based) in VF_BWUWN's list pass, VF_VPCOEF,
+// VF_VMODIF and VF_BWUWT: a wave-maker face (5) on a side with a fixed level
+// (LEVEL-ONLY table) is taken there as a free face (4) (vf_bwuwn.f:161-211,
+// vf_vpcoef.f:162-175, vf_vmodif.f:87-110, vf_bwuwt.f:80-116).
+__device__ __forceinline__ int face_type(const int *indb, int lvo, int is)
+{
+    const int *e = indb + (long long)(is - 1) * 4;
+    const int ib = e[2], ns = e[1];
+    return (ib == 5 && ns >= 1 && ns <= 4 && ((lvo >> (ns - 1)) & 1)) ? 4 : ib;
+}
 
 __device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
 {
@@ -198,6 +210,7 @@ struct cad_ctx {
     // persistent secant step of VF_STK0; per side the columns' level scales
     // and face velocities (device and pinned host)
     int wv_off;         // the table is LEVEL-OFF
+    int wv_only;        // the table is LEVEL-ONLY (a fixed level, no profiles)
     int wv_kind[4];
     double wv_h[4], wv_t0s[4];
     double wv_wl[4];        // wave length of VF_CWMAK0 at set-up (BCTYP(4,side))
@@ -265,6 +278,8 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
 int cad_wave_apply(cad_ctx *c, bool initial);
 int cad_open_side(cad_ctx *c, int side, double depth, double period);
 int cad_wave_check(cad_ctx *c);
+int cad_launch_bwffsf(cad_ctx *c);
+int cad_level_sides(const cad_ctx *c);
 int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double depth, double height,
                          double period, double ampl, double wvlvl, double angle, double ax, double ay);
 int cad_wave_theory_apply(cad_ctx *c, int jd);

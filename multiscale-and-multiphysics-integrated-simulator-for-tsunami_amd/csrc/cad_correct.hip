@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) k_vmodif(Dims d, Static s, const double *
     if (is == 0) {
         double gv = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * ggv[n] / glv[n] + met<A>(s, 2, ia - 1) * ggv[n - sa] / glv[n - sa]);
         q[n] = q[n] + met<A>(s, 5, ia) * gv * (pt[n] - pt[n - sa]);
-    } else if (is >= 1 && s.indb[(long long)(is - 1) * 4 + 2] == 4) {
+    } else if (is >= 1 && face_type(s.indb, s.lvo, is) == 4) {
         if (s.indb[(long long)(is - 1) * 4 + 1] == 2 * A + 1)
             q[n] = q[n] + 2.0 * met<A>(s, 4, ia) * ggv[n] / glv[n] * pt[n];
         else
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) k_bsuwn(Dims d, Static s, const int *__re
 }
 
 // Tangential wall velocities, vf_bwuwt.f:42-176 (types 1 and 4).
-__global__ void k_bwuwt(Dims d, const int *__restrict__ indb, const double *__restrict__ u,
+__global__ void k_bwuwt(Dims d, const int *__restrict__ indb, int lvo, const double *__restrict__ u,
                         const double *__restrict__ v, const double *__restrict__ w, double *bcu,
                         double *bcv, double *bcw)
 {
@@ -170,7 +170,7 @@ __global__ void k_bwuwt(Dims d, const int *__restrict__ indb, const double *__re
     if (l >= d.numb) return;
     const int *e = indb + (long long)l * 4;
     const long long n = (long long)e[0] - 1;
-    const int ns = e[1], ib = e[2];
+    const int ns = e[1], ib = face_type(indb, lvo, l + 1);
     if (!(ib == 1 || ib == 4)) return;
     const long long ni = d.ni, nij = d.nij;
     switch (ns) {
@@ -252,6 +252,7 @@ int cad_launch_bwuwt(cad_ctx *c)
 {
     if (c->d.numb <= 0) return CAD_OK;
     hipLaunchKernelGGL(k_bwuwt, dim3((c->d.numb + 255) / 256), dim3(256), 0, c->stream, c->d, c->indb,
+                       cad_level_sides(c),
                        c->f.uu, c->f.vv, c->f.ww, c->f.bcu, c->f.bcv, c->f.bcw);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;

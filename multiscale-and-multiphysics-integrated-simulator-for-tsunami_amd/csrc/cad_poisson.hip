@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_vpcoef(Dims d, Static s, const double *
             aim = ayz * met<0>(s, 5, i) * s.ggx[n] * gv;
             vli = -aim;
         } else if (is >= 1) {
-            aim = s.indb[(long long)(is - 1) * 4 + 2] == 4 ? 2.0 * ayz * met<0>(s, 4, i) * s.ggx[n] * ggv[n] / glv[n] : 0.0;
+            aim = face_type(s.indb, s.lvo, is) == 4 ? 2.0 * ayz * met<0>(s, 4, i) * s.ggx[n] * ggv[n] / glv[n] : 0.0;
         } else { atomicExch(err, 1); aim = 0.0; }
         is = s.indx[n + 1];
         if (is == 0) {
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_vpcoef(Dims d, Static s, const double *
             aip = ayz * met<0>(s, 5, i + 1) * s.ggx[n + 1] * gv;
             vui = -aip;
         } else if (is >= 1) {
-            aip = s.indb[(long long)(is - 1) * 4 + 2] == 4 ? 2.0 * ayz * met<0>(s, 4, i) * s.ggx[n + 1] * ggv[n] / glv[n] : 0.0;
+            aip = face_type(s.indb, s.lvo, is) == 4 ? 2.0 * ayz * met<0>(s, 4, i) * s.ggx[n + 1] * ggv[n] / glv[n] : 0.0;
         } else { atomicExch(err, 1); aip = 0.0; }
         is = s.indy[n];
         if (is == 0) {
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) k_vpcoef(Dims d, Static s, const double *
             ajm = axz * met<1>(s, 5, j) * s.ggy[n] * gv;
             vlj = -ajm;
         } else if (is >= 1) {
-            ajm = s.indb[(long long)(is - 1) * 4 + 2] == 4 ? 2.0 * axz * met<1>(s, 4, j) * s.ggy[n] * ggv[n] / glv[n] : 0.0;
+            ajm = face_type(s.indb, s.lvo, is) == 4 ? 2.0 * axz * met<1>(s, 4, j) * s.ggy[n] * ggv[n] / glv[n] : 0.0;
         } else { atomicExch(err, 1); ajm = 0.0; }
         is = s.indy[n + ni];
         if (is == 0) {
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) k_vpcoef(Dims d, Static s, const double *
             ajp = axz * met<1>(s, 5, j + 1) * s.ggy[n + ni] * gv;
             vuj = -ajp;
         } else if (is >= 1) {
-            ajp = s.indb[(long long)(is - 1) * 4 + 2] == 4 ? 2.0 * axz * met<1>(s, 4, j) * s.ggy[n + ni] * ggv[n] / glv[n] : 0.0;
+            ajp = face_type(s.indb, s.lvo, is) == 4 ? 2.0 * axz * met<1>(s, 4, j) * s.ggy[n + ni] * ggv[n] / glv[n] : 0.0;
         } else { atomicExch(err, 1); ajp = 0.0; }
         is = s.indz[n];
         if (is == 0) {

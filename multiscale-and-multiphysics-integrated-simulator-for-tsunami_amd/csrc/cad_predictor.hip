@@ -317,14 +317,14 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Da
 // VF_BWUWN boundary-list pass (vf_bwuwn.f:84-254): normal velocity on wall and
 // free boundary faces.  Type 5 (wave-maker) and type 7 (radiation) faces are
 // set afterwards by the special sides (cad_wave.hip).
-__global__ void k_bwuwn(Dims d, const int *__restrict__ indb, double *u, double *v, double *w,
+__global__ void k_bwuwn(Dims d, const int *__restrict__ indb, int lvo, double *u, double *v, double *w,
                         double *bcu, double *bcv, double *bcw)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= d.numb) return;
     const int *e = indb + (long long)l * 4;
     const long long n = (long long)e[0] - 1;
-    const int ns = e[1], ib = e[2];
+    const int ns = e[1], ib = face_type(indb, lvo, l + 1);
     if (ib == 1 || ib == 6 || ib == 8) {
         if (ns == 1 || ns == 2) { bcu[l] = 0.0; u[n] = 0.0; }
         else if (ns == 3 || ns == 4) { bcv[l] = 0.0; v[n] = 0.0; }
@@ -417,6 +417,7 @@ int cad_launch_bwuwn(cad_ctx *c, bool initial)
     const Dims &d = c->d;
     if (d.numb <= 0) return CAD_OK;
     hipLaunchKernelGGL(k_bwuwn, dim3((d.numb + 255) / 256), dim3(256), 0, c->stream, d, c->indb,
+                       cad_level_sides(c),
                        c->f.uu, c->f.vv, c->f.ww, c->f.bcu, c->f.bcv, c->f.bcw);
     CAD_CHECK(hipGetLastError());
     // the wave-maker and radiation sides (vf_bwuwn.f:257-620)

@@ -736,6 +736,8 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
     // VF_FEULER (vf_f1cal.f:160)
     hipLaunchKernelGGL(k_feuler, g, b, 0, st, d, s, c->flf[0], c->flf[1], c->flf[2], f.nf, f.ff, dt,
                        (double)c->prm.loops);
+    // VF_BWFFSF: fixed-level sides (vf_f1cal.f:180)
+    RET_IF(cad_launch_bwffsf(c));
     // VF_FMOD1 (vf_f1cal.f:185)
     hipLaunchKernelGGL(k_fmod_count, g, b, 0, st, d, f.nf, f.nlim);
     hipLaunchKernelGGL(k_fmod_redist, g, b, 0, st, d, s, f.nf, f.nlim, f.ff);
@@ -801,6 +803,8 @@ int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, doubl
     Fields &f = c->f;
     dim3 g = cell_grid(d), b = cell_block();
     hipStream_t st = c->stream;
+    // VF_BWFFSF: fixed-level sides (vf_cinit.f:177)
+    RET_IF(cad_launch_bwffsf(c));
     hipLaunchKernelGGL(k_fcut, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, d, s, f.nf, f.ff,
                        c->prm.flower, c->prm.fupper, c->red_partial);
     CAD_CHECK(hipGetLastError());

@@ -129,6 +129,29 @@ __global__ void __launch_bounds__(64) k_wave_maker(Dims d, const double *__restr
     }
 }
 
+// VF_BWFFSF (vf_bwffsf.f:84-122): the cells next to a fixed-level side take F
+// from the level wl: 0 above it, 1 below it, the filled fraction in the cell
+// it cuts; obstacle cells are left.  One thread per (column, K).
+__global__ void __launch_bounds__(64) k_bwffsf(Dims d, const double *__restrict__ zz,
+                                               const int *__restrict__ nf, double *ff, int xside,
+                                               int cpos, int q0, int nq, double wl)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nk = d.nk - 2;
+    if (t >= nq * nk) return;
+    const int q = q0 + t / nk, k = 2 + t % nk;
+    const double z0 = zz[(long long)(k - 1) * 6], z1 = zz[(long long)k * 6];
+    double val;
+    if (z0 > wl)
+        val = 0.0;
+    else if (z1 < wl)
+        val = 1.0;
+    else
+        val = (wl - z0) / (z1 - z0);
+    const long long c = xside ? cid(d, cpos, q, k) : cid(d, q, cpos, k);
+    if (nf[c] != -1) ff[c] = val;
+}
+
 struct OpenLaunch {
     int xside, pos, ic, i2, nq;
     double aln, alt;
@@ -274,6 +297,7 @@ void cad_wave_free(cad_ctx *c)
     c->wv_err_host = nullptr;
     c->wv_tt = c->wv_hh = nullptr;
     c->wv_nz = c->wv_nt = 0;
+    c->wv_off = c->wv_only = 0;
     // the table's sides refer to it: a new table needs them set again
     for (int q = 0; q < 4; q++)
         if (c->wv_kind[q] == -3) c->wv_on[q] = 0;
@@ -282,8 +306,11 @@ void cad_wave_free(cad_ctx *c)
 int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
                    const double *hh, const double *un, const double *ut, double t0)
 {
-    if (nz < 2 || nt < 2 || !zz || !tt || !un || !ut) {
-        cad_set_error("cad_set_wave_table: need at least 2 depths and 2 times, non-null arrays");
+    // nz = 0: a LEVEL-ONLY table (times and levels only, vf_im1inp.f:71-80)
+    const bool only = nz == 0;
+    if ((!only && nz < 2) || nt < 2 || !tt || (!only && (!zz || !un || !ut)) || (only && !hh)) {
+        cad_set_error("cad_set_wave_table: need 2 or more depths (0 for LEVEL-ONLY, which needs the "
+                      "levels) and 2 or more times, non-null arrays");
         return CAD_ERR_ARG;
     }
     for (int k = 1; k < nz; k++)
@@ -300,7 +327,7 @@ int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *t
     CAD_CHECK(hipStreamSynchronize(c->stream));
     cad_wave_free(c);
     const size_t tab = (size_t)nz * (size_t)nt;
-    CAD_CHECK(hipMalloc((void **)&c->wv_tab, (nz + 2 * tab) * sizeof(double)));
+    if (!only) CAD_CHECK(hipMalloc((void **)&c->wv_tab, (nz + 2 * tab) * sizeof(double)));
     CAD_CHECK(hipMalloc((void **)&c->wv_err, 64));
     CAD_CHECK(hipHostMalloc((void **)&c->wv_err_host, 64, hipHostMallocDefault));
     c->wv_tt = (double *)std::malloc(nt * sizeof(double));
@@ -317,9 +344,12 @@ int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *t
         std::memcpy(c->wv_hh, hh, nt * sizeof(double));
     else
         std::memset(c->wv_hh, 0, nt * sizeof(double));
-    CAD_CHECK(hipMemcpy(c->wv_tab, zz, nz * sizeof(double), hipMemcpyHostToDevice));
-    CAD_CHECK(hipMemcpy(c->wv_tab + nz, un, tab * sizeof(double), hipMemcpyHostToDevice));
-    CAD_CHECK(hipMemcpy(c->wv_tab + nz + tab, ut, tab * sizeof(double), hipMemcpyHostToDevice));
+    c->wv_only = only;
+    if (!only) {
+        CAD_CHECK(hipMemcpy(c->wv_tab, zz, nz * sizeof(double), hipMemcpyHostToDevice));
+        CAD_CHECK(hipMemcpy(c->wv_tab + nz, un, tab * sizeof(double), hipMemcpyHostToDevice));
+        CAD_CHECK(hipMemcpy(c->wv_tab + nz + tab, ut, tab * sizeof(double), hipMemcpyHostToDevice));
+    }
     CAD_CHECK(hipMemset(c->wv_err, 0, 64));
     c->wv_err_host[0] = 0;
     c->wv_nz = nz;
@@ -360,7 +390,7 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
         cad_set_error("cad_set_wave: side %d is a radiation side (DEFINED DOUBLY.)", side);
         return CAD_ERR_ARG;
     }
-    if (!c->wv_tab) {
+    if (!c->wv_tt) {
         cad_set_error("cad_set_wave: set the table first (cad_set_wave_table)");
         return CAD_ERR_STATE;
     }
@@ -416,6 +446,9 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         const int rc = mtb_time(c, (double)r4tn, &wmt1, &wmt2, &wvz);
         if (rc) return rc;
         c->wv_level[q] = wvz * aw;
+        // a fixed level (LEVEL-ONLY) sets no velocity here: its faces are
+        // taken as free faces by the list pass (vf_bwuwn.f:326-327)
+        if (c->wv_only) continue;
         WaveLaunch w;
         w.xside = jd <= 2;
         w.pos = w.xside ? (jd == 1 ? 2 : d.ni) : (jd == 3 ? 2 : d.nj);
@@ -478,6 +511,51 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         hipLaunchKernelGGL(k_open_side, dim3((o.nq + 63) / 64), dim3(64), 0, c->stream, d, c->f.nf,
                            o.xside ? c->indx : c->indy, c->u2, c->v2, c->w2,
                            o.xside ? c->f.uu : c->f.vv, c->f.bcu, c->f.bcv, c->f.bcw, o);
+        CAD_CHECK(hipGetLastError());
+    }
+    return CAD_OK;
+}
+
+int cad_level_sides(const cad_ctx *c)
+{
+    int m = 0;
+    if (c->wv_only)
+        for (int q = 0; q < 4; q++)
+            if (c->wv_on[q] && c->wv_kind[q] == -3) m |= 1 << q;
+    return m;
+}
+
+int cad_launch_bwffsf(cad_ctx *c)
+{
+    const int lvo = cad_level_sides(c);
+    if (!lvo) return CAD_OK;
+    const Dims &d = c->d;
+    for (int jd = 1; jd <= 4; jd++) {
+        const int q = jd - 1;
+        if (!((lvo >> q) & 1)) continue;
+        // the level at TNOW, ramped (vf_bwffsf.f:58-82)
+        const double t = c->wv_t[q], t0 = c->wv_t0;
+        double a = c->wv_a[q];
+        float r4tn = (float)(t0 - c->prm.tnow / t);
+        r4tn = r4tn - (float)(int)r4tn;
+        if (r4tn < 0.0f) r4tn = r4tn + 1.0f;
+        double aw = 1.0;
+        if (a >= ZERO_REF) {
+            a = c->prm.tnow / t / a;
+            if (a < 1.0) aw = 0.5 * std::sin(PI_REF * (a - 0.5)) + 0.5;
+        }
+        double wmt1, wmt2, wvz;
+        const int rc = mtb_time(c, (double)r4tn, &wmt1, &wmt2, &wvz);
+        if (rc) return rc;
+        c->wv_level[q] = wvz * aw;
+        const double wl = c->wv_wvlvl + c->wv_level[q];
+        const int xside = jd <= 2;
+        const int cpos = xside ? (jd == 1 ? 2 : d.ni - 1) : (jd == 3 ? 2 : d.nj - 1);
+        const int q0 = c->wv_l1[q], nq = c->wv_l2[q] - c->wv_l1[q] + 1;
+        if (nq <= 0) continue;
+        const int nk = d.nk - 2;
+        hipLaunchKernelGGL(k_bwffsf, dim3((nq * nk + 63) / 64), dim3(64), 0, c->stream, d, c->zz,
+                           c->f.nf, c->f.ff, xside, cpos, q0, nq, wl);
         CAD_CHECK(hipGetLastError());
     }
     return CAD_OK;

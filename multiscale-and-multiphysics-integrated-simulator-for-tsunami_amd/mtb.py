@@ -6,9 +6,9 @@ LEVEL-OFF: profiles only; LEVEL-ONLY: level only), the number of z levels
 MTBZZ (not for LEVEL-ONLY), the number of phases MTBTT, the initial phase in
 [0,1], the z levels (mean water level at 0, increasing), then per phase
 'T', the phase (increasing in [0,1]), the level (not for LEVEL-OFF) and for
-every z level the normal and tangential velocity.  LEVEL-ON and LEVEL-OFF are
-produced (a LEVEL-OFF table's level is 0 and its profiles are not scaled by the
-measured level); LEVEL-ONLY changes the boundary treatment and is rejected.
+every z level the normal and tangential velocity.  A LEVEL-OFF table's level is
+0 and its profiles are not scaled by the measured level; a LEVEL-ONLY table
+holds the level alone and fixes the water level at its sides.
 """
 from dataclasses import dataclass
 
@@ -23,7 +23,7 @@ class MtbError(ValueError):
 
 @dataclass
 class Table:
-    kind: str          # LEVEL-ON or LEVEL-OFF
+    kind: str          # LEVEL-ON, LEVEL-OFF or LEVEL-ONLY
     t0: float          # DMTBT0
     zz: np.ndarray     # DMTBZZ (MTBZZ)
     tt: np.ndarray     # DMTBTT (MTBTT)
@@ -53,13 +53,13 @@ def parse(text) -> Table:
     kind = nxt()
     if kind not in ("LEVEL-ON", "LEVEL-OFF", "LEVEL-ONLY"):
         raise MtbError(f"VF_IM1INP: UNKNOWN WORD {kind}.")
-    if kind == "LEVEL-ONLY":
-        raise MtbError("LEVEL-ONLY tables are not implemented (LEVEL-ON and LEVEL-OFF are)")
-    nz = zstoi(nxt())
+    only = kind == "LEVEL-ONLY"
+    # a LEVEL-ONLY table has no z levels (vf_im1inp.f:71-80)
+    nz = 0 if only else zstoi(nxt())
     nt = zstoi(nxt())
-    if nz <= 0 or nt <= 0:
+    if (nz <= 0 and not only) or nt <= 0:
         raise MtbError("VF_IM1INP: INVALID VALUE.")
-    if nz < 2 or nt < 2:
+    if (nz < 2 and not only) or nt < 2:
         raise MtbError("a table needs at least two z levels and two phases")
     t0 = num()
     if not 0.0 <= t0 <= 1.0:

@@ -56,6 +56,18 @@ static void decode(const orc_state *s, int ijk, int *i, int *j, int *k)
     *i = r - ni * (*j - 1);
 }
 
+/* The boundary type a face takes in VF_BWUWN's list pass, VF_VPCOEF,
+ * VF_VMODIF and VF_BWUWT: a wave-maker face (5) of a side driven by a
+ * LEVEL-ONLY table (a fixed level) is treated there as a free face (4)
+ * (vf_bwuwn.f:161-211, vf_vpcoef.f:162-175, vf_vmodif.f:87-110,
+ * vf_bwuwt.f:80-116). */
+static int ib_lv(const orc_state *s, int l)
+{
+    const int ib = INDB(3, l), ns = INDB(2, l);
+    if (ib == 5 && s->mtb_only && ns >= 1 && ns <= 4 && s->wv_on[ns - 1]) return 4;
+    return ib;
+}
+
 /* ======================================================================
  * Phase (a): momentum predictor
  * ====================================================================== */
@@ -466,6 +478,8 @@ static int wave_bc(orc_state *s)
         int rc = mtb_time(s, wvt, &wmt1, &wmt2, &wvz);
         if (rc) return rc;
         s->wv_level[jd - 1] = wvz * aw;
+        /* a LEVEL-ONLY table sets no velocity here (vf_bwuwn.f:326-327) */
+        if (s->mtb_only) continue;
         const double sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
         const double sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
         const int xside = jd <= 2;
@@ -617,7 +631,7 @@ static int bwuwn(orc_state *s, const double *ub, const double *vb, const double 
     for (int l = 1; l <= s->numb; l++) {
         int i, j, k;
         decode(s, INDB(1, l), &i, &j, &k);
-        int ns = INDB(2, l), ib = INDB(3, l);
+        int ns = INDB(2, l), ib = ib_lv(s, l);
         size_t n = ID(i, j, k);
         if (ib == 1 || ib == 6 || ib == 8) {
             if (ns == 1 || ns == 2) { s->bcu[l - 1] = 0.0; s->uu[n] = 0.0; }
@@ -753,7 +767,7 @@ int orc_vpcoef(orc_state *s, orc_matrix *m)
                         aim = ayz * XX(5, i) * s->ggx[n] * gv;
                         m->ali[n] = -aim;
                     } else if (is >= 1) {
-                        if (INDB(3, is) == 4) aim = 2.0 * ayz * XX(4, i) * s->ggx[n] * ggv[n] / glv[n];
+                        if (ib_lv(s, is) == 4) aim = 2.0 * ayz * XX(4, i) * s->ggx[n] * ggv[n] / glv[n];
                         else aim = 0.0;
                         m->ali[n] = 0.0;
                     } else return -20;
@@ -764,7 +778,7 @@ int orc_vpcoef(orc_state *s, orc_matrix *m)
                         aip = ayz * XX(5, i + 1) * s->ggx[n + 1] * gv;
                         m->aui[n] = -aip;
                     } else if (is >= 1) {
-                        if (INDB(3, is) == 4) aip = 2.0 * ayz * XX(4, i) * s->ggx[n + 1] * ggv[n] / glv[n];
+                        if (ib_lv(s, is) == 4) aip = 2.0 * ayz * XX(4, i) * s->ggx[n + 1] * ggv[n] / glv[n];
                         else aip = 0.0;
                         m->aui[n] = 0.0;
                     } else return -20;
@@ -775,7 +789,7 @@ int orc_vpcoef(orc_state *s, orc_matrix *m)
                         ajm = axz * YY(5, j) * s->ggy[n] * gv;
                         m->alj[n] = -ajm;
                     } else if (is >= 1) {
-                        if (INDB(3, is) == 4) ajm = 2.0 * axz * YY(4, j) * s->ggy[n] * ggv[n] / glv[n];
+                        if (ib_lv(s, is) == 4) ajm = 2.0 * axz * YY(4, j) * s->ggy[n] * ggv[n] / glv[n];
                         else ajm = 0.0;
                         m->alj[n] = 0.0;
                     } else return -20;
@@ -786,7 +800,7 @@ int orc_vpcoef(orc_state *s, orc_matrix *m)
                         ajp = axz * YY(5, j + 1) * s->ggy[n + ni] * gv;
                         m->auj[n] = -ajp;
                     } else if (is >= 1) {
-                        if (INDB(3, is) == 4) ajp = 2.0 * axz * YY(4, j) * s->ggy[n + ni] * ggv[n] / glv[n];
+                        if (ib_lv(s, is) == 4) ajp = 2.0 * axz * YY(4, j) * s->ggy[n + ni] * ggv[n] / glv[n];
                         else ajp = 0.0;
                         m->auj[n] = 0.0;
                     } else return -20;
@@ -1020,7 +1034,7 @@ int orc_vmodif(orc_state *s, const double *pt)
                 if (is == 0) {
                     double gv = XX(6, i) * (XX(2, i) * ggv[n] / glv[n] + XX(2, i - 1) * ggv[n - 1] / glv[n - 1]);
                     s->uu[n] = s->uu[n] + XX(5, i) * gv * (pt[n] - pt[n - 1]);
-                } else if (is >= 1 && INDB(3, is) == 4) {
+                } else if (is >= 1 && ib_lv(s, is) == 4) {
                     if (INDB(2, is) == 1)
                         s->uu[n] = s->uu[n] + 2.0 * XX(4, i) * ggv[n] / glv[n] * pt[n];
                     else
@@ -1037,7 +1051,7 @@ int orc_vmodif(orc_state *s, const double *pt)
                 if (is == 0) {
                     double gv = YY(6, j) * (YY(2, j) * ggv[n] / glv[n] + YY(2, j - 1) * ggv[n - ni] / glv[n - ni]);
                     s->vv[n] = s->vv[n] + YY(5, j) * gv * (pt[n] - pt[n - ni]);
-                } else if (is >= 1 && INDB(3, is) == 4) {
+                } else if (is >= 1 && ib_lv(s, is) == 4) {
                     if (INDB(2, is) == 3)
                         s->vv[n] = s->vv[n] + 2.0 * YY(4, j) * ggv[n] / glv[n] * pt[n];
                     else
@@ -1193,15 +1207,14 @@ static void bsuwn(orc_state *s)
 
 /* Tangential wall velocities on slip/free boundaries.  vf_bwuwt.f:42-176
  * (types 1,4; 2,3 leave them; 7 radiation does nothing; 5, a wave-maker
- * face, sets them only for LEVEL-ONLY tables, vf_bwuwt.f:80-116, and the
- * tables here are LEVEL-ON). */
+ * face, sets them only for LEVEL-ONLY tables, vf_bwuwt.f:80-116, as 4). */
 static int bwuwt(orc_state *s)
 {
     DECL_DIMS(s);
     for (int l = 1; l <= s->numb; l++) {
         int i, j, k;
         decode(s, INDB(1, l), &i, &j, &k);
-        int ns = INDB(2, l), ib = INDB(3, l);
+        int ns = INDB(2, l), ib = ib_lv(s, l);
         size_t n = ID(i, j, k);
         const double *u = s->uu, *v = s->vv, *w = s->ww;
         if (ib == 1 || ib == 4) {
@@ -2031,6 +2044,49 @@ int orc_fdropf(orc_state *s)
     return 0;
 }
 
+/* VF_BWFFSF (vf_bwffsf.f:44-125): the cells next to a side driven by a
+ * LEVEL-ONLY table take F from the table's level at TNOW (ramped):
+ * 0 above it, 1 below it, the filled fraction in the cell it cuts. */
+static int bwffsf(orc_state *s)
+{
+    DECL_DIMS(s);
+    if (!s->mtb_only) return 0;
+    for (int jd = 1; jd <= 4; jd++) {
+        if (!s->wv_on[jd - 1]) continue;
+        const double t = s->wv_t[jd - 1], t0 = s->wv_t0[jd - 1];
+        double a = s->wv_a[jd - 1];
+        float r4tn = (float)(t0 - s->tnow / t);
+        r4tn = r4tn - (float)(int)r4tn;
+        if (r4tn < 0.0f) r4tn = r4tn + 1.0f;
+        double aw = 1.0;
+        if (a >= ZERO) {
+            a = s->tnow / t / a;
+            if (a < 1.0) aw = 0.5 * sin(PI_REF * (a - 0.5)) + 0.5;
+        }
+        double wmt1, wmt2, wvz;
+        int rc = mtb_time(s, (double)r4tn, &wmt1, &wmt2, &wvz);
+        if (rc) return rc;
+        s->wv_level[jd - 1] = wvz * aw;
+        const double wl = s->wvlvl + s->wv_level[jd - 1];
+        const int xside = jd <= 2;
+        const int cpos = xside ? (jd == 1 ? 2 : ni - 1) : (jd == 3 ? 2 : nj - 1);
+        for (int k = 2; k <= nk - 1; k++) {
+            double val;
+            if (ZZ(1, k) > wl)
+                val = 0.0;
+            else if (ZZ(1, k + 1) < wl)
+                val = 1.0;
+            else
+                val = (wl - ZZ(1, k)) / (ZZ(1, k + 1) - ZZ(1, k));
+            for (int q = s->wv_l1[jd - 1]; q <= s->wv_l2[jd - 1]; q++) {
+                const size_t c = xside ? ID(cpos, q, k) : ID(q, cpos, k);
+                if (s->nf[c] != -1) s->ff[c] = val;
+            }
+        }
+    }
+    return 0;
+}
+
 int orc_f1cal(orc_state *s)
 {
     DECL_DIMS(s);
@@ -2044,6 +2100,8 @@ int orc_f1cal(orc_state *s)
     fconv_axis(s, 2, flfw);
     feuler(s, flfu, flfv, flfw);
     free(w);
+    int rc0 = bwffsf(s);
+    if (rc0) return rc0;
     fmod1(s);
     fcut01(s);
     int rc = bwff(s);
@@ -2104,8 +2162,11 @@ int orc_init_state(orc_state *s, double wvlvl, double uini, double vini, double 
     DECL_DIMS(s);
     s->wvlvl = wvlvl;
     size_t nc = nij * (size_t)nk;
+    /* VF_BWFFSF before the clip (vf_cinit.f:177) */
+    int rc = bwffsf(s);
+    if (rc) return rc;
     fcut01(s);
-    int rc = bwff(s);
+    rc = bwff(s);
     if (rc) return rc;
     fnfini(s);
     rc = fnfprv(s);

@@ -85,8 +85,9 @@ typedef struct orc_state {
     int dm_on[4], dm_deg[4];
     double dm_par[4][4];
     double glxmin, glxmax, glymin, glymax;
-    /* the table is LEVEL-OFF (MTBTYP=2): no level column, profiles unscaled */
-    int mtb_off;
+    /* the table is LEVEL-OFF (MTBTYP=2): no level column, profiles unscaled;
+     * LEVEL-ONLY (MTBTYP=3): a fixed level, no profiles */
+    int mtb_off, mtb_only;
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */

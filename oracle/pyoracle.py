@@ -53,7 +53,7 @@ class OrcState(C.Structure):
         ("drgap", C.c_double * 10), ("drgbt", C.c_double * 10),
         ("dm_on", C.c_int * 4), ("dm_deg", C.c_int * 4), ("dm_par", (C.c_double * 4) * 4),
         ("glxmin", C.c_double), ("glxmax", C.c_double), ("glymin", C.c_double),
-        ("glymax", C.c_double), ("mtb_off", C.c_int),
+        ("glymax", C.c_double), ("mtb_off", C.c_int), ("mtb_only", C.c_int),
     ]
 
 
@@ -187,6 +187,7 @@ class Fields:
                     for k in ("zz", "tt", "hh", "un", "ut")}
         st.mtbzz, st.mtbtt, st.mtbnow = len(table.zz), len(table.tt), 0
         st.mtb_off = 1 if table.kind == "LEVEL-OFF" else 0
+        st.mtb_only = 1 if table.kind == "LEVEL-ONLY" else 0
         st.mtb_zz, st.mtb_tt, st.mtb_hh = (_pd(self.tab[k]) for k in ("zz", "tt", "hh"))
         st.mtb_un, st.mtb_ut = _pd(self.tab["un"]), _pd(self.tab["ut"])
 

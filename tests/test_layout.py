@@ -11,7 +11,7 @@ import fixtures as fx
 
 CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume",
          "porous_df", "wave_damp", "damp3d", "stokes_flume", "cnoidal_flume",
-         "oblique_basin", "wave_off"]
+         "oblique_basin", "wave_off", "wave_only"]
 STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
           "INDX", "INDY", "INDZ", "INDB", "BCVI")
 
@@ -262,7 +262,7 @@ def test_deck_special_side_words(pkg):
 def test_mtb_table_kinds(pkg):
     """<name>.mtb as VF_IM1INP reads it (Source code/vf_im1inp.f:61-175): a
     LEVEL-OFF table has no level after each phase (its level is 0); a
-    LEVEL-ONLY table is outside what is produced here; the kind word is
+    LEVEL-ONLY table has no z levels and no profiles; the kind word is
     checked."""
     on = pkg.configs.solitary_mtb(0.2, 0.04, 2.0, 0.2, nz=5, nt=4)
     off = pkg.configs.solitary_mtb(0.2, 0.04, 2.0, 0.2, nz=5, nt=4, kind="LEVEL-OFF")
@@ -270,7 +270,7 @@ def test_mtb_table_kinds(pkg):
     assert (a.kind, b.kind) == ("LEVEL-ON", "LEVEL-OFF")
     assert np.array_equal(a.un, b.un) and np.array_equal(a.tt, b.tt) and np.array_equal(a.zz, b.zz)
     assert np.any(a.hh != 0.0) and np.all(b.hh == 0.0)
-    with pytest.raises(pkg.mtb.MtbError):
-        pkg.mtb.parse(on.replace("LEVEL-ON", "LEVEL-ONLY", 1))
+    only = pkg.mtb.parse(pkg.configs.solitary_mtb(0.2, 0.04, 2.0, 0.2, nz=5, nt=4, kind="LEVEL-ONLY"))
+    assert only.kind == "LEVEL-ONLY" and len(only.zz) == 0 and np.array_equal(only.hh, a.hh)
     with pytest.raises(pkg.mtb.MtbError):
         pkg.mtb.parse(on.replace("LEVEL-ON", "LEVEL-X", 1))

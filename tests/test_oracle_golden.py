@@ -44,7 +44,7 @@ def test_init_state_matches_reference(pkg, case):
     assert np.array_equal(F.inds, ref["INDS"][: int(ref["NUMS"][0])])
 
 
-@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off"])
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only"])
 def test_init_state_with_special_sides(pkg, case):
     """VF_CINIT's boundary pass with a table wave-maker (at TNOW = 0) and a
     radiation side (the initial pass, ALN = ALT = 1) == the reference's
@@ -311,7 +311,7 @@ def _wave_fields(pkg, case, rec, bcf, damps=True):
     return st, p, F
 
 
-@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off"])
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only"])
 def test_wave_maker_phases_bitexact(pkg, case):
     """MODEL WAVE-BC X- FUNC MATRIX, and in wave_open a radiation boundary
     (MODEL OPEN-BC X+ FUNC TYPE1) at the far end: V1CAL (the wave-maker and
@@ -331,12 +331,13 @@ def test_wave_maker_phases_bitexact(pkg, case):
         for k in ("UU", "VV", "WW", "PP", "BCU", "BCV", "BCW", "BCP"):
             assert np.array_equal(F[k], r["v1_out"][k]), (s, k)
         st, p, F = _wave_fields(pkg, case, r["f1_in"], None)
+        F.st.tnow = float(r["f1_in"]["TNOW"][0])   # VF_BWFFSF's level (LEVEL-ONLY)
         assert F.f1cal() == 0
         for k in ("FF", "NF", "BCF", "UU"):
             assert np.array_equal(F[k], r["f1_out"][k]), (s, k)
 
 
-@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off"])
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only"])
 def test_wave_maker_free_run_bitexact(pkg, case):
     """The wave flume (400 steps) and the flume with the radiation end (1000
     steps) from the reference's initial state: ITR and ||div|| every step and
