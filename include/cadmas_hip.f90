@@ -187,6 +187,26 @@ module cadmas_hip
       integer(c_int) :: cad_set_wave
     end function
 
+    ! the second table, <name>.mtb2 (VF_IM2INP), and a side of FUNC MATRIX2
+    function cad_set_wave_table2(ctx, nz, nt, zz, tt, hh, un, ut, t0) &
+        bind(c, name='cad_set_wave_table2')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nz, nt
+      real(c_double), intent(in) :: zz(*), tt(*), hh(*), un(*), ut(*)
+      real(c_double), value :: t0
+      integer(c_int) :: cad_set_wave_table2
+    end function
+
+    function cad_set_wave2(ctx, side, l1, l2, depth, period, ampl, wvlvl) &
+        bind(c, name='cad_set_wave2')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: side, l1, l2
+      real(c_double), value :: depth, period, ampl, wvlvl
+      integer(c_int) :: cad_set_wave2
+    end function
+
     ! MODEL WAVE-BC side (1..4) FUNC STOKES (-2), CNOIDAL (-1) or STK-CND (0)
     function cad_set_wave_theory(ctx, side, func, l1, l2, depth, height, period, ampl, wvlvl, &
         angle, ax, ay) bind(c, name='cad_set_wave_theory')

@@ -140,6 +140,13 @@ int cad_set_wave_table(cad_ctx *ctx, int nz, int nt, const double *zz, const dou
  * sub-loop passes) sets their velocities at the context's TNOW. */
 int cad_set_wave(cad_ctx *ctx, int side, int l1, int l2, double depth, double period,
                  double ampl, double wvlvl);
+/* The second table, FUNC MATRIX2 with <name>.mtb2 (VF_IM2INP, vf_im2inp.f;
+ * VF_CWMTB12/22): the same as cad_set_wave_table / cad_set_wave for a table of
+ * its own, with its own kind, search start (MTBNOW2) and phase origin. */
+int cad_set_wave_table2(cad_ctx *ctx, int nz, int nt, const double *zz, const double *tt,
+                        const double *hh, const double *un, const double *ut, double t0);
+int cad_set_wave2(cad_ctx *ctx, int side, int l1, int l2, double depth, double period,
+                  double ampl, double wvlvl);
 /* A theory wave-maker side, MODEL WAVE-BC <side> FUNC STOKES / CNOIDAL / STK-CND
  * (vf_iimdl.f:51-70; VF_CWMAK0/1/2, vf_wstk0.f, vf_wcnd0.f): func -2 fifth-order
  * Stokes, -1 third-order cnoidal, 0 one of the two by the Ursell number

@@ -80,7 +80,9 @@ def lib():
     L.cad_cdtcal.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, PD]
     L.cad_download_tdoor.argtypes = [C.c_void_p] + [PD] * 7
     L.cad_set_wave_table.argtypes = [C.c_void_p, C.c_int, C.c_int] + [PD] * 5 + [C.c_double]
+    L.cad_set_wave_table2.argtypes = [C.c_void_p, C.c_int, C.c_int] + [PD] * 5 + [C.c_double]
     L.cad_set_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
+    L.cad_set_wave2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
     L.cad_get_wave_level.argtypes = [C.c_void_p, C.c_int, PD]
     L.cad_set_open.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     L.cad_set_wave_theory.argtypes = [C.c_void_p] + [C.c_int] * 4 + [C.c_double] * 8
@@ -121,7 +123,8 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
            "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
-           "cad_set_drag_df", "cad_set_damp", "cad_set_wave_theory", "cad_wave_theory_eval")
+           "cad_set_drag_df", "cad_set_damp", "cad_set_wave_theory", "cad_wave_theory_eval",
+           "cad_set_wave_table2", "cad_set_wave2")
 
 class TheoryWave:
     """A theory wave evaluated on the host (cad_wave_theory_eval), with
@@ -251,31 +254,33 @@ class Context:
         _check(lib().cad_download_tdoor(self._h, _pd(out["TBUB"]), *[_pd(out[k]) for k in DROP]))
         return out
 
-    def set_waves(self, waves, table, wvlvl):
+    def set_waves(self, waves, table, wvlvl, table2=None):
         """Wave-makers: waves = deck.waves (side -> func, depth, height,
-        period, ampl, area); table = mtb.Table (LEVEL-ON) for the sides with
-        FUNC MATRIX, None when there are none.  Sides are set in side order,
-        as the reference's set-up visits them."""
-        if any(wv["func"] == "MATRIX" for wv in waves.values()):
-            zz, tt, hh = (_f64(getattr(table, k)) for k in ("zz", "tt", "hh"))
+        period, ampl, area); table = mtb.Table for the sides with FUNC MATRIX
+        and table2 for those with MATRIX2 (None when there are none).  Sides
+        are set in side order, as the reference's set-up visits them."""
+        for tab, fn, which in ((table, "MATRIX", 0), (table2, "MATRIX2", 1)):
+            if not any(wv["func"] == fn for wv in waves.values()):
+                continue
+            zz, tt, hh = (_f64(getattr(tab, k)) for k in ("zz", "tt", "hh"))
             nz, nt = len(zz), len(tt)
-            un, ut = _f64(table.un, (nz, nt)), _f64(table.ut, (nz, nt))
+            un, ut = _f64(tab.un, (nz, nt)), _f64(tab.ut, (nz, nt))
             # a LEVEL-OFF table goes without its level (a null level array), a
             # LEVEL-ONLY table without depths and profiles (nz = 0)
-            hp = None if table.kind == "LEVEL-OFF" else _pd(hh)
-            if table.kind == "LEVEL-ONLY":
-                _check(lib().cad_set_wave_table(self._h, 0, nt, None, _pd(tt), hp, None, None,
-                                                float(table.t0)))
+            hp = None if tab.kind == "LEVEL-OFF" else _pd(hh)
+            f = lib().cad_set_wave_table2 if which else lib().cad_set_wave_table
+            if tab.kind == "LEVEL-ONLY":
+                _check(f(self._h, 0, nt, None, _pd(tt), hp, None, None, float(tab.t0)))
             else:
-                _check(lib().cad_set_wave_table(self._h, nz, nt, _pd(zz), _pd(tt), hp, _pd(un),
-                                                _pd(ut), float(table.t0)))
+                _check(f(self._h, nz, nt, _pd(zz), _pd(tt), hp, _pd(un), _pd(ut), float(tab.t0)))
         ni, nj, _ = self.shape
         for side, wv in sorted(waves.items()):
             lim = nj if side <= 2 else ni
             l1, l2 = wv["area"] if wv["area"] else (2, lim - 1)
-            if wv["func"] == "MATRIX":
-                _check(lib().cad_set_wave(self._h, int(side), int(l1), int(l2), float(wv["depth"]),
-                                          float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
+            if wv["func"] in ("MATRIX", "MATRIX2"):
+                f = lib().cad_set_wave2 if wv["func"] == "MATRIX2" else lib().cad_set_wave
+                _check(f(self._h, int(side), int(l1), int(l2), float(wv["depth"]),
+                         float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
             else:
                 _check(lib().cad_set_wave_theory(
                     self._h, int(side), WAVE_FUNC[wv["func"]], int(l1), int(l2), float(wv["depth"]),

@@ -389,6 +389,20 @@ def wave_flume(ni=60, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400,
     return case
 
 
+def wave_two(ni=40, nj=4, nk=20, d=0.02, h=0.2, H=0.04, period=2.0, steps=400, dt=1.0e-3,
+             eps_r=1.0e-10, name=None):
+    """The flume with a wave-maker at each end from two tables: X- FUNC MATRIX
+    (<name>.mtb, the solitary wave's LEVEL-ON table) and X+ FUNC MATRIX2
+    (<name>.mtb2, a LEVEL-ONLY table holding the far end at a smaller
+    solitary rise later in the period; Source code/vf_im2inp.f)."""
+    c = wave_flume(ni=ni, nj=nj, nk=nk, d=d, h=h, H=H, period=period, steps=steps, dt=dt,
+                   eps_r=eps_r, name=name or f"wave_two_{ni}")
+    c.deck += (f"MODEL WAVE-BC X+ FUNC MATRIX2\nMODEL WAVE-BC X+ DEPTH {h}\n"
+               f"MODEL WAVE-BC X+ PERIOD {period}\n")
+    c.meta["files"]["mtb2"] = solitary_mtb(h, H * 0.5, period, t_crest=0.6, kind="LEVEL-ONLY")
+    return c
+
+
 def theory_flume(func="STOKES", ni=40, nj=4, nk=20, d=0.02, h=0.2, H=0.02, period=1.0, ampl=1.0,
                  steps=600, dt=1.0e-3, eps_r=1.0e-10, angle=None, name=None):
     """A flume driven by a theory wave on its x- side (MODEL WAVE-BC X- FUNC
@@ -453,6 +467,7 @@ def named(name):
         "wave_damp": lambda: wave_damp(),
         "wave_off": lambda: wave_flume(ni=40, steps=400, table="LEVEL-OFF"),
         "wave_only": lambda: wave_flume(ni=40, steps=400, table="LEVEL-ONLY"),
+        "wave_two": lambda: wave_two(),
         "stokes_flume": lambda: theory_flume("STOKES"),
         "cnoidal_flume": lambda: theory_flume("STK-CND", H=0.03, period=2.0),
         "oblique_basin": lambda: theory_flume("STOKES", ni=30, nj=16, nk=16, h=0.16, period=0.8,

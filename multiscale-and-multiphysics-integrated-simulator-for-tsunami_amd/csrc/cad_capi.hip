@@ -665,14 +665,28 @@ int cad_set_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const doubl
                        const double *hh, const double *un, const double *ut, double t0)
 {
     if (!c) return CAD_ERR_ARG;
-    return cad_wave_table(c, nz, nt, zz, tt, hh, un, ut, t0);
+    return cad_wave_table(c, 0, nz, nt, zz, tt, hh, un, ut, t0);
+}
+
+int cad_set_wave_table2(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
+                        const double *hh, const double *un, const double *ut, double t0)
+{
+    if (!c) return CAD_ERR_ARG;
+    return cad_wave_table(c, 1, nz, nt, zz, tt, hh, un, ut, t0);
 }
 
 int cad_set_wave(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
                  double wvlvl)
 {
     if (!c) return CAD_ERR_ARG;
-    return cad_wave_side(c, side, l1, l2, depth, period, ampl, wvlvl);
+    return cad_wave_side(c, 0, side, l1, l2, depth, period, ampl, wvlvl);
+}
+
+int cad_set_wave2(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
+                  double wvlvl)
+{
+    if (!c) return CAD_ERR_ARG;
+    return cad_wave_side(c, 1, side, l1, l2, depth, period, ampl, wvlvl);
 }
 
 int cad_set_wave_theory(cad_ctx *c, int side, int func, int l1, int l2, double depth,

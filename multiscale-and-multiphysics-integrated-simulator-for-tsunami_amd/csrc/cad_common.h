@@ -198,19 +198,24 @@ struct cad_ctx {
     // matrix-table wave-maker (cad_wave.hip): the table's heights and
     // velocities on the device (ZZ, UN, UT), its times and levels on the
     // host, the time search start MTBNOW, the sides and an error flag
-    double *wv_tab, *wv_tt, *wv_hh;
-    int wv_nz, wv_nt, wv_mtbnow;
-    double wv_t0, wv_wvlvl;
+    // the wave-maker tables: FUNC MATRIX (<name>.mtb) and MATRIX2
+    // (<name>.mtb2), each with its own kind, search start and phase origin
+    struct WaveTable {
+        double *tab;        // device: ZZ(NZ) | UN(NZ,NT) | UT(NZ,NT)
+        double *tt, *hh;    // host: times and levels
+        int nz, nt, mtbnow; // MTBZZ, MTBTT, MTBNOW
+        double t0;          // DMTBT0
+        int off, only;      // LEVEL-OFF, LEVEL-ONLY
+    } wt[2];
+    double wv_wvlvl;
     int wv_on[4], wv_l1[4], wv_l2[4];
     double wv_d[4], wv_t[4], wv_a[4], wv_level[4];
     int *wv_err, *wv_err_host;
     int n_wave_faces;   // boundary faces of type 5 in INDB
     // the theory sides (cad_wave_theory.hip): per side the function
-    // (-3 the table, -2 Stokes, -1 cnoidal), height, phase origin and the
+    // (-3 / -4 the first / second table, -2 Stokes, -1 cnoidal), height, phase origin and the
     // persistent secant step of VF_STK0; per side the columns' level scales
     // and face velocities (device and pinned host)
-    int wv_off;         // the table is LEVEL-OFF
-    int wv_only;        // the table is LEVEL-ONLY (a fixed level, no profiles)
     int wv_kind[4];
     double wv_h[4], wv_t0s[4];
     double wv_wl[4];        // wave length of VF_CWMAK0 at set-up (BCTYP(4,side))
@@ -271,10 +276,10 @@ int cad_tdoor_alloc(cad_ctx *c);
 void cad_tdoor_free(cad_ctx *c);
 int cad_tdoor_init(cad_ctx *c);
 int cad_tdoor_phase(cad_ctx *c);
-int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
+int cad_wave_table(cad_ctx *c, int which, int nz, int nt, const double *zz, const double *tt,
                    const double *hh, const double *un, const double *ut, double t0);
-int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
-                  double wvlvl);
+int cad_wave_side(cad_ctx *c, int which, int side, int l1, int l2, double depth, double period,
+                  double ampl, double wvlvl);
 int cad_wave_apply(cad_ctx *c, bool initial);
 int cad_open_side(cad_ctx *c, int side, double depth, double period);
 int cad_wave_check(cad_ctx *c);

@@ -239,32 +239,33 @@ double comak0(double dep, double t, double gg)
     return 2.0 * PI_REF * dep / x;
 }
 
-// VF_CWMTB1 (vf_cwmtb1.f): the time weights of the table at phase wvt and the
-// table level there; searches down from MTBNOW, then down from the top.
-int mtb_time(cad_ctx *c, double wvt, double *w1o, double *w2o, double *wvz)
+// VF_CWMTB1 (vf_cwmtb1.f; VF_CWMTB12 for the second table): the time weights
+// of table T at phase wvt and the table level there; searches down from its
+// MTBNOW, then down from the top.
+int mtb_time(cad_ctx::WaveTable &T, double wvt, double *w1o, double *w2o, double *wvz)
 {
-    const int nt = c->wv_nt;
-    const double *tt = c->wv_tt - 1;  // 1-based
+    const int nt = T.nt;
+    const double *tt = T.tt - 1;  // 1-based
     double w1 = 0.0, w2 = 0.0;
-    if (c->wv_mtbnow <= 0) c->wv_mtbnow = nt - 1;
+    if (T.mtbnow <= 0) T.mtbnow = nt - 1;
     if (wvt <= tt[1]) {
-        c->wv_mtbnow = 1;
+        T.mtbnow = 1;
         w1 = 1.0;
         w2 = 0.0;
     } else if (wvt >= tt[nt]) {
-        c->wv_mtbnow = nt - 1;
+        T.mtbnow = nt - 1;
         w1 = 0.0;
         w2 = 1.0;
     } else {
         int ip = 0;
-        for (int i = c->wv_mtbnow; i >= 1 && !ip; i--)
+        for (int i = T.mtbnow; i >= 1 && !ip; i--)
             if (tt[i] <= wvt && wvt <= tt[i + 1]) {
                 ip = i;
                 w2 = 1.0 / (tt[i + 1] - tt[i]);
                 w1 = (tt[i + 1] - wvt) * w2;
                 w2 = (wvt - tt[i]) * w2;
             }
-        for (int i = nt - 1; i >= c->wv_mtbnow + 1 && !ip; i--)
+        for (int i = nt - 1; i >= T.mtbnow + 1 && !ip; i--)
             if (tt[i] <= wvt && wvt <= tt[i + 1]) {
                 ip = i;
                 w2 = 1.0 / (tt[i + 1] - tt[i]);
@@ -275,37 +276,48 @@ int mtb_time(cad_ctx *c, double wvt, double *w1o, double *w2o, double *wvz)
             cad_set_error("VF_CWMTB1: P.G ERROR. (phase %.17g outside the table's times)", wvt);
             return CAD_ERR_WAVE;
         }
-        c->wv_mtbnow = ip;
+        T.mtbnow = ip;
     }
-    *wvz = w1 * c->wv_hh[c->wv_mtbnow - 1] + w2 * c->wv_hh[c->wv_mtbnow];
+    *wvz = w1 * T.hh[T.mtbnow - 1] + w2 * T.hh[T.mtbnow];
     *w1o = w1;
     *w2o = w2;
     return CAD_OK;
+}
+
+// the table a wave-maker side reads (-3: the first, -4: the second)
+inline int table_of(const cad_ctx *c, int q) { return c->wv_kind[q] == -4 ? 1 : 0; }
+
+void table_free(cad_ctx *c, int which)
+{
+    cad_ctx::WaveTable &T = c->wt[which];
+    if (T.tab) hipFree(T.tab);
+    std::free(T.tt);
+    std::free(T.hh);
+    T = cad_ctx::WaveTable{};
+    // the table's sides refer to it: a new table needs them set again
+    for (int q = 0; q < 4; q++)
+        if (c->wv_kind[q] == -3 - which) c->wv_on[q] = 0;
 }
 
 }  // namespace
 
 void cad_wave_free(cad_ctx *c)
 {
-    if (c->wv_tab) hipFree(c->wv_tab);
+    table_free(c, 0);
+    table_free(c, 1);
     if (c->wv_err) hipFree(c->wv_err);
     if (c->wv_err_host) hipHostFree(c->wv_err_host);
-    std::free(c->wv_tt);
-    std::free(c->wv_hh);
-    c->wv_tab = nullptr;
     c->wv_err = nullptr;
     c->wv_err_host = nullptr;
-    c->wv_tt = c->wv_hh = nullptr;
-    c->wv_nz = c->wv_nt = 0;
-    c->wv_off = c->wv_only = 0;
-    // the table's sides refer to it: a new table needs them set again
-    for (int q = 0; q < 4; q++)
-        if (c->wv_kind[q] == -3) c->wv_on[q] = 0;
 }
 
-int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *tt,
+int cad_wave_table(cad_ctx *c, int which, int nz, int nt, const double *zz, const double *tt,
                    const double *hh, const double *un, const double *ut, double t0)
 {
+    if (which != 0 && which != 1) {
+        cad_set_error("cad_set_wave_table: table 0 (MATRIX) or 1 (MATRIX2)");
+        return CAD_ERR_ARG;
+    }
     // nz = 0: a LEVEL-ONLY table (times and levels only, vf_im1inp.f:71-80)
     const bool only = nz == 0;
     if ((!only && nz < 2) || nt < 2 || !tt || (!only && (!zz || !un || !ut)) || (only && !hh)) {
@@ -325,37 +337,40 @@ int cad_wave_table(cad_ctx *c, int nz, int nt, const double *zz, const double *t
         }
     CAD_CHECK(hipSetDevice(c->device));
     CAD_CHECK(hipStreamSynchronize(c->stream));
-    cad_wave_free(c);
+    table_free(c, which);
+    cad_ctx::WaveTable &T = c->wt[which];
     const size_t tab = (size_t)nz * (size_t)nt;
-    if (!only) CAD_CHECK(hipMalloc((void **)&c->wv_tab, (nz + 2 * tab) * sizeof(double)));
-    CAD_CHECK(hipMalloc((void **)&c->wv_err, 64));
-    CAD_CHECK(hipHostMalloc((void **)&c->wv_err_host, 64, hipHostMallocDefault));
-    c->wv_tt = (double *)std::malloc(nt * sizeof(double));
-    c->wv_hh = (double *)std::malloc(nt * sizeof(double));
-    if (!c->wv_tt || !c->wv_hh) {
+    if (!only) CAD_CHECK(hipMalloc((void **)&T.tab, (nz + 2 * tab) * sizeof(double)));
+    if (!c->wv_err) {
+        CAD_CHECK(hipMalloc((void **)&c->wv_err, 64));
+        CAD_CHECK(hipHostMalloc((void **)&c->wv_err_host, 64, hipHostMallocDefault));
+    }
+    T.tt = (double *)std::malloc(nt * sizeof(double));
+    T.hh = (double *)std::malloc(nt * sizeof(double));
+    if (!T.tt || !T.hh) {
         cad_set_error("cad_set_wave_table: out of host memory");
         return CAD_ERR_HIP;
     }
-    std::memcpy(c->wv_tt, tt, nt * sizeof(double));
+    std::memcpy(T.tt, tt, nt * sizeof(double));
     // LEVEL-OFF (no level column): the level is 0 and the profiles are not
     // scaled (vf_im1inp.f:148-155, vf_bwuwn.f:326-329)
-    c->wv_off = hh == nullptr;
+    T.off = hh == nullptr;
     if (hh)
-        std::memcpy(c->wv_hh, hh, nt * sizeof(double));
+        std::memcpy(T.hh, hh, nt * sizeof(double));
     else
-        std::memset(c->wv_hh, 0, nt * sizeof(double));
-    c->wv_only = only;
+        std::memset(T.hh, 0, nt * sizeof(double));
+    T.only = only;
     if (!only) {
-        CAD_CHECK(hipMemcpy(c->wv_tab, zz, nz * sizeof(double), hipMemcpyHostToDevice));
-        CAD_CHECK(hipMemcpy(c->wv_tab + nz, un, tab * sizeof(double), hipMemcpyHostToDevice));
-        CAD_CHECK(hipMemcpy(c->wv_tab + nz + tab, ut, tab * sizeof(double), hipMemcpyHostToDevice));
+        CAD_CHECK(hipMemcpy(T.tab, zz, nz * sizeof(double), hipMemcpyHostToDevice));
+        CAD_CHECK(hipMemcpy(T.tab + nz, un, tab * sizeof(double), hipMemcpyHostToDevice));
+        CAD_CHECK(hipMemcpy(T.tab + nz + tab, ut, tab * sizeof(double), hipMemcpyHostToDevice));
     }
     CAD_CHECK(hipMemset(c->wv_err, 0, 64));
     c->wv_err_host[0] = 0;
-    c->wv_nz = nz;
-    c->wv_nt = nt;
-    c->wv_t0 = t0;
-    c->wv_mtbnow = 0;
+    T.nz = nz;
+    T.nt = nt;
+    T.t0 = t0;
+    T.mtbnow = 0;
     return CAD_OK;
 }
 
@@ -379,9 +394,13 @@ int cad_open_side(cad_ctx *c, int side, double depth, double period)
     return CAD_OK;
 }
 
-int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double period, double ampl,
-                  double wvlvl)
+int cad_wave_side(cad_ctx *c, int which, int side, int l1, int l2, double depth, double period,
+                  double ampl, double wvlvl)
 {
+    if (which != 0 && which != 1) {
+        cad_set_error("cad_set_wave: table 0 (MATRIX) or 1 (MATRIX2)");
+        return CAD_ERR_ARG;
+    }
     if (side < 1 || side > 4) {
         cad_set_error("cad_set_wave: side %d (1..4 = X-, X+, Y-, Y+)", side);
         return CAD_ERR_ARG;
@@ -390,7 +409,7 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
         cad_set_error("cad_set_wave: side %d is a radiation side (DEFINED DOUBLY.)", side);
         return CAD_ERR_ARG;
     }
-    if (!c->wv_tt) {
+    if (!c->wt[which].tt) {
         cad_set_error("cad_set_wave: set the table first (cad_set_wave_table)");
         return CAD_ERR_STATE;
     }
@@ -405,7 +424,7 @@ int cad_wave_side(cad_ctx *c, int side, int l1, int l2, double depth, double per
     }
     const int q = side - 1;
     c->wv_on[q] = 1;
-    c->wv_kind[q] = -3;
+    c->wv_kind[q] = -3 - which;
     c->wv_l1[q] = l1;
     c->wv_l2[q] = l2;
     c->wv_d[q] = depth;
@@ -424,13 +443,14 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         const int q = jd - 1;
         if (!c->wv_on[q]) continue;
         any = true;
-        if (c->wv_kind[q] != -3) {
+        if (c->wv_kind[q] == -2 || c->wv_kind[q] == -1) {
             // a theory side: STOKES or CNOIDAL (cad_wave_theory.hip)
             const int rc = cad_wave_theory_apply(c, jd);
             if (rc) return rc;
             continue;
         }
-        const double dep = c->wv_d[q], t = c->wv_t[q], t0 = c->wv_t0;
+        cad_ctx::WaveTable &T = c->wt[table_of(c, q)];
+        const double dep = c->wv_d[q], t = c->wv_t[q], t0 = T.t0;
         double a = c->wv_a[q];
         // the table's phase in single precision (vf_bwuwn.f:270-276)
         float r4tn = (float)(t0 - c->prm.tnow / t);
@@ -443,21 +463,21 @@ int cad_wave_apply(cad_ctx *c, bool initial)
             if (a < 1.0) aw = 0.5 * std::sin(PI_REF * (a - 0.5)) + 0.5;
         }
         double wmt1, wmt2, wvz;
-        const int rc = mtb_time(c, (double)r4tn, &wmt1, &wmt2, &wvz);
+        const int rc = mtb_time(T, (double)r4tn, &wmt1, &wmt2, &wvz);
         if (rc) return rc;
         c->wv_level[q] = wvz * aw;
         // a fixed level (LEVEL-ONLY) sets no velocity here: its faces are
         // taken as free faces by the list pass (vf_bwuwn.f:326-327)
-        if (c->wv_only) continue;
+        if (T.only) continue;
         WaveLaunch w;
         w.xside = jd <= 2;
         w.pos = w.xside ? (jd == 1 ? 2 : d.ni) : (jd == 3 ? 2 : d.nj);
         w.cpos = w.xside ? (jd == 1 ? 2 : d.ni - 1) : (jd == 3 ? 2 : d.nj - 1);
         w.q0 = c->wv_l1[q];
         w.nq = c->wv_l2[q] - c->wv_l1[q] + 1;
-        w.nz = c->wv_nz;
-        w.im = c->wv_mtbnow;
-        w.off = c->wv_off;
+        w.nz = T.nz;
+        w.im = T.mtbnow;
+        w.off = T.off;
         w.d = dep;
         w.wvz = wvz;
         w.aw = aw;
@@ -466,8 +486,8 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         w.sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
         w.sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
         w.wvlvl = c->wv_wvlvl;
-        const size_t tab = (size_t)c->wv_nz * (size_t)c->wv_nt;
-        const double *tz = c->wv_tab, *tun = c->wv_tab + c->wv_nz, *tut = c->wv_tab + c->wv_nz + tab;
+        const size_t tab = (size_t)T.nz * (size_t)T.nt;
+        const double *tz = T.tab, *tun = T.tab + T.nz, *tut = T.tab + T.nz + tab;
         hipLaunchKernelGGL(k_wave_maker, dim3((w.nq + 63) / 64), dim3(64), 0, c->stream, d, c->zz,
                            c->f.nf, c->f.ff, w.xside ? c->indx : c->indy, tz, tun, tut,
                            w.xside ? c->f.uu : c->f.vv, c->f.bcu, c->f.bcv, c->f.bcw, w, c->wv_err);
@@ -519,9 +539,9 @@ int cad_wave_apply(cad_ctx *c, bool initial)
 int cad_level_sides(const cad_ctx *c)
 {
     int m = 0;
-    if (c->wv_only)
-        for (int q = 0; q < 4; q++)
-            if (c->wv_on[q] && c->wv_kind[q] == -3) m |= 1 << q;
+    for (int q = 0; q < 4; q++)
+        if (c->wv_on[q] && (c->wv_kind[q] == -3 || c->wv_kind[q] == -4) && c->wt[table_of(c, q)].only)
+            m |= 1 << q;
     return m;
 }
 
@@ -534,7 +554,8 @@ int cad_launch_bwffsf(cad_ctx *c)
         const int q = jd - 1;
         if (!((lvo >> q) & 1)) continue;
         // the level at TNOW, ramped (vf_bwffsf.f:58-82)
-        const double t = c->wv_t[q], t0 = c->wv_t0;
+        cad_ctx::WaveTable &T = c->wt[table_of(c, q)];
+        const double t = c->wv_t[q], t0 = T.t0;
         double a = c->wv_a[q];
         float r4tn = (float)(t0 - c->prm.tnow / t);
         r4tn = r4tn - (float)(int)r4tn;
@@ -545,7 +566,7 @@ int cad_launch_bwffsf(cad_ctx *c)
             if (a < 1.0) aw = 0.5 * std::sin(PI_REF * (a - 0.5)) + 0.5;
         }
         double wmt1, wmt2, wvz;
-        const int rc = mtb_time(c, (double)r4tn, &wmt1, &wmt2, &wvz);
+        const int rc = mtb_time(T, (double)r4tn, &wmt1, &wmt2, &wvz);
         if (rc) return rc;
         c->wv_level[q] = wvz * aw;
         const double wl = c->wv_wvlvl + c->wv_level[q];

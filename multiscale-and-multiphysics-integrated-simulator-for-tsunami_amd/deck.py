@@ -398,10 +398,10 @@ def _parse_model(d, w):
     if key == "FUNC":
         if len(w) < 5:
             raise DeckError("MODEL WAVE-BC FUNC: SYNTAX ERROR.")
-        if w[4] not in ("MATRIX", "STOKES", "CNOIDAL", "STK-CND"):
-            if w[4] in ("STREAM", "MATRIX2"):
-                raise UnsupportedDeck(f"MODEL WAVE-BC FUNC {w[4]}: the stream-function wave and the "
-                                      "second table are not implemented")
+        if w[4] not in ("MATRIX", "MATRIX2", "STOKES", "CNOIDAL", "STK-CND"):
+            if w[4] == "STREAM":
+                raise UnsupportedDeck("MODEL WAVE-BC FUNC STREAM: the stream-function wave is not "
+                                      "implemented")
             raise DeckError(f"MODEL WAVE-BC FUNC {w[4]}: UNKNOWN WORD.")
         wv["func"] = w[4]
     elif key in ("DEPTH", "HEIGHT", "PERIOD", "AMPL"):
@@ -622,11 +622,11 @@ def _validate(d):
             raise DeckError("MODEL WAVE-BC: FUNC missing")
         if wv["depth"] < 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC DEPTH) NOT FOUND.")
-        if wv["func"] != "MATRIX" and wv["height"] < 1.0e-20:
+        if wv["func"] not in ("MATRIX", "MATRIX2") and wv["height"] < 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC HEIGHT) NOT FOUND.")
         if wv["period"] < 1.0e-20:
             raise DeckError("MODEL WAVE-BC: (WAVE-BC PERIOD) NOT FOUND.")
-        if abs(wv["angle"]) > 1.0e-20 and wv["func"] == "MATRIX":
+        if abs(wv["angle"]) > 1.0e-20 and wv["func"] in ("MATRIX", "MATRIX2"):
             raise DeckError("MODEL WAVE-BC: (WAVE-BC ANGLE) CAN NOT USE.")
     # vf_csetup.f:221-234: a damping side needs DEGREE, WIDTH and DEPTH
     for jd, dm in d.damps.items():

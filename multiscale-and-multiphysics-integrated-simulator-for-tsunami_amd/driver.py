@@ -69,10 +69,12 @@ def run(deck_text, name, workdir=".", nmax=None, device=0, log=None, precision=N
         static = lay.static_arrays()
         ctx.upload_static(static)
         if d.waves:
-            table = None
-            if any(wv["func"] == "MATRIX" for wv in d.waves.values()):
-                table = mtbmod.parse((workdir / f"{name}.mtb").read_text())
-            ctx.set_waves(d.waves, table, d.wvlvl)
+            # the tables of FUNC MATRIX and MATRIX2: <name>.mtb and <name>.mtb2
+            tables = [None, None]
+            for i, (fn, ext) in enumerate((("MATRIX", "mtb"), ("MATRIX2", "mtb2"))):
+                if any(wv["func"] == fn for wv in d.waves.values()):
+                    tables[i] = mtbmod.parse((workdir / f"{name}.{ext}").read_text())
+            ctx.set_waves(d.waves, tables[0], d.wvlvl, tables[1])
         ctx.set_opens(d.opens)
         ctx.set_drag_df(d.drag_df)
         ctx.set_damps(d.damps)

@@ -64,7 +64,9 @@ static void decode(const orc_state *s, int ijk, int *i, int *j, int *k)
 static int ib_lv(const orc_state *s, int l)
 {
     const int ib = INDB(3, l), ns = INDB(2, l);
-    if (ib == 5 && s->mtb_only && ns >= 1 && ns <= 4 && s->wv_on[ns - 1]) return 4;
+    if (ib == 5 && ns >= 1 && ns <= 4 && s->wv_on[ns - 1] &&
+        (s->wv_tab[ns - 1] == 1 ? s->mtb_only2 : s->mtb_only))
+        return 4;
     return ib;
 }
 
@@ -457,11 +459,10 @@ static int mtb_depth(const orc_state *s, int *km, double wt1, double wt2, double
  * with a LEVEL-ON table, normal incidence): for every boundary column the
  * table profile, scaled by (level + depth) / (measured level + depth), gives
  * the normal and tangential velocity of each face. */
-static int wave_bc(orc_state *s)
+static int wave_bc_side(orc_state *s, int jd)
 {
     DECL_DIMS(s);
-    for (int jd = 1; jd <= 4; jd++) {
-        if (!s->wv_on[jd - 1]) continue;
+    {
         const double d = s->wv_d[jd - 1], t = s->wv_t[jd - 1], t0 = s->wv_t0[jd - 1];
         double a = s->wv_a[jd - 1];
         /* the table phase in single precision, as the reference keeps it */
@@ -479,7 +480,7 @@ static int wave_bc(orc_state *s)
         if (rc) return rc;
         s->wv_level[jd - 1] = wvz * aw;
         /* a LEVEL-ONLY table sets no velocity here (vf_bwuwn.f:326-327) */
-        if (s->mtb_only) continue;
+        if (s->mtb_only) return 0;
         const double sa = jd == 1 ? 1.0 : (jd == 2 ? -1.0 : 0.0);
         const double sb = jd == 3 ? 1.0 : (jd == 4 ? -1.0 : 0.0);
         const int xside = jd <= 2;
@@ -524,6 +525,37 @@ static int wave_bc(orc_state *s)
                     s->vv[f] = s->bcv[l - 1];
             }
         }
+    }
+    return 0;
+}
+
+/* a side of the second table (FUNC MATRIX2) runs with the two tables'
+ * fields exchanged, so one restatement serves both */
+static void swap_tables(orc_state *s)
+{
+#define SWP(T, a, b) do { T t_ = s->a; s->a = s->b; s->b = t_; } while (0)
+    SWP(int, mtbzz, mtbzz2);
+    SWP(int, mtbtt, mtbtt2);
+    SWP(int, mtbnow, mtbnow2);
+    SWP(const double *, mtb_zz, mtb_zz2);
+    SWP(const double *, mtb_tt, mtb_tt2);
+    SWP(const double *, mtb_hh, mtb_hh2);
+    SWP(const double *, mtb_un, mtb_un2);
+    SWP(const double *, mtb_ut, mtb_ut2);
+    SWP(int, mtb_off, mtb_off2);
+    SWP(int, mtb_only, mtb_only2);
+#undef SWP
+}
+
+static int wave_bc(orc_state *s)
+{
+    for (int jd = 1; jd <= 4; jd++) {
+        if (!s->wv_on[jd - 1]) continue;
+        const int t2 = s->wv_tab[jd - 1] == 1;
+        if (t2) swap_tables(s);
+        const int rc = wave_bc_side(s, jd);
+        if (t2) swap_tables(s);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -2047,12 +2079,10 @@ int orc_fdropf(orc_state *s)
 /* VF_BWFFSF (vf_bwffsf.f:44-125): the cells next to a side driven by a
  * LEVEL-ONLY table take F from the table's level at TNOW (ramped):
  * 0 above it, 1 below it, the filled fraction in the cell it cuts. */
-static int bwffsf(orc_state *s)
+static int bwffsf_side(orc_state *s, int jd)
 {
     DECL_DIMS(s);
-    if (!s->mtb_only) return 0;
-    for (int jd = 1; jd <= 4; jd++) {
-        if (!s->wv_on[jd - 1]) continue;
+    {
         const double t = s->wv_t[jd - 1], t0 = s->wv_t0[jd - 1];
         double a = s->wv_a[jd - 1];
         float r4tn = (float)(t0 - s->tnow / t);
@@ -2083,6 +2113,20 @@ static int bwffsf(orc_state *s)
                 if (s->nf[c] != -1) s->ff[c] = val;
             }
         }
+    }
+    return 0;
+}
+
+static int bwffsf(orc_state *s)
+{
+    for (int jd = 1; jd <= 4; jd++) {
+        if (!s->wv_on[jd - 1]) continue;
+        const int t2 = s->wv_tab[jd - 1] == 1;
+        if (!(t2 ? s->mtb_only2 : s->mtb_only)) continue;
+        if (t2) swap_tables(s);
+        const int rc = bwffsf_side(s, jd);
+        if (t2) swap_tables(s);
+        if (rc) return rc;
     }
     return 0;
 }

@@ -88,6 +88,12 @@ typedef struct orc_state {
     /* the table is LEVEL-OFF (MTBTYP=2): no level column, profiles unscaled;
      * LEVEL-ONLY (MTBTYP=3): a fixed level, no profiles */
     int mtb_off, mtb_only;
+    /* the second table (FUNC MATRIX2, <name>.mtb2; VF_IM2INP, VF_CWMTB12/22)
+     * and per side the table it reads (0 the first, 1 the second) */
+    int mtbzz2, mtbtt2, mtbnow2;
+    const double *mtb_zz2, *mtb_tt2, *mtb_hh2, *mtb_un2, *mtb_ut2;
+    int mtb_off2, mtb_only2;
+    int wv_tab[4];
 } orc_state;
 
 /* Poisson matrix and solver (vf_vpcoef.f, vf_m1bcgs.f and vf_mz*.f) */

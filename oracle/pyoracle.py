@@ -54,6 +54,9 @@ class OrcState(C.Structure):
         ("dm_on", C.c_int * 4), ("dm_deg", C.c_int * 4), ("dm_par", (C.c_double * 4) * 4),
         ("glxmin", C.c_double), ("glxmax", C.c_double), ("glymin", C.c_double),
         ("glymax", C.c_double), ("mtb_off", C.c_int), ("mtb_only", C.c_int),
+        ("mtbzz2", C.c_int), ("mtbtt2", C.c_int), ("mtbnow2", C.c_int),
+        ("mtb_zz2", PD), ("mtb_tt2", PD), ("mtb_hh2", PD), ("mtb_un2", PD), ("mtb_ut2", PD),
+        ("mtb_off2", C.c_int), ("mtb_only2", C.c_int), ("wv_tab", C.c_int * 4),
     ]
 
 
@@ -169,27 +172,35 @@ class Fields:
         for k, v in self.params.items():
             setattr(st, k, v)
 
-    def set_waves(self, waves, table, wvlvl):
+    def set_waves(self, waves, table, wvlvl, table2=None):
         """Table wave-makers: waves = deck.waves (side -> DEPTH, PERIOD, AMPL,
-        AREA), table = mtb.Table (LEVEL-ON)."""
+        AREA; FUNC MATRIX reads table, MATRIX2 table2, each an mtb.Table)."""
         st = self.st
         st.wvlvl = wvlvl
         numi, numj, _ = self.shape
         for jd, wv in waves.items():
             q = jd - 1
             st.wv_on[q] = 1
+            st.wv_tab[q] = 1 if wv.get("func") == "MATRIX2" else 0
             lim = numj if jd <= 2 else numi
             l1, l2 = wv["area"] if wv["area"] else (2, lim - 1)
             st.wv_l1[q], st.wv_l2[q] = l1, l2
             st.wv_d[q], st.wv_t[q], st.wv_a[q] = wv["depth"], wv["period"], wv["ampl"]
-            st.wv_t0[q] = table.t0
-        self.tab = {k: np.asfortranarray(np.array(getattr(table, k), dtype=np.float64))
+            st.wv_t0[q] = (table2 if st.wv_tab[q] else table).t0
+        self.tabs = []
+        for tab, sfx in ((table, ""), (table2, "2")):
+            if tab is None:
+                continue
+            arrs = {k: np.asfortranarray(np.array(getattr(tab, k), dtype=np.float64))
                     for k in ("zz", "tt", "hh", "un", "ut")}
-        st.mtbzz, st.mtbtt, st.mtbnow = len(table.zz), len(table.tt), 0
-        st.mtb_off = 1 if table.kind == "LEVEL-OFF" else 0
-        st.mtb_only = 1 if table.kind == "LEVEL-ONLY" else 0
-        st.mtb_zz, st.mtb_tt, st.mtb_hh = (_pd(self.tab[k]) for k in ("zz", "tt", "hh"))
-        st.mtb_un, st.mtb_ut = _pd(self.tab["un"]), _pd(self.tab["ut"])
+            self.tabs.append(arrs)
+            setattr(st, "mtbzz" + sfx, len(tab.zz))
+            setattr(st, "mtbtt" + sfx, len(tab.tt))
+            setattr(st, "mtbnow" + sfx, 0)
+            setattr(st, "mtb_off" + sfx, 1 if tab.kind == "LEVEL-OFF" else 0)
+            setattr(st, "mtb_only" + sfx, 1 if tab.kind == "LEVEL-ONLY" else 0)
+            for k in ("zz", "tt", "hh", "un", "ut"):
+                setattr(st, f"mtb_{k}{sfx}", _pd(arrs[k]))
 
     def set_opens(self, opens):
         """Radiation sides: opens = deck.opens (side -> depth, period)."""

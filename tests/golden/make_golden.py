@@ -75,6 +75,8 @@ def _cases():
         # ... and by a LEVEL-ONLY table (a fixed water level, vf_bwffsf.f)
         "wave_only": (lambda: cfg.wave_flume(ni=40, steps=400, table="LEVEL-ONLY", name="wave_only"),
                       [1, 2, 100, 200, 400]),
+        # two tables: MATRIX (LEVEL-ON) at X-, MATRIX2 (LEVEL-ONLY) at X+
+        "wave_two": (lambda: cfg.wave_two(name="wave_two"), [1, 2, 100, 200, 400]),
         # Dupuit-Forchheimer drag (OPTION DRAG-DF, vf_vgene.f:111-146)
         "porous_df": (lambda: cfg.porous_df(name="porous_df"), [1, 2, 3, 30, 60]),
         # implicit damping zones (MODEL DAMP, vf_veuler.f:131-166)
@@ -108,6 +110,8 @@ _KEEP = {
                      "f1_out": None} for s in (100, 200, 400)},
     "wave_only": {s: {"v1_in": None, "ps_in": None, "v1_out": None, "f1_in": None,
                       "f1_out": None} for s in (100, 200, 400)},
+    "wave_two": {s: {"v1_in": None, "ps_in": None, "v1_out": None, "f1_in": None,
+                     "f1_out": None} for s in (100, 200, 400)},
     "wave_damp": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                       "f1_out": None} for s in (300, 450, 600)},
     "stokes_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,

@@ -13,7 +13,9 @@ third-order cnoidal wave, 600 steps recorded at 1, 2, 100, 300, 600;
 (no level column: profiles unscaled), 400 steps recorded at 1, 2, 100, 200,
 400; "wave_only", the same flume with its x- side held at the wave's level by
 a LEVEL-ONLY table (faces taken as free faces, the cells next to the side
-filled to the level); "oblique_basin", a 30x16x16 basin with the Stokes wave at 25 degrees to the
+filled to the level); "wave_two", the flume with both ends driven, X- by
+the LEVEL-ON table and X+ by a second table (FUNC MATRIX2, <name>.mtb2)
+holding a fixed level; "oblique_basin", a 30x16x16 basin with the Stokes wave at 25 degrees to the
 side's normal (MODEL WAVE-BC ANGLE), 300 steps recorded at 1, 2, 100, 300.  The
 predictor and the boundary pass
 that sets the special faces have no reduction, so they are compared bit for
@@ -28,12 +30,12 @@ from helpers import assert_close, assert_same_int, assert_vel_close, rel_err
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["wave_flume", "wave_open", "wave_off", "wave_only", "stokes_flume", "cnoidal_flume",
-         "oblique_basin"]
+CASES = ["wave_flume", "wave_open", "wave_off", "wave_only", "wave_two", "stokes_flume",
+         "cnoidal_flume", "oblique_basin"]
 RSL = {"wave_flume": (100, 400, 100), "wave_open": (250, 1000, 250),
        "stokes_flume": (300, 600, 300), "cnoidal_flume": (300, 600, 300),
        "oblique_basin": (100, 300, 200), "wave_off": (200, 400, 200),
-       "wave_only": (200, 400, 200)}
+       "wave_only": (200, 400, 200), "wave_two": (200, 400, 200)}
 STATE = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")
 KEYS = ("rho0", "grz0", "anu0", "scmvp", "flower", "fupper", "plower", "cgpara", "cgepsa",
         "cgepsr", "cgdiv", "icgtyp", "icgmax", "wbub", "idrop")
@@ -48,7 +50,8 @@ def _ctx(pkg, st, dtnow, tnow):
     c.upload_static(st)
     d = pkg.deck.parse(str(st["DECK"]))
     table = pkg.mtb.parse(str(st["FILE_MTB"])) if "FILE_MTB" in st else None
-    c.set_waves(d.waves, table, d.wvlvl)
+    table2 = pkg.mtb.parse(str(st["FILE_MTB2"])) if "FILE_MTB2" in st else None
+    c.set_waves(d.waves, table, d.wvlvl, table2)
     c.set_opens(d.opens)
     return c, d
 
@@ -99,7 +102,7 @@ def test_predictor_with_wave_faces_bitexact(pkg, case):
             level = c.wave_level(1)
         for k, kk in (("UU", "US"), ("VV", "VS"), ("WW", "WS")):
             assert np.array_equal(out[k], r["ps_in"][kk]), f"step {s} {k}: {rel_err(out[k], r['ps_in'][kk]):.3e}"
-        if case != "wave_only":
+        if case not in ("wave_only", "wave_two"):
             # a fixed level's faces are free faces, reset after the solve
             for k in ("BCU", "BCV", "BCW"):
                 assert np.array_equal(out[k][wave], r["v1_out"][k][wave]), f"step {s} {k}"
@@ -124,7 +127,7 @@ def test_v1cal_with_wave_faces(pkg, case):
         assert info["icgitr"] == int(ref["ICGITR"][0]), f"step {s}"
         assert_vel_close(f"step {s} velocity", out, ref, 1e-10)
         assert_close(f"step {s} PP", out["PP"], ref["PP"], 1e-10)
-        if case == "wave_only":
+        if case in ("wave_only", "wave_two"):
             # set from the solved pressure: to the bar, against the faces'
             # velocity scale (v is round-off in this quasi-2-D flume)
             scale = max(np.abs(ref[k][wave]).max() for k in ("BCU", "BCV", "BCW"))
@@ -164,8 +167,9 @@ def test_wave_flume_free_run(pkg, tmp_path, case):
     tr = fx.trajectory(case)
     a, b, n = RSL[case]
     deck = str(st["DECK"]) + f"\nFILE RSL STEP {a} {b} {n}\n"
-    if "FILE_MTB" in st:
-        (tmp_path / "flume.mtb").write_text(str(st["FILE_MTB"]))
+    for ext in ("mtb", "mtb2"):
+        if f"FILE_{ext.upper()}" in st:
+            (tmp_path / f"flume.{ext}").write_text(str(st[f"FILE_{ext.upper()}"]))
     res = pkg.driver.run(deck, "flume", tmp_path)
     last = int(tr["STEP"][-1])
     assert res["nnow"] == last

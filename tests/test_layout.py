@@ -11,7 +11,7 @@ import fixtures as fx
 
 CASES = ["tiny", "beach_small", "corner3d", "dam_a", "tdoor", "auto_dam", "subloop", "wave_flume",
          "porous_df", "wave_damp", "damp3d", "stokes_flume", "cnoidal_flume",
-         "oblique_basin", "wave_off", "wave_only"]
+         "oblique_basin", "wave_off", "wave_only", "wave_two"]
 STATIC = ("XX", "YY", "ZZ", "ANU", "CD0", "GGV", "GGX", "GGY", "GGZ", "GLV", "GLX", "GLY", "GLZ",
           "INDX", "INDY", "INDZ", "INDB", "BCVI")
 
@@ -102,7 +102,7 @@ def test_deck_theory_wave_words(pkg):
     """MODEL WAVE-BC FUNC STOKES / CNOIDAL / STK-CND and ANGLE (Source
     code/vf_iimdl.f:51-70, 107-111) with the set-up checks of
     vf_csetup.f:86-100: a theory wave needs DEPTH, HEIGHT and PERIOD, the table
-    takes no ANGLE; STREAM and MATRIX2 are outside what is produced here."""
+    takes no ANGLE; STREAM is outside what is produced here."""
     base = ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
             "B.C. D VP SLIP\nB.C. D F FREE\n")
     full = ("MODEL WAVE-BC X- DEPTH 0.2\nMODEL WAVE-BC X- HEIGHT 0.02\n"
@@ -115,7 +115,7 @@ def test_deck_theory_wave_words(pkg):
                        "MODEL WAVE-BC X- PERIOD 1.0\n")
     with pytest.raises(pkg.deck.DeckError):
         pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC AIRY\n" + full)
-    for func in ("STREAM 5", "MATRIX2"):
+    for func in ("STREAM 5",):
         with pytest.raises(pkg.deck.UnsupportedDeck):
             pkg.deck.parse(base + f"MODEL WAVE-BC X- FUNC {func}\n" + full)
     d = pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STOKES\n" + full +

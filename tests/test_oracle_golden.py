@@ -44,7 +44,7 @@ def test_init_state_matches_reference(pkg, case):
     assert np.array_equal(F.inds, ref["INDS"][: int(ref["NUMS"][0])])
 
 
-@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only"])
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only", "wave_two"])
 def test_init_state_with_special_sides(pkg, case):
     """VF_CINIT's boundary pass with a table wave-maker (at TNOW = 0) and a
     radiation side (the initial pass, ALN = ALT = 1) == the reference's
@@ -55,7 +55,7 @@ def test_init_state_with_special_sides(pkg, case):
     arr.update(lay.raw_state())
     F = po.Fields(arr, lay.params())
     d = lay.deck
-    F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+    F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl, _table2(pkg, st))
     F.set_opens(d.opens)
     F.st.tnow = 0.0
     assert F.init_state(d.wvlvl, d.uini, 3) == 0
@@ -295,6 +295,11 @@ def test_cdtcal_is_the_cell_minimum():
     assert worst <= 1e-12, worst
 
 
+def _table2(pkg, st):
+    """The second wave-maker table (FUNC MATRIX2) of a case, if it has one."""
+    return pkg.mtb.parse(str(st["FILE_MTB2"])) if "FILE_MTB2" in st else None
+
+
 def _wave_fields(pkg, case, rec, bcf, damps=True):
     """Oracle fields with the deck's models around the step: wave-maker and
     radiation sides, the drag table and the damping zones."""
@@ -303,7 +308,7 @@ def _wave_fields(pkg, case, rec, bcf, damps=True):
     F = _fields(st, rec, p, bcf)
     d = pkg.deck.parse(str(st["DECK"]))
     if d.waves:
-        F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl)
+        F.set_waves(d.waves, pkg.mtb.parse(str(st["FILE_MTB"])), d.wvlvl, _table2(pkg, st))
     F.set_opens(d.opens)
     F.set_drag_df(d.drag_df)
     if damps:
@@ -311,7 +316,7 @@ def _wave_fields(pkg, case, rec, bcf, damps=True):
     return st, p, F
 
 
-@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only"])
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only", "wave_two"])
 def test_wave_maker_phases_bitexact(pkg, case):
     """MODEL WAVE-BC X- FUNC MATRIX, and in wave_open a radiation boundary
     (MODEL OPEN-BC X+ FUNC TYPE1) at the far end: V1CAL (the wave-maker and
@@ -337,7 +342,7 @@ def test_wave_maker_phases_bitexact(pkg, case):
             assert np.array_equal(F[k], r["f1_out"][k]), (s, k)
 
 
-@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only"])
+@pytest.mark.parametrize("case", ["wave_flume", "wave_open", "wave_off", "wave_only", "wave_two"])
 def test_wave_maker_free_run_bitexact(pkg, case):
     """The wave flume (400 steps) and the flume with the radiation end (1000
     steps) from the reference's initial state: ITR and ||div|| every step and
