@@ -243,3 +243,40 @@ def test_special_side_errors(pkg):
         # the level of VF_CINIT's pass: the table's level at its phase origin
         # (the same table as the wave flume's, whose .tran prints 3.61819724E-06)
         assert abs(c.wave_level(1) - 3.61819724e-06) <= 1e-14
+
+
+def test_theory_and_second_table_errors(pkg):
+    """cad_set_wave_theory refuses a bad side or function, missing depth,
+    height or period, columns outside the side and a radiation side;
+    cad_set_wave2 needs its table first, and cad_set_wave_table2 checks its
+    arrays as the first table's entry does."""
+    st = fx.static("stokes_flume")
+    lay = pkg.layout.from_text(str(st["DECK"]))
+    numi, numj, numk = st["GGV"].shape
+    lib = pkg.capi.lib()
+    with pkg.capi.Context(numi, numj, numk, st["INDB"].shape[1]) as c:
+        p = lay.params()
+        c.set_params(**{k: v for k, v in p.items() if k in c.params})
+        c.upload_static(st)
+        good = (2, numj - 1, 0.2, 0.02, 1.0, 1.0, 0.2, 0.0, 0.0, 0.0)
+        assert lib.cad_set_wave_theory(c._h, 1, -2, *good) == 0
+        assert lib.cad_set_wave_theory(c._h, 5, -2, *good) == -1
+        assert lib.cad_set_wave_theory(c._h, 1, -3, *good) == -1
+        assert lib.cad_set_wave_theory(c._h, 1, 1, *good) == -1
+        for i in (2, 3, 4):   # depth, height, period
+            bad = list(good)
+            bad[i] = 0.0
+            assert lib.cad_set_wave_theory(c._h, 1, -2, *bad) == -1
+        assert lib.cad_set_wave_theory(c._h, 1, -2, 1, numj - 1, *good[2:]) == -1
+        assert lib.cad_set_wave_theory(c._h, 1, -2, 2, numj, *good[2:]) == -1
+        assert lib.cad_set_open(c._h, 2, 0.2, 1.0) == 0
+        assert lib.cad_set_wave_theory(c._h, 2, -2, *good) == -1    # a radiation side
+        assert lib.cad_set_wave2(c._h, 3, 2, numi - 1, 0.2, 1.0, 0.0, 0.2) == -3   # no table 2
+        t = np.linspace(0.0, 1.0, 5)
+        assert lib.cad_set_wave_table2(c._h, 1, 5, None, t.ctypes.data_as(pkg.capi.PD), None,
+                                       None, None, 0.0) == -1
+        z = np.array([0.0, -1.0])
+        assert lib.cad_set_wave_table2(c._h, 2, 5, z.ctypes.data_as(pkg.capi.PD),
+                                       t.ctypes.data_as(pkg.capi.PD), None,
+                                       np.zeros(10).ctypes.data_as(pkg.capi.PD),
+                                       np.zeros(10).ctypes.data_as(pkg.capi.PD), 0.0) == -1

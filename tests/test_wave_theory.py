@@ -155,3 +155,21 @@ def test_theory_wave_is_a_wave(pkg):
         crest = ph[int(np.argmax(lv))]
         _, un, _, _ = w.call(crest, [-0.15, -0.1, -0.05, 0.0])
         assert np.all(un > 0) and np.all(np.diff(un) > 0)
+
+
+def test_theory_eval_refuses_bad_arguments(pkg):
+    """cad_wave_theory_eval refuses a function other than Stokes (-2) or
+    cnoidal (-1), a missing state, and a depth, height or period below the
+    set-up checks' 1e-20 (vf_csetup.f:86-94)."""
+    import ctypes as C
+    lib = pkg.capi.lib()
+    st = C.c_float(0.0)
+    lv = C.c_double()
+    for func, d, h, t in ((0, 0.2, 0.02, 1.0), (-3, 0.2, 0.02, 1.0), (-2, 0.0, 0.02, 1.0),
+                          (-2, 0.2, 0.0, 1.0), (-1, 0.2, 0.02, 0.0)):
+        assert lib.cad_wave_theory_eval(func, d, h, t, C.byref(st), 0.5, 0, None, None, None,
+                                        C.byref(lv), None, None) == -1
+    assert lib.cad_wave_theory_eval(-2, 0.2, 0.02, 1.0, None, 0.5, 0, None, None, None,
+                                    C.byref(lv), None, None) == -1
+    assert lib.cad_wave_theory_eval(-2, 0.2, 0.02, 1.0, C.byref(st), 0.5, 0, None, None, None,
+                                    C.byref(lv), None, None) == 0
