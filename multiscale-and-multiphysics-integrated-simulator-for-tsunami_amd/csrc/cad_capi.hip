@@ -174,7 +174,8 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
     rc = rc ? rc : dalloc(&c->red_out, 16);
     rc = rc ? rc : dalloc(&c->ksc, 32);
     rc = rc ? rc : dalloc(&c->act_rng, 2);
-    rc = rc ? rc : dalloc(&c->icount, 16);
+    // 16 scalar counters, then the surface-orientation count slots (2 x 1024)
+    rc = rc ? rc : dalloc(&c->icount, 16 + 2 * 1024);
     rc = rc ? rc : cad_sweep_alloc(c);
     if (rc) {
         cad_destroy(c);

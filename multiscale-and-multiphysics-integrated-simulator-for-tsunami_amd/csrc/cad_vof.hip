@@ -503,8 +503,13 @@ __global__ void __launch_bounds__(256) k_nf_fix(Dims d, int *nf)
 }
 
 // Surface orientation, vf_fnfprv.f:34-223; counts surface cells.
+// the oriented / unorientable surface counts go to ORIENT_SLOTS pairs of
+// counters picked by the block, then one block sums them: atomics from every
+// wavefront with a surface cell on one address serialise in L2
+#define ORIENT_SLOTS 1024
+
 __global__ void __launch_bounds__(256) k_nf_orient(Dims d, Static s, const double *__restrict__ f,
-                                                   const double *__restrict__ bcf, int *nf, int *cnt)
+                                                   const double *__restrict__ bcf, int *nf, int *slots)
 {
     int i, j, k;
     if (!cell_of_thread(d, i, j, k)) return;
@@ -539,12 +544,40 @@ __global__ void __launch_bounds__(256) k_nf_orient(Dims d, Static s, const doubl
     }
     // orientation written after all candidates: the tests read only ==0 / ==8
     nf[n] = code;
-    // counts aggregated per wavefront: one atomic per wave, not per cell
+    // counts aggregated per wavefront, one atomic per wave into its block's slot
     const unsigned long long all = __ballot(1), bad = __ballot(code == 7);
     if ((int)__lane_id() == __builtin_ctzll(all)) {
         const int nb = __popcll(bad), ng = __popcll(all) - nb;
-        if (ng) atomicAdd(cnt, ng);
-        if (nb) atomicAdd(cnt + 1, nb);
+        const unsigned blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        int *sl = slots + 2 * (int)(blk % ORIENT_SLOTS);
+        if (ng) atomicAdd(sl, ng);
+        if (nb) atomicAdd(sl + 1, nb);
+    }
+}
+
+// the two counts of k_nf_orient's slots into out[0], out[1]
+__global__ void __launch_bounds__(256) k_orient_sum(const int *__restrict__ slots, int *out)
+{
+    __shared__ int a[256], b[256];
+    const int t = threadIdx.x;
+    int x = 0, y = 0;
+    for (int q = t; q < ORIENT_SLOTS; q += 256) {
+        x += slots[2 * q];
+        y += slots[2 * q + 1];
+    }
+    a[t] = x;
+    b[t] = y;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            a[t] += a[t + h];
+            b[t] += b[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = a[0];
+        out[1] = b[0];
     }
 }
 
@@ -697,9 +730,10 @@ static int orient_cells(cad_ctx *c, int *nums)
     Fields &f = c->f;
     dim3 g = cell_grid(d), b = cell_block();
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 3, 0);
-    hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 4, 0);
-    hipLaunchKernelGGL(k_nf_orient, g, b, 0, st, d, s, f.ff, f.bcf, f.nf, c->icount + 3);
+    int *slots = c->icount + 16;   // 2 * ORIENT_SLOTS counters after the scalar counts
+    hipLaunchKernelGGL(k_ifill, dim3(8), dim3(256), 0, st, (long long)(2 * ORIENT_SLOTS), slots, 0);
+    hipLaunchKernelGGL(k_nf_orient, g, b, 0, st, d, s, f.ff, f.bcf, f.nf, slots);
+    hipLaunchKernelGGL(k_orient_sum, dim3(1), dim3(256), 0, st, slots, c->icount + 3);
     hipLaunchKernelGGL(k_cindc, g, b, 0, st, d, f.nf, f.indc);
     CAD_CHECK(hipGetLastError());
     CAD_CHECK(hipMemcpyAsync(c->host_cnt + 3, c->icount + 3, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
