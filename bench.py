@@ -90,17 +90,20 @@ def prepare(pkg, case):
     return lay, state, flags
 
 
-def measured_traffic(workload_name):
-    """HBM bytes of one M^-1 application measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) for the default
-    workload, from the committed profile summary; None for other workloads."""
+def measured_traffic(workload_name, key="traffic_bytes_per_minv",
+                     summary="sweep_traffic_buildings_512x256x128.md"):
+    """HBM bytes of one application of the dominant kernel (the plain form's
+    M^-1, or with key="traffic_bytes_per_vcycle" the mixed form's V-cycle)
+    measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    of known size) for the default workload, from the committed profile
+    summary; None for other workloads."""
     if workload_name != "buildings_c":
         return None
-    f = ROOT / "profiles" / "sweep_traffic_buildings_512x256x128.md"
+    f = ROOT / "profiles" / summary
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_minv:"):
+        if line.startswith(key + ":"):
             return int(line.split(":")[1])
     return None
 
@@ -189,7 +192,9 @@ def run_form(pkg, lay, state, flags, form, args):
         vc_ms, vc_bytes = ctx.time_precond(args.minv_reps)
         achieved = vc_bytes / (vc_ms * 1e-3) / 1e9
         r["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                         "traffic": measured_traffic(args.workload, "traffic_bytes_per_vcycle",
+                                                     "multigrid_traffic_buildings_512x256x128.md"),
                          "kernel": "FP32 multigrid V-cycle (the mixed form's preconditioner)",
                          "avg_ms": round(vc_ms, 4), "bytes": round(vc_bytes)}
     ctx.close()
