@@ -3,8 +3,8 @@
 Workload (default): the 512x256x128-cell tank (16.8 M cells, 0.02 m cells) with
 a 1:20 plane beach from x=6 m, 40 seeded rectangular buildings on the beach
 half and a Boussinesq solitary wave H/h=0.15 on h=0.8 m, plain FP64 form, MILU
-preconditioned BiCGSTAB stopped at a relative (preconditioned) residual of
-1e-8.  A "step" is one SMAC time step (predictor, Poisson solve, correction,
+preconditioned BiCGSTAB stopped at a relative preconditioned residual of 1e-8
+and a true relative residual of 1e-8.  A "step" is one SMAC time step (predictor, Poisson solve, correction,
 VOF advection and flagging) over the whole grid; fields stay resident in HBM.
 
 Both forms run on the same workload from the same initial state (--form
@@ -15,7 +15,11 @@ stopped on the true residual ||BB - A PT|| <= 1e-8 ||BB||) is reported under
 "mixed" with its speed-up and its fields' differences from the plain form's
 after the same steps.
 
-value  = interior cells x timed steps / wall time of the timed steps.
+value  = interior cells x timed steps / wall time of the timed steps (each step
+         timed on its own; the true residual of its solve, checked after each
+         step, is outside the timer).  Both forms are held to a true relative
+         residual ||BB - A PT|| / ||BB|| <= 1e-8; the worst over the timed
+         steps is reported.
 roofline = (plain form) the MILU preconditioner application M^-1 (forward + backward
            triangular sweeps, the solver's dominant kernel): 12 FP64 streams per
            active cell (DD, ALI, ALJ, ALK, y, x forward; DD, AUI, AUJ, AUK, x,
@@ -50,6 +54,7 @@ ROOT = Path(__file__).resolve().parent
 PKG_DIR = ROOT / "multiscale-and-multiphysics-integrated-simulator-for-tsunami_amd"
 METRIC = "SMAC cell-updates/s + Poisson ms/step at 1e-8 resid: learned/mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+TRUE_RESIDUAL = 1.0e-8
 
 
 def _pkg():
@@ -141,6 +146,11 @@ def run_form(pkg, lay, state, flags, form, args):
     ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
     if form == "mixed":
         ctx.set_params(precision=pkg.capi.PRECISION_MIXED)
+    else:
+        # the plain form also stops only at a true relative residual of 1e-8
+        # (rechecked whenever the reference's preconditioned test passes), the
+        # residual the mixed form stops on
+        ctx.set_true_residual_stop(TRUE_RESIDUAL)
     ctx.upload_static(lay.static_arrays())
     ctx.upload_state(state)
     ctx.init_state(lay.deck.wvlvl, lay.deck.uini, flags)
@@ -149,28 +159,36 @@ def run_form(pkg, lay, state, flags, form, args):
         info = ctx.step()
         log(f"bench: {form} warmup {w} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms "
             f"true residual {ctx.poisson_residual():.3e}")
-    itr, poisson, pred, corr, vof = [], [], [], [], []
-    t0 = time.perf_counter()
+    itr, poisson, pred, corr, vof, res = [], [], [], [], [], []
+    wall = 0.0
     for s in range(args.steps):
+        # each step returns after its work has finished on the device; the
+        # true residual of its solve is checked after the timer stops
+        t0 = time.perf_counter()
         info = ctx.step()
+        wall += time.perf_counter() - t0
         itr.append(info["icgitr"])
         poisson.append(info["ms_poisson"])
         pred.append(info["ms_predictor"])
         corr.append(info["ms_correction"])
         vof.append(info["ms_vof"])
-        log(f"bench: {form} step {s} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms")
-    wall = time.perf_counter() - t0
-    true_res = ctx.poisson_residual()
-    log(f"bench: {form} timed {args.steps} steps in {wall:.3f}s ITR={itr} true residual {true_res:.3e}")
+        res.append(ctx.poisson_residual())
+        log(f"bench: {form} step {s} ITR={info['icgitr']} poisson={info['ms_poisson']:.1f}ms "
+            f"true residual {res[-1]:.3e}")
+    true_res = max(res)
+    log(f"bench: {form} timed {args.steps} steps in {wall:.3f}s ITR={itr} worst true residual "
+        f"{true_res:.3e}")
     end_state = ctx.download_state() if args.form == "both" else None
     r = {"form": "plain FP64" if form == "plain" else "mixed FP64/FP32",
-         "form_text": ("plain FP64 form, MILU-BiCGSTAB to 1e-8 relative (preconditioned) residual"
+         "form_text": ("plain FP64 form, MILU-BiCGSTAB to 1e-8 relative preconditioned and true "
+                       "residual"
                        if form == "plain" else
                        "mixed form, FP64 BiCGSTAB with an FP32 aggregation-multigrid preconditioner "
                        "to 1e-8 relative true residual"),
          "value": round(cells * args.steps / wall, 1), "ms_per_step": round(1e3 * wall / args.steps, 3),
          "poisson_ms_per_step": round(float(np.mean(poisson)), 3),
          "poisson_true_residual": float(f"{true_res:.3e}"),
+         "poisson_true_residual_note": "worst ||BB - A PT|| / ||BB|| over the timed steps",
          "itr_per_step": float(np.mean(itr)),
          "phase_ms": {"predictor": round(float(np.mean(pred)), 3),
                       "poisson": round(float(np.mean(poisson)), 3),
@@ -323,8 +341,15 @@ def main():
     }
     if "plain" in res:
         pl = res["plain"]
-        step_bytes = (1112.0 + 496.0 * pl["itr_per_step"]) * cells
+        # the step's byte model over the cells each part touches: predictor,
+        # assembly, correction and VOF (716 B) over the interior cells; MILU
+        # factorisation and solver set-up (396 B) and each BiCGSTAB iteration
+        # (496 B) over the active cells the solver's vectors cover
+        active = pl["roofline"]["active_cells"]
+        step_bytes = 716.0 * cells + (396.0 + 496.0 * pl["itr_per_step"]) * active
         out["step_algorithmic_GBps"] = round(step_bytes / (pl["ms_per_step"] * 1e-3) / 1e9, 2)
+        out["step_algorithmic_frac"] = round(step_bytes / (pl["ms_per_step"] * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS, 4)
         out["roofline"] = pl["roofline"]
     if "mixed" in res:
         mx = res["mixed"]

@@ -8,6 +8,12 @@
 !
 ! Build: flang -c cadmas_hip.f90 (module cadmas_hip), link with
 ! -L<package dir> -lcadmas_hip.
+!
+! The pressure-solve drop-ins VF_M1BCGS and VF_VPSOL need no interface: the
+! library exports them under the reference's own link names (vf_m1bcgs_,
+! vf_vpsol_) with the reference's argument lists, so the reference's own
+! implicit-interface CALLs (vf_vpsol.f:64-69, vf_v1cal.f:278-281) bind to them
+! when the library is linked in place of vf_m1bcgs.o / vf_vpsol.o.
 module cadmas_hip
   use, intrinsic :: iso_c_binding
   implicit none
@@ -291,19 +297,5 @@ module cadmas_hip
       integer(c_int) :: cad_cdiv00
     end function
 
-    ! drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5): the same by-reference
-    ! arguments minus DBUF and the work arrays DD..AE
-    subroutine cadmas_hip_m1bcgs(ad, ali, alj, alk, aui, auj, auk, bb, x0, cgpara, cgepsa, cgepsr, &
-                                 cgdiv, bnorm, xnorm, indc, mi, mj, mk, ni, nj, nk, icgtyp, icgmax, &
-                                 iter) bind(c, name='cadmas_hip_m1bcgs_')
-      import :: c_int, c_double
-      real(c_double), intent(in) :: ad(*), ali(*), alj(*), alk(*), aui(*), auj(*), auk(*), bb(*)
-      real(c_double), intent(inout) :: x0(*)
-      real(c_double), intent(in) :: cgpara, cgepsa, cgepsr, cgdiv
-      real(c_double), intent(out) :: bnorm, xnorm
-      integer(c_int), intent(in) :: indc(*)
-      integer(c_int), intent(in) :: mi, mj, mk, ni, nj, nk, icgtyp, icgmax
-      integer(c_int), intent(out) :: iter
-    end subroutine
   end interface
 end module cadmas_hip

@@ -264,6 +264,26 @@ int cad_time_precond(cad_ctx *ctx, int reps, float *ms_avg, double *bytes);
 int cad_time_minv(cad_ctx *ctx, int reps, float *ms_avg);
 int cad_count_active(cad_ctx *ctx, long long *active);
 
+/* Cap on the flag-fix passes of VF_FNFINI (vf_fnfini.f:108-180, repeated
+ * until no surface cell is left without fluid on one side and gas on the
+ * other).  Each pass that fixes a cell turns at least one cell into gas and
+ * none back, so the loop ends within (interior cells + 1) passes, the
+ * default (passes = 0).  A state still unfixed after the cap makes the
+ * flagging pass fail with CAD_ERR_STATE instead of going on with
+ * inconsistent flags.  No counterpart in the reference, whose loop has no
+ * cap. */
+int cad_set_flag_pass_limit(cad_ctx *ctx, long long passes);
+
+/* The plain form's MILU-BiCGSTAB (cad_vpsol with CAD_PRECISION_FP64) stops
+ * on the reference's test ||M^-1 r||^2 <= ||M^-1 b||^2 CGEPSR^2 + CGEPSA^2
+ * (vf_m1bcgs.f:144,219).  With rel > 0 it also requires the true relative
+ * residual ||BB - A PT|| / ||BB|| <= rel, rechecked in FP64 whenever the
+ * reference's test passes, and iterates on otherwise; 0 (the default)
+ * restores the reference's test alone.  No counterpart in the reference: it
+ * holds the plain form to the true residual the mixed and learned forms stop
+ * on, so that their solves are compared at the same residual. */
+int cad_set_true_residual_stop(cad_ctx *ctx, double rel);
+
 /* Schedule of the MILU triangular sweeps (vf_mzdcmp.f, vf_mzfrwd.f,
  * vf_mzbkwd.f).  All give the serial sweep's results bit for bit:
  *   CAD_SWEEP_PIPELINED       (default) one launch per sweep, wavefront tiles
@@ -289,21 +309,63 @@ int cad_set_sweep_schedule(cad_ctx *ctx, int schedule);
 int cad_milu_factor(cad_ctx *ctx, int *npos);
 int cad_milu_apply(cad_ctx *ctx, const double *y, double *x);
 
-/* Drop-in for VF_M1BCGS (vf_m1bcgs.f:1-5) with the reference's by-reference
- * argument list (trailing underscore: callable from Fortran as
- * CALL CADMAS_HIP_M1BCGS(...)).  Host arrays in, host X0 out; the device
- * context is created on first call and reused while the grid size is unchanged.
- * DBUF and the eight work arrays DD..AE of the reference are not needed and
- * are omitted.  MI/MJ/MK must equal NI/NJ/NK (VF_VPSOL passes NUMI, NUMJ,
- * NUMK for both, vf_vpsol.f:61-69).  ITER keeps VF_M1BCGS's meaning; -3 means
- * the library itself failed (message in cad_last_error()). */
-void cadmas_hip_m1bcgs_(const double *ad, const double *ali, const double *alj, const double *alk,
-                        const double *aui, const double *auj, const double *auk, const double *bb,
-                        double *x0, const double *cgpara, const double *cgepsa,
-                        const double *cgepsr, const double *cgdiv, double *bnorm, double *xnorm,
-                        const int *indc, const int *mi, const int *mj, const int *mk,
-                        const int *ni, const int *nj, const int *nk, const int *icgtyp,
-                        const int *icgmax, int *iter);
+/* Link-level drop-in of the reference's pressure solve -------------------
+ * The reference's own routines under their own link names (flang/gfortran:
+ * lowercase with a trailing underscore) and by-reference argument lists, so
+ * that the reference links this library in place of vf_m1bcgs.o, or of
+ * vf_vpsol.o and vf_m1bcgs.o, with no other change.  Arrays are the
+ * reference's explicit-shape column-major host arrays.  Single MPI rank
+ * (loops over 2..NI-1, 2..NJ-1 as MYIS..MYIE, MYJS..MYJE are at one rank);
+ * DBUF and the work arrays DD..AE are accepted and left untouched (the
+ * reference's callers do not read them afterwards, vf_v1cal.f:278-297).
+ * The device context is created on the first call and kept while the grid
+ * size is unchanged. */
+
+/* COMMON blocks with the reference's member order (VF_ACOMPR.h, VF_ACOMPI.h,
+ * VF_ANUMBI.h; MAXNC = 10 in VF_A0PRM.h).  The library defines them weakly:
+ * linked into the reference, the reference's blocks are the ones used. */
+typedef struct vf_acompr_block {
+    double cgpara, cgepsa, cgepsr, cgdiv, cgbnrm, cgxnrm;
+    double feps, flower, fupper, fsum, fcut, plower;
+    double scmvp, scmk, scmt, scmc[10];
+} vf_acompr_block;
+typedef struct vf_acompi_block {
+    int icgtyp, icgmax, icgitr, iscmvp, iscmff, iscmk, iscmt, iscmc[10], ibsuw0;
+} vf_acompi_block;
+typedef struct vf_anumbi_block {
+    int numi, numj, numk, numb, nums, leqk, leqt, leqc, numij, numbx;
+} vf_anumbi_block;
+extern vf_acompr_block vf_acompr_;
+extern vf_acompi_block vf_acompi_;
+extern vf_anumbi_block vf_anumbi_;
+
+/* VF_M1BCGS (vf_m1bcgs.f:1-5): MILU-BiCGSTAB on A x = b, arrays (MI,MJ,MK)
+ * with the loop range (NI,NJ,NK), MI >= NI etc.  X0 in: the initial guess;
+ * out: the solution where ITER >= 0 (active cells), unchanged otherwise.
+ * BNORM = ||M^-1 b||, XNORM = the last ||M^-1 r||, ITER as the reference's:
+ * >= 0 iterations, -1 MILU pivot <= 0, -2 divergence, > ICGMAX not
+ * converged.  A failure of the library itself (sizes, device, memory) also
+ * returns ITER = ICGMAX + 1, which VF_VPSOL stops on (vf_vpsol.f:70-71),
+ * with the cause on stderr and in cad_last_error(). */
+void vf_m1bcgs_(double *dbuf, const double *ad, const double *ali, const double *alj,
+                const double *alk, const double *aui, const double *auj, const double *auk,
+                const double *bb, double *x0, double *dd, double *xw, double *yy, double *r0,
+                double *rr, double *pp, double *ap, double *ae, const double *cgpara,
+                const double *cgepsa, const double *cgepsr, const double *cgdiv, double *bnorm,
+                double *xnorm, const int *indc, const int *mi, const int *mj, const int *mk,
+                const int *ni, const int *nj, const int *nk, const int *icgtyp,
+                const int *icgmax, int *iter);
+
+/* VF_VPSOL (vf_vpsol.f:1-3): PT = 0, then VF_M1BCGS on (NUMI,NUMJ,NUMK) of
+ * /VF_ANUMBI/ with CGPARA, CGEPSA, CGEPSR, CGDIV of /VF_ACOMPR/ and ICGTYP,
+ * ICGMAX of /VF_ACOMPI/; CGBNRM, CGXNRM and ICGITR are written back there.
+ * Its error checks (vf_vpsol.f:70-75) call the reference's VF_A2ERR when the
+ * program has one (linked into the reference); other callers read ICGITR and
+ * cad_last_error(). */
+void vf_vpsol_(double *dbuf, const double *ad, const double *ali, const double *alj,
+               const double *alk, const double *aui, const double *auj, const double *auk,
+               const double *bb, double *pt, double *dd, double *xw, double *yy, double *r0,
+               double *rr, double *pp, double *ap, double *ae, const int *indc);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,12 @@ def lib():
     L.cad_time_minv.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
     L.cad_count_active.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.cad_set_sweep_schedule.argtypes = [C.c_void_p, C.c_int]
+    L.cad_set_flag_pass_limit.argtypes = [C.c_void_p, C.c_longlong]
+    L.cad_set_true_residual_stop.argtypes = [C.c_void_p, C.c_double]
+    L.vf_m1bcgs_.argtypes = [PD] * 24 + [PI] * 10
+    L.vf_m1bcgs_.restype = None
+    L.vf_vpsol_.argtypes = [PD] * 18 + [PI]
+    L.vf_vpsol_.restype = None
     L.cad_milu_factor.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.cad_milu_apply.argtypes = [C.c_void_p, PD, PD]
     _lib = L
@@ -118,13 +124,14 @@ def lib():
 EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad_upload_static",
            "cad_upload_state", "cad_download_state", "cad_v1cal", "cad_f1cal", "cad_step",
            "cad_cdiv00", "cad_predictor", "cad_vpcoef", "cad_vpsol", "cad_correction",
-           "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "cadmas_hip_m1bcgs_",
+           "cad_download_matrix", "cad_upload_matrix", "cad_download_pt", "vf_m1bcgs_", "vf_vpsol_",
            "cad_init_state", "cad_time_minv", "cad_count_active", "cad_set_sweep_schedule",
            "cad_milu_factor", "cad_milu_apply", "cad_get_params", "cad_upload_tdoor",
            "cad_download_tdoor", "cad_cdtcal", "cad_set_wave_table", "cad_set_wave",
            "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
            "cad_set_drag_df", "cad_set_damp", "cad_set_wave_theory", "cad_wave_theory_eval",
-           "cad_set_wave_table2", "cad_set_wave2")
+           "cad_set_wave_table2", "cad_set_wave2", "cad_set_flag_pass_limit",
+           "cad_set_true_residual_stop")
 
 class TheoryWave:
     """A theory wave evaluated on the host (cad_wave_theory_eval), with
@@ -424,6 +431,15 @@ class Context:
         _check(lib().cad_time_minv(self._h, reps, C.byref(ms)))
         return ms.value
 
+    def set_flag_pass_limit(self, passes):
+        """Cap on VF_FNFINI's flag-fix passes (0: the default bound)."""
+        _check(lib().cad_set_flag_pass_limit(self._h, int(passes)))
+
+    def set_true_residual_stop(self, rel):
+        """Plain form: also stop only once ||BB - A PT|| / ||BB|| <= rel (0: the
+        reference's stop test alone)."""
+        _check(lib().cad_set_true_residual_stop(self._h, float(rel)))
+
     def set_sweep_schedule(self, schedule):
         _check(lib().cad_set_sweep_schedule(self._h, int(schedule)))
 
@@ -451,18 +467,77 @@ class Context:
         return pt
 
 
+class AcomprBlock(C.Structure):
+    """COMMON /VF_ACOMPR/ (VF_ACOMPR.h)."""
+    _fields_ = [(k, C.c_double) for k in ("cgpara", "cgepsa", "cgepsr", "cgdiv", "cgbnrm", "cgxnrm",
+                                          "feps", "flower", "fupper", "fsum", "fcut", "plower",
+                                          "scmvp", "scmk", "scmt")] + [("scmc", C.c_double * 10)]
+
+
+class AcompiBlock(C.Structure):
+    """COMMON /VF_ACOMPI/ (VF_ACOMPI.h)."""
+    _fields_ = [(k, C.c_int) for k in ("icgtyp", "icgmax", "icgitr", "iscmvp", "iscmff", "iscmk",
+                                       "iscmt")] + [("iscmc", C.c_int * 10), ("ibsuw0", C.c_int)]
+
+
+class AnumbiBlock(C.Structure):
+    """COMMON /VF_ANUMBI/ (VF_ANUMBI.h)."""
+    _fields_ = [(k, C.c_int) for k in ("numi", "numj", "numk", "numb", "nums", "leqk", "leqt",
+                                       "leqc", "numij", "numbx")]
+
+
+def common_blocks():
+    """The library's COMMON blocks /VF_ACOMPR/, /VF_ACOMPI/, /VF_ANUMBI/ (its
+    own instances when no Fortran program supplies them)."""
+    L = lib()
+    return (AcomprBlock.in_dll(L, "vf_acompr_"), AcompiBlock.in_dll(L, "vf_acompi_"),
+            AnumbiBlock.in_dll(L, "vf_anumbi_"))
+
+
+def _work(shape):
+    return [np.zeros(shape, order="F") for _ in range(8)]
+
+
 def m1bcgs(mat, indc, x0=None, cgpara=0.95, cgepsa=1.0e-15, cgepsr=1.0e-12, cgdiv=1.0e30,
-           icgtyp=1, icgmax=500):
-    """VF_M1BCGS through the Fortran-style drop-in entry point cadmas_hip_m1bcgs_."""
+           icgtyp=1, icgmax=500, loop=None):
+    """VF_M1BCGS through the drop-in vf_m1bcgs_ with the reference's full
+    argument list (vf_m1bcgs.f:1-5).  The arrays' shape is (MI,MJ,MK); loop =
+    (NI,NJ,NK), the loop range, defaults to the shape.  Returns X0, BNORM,
+    XNORM, ITER."""
     L = lib()
     shape = mat["AD"].shape
+    nijk = tuple(shape if loop is None else loop)
     ms = [_f64(mat[k], shape) for k in MATRIX]
     x = _f64(np.zeros(shape) if x0 is None else x0, shape).copy(order="F")
     ic = _i32(indc, shape)
     d = lambda v: C.byref(C.c_double(v))  # noqa: E731
-    n = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    n = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
     bn, xn, it = C.c_double(), C.c_double(), C.c_int()
-    L.cadmas_hip_m1bcgs_(*[_pd(a) for a in ms], _pd(x), d(cgpara), d(cgepsa), d(cgepsr), d(cgdiv),
-                         C.byref(bn), C.byref(xn), _pi(ic), n(shape[0]), n(shape[1]), n(shape[2]),
-                         n(shape[0]), n(shape[1]), n(shape[2]), n(icgtyp), n(icgmax), C.byref(it))
+    dbuf = np.zeros(1)
+    wk = _work(shape)
+    L.vf_m1bcgs_(_pd(dbuf), *[_pd(a) for a in ms], _pd(x), *[_pd(w) for w in wk], d(cgpara),
+                 d(cgepsa), d(cgepsr), d(cgdiv), C.byref(bn), C.byref(xn), _pi(ic), n(shape[0]),
+                 n(shape[1]), n(shape[2]), n(nijk[0]), n(nijk[1]), n(nijk[2]), n(icgtyp),
+                 n(icgmax), C.byref(it))
     return x, bn.value, xn.value, it.value
+
+
+def vpsol(mat, indc, cgpara=0.95, cgepsa=1.0e-15, cgepsr=1.0e-12, cgdiv=1.0e30, icgtyp=1,
+          icgmax=500):
+    """VF_VPSOL through the drop-in vf_vpsol_ (vf_vpsol.f:1-3): sizes and
+    solver parameters go in through the COMMON blocks, CGBNRM, CGXNRM and
+    ICGITR come back there.  Returns PT, CGBNRM, CGXNRM, ICGITR."""
+    L = lib()
+    shape = mat["AD"].shape
+    pr, pi, nb = common_blocks()
+    nb.numi, nb.numj, nb.numk = shape
+    pr.cgpara, pr.cgepsa, pr.cgepsr, pr.cgdiv = cgpara, cgepsa, cgepsr, cgdiv
+    pi.icgtyp, pi.icgmax, pi.icgitr = icgtyp, icgmax, 0
+    pr.cgbnrm = pr.cgxnrm = 0.0
+    ms = [_f64(mat[k], shape) for k in MATRIX]
+    pt = np.full(shape, np.nan, order="F")      # VF_VPSOL zeroes it first
+    ic = _i32(indc, shape)
+    dbuf = np.zeros(1)
+    wk = _work(shape)
+    L.vf_vpsol_(_pd(dbuf), *[_pd(a) for a in ms], _pd(pt), *[_pd(w) for w in wk], _pi(ic))
+    return pt, pr.cgbnrm, pr.cgxnrm, pi.icgitr

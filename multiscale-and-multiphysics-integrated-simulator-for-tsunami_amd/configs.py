@@ -238,7 +238,9 @@ def buildings_tank(ni=512, nj=256, nk=128, d=0.02, h=0.8, H_over_h=0.15, nbuild=
     lines += _grid_lines("Y", _coords(nj, d))
     lines += _grid_lines("Z", _coords(nk, d))
     lines += ["B.C. D VP SLIP", "B.C. D F FREE", f"B.C. X 1 1 1 1 {nj} {nk} VP FREE"]
-    x0 = min(6.0, 0.5 * ni * d)
+    # the run-up case's beach toe at x = 6 m on tanks long enough to hold it
+    # (the 10.24 m tank of 512 cells); shorter test tanks start it half-way
+    x0 = 6.0 if ni * d >= 9.0 else 0.5 * ni * d
     _beach_obst(lines, xs, d, ni, nj, x0=x0, slope=1.0 / 20.0, zcap=(nk - 4) * d)
     for _ in range(nbuild):
         fx, fy = rng.integers(4, 17, size=2)
@@ -448,12 +450,54 @@ def damp3d(steps=60, name=None):
     return c
 
 
+def bore_tank(ni=1024, nj=512, nk=192, d=0.02, h_up=0.6, h_down=0.1, x_gate=4.0, nbuild=120,
+              seed=0, bw=(8.0, 8.6), bw_top=0.8, steps=100, dt=2.0e-4, eps_r=1.0e-10, name=None):
+    """A dam-break bore (water 0.6 m deep for x < 4 m over 0.1 m downstream)
+    running into a full-width porous breakwater (x in [8, 8.6] m, up to
+    0.8 m, porosity 0.45, CM 1.2, CD 1.0) and a seeded array of rectangular
+    buildings behind it (footprints 6..24 cells, heights 10..60 cells, on
+    the grid's last half), in a closed tank.
+
+    The initial water is deck-expressible: MATE W-LEVEL for the downstream
+    depth and DEBUG F-BOX for the upstream column, whose faces lie on grid
+    lines (x_gate and h_up are whole cells), so F is exactly 0/1 and the
+    reference can run the deck as it is.  The pressure is VF_CINIT's
+    RHO0*GRZ0*(W-LEVEL - z) on every active cell (vf_cinit.f:283).
+    """
+    rng = np.random.default_rng(seed)
+    lines = [f"# bore {ni}x{nj}x{nk} seed {seed}"]
+    lines += _grid_lines("X", _coords(ni, d))
+    lines += _grid_lines("Y", _coords(nj, d))
+    lines += _grid_lines("Z", _coords(nk, d))
+    lines += ["B.C. D VP SLIP", "B.C. D F FREE"]
+    i1 = int(round(bw[0] / d)) + 1
+    i2 = int(round(bw[1] / d))
+    kt = min(int(round(bw_top / d)), nk)
+    lines.append(f"POROUS V {i1} 1 1 {i2} {nj} {kt} 0.45 0.0")
+    lines.append(f"POROUS X {i1} 1 1 {i2 + 1} {nj} {kt} 0.45 0.0")
+    lines.append(f"POROUS Y {i1} 1 1 {i2} {nj + 1} {kt} 0.45 0.0")
+    lines.append(f"POROUS Z {i1} 1 1 {i2} {nj} {kt + 1} 0.45 0.0")
+    lines.append(f"POROUS CM {i1} 1 1 {i2} {nj} {kt} 1.2")
+    lines.append(f"POROUS CD {i1} 1 1 {i2} {nj} {kt} 1.0")
+    ilo = max(i2 + 4, ni // 2 + 1)
+    for _ in range(nbuild):
+        fx, fy = rng.integers(6, 25, size=2)
+        ht = int(rng.integers(10, 61))
+        ia = int(rng.integers(ilo, ni - fx))
+        ja = int(rng.integers(1, nj - fy))
+        lines.append(f"OBST {ia} {ja} 1 {ia + fx - 1} {ja + fy - 1} {min(ht, nk - 2)}")
+    _common(lines, dt, steps, eps_r, wlevel=h_down)
+    lines.append(f"DEBUG F-BOX 1 1 1 {int(round(x_gate / d))} {nj} {int(round(h_up / d))} 1.0")
+    return Case(name or f"bore_{ni}x{nj}x{nk}", "\n".join(lines) + "\n", ni, nj, nk, steps, dt)
+
+
 def named(name):
     """The workloads by name (sizes as in the benchmark table)."""
     table = {
         "dam_a": lambda: dam_break(),
         "runup_b": lambda: solitary_runup(),
         "buildings_c": lambda: buildings_tank(),
+        "bore_e": lambda: bore_tank(),
         "tiny": lambda: dam_break(8, 3, 6, 0.01, 0.03, 0.04, steps=5),
         "beach_small": lambda: beach_small(),
         "corner3d": lambda: corner3d(),

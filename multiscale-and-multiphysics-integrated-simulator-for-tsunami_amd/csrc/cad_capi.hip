@@ -822,6 +822,26 @@ int cad_count_active(cad_ctx *c, long long *active)
     return CAD_OK;
 }
 
+int cad_set_flag_pass_limit(cad_ctx *c, long long passes)
+{
+    if (!c || passes < 0) {
+        cad_set_error("cad_set_flag_pass_limit: a context and passes >= 0");
+        return CAD_ERR_ARG;
+    }
+    c->flag_pass_limit = passes;
+    return CAD_OK;
+}
+
+int cad_set_true_residual_stop(cad_ctx *c, double rel)
+{
+    if (!c || !(rel >= 0.0)) {
+        cad_set_error("cad_set_true_residual_stop: a context and rel >= 0");
+        return CAD_ERR_ARG;
+    }
+    c->true_stop = rel;
+    return CAD_OK;
+}
+
 int cad_set_sweep_schedule(cad_ctx *c, int schedule)
 {
     if (!c || schedule < CAD_SWEEP_PIPELINED || schedule > CAD_SWEEP_PIPELINED_16X4) {
@@ -896,62 +916,6 @@ int cad_download_pt(cad_ctx *c, double *pt)
     RET_IF(d2h(c, pt, c->pt, (size_t)c->d.nc));
     CAD_CHECK(hipStreamSynchronize(c->stream));
     return CAD_OK;
-}
-
-// Fortran-callable VF_M1BCGS replacement (vf_m1bcgs.f:1-5 argument order,
-// DBUF and the eight work arrays omitted).
-static cad_ctx *g_solver_ctx = nullptr;
-
-void cadmas_hip_m1bcgs_(const double *ad, const double *ali, const double *alj, const double *alk,
-                        const double *aui, const double *auj, const double *auk, const double *bb,
-                        double *x0, const double *cgpara, const double *cgepsa,
-                        const double *cgepsr, const double *cgdiv, double *bnorm, double *xnorm,
-                        const int *indc, const int *mi, const int *mj, const int *mk,
-                        const int *ni, const int *nj, const int *nk, const int *icgtyp,
-                        const int *icgmax, int *iter)
-{
-    *iter = -1;
-    *bnorm = 0.0;
-    *xnorm = 0.0;
-    if (*mi != *ni || *mj != *nj || *mk != *nk) {
-        cad_set_error("cadmas_hip_m1bcgs: MI/MJ/MK must equal NI/NJ/NK");
-        *iter = -3;
-        return;
-    }
-    if (g_solver_ctx && (g_solver_ctx->d.ni != *ni || g_solver_ctx->d.nj != *nj ||
-                         g_solver_ctx->d.nk != *nk)) {
-        cad_destroy(g_solver_ctx);
-        g_solver_ctx = nullptr;
-    }
-    if (!g_solver_ctx && cad_create(*ni, *nj, *nk, 0, 0, &g_solver_ctx) != CAD_OK) {
-        *iter = -3;
-        return;
-    }
-    cad_ctx *c = g_solver_ctx;
-    c->prm.cgpara = *cgpara;
-    c->prm.cgepsa = *cgepsa;
-    c->prm.cgepsr = *cgepsr;
-    c->prm.cgdiv = *cgdiv;
-    c->prm.icgtyp = *icgtyp;
-    c->prm.icgmax = *icgmax;
-    if (cad_upload_matrix(c, ad, ali, alj, alk, aui, auj, auk, bb, indc) != CAD_OK ||
-        h2d(c, c->pt, x0, (size_t)c->d.nc) != CAD_OK) {
-        *iter = -3;
-        return;
-    }
-    double bn = 0.0, xn = 0.0;
-    int it = -1;
-    if (cad_solve_m1bcgs(c, c->pt, &bn, &xn, &it) != CAD_OK) {
-        *iter = -3;
-        return;
-    }
-    if (d2h(c, x0, c->pt, (size_t)c->d.nc) != CAD_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
-        *iter = -3;
-        return;
-    }
-    *bnorm = bn;
-    *xnorm = xn;
-    *iter = it;
 }
 
 }  // extern "C"

@@ -186,6 +186,12 @@ struct cad_ctx {
     bool have_static, have_state;
     // the surface-cell count of the last flagging pass
     int nums;
+    // cap on VF_FNFINI's flag-fix passes (0: interior cells + 1, a bound no
+    // valid state reaches); cad_set_flag_pass_limit
+    long long flag_pass_limit;
+    // the plain solve's extra stop on the true relative residual (0: the
+    // reference's stop test alone); cad_set_true_residual_stop
+    double true_stop;
     // TimerDoor models (cad_tdoor.hip), allocated on first use: droplet
     // fields DROPTX..DROPWW, per-line droplet flags, the level mask of the
     // bubble timers, counters and their pinned mirror; TBUB(NUMK) and the

@@ -194,7 +194,8 @@ __global__ void k_bkwd_plane(Dims d, Plane pl, const double *__restrict__ dd,
 // fixed grid-stride order; cells outside it are inactive everywhere.  The
 // BiCGSTAB scalars live in a small device array (slots SC_*), so the only
 // value the host reads per iteration is (R,R) for the stop test.
-enum { SC_S = 0, SC_W1, SC_ALPHA, SC_W1B, SC_W2, SC_OMEGA, SC_XN, SC_W1C, SC_BETA, SC_BN, SC_N };
+enum { SC_S = 0, SC_W1, SC_ALPHA, SC_W1B, SC_W2, SC_OMEGA, SC_XN, SC_W1C, SC_BETA, SC_BN, SC_N,
+       SC_TRR = 20, SC_TBB = 21 };
 enum { RED_PLAIN = 0, RED_ALPHA, RED_OMEGA, RED_BETA };
 constexpr int VEC_BLOCKS = 2048;
 
@@ -693,7 +694,28 @@ int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_ou
     RET_IF_(fetch_sc(c, SC_BN, 1, &init[1]));
     xnorm = init[0];
     bnorm = init[1];
-    if (!(xnorm <= bnorm * epsr2 + epsa2)) {
+    // optional second stop condition: the true relative residual
+    // ||BB - A XW|| / ||BB|| at most c->true_stop (no counterpart in the
+    // reference; holds the plain form to the residual the other forms stop on)
+    const bool true_stop = c->true_stop > 0.0;
+    double bb2 = 0.0;
+    if (true_stop) {
+        RET_IF_(dot_to(c, c->bb, c->bb, SC_TBB, RED_PLAIN));
+        RET_IF_(fetch_sc(c, SC_TBB, 1, &bb2));
+    }
+    auto true_ok = [&](bool *ok) -> int {
+        RET_IF_(apply_a(c, c->xw, c->yv));
+        hipLaunchKernelGGL(k_bminus, vg, sb, 0, st, rng, mask, c->bb, c->yv);
+        CAD_CHECK(hipGetLastError());
+        RET_IF_(dot_to(c, c->yv, c->yv, SC_TRR, RED_PLAIN));
+        double rr2 = 0.0;
+        RET_IF_(fetch_sc(c, SC_TRR, 1, &rr2));
+        *ok = rr2 <= c->true_stop * c->true_stop * bb2;
+        return CAD_OK;
+    };
+    bool done = xnorm <= bnorm * epsr2 + epsa2;
+    if (done && true_stop) RET_IF_(true_ok(&done));
+    if (!done) {
         hipLaunchKernelGGL(k_init_rp, vg, sb, 0, st, rng, mask, c->r0, c->rr, c->pv);
         for (;;) {
             iter = iter + 1;
@@ -711,7 +733,12 @@ int cad_solve_m1bcgs(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_ou
             // (R,R) -> SC_XN, (R0,R) -> SC_W1C; beta and the new s on the device
             RET_IF_(reduce_to(c, 2, SC_XN, RED_BETA));
             RET_IF_(fetch_sc(c, SC_XN, 1, &xnorm));
-            if (xnorm <= bnorm * epsr2 + epsa2) break;
+            if (xnorm <= bnorm * epsr2 + epsa2) {
+                if (!true_stop) break;
+                bool ok = false;
+                RET_IF_(true_ok(&ok));
+                if (ok) break;
+            }
             if (xnorm > div2) {
                 iter = -2;
                 break;

@@ -707,7 +707,14 @@ int cad_fnfini(cad_ctx *c)
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_nf_reset, g, b, 0, st, d, f.nf);
     hipLaunchKernelGGL(k_nf_gas, g, b, 0, st, d, s, f.ff, f.bcf, f.nf, c->prm.flower, c->prm.fupper);
-    for (int it = 0; it < 100000; it++) {
+    // Every pass that finds a cell to fix turns at least one non-gas cell into
+    // gas, and no pass turns gas back, so the loop ends within (interior
+    // cells + 1) passes.  c->flag_pass_limit (0: that bound) caps it; a count
+    // still positive after the cap is an error, not a silent exit.
+    const long long bound = (long long)(d.ni - 2) * (d.nj - 2) * (d.nk - 2) + 1;
+    const long long cap = c->flag_pass_limit > 0 ? c->flag_pass_limit : bound;
+    int left = 0;
+    for (long long it = 0; it < cap; it++) {
         hipLaunchKernelGGL(k_nf_surface, g, b, 0, st, d, f.nf);
         hipLaunchKernelGGL(k_nf_nofluid, g, b, 0, st, d, f.nf);
         hipLaunchKernelGGL(k_iset, dim3(1), dim3(1), 0, st, c->icount + 2, 0);
@@ -715,8 +722,14 @@ int cad_fnfini(cad_ctx *c)
         CAD_CHECK(hipGetLastError());
         CAD_CHECK(hipMemcpyAsync(c->host_cnt + 2, c->icount + 2, sizeof(int), hipMemcpyDeviceToHost, st));
         CAD_CHECK(hipStreamSynchronize(st));
-        if (c->host_cnt[2] <= 0) break;
+        left = c->host_cnt[2];
+        if (left <= 0) break;
         hipLaunchKernelGGL(k_nf_fix, g, b, 0, st, d, f.nf);
+    }
+    if (left > 0) {
+        cad_set_error("VF_FNFINI: %d surface cells still not between fluid and gas after %lld "
+                      "flag passes", left, cap);
+        return CAD_ERR_STATE;
     }
     return CAD_OK;
 }

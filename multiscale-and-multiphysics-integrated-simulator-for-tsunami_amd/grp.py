@@ -14,7 +14,7 @@ import struct
 
 import numpy as np
 
-from .rsl import VERSION, _f64, _i32, _put
+from .rsl import VERSION, _f64, _get, _i32, _put
 
 
 def _box(shape, area):
@@ -119,3 +119,60 @@ class Writer:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def read(path):
+    """Parse a .grp file written by VF_OG1INI / VF_OG1TRN (or Writer): returns
+    (header, steps).  header holds the file's version, cell counts, extent,
+    output box (I1,J1,K1,I2,J2,K2), boundary counts (NBX,NBY,NBZ), the ten
+    output switches, the grid coordinates, the boundary cell numbers and GGV
+    over the box; each step holds NNOW, TNOW, NF, UU, VV, WW, PP, FF over the
+    box (the velocities with their extra face) and, when the box has
+    boundary faces, BCU, BCV, BCW, BCP, BCF in graphics order."""
+    with open(path, "rb") as f:
+        rec = []
+        while True:
+            r = _get(f)
+            if r is None:
+                break
+            rec.append(r)
+    f64 = lambda b: np.frombuffer(b, "<f8")  # noqa: E731
+    i32 = lambda b: np.frombuffer(b, "<i4")  # noqa: E731
+    h = {"VERSION": i32(rec[0]), "N": i32(rec[1]), "XMIN": f64(rec[2]), "XMAX": f64(rec[3]),
+         "BOX": i32(rec[4]), "NB": i32(rec[5]), "SWITCH": i32(rec[6]),
+         "X": f64(rec[7]), "Y": f64(rec[8]), "Z": f64(rec[9])}
+    i1, j1, k1, i2, j2, k2 = (int(v) for v in h["BOX"])
+    nb = int(h["NB"].sum())
+    q = 10
+    if nb:
+        h["IJK"] = i32(rec[q])
+        q += 1
+    dims = (i2 - i1 + 1, j2 - j1 + 1, k2 - k1 + 1)
+    h["GGV"] = f64(rec[q]).reshape(dims, order="F")
+    q += 1
+    steps = []
+    while q < len(rec):
+        st = {}
+        nnow, tnow = struct.unpack("<id", rec[q])
+        st["NNOW"], st["TNOW"] = nnow, tnow
+        st["NF"] = i32(rec[q + 1]).reshape(dims, order="F")
+        q += 2
+        for key, di in (("UU", (1, 0, 0)), ("VV", (0, 1, 0)), ("WW", (0, 0, 1))):
+            st[key] = f64(rec[q]).reshape(tuple(a + b for a, b in zip(dims, di)), order="F")
+            q += 1
+        if nb:
+            for key in ("BCU", "BCV", "BCW"):
+                st[key] = f64(rec[q])
+                q += 1
+        st["PP"] = f64(rec[q]).reshape(dims, order="F")
+        q += 1
+        if nb:
+            st["BCP"] = f64(rec[q])
+            q += 1
+        st["FF"] = f64(rec[q]).reshape(dims, order="F")
+        q += 1
+        if nb:
+            st["BCF"] = f64(rec[q])
+            q += 1
+        steps.append(st)
+    return h, steps

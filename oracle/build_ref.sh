@@ -66,3 +66,31 @@ done
 "$FC" -o "$OUT/cadmas_ref.x" "$OBJ/mod_comm.o" "$OBJ/mod_fault.o" "$OBJ/mod_apara.o" $OBJS \
     "$OBJ/ref_dump.o" -L"$MPILIB" -lmpifort -lmpi -Wl,-rpath,"$MPILIB"
 echo "built $OUT/cadmas_ref.x"
+
+# The reference relinked with the gfx950 library in place of its own solve
+# routines (link-level drop-in, no reference file changed): the product's
+# vf_m1bcgs_ (cadmas_ref_m1bcgs_hip.x) or vf_vpsol_ (cadmas_ref_vpsol_hip.x)
+# is what the reference's own calls bind to.  -rdynamic exports the
+# reference's COMMON blocks and VF_A2ERR to the library.  Built only when the
+# library exists; the tests run these on the GPU.
+PKG="$(cd "$HERE/.." && pwd)/multiscale-and-multiphysics-integrated-simulator-for-tsunami_amd"
+if [ -f "$PKG/libcadmas_hip.so" ]; then
+    for v in m1bcgs vpsol; do
+        case "$v" in
+            m1bcgs) drop="vf_m1bcgs" ;;
+            vpsol) drop="vf_vpsol vf_m1bcgs" ;;
+        esac
+        DOBJS=""
+        for f in $SRCS; do
+            b="${f%.f}"
+            skip=0
+            for x in $drop; do [ "$b" = "$x" ] && skip=1; done
+            [ $skip = 1 ] || DOBJS="$DOBJS $OBJ/$b.o"
+        done
+        "$FC" -o "$OUT/cadmas_ref_${v}_hip.x" "$OBJ/mod_comm.o" "$OBJ/mod_fault.o" "$OBJ/mod_apara.o" \
+            $DOBJS -L"$PKG" -lcadmas_hip -rdynamic \
+            -Wl,-rpath,'$ORIGIN/../../multiscale-and-multiphysics-integrated-simulator-for-tsunami_amd' \
+            -L"$MPILIB" -lmpifort -lmpi -Wl,-rpath,"$MPILIB"
+        echo "built $OUT/cadmas_ref_${v}_hip.x"
+    done
+fi

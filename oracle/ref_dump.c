@@ -61,12 +61,30 @@ static int step_selected(int step)
     return 0;
 }
 
+/* CADMAS_DUMP_TAGS="f1_out,static": only these records (the per-step "dv"
+ * record is always written); unset: every record. */
+static int tag_selected(const char *tag)
+{
+    const char *s = getenv("CADMAS_DUMP_TAGS");
+    if (!s || !*s || strcmp(tag, "dv") == 0) return 1;
+    size_t n = strlen(tag);
+    for (const char *p = s; *p;) {
+        const char *e = strchr(p, ',');
+        size_t m = e ? (size_t)(e - p) : strlen(p);
+        if (m == n && strncmp(p, tag, n) == 0) return 1;
+        if (!e) break;
+        p = e + 1;
+    }
+    return 0;
+}
+
 static int open_dump(const char *tag)
 {
     const char *dir = getenv("CADMAS_DUMP_DIR");
     if (!dir || !*dir) return 0;
     if (!step_selected(vf_atimei_.nnow) && strcmp(tag, "static") != 0 && strcmp(tag, "dv") != 0)
         return 0;
+    if (!tag_selected(tag)) return 0;
     char path[4096];
     if (strcmp(tag, "static") == 0)
         snprintf(path, sizeof path, "%s/static.bin", dir);
@@ -280,6 +298,13 @@ void vf_cdiv00_(P val, P xx, P yy, P zz, P uu, P vv, P ww, P ggx, P ggy, P ggz, 
         /* the step size and time of step NNOW, full precision */
         sd("DTNOW", vf_atimer_.dtnow);
         sd("TNOW", vf_atimer_.tnow);
+        /* the volume sums and the solve of the step that produced this
+         * state (the list line's FSUM, FCUT, !B!, !R!, ITR), full precision */
+        sd("FSUM", vf_acompr_.fsum);
+        sd("FCUT", vf_acompr_.fcut);
+        sd("CGBNRM", vf_acompr_.cgbnrm);
+        sd("CGXNRM", vf_acompr_.cgxnrm);
+        si("ICGITR", vf_acompi_.icgitr);
         close_dump();
     }
 }

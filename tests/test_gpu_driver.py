@@ -39,9 +39,26 @@ def test_run_writes_the_reference_outputs(pkg, tmp_path):
             assert_close(f"step {s} {k}", a[k], b[k], 1e-10)
         assert_same_int(f"step {s} NF", a["NF"], b["NF"])
     with np.load(fx.HERE / f"{case}_grp.npz") as z:
-        graw = z["GRP"].tobytes()
-    gmine = (tmp_path / "run.grp").read_bytes()
-    assert len(gmine) == len(graw)
+        (tmp_path / "ref.grp").write_bytes(z["GRP"].tobytes())
+    # the graphics file: its header exactly, each step's time and flags
+    # exactly, its fields to the same bars as the result file
+    gh, gsteps = pkg.grp.read(tmp_path / "run.grp")
+    rh, rsteps = pkg.grp.read(tmp_path / "ref.grp")
+    assert gh.keys() == rh.keys()
+    for k in gh:
+        assert np.array_equal(gh[k], rh[k]), f".grp header {k}"
+    assert len(gsteps) == len(rsteps) == 6
+    for a, b in zip(gsteps, rsteps):
+        s = b["NNOW"]
+        assert a.keys() == b.keys()
+        assert a["NNOW"] == s and a["TNOW"] == b["TNOW"]
+        assert_same_int(f".grp step {s} NF", a["NF"], b["NF"])
+        assert_vel_close(f".grp step {s} velocity", a, b, 1e-10)
+        if "BCU" in b:
+            assert_vel_close(f".grp step {s} boundary velocity", a, b, 1e-10, ("BCU", "BCV", "BCW"))
+        for k in ("PP", "FF", "BCP", "BCF"):
+            if k in b:
+                assert_close(f".grp step {s} {k}", a[k], b[k], 1e-10)
     lines = (tmp_path / "run.list").read_text().splitlines()
     ref_lines = [ln for x in fx.trajectory(case)["LIST"] for ln in str(x).splitlines()]
     assert len(lines) == len(ref_lines)

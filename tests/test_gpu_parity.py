@@ -200,3 +200,24 @@ def test_free_run_against_reference(pkg, case, last):
     for k in ("PP", "FF"):
         assert_close(f"step {last} {k}", out[k], ref[k], 1e-8)
     assert_same_int(f"step {last} NF", out["NF"], ref["NF"])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_true_residual_stop(pkg, case):
+    """With cad_set_true_residual_stop(rel) the plain solve runs the
+    reference's iteration until its preconditioned test passes and the FP64
+    true residual is at most rel: at least the reference's ITR, the true
+    residual within the bar, PT still within 1e-8 of the reference's."""
+    st = fx.static(case)
+    for s in fx.steps(case)[:2]:
+        r = fx.step(case, s)
+        with _ctx(pkg, case, st, r["v1_in"]) as c:
+            c.upload_state(_state(r["v1_in"], r["f1_in"]["BCF"]))
+            c.upload_matrix(r["ps_in"], r["ps_in"]["INDC"])
+            c.set_true_residual_stop(1e-11)
+            info = c.vpsol()
+            res = c.poisson_residual()
+            pt = c.download_pt()
+        assert info["icgitr"] >= int(r["ps_out"]["ICGITR"][0])
+        assert res <= 1e-11, res
+        assert_close(f"{case} step {s} PT", pt, r["ps_out"]["PT"], 1e-8)
