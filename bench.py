@@ -7,13 +7,18 @@ preconditioned BiCGSTAB stopped at a relative preconditioned residual of 1e-8
 and a true relative residual of 1e-8.  A "step" is one SMAC time step (predictor, Poisson solve, correction,
 VOF advection and flagging) over the whole grid; fields stay resident in HBM.
 
-Both forms run on the same workload from the same initial state (--form
-selects one): the plain FP64 form (the reference's MILU-BiCGSTAB, stopped on
-its preconditioned residual) gives value, roofline and the step figures; the
-mixed form (FP64 BiCGSTAB with an FP32 aggregation-multigrid preconditioner,
-stopped on the true residual ||BB - A PT|| <= 1e-8 ||BB||) is reported under
-"mixed" with its speed-up and its fields' differences from the plain form's
-after the same steps.
+The forms run on the same workload from the same initial state (--form
+selects them; the default "all" runs the learned form when its weights file
+exists): the plain FP64 form (the reference's MILU-BiCGSTAB, stopped on its
+preconditioned residual and held to a true relative residual of 1e-8) gives
+value, roofline and the step figures; the mixed form (FP64 iterative
+refinement around an FP32 BiCGSTAB with FP32 stencil matvecs and an FP32
+aggregation-multigrid preconditioner, FP64 dots, stopped on the true residual
+||BB - A PT|| <= 1e-8 ||BB||) and the learned form (FP64 flexible GMRES
+preconditioned by one FP32 V-cycle and the BF16 3-D U-Net, MILU-BiCGSTAB as
+the fallback, same stop) are reported under "mixed" and "learned" with their
+speed-ups and their fields' differences from the plain form's after the same
+steps.
 
 value  = interior cells x timed steps / wall time of the timed steps (each step
          timed on its own; the true residual of its solve, checked after each
@@ -26,7 +31,11 @@ roofline = (plain form) the MILU preconditioner application M^-1 (forward + back
            x backward) = 96 B per active cell per application, divided by its
            average duration timed with HIP events on the library's stream.
 mixed.roofline = one FP32 multigrid V-cycle, timed the same way, against its
-           algorithmic bytes from the level sizes (cad_time_precond).
+           algorithmic bytes from the level sizes (cad_time_precond); beside
+           it the same V-cycle in FP64 (cad_time_vcycle), what FP32 buys.
+learned.roofline = one forward pass of the U-Net on the solve's box, timed the
+           same way, against the BF16 dense MFMA peak (2.5 PFLOP/s), 2 FLOP
+           per multiply-add (cad_time_learned).
 cpu_baseline = the reference itself (oracle/_ref/cadmas_ref.x, compiled from
            its Fortran sources) on one host core: one SMAC step of the
            512x64x128 slab of the same tank (same beach, wave and cell size, 10
@@ -54,6 +63,7 @@ ROOT = Path(__file__).resolve().parent
 PKG_DIR = ROOT / "multiscale-and-multiphysics-integrated-simulator-for-tsunami_amd"
 METRIC = "SMAC cell-updates/s + Poisson ms/step at 1e-8 resid: learned/mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 TRUE_RESIDUAL = 1.0e-8
 
 
@@ -77,6 +87,10 @@ def workload(pkg, name):
         return cfg.solitary_runup(eps_r=1.0e-8)
     if name == "dam_a":
         return cfg.dam_break(eps_r=1.0e-8)
+    if name == "bore_e":
+        return cfg.bore_tank(eps_r=1.0e-8)
+    if name == "bore_e_slab":
+        return cfg.bore_tank(1024, 32, 192, nbuild=8, eps_r=1.0e-8, name="bore_e_slab")
     if name == "buildings_c_slab":
         return cfg.buildings_tank(512, 64, 128, nbuild=10, eps_r=1.0e-8, name="buildings_c_slab")
     if name.startswith("buildings_small"):
@@ -146,6 +160,9 @@ def run_form(pkg, lay, state, flags, form, args):
     ctx.set_params(**{k: v for k, v in p.items() if k in ctx.params})
     if form == "mixed":
         ctx.set_params(precision=pkg.capi.PRECISION_MIXED)
+    elif form == "learned":
+        ctx.set_params(precision=pkg.capi.PRECISION_LEARNED)
+        pkg.learned.set_weights(ctx, pkg.learned.load_weights(args.weights))
     else:
         # the plain form also stops only at a true relative residual of 1e-8
         # (rechecked whenever the reference's preconditioned test passes), the
@@ -178,13 +195,16 @@ def run_form(pkg, lay, state, flags, form, args):
     true_res = max(res)
     log(f"bench: {form} timed {args.steps} steps in {wall:.3f}s ITR={itr} worst true residual "
         f"{true_res:.3e}")
-    end_state = ctx.download_state() if args.form == "both" else None
-    r = {"form": "plain FP64" if form == "plain" else "mixed FP64/FP32",
-         "form_text": ("plain FP64 form, MILU-BiCGSTAB to 1e-8 relative preconditioned and true "
-                       "residual"
-                       if form == "plain" else
-                       "mixed form, FP64 BiCGSTAB with an FP32 aggregation-multigrid preconditioner "
-                       "to 1e-8 relative true residual"),
+    end_state = ctx.download_state() if args.form in ("both", "all") else None
+    names = {"plain": "plain FP64", "mixed": "mixed FP64/FP32", "learned": "learned BF16/FP32/FP64"}
+    texts = {"plain": "plain FP64 form, MILU-BiCGSTAB to 1e-8 relative preconditioned and true residual",
+             "mixed": "mixed form, FP64 iterative refinement around FP32 BiCGSTAB (FP32 stencil "
+                      "matvecs, FP32 aggregation-multigrid preconditioner, FP64 dots) to 1e-8 relative "
+                      "true residual",
+             "learned": "learned form, FP64 flexible GMRES preconditioned by an FP32 V-cycle and the "
+                        "BF16 3-D U-Net on the matrix cores, MILU-BiCGSTAB fallback, to 1e-8 relative "
+                        "true residual"}
+    r = {"form": names[form], "form_text": texts[form],
          "value": round(cells * args.steps / wall, 1), "ms_per_step": round(1e3 * wall / args.steps, 3),
          "poisson_ms_per_step": round(float(np.mean(poisson)), 3),
          "poisson_true_residual": float(f"{true_res:.3e}"),
@@ -205,6 +225,19 @@ def run_form(pkg, lay, state, flags, form, args):
                          "traffic": measured_traffic(args.workload),
                          "kernel": "MILU M^-1 sweeps (forward+backward), 96 B/active cell",
                          "avg_ms": round(minv_ms, 4), "active_cells": active}
+    elif form == "learned":
+        # the learned form's network: one forward pass against the BF16 MFMA peak
+        st = pkg.learned.stats(ctx)
+        net_ms, net_flop = ctx.time_learned(args.minv_reps)
+        achieved = net_flop / (net_ms * 1e-3) / 1e12
+        r["roofline"] = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 5),
+                         "traffic": None,
+                         "kernel": "3-D U-Net forward pass (BF16 MFMA, FP32 accumulation), "
+                                   f"{pkg.unet.param_count()} parameters, box depth {st['depth']}",
+                         "avg_ms": round(net_ms, 4), "flop": round(net_flop)}
+        r["fallbacks"] = st["fallbacks"]
+        r["fgmres_iters_total"] = st["total_iters"]
     else:
         # dominant kernel of the mixed form: one multigrid V-cycle
         vc_ms, vc_bytes = ctx.time_precond(args.minv_reps)
@@ -215,6 +248,10 @@ def run_form(pkg, lay, state, flags, form, args):
                                                      "multigrid_traffic_buildings_512x256x128.md"),
                          "kernel": "FP32 multigrid V-cycle (the mixed form's preconditioner)",
                          "avg_ms": round(vc_ms, 4), "bytes": round(vc_bytes)}
+        # the same V-cycle with every level in FP64: what FP32 buys
+        v64_ms, v64_bytes = ctx.time_vcycle(args.minv_reps, fp64=True)
+        r["vcycle_fp64"] = {"avg_ms": round(v64_ms, 4), "bytes": round(v64_bytes),
+                            "fp32_speedup": round(v64_ms / vc_ms, 3)}
     ctx.close()
     r["state"] = end_state
     return r
@@ -300,7 +337,9 @@ def main():
     ap.add_argument("--minv-reps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--form", choices=("plain", "mixed", "both"), default="both")
+    ap.add_argument("--form", choices=("plain", "mixed", "learned", "both", "all"), default="all",
+                    help="both: plain and mixed; all: also the learned form when its weights exist")
+    ap.add_argument("--weights", default=None, help="the learned form's network parameters (.npy)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -322,7 +361,14 @@ def main():
     lay, state, flags = prepare(pkg, case)
     log(f"bench: {case.name} {lay.deck.ncells} layout {time.perf_counter() - t0:.1f}s")
     cells = int(np.prod(lay.deck.ncells))
-    forms = ("plain", "mixed") if args.form == "both" else (args.form,)
+    if args.weights is None:
+        args.weights = str(pkg.learned.DEFAULT_WEIGHTS)
+    if args.form == "all":
+        forms = ("plain", "mixed") + (("learned",) if Path(args.weights).exists() else ())
+    elif args.form == "both":
+        forms = ("plain", "mixed")
+    else:
+        forms = (args.form,)
     res = {f: run_form(pkg, lay, state, flags, f, args) for f in forms}
     head = res["plain"] if "plain" in res else res[forms[0]]
     out = {
@@ -355,7 +401,7 @@ def main():
         mx = res["mixed"]
         out["mixed"] = {k: mx[k] for k in ("value", "ms_per_step", "poisson_ms_per_step",
                                            "poisson_true_residual", "itr_per_step", "phase_ms",
-                                           "form_text", "roofline")}
+                                           "form_text", "roofline", "vcycle_fp64")}
         if "plain" in res:
             out["mixed"]["field_error_vs_plain"] = field_errors(mx["state"], res["plain"]["state"])
             out["mixed"]["speedup_vs_plain"] = round(mx["value"] / res["plain"]["value"], 3)
@@ -363,8 +409,22 @@ def main():
                 res["plain"]["poisson_ms_per_step"] / mx["poisson_ms_per_step"], 3)
         if "roofline" not in out:
             out["roofline"] = None
+    if "learned" in res:
+        ln = res["learned"]
+        out["learned"] = {k: ln[k] for k in ("value", "ms_per_step", "poisson_ms_per_step",
+                                             "poisson_true_residual", "itr_per_step", "phase_ms",
+                                             "form_text", "roofline", "fallbacks", "fgmres_iters_total")}
+        out["learned"]["weights"] = Path(args.weights).name
+        if "plain" in res:
+            out["learned"]["field_error_vs_plain"] = field_errors(ln["state"], res["plain"]["state"])
+            out["learned"]["speedup_vs_plain"] = round(ln["value"] / res["plain"]["value"], 3)
+            out["learned"]["poisson_speedup_vs_plain"] = round(
+                res["plain"]["poisson_ms_per_step"] / ln["poisson_ms_per_step"], 3)
+        if "roofline" not in out:
+            out["roofline"] = None
     if not args.no_cpu_baseline:
-        sample = workload(pkg, "buildings_c_slab") if args.workload == "buildings_c" else case
+        slabs = {"buildings_c": "buildings_c_slab", "bore_e": "bore_e_slab"}
+        sample = workload(pkg, slabs[args.workload]) if args.workload in slabs else case
         slay, sstate, sflags = prepare(pkg, sample)
         scells = int(np.prod(slay.deck.ncells))
         v, dt, it = cpu_baseline(pkg, slay, sstate, sflags, steps=args.cpu_steps)

@@ -36,6 +36,9 @@ extern "C" {
 #define CAD_PRECISION_FP64 0          /* plain form: all FP64 */
 #define CAD_PRECISION_MIXED 1         /* FP64 BiCGSTAB with an FP32 multigrid preconditioner,
                                          stopped on the true residual (cad_vpsol) */
+#define CAD_PRECISION_LEARNED 2       /* FP64 flexible GMRES preconditioned by an FP32 V-cycle and
+                                         the BF16 U-Net, MILU-BiCGSTAB as the fallback, stopped
+                                         on the true residual (cad_vpsol) */
 
 typedef struct cad_ctx cad_ctx;
 
@@ -366,6 +369,71 @@ void vf_vpsol_(double *dbuf, const double *ad, const double *ali, const double *
                const double *alk, const double *aui, const double *auj, const double *auk,
                const double *bb, double *pt, double *dd, double *xw, double *yy, double *r0,
                double *rr, double *pp, double *ap, double *ae, const int *indc);
+
+/* The mixed and learned forms --------------------------------------------
+ * No counterparts in the reference (its only solver is VF_M1BCGS).  Both
+ * stop on the true relative residual ||BB - A PT|| / ||BB|| <= CGEPSR and,
+ * when their own iteration stalls (no convergence within the cap, 200 by
+ * default and at most ICGMAX, or for the learned form a restart cycle that
+ * does not halve the residual), hand the solve to VF_M1BCGS from their
+ * current iterate with the same true-residual stop.  cad_set_krylov_cap sets
+ * the cap (0: the default); cad_mixed_stats gives the mixed form's fallback
+ * count and the last fallback's BiCGSTAB iterations. */
+int cad_set_krylov_cap(cad_ctx *ctx, int cap);
+/* One multigrid V-cycle on the hierarchy and K range of the last mixed or
+ * learned solve, timed with HIP events on the context's stream, and its
+ * algorithmic bytes; fp64 != 0 runs the same V-cycle with every level in
+ * FP64 (a hierarchy of its own from the same matrix), to measure what FP32
+ * buys. */
+int cad_time_vcycle(cad_ctx *ctx, int reps, int fp64, float *ms_avg, double *bytes);
+int cad_mixed_stats(cad_ctx *ctx, long long *out2);
+
+/* The learned form's network: a 3-level 3-D U-Net, 3x3x3 convolutions with
+ * 16/32/64 channels, 8 input channels (the residual scaled by its RMS, GGV,
+ * F, the cell-type one-hot, the activity), one output (the correction on the
+ * same scale), cad_unet_param_count() = 390,913 FP32 parameters.  Weights and
+ * activations in BF16 on the matrix cores, FP32 accumulation.  cad_set_learned_
+ * weights gives a context its parameters (host FP32 vector); cad_set_learned_
+ * mode selects mode 1 (the default: one V-cycle, then the network on its
+ * residual) or 0 (the network alone) and the restart length (1..32, default
+ * 30).  cad_learned_stats: the last solve's FGMRES and fallback iterations,
+ * solves, fallbacks, all FGMRES iterations, the network box's depth.
+ * cad_learned_box: the network's box for the current flags (nx, ny, nz, and
+ * the first K plane).  cad_learned_sample: training pairs from the plain
+ * solver on the context's current system, for the iterates listed in iters
+ * (see learned.py); feat, target, mask are device buffers. */
+int cad_set_learned_weights(cad_ctx *ctx, const float *params, int n);
+int cad_set_learned_mode(cad_ctx *ctx, int mode, int restart);
+int cad_learned_stats(cad_ctx *ctx, long long *out6);
+int cad_learned_box(cad_ctx *ctx, int *box4);
+/* Average duration of the network's forward pass on the current box (HIP
+ * events on the context's stream) and its work, 2 FLOP per multiply-add. */
+int cad_time_learned(cad_ctx *ctx, int reps, float *ms_avg, double *flops);
+int cad_learned_sample(cad_ctx *ctx, int mode, const int *iters, int n, void *feat, float *target,
+                       float *mask);
+
+/* The network on its own (training and tests): a batch of nb boxes of
+ * nx x ny x nz voxels (nx a multiple of 64, ny and nz of 4); inputs are BF16
+ * (nb, nz+2, ny+2, nx+2, 8) with zero halos, outputs FP32 nb*nz*ny*nx (x
+ * fastest).  train != 0 also holds the gradients and Adam's moments.
+ * cad_unet_train_step: one Adam step on the loss mean over boxes of
+ * sum m (y-e)^2 / sum m e^2; cad_unet_gradients: that loss and its gradient.
+ * stream: a hipStream_t or NULL (the network's own). */
+typedef struct cad_unet cad_unet;
+int cad_unet_param_count(void);
+double cad_unet_mac_per_voxel(void);
+int cad_unet_init_params(int seed, float *params);
+int cad_unet_create(int nx, int ny, int nz, int nb, int train, void *stream, cad_unet **out);
+void cad_unet_destroy(cad_unet *net);
+int cad_unet_set_params(cad_unet *net, const float *params);
+int cad_unet_get_params(cad_unet *net, float *params);
+int cad_unet_set_input(cad_unet *net, const void *dev);
+int cad_unet_forward(cad_unet *net);
+int cad_unet_get_output(cad_unet *net, float *dev);
+int cad_unet_set_depth(cad_unet *net, int nz);
+int cad_unet_train_step(cad_unet *net, const float *target, const float *mask, float lr, float *loss);
+int cad_unet_gradients(cad_unet *net, const float *target, const float *mask, float *grad,
+                       float *loss);
 
 #ifdef __cplusplus
 }

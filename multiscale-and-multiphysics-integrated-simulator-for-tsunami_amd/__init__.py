@@ -12,7 +12,9 @@ from . import deck  # noqa: F401
 from . import driver  # noqa: F401
 from . import grp  # noqa: F401
 from . import layout  # noqa: F401
+from . import learned  # noqa: F401
 from . import mtb  # noqa: F401
 from . import rsl  # noqa: F401
 from . import timeloop  # noqa: F401
 from . import tran  # noqa: F401
+from . import unet  # noqa: F401

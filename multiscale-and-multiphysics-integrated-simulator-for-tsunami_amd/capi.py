@@ -20,6 +20,7 @@ PI = C.POINTER(C.c_int)
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_LEARNED = 2
 
 ERRORS = {-1: "ARG", -2: "HIP", -3: "STATE", -4: "UNSUPPORTED", -30: "NOT CONVERGED",
           -31: "INSTABILITY", -32: "CAN NOT DECOMPOSITION", -40: "ORIENTATION", -50: "WAVE TABLE"}
@@ -117,6 +118,11 @@ def lib():
     L.vf_vpsol_.restype = None
     L.cad_milu_factor.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.cad_milu_apply.argtypes = [C.c_void_p, PD, PD]
+    L.cad_set_krylov_cap.argtypes = [C.c_void_p, C.c_int]
+    L.cad_mixed_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    L.cad_time_vcycle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), PD]
+    L.cad_time_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), PD]
+    L.cad_unet_mac_per_voxel.restype = C.c_double
     _lib = L
     return L
 
@@ -131,7 +137,12 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
            "cad_set_drag_df", "cad_set_damp", "cad_set_wave_theory", "cad_wave_theory_eval",
            "cad_set_wave_table2", "cad_set_wave2", "cad_set_flag_pass_limit",
-           "cad_set_true_residual_stop")
+           "cad_set_true_residual_stop", "cad_set_krylov_cap",
+           "cad_mixed_stats", "cad_time_vcycle", "cad_set_learned_weights", "cad_set_learned_mode", "cad_learned_stats",
+           "cad_learned_box", "cad_learned_sample", "cad_time_learned", "cad_unet_param_count",
+           "cad_unet_mac_per_voxel", "cad_unet_init_params", "cad_unet_create", "cad_unet_destroy",
+           "cad_unet_set_params", "cad_unet_get_params", "cad_unet_set_input", "cad_unet_forward",
+           "cad_unet_get_output", "cad_unet_set_depth", "cad_unet_train_step", "cad_unet_gradients")
 
 class TheoryWave:
     """A theory wave evaluated on the host (cad_wave_theory_eval), with
@@ -440,6 +451,29 @@ class Context:
         reference's stop test alone)."""
         _check(lib().cad_set_true_residual_stop(self._h, float(rel)))
 
+    def set_krylov_cap(self, cap):
+        """Iterations of the mixed / learned forms before MILU-BiCGSTAB takes
+        over the solve (0: the default, 200)."""
+        _check(lib().cad_set_krylov_cap(self._h, int(cap)))
+
+    def mixed_stats(self):
+        """The mixed form's fallbacks so far and the last one's iterations."""
+        a = (C.c_longlong * 2)()
+        _check(lib().cad_mixed_stats(self._h, a))
+        return {"fallbacks": int(a[0]), "fallback_iters": int(a[1])}
+
+    def time_vcycle(self, reps=3, fp64=False):
+        """(ms, algorithmic bytes) of one multigrid V-cycle, FP32 or FP64."""
+        ms, by = C.c_float(), C.c_double()
+        _check(lib().cad_time_vcycle(self._h, int(reps), int(bool(fp64)), C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def time_learned(self, reps=3):
+        """(ms, FLOP) of one forward pass of the learned form's network."""
+        ms, fl = C.c_float(), C.c_double()
+        _check(lib().cad_time_learned(self._h, int(reps), C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
     def set_sweep_schedule(self, schedule):
         _check(lib().cad_set_sweep_schedule(self._h, int(schedule)))
 
@@ -541,3 +575,4 @@ def vpsol(mat, indc, cgpara=0.95, cgepsa=1.0e-15, cgepsr=1.0e-12, cgdiv=1.0e30, 
     wk = _work(shape)
     L.vf_vpsol_(_pd(dbuf), *[_pd(a) for a in ms], _pd(pt), *[_pd(w) for w in wk], _pi(ic))
     return pt, pr.cgbnrm, pr.cgxnrm, pi.icgitr
+

@@ -227,6 +227,7 @@ void cad_destroy(cad_ctx *c)
     cad_wave_free(c);
     cad_wave_theory_free(c);
     cad_mg_free(c);
+    cad_learned_free(c);
     delete[] c->zz_host;
     delete[] c->xx_host;
     delete[] c->yy_host;
@@ -248,7 +249,8 @@ int cad_set_params(cad_ctx *c, const cad_params *p)
         cad_set_error("OPTION SUB-LOOP %d: LOOPS must be >= 1 (vf_iiopt.f:44)", p->loops);
         return CAD_ERR_ARG;
     }
-    if (p->precision != CAD_PRECISION_FP64 && p->precision != CAD_PRECISION_MIXED) {
+    if (p->precision != CAD_PRECISION_FP64 && p->precision != CAD_PRECISION_MIXED &&
+        p->precision != CAD_PRECISION_LEARNED) {
         cad_set_error("precision mode %d is not available in this build", p->precision);
         return CAD_ERR_UNSUPPORTED;
     }
@@ -514,6 +516,8 @@ int cad_vpsol(cad_ctx *c, cad_step_info *info)
     int it = 0;
     if (c->prm.precision == CAD_PRECISION_MIXED)
         RET_IF(cad_solve_mixed(c, c->pt, &bn, &xn, &it));
+    else if (c->prm.precision == CAD_PRECISION_LEARNED)
+        RET_IF(cad_solve_learned(c, c->pt, &bn, &xn, &it));
     else
         RET_IF(cad_solve_m1bcgs(c, c->pt, &bn, &xn, &it));
     CAD_CHECK(hipStreamSynchronize(c->stream));
@@ -793,7 +797,32 @@ int cad_time_precond(cad_ctx *c, int reps, float *ms_avg, double *bytes)
         *bytes = 96.0 * (double)active;
         return CAD_OK;
     }
-    return cad_mg_time(c, reps, ms_avg, bytes);
+    return cad_mg_time(c, reps, 0, ms_avg, bytes);
+}
+
+int cad_time_vcycle(cad_ctx *c, int reps, int fp64, float *ms_avg, double *bytes)
+{
+    RET_IF(ready(c));
+    if (!ms_avg || !bytes) return CAD_ERR_ARG;
+    return cad_mg_time(c, reps, fp64, ms_avg, bytes);
+}
+
+int cad_set_krylov_cap(cad_ctx *c, int cap)
+{
+    if (!c || cap < 0) {
+        cad_set_error("cad_set_krylov_cap: cap %d (0: the default)", cap);
+        return CAD_ERR_ARG;
+    }
+    c->krylov_cap = cap;
+    return CAD_OK;
+}
+
+int cad_mixed_stats(cad_ctx *c, long long *out)
+{
+    if (!c || !out) return CAD_ERR_ARG;
+    out[0] = c->mixed_fallbacks;
+    out[1] = c->mixed_fallback_iters;
+    return CAD_OK;
 }
 
 int cad_poisson_residual(cad_ctx *c, double *rel)

@@ -126,16 +126,23 @@ inline dim3 cell_grid(const Dims &d)
 constexpr int REDUCE_BLOCKS = 1024;
 constexpr int REDUCE_THREADS = 256;
 
-// One level of the mixed form's multigrid (cad_mg.hip): a grid with one ghost
-// layer, its FP32 7-point rows (inactive rows AD = 1, no couplings), the
-// level's unknowns and right-hand side, and the row activity.
+// One level of the multigrid (cad_mg.hip): a grid with one ghost layer, its
+// 7-point rows (inactive rows AD = 1, no couplings), the level's unknowns and
+// right-hand side, and the row activity, stored colour by colour in lines of
+// ld cells; level 0 of the FP32 hierarchy also holds its rows and activity in
+// the grid's natural order (nad .. nauk, nact; null elsewhere).
 constexpr int CAD_MG_MAX = 12;
-struct MgLevel {
+template <typename V>
+struct MgLevelT {
     Dims d;
-    float *ad, *ali, *alj, *alk, *aui, *auj, *auk;
-    float *x, *b;
+    int ld;
+    V *ad, *ali, *alj, *alk, *aui, *auj, *auk;
+    V *x, *b;
     unsigned char *act;
+    V *nad, *nali, *nalj, *nalk, *naui, *nauj, *nauk;
+    unsigned char *nact;
 };
+typedef MgLevelT<float> MgLevel;
 
 // Device-side context of one grid (all fields HBM resident).
 struct Fields {
@@ -242,8 +249,20 @@ struct cad_ctx {
     // the mixed form's multigrid levels (allocated on first use)
     MgLevel mg[CAD_MG_MAX];
     int mg_n;
+    // an FP64 hierarchy of the same matrix, built only to time an FP64 V-cycle
+    MgLevelT<double> mg64[CAD_MG_MAX];
+    int mg64_n;
     unsigned long long mg_span;   // cells in the active index range of the current solve
     int mg_k0, mg_k1;             // K planes of level 0's passes in the current solve
+    // the learned form's network, Krylov basis and counters (cad_learned.hip),
+    // allocated when its weights are set
+    struct Learned *lrn;
+    // the mixed form's solves that stalled and were finished by MILU-BiCGSTAB,
+    // and the BiCGSTAB iterations of the last such solve
+    long long mixed_fallbacks;
+    int mixed_fallback_iters;
+    // cap on the mixed and learned forms' own iterations (0: 200)
+    int krylov_cap;
 };
 
 // ---- launchers implemented in the phase files ---------------------------
@@ -269,12 +288,15 @@ int cad_sweep_minv(cad_ctx *c, double *x, const double *y);
 int cad_sweep_status(cad_ctx *c, int *code);
 int cad_milu_factor_impl(cad_ctx *c, int *npos);
 int cad_poisson_residual_impl(cad_ctx *c, double *rel);
+int cad_poisson_residual_x(cad_ctx *c, const double *x, double *rel);
+int cad_krylov_cap(const cad_ctx *c);
 int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm, double *xnorm, int *iter);
 int cad_mg_alloc(cad_ctx *c);
 void cad_mg_free(cad_ctx *c);
 int cad_mg_setup(cad_ctx *c);
 int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1);
-int cad_mg_time(cad_ctx *c, int reps, float *ms_avg, double *bytes);
+int cad_mg_apply32(cad_ctx *c, const float *y, float *x, int k0, int k1);
+int cad_mg_time(cad_ctx *c, int reps, int fp64, float *ms_avg, double *bytes);
 int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x);
 Static cad_static(const cad_ctx *c);
 int cad_fnfini(cad_ctx *c);
@@ -296,3 +318,9 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
 int cad_wave_theory_apply(cad_ctx *c, int jd);
 void cad_wave_theory_free(cad_ctx *c);
 void cad_wave_free(cad_ctx *c);
+int cad_ext_active_range(cad_ctx *c, unsigned long long hr[2]);
+int cad_ext_apply_a(cad_ctx *c, const double *x, double *y);
+int cad_ext_residual(cad_ctx *c, const double *x, double *y);
+int cad_ext_dot(cad_ctx *c, const double *x, const double *y, double *out);
+int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm, double *xnorm, int *iter);
+void cad_learned_free(cad_ctx *c);

@@ -1,0 +1,796 @@
+// cad_learned.hip -- the learned form's pressure solve: flexible GMRES in FP64
+// on the same matrix and right-hand side as VF_M1BCGS (vf_m1bcgs.f:101-262),
+// preconditioned by the 3-D U-Net of cad_unet.hip, with the reference's
+// MILU-BiCGSTAB as the fallback when the Krylov residual stalls.
+//
+// Preconditioner.  One application to a vector v (mode 1, the default):
+//   z1 = V(v)                   one FP32 multigrid V-cycle (cad_mg.hip)
+//   r1 = v - A z1               FP64
+//   z  = z1 + s N(r1 / s)       s = RMS of r1 over the active cells
+// where N is the network (BF16 on the matrix cores, FP32 accumulation): it
+// maps a residual, with the porosity GGV, F and the cell type, to the
+// correction the V-cycle left out.  Mode 0 is the network alone, z = s N(v/s).
+// The network is nonlinear, hence flexible GMRES, which keeps the
+// preconditioned vectors Z beside the Arnoldi basis V.  Orthogonalisation is
+// classical Gram-Schmidt done twice, as batched dot products (one host read
+// per pass).  The stop test is on the true relative residual
+// ||BB - A x|| / ||BB|| <= CGEPSR, recomputed in FP64 at every restart.  A
+// cycle that does not halve the residual, or more than ICGMAX iterations,
+// hands over to VF_M1BCGS from the current x with the same true-residual stop.
+//
+// Net box.  Voxel (x, y, z) of the network is cell (I, J, K) = (x+2, y+2,
+// z+kb): the interior cells in I and J, and in K the planes from the first
+// that holds an active cell, rounded up to a multiple of 4; voxels outside
+// the grid are inactive.  Input channels: the scaled residual, GGV, F, the
+// cell type one-hot (obstacle, fluid, surface, gas) and the activity.
+#include "cad_common.h"
+#include "cad_unet.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int MAXM = 32;            // largest restart length
+constexpr int VBLK = 2048;          // blocks of the vector kernels
+
+}  // namespace
+
+struct Learned {
+    cad_unet *net;
+    int nx, ny, nzmax;              // the network's box
+    int mode;                       // 1: V-cycle then network; 0: network alone
+    int m;                          // restart length
+    double *vecs;                   // V (m+1), Z (m), then 6 scratch vectors
+    int nvec;
+    double *part;                   // REDUCE_BLOCKS x (MAXM+1) partial sums
+    double *hdev;                   // MAXM+1 coefficients
+    double *scal;                   // [0] s, [1] 1/s, [2] active count
+    double *hpin;                   // pinned host mirror (MAXM+1)
+    float *y;                       // network output
+    unsigned long long prev_rng[2]; // active range of the previous solve
+    int kb, nzr;                    // the current box placement
+    long long last_iters, last_fallback_iters, solves, fallbacks, total_iters;
+    double last_resid;
+};
+
+namespace {
+
+__device__ __forceinline__ long long uvox(const UGeo &g, int x, int y, int z)
+{
+    return (long long)(x + 1) + (long long)g.px * ((y + 1) + (long long)g.py * (z + 1));
+}
+
+// network input of the box from the residual r (scaled by scal[1]); plane nzr
+// cleared (it is the halo of the run)
+__global__ void k_features(Dims d, UGeo g, int kb, int nzr, const double *__restrict__ r,
+                           const double *__restrict__ scal, const double *__restrict__ ggv,
+                           const double *__restrict__ ff, const int *__restrict__ nf,
+                           const int *__restrict__ indc, bf16 *__restrict__ x)
+{
+    const long long total = (long long)g.nx * g.ny * (nzr + 1);
+    const double inv = scal[1];
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int xq = (int)(e % g.nx);
+        const int yq = (int)((e / g.nx) % g.ny);
+        const int zq = (int)(e / ((long long)g.nx * g.ny));
+        bf16x8 o;
+#pragma unroll
+        for (int c = 0; c < 8; c++) o[c] = (bf16)0.f;
+        const int i = xq + 2, j = yq + 2, k = zq + kb;
+        if (zq < nzr && i <= d.ni - 1 && j <= d.nj - 1 && k <= d.nk - 1) {
+            const long long n = cid(d, i, j, k);
+            const int t = nf[n];
+            const bool act = indc[n] != -1;
+            o[0] = (bf16)(act ? (float)(r[n] * inv) : 0.f);
+            o[1] = (bf16)(float)ggv[n];
+            o[2] = (bf16)(float)ff[n];
+            o[3] = (bf16)(t == -1 ? 1.f : 0.f);
+            o[4] = (bf16)(t == 0 ? 1.f : 0.f);
+            o[5] = (bf16)(t >= 1 && t <= 6 ? 1.f : 0.f);
+            o[6] = (bf16)(t >= 7 ? 1.f : 0.f);
+            o[7] = (bf16)(act ? 1.f : 0.f);
+        }
+        *reinterpret_cast<bf16x8 *>(x + uvox(g, xq, yq, zq) * 8) = o;
+    }
+}
+
+__device__ __forceinline__ bool box_voxel(const Dims &d, long long n, int nx, int ny, int kb, int nzr,
+                                          long long &u)
+{
+    const int i = (int)(n % d.ni) + 1;
+    const int j = (int)((n / d.ni) % d.nj) + 1;
+    const int k = (int)(n / d.nij) + 1;
+    const int xq = i - 2, yq = j - 2, zq = k - kb;
+    if (xq < 0 || xq >= nx || yq < 0 || yq >= ny || zq < 0 || zq >= nzr) return false;
+    u = xq + (long long)nx * (yq + (long long)ny * zq);
+    return true;
+}
+
+// z = z1 + s y over the active range (z1 may be null), 0 on inactive cells
+__global__ void __launch_bounds__(256) k_scatter(Dims d, const unsigned long long *rng, int nx, int ny, int kb,
+                                                 int nzr, const int *__restrict__ indc,
+                                                 const double *__restrict__ z1,
+                                                 const double *__restrict__ scal,
+                                                 const float *__restrict__ y, double *__restrict__ z)
+{
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    const double s = scal[0];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (indc[n] == -1) {
+            z[n] = 0.0;
+            continue;
+        }
+        long long u;
+        const double c = box_voxel(d, n, nx, ny, kb, nzr, u) ? s * (double)y[u] : 0.0;
+        z[n] = (z1 ? z1[n] : 0.0) + c;
+    }
+}
+
+// sampling: target e/s and activity mask in the box's voxel order
+__global__ void k_box_target(Dims d, int nx, int ny, int kb, int nzr, const int *__restrict__ indc,
+                             const double *__restrict__ e, const double *__restrict__ scal,
+                             float *__restrict__ t, float *__restrict__ m)
+{
+    const long long total = (long long)nx * ny * nzr;
+    const double inv = scal[1];
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int xq = (int)(u % nx), yq = (int)((u / nx) % ny), zq = (int)(u / ((long long)nx * ny));
+        const int i = xq + 2, j = yq + 2, k = zq + kb;
+        float tv = 0.f, mv = 0.f;
+        if (i <= d.ni - 1 && j <= d.nj - 1 && k <= d.nk - 1) {
+            const long long n = cid(d, i, j, k);
+            if (indc[n] != -1) {
+                tv = (float)(e[n] * inv);
+                mv = 1.f;
+            }
+        }
+        t[u] = tv;
+        m[u] = mv;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// partials of (V_i, w) for i < nv over the active cells (masked), block order
+__global__ void __launch_bounds__(256) k_mdot(const unsigned long long *rng, const int *__restrict__ mask,
+                                              const double *__restrict__ V, long long nc, int nv,
+                                              const double *__restrict__ w, double *__restrict__ part)
+{
+    double acc[MAXM + 1];
+#pragma unroll
+    for (int i = 0; i <= MAXM; i++) acc[i] = 0.0;
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (mask[n] == -1) continue;
+        const double wn = w[n];
+#pragma unroll
+        for (int i = 0; i <= MAXM; i++)
+            if (i < nv) acc[i] += V[i * nc + n] * wn;
+    }
+    __shared__ double sh[4][MAXM + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i <= MAXM; i++) {
+        if (i < nv) {
+            const double s = wave_sum(acc[i]);
+            if (lane == 0) sh[wv][i] = s;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv)
+        part[(long long)blockIdx.x * (MAXM + 1) + threadIdx.x] =
+            sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+
+__global__ void k_mdot_final(const double *__restrict__ part, int nblk, int nv, double *__restrict__ out)
+{
+    const int i = threadIdx.x;
+    if (i >= nv) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(long long)b * (MAXM + 1) + i];
+    out[i] = s;
+}
+
+// w += sign * sum_i h_i V_i (active cells)
+__global__ void __launch_bounds__(256) k_maxpy(const unsigned long long *rng, const int *__restrict__ mask,
+                                               const double *__restrict__ V, long long nc, int nv,
+                                               const double *__restrict__ h, double sign,
+                                               double *__restrict__ w)
+{
+    __shared__ double hs[MAXM + 1];
+    if ((int)threadIdx.x < nv) hs[threadIdx.x] = sign * h[threadIdx.x];
+    __syncthreads();
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (mask[n] == -1) continue;
+        double s = w[n];
+        for (int i = 0; i < nv; i++) s += hs[i] * V[i * nc + n];
+        w[n] = s;
+    }
+}
+
+// w *= a (active cells)
+__global__ void k_vscale(const unsigned long long *rng, const int *__restrict__ mask, double a,
+                         double *__restrict__ w)
+{
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x)
+        if (mask[n] != -1) w[n] *= a;
+}
+
+// a = b - a, or a = b - c into a (active cells)
+__global__ void k_vsub(const unsigned long long *rng, const int *__restrict__ mask,
+                       const double *__restrict__ b, const double *__restrict__ c,
+                       double *__restrict__ a)
+{
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x)
+        if (mask[n] != -1) a[n] = b[n] - c[n];
+}
+
+// a = b (active cells)
+__global__ void k_vcopy(const unsigned long long *rng, const int *__restrict__ mask,
+                        const double *__restrict__ b, double *__restrict__ a)
+{
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x)
+        if (mask[n] != -1) a[n] = b[n];
+}
+
+__global__ void k_vzero(long long n0, long long n1, double *__restrict__ a)
+{
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x)
+        a[n] = 0.0;
+}
+
+// partial sums of r^2 and of the active count
+__global__ void __launch_bounds__(256) k_sumsq(const unsigned long long *rng, const int *__restrict__ mask,
+                                               const double *__restrict__ r, double *__restrict__ part)
+{
+    double a = 0.0, c = 0.0;
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x)
+        if (mask[n] != -1) {
+            a += r[n] * r[n];
+            c += 1.0;
+        }
+    __shared__ double sh[2][4];
+    a = wave_sum(a);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[(long long)blockIdx.x * (MAXM + 1)] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+        part[(long long)blockIdx.x * (MAXM + 1) + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+    }
+}
+
+// s = sqrt(sum r^2 / count), 1/s (1 when r is zero)
+__global__ void k_rms_final(const double *__restrict__ part, int nblk, double *__restrict__ scal)
+{
+    if (threadIdx.x != 0) return;
+    double a = 0.0, c = 0.0;
+    for (int b = 0; b < nblk; b++) {
+        a += part[(long long)b * (MAXM + 1)];
+        c += part[(long long)b * (MAXM + 1) + 1];
+    }
+    const double s = c > 0.0 ? sqrt(a / c) : 0.0;
+    scal[0] = s;
+    scal[1] = s > 0.0 ? 1.0 / s : 1.0;
+    scal[2] = c;
+}
+
+#define RET_IF(x)              \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+int lalloc(void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        cad_set_error("learned form: hipMalloc of %zu bytes failed", bytes);
+        *p = nullptr;
+        return CAD_ERR_HIP;
+    }
+    return CAD_OK;
+}
+
+int learned_init(cad_ctx *c)
+{
+    if (c->lrn) return CAD_OK;
+    Learned *L = new Learned();
+    std::memset((void *)L, 0, sizeof(Learned));
+    c->lrn = L;
+    const Dims &d = c->d;
+    L->nx = ((d.ni - 2) + 63) / 64 * 64;
+    L->ny = ((d.nj - 2) + 3) / 4 * 4;
+    L->nzmax = ((d.nk - 2) + 3) / 4 * 4;
+    L->mode = 1;
+    L->m = 30;
+    CAD_CHECK(hipSetDevice(c->device));
+    RET_IF(cad_unet_create(L->nx, L->ny, L->nzmax, 1, 0, c->stream, &L->net));
+    L->nvec = 2 * MAXM + 1 + 6;
+    RET_IF(lalloc((void **)&L->vecs, (size_t)L->nvec * d.nc * sizeof(double)));
+    RET_IF(lalloc((void **)&L->part, (size_t)REDUCE_BLOCKS * (MAXM + 1) * sizeof(double)));
+    RET_IF(lalloc((void **)&L->hdev, (MAXM + 1) * sizeof(double)));
+    RET_IF(lalloc((void **)&L->scal, 4 * sizeof(double)));
+    RET_IF(lalloc((void **)&L->y, (size_t)L->nx * L->ny * L->nzmax * sizeof(float)));
+    if (hipHostMalloc((void **)&L->hpin, (MAXM + 1) * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        cad_set_error("learned form: hipHostMalloc failed");
+        return CAD_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, 0LL, (long long)L->nvec * d.nc, L->vecs);
+    CAD_CHECK(hipGetLastError());
+    // weights until the caller sets trained ones: the library's initialiser
+    std::vector<float> p(cad_unet_param_count());
+    RET_IF(cad_unet_init_params(0, p.data()));
+    RET_IF(cad_unet_set_params(L->net, p.data()));
+    L->prev_rng[0] = 0;
+    L->prev_rng[1] = (unsigned long long)d.nc;
+    return CAD_OK;
+}
+
+double *vec(cad_ctx *c, int i) { return c->lrn->vecs + (long long)i * c->d.nc; }
+// V_j = vec(j), Z_j = vec(MAXM+1+j), scratch S_q = vec(2*MAXM+1+q)
+double *Vv(cad_ctx *c, int j) { return vec(c, j); }
+double *Zv(cad_ctx *c, int j) { return vec(c, MAXM + 1 + j); }
+double *Sv(cad_ctx *c, int q) { return vec(c, 2 * MAXM + 1 + q); }
+
+int rms(cad_ctx *c, const double *r)
+{
+    Learned *L = c->lrn;
+    hipLaunchKernelGGL(k_sumsq, dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, c->act_rng, c->f.indc, r, L->part);
+    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(64), 0, c->stream, L->part, REDUCE_BLOCKS, L->scal);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+// features of r into the destination (network input or a sample buffer)
+int features(cad_ctx *c, const double *r, bf16 *dst, const UGeo &g)
+{
+    Learned *L = c->lrn;
+    RET_IF(rms(c, r));
+    hipLaunchKernelGGL(k_features, dim3(4096), dim3(256), 0, c->stream, c->d, g, L->kb, L->nzr, r, L->scal,
+                       c->ggv, c->f.ff, c->f.nf, c->f.indc, dst);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+// z = P(v)
+int precond(cad_ctx *c, const double *v, double *z)
+{
+    Learned *L = c->lrn;
+    const double *r = v;
+    const double *z1 = nullptr;
+    if (L->mode == 1) {
+        double *t0 = Sv(c, 0), *t1 = Sv(c, 1);
+        RET_IF(cad_mg_apply(c, v, t0, c->mg_k0, c->mg_k1));
+        RET_IF(cad_ext_apply_a(c, t0, t1));
+        hipLaunchKernelGGL(k_vsub, dim3(VBLK), dim3(256), 0, c->stream, c->act_rng, c->f.indc, v, t1, t1);
+        CAD_CHECK(hipGetLastError());
+        r = t1;
+        z1 = t0;
+    }
+    UGeo g = cad_unet_geo(L->net);
+    g.nz = L->nzr;
+    RET_IF(features(c, r, reinterpret_cast<bf16 *>(cad_unet_input(L->net)), g));
+    RET_IF(cad_unet_forward_impl(L->net));
+    hipLaunchKernelGGL(k_scatter, dim3(VBLK), dim3(256), 0, c->stream, c->d, c->act_rng, L->nx, L->ny, L->kb,
+                       L->nzr, c->f.indc, z1, L->scal, cad_unet_output(L->net), z);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+int mdot(cad_ctx *c, const double *V, int nv, const double *w, double *host)
+{
+    Learned *L = c->lrn;
+    hipLaunchKernelGGL(k_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, c->act_rng, c->f.indc, V,
+                       c->d.nc, nv, w, L->part);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, c->stream, L->part, REDUCE_BLOCKS, nv, L->hdev);
+    CAD_CHECK(hipGetLastError());
+    CAD_CHECK(hipMemcpyAsync(L->hpin, L->hdev, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nv; i++) host[i] = L->hpin[i];
+    return CAD_OK;
+}
+
+// the solve's set-up: active range, multigrid levels, the network's box
+int setup(cad_ctx *c, bool *empty)
+{
+    Learned *L = c->lrn;
+    const Dims &d = c->d;
+    unsigned long long hr[2];
+    RET_IF(cad_ext_active_range(c, hr));
+    *empty = hr[0] >= hr[1];
+    // vectors read by the matvec must be 0 outside the new active range:
+    // clear what the previous solve's range may have left
+    const long long z0 = (long long)L->prev_rng[0], z1 = (long long)L->prev_rng[1];
+    if (z1 > z0)
+        for (int j = 0; j < MAXM; j++)
+            hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, z0, z1, Zv(c, j));
+    for (int q = 0; q < 6; q++)
+        if (z1 > z0) hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, z0, z1, Sv(c, q));
+    CAD_CHECK(hipGetLastError());
+    L->prev_rng[0] = hr[0];
+    L->prev_rng[1] = hr[1];
+    if (*empty) return CAD_OK;
+    c->mg_span = hr[1] - hr[0];
+    const int k0 = (int)(hr[0] / (unsigned long long)d.nij) + 1;
+    const int k1 = (int)((hr[1] - 1) / (unsigned long long)d.nij) + 1;
+    c->mg_k0 = k0;
+    c->mg_k1 = k1;
+    if (L->mode == 1) RET_IF(cad_mg_setup(c));
+    L->kb = k0;
+    L->nzr = std::min(((k1 - k0 + 1) + 3) / 4 * 4, L->nzmax);
+    RET_IF(cad_unet_set_depth(L->net, L->nzr));
+    return CAD_OK;
+}
+
+}  // namespace
+
+void cad_learned_free(cad_ctx *c)
+{
+    Learned *L = c->lrn;
+    if (!L) return;
+    if (L->net) cad_unet_destroy(L->net);
+    void *p[] = {L->vecs, L->part, L->hdev, L->scal, L->y};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    if (L->hpin) (void)hipHostFree(L->hpin);
+    delete L;
+    c->lrn = nullptr;
+}
+
+int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out, int *iter_out)
+{
+    RET_IF(learned_init(c));
+    Learned *L = c->lrn;
+    const Dims &d = c->d;
+    const long long nc = d.nc;
+    const int *mask = c->f.indc;
+    hipStream_t st = c->stream;
+    bool empty = false;
+    RET_IF(setup(c, &empty));
+    L->last_iters = 0;
+    L->last_fallback_iters = 0;
+    L->solves++;
+    if (empty) {
+        *bnorm_out = 0.0;
+        *xnorm_out = 0.0;
+        *iter_out = 0;
+        return CAD_OK;
+    }
+    double bn2 = 0.0;
+    RET_IF(cad_ext_dot(c, c->bb, c->bb, &bn2));
+    const double bnorm = std::sqrt(bn2);
+    const double tol = c->prm.cgepsr * bnorm + c->prm.cgepsa;
+    double *x = Sv(c, 2), *r = Sv(c, 3);
+    hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, x0, x);
+    CAD_CHECK(hipGetLastError());
+    const int m = L->m;
+    const int cap = cad_krylov_cap(c);
+    int iters = 0;
+    double rnorm = 0.0;
+    bool done = false, stalled = false;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+    double prev = -1.0;
+    while (!done && !stalled) {
+        // r = BB - A x into V_0
+        RET_IF(cad_ext_residual(c, x, Vv(c, 0)));
+        double rr = 0.0;
+        RET_IF(cad_ext_dot(c, Vv(c, 0), Vv(c, 0), &rr));
+        rnorm = std::sqrt(rr);
+        if (!(rnorm > tol) || !std::isfinite(rnorm)) {
+            done = std::isfinite(rnorm);
+            stalled = !done;
+            break;
+        }
+        // a cycle that did not halve the true residual, or the iteration cap,
+        // hands over to VF_M1BCGS
+        if (iters >= cap || (prev > 0.0 && !(rnorm < 0.5 * prev))) {
+            stalled = true;
+            break;
+        }
+        prev = rnorm;
+        hipLaunchKernelGGL(k_vscale, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, 1.0 / rnorm, Vv(c, 0));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = rnorm;
+        int j = 0;
+        double est = rnorm;
+        for (; j < m && iters < cap; j++) {
+            RET_IF(precond(c, Vv(c, j), Zv(c, j)));
+            double *w = Vv(c, j + 1);
+            RET_IF(cad_ext_apply_a(c, Zv(c, j), w));
+            // classical Gram-Schmidt, twice
+            RET_IF(mdot(c, Vv(c, 0), j + 1, w, h.data()));
+            CAD_CHECK(hipMemcpyAsync(L->hdev, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_maxpy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, j + 1,
+                               L->hdev, -1.0, w);
+            RET_IF(mdot(c, Vv(c, 0), j + 1, w, h2.data()));
+            CAD_CHECK(hipMemcpyAsync(L->hdev, h2.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_maxpy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, j + 1,
+                               L->hdev, -1.0, w);
+            CAD_CHECK(hipGetLastError());
+            for (int i = 0; i <= j; i++) h[i] += h2[i];
+            double ww = 0.0;
+            RET_IF(cad_ext_dot(c, w, w, &ww));
+            const double hn = std::sqrt(ww);
+            iters++;
+            // column j of H, rotated by the earlier Givens rotations
+            for (int i = 0; i < j; i++) {
+                const double a = cs[i] * h[i] + sn[i] * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = a;
+            }
+            const double den = std::hypot(h[j], hn);
+            cs[j] = den > 0.0 ? h[j] / den : 1.0;
+            sn[j] = den > 0.0 ? hn / den : 0.0;
+            h[j] = den;
+            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            est = std::fabs(g[j + 1]);
+            if (!(hn > 0.0) || !std::isfinite(est)) {
+                j++;
+                break;
+            }
+            hipLaunchKernelGGL(k_vscale, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, 1.0 / hn, w);
+            CAD_CHECK(hipGetLastError());
+            if (est <= tol) {
+                j++;
+                break;
+            }
+        }
+        // y = H^-1 g (upper triangular), x += Z y
+        for (int i = j - 1; i >= 0; i--) {
+            double s = g[i];
+            for (int k = i + 1; k < j; k++) s -= H[(size_t)i * m + k] * y[k];
+            y[i] = H[(size_t)i * m + i] != 0.0 ? s / H[(size_t)i * m + i] : 0.0;
+        }
+        CAD_CHECK(hipMemcpyAsync(L->hdev, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_maxpy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, Zv(c, 0), nc, j, L->hdev,
+                           1.0, x);
+        CAD_CHECK(hipGetLastError());
+    }
+    L->last_iters = iters;
+    L->total_iters += iters;
+    int it = iters;
+    double xrel = bnorm > 0.0 ? rnorm / bnorm : rnorm;
+    if (!done) {
+        // fallback: the reference's MILU-BiCGSTAB from the current x, held to
+        // the same true relative residual
+        L->fallbacks++;
+        const double keep_stop = c->true_stop;
+        c->true_stop = c->prm.cgepsr;
+        hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, x, r);
+        CAD_CHECK(hipGetLastError());
+        double bn = 0.0, xn = 0.0;
+        int fit = 0;
+        const int rc = cad_solve_m1bcgs(c, r, &bn, &xn, &fit);
+        c->true_stop = keep_stop;
+        RET_IF(rc);
+        L->last_fallback_iters = fit;
+        if (fit < 0 || fit > c->prm.icgmax) {
+            *bnorm_out = bnorm;
+            *xnorm_out = xrel;
+            *iter_out = fit < 0 ? fit : c->prm.icgmax + 1;
+            return CAD_OK;
+        }
+        hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, r, x);
+        CAD_CHECK(hipGetLastError());
+        RET_IF(cad_ext_residual(c, x, Sv(c, 4)));
+        double rr = 0.0;
+        RET_IF(cad_ext_dot(c, Sv(c, 4), Sv(c, 4), &rr));
+        xrel = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+        it = iters + fit;
+    }
+    L->last_resid = xrel;
+    hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, x, x0);
+    CAD_CHECK(hipGetLastError());
+    *bnorm_out = bnorm;
+    *xnorm_out = xrel;
+    *iter_out = it;
+    return CAD_OK;
+}
+
+extern "C" {
+
+int cad_set_learned_weights(cad_ctx *c, const float *params, int n)
+{
+    if (!c || !params || n != cad_unet_param_count()) {
+        cad_set_error("cad_set_learned_weights: %d parameters, the network has %d", n, cad_unet_param_count());
+        return CAD_ERR_ARG;
+    }
+    RET_IF(learned_init(c));
+    return cad_unet_set_params(c->lrn->net, params);
+}
+
+int cad_set_learned_mode(cad_ctx *c, int mode, int restart)
+{
+    if (!c || mode < 0 || mode > 1 || restart < 1 || restart > MAXM) {
+        cad_set_error("cad_set_learned_mode: mode %d (0 network, 1 V-cycle + network), restart %d (1..%d)",
+                      mode, restart, MAXM);
+        return CAD_ERR_ARG;
+    }
+    RET_IF(learned_init(c));
+    c->lrn->mode = mode;
+    c->lrn->m = restart;
+    return CAD_OK;
+}
+
+// Average duration of one forward pass of the network on the box of the
+// context's current flags (input: the right-hand side's features), timed with
+// HIP events on the context's stream, and its multiply-add work counted as 2
+// FLOP per MAC (the 3x3x3 layers at their levels and the output layer).
+int cad_time_learned(cad_ctx *c, int reps, float *ms_avg, double *flops)
+{
+    if (!c || !ms_avg || !flops) return CAD_ERR_ARG;
+    if (reps < 1) reps = 1;
+    RET_IF(learned_init(c));
+    Learned *L = c->lrn;
+    bool empty = false;
+    RET_IF(setup(c, &empty));
+    if (empty) {
+        cad_set_error("cad_time_learned: no active cell");
+        return CAD_ERR_STATE;
+    }
+    UGeo g = cad_unet_geo(L->net);
+    g.nz = L->nzr;
+    RET_IF(features(c, c->bb, reinterpret_cast<bf16 *>(cad_unet_input(L->net)), g));
+    RET_IF(cad_unet_forward_impl(L->net));
+    hipEvent_t a, b;
+    CAD_CHECK(hipEventCreate(&a));
+    CAD_CHECK(hipEventCreate(&b));
+    CAD_CHECK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; r++) RET_IF(cad_unet_forward_impl(L->net));
+    CAD_CHECK(hipEventRecord(b, c->stream));
+    CAD_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    CAD_CHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms_avg = ms / reps;
+    *flops = 2.0 * cad_unet_mac_per_voxel() * (double)L->nx * L->ny * L->nzr;
+    return CAD_OK;
+}
+
+int cad_learned_stats(cad_ctx *c, long long *out)
+{
+    if (!c || !out) return CAD_ERR_ARG;
+    const Learned *L = c->lrn;
+    if (!L) {
+        for (int i = 0; i < 6; i++) out[i] = 0;
+        return CAD_OK;
+    }
+    out[0] = L->last_iters;
+    out[1] = L->last_fallback_iters;
+    out[2] = L->solves;
+    out[3] = L->fallbacks;
+    out[4] = L->total_iters;
+    out[5] = (long long)L->nzr;
+    return CAD_OK;
+}
+
+// Box of the network for the context's current flags: nx, ny, nz (the run's
+// depth) and kb (the first K plane).
+int cad_learned_box(cad_ctx *c, int *box)
+{
+    if (!c || !box) return CAD_ERR_ARG;
+    RET_IF(learned_init(c));
+    bool empty = false;
+    RET_IF(setup(c, &empty));
+    box[0] = c->lrn->nx;
+    box[1] = c->lrn->ny;
+    box[2] = c->lrn->nzr;
+    box[3] = c->lrn->kb;
+    return CAD_OK;
+}
+
+// Training pairs from the plain solver on the context's current system (the
+// matrix and right-hand side of its last step): x* from a full MILU-BiCGSTAB
+// solve, and for each k of iters the k-th iterate x_k of the same solve from
+// 0; the pair is the residual r = BB - A x_k and the correction e = x* - x_k,
+// in mode 1 after one V-cycle z1 = V(r): r - A z1 and e - z1.  Written per
+// pair: the network input (box of cad_learned_box, one-voxel zero halo, 8
+// BF16 channels, (nz+2)(ny+2)(nx+2) voxels), the target e / s and the
+// activity mask (FP32, nx*ny*nz voxels, x fastest).
+int cad_learned_sample(cad_ctx *c, int mode, const int *iters, int n, void *feat, float *target, float *mask)
+{
+    if (!c || !iters || n < 1 || !feat || !target || !mask || mode < 0 || mode > 1) {
+        cad_set_error("cad_learned_sample: bad argument");
+        return CAD_ERR_ARG;
+    }
+    RET_IF(learned_init(c));
+    Learned *L = c->lrn;
+    const int keep_mode = L->mode;
+    L->mode = mode;
+    bool empty = false;
+    int rc = setup(c, &empty);
+    L->mode = keep_mode;
+    RET_IF(rc);
+    if (empty) {
+        cad_set_error("cad_learned_sample: no active cell");
+        return CAD_ERR_STATE;
+    }
+    const long long nc = c->d.nc;
+    hipStream_t st = c->stream;
+    const int *msk = c->f.indc;
+    double *xs = Sv(c, 0), *xk = Sv(c, 1), *r = Sv(c, 2), *e = Sv(c, 3), *z1 = Sv(c, 4), *t = Sv(c, 5);
+    const cad_params keep = c->prm;
+    const double keep_stop = c->true_stop;
+    c->prm.cgepsr = 1.0e-10;
+    c->prm.cgepsa = 0.0;
+    c->prm.icgmax = 2000;
+    c->true_stop = 1.0e-10;
+    double bn = 0.0, xn = 0.0;
+    int it = 0;
+    hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, st, 0LL, nc, xs);
+    rc = cad_solve_m1bcgs(c, xs, &bn, &xn, &it);
+    c->true_stop = keep_stop;
+    if (!rc && (it < 0 || it > c->prm.icgmax)) {
+        cad_set_error("cad_learned_sample: the reference solve did not converge (ITER %d)", it);
+        rc = CAD_ERR_STATE;
+    }
+    UGeo g;
+    g.nx = L->nx;
+    g.ny = L->ny;
+    g.nz = L->nzr;
+    g.nb = 1;
+    g.px = g.nx + 2;
+    g.py = g.ny + 2;
+    g.pz = g.nz + 2;
+    const long long fstride = (long long)g.px * g.py * g.pz * 8;
+    const long long tstride = (long long)g.nx * g.ny * g.nz;
+    for (int q = 0; q < n && !rc; q++) {
+        hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, st, 0LL, nc, xk);
+        c->prm.icgmax = iters[q];
+        rc = cad_solve_m1bcgs(c, xk, &bn, &xn, &it);
+        if (rc) break;
+        hipLaunchKernelGGL(k_vsub, dim3(VBLK), dim3(256), 0, st, c->act_rng, msk, xs, xk, e);
+        CAD_CHECK(hipGetLastError());
+        if ((rc = cad_ext_residual(c, xk, r))) break;
+        if (mode == 1) {
+            if ((rc = cad_mg_apply(c, r, z1, c->mg_k0, c->mg_k1))) break;
+            if ((rc = cad_ext_apply_a(c, z1, t))) break;
+            hipLaunchKernelGGL(k_vsub, dim3(VBLK), dim3(256), 0, st, c->act_rng, msk, r, t, r);
+            hipLaunchKernelGGL(k_vsub, dim3(VBLK), dim3(256), 0, st, c->act_rng, msk, e, z1, e);
+            CAD_CHECK(hipGetLastError());
+        }
+        bf16 *fd = reinterpret_cast<bf16 *>(feat) + q * fstride;
+        hipLaunchKernelGGL(k_vzero, dim3(64), dim3(256), 0, st, 0LL, fstride / 4,
+                           reinterpret_cast<double *>(fd));
+        if ((rc = features(c, r, fd, g))) break;
+        hipLaunchKernelGGL(k_box_target, dim3(4096), dim3(256), 0, st, c->d, g.nx, g.ny, L->kb, L->nzr, msk, e,
+                           L->scal, target + q * tstride, mask + q * tstride);
+        CAD_CHECK(hipGetLastError());
+    }
+    c->prm = keep;
+    RET_IF(rc);
+    CAD_CHECK(hipStreamSynchronize(st));
+    return CAD_OK;
+}
+
+}  // extern "C"

@@ -1,9 +1,10 @@
-// cad_mg.hip -- FP32 aggregation multigrid, the preconditioner of the mixed form.
+// cad_mg.hip -- aggregation multigrid, the preconditioner of the mixed and
+// learned forms, in FP32 (and in FP64 for measuring what FP32 buys).
 //
 // The mixed form solves the pressure system of VF_VPCOEF (the same FP64
-// matrix AD..AUK, BB as the plain form) with an FP64 right-preconditioned
-// BiCGSTAB (cad_poisson.hip) whose preconditioner is one V-cycle of this
-// multigrid in FP32.  It replaces the reference's MILU preconditioner
+// matrix AD..AUK, BB as the plain form) by FP64 iterative refinement around
+// an FP32 BiCGSTAB (cad_refine.hip) whose preconditioner is one V-cycle of
+// this multigrid.  It replaces the reference's MILU preconditioner
 // (vf_mzdcmp.f, vf_mzminv.f), whose triangular sweeps are a serial chain:
 // every multigrid pass is a data-parallel 7-point stencil bound by HBM
 // bandwidth.
@@ -18,6 +19,15 @@
 // cross the block face.  The structure stays 7-point on every level.  A
 // coarse row without active fine cells, or with a diagonal that is not
 // positive, is inactive.
+//
+// Storage.  Each level stores the two colours of the red-black ordering
+// apart: a (J,K) line of ld cells (NUMI rounded up to even) holds first its
+// cells with I+J+K even, then those with I+J+K odd, each half in ascending
+// I.  A half-sweep then reads and writes whole lines of its own colour and
+// reads the other colour's x at the same positions of the neighbouring lines,
+// so a thread's neighbours are its neighbour threads' cells and no line is
+// fetched for half of its cells.  Level 0 also keeps its rows in the grid's
+// natural order (for the FP32 matvec of cad_refine.hip).
 //
 // V-cycle.  Red-black Gauss-Seidel (one sweep = the cells with I+J+K even,
 // then odd) before and after the coarse correction; the residual of the fine
@@ -43,12 +53,22 @@ constexpr float MG_SCALE = 1.8f;
 constexpr int MG_COARSE = 12;
 constexpr long long MG_TAIL_CELLS = 4096;
 
-__device__ __forceinline__ long long mid(const Dims &d, int i, int j, int k)
+// position of cell (i,j,k) (1-based, ghosts included) in a level's arrays
+template <typename L>
+__device__ __forceinline__ long long pos(const L &v, int i, int j, int k)
+{
+    const long long line = (long long)v.ld * ((j - 1) + (long long)v.d.nj * (k - 1));
+    return line + (long long)((i + j + k) & 1) * (v.ld >> 1) + ((i - 1) >> 1);
+}
+
+__device__ __forceinline__ long long nat(const Dims &d, int i, int j, int k)
 {
     return (long long)(i - 1) + (long long)d.ni * (j - 1) + d.nij * (k - 1);
 }
 
-// level 0 from the FP64 matrix
+// level 0 from the FP64 matrix: the colour-split rows and, where the level
+// has them, the natural-order copy
+template <typename V>
 __global__ void __launch_bounds__(256) k_mg_fine(Dims d, const double *__restrict__ ad,
                                                  const double *__restrict__ ali,
                                                  const double *__restrict__ alj,
@@ -56,211 +76,239 @@ __global__ void __launch_bounds__(256) k_mg_fine(Dims d, const double *__restric
                                                  const double *__restrict__ aui,
                                                  const double *__restrict__ auj,
                                                  const double *__restrict__ auk,
-                                                 const int *__restrict__ indc, MgLevel L)
+                                                 const int *__restrict__ indc, MgLevelT<V> L)
 {
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= d.nc) return;
+    const int i = (int)(n % d.ni) + 1, j = (int)((n / d.ni) % d.nj) + 1, k = (int)(n / d.nij) + 1;
+    const long long m = pos(L, i, j, k);
     const bool a = indc[n] != -1 && ad[n] > 0.0;
-    L.act[n] = a ? 1 : 0;
-    L.ad[n] = a ? (float)ad[n] : 1.0f;
-    L.ali[n] = a ? (float)ali[n] : 0.0f;
-    L.alj[n] = a ? (float)alj[n] : 0.0f;
-    L.alk[n] = a ? (float)alk[n] : 0.0f;
-    L.aui[n] = a ? (float)aui[n] : 0.0f;
-    L.auj[n] = a ? (float)auj[n] : 0.0f;
-    L.auk[n] = a ? (float)auk[n] : 0.0f;
+    const V r[7] = {a ? (V)ad[n] : (V)1, a ? (V)ali[n] : (V)0, a ? (V)alj[n] : (V)0, a ? (V)alk[n] : (V)0,
+                    a ? (V)aui[n] : (V)0, a ? (V)auj[n] : (V)0, a ? (V)auk[n] : (V)0};
+    L.act[m] = a ? 1 : 0;
+    L.ad[m] = r[0];
+    L.ali[m] = r[1];
+    L.alj[m] = r[2];
+    L.alk[m] = r[3];
+    L.aui[m] = r[4];
+    L.auj[m] = r[5];
+    L.auk[m] = r[6];
+    if (L.nad) {
+        L.nact[n] = a ? 1 : 0;
+        L.nad[n] = r[0];
+        L.nali[n] = r[1];
+        L.nalj[n] = r[2];
+        L.nalk[n] = r[3];
+        L.naui[n] = r[4];
+        L.nauj[n] = r[5];
+        L.nauk[n] = r[6];
+    }
 }
 
 // Galerkin coarse row of coarse interior cell (I,J,K); one thread per cell
-__global__ void __launch_bounds__(256) k_mg_coarsen(MgLevel F, MgLevel C)
+template <typename V>
+__global__ void __launch_bounds__(256) k_mg_coarsen(MgLevelT<V> F, MgLevelT<V> C)
 {
     const int I = blockIdx.x * blockDim.x + threadIdx.x + 2;
     const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int K = blockIdx.z * blockDim.z + threadIdx.z + 2;
     if (I > C.d.ni - 1 || J > C.d.nj - 1 || K > C.d.nk - 1) return;
     const Dims &fd = F.d;
-    float s_ad = 0.0f, s_li = 0.0f, s_lj = 0.0f, s_lk = 0.0f, s_ui = 0.0f, s_uj = 0.0f, s_uk = 0.0f;
+    V s_ad = 0, s_li = 0, s_lj = 0, s_lk = 0, s_ui = 0, s_uj = 0, s_uk = 0;
     bool any = false;
     for (int c = 0; c < 8; c++) {
         const int oi = c & 1, oj = (c >> 1) & 1, ok = c >> 2;
         const int i = 2 * I - 2 + oi, j = 2 * J - 2 + oj, k = 2 * K - 2 + ok;
         if (i > fd.ni - 1 || j > fd.nj - 1 || k > fd.nk - 1) continue;
-        const long long n = mid(fd, i, j, k);
+        const long long n = pos(F, i, j, k);
         if (!F.act[n]) continue;
         any = true;
         s_ad += F.ad[n];
         // -I: inside the block when oi = 1
-        if (F.ali[n] != 0.0f && F.act[n - 1]) {
+        if (F.ali[n] != (V)0 && F.act[pos(F, i - 1, j, k)]) {
             if (oi) s_ad += F.ali[n]; else s_li += F.ali[n];
         }
-        if (F.aui[n] != 0.0f && F.act[n + 1]) {
+        if (F.aui[n] != (V)0 && F.act[pos(F, i + 1, j, k)]) {
             if (!oi && i + 1 <= fd.ni - 1) s_ad += F.aui[n]; else s_ui += F.aui[n];
         }
-        if (F.alj[n] != 0.0f && F.act[n - fd.ni]) {
+        if (F.alj[n] != (V)0 && F.act[pos(F, i, j - 1, k)]) {
             if (oj) s_ad += F.alj[n]; else s_lj += F.alj[n];
         }
-        if (F.auj[n] != 0.0f && F.act[n + fd.ni]) {
+        if (F.auj[n] != (V)0 && F.act[pos(F, i, j + 1, k)]) {
             if (!oj && j + 1 <= fd.nj - 1) s_ad += F.auj[n]; else s_uj += F.auj[n];
         }
-        if (F.alk[n] != 0.0f && F.act[n - fd.nij]) {
+        if (F.alk[n] != (V)0 && F.act[pos(F, i, j, k - 1)]) {
             if (ok) s_ad += F.alk[n]; else s_lk += F.alk[n];
         }
-        if (F.auk[n] != 0.0f && F.act[n + fd.nij]) {
+        if (F.auk[n] != (V)0 && F.act[pos(F, i, j, k + 1)]) {
             if (!ok && k + 1 <= fd.nk - 1) s_ad += F.auk[n]; else s_uk += F.auk[n];
         }
     }
-    const long long m = mid(C.d, I, J, K);
-    const bool a = any && s_ad > 0.0f;
+    const long long m = pos(C, I, J, K);
+    const bool a = any && s_ad > (V)0;
     C.act[m] = a ? 1 : 0;
-    C.ad[m] = a ? s_ad : 1.0f;
-    C.ali[m] = a ? s_li : 0.0f;
-    C.alj[m] = a ? s_lj : 0.0f;
-    C.alk[m] = a ? s_lk : 0.0f;
-    C.aui[m] = a ? s_ui : 0.0f;
-    C.auj[m] = a ? s_uj : 0.0f;
-    C.auk[m] = a ? s_uk : 0.0f;
+    C.ad[m] = a ? s_ad : (V)1;
+    C.ali[m] = a ? s_li : (V)0;
+    C.alj[m] = a ? s_lj : (V)0;
+    C.alk[m] = a ? s_lk : (V)0;
+    C.aui[m] = a ? s_ui : (V)0;
+    C.auj[m] = a ? s_uj : (V)0;
+    C.auk[m] = a ? s_uk : (V)0;
 }
 
-__device__ __forceinline__ float row_off(const MgLevel &L, long long n)
+// the off-diagonal part of row (i,j,k) at position n
+template <typename V>
+__device__ __forceinline__ V row_off(const MgLevelT<V> &L, long long n, int i, int j, int k)
 {
-    const long long ni = L.d.ni, nij = L.d.nij;
-    const float *x = L.x;
-    return L.ali[n] * x[n - 1] + L.aui[n] * x[n + 1] + L.alj[n] * x[n - ni] + L.auj[n] * x[n + ni]
-         + L.alk[n] * x[n - nij] + L.auk[n] * x[n + nij];
+    const V *x = L.x;
+    return L.ali[n] * x[pos(L, i - 1, j, k)] + L.aui[n] * x[pos(L, i + 1, j, k)]
+         + L.alj[n] * x[pos(L, i, j - 1, k)] + L.auj[n] * x[pos(L, i, j + 1, k)]
+         + L.alk[n] * x[pos(L, i, j, k - 1)] + L.auk[n] * x[pos(L, i, j, k + 1)];
 }
 
-// one colour of a Gauss-Seidel sweep: the cells with (I+J+K) % 2 == color,
-// thread x covering every second I of its (J,K) line, K in [k0, k1]
-__global__ void __launch_bounds__(256) k_mg_smooth(MgLevel L, int color, int k0)
+// one colour of a Gauss-Seidel sweep: thread x covers position q of its
+// (J,K) line's half of that colour (cell I = 2q+1 or 2q+2), K in [k0, k1]
+template <typename V>
+__global__ void __launch_bounds__(256) k_mg_smooth(MgLevelT<V> L, int color, int k0)
 {
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int k = blockIdx.z + k0;
     if (j > L.d.nj - 1 || k > L.d.nk - 1) return;
-    const int i = 2 + ((color + j + k) & 1) + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i > L.d.ni - 1) return;
-    const long long n = mid(L.d, i, j, k);
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = 2 * q + 1 + ((color + 1 + j + k) & 1);
+    if (i < 2 || i > L.d.ni - 1) return;
+    const long long n = pos(L, i, j, k);
     if (!L.act[n]) return;
-    L.x[n] = (L.b[n] - row_off(L, n)) / L.ad[n];
+    L.x[n] = (L.b[n] - row_off(L, n, i, j, k)) / L.ad[n];
 }
 
 // coarse right-hand side = sum of the fine residuals of the block; coarse x = 0
-__global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C)
+template <typename V>
+__global__ void __launch_bounds__(256) k_mg_restrict(MgLevelT<V> F, MgLevelT<V> C)
 {
     const int I = blockIdx.x * blockDim.x + threadIdx.x + 2;
     const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int K = blockIdx.z * blockDim.z + threadIdx.z + 2;
     if (I > C.d.ni - 1 || J > C.d.nj - 1 || K > C.d.nk - 1) return;
     const Dims &fd = F.d;
-    float s = 0.0f;
+    V s = 0;
     for (int c = 0; c < 8; c++) {
         const int i = 2 * I - 2 + (c & 1), j = 2 * J - 2 + ((c >> 1) & 1), k = 2 * K - 2 + (c >> 2);
         if (i > fd.ni - 1 || j > fd.nj - 1 || k > fd.nk - 1) continue;
-        const long long n = mid(fd, i, j, k);
+        const long long n = pos(F, i, j, k);
         if (!F.act[n]) continue;
-        s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n);
+        s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n, i, j, k);
     }
-    const long long m = mid(C.d, I, J, K);
-    const float bc = C.act[m] ? s : 0.0f;
+    const long long m = pos(C, I, J, K);
+    const V bc = C.act[m] ? s : (V)0;
     C.b[m] = bc;
     // from x = 0 the even half of the first sweep is x = b/AD
-    C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : 0.0f;
+    C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : (V)0;
 }
 
 // x += MG_SCALE * coarse x of the block (active fine cells)
-__global__ void __launch_bounds__(256) k_mg_prolong(MgLevel F, MgLevel C, int k0)
+template <typename V>
+__global__ void __launch_bounds__(256) k_mg_prolong(MgLevelT<V> F, MgLevelT<V> C, int k0)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int k = blockIdx.z + k0;
     if (i > F.d.ni - 1 || j > F.d.nj - 1 || k > F.d.nk - 1) return;
-    const long long n = mid(F.d, i, j, k);
+    const long long n = pos(F, i, j, k);
     if (!F.act[n]) return;
-    const long long m = mid(C.d, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2);
-    F.x[n] += MG_SCALE * C.x[m];
+    F.x[n] += (V)MG_SCALE * C.x[pos(C, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)];
 }
 
-// level-0 right-hand side from the FP64 vector over the active index range
-// [rng[0], rng[1]) (rows outside it are inactive: their x stays 0); from
+// level-0 right-hand side from a vector of the grid (natural order) on the K
+// planes k0..k1 (the rows outside them are inactive: their x stays 0); from
 // x = 0 the even half of the first sweep is x = b/AD
-__global__ void __launch_bounds__(256) k_mg_in(const unsigned long long *rng, const double *__restrict__ y,
-                                               MgLevel L)
+template <typename V, typename T>
+__global__ void __launch_bounds__(256) k_mg_in(MgLevelT<V> L, const T *__restrict__ y, int k0)
 {
-    const long long n = (long long)rng[0] + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (long long)rng[1]) return;
-    const bool a = L.act[n] != 0;
-    const float b = a ? (float)y[n] : 0.0f;
-    L.b[n] = b;
-    const long long ni = L.d.ni;
-    const long long i = n % ni, j = (n / ni) % L.d.nj, k = n / L.d.nij;
-    L.x[n] = a && ((i + j + k + 3) & 1) == 0 ? b / L.ad[n] : 0.0f;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int k = blockIdx.z + k0;
+    if (i > L.d.ni - 1 || j > L.d.nj - 1) return;
+    const long long m = pos(L, i, j, k);
+    const bool a = L.act[m] != 0;
+    const V b = a ? (V)y[nat(L.d, i, j, k)] : (V)0;
+    L.b[m] = b;
+    L.x[m] = a && ((i + j + k) & 1) == 0 ? b / L.ad[m] : (V)0;
 }
 
-// FP64 result from level 0 over the active index range (0 on inactive rows)
-__global__ void __launch_bounds__(256) k_mg_out(const unsigned long long *rng, const MgLevel L,
-                                                double *__restrict__ x)
+// the result from level 0 into a vector of the grid on the K planes k0..k1
+// (0 on inactive rows)
+template <typename V, typename T>
+__global__ void __launch_bounds__(256) k_mg_out(const MgLevelT<V> L, T *__restrict__ x, int k0)
 {
-    const long long n = (long long)rng[0] + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (long long)rng[1]) return;
-    x[n] = L.act[n] ? (double)L.x[n] : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int k = blockIdx.z + k0;
+    if (i > L.d.ni - 1 || j > L.d.nj - 1) return;
+    const long long m = pos(L, i, j, k);
+    x[nat(L.d, i, j, k)] = L.act[m] ? (T)L.x[m] : (T)0;
 }
 
 // The V-cycle below level t0 inside one workgroup: for each level down to the
 // coarsest the odd half-sweep (the even half was set with b), the coarse
 // right-hand side; MG_COARSE sweeps on the coarsest; then up: correction and
 // one sweep.  Levels are a kernel argument (at most CAD_MG_MAX).
+template <typename V>
 struct MgTail {
-    MgLevel L[CAD_MG_MAX];
+    MgLevelT<V> L[CAD_MG_MAX];
 };
 
-__device__ void tail_half(const MgLevel &L, int color)
+template <typename V>
+__device__ void tail_half(const MgLevelT<V> &L, int color)
 {
     const int ni = L.d.ni - 2, nj = L.d.nj - 2, nk = L.d.nk - 2;
     const int cells = ni * nj * nk;
     for (int q = threadIdx.x; q < cells; q += blockDim.x) {
         const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
         if (((i + j + k) & 1) != color) continue;
-        const long long n = mid(L.d, i, j, k);
-        if (L.act[n]) L.x[n] = (L.b[n] - row_off(L, n)) / L.ad[n];
+        const long long n = pos(L, i, j, k);
+        if (L.act[n]) L.x[n] = (L.b[n] - row_off(L, n, i, j, k)) / L.ad[n];
     }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) k_mg_tail(MgTail T, int t0, int nl)
+template <typename V>
+__global__ void __launch_bounds__(1024) k_mg_tail(MgTail<V> T, int t0, int nl)
 {
     for (int l = t0; l < nl - 1; l++) {
-        const MgLevel &F = T.L[l], &C = T.L[l + 1];
+        const MgLevelT<V> &F = T.L[l], &C = T.L[l + 1];
         tail_half(F, 1);
         const int ni = C.d.ni - 2, nj = C.d.nj - 2, nk = C.d.nk - 2;
         for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
             const int I = q % ni + 2, J = (q / ni) % nj + 2, K = q / (ni * nj) + 2;
-            float s = 0.0f;
+            V s = 0;
             for (int c = 0; c < 8; c++) {
                 const int i = 2 * I - 2 + (c & 1), j = 2 * J - 2 + ((c >> 1) & 1), k = 2 * K - 2 + (c >> 2);
                 if (i > F.d.ni - 1 || j > F.d.nj - 1 || k > F.d.nk - 1) continue;
-                const long long n = mid(F.d, i, j, k);
+                const long long n = pos(F, i, j, k);
                 if (!F.act[n]) continue;
-                s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n);
+                s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n, i, j, k);
             }
-            const long long m = mid(C.d, I, J, K);
-            const float bc = C.act[m] ? s : 0.0f;
+            const long long m = pos(C, I, J, K);
+            const V bc = C.act[m] ? s : (V)0;
             C.b[m] = bc;
-            C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : 0.0f;
+            C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : (V)0;
         }
         __syncthreads();
     }
-    const MgLevel &B = T.L[nl - 1];
+    const MgLevelT<V> &B = T.L[nl - 1];
     tail_half(B, 1);
     for (int s = 1; s < MG_COARSE; s++) {
         tail_half(B, 0);
         tail_half(B, 1);
     }
     for (int l = nl - 2; l >= t0; l--) {
-        const MgLevel &F = T.L[l], &C = T.L[l + 1];
+        const MgLevelT<V> &F = T.L[l], &C = T.L[l + 1];
         const int ni = F.d.ni - 2, nj = F.d.nj - 2, nk = F.d.nk - 2;
         for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
             const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
-            const long long n = mid(F.d, i, j, k);
-            if (F.act[n]) F.x[n] += MG_SCALE * C.x[mid(C.d, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)];
+            const long long n = pos(F, i, j, k);
+            if (F.act[n]) F.x[n] += (V)MG_SCALE * C.x[pos(C, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)];
         }
         __syncthreads();
         tail_half(F, 0);
@@ -273,54 +321,74 @@ dim3 lvl_grid(const Dims &d, int k0, int k1)
     return dim3((unsigned)((d.ni - 2 + 63) / 64), (unsigned)((d.nj - 2 + 3) / 4), (unsigned)(k1 - k0 + 1));
 }
 
+// threads over one colour's half lines: positions 0 .. NUMI/2
 dim3 half_grid(const Dims &d, int k0, int k1)
 {
-    const int half = (d.ni - 2 + 1) / 2;
+    const int half = d.ni / 2 + 1;
     return dim3((unsigned)((half + 63) / 64), (unsigned)((d.nj - 2 + 3) / 4), (unsigned)(k1 - k0 + 1));
 }
 
 // half-sweeps of colours c0..1 on K planes k0..k1
-int smooth(cad_ctx *c, const MgLevel &L, int k0, int k1, int c0)
+template <typename V>
+int smooth(cad_ctx *c, const MgLevelT<V> &L, int k0, int k1, int c0)
 {
     const dim3 b(64, 4, 1), g = half_grid(L.d, k0, k1);
     for (int color = c0; color <= 1; color++)
-        hipLaunchKernelGGL(k_mg_smooth, g, b, 0, c->stream, L, color, k0);
+        hipLaunchKernelGGL(k_mg_smooth<V>, g, b, 0, c->stream, L, color, k0);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
 
-}  // namespace
-
-void cad_mg_free(cad_ctx *c)
+template <typename V>
+void free_levels(MgLevelT<V> *mg, int &n)
 {
-    for (int l = 0; l < c->mg_n; l++) {
-        MgLevel &L = c->mg[l];
-        void *p[] = {L.ad, L.ali, L.alj, L.alk, L.aui, L.auj, L.auk, L.x, L.b, L.act};
+    for (int l = 0; l < n; l++) {
+        MgLevelT<V> &L = mg[l];
+        void *p[] = {L.ad, L.ali, L.alj, L.alk, L.aui, L.auj, L.auk, L.x, L.b, L.act, L.nad, L.nact};
         for (void *q : p)
             if (q) (void)hipFree(q);
+        L = MgLevelT<V>();
     }
-    c->mg_n = 0;
+    n = 0;
 }
 
-int cad_mg_alloc(cad_ctx *c)
+// The levels of a grid: colour-split arrays of ld x NUMJ x NUMK per level
+// (and level 0's natural-order copy when natural is set), each level counted
+// before its arrays so that a failed allocation frees what was allocated.
+template <typename V>
+int alloc_levels(const Dims &d0, MgLevelT<V> *mg, int &nl, bool natural)
 {
-    if (c->mg_n > 0) return CAD_OK;
-    Dims d = c->d;
+    if (nl > 0) return CAD_OK;
+    Dims d = d0;
     int n = 0;
     for (;;) {
         if (n == CAD_MG_MAX) break;
-        MgLevel &L = c->mg[n];
-        L = MgLevel();
+        MgLevelT<V> &L = mg[n];
+        L = MgLevelT<V>();
         L.d = d;
-        c->mg_n = n + 1;   // counted before its arrays, so a failure frees what was allocated
-        const size_t nc = (size_t)d.nc;
-        float **f[] = {&L.ad, &L.ali, &L.alj, &L.alk, &L.aui, &L.auj, &L.auk, &L.x, &L.b};
-        for (float **p : f) {
-            CAD_CHECK(hipMalloc((void **)p, nc * sizeof(float)));
-            CAD_CHECK(hipMemset(*p, 0, nc * sizeof(float)));
+        L.ld = (d.ni + 1) / 2 * 2;
+        nl = n + 1;
+        const size_t ns = (size_t)L.ld * d.nj * d.nk;
+        V **f[] = {&L.ad, &L.ali, &L.alj, &L.alk, &L.aui, &L.auj, &L.auk, &L.x, &L.b};
+        for (V **p : f) {
+            CAD_CHECK(hipMalloc((void **)p, ns * sizeof(V)));
+            CAD_CHECK(hipMemset(*p, 0, ns * sizeof(V)));
         }
-        CAD_CHECK(hipMalloc((void **)&L.act, nc));
-        CAD_CHECK(hipMemset(L.act, 0, nc));
+        CAD_CHECK(hipMalloc((void **)&L.act, ns));
+        CAD_CHECK(hipMemset(L.act, 0, ns));
+        if (n == 0 && natural) {
+            const size_t nc = (size_t)d.nc;
+            CAD_CHECK(hipMalloc((void **)&L.nad, 7 * nc * sizeof(V)));
+            CAD_CHECK(hipMalloc((void **)&L.nact, nc));
+            CAD_CHECK(hipMemset(L.nad, 0, 7 * nc * sizeof(V)));
+            CAD_CHECK(hipMemset(L.nact, 0, nc));
+            L.nali = L.nad + nc;
+            L.nalj = L.nali + nc;
+            L.nalk = L.nalj + nc;
+            L.naui = L.nalk + nc;
+            L.nauj = L.naui + nc;
+            L.nauk = L.nauj + nc;
+        }
         n++;
         // coarsen while every interior extent is at least 4
         const int ei = d.ni - 2, ej = d.nj - 2, ek = d.nk - 2;
@@ -334,7 +402,7 @@ int cad_mg_alloc(cad_ctx *c)
         cd.numb = 0;
         d = cd;
     }
-    const MgLevel &top = c->mg[c->mg_n - 1];
+    const MgLevelT<V> &top = mg[nl - 1];
     if ((long long)(top.d.ni - 2) * (top.d.nj - 2) * (top.d.nk - 2) > (1 << 20)) {
         cad_set_error("multigrid: coarsest level %d x %d x %d too large", top.d.ni - 2, top.d.nj - 2,
                       top.d.nk - 2);
@@ -344,17 +412,16 @@ int cad_mg_alloc(cad_ctx *c)
     return CAD_OK;
 }
 
-// the hierarchy of the context's current matrix
-int cad_mg_setup(cad_ctx *c)
+template <typename V>
+int setup_levels(cad_ctx *c, MgLevelT<V> *mg, int nl)
 {
-    RET_IF(cad_mg_alloc(c));
     const Dims &d = c->d;
-    hipLaunchKernelGGL(k_mg_fine, dim3((unsigned)((d.nc + 255) / 256)), dim3(256), 0, c->stream, d,
-                       c->ad, c->ali, c->alj, c->alk, c->aui, c->auj, c->auk, c->f.indc, c->mg[0]);
-    for (int l = 1; l < c->mg_n; l++) {
-        const Dims &cd = c->mg[l].d;
-        hipLaunchKernelGGL(k_mg_coarsen, lvl_grid(cd, 2, cd.nk - 1), dim3(64, 4, 1), 0, c->stream,
-                           c->mg[l - 1], c->mg[l]);
+    hipLaunchKernelGGL(k_mg_fine<V>, dim3((unsigned)((d.nc + 255) / 256)), dim3(256), 0, c->stream, d,
+                       c->ad, c->ali, c->alj, c->alk, c->aui, c->auj, c->auk, c->f.indc, mg[0]);
+    for (int l = 1; l < nl; l++) {
+        const Dims &cd = mg[l].d;
+        hipLaunchKernelGGL(k_mg_coarsen<V>, lvl_grid(cd, 2, cd.nk - 1), dim3(64, 4, 1), 0, c->stream,
+                           mg[l - 1], mg[l]);
     }
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
@@ -362,78 +429,130 @@ int cad_mg_setup(cad_ctx *c)
 
 // x = M^-1 y: one V-cycle from x = 0; k0..k1 limits level 0 to the K planes
 // that hold active cells
-int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1)
+template <typename V, typename T>
+int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, int k1)
 {
     const dim3 b(64, 4, 1);
-    const int nl = c->mg_n;
-    const unsigned long long span = c->mg_span;
-    const unsigned vblocks = (unsigned)((span + 255) / 256);
-    hipLaunchKernelGGL(k_mg_in, dim3(vblocks), dim3(256), 0, c->stream, c->act_rng, y, c->mg[0]);
+    hipLaunchKernelGGL((k_mg_in<V, T>), lvl_grid(mg[0].d, k0, k1), b, 0, c->stream, mg[0], y, k0);
     CAD_CHECK(hipGetLastError());
     // the first level small enough for one workgroup
     int t0 = nl - 1;
-    while (t0 > 0 && (long long)(c->mg[t0 - 1].d.ni - 2) * (c->mg[t0 - 1].d.nj - 2) *
-                             (c->mg[t0 - 1].d.nk - 2) <= MG_TAIL_CELLS)
+    while (t0 > 0 && (long long)(mg[t0 - 1].d.ni - 2) * (mg[t0 - 1].d.nj - 2) * (mg[t0 - 1].d.nk - 2) <=
+                         MG_TAIL_CELLS)
         t0--;
     // down: the odd half-sweep (the even half came with b), the coarse b
     for (int l = 0; l < t0; l++) {
-        const MgLevel &F = c->mg[l], &C = c->mg[l + 1];
+        const MgLevelT<V> &F = mg[l], &C = mg[l + 1];
         const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
         RET_IF(smooth(c, F, a0, a1, 1));
-        hipLaunchKernelGGL(k_mg_restrict, lvl_grid(C.d, 2, C.d.nk - 1), b, 0, c->stream, F, C);
+        hipLaunchKernelGGL(k_mg_restrict<V>, lvl_grid(C.d, 2, C.d.nk - 1), b, 0, c->stream, F, C);
         CAD_CHECK(hipGetLastError());
     }
-    MgTail T;
-    for (int l = 0; l < nl; l++) T.L[l] = c->mg[l];
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, c->stream, T, t0, nl);
+    MgTail<V> tl;
+    for (int l = 0; l < nl; l++) tl.L[l] = mg[l];
+    hipLaunchKernelGGL(k_mg_tail<V>, dim3(1), dim3(1024), 0, c->stream, tl, t0, nl);
     CAD_CHECK(hipGetLastError());
     // up: correct, then one sweep
     for (int l = t0 - 1; l >= 0; l--) {
-        const MgLevel &F = c->mg[l], &C = c->mg[l + 1];
+        const MgLevelT<V> &F = mg[l], &C = mg[l + 1];
         const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
-        hipLaunchKernelGGL(k_mg_prolong, lvl_grid(F.d, a0, a1), b, 0, c->stream, F, C, a0);
+        hipLaunchKernelGGL(k_mg_prolong<V>, lvl_grid(F.d, a0, a1), b, 0, c->stream, F, C, a0);
         CAD_CHECK(hipGetLastError());
         RET_IF(smooth(c, F, a0, a1, 0));
     }
-    hipLaunchKernelGGL(k_mg_out, dim3(vblocks), dim3(256), 0, c->stream, c->act_rng, c->mg[0], x);
+    hipLaunchKernelGGL((k_mg_out<V, T>), lvl_grid(mg[0].d, k0, k1), b, 0, c->stream, mg[0], x, k0);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
 
-// One V-cycle on the hierarchy and K range of the last mixed solve, timed
-// with HIP events on the library's stream (input BB, output AE), and its
-// algorithmic bytes: per cell of a level below the coarsest, three
-// half-sweeps of 20.5 B (one colour's 7 coefficients, b, activity and x
-// written, the other colour's x read), the residual-restriction 38 B (7
-// coefficients, b, x, activity; the coarse b and x) and the prolongation
-// 9.5 B; the coarsest level's sweeps 41 B each; and per cell of the active
-// index range the FP64 -> FP32 input 21 B and the FP32 -> FP64 output 13 B.
-int cad_mg_time(cad_ctx *c, int reps, float *ms_avg, double *bytes)
+// Algorithmic bytes of one V-cycle with vs-byte values: per cell of a level
+// below the coarsest, three half-sweeps (each cell once in one of them: 7
+// coefficients, b, x written and the activity byte; x of the other colour's
+// neighbours read once: 10 values + 1 byte per cell and colour pair), the
+// residual-restriction (7 coefficients, b, x, activity, and a share of the
+// coarse b and x) and the prolongation (x read and written, activity, the
+// coarse x); on the coarsest level MG_COARSE full sweeps; per cell of level
+// 0's K range the FP64 input read with b and x written (AD and the activity
+// read), and x and the activity read with the FP64 output written.
+double vcycle_bytes(const cad_ctx *c, const Dims *dims, int nl, int vs)
+{
+    const double half = (10.0 * vs + 1.0) / 2.0;
+    const double restr = 9.0 * vs + 1.0 + 2.0 * vs / 8.0;
+    const double prol = 2.0 * vs + 1.0 + vs / 8.0;
+    double by = 0.0;
+    for (int l = 0; l < nl; l++) {
+        const Dims &d = dims[l];
+        const int planes = l == 0 ? c->mg_k1 - c->mg_k0 + 1 : d.nk - 2;
+        const double cells = (double)(d.ni - 2) * (double)(d.nj - 2) * (double)planes;
+        if (l == 0) by += ((8.0 + 3.0 * vs + 1.0) + (vs + 8.0 + 1.0)) * cells;
+        by += l == nl - 1 ? 2.0 * half * MG_COARSE * cells : (3.0 * half + restr + prol) * cells;
+    }
+    return by;
+}
+
+}  // namespace
+
+void cad_mg_free(cad_ctx *c)
+{
+    free_levels(c->mg, c->mg_n);
+    free_levels(c->mg64, c->mg64_n);
+}
+
+int cad_mg_alloc(cad_ctx *c) { return alloc_levels(c->d, c->mg, c->mg_n, true); }
+
+// the hierarchy of the context's current matrix
+int cad_mg_setup(cad_ctx *c)
+{
+    RET_IF(cad_mg_alloc(c));
+    return setup_levels(c, c->mg, c->mg_n);
+}
+
+int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1)
+{
+    return vcycle(c, c->mg, c->mg_n, y, x, k0, k1);
+}
+
+int cad_mg_apply32(cad_ctx *c, const float *y, float *x, int k0, int k1)
+{
+    return vcycle(c, c->mg, c->mg_n, y, x, k0, k1);
+}
+
+// One V-cycle on the hierarchy and K range of the last mixed or learned
+// solve, timed with HIP events on the library's stream (input BB, output AE),
+// with its algorithmic bytes (vcycle_bytes); fp64 != 0: the same V-cycle with
+// every level in FP64 (a hierarchy of its own, built from the same matrix),
+// for comparison.
+int cad_mg_time(cad_ctx *c, int reps, int fp64, float *ms_avg, double *bytes)
 {
     if (c->mg_n <= 0 || c->mg_span == 0) {
         cad_set_error("cad_time_precond: no mixed solve has run on this context");
         return CAD_ERR_STATE;
     }
     if (reps < 1) reps = 1;
+    if (fp64) {
+        RET_IF(alloc_levels(c->d, c->mg64, c->mg64_n, false));
+        RET_IF(setup_levels(c, c->mg64, c->mg64_n));
+    }
     hipEvent_t a, b;
     CAD_CHECK(hipEventCreate(&a));
     CAD_CHECK(hipEventCreate(&b));
     CAD_CHECK(hipEventRecord(a, c->stream));
-    for (int r = 0; r < reps; r++) RET_IF(cad_mg_apply(c, c->bb, c->ae, c->mg_k0, c->mg_k1));
+    for (int r = 0; r < reps; r++) {
+        if (fp64)
+            RET_IF(vcycle(c, c->mg64, c->mg64_n, c->bb, c->ae, c->mg_k0, c->mg_k1));
+        else
+            RET_IF(vcycle(c, c->mg, c->mg_n, c->bb, c->ae, c->mg_k0, c->mg_k1));
+    }
     CAD_CHECK(hipEventRecord(b, c->stream));
     CAD_CHECK(hipEventSynchronize(b));
     float ms = 0.0f;
     CAD_CHECK(hipEventElapsedTime(&ms, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     *ms_avg = ms / reps;
-    double by = 34.0 * (double)c->mg_span;
-    for (int l = 0; l < c->mg_n; l++) {
-        const Dims &d = c->mg[l].d;
-        const int planes = l == 0 ? c->mg_k1 - c->mg_k0 + 1 : d.nk - 2;
-        const double cells = (double)(d.ni - 2) * (double)(d.nj - 2) * (double)planes;
-        by += l == c->mg_n - 1 ? 41.0 * MG_COARSE * cells : (3.0 * 20.5 + 38.0 + 9.5) * cells;
-    }
-    *bytes = by;
+    Dims dims[CAD_MG_MAX];
+    const int nl = fp64 ? c->mg64_n : c->mg_n;
+    for (int l = 0; l < nl; l++) dims[l] = fp64 ? c->mg64[l].d : c->mg[l].d;
+    *bytes = vcycle_bytes(c, dims, nl, fp64 ? 8 : 4);
     return CAD_OK;
 }

@@ -442,77 +442,6 @@ __global__ void __launch_bounds__(256) k_copy_active(const unsigned long long *r
 
 __global__ void k_iset1(int *p, int v) { *p = v; }
 
-// y = A x fused with the partials of (w, y) and, with NPROD = 2, (y, y)
-// (active cells); the mixed form's matvecs, whose next step needs those dots
-template <int NPROD>
-__global__ void __launch_bounds__(REDUCE_THREADS) k_mzax_dot(Dims d, const unsigned long long *rng,
-                                                            const double *__restrict__ ad,
-                                                            const double *__restrict__ ali,
-                                                            const double *__restrict__ alj,
-                                                            const double *__restrict__ alk,
-                                                            const double *__restrict__ aui,
-                                                            const double *__restrict__ auj,
-                                                            const double *__restrict__ auk,
-                                                            const int *__restrict__ indc,
-                                                            const double *__restrict__ x,
-                                                            const double *__restrict__ w,
-                                                            double *__restrict__ y, double *partial)
-{
-    const long long ni = d.ni, nij = d.nij;
-    double acc[3] = {0.0, 0.0, 0.0};
-    ACTIVE_LOOP(n, rng, REDUCE_BLOCKS) {
-        if (indc[n] == -1) continue;
-        const double yn = ad[n] * x[n] + ali[n] * x[n - 1] + alj[n] * x[n - ni] + alk[n] * x[n - nij]
-                        + aui[n] * x[n + 1] + auj[n] * x[n + ni] + auk[n] * x[n + nij];
-        y[n] = yn;
-        acc[0] += w[n] * yn;
-        if (NPROD == 2) acc[1] += yn * yn;
-    }
-    block_reduce_store<NPROD>(acc, partial);
-}
-
-// vector updates of the mixed form's BiCGSTAB (active cells)
-__global__ void __launch_bounds__(256) k_bicg_p(const unsigned long long *rng, const int *__restrict__ mask,
-                                                double beta, double om, const double *__restrict__ r,
-                                                const double *__restrict__ v, double *p)
-{
-    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
-        if (mask[n] != -1) p[n] = r[n] + beta * (p[n] - om * v[n]);
-    }
-}
-
-__global__ void __launch_bounds__(256) k_bicg_s(const unsigned long long *rng, const int *__restrict__ mask,
-                                                double alpha, const double *__restrict__ r,
-                                                const double *__restrict__ v, double *s)
-{
-    ACTIVE_LOOP(n, rng, VEC_BLOCKS) {
-        if (mask[n] != -1) s[n] = r[n] - alpha * v[n];
-    }
-}
-
-// x += alpha ph + om sh ; r = s - om t ; partials of (r, r) and (r0, r)
-__global__ void __launch_bounds__(REDUCE_THREADS) k_bicg_xr(const unsigned long long *rng,
-                                                            const int *__restrict__ mask, double alpha,
-                                                            double om, const double *__restrict__ ph,
-                                                            const double *__restrict__ sh,
-                                                            const double *__restrict__ s,
-                                                            const double *__restrict__ t,
-                                                            const double *__restrict__ r0, double *x,
-                                                            double *r, double *partial)
-{
-    double acc[3] = {0.0, 0.0, 0.0};
-    ACTIVE_LOOP(n, rng, REDUCE_BLOCKS) {
-        if (mask[n] != -1) {
-            x[n] = x[n] + (alpha * ph[n] + om * sh[n]);
-            const double rn = s[n] - om * t[n];
-            r[n] = rn;
-            acc[0] += rn * rn;
-            acc[1] += r0[n] * rn;
-        }
-    }
-    block_reduce_store<2>(acc, partial);
-}
-
 inline dim3 stream_grid(long long nc)
 {
     long long b = (nc + 255) / 256;
@@ -781,131 +710,27 @@ int cad_poisson_residual_impl(cad_ctx *c, double *rel)
     return CAD_OK;
 }
 
-// The mixed form's pressure solve: BiCGSTAB in FP64 on the same matrix and
-// right-hand side as VF_M1BCGS, right-preconditioned by one FP32 multigrid
-// V-cycle (cad_mg.hip), stopped when the true relative residual
-// ||BB - A x|| / ||BB|| is at most CGEPSR.  The recurrence's residual is
-// checked against an explicit FP64 residual at convergence; if they disagree
-// the iteration restarts from the explicit one.  Starts from x0 and leaves
-// the solution there; *xnorm_out is the final true relative residual.
-int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out, int *iter_out)
+// Iterations the mixed and learned forms take before handing a solve to
+// MILU-BiCGSTAB: cad_set_krylov_cap's value, else 200, at most ICGMAX.
+int cad_krylov_cap(const cad_ctx *c)
 {
-    const Dims &d = c->d;
-    const long long nc = d.nc;
-    const int *mask = c->f.indc;
-    const unsigned long long *rng = c->act_rng;
-    hipStream_t st = c->stream;
-    const dim3 sg = stream_grid(nc), sb(256), vg(VEC_BLOCKS);
-    const dim3 rgd(REDUCE_BLOCKS), rbd(REDUCE_THREADS);
-    double *x = c->xw, *r = c->rr, *r0 = c->r0, *p = c->pv, *v = c->ap, *s = c->yv, *t = c->ae;
-    double *ph = c->lpi, *sh = c->lpj;
-    hipLaunchKernelGGL(k_copy, sg, sb, 0, st, nc, x0, x);
-    double *zero_vecs[8] = {r, r0, p, v, s, t, ph, sh};
-    for (double *z : zero_vecs) hipLaunchKernelGGL(k_fill, sg, sb, 0, st, nc, z, 0.0);
-    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, st, c->act_rng, nc);
-    hipLaunchKernelGGL(k_active_range, dim3(1024), dim3(256), 0, st, nc, mask, c->act_rng);
+    const int cap = c->krylov_cap > 0 ? c->krylov_cap : 200;
+    return cap < c->prm.icgmax ? cap : c->prm.icgmax;
+}
+
+// The true relative residual ||BB - A x|| / ||BB|| of a device vector x.
+int cad_poisson_residual_x(cad_ctx *c, const double *x, double *rel)
+{
+    const dim3 sb(256), vg(VEC_BLOCKS);
+    RET_IF_(apply_a(c, x, c->yv));
+    hipLaunchKernelGGL(k_bminus, vg, sb, 0, c->stream, c->act_rng, c->f.indc, c->bb, c->yv);
     CAD_CHECK(hipGetLastError());
-    // the K planes that hold active cells bound the finest level's passes
-    unsigned long long hr[2];
-    CAD_CHECK(hipMemcpyAsync(hr, c->act_rng, sizeof hr, hipMemcpyDeviceToHost, st));
-    CAD_CHECK(hipStreamSynchronize(st));
-    if (hr[0] >= hr[1]) {
-        *bnorm_out = 0.0;
-        *xnorm_out = 0.0;
-        *iter_out = 0;
-        return CAD_OK;
-    }
-    c->mg_span = hr[1] - hr[0];
-    const int k0 = (int)(hr[0] / (unsigned long long)d.nij) + 1;
-    const int k1 = (int)((hr[1] - 1) / (unsigned long long)d.nij) + 1;
-    c->mg_k0 = k0;
-    c->mg_k1 = k1;
-    RET_IF_(cad_mg_setup(c));
+    RET_IF_(dot_to(c, c->yv, c->yv, SC_S, RED_PLAIN));
     RET_IF_(dot_to(c, c->bb, c->bb, SC_BN, RED_PLAIN));
-    double bn2 = 0.0;
-    RET_IF_(fetch_sc(c, SC_BN, 1, &bn2));
-    const double bnorm = std::sqrt(bn2);
-    const double tol = c->prm.cgepsr * bnorm + c->prm.cgepsa;
-    int iter = 0;
-    double rnorm = 0.0;
-    // explicit residual r = BB - A x into r and r0; returns ||r||
-    auto explicit_residual = [&](double *out) -> int {
-        RET_IF_(apply_a(c, x, t));
-        hipLaunchKernelGGL(k_bminus, vg, sb, 0, st, rng, mask, c->bb, t);
-        hipLaunchKernelGGL(k_copy_active, vg, sb, 0, st, rng, mask, t, r);
-        hipLaunchKernelGGL(k_copy_active, vg, sb, 0, st, rng, mask, t, r0);
-        CAD_CHECK(hipGetLastError());
-        RET_IF_(dot_to(c, r, r, SC_XN, RED_PLAIN));
-        double rr = 0.0;
-        RET_IF_(fetch_sc(c, SC_XN, 1, &rr));
-        *out = std::sqrt(rr);
-        return CAD_OK;
-    };
-    RET_IF_(explicit_residual(&rnorm));
-    int restarts = 0;
-    bool fail = false;
-    while (rnorm > tol && !fail) {
-        double rho = 1.0, alpha = 1.0, om = 1.0, rho1 = 0.0;
-        hipLaunchKernelGGL(k_fill, sg, sb, 0, st, nc, p, 0.0);
-        hipLaunchKernelGGL(k_fill, sg, sb, 0, st, nc, v, 0.0);
-        bool converged = false;
-        while (!converged) {
-            if (iter >= c->prm.icgmax) {
-                fail = true;
-                break;
-            }
-            iter++;
-            // (r0, r): explicit after a (re)start, else from the last update
-            if (rho1 == 0.0) {
-                RET_IF_(dot_to(c, r0, r, SC_W1, RED_PLAIN));
-                RET_IF_(fetch_sc(c, SC_W1, 1, &rho1));
-            }
-            if (!(std::fabs(rho1) > 0.0) || !std::isfinite(rho1)) break;   // breakdown: restart
-            const double beta = (rho1 / rho) * (alpha / om);
-            hipLaunchKernelGGL(k_bicg_p, vg, sb, 0, st, rng, mask, beta, om, r, v, p);
-            RET_IF_(cad_mg_apply(c, p, ph, k0, k1));
-            // v = A ph with (r0, v)
-            hipLaunchKernelGGL(k_mzax_dot<1>, rgd, rbd, 0, st, d, rng, c->ad, c->ali, c->alj, c->alk,
-                               c->aui, c->auj, c->auk, mask, ph, r0, v, c->red_partial);
-            RET_IF_(reduce_to(c, 1, SC_W2, RED_PLAIN));
-            double rv = 0.0;
-            RET_IF_(fetch_sc(c, SC_W2, 1, &rv));
-            if (!(std::fabs(rv) > 0.0) || !std::isfinite(rv)) break;
-            alpha = rho1 / rv;
-            hipLaunchKernelGGL(k_bicg_s, vg, sb, 0, st, rng, mask, alpha, r, v, s);
-            RET_IF_(cad_mg_apply(c, s, sh, k0, k1));
-            // t = A sh with (s, t) and (t, t)
-            hipLaunchKernelGGL(k_mzax_dot<2>, rgd, rbd, 0, st, d, rng, c->ad, c->ali, c->alj, c->alk,
-                               c->aui, c->auj, c->auk, mask, sh, s, t, c->red_partial);
-            RET_IF_(reduce_to(c, 2, SC_W1B, RED_PLAIN));
-            double ts[2] = {0.0, 0.0};
-            RET_IF_(fetch_sc(c, SC_W1B, 2, ts));
-            om = ts[1] > 0.0 ? ts[0] / ts[1] : 0.0;
-            hipLaunchKernelGGL(k_bicg_xr, rgd, rbd, 0, st, rng, mask, alpha, om, ph, sh, s, t, r0, x, r,
-                               c->red_partial);
-            RET_IF_(reduce_to(c, 2, SC_XN, RED_PLAIN));
-            double rn[2] = {0.0, 0.0};
-            RET_IF_(fetch_sc(c, SC_XN, 2, rn));
-            rnorm = std::sqrt(rn[0]);
-            if (!std::isfinite(rnorm)) {
-                fail = true;
-                break;
-            }
-            rho = rho1;
-            rho1 = rn[1];           // (r0, r) of the new residual
-            if (!(om != 0.0)) break;
-            converged = rnorm <= tol;
-        }
-        if (fail) break;
-        // the recurrence says converged (or broke down): the explicit residual decides
-        RET_IF_(explicit_residual(&rnorm));
-        if (rnorm > tol && ++restarts > 8) fail = true;
-    }
-    if (!fail) hipLaunchKernelGGL(k_copy_active, vg, sb, 0, st, rng, mask, x, x0);
-    CAD_CHECK(hipGetLastError());
-    *bnorm_out = bnorm;
-    *xnorm_out = bnorm > 0.0 ? rnorm / bnorm : rnorm;
-    *iter_out = fail ? (std::isfinite(rnorm) ? c->prm.icgmax + 1 : -2) : iter;
+    double rr = 0.0, bb = 0.0;
+    RET_IF_(fetch_sc(c, SC_S, 1, &rr));
+    RET_IF_(fetch_sc(c, SC_BN, 1, &bb));
+    *rel = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
     return CAD_OK;
 }
 
@@ -967,4 +792,39 @@ int cad_time_minv_impl(cad_ctx *c, int reps, float *ms_avg)
     hipEventDestroy(b);
     *ms_avg = ms / reps;
     return CAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pieces of the solve shared with the learned form (cad_learned.hip).
+
+// The active index range [n0, n1) of the context's INDC into act_rng and hr.
+int cad_ext_active_range(cad_ctx *c, unsigned long long hr[2])
+{
+    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, c->stream, c->act_rng, c->d.nc);
+    hipLaunchKernelGGL(k_active_range, dim3(1024), dim3(256), 0, c->stream, c->d.nc, c->f.indc,
+                       c->act_rng);
+    CAD_CHECK(hipGetLastError());
+    CAD_CHECK(hipMemcpyAsync(hr, c->act_rng, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    CAD_CHECK(hipStreamSynchronize(c->stream));
+    return CAD_OK;
+}
+
+// y = A x (active cells)
+int cad_ext_apply_a(cad_ctx *c, const double *x, double *y) { return apply_a(c, x, y); }
+
+// y = b - A x (active cells)
+int cad_ext_residual(cad_ctx *c, const double *x, double *y)
+{
+    RET_IF_(apply_a(c, x, y));
+    hipLaunchKernelGGL(k_bminus, dim3(VEC_BLOCKS), dim3(256), 0, c->stream, c->act_rng, c->f.indc, c->bb, y);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+// (x, y) over the active cells, to the host
+int cad_ext_dot(cad_ctx *c, const double *x, const double *y, double *out)
+{
+    RET_IF_(dot_to(c, x, y, SC_S, RED_PLAIN));
+    return fetch_sc(c, SC_S, 1, out);
 }

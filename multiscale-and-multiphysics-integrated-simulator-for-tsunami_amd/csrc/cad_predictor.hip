@@ -3,9 +3,9 @@
 // The reference computes nine flux arrays and three source arrays in separate
 // passes (VF_VFLXDU/V/W, VF_VGENE), stores them in WK01..WK12 and re-reads them
 // in VF_VEULER (Source code/vf_v1cal.f:126-227): about fifteen full-field
-// streams.  Here one kernel per velocity component evaluates, for each face it
-// owns, the six fluxes and the source that face's update needs and writes the
-// predicted velocity directly: the flux arrays never reach HBM.  A flux value
+// streams.  Here one kernel evaluates, for each cell's three faces, the six
+// fluxes and the source each face's update needs and writes the predicted
+// velocities directly: the flux arrays never reach HBM.  A flux value
 // depends only on its own location's inputs, so re-evaluating it for each of
 // the two faces that difference it gives bit-identical results to the stored
 // array.  The kernel is out of place (old u, v, w read, new component written to
@@ -155,25 +155,23 @@ __device__ __forceinline__ double ipow(double x, int n)
     return r;
 }
 
-// One face of component A: VF_VGENE source (vf_vgene.f:84-330) and
+// One face of component A (its value when the face is not updated): VF_VGENE
+// source (vf_vgene.f:84-330) and
 // VF_VEULER update with the VVMAX clip (vf_veuler.f:116-249).  DF selects
 // the Dupuit-Forchheimer drag of OPTION DRAG-DF (vf_vgene.f:111-146) in
 // place of the CD form (:102-110); SP the implicit damping of MODEL DAMP
 // after the clip (vf_veuler.f:131-166, 187-221, 243-277).
 template <int A, bool DF, bool SP>
-__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Damp dz,
-                                                 const double *__restrict__ u,
-                                                 const double *__restrict__ v,
-                                                 const double *__restrict__ w,
-                                                 const double *__restrict__ p,
-                                                 const double *__restrict__ bcq,
-                                                 const int *__restrict__ nf,
-                                                 const int *__restrict__ indc,
-                                                 double *__restrict__ out, cad_params prm)
+__device__ __forceinline__ double predict_face(const Dims &d, const Static &s, const DragDF &df,
+                                               const Damp &dz, const double *__restrict__ u,
+                                               const double *__restrict__ v,
+                                               const double *__restrict__ w,
+                                               const double *__restrict__ p,
+                                               const double *__restrict__ bcq,
+                                               const int *__restrict__ nf,
+                                               const int *__restrict__ indc, const cad_params &prm,
+                                               int i, int j, int k, long long n)
 {
-    int i, j, k;
-    if (!cell_of_thread(d, i, j, k)) return;
-    const long long n = cid(d, i, j, k);
     const double *q = vel<A>(u, v, w);
     const int idx[3] = {i, j, k};
     const int lo[3] = {A == 0 ? 3 : 2, A == 1 ? 3 : 2, A == 2 ? 3 : 2};
@@ -183,10 +181,7 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Da
     bool upd = idx[0] >= lo[0] && idx[0] <= hi[0] && idx[1] >= lo[1] && idx[1] <= hi[1] &&
                idx[2] >= lo[2] && idx[2] <= hi[2];
     if (upd) upd = ind[n] == 0 && nf[n - sa] * nf[n] == 0;
-    if (!upd) {
-        out[n] = q[n];
-        return;
-    }
+    if (!upd) return q[n];
     const double su = prm.scmvp, sc = 1.0 - prm.scmvp;
     const int ia = idx[A];
     // fluxes differenced by this face, in the x, y, z order of VF_VEULER
@@ -311,7 +306,32 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Da
         const double gg = met<A>(s, 6, ia) * (met<A>(s, 2, ia) * s.ggv[n] + met<A>(s, 2, ia - 1) * s.ggv[n - sa]);
         r = r * (g0 / (g0 + prm.dtnow * gg * dmx));
     }
-    out[n] = r;
+    return r;
+}
+
+// The three components of one cell's faces in one pass: the neighbourhood
+// each component reads (velocities, porosities, flags, viscosity) is shared,
+// so it comes from HBM once for all three instead of once per component.
+template <bool DF, bool SP>
+__global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Damp dz,
+                                                 const double *__restrict__ u,
+                                                 const double *__restrict__ v,
+                                                 const double *__restrict__ w,
+                                                 const double *__restrict__ p,
+                                                 const double *__restrict__ bcu,
+                                                 const double *__restrict__ bcv,
+                                                 const double *__restrict__ bcw,
+                                                 const int *__restrict__ nf,
+                                                 const int *__restrict__ indc, double *__restrict__ u2,
+                                                 double *__restrict__ v2, double *__restrict__ w2,
+                                                 cad_params prm)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    const long long n = cid(d, i, j, k);
+    u2[n] = predict_face<0, DF, SP>(d, s, df, dz, u, v, w, p, bcu, nf, indc, prm, i, j, k, n);
+    v2[n] = predict_face<1, DF, SP>(d, s, df, dz, u, v, w, p, bcv, nf, indc, prm, i, j, k, n);
+    w2[n] = predict_face<2, DF, SP>(d, s, df, dz, u, v, w, p, bcw, nf, indc, prm, i, j, k, n);
 }
 
 // VF_BWUWN boundary-list pass (vf_bwuwn.f:84-254): normal velocity on wall and
@@ -359,12 +379,8 @@ void launch_predict(cad_ctx *c, const Static &s, const DragDF &df, const Damp &d
     const Dims &d = c->d;
     dim3 b = cell_block(), g = cell_grid(d);
     Fields &f = c->f;
-    hipLaunchKernelGGL((k_predict<0, DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww,
-                       f.pp, f.bcu, f.nf, f.indc, c->u2, c->prm);
-    hipLaunchKernelGGL((k_predict<1, DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww,
-                       f.pp, f.bcv, f.nf, f.indc, c->v2, c->prm);
-    hipLaunchKernelGGL((k_predict<2, DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww,
-                       f.pp, f.bcw, f.nf, f.indc, c->w2, c->prm);
+    hipLaunchKernelGGL((k_predict<DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww, f.pp,
+                       f.bcu, f.bcv, f.bcw, f.nf, f.indc, c->u2, c->v2, c->w2, c->prm);
 }
 
 // The damping zones at the context's GRZ0, as VF_VEULER derives them on each

@@ -439,14 +439,21 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
     const int nsteps = len + (TJ - 1) + (TK - 1);
     const int nstage = (nsteps + S - 1) / S;
 
-    // results of stage b (in xo[b&1]): store q covers lines q*LPS .. q*LPS+LPS-1
+    // results of stage b (in xo[b&1]): store q covers lines q*LPS .. q*LPS+LPS-1.
+    // Every cell of the tile's active I range is stored, inactive ones as 0
+    // (the value the output array holds there), so that each line's window is
+    // one contiguous run and HBM sees whole lines rather than masked pieces;
+    // cells outside the range are not stored.
     auto store_stage = [&] __device__(int b) {
         const unsigned adv = stage_step * (unsigned)b;
 #pragma unroll
         for (int q = 0; q < S; q++) {
             const int L = q * LPS + lane / S, e = lane % S;
-            const double x = sm.xo[b & 1][L][e];
-            const unsigned o = st_off[q] == OOB || is_mark(x) ? OOB : st_off[q] + adv;
+            const int skl = L % TJ + L / TJ;
+            const int i = (DIR > 0 ? rg.x - skl + S * b : rg.y - (S - 1) + skl - S * b) + e;
+            double x = sm.xo[b & 1][L][e];
+            if (is_mark(x)) x = 0.0;
+            const unsigned o = st_off[q] == OOB || i < rg.x || i > rg.y ? OOB : st_off[q] + adv;
             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), rout, o, 0, 0);
         }
     };

@@ -1,0 +1,1067 @@
+// cad_unet.hip -- the learned preconditioner's network: a 3-level 3-D U-Net
+// (3x3x3 convolutions, 16/32/64 channels, 390,913 parameters) with BF16
+// weights and activations on the gfx950 matrix cores and FP32 accumulation,
+// forward and backward, and an Adam step, all hand-written.
+//
+// Layout.  Activations are channels-last BF16: voxel v holds its C channels
+// contiguously, so one lane's 8 consecutive channels are one 16-byte load.
+// A level's volumes (a batch of nb boxes of nx x ny x nz voxels) are stored
+// with a one-voxel zero halo on every side of every box, so a 3x3x3
+// convolution with zero padding reads its neighbours with no bounds tests and
+// boxes of a batch never see each other.  Kernels write interior voxels only;
+// halos stay zero from allocation.  A tensor is (pointer, channel stride):
+// the decoder's concatenation buffers are written in channel slices (the
+// skip connection's encoder output in the upper half, the upsampled coarse
+// features in the lower half), so the concatenation costs no copy.
+//
+// Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16:
+//   D (Cout x voxels) = W (Cout x K) * X^T (K x voxels),  K = 27*Cin taps x channels.
+// A lane holds A[m = l%16][k = 8*(l/16)..+7] (8 weights, packed so that it is
+// one 16-byte load), B[k = 8*(l/16)..+7][n = l%16] (8 channels of one tap of
+// voxel n: one 16-byte load) and D[m = 4*(l/16)+i][n = l%16] (4 consecutive
+// output channels of one voxel: one 8-byte store).  A wave computes MT*16
+// output channels of NT*16 voxels along x.  The backward pass for the
+// activations is the same kernel with the taps flipped and the weight
+// matrix transposed (packed that way), and the weight gradients are an FP32
+// reduction over voxels.
+#include "cad_common.h"
+#include "cad_unet.h"
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ long long vox(const UGeo &g, int x, int y, int z, int b)
+{
+    return (long long)(x + 1) + (long long)g.px * ((y + 1) + (long long)g.py * ((z + 1) + (long long)g.pz * b));
+}
+
+// interior voxel number u (x fastest, then y, z, box) -> padded index
+__device__ __forceinline__ long long vox_of(const UGeo &g, long long u)
+{
+    const int x = (int)(u % g.nx);
+    u /= g.nx;
+    const int y = (int)(u % g.ny);
+    u /= g.ny;
+    const int z = (int)(u % g.nz);
+    const int b = (int)(u / g.nz);
+    return vox(g, x, y, z, b);
+}
+
+__device__ __forceinline__ long long tap_off(const UGeo &g, int t)
+{
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    return dx + (long long)g.px * (dy + (long long)g.py * dz);
+}
+
+// 3x3x3 convolution, CIN -> 16*MT channels, NT*16 voxels along x per wave.
+// wp: packed A fragments [MT][KC][64 lanes] of 8 BF16.  bias may be null.
+template <int CIN, int MT, int NT, bool RELU>
+__global__ void __launch_bounds__(256) k_conv3(UGeo g, const bf16 *__restrict__ in, int cs_in,
+                                               const bf16x8 *__restrict__ wp,
+                                               const float *__restrict__ bias,
+                                               bf16 *__restrict__ out, int cs_out)
+{
+    constexpr int KC = (27 * CIN + 31) / 32;
+    const int lane = threadIdx.x & 63;
+    long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ntx = g.nx / (16 * NT);
+    const int xt = (int)(w % ntx);
+    w /= ntx;
+    const int y = (int)(w % g.ny);
+    w /= g.ny;
+    const int z = (int)(w % g.nz);
+    const int b = (int)(w / g.nz);
+    if (b >= g.nb) return;
+    const int n = lane & 15, kg = lane >> 4;
+    const long long v0 = vox(g, xt * 16 * NT + n, y, z, b);
+    f4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[m][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int q = 0; q < KC; q++) {
+        const int k0 = 32 * q + 8 * kg;
+        int t = k0 / CIN;
+        const int ci0 = k0 % CIN;
+        if (t > 26) t = 26;   // K padding: the weights there are zero
+        const long long off = tap_off(g, t);
+        bf16x8 bf[NT];
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            bf[j] = *reinterpret_cast<const bf16x8 *>(in + (v0 + 16 * j + off) * cs_in + ci0);
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const bf16x8 af = wp[((long long)m * KC + q) * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[j], acc[m][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int co = 16 * m + 4 * kg;
+        float bs[4] = {0.f, 0.f, 0.f, 0.f};
+        if (bias)
+#pragma unroll
+            for (int i = 0; i < 4; i++) bs[i] = bias[co + i];
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            bf16x4 o;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float r = acc[m][j][i] + bs[i];
+                if (RELU) r = r > 0.f ? r : 0.f;
+                o[i] = (bf16)r;
+            }
+            *reinterpret_cast<bf16x4 *>(out + (v0 + 16 * j) * cs_out + co) = o;
+        }
+    }
+}
+
+// Packs FP32 weights W[t][ci][co] (27 x cin x cout) into the A fragments of
+// k_conv3 for cin -> cout; flip: the backward-data operator, whose input has
+// cout channels and output cin, W'[t][co][ci] = W[26-t][ci][co].
+__global__ void k_pack(const float *__restrict__ w, int cin, int cout, int flip, bf16 *__restrict__ wp)
+{
+    const int ki = flip ? cout : cin;   // input channels of the packed operator
+    const int ko = flip ? cin : cout;   // its output channels
+    const int kc = (27 * ki + 31) / 32, mt = ko / 16;
+    const long long total = (long long)mt * kc * 64 * 8;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int el = (int)(e % 8);
+        const int lane = (int)((e / 8) % 64);
+        const int q = (int)((e / 512) % kc);
+        const int m = (int)(e / (512LL * kc));
+        const int k = 32 * q + 8 * (lane >> 4) + el;
+        const int o = 16 * m + (lane & 15);
+        float v = 0.f;
+        const int t = k / ki, c = k % ki;
+        if (t < 27) {
+            if (!flip)
+                v = w[((long long)t * cin + c) * cout + o];
+            else
+                v = w[((long long)(26 - t) * cin + o) * cout + c];
+        }
+        wp[e] = (bf16)v;
+    }
+}
+
+// FP32 weight gradient of one 3x3x3 layer: dW[t][ci][co] += sum_v X[v+off_t][ci] dY[v][co]
+// and (tap 13's blocks) db[co] += sum_v dY[v][co].  Block: one tap, a range of
+// interior voxels; threads own 4x4 (ci, co) blocks, several thread groups
+// split each tile of voxels; partial sums are added atomically.
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_dw(UGeo g, const bf16 *__restrict__ x, int cs_x,
+                                                 const bf16 *__restrict__ dy, int cs_dy,
+                                                 long long per_block, float *__restrict__ dw,
+                                                 float *__restrict__ db)
+{
+    constexpr int TV = 32;
+    constexpr int NBLK = (CIN / 4) * (COUT / 4);
+    constexpr int GROUPS = NBLK >= 256 ? 1 : 256 / NBLK;
+    __shared__ float xs[TV][CIN + 1];
+    __shared__ float ds[TV][COUT + 1];
+    __shared__ float red[GROUPS > 1 ? GROUPS : 1][NBLK > 256 ? 1 : NBLK][17];
+    const int t = blockIdx.y;
+    const long long off = tap_off(g, t);
+    const long long total = (long long)g.nx * g.ny * g.nz * g.nb;
+    const long long u0 = blockIdx.x * per_block;
+    const long long u1 = u0 + per_block < total ? u0 + per_block : total;
+    const int tid = threadIdx.x;
+    // thread -> (group, 4x4 block)
+    const int grp = tid / (NBLK >= 256 ? 256 : NBLK);
+    const int blk0 = tid % (NBLK >= 256 ? 256 : NBLK);
+    constexpr int BPT = NBLK > 256 ? NBLK / 256 : 1;   // 4x4 blocks per thread
+    float acc[BPT][16];
+    float dbacc[BPT][4];
+#pragma unroll
+    for (int p = 0; p < BPT; p++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[p][e] = 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; e++) dbacc[p][e] = 0.f;
+    }
+    for (long long ub = u0; ub < u1; ub += TV) {
+        __syncthreads();
+        for (int e = tid; e < TV * (CIN + COUT); e += 256) {
+            const int vi = e / (CIN + COUT), c = e % (CIN + COUT);
+            const long long u = ub + vi;
+            float val = 0.f;
+            if (u < u1) {
+                const long long v = vox_of(g, u);
+                val = c < CIN ? (float)x[(v + off) * cs_x + c] : (float)dy[v * cs_dy + (c - CIN)];
+            }
+            if (c < CIN)
+                xs[vi][c] = val;
+            else
+                ds[vi][c - CIN] = val;
+        }
+        __syncthreads();
+        if (grp < GROUPS) {
+#pragma unroll
+            for (int p = 0; p < BPT; p++) {
+                const int blk = blk0 + p * 256;
+                const int bi = blk / (COUT / 4), bo = blk % (COUT / 4);
+                for (int vi = grp; vi < TV; vi += GROUPS) {
+                    float xv[4], dv[4];
+#pragma unroll
+                    for (int a = 0; a < 4; a++) {
+                        xv[a] = xs[vi][4 * bi + a];
+                        dv[a] = ds[vi][4 * bo + a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[p][a * 4 + c] += xv[a] * dv[c];
+                    if (bi == 0)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) dbacc[p][c] += dv[c];
+                }
+            }
+        }
+    }
+    // reduce over the thread groups, then one atomic per weight
+    if (GROUPS > 1) {
+        __syncthreads();
+        if (grp < GROUPS) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) red[grp][blk0][e] = acc[0][e];
+        }
+        __syncthreads();
+        if (grp == 0) {
+            for (int gq = 1; gq < GROUPS; gq++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[0][e] += red[gq][blk0][e];
+        }
+        __syncthreads();
+        if (grp < GROUPS) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) red[grp][blk0][e] = dbacc[0][e];
+        }
+        __syncthreads();
+        if (grp == 0) {
+            for (int gq = 1; gq < GROUPS; gq++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) dbacc[0][e] += red[gq][blk0][e];
+        }
+    }
+    if (grp != 0) return;
+#pragma unroll
+    for (int p = 0; p < BPT; p++) {
+        const int blk = blk0 + p * 256;
+        const int bi = blk / (COUT / 4), bo = blk % (COUT / 4);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                atomicAdd(&dw[((long long)t * CIN + 4 * bi + a) * COUT + 4 * bo + c], acc[p][a * 4 + c]);
+        if (t == 13 && bi == 0 && db)
+#pragma unroll
+            for (int c = 0; c < 4; c++) atomicAdd(&db[4 * bo + c], dbacc[p][c]);
+    }
+}
+
+// 2x2x2 average pooling of C channels (fine level gf -> coarse level gc)
+__global__ void k_pool(UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
+                       bf16 *__restrict__ out, int cs_out)
+{
+    const int cg = C / 8;
+    const long long total = (long long)gc.nx * gc.ny * gc.nz * gc.nb * cg;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c0 = (int)(e % cg) * 8;
+        long long u = e / cg;
+        const int x = (int)(u % gc.nx);
+        u /= gc.nx;
+        const int y = (int)(u % gc.ny);
+        u /= gc.ny;
+        const int z = (int)(u % gc.nz);
+        const int b = (int)(u / gc.nz);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const long long vf = vox(gf, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2), b);
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(in + vf * cs_in + c0);
+#pragma unroll
+            for (int i = 0; i < 8; i++) s[i] += (float)a[i];
+        }
+        bf16x8 o;
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = (bf16)(0.125f * s[i]);
+        *reinterpret_cast<bf16x8 *>(out + vox(gc, x, y, z, b) * cs_out + c0) = o;
+    }
+}
+
+// backward of k_pool: the fine gradient (accumulated into) += coarse / 8
+__global__ void k_pool_bwd(UGeo gc, UGeo gf, const bf16 *__restrict__ dc, int cs_c, int C,
+                           bf16 *__restrict__ df, int cs_f)
+{
+    const int cg = C / 8;
+    const long long total = (long long)gf.nx * gf.ny * gf.nz * gf.nb * cg;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c0 = (int)(e % cg) * 8;
+        long long u = e / cg;
+        const int x = (int)(u % gf.nx);
+        u /= gf.nx;
+        const int y = (int)(u % gf.ny);
+        u /= gf.ny;
+        const int z = (int)(u % gf.nz);
+        const int b = (int)(u / gf.nz);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dc + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_c + c0);
+        bf16x8 *p = reinterpret_cast<bf16x8 *>(df + vox(gf, x, y, z, b) * cs_f + c0);
+        bf16x8 o = *p;
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = (bf16)((float)o[i] + 0.125f * (float)a[i]);
+        *p = o;
+    }
+}
+
+// nearest-neighbour upsampling of C channels (coarse gc -> fine gf)
+__global__ void k_up(UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
+                     bf16 *__restrict__ out, int cs_out)
+{
+    const int cg = C / 8;
+    const long long total = (long long)gf.nx * gf.ny * gf.nz * gf.nb * cg;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c0 = (int)(e % cg) * 8;
+        long long u = e / cg;
+        const int x = (int)(u % gf.nx);
+        u /= gf.nx;
+        const int y = (int)(u % gf.ny);
+        u /= gf.ny;
+        const int z = (int)(u % gf.nz);
+        const int b = (int)(u / gf.nz);
+        *reinterpret_cast<bf16x8 *>(out + vox(gf, x, y, z, b) * cs_out + c0) =
+            *reinterpret_cast<const bf16x8 *>(in + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_in + c0);
+    }
+}
+
+// backward of k_up: coarse gradient = sum of its 8 fine voxels' gradients
+__global__ void k_up_bwd(UGeo gc, UGeo gf, const bf16 *__restrict__ df, int cs_f, int C,
+                         bf16 *__restrict__ dc, int cs_c)
+{
+    const int cg = C / 8;
+    const long long total = (long long)gc.nx * gc.ny * gc.nz * gc.nb * cg;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c0 = (int)(e % cg) * 8;
+        long long u = e / cg;
+        const int x = (int)(u % gc.nx);
+        u /= gc.nx;
+        const int y = (int)(u % gc.ny);
+        u /= gc.ny;
+        const int z = (int)(u % gc.nz);
+        const int b = (int)(u / gc.nz);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const long long vf = vox(gf, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2), b);
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(df + vf * cs_f + c0);
+#pragma unroll
+            for (int i = 0; i < 8; i++) s[i] += (float)a[i];
+        }
+        bf16x8 o;
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = (bf16)s[i];
+        *reinterpret_cast<bf16x8 *>(dc + vox(gc, x, y, z, b) * cs_c + c0) = o;
+    }
+}
+
+// dA *= (A > 0) over C channels: the gradient through a ReLU output A
+__global__ void k_relu_bwd(UGeo g, const bf16 *__restrict__ a, int cs_a, int C, bf16 *__restrict__ d,
+                           int cs_d)
+{
+    const int cg = C / 8;
+    const long long total = (long long)g.nx * g.ny * g.nz * g.nb * cg;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c0 = (int)(e % cg) * 8;
+        const long long v = vox_of(g, e / cg);
+        const bf16x8 av = *reinterpret_cast<const bf16x8 *>(a + v * cs_a + c0);
+        bf16x8 *p = reinterpret_cast<bf16x8 *>(d + v * cs_d + c0);
+        bf16x8 o = *p;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (!((float)av[i] > 0.f)) o[i] = (bf16)0.f;
+        *p = o;
+    }
+}
+
+// output layer: y[u] = b + sum_c w[c] h[v][c] (FP32, interior order u)
+__global__ void k_out(UGeo g, const bf16 *__restrict__ h, const float *__restrict__ w, float *__restrict__ y)
+{
+    const long long total = (long long)g.nx * g.ny * g.nz * g.nb;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const long long v = vox_of(g, u);
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(h + v * 16);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(h + v * 16 + 8);
+        float s = w[16];
+#pragma unroll
+        for (int i = 0; i < 8; i++) s += w[i] * (float)a0[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) s += w[8 + i] * (float)a1[i];
+        y[u] = s;
+    }
+}
+
+// backward of k_out: dh[v][c] = w[c] dy[u]; dw[c] += sum dy h[c]; db += sum dy
+__global__ void __launch_bounds__(256) k_out_bwd(UGeo g, const bf16 *__restrict__ h,
+                                                 const float *__restrict__ w,
+                                                 const float *__restrict__ dy, bf16 *__restrict__ dh,
+                                                 float *__restrict__ dwb)
+{
+    __shared__ float part[256][17];
+    float acc[17];
+#pragma unroll
+    for (int c = 0; c < 17; c++) acc[c] = 0.f;
+    const long long total = (long long)g.nx * g.ny * g.nz * g.nb;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const long long v = vox_of(g, u);
+        const float d = dy[u];
+        bf16x8 o0, o1;
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(h + v * 16);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(h + v * 16 + 8);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            o0[i] = (bf16)(w[i] * d);
+            o1[i] = (bf16)(w[8 + i] * d);
+            acc[i] += d * (float)a0[i];
+            acc[8 + i] += d * (float)a1[i];
+        }
+        acc[16] += d;
+        *reinterpret_cast<bf16x8 *>(dh + v * 16) = o0;
+        *reinterpret_cast<bf16x8 *>(dh + v * 16 + 8) = o1;
+    }
+#pragma unroll
+    for (int c = 0; c < 17; c++) part[threadIdx.x][c] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < 17) {
+        float s = 0.f;
+        for (int t = 0; t < 256; t++) s += part[t][threadIdx.x];
+        atomicAdd(&dwb[threadIdx.x], s);
+    }
+}
+
+// masked relative squared error per box: loss_b = sum m (y - e)^2 / sum m e^2;
+// dy = 2 m (y - e) / (sum m e^2 * nb); loss summed into *loss (sum over boxes / nb)
+__global__ void __launch_bounds__(256) k_loss(UGeo g, const float *__restrict__ y,
+                                              const float *__restrict__ e, const float *__restrict__ m,
+                                              const float *__restrict__ esq, float *__restrict__ dy,
+                                              float *__restrict__ loss)
+{
+    __shared__ float part[256];
+    float acc = 0.f;
+    const long long per = (long long)g.nx * g.ny * g.nz;
+    const long long total = per * g.nb;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(u / per);
+        const float s = esq[b] > 0.f ? 1.f / (esq[b] * (float)g.nb) : 0.f;
+        const float r = m[u] * (y[u] - e[u]);
+        dy[u] = 2.f * r * s;
+        acc += r * r * s;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+// per box: sum m e^2
+__global__ void __launch_bounds__(256) k_esq(UGeo g, const float *__restrict__ e,
+                                             const float *__restrict__ m, float *__restrict__ esq)
+{
+    __shared__ float part[256];
+    const long long per = (long long)g.nx * g.ny * g.nz;
+    const int b = blockIdx.y;
+    float acc = 0.f;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < per;
+         u += (long long)gridDim.x * blockDim.x) {
+        const float v = e[b * per + u];
+        acc += m[b * per + u] * v * v;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&esq[b], part[0]);
+}
+
+__global__ void k_adam(long long n, float *__restrict__ p, const float *__restrict__ gr,
+                       float *__restrict__ m1, float *__restrict__ m2, float lr, float b1, float b2,
+                       float c1, float c2, float eps)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float g = gr[i];
+        const float a = b1 * m1[i] + (1.f - b1) * g;
+        const float b = b2 * m2[i] + (1.f - b2) * g * g;
+        m1[i] = a;
+        m2[i] = b;
+        p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+    }
+}
+
+// zeroes plane z (interior numbering) of box 0, all cs channels of each voxel
+__global__ void k_zero_plane(UGeo g, int z, bf16 *__restrict__ a, int cs)
+{
+    const long long total = (long long)g.px * g.py * cs;
+    const long long base = ((long long)(z + 1) * g.py) * g.px * cs;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x)
+        a[base + e] = (bf16)0.f;
+}
+
+__global__ void k_zero_f(long long n, float *p)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+// ---------------------------------------------------------------- layers
+// The network's 3x3x3 layers in order: (level, cin, cout).
+struct LayerDef {
+    int level, cin, cout;
+};
+constexpr LayerDef LAYERS[] = {
+    {0, 8, 16},  {0, 16, 16},  // e1a, e1b
+    {1, 16, 32}, {1, 32, 32},  // e2a, e2b
+    {2, 32, 64}, {2, 64, 64},  // b3a, b3b
+    {2, 64, 32},               // u3 (coarse conv before upsampling)
+    {1, 64, 32}, {1, 32, 32},  // d2a, d2b
+    {1, 32, 16},               // u2
+    {0, 32, 16}, {0, 16, 16},  // d1a, d1b
+};
+constexpr int NL = sizeof(LAYERS) / sizeof(LAYERS[0]);
+enum { E1A, E1B, E2A, E2B, B3A, B3B, U3, D2A, D2B, U2, D1A, D1B };
+
+unsigned grid_for(long long n) { return (unsigned)std::min<long long>((n + 255) / 256, 8192); }
+
+}  // namespace
+
+struct cad_unet {
+    int device;
+    hipStream_t stream;
+    bool own_stream;
+    UGeo g[3];
+    long long wofs[NL], bofs[NL], out_ofs;
+    long long nparam;
+    float *p, *gr, *m1, *m2;        // parameters, gradients, Adam moments (FP32)
+    bf16 *wp[NL], *wt[NL];          // packed forward and backward-data weights
+    long long wp_n[NL], wt_n[NL];
+    // activations (channels-last BF16 with halos)
+    bf16 *x, *h1a, *cat1, *p1, *h2a, *cat2, *p2, *h3a, *h3b, *h3c, *h2d, *h2e, *h2f, *h1d, *h1e;
+    // gradients
+    bf16 *dh1e, *dh1d, *dcat1, *dh2f, *dh2e, *dh2d, *dcat2, *dh3c, *dh3b, *dh3a, *dp2, *dh2a, *dp1, *dh1a;
+    float *y, *dy, *esq, *loss;
+    bool train;
+    long long step;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T>
+int ualloc(cad_unet *u, T **p, size_t n)
+{
+    void *q = nullptr;
+    if (hipMalloc(&q, n * sizeof(T) + 256) != hipSuccess) {
+        cad_set_error("cad_unet: hipMalloc of %zu bytes failed", n * sizeof(T));
+        return CAD_ERR_HIP;
+    }
+    if (hipMemsetAsync(q, 0, n * sizeof(T) + 256, u->stream) != hipSuccess) {
+        cad_set_error("cad_unet: hipMemset failed");
+        return CAD_ERR_HIP;
+    }
+    u->allocs.push_back(q);
+    *p = reinterpret_cast<T *>(q);
+    return CAD_OK;
+}
+
+long long padded(const UGeo &g) { return (long long)g.px * g.py * g.pz * g.nb; }
+
+template <int CIN>
+int conv_ci(cad_unet *u, const UGeo &g, const bf16 *in, int cs_in, const bf16 *wp, const float *bias,
+            bf16 *out, int cs_out, int cout, bool relu)
+{
+    const int nt = (g.nx % 64) == 0 ? 4 : 1;
+    const long long waves = (long long)(g.nx / (16 * nt)) * g.ny * g.nz * g.nb;
+    const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
+    const bf16x8 *w = reinterpret_cast<const bf16x8 *>(wp);
+#define CAD_CONV(MT, NT, R) \
+    hipLaunchKernelGGL((k_conv3<CIN, MT, NT, R>), grid, blk, 0, u->stream, g, in, cs_in, w, bias, out, cs_out)
+#define CAD_CONV_NT(MT, R) \
+    do {                   \
+        if (nt == 4)       \
+            CAD_CONV(MT, 4, R); \
+        else               \
+            CAD_CONV(MT, 1, R); \
+    } while (0)
+    if (relu) {
+        if (cout == 16) CAD_CONV_NT(1, true);
+        else if (cout == 32) CAD_CONV_NT(2, true);
+        else if (cout == 64) CAD_CONV_NT(4, true);
+    } else {
+        if (cout == 16) CAD_CONV_NT(1, false);
+        else if (cout == 32) CAD_CONV_NT(2, false);
+        else if (cout == 64) CAD_CONV_NT(4, false);
+    }
+#undef CAD_CONV_NT
+#undef CAD_CONV
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+int conv(cad_unet *u, const UGeo &g, int cin, const bf16 *in, int cs_in, const bf16 *wp, const float *bias,
+         bf16 *out, int cs_out, int cout, bool relu)
+{
+    switch (cin) {
+    case 8: return conv_ci<8>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
+    case 16: return conv_ci<16>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
+    case 32: return conv_ci<32>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
+    case 64: return conv_ci<64>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
+    }
+    cad_set_error("cad_unet: no convolution for %d input channels", cin);
+    return CAD_ERR_ARG;
+}
+
+int layer_fwd(cad_unet *u, int l, const bf16 *in, int cs_in, bf16 *out, int cs_out)
+{
+    const LayerDef &L = LAYERS[l];
+    return conv(u, u->g[L.level], L.cin, in, cs_in, u->wp[l], u->p + u->bofs[l], out, cs_out, L.cout, true);
+}
+
+// gradient of the layer's input (from dY at its output) and of its weights
+int layer_bwd(cad_unet *u, int l, const bf16 *in, int cs_in, const bf16 *dy, int cs_dy, bf16 *dx, int cs_dx)
+{
+    const LayerDef &L = LAYERS[l];
+    const UGeo &g = u->g[L.level];
+    if (dx) {
+        int rc = conv(u, g, L.cout, dy, cs_dy, u->wt[l], nullptr, dx, cs_dx, L.cin, false);
+        if (rc) return rc;
+    }
+    const long long total = (long long)g.nx * g.ny * g.nz * g.nb;
+    const long long per = 4096;
+    const dim3 grid((unsigned)((total + per - 1) / per), 27), blk(256);
+    float *dw = u->gr + u->wofs[l], *db = u->gr + u->bofs[l];
+#define CAD_DW(CI, CO)                                                                                \
+    if (L.cin == CI && L.cout == CO)                                                                  \
+        hipLaunchKernelGGL((k_conv_dw<CI, CO>), grid, blk, 0, u->stream, g, in, cs_in, dy, cs_dy, per, dw, db);
+    CAD_DW(8, 16) CAD_DW(16, 16) CAD_DW(16, 32) CAD_DW(32, 32) CAD_DW(32, 64) CAD_DW(64, 64)
+    CAD_DW(64, 32) CAD_DW(32, 16)
+#undef CAD_DW
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+int pack_all(cad_unet *u)
+{
+    for (int l = 0; l < NL; l++) {
+        const LayerDef &L = LAYERS[l];
+        hipLaunchKernelGGL(k_pack, dim3(grid_for(u->wp_n[l])), dim3(256), 0, u->stream, u->p + u->wofs[l],
+                           L.cin, L.cout, 0, u->wp[l]);
+        if (u->train && l != E1A)
+            hipLaunchKernelGGL(k_pack, dim3(grid_for(u->wt_n[l])), dim3(256), 0, u->stream,
+                               u->p + u->wofs[l], L.cin, L.cout, 1, u->wt[l]);
+    }
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+template <typename K, typename... A>
+int ew(cad_unet *u, K kern, long long n, A... args)
+{
+    hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(256), 0, u->stream, args...);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+long long nvox(const UGeo &g) { return (long long)g.nx * g.ny * g.nz * g.nb; }
+
+}  // namespace
+
+#define RET_IF(x)              \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+int cad_unet_forward_impl(cad_unet *u)
+{
+    const UGeo *g = u->g;
+    RET_IF(layer_fwd(u, E1A, u->x, 8, u->h1a, 16));
+    RET_IF(layer_fwd(u, E1B, u->h1a, 16, u->cat1 + 16, 32));
+    RET_IF(ew(u, k_pool, nvox(g[1]) * 2, g[1], g[0], u->cat1 + 16, 32, 16, u->p1, 16));
+    RET_IF(layer_fwd(u, E2A, u->p1, 16, u->h2a, 32));
+    RET_IF(layer_fwd(u, E2B, u->h2a, 32, u->cat2 + 32, 64));
+    RET_IF(ew(u, k_pool, nvox(g[2]) * 4, g[2], g[1], u->cat2 + 32, 64, 32, u->p2, 32));
+    RET_IF(layer_fwd(u, B3A, u->p2, 32, u->h3a, 64));
+    RET_IF(layer_fwd(u, B3B, u->h3a, 64, u->h3b, 64));
+    RET_IF(layer_fwd(u, U3, u->h3b, 64, u->h3c, 32));
+    RET_IF(ew(u, k_up, nvox(g[1]) * 4, g[2], g[1], u->h3c, 32, 32, u->cat2, 64));
+    RET_IF(layer_fwd(u, D2A, u->cat2, 64, u->h2d, 32));
+    RET_IF(layer_fwd(u, D2B, u->h2d, 32, u->h2e, 32));
+    RET_IF(layer_fwd(u, U2, u->h2e, 32, u->h2f, 16));
+    RET_IF(ew(u, k_up, nvox(g[0]) * 2, g[1], g[0], u->h2f, 16, 16, u->cat1, 32));
+    RET_IF(layer_fwd(u, D1A, u->cat1, 32, u->h1d, 16));
+    RET_IF(layer_fwd(u, D1B, u->h1d, 16, u->h1e, 16));
+    RET_IF(ew(u, k_out, nvox(g[0]), g[0], u->h1e, u->p + u->out_ofs, u->y));
+    return CAD_OK;
+}
+
+static int backward(cad_unet *u)
+{
+    const UGeo *g = u->g;
+    float *gout = u->gr + u->out_ofs;
+    hipLaunchKernelGGL(k_out_bwd, dim3(512), dim3(256), 0, u->stream, g[0], u->h1e, u->p + u->out_ofs, u->dy,
+                       u->dh1e, gout);
+    CAD_CHECK(hipGetLastError());
+    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->h1e, 16, 16, u->dh1e, 16));
+    RET_IF(layer_bwd(u, D1B, u->h1d, 16, u->dh1e, 16, u->dh1d, 16));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->h1d, 16, 16, u->dh1d, 16));
+    RET_IF(layer_bwd(u, D1A, u->cat1, 32, u->dh1d, 16, u->dcat1, 32));
+    // lower half of cat1: the upsampled h2f
+    RET_IF(ew(u, k_up_bwd, nvox(g[1]) * 2, g[1], g[0], u->dcat1, 32, 16, u->dh2f, 16));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 2, g[1], u->h2f, 16, 16, u->dh2f, 16));
+    RET_IF(layer_bwd(u, U2, u->h2e, 32, u->dh2f, 16, u->dh2e, 32));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->h2e, 32, 32, u->dh2e, 32));
+    RET_IF(layer_bwd(u, D2B, u->h2d, 32, u->dh2e, 32, u->dh2d, 32));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->h2d, 32, 32, u->dh2d, 32));
+    RET_IF(layer_bwd(u, D2A, u->cat2, 64, u->dh2d, 32, u->dcat2, 64));
+    RET_IF(ew(u, k_up_bwd, nvox(g[2]) * 4, g[2], g[1], u->dcat2, 64, 32, u->dh3c, 32));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[2]) * 4, g[2], u->h3c, 32, 32, u->dh3c, 32));
+    RET_IF(layer_bwd(u, U3, u->h3b, 64, u->dh3c, 32, u->dh3b, 64));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[2]) * 8, g[2], u->h3b, 64, 64, u->dh3b, 64));
+    RET_IF(layer_bwd(u, B3B, u->h3a, 64, u->dh3b, 64, u->dh3a, 64));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[2]) * 8, g[2], u->h3a, 64, 64, u->dh3a, 64));
+    RET_IF(layer_bwd(u, B3A, u->p2, 32, u->dh3a, 64, u->dp2, 32));
+    // upper half of cat2 (the skip, e2b's output): decoder part + pooling part
+    RET_IF(ew(u, k_pool_bwd, nvox(g[1]) * 4, g[2], g[1], u->dp2, 32, 32, u->dcat2 + 32, 64));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->cat2 + 32, 64, 32, u->dcat2 + 32, 64));
+    RET_IF(layer_bwd(u, E2B, u->h2a, 32, u->dcat2 + 32, 64, u->dh2a, 32));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->h2a, 32, 32, u->dh2a, 32));
+    RET_IF(layer_bwd(u, E2A, u->p1, 16, u->dh2a, 32, u->dp1, 16));
+    RET_IF(ew(u, k_pool_bwd, nvox(g[0]) * 2, g[1], g[0], u->dp1, 16, 16, u->dcat1 + 16, 32));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->cat1 + 16, 32, 16, u->dcat1 + 16, 32));
+    RET_IF(layer_bwd(u, E1B, u->h1a, 16, u->dcat1 + 16, 32, u->dh1a, 16));
+    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->h1a, 16, 16, u->dh1a, 16));
+    RET_IF(layer_bwd(u, E1A, u->x, 8, u->dh1a, 16, nullptr, 0));
+    return CAD_OK;
+}
+
+extern "C" {
+
+// multiply-adds per level-1 voxel of one forward pass
+double cad_unet_mac_per_voxel(void)
+{
+    double m = 16.0;
+    for (int l = 0; l < NL; l++) m += 27.0 * LAYERS[l].cin * LAYERS[l].cout / (double)(1 << (3 * LAYERS[l].level));
+    return m;
+}
+
+int cad_unet_param_count(void)
+{
+    long long n = 17;
+    for (int l = 0; l < NL; l++) n += 27LL * LAYERS[l].cin * LAYERS[l].cout + LAYERS[l].cout;
+    return (int)n;
+}
+
+int cad_unet_create(int nx, int ny, int nz, int nb, int train, void *stream, cad_unet **out)
+{
+    if (!out || nx < 64 || nx % 64 || ny < 4 || ny % 4 || nz < 4 || nz % 4 || nb < 1) {
+        cad_set_error("cad_unet_create: box %d x %d x %d (x a multiple of 64, y and z of 4), batch %d",
+                      nx, ny, nz, nb);
+        return CAD_ERR_ARG;
+    }
+    *out = nullptr;
+    cad_unet *u = new cad_unet();
+    CAD_CHECK(hipGetDevice(&u->device));
+    if (stream) {
+        u->stream = reinterpret_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete u;
+            cad_set_error("cad_unet_create: hipStreamCreate failed");
+            return CAD_ERR_HIP;
+        }
+        u->own_stream = true;
+    }
+    u->train = train != 0;
+    for (int l = 0; l < 3; l++) {
+        UGeo &g = u->g[l];
+        g.nx = nx >> l;
+        g.ny = ny >> l;
+        g.nz = nz >> l;
+        g.nb = nb;
+        g.px = g.nx + 2;
+        g.py = g.ny + 2;
+        g.pz = g.nz + 2;
+    }
+    long long o = 0;
+    for (int l = 0; l < NL; l++) {
+        u->wofs[l] = o;
+        o += 27LL * LAYERS[l].cin * LAYERS[l].cout;
+        u->bofs[l] = o;
+        o += LAYERS[l].cout;
+    }
+    u->out_ofs = o;
+    o += 17;
+    u->nparam = o;
+    int rc = 0;
+    auto A = [&](auto **p, size_t n) {
+        if (!rc) rc = ualloc(u, p, n);
+    };
+    A(&u->p, u->nparam);
+    for (int l = 0; l < NL; l++) {
+        const LayerDef &L = LAYERS[l];
+        u->wp_n[l] = (long long)(L.cout / 16) * ((27 * L.cin + 31) / 32) * 64 * 8;
+        A(&u->wp[l], u->wp_n[l]);
+        if (u->train && l != E1A) {
+            u->wt_n[l] = (long long)(L.cin / 16) * ((27 * L.cout + 31) / 32) * 64 * 8;
+            A(&u->wt[l], u->wt_n[l]);
+        }
+    }
+    const long long v0 = padded(u->g[0]), v1 = padded(u->g[1]), v2 = padded(u->g[2]);
+    A(&u->x, v0 * 8);
+    A(&u->h1a, v0 * 16);
+    A(&u->cat1, v0 * 32);
+    A(&u->p1, v1 * 16);
+    A(&u->h2a, v1 * 32);
+    A(&u->cat2, v1 * 64);
+    A(&u->p2, v2 * 32);
+    A(&u->h3a, v2 * 64);
+    A(&u->h3b, v2 * 64);
+    A(&u->h3c, v2 * 32);
+    A(&u->h2d, v1 * 32);
+    A(&u->h2e, v1 * 32);
+    A(&u->h2f, v1 * 16);
+    A(&u->h1d, v0 * 16);
+    A(&u->h1e, v0 * 16);
+    A(&u->y, nvox(u->g[0]));
+    if (u->train) {
+        A(&u->gr, u->nparam);
+        A(&u->m1, u->nparam);
+        A(&u->m2, u->nparam);
+        A(&u->dh1e, v0 * 16);
+        A(&u->dh1d, v0 * 16);
+        A(&u->dcat1, v0 * 32);
+        A(&u->dh2f, v1 * 16);
+        A(&u->dh2e, v1 * 32);
+        A(&u->dh2d, v1 * 32);
+        A(&u->dcat2, v1 * 64);
+        A(&u->dh3c, v2 * 32);
+        A(&u->dh3b, v2 * 64);
+        A(&u->dh3a, v2 * 64);
+        A(&u->dp2, v2 * 32);
+        A(&u->dh2a, v1 * 32);
+        A(&u->dp1, v1 * 16);
+        A(&u->dh1a, v0 * 16);
+        A(&u->dy, nvox(u->g[0]));
+        A(&u->esq, (size_t)nb);
+        A(&u->loss, 1);
+    }
+    if (!rc && hipStreamSynchronize(u->stream) != hipSuccess) rc = CAD_ERR_HIP;
+    if (rc) {
+        cad_unet_destroy(u);
+        return rc;
+    }
+    *out = u;
+    return CAD_OK;
+}
+
+void cad_unet_destroy(cad_unet *u)
+{
+    if (!u) return;
+    (void)hipStreamSynchronize(u->stream);
+    for (void *q : u->allocs) (void)hipFree(q);
+    if (u->own_stream) (void)hipStreamDestroy(u->stream);
+    delete u;
+}
+
+int cad_unet_set_params(cad_unet *u, const float *host)
+{
+    if (!u || !host) {
+        cad_set_error("cad_unet_set_params: null argument");
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipMemcpyAsync(u->p, host, u->nparam * sizeof(float), hipMemcpyHostToDevice, u->stream));
+    if (u->train) {
+        RET_IF(ew(u, k_zero_f, u->nparam, u->nparam, u->m1));
+        RET_IF(ew(u, k_zero_f, u->nparam, u->nparam, u->m2));
+        u->step = 0;
+    }
+    RET_IF(pack_all(u));
+    CAD_CHECK(hipStreamSynchronize(u->stream));
+    return CAD_OK;
+}
+
+int cad_unet_get_params(cad_unet *u, float *host)
+{
+    if (!u || !host) {
+        cad_set_error("cad_unet_get_params: null argument");
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipMemcpyAsync(host, u->p, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->stream));
+    CAD_CHECK(hipStreamSynchronize(u->stream));
+    return CAD_OK;
+}
+
+int cad_unet_init_params(int seed, float *host)
+{
+    if (!host) {
+        cad_set_error("cad_unet_init_params: null argument");
+        return CAD_ERR_ARG;
+    }
+    std::mt19937_64 rng((unsigned long long)seed);
+    long long o = 0;
+    for (int l = 0; l < NL; l++) {
+        const LayerDef &L = LAYERS[l];
+        std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (27.f * L.cin)));
+        for (long long i = 0; i < 27LL * L.cin * L.cout; i++) host[o++] = nd(rng);
+        for (int i = 0; i < L.cout; i++) host[o++] = 0.f;
+    }
+    std::normal_distribution<float> nd(0.f, std::sqrt(1.f / 16.f));
+    for (int i = 0; i < 16; i++) host[o++] = nd(rng);
+    host[o++] = 0.f;
+    return CAD_OK;
+}
+
+// Runs the network (one box) on its first nz planes from now on: nz a
+// multiple of 4, at most the box's depth; the plane above the run's last one
+// is cleared in every buffer a convolution reads, so it acts as the halo.
+int cad_unet_set_depth(cad_unet *u, int nz)
+{
+    if (!u || u->g[0].nb != 1 || nz < 4 || nz % 4 || nz > u->g[0].pz - 2) {
+        cad_set_error("cad_unet_set_depth: depth %d (a multiple of 4 within the box, one box)", nz);
+        return CAD_ERR_ARG;
+    }
+    for (int l = 0; l < 3; l++) u->g[l].nz = nz >> l;
+    struct B {
+        bf16 *a;
+        int lvl, cs;
+    } bufs[] = {{u->x, 0, 8},     {u->h1a, 0, 16},  {u->cat1, 0, 32}, {u->h1d, 0, 16},
+                {u->h1e, 0, 16},  {u->p1, 1, 16},   {u->h2a, 1, 32},  {u->cat2, 1, 64},
+                {u->h2d, 1, 32},  {u->h2e, 1, 32},  {u->h2f, 1, 16},  {u->p2, 2, 32},
+                {u->h3a, 2, 64},  {u->h3b, 2, 64},  {u->h3c, 2, 32}};
+    for (const B &b : bufs) {
+        const UGeo &g = u->g[b.lvl];
+        if (g.nz + 1 > g.pz - 2) continue;   // the box's own halo plane
+        hipLaunchKernelGGL(k_zero_plane, dim3(64), dim3(256), 0, u->stream, g, g.nz, b.a, b.cs);
+    }
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
+hipStream_t cad_unet_stream(cad_unet *u) { return u->stream; }
+UGeo cad_unet_geo(cad_unet *u) { return u->g[0]; }
+
+void *cad_unet_input(cad_unet *u) { return u ? (void *)u->x : nullptr; }
+float *cad_unet_output(cad_unet *u) { return u ? u->y : nullptr; }
+
+// Copies the input features (BF16, 8 channels, padded boxes with zero halos)
+// from a device buffer of the caller.
+int cad_unet_set_input(cad_unet *u, const void *dev)
+{
+    if (!u || !dev) {
+        cad_set_error("cad_unet_set_input: null argument");
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipMemcpyAsync(u->x, dev, padded(u->g[0]) * 8 * sizeof(bf16), hipMemcpyDeviceToDevice,
+                             u->stream));
+    return CAD_OK;
+}
+
+// Copies the output (FP32, interior voxels, x fastest) to a device buffer of
+// the caller and waits for it.
+int cad_unet_get_output(cad_unet *u, float *dev)
+{
+    if (!u || !dev) {
+        cad_set_error("cad_unet_get_output: null argument");
+        return CAD_ERR_ARG;
+    }
+    CAD_CHECK(hipMemcpyAsync(dev, u->y, nvox(u->g[0]) * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
+    CAD_CHECK(hipStreamSynchronize(u->stream));
+    return CAD_OK;
+}
+
+int cad_unet_forward(cad_unet *u)
+{
+    if (!u) {
+        cad_set_error("cad_unet_forward: null network");
+        return CAD_ERR_ARG;
+    }
+    return cad_unet_forward_impl(u);
+}
+
+int cad_unet_train_step(cad_unet *u, const float *target, const float *mask, float lr, float *loss_out)
+{
+    if (!u || !u->train || !target || !mask) {
+        cad_set_error("cad_unet_train_step: not a training network or null argument");
+        return CAD_ERR_ARG;
+    }
+    RET_IF(cad_unet_forward_impl(u));
+    const UGeo &g = u->g[0];
+    RET_IF(ew(u, k_zero_f, u->nparam, u->nparam, u->gr));
+    RET_IF(ew(u, k_zero_f, (long long)g.nb, (long long)g.nb, u->esq));
+    RET_IF(ew(u, k_zero_f, 1, 1LL, u->loss));
+    hipLaunchKernelGGL(k_esq, dim3(64, g.nb), dim3(256), 0, u->stream, g, target, mask, u->esq);
+    hipLaunchKernelGGL(k_loss, dim3(grid_for(nvox(g))), dim3(256), 0, u->stream, g, u->y, target, mask,
+                       u->esq, u->dy, u->loss);
+    CAD_CHECK(hipGetLastError());
+    RET_IF(backward(u));
+    u->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)u->step), c2 = 1.f - std::pow(b2, (float)u->step);
+    RET_IF(ew(u, k_adam, u->nparam, u->nparam, u->p, u->gr, u->m1, u->m2, lr, b1, b2, c1, c2, 1e-8f));
+    RET_IF(pack_all(u));
+    if (loss_out) {
+        CAD_CHECK(hipMemcpyAsync(loss_out, u->loss, sizeof(float), hipMemcpyDeviceToHost, u->stream));
+        CAD_CHECK(hipStreamSynchronize(u->stream));
+    }
+    return CAD_OK;
+}
+
+int cad_unet_gradients(cad_unet *u, const float *target, const float *mask, float *host_grad, float *loss_out)
+{
+    if (!u || !u->train || !target || !mask || !host_grad) {
+        cad_set_error("cad_unet_gradients: not a training network or null argument");
+        return CAD_ERR_ARG;
+    }
+    RET_IF(cad_unet_forward_impl(u));
+    const UGeo &g = u->g[0];
+    RET_IF(ew(u, k_zero_f, u->nparam, u->nparam, u->gr));
+    RET_IF(ew(u, k_zero_f, (long long)g.nb, (long long)g.nb, u->esq));
+    RET_IF(ew(u, k_zero_f, 1, 1LL, u->loss));
+    hipLaunchKernelGGL(k_esq, dim3(64, g.nb), dim3(256), 0, u->stream, g, target, mask, u->esq);
+    hipLaunchKernelGGL(k_loss, dim3(grid_for(nvox(g))), dim3(256), 0, u->stream, g, u->y, target, mask,
+                       u->esq, u->dy, u->loss);
+    CAD_CHECK(hipGetLastError());
+    RET_IF(backward(u));
+    CAD_CHECK(hipMemcpyAsync(host_grad, u->gr, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->stream));
+    if (loss_out) CAD_CHECK(hipMemcpyAsync(loss_out, u->loss, sizeof(float), hipMemcpyDeviceToHost, u->stream));
+    CAD_CHECK(hipStreamSynchronize(u->stream));
+    return CAD_OK;
+}
+
+}  // extern "C"

@@ -14,6 +14,8 @@
 //    listed cells at every stage of its iteration.
 #include "cad_common.h"
 
+#include <utility>
+
 namespace {
 
 // fluid-side offset axis/sign of surface orientation c (1..6): c = 2*axis+1
@@ -25,25 +27,20 @@ __device__ __forceinline__ long long fluid_off(const Dims &d, int c)
 }
 
 // ------------------------------------------------------------------ FCONV
-// Donor-acceptor flux through faces normal to A, vf_fconv.f:56-200.
+// Donor-acceptor flux through the face normal to A at cell (i,j,k) (index n),
+// vf_fconv.f:56-200: the flux, and NLIM(donor) = 1 where the flux is limited.
 template <int A>
-__global__ void __launch_bounds__(256) k_fconv(Dims d, Static s, const double *__restrict__ q,
-                                               const double *__restrict__ f,
-                                               const double *__restrict__ bcf,
-                                               const int *__restrict__ nf, double *__restrict__ flf,
-                                               int *nlim, double dtnow, double flower)
+__device__ __forceinline__ double fconv_face(const Dims &d, const Static &s, const double *__restrict__ q,
+                                             const double *__restrict__ f,
+                                             const double *__restrict__ bcf,
+                                             const int *__restrict__ nf, int *nlim, double dtnow,
+                                             double flower, int i, int j, int k, long long n)
 {
-    int i, j, k;
-    if (!cell_of_thread(d, i, j, k)) return;
-    const long long n = cid(d, i, j, k);
     const int idx[3] = {i, j, k};
     const int hi[3] = {A == 0 ? d.ni : d.ni - 1, A == 1 ? d.nj : d.nj - 1, A == 2 ? d.nk : d.nk - 1};
     const int *ind = A == 0 ? s.indx : (A == 1 ? s.indy : s.indz);
     const double *gga = A == 0 ? s.ggx : (A == 1 ? s.ggy : s.ggz);
-    if (i < 2 || j < 2 || k < 2 || i > hi[0] || j > hi[1] || k > hi[2] || ind[n] == -1) {
-        flf[n] = 0.0;
-        return;
-    }
+    if (i < 2 || j < 2 || k < 2 || i > hi[0] || j > hi[1] || k > hi[2] || ind[n] == -1) return 0.0;
     const long long sa = stride(d, A);
     double c = dtnow * gga[n] * q[n];
     double ac = fabs(c);
@@ -75,27 +72,43 @@ __global__ void __launch_bounds__(256) k_fconv(Dims d, Static s, const double *_
     double cfx = dmaxd((fdm - fad) * ac - (fdm - fd) * v, 0.0);
     double fvx = fad * ac + cfx;
     double fvm = fd * v;
-    flf[n] = 0.0 - (double)lsn * dmind(fvx, fvm);
     if (fvx > fvm) nlim[ld] = 1;
+    return 0.0 - (double)lsn * dmind(fvx, fvm);
 }
 
-// F update, vf_feuler.f:43-58 (static porosity, LOOPS=1, no source).
-__global__ void __launch_bounds__(256) k_feuler(Dims d, Static s, const double *__restrict__ flu,
-                                                const double *__restrict__ flv,
-                                                const double *__restrict__ flw,
-                                                const int *__restrict__ nf, double *f, double dtnow,
-                                                double loops)
+// VF_FCONV and VF_FEULER in one pass (vf_f1cal.f:124-160, vf_feuler.f:43-58;
+// static porosity, LOOPS=1, no source): each cell evaluates the fluxes of its
+// six faces -- the same expression on the same operands as the face's other
+// cell, so both get its bits -- and writes its new F out of place (fo), so
+// that every flux reads the old F.  The flux arrays never reach HBM.  A face
+// with a limited flux sets NLIM of its donor from both sides (the same 1).
+__global__ void __launch_bounds__(256) k_fconv_feuler(Dims d, Static s, const double *__restrict__ u,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ w,
+                                                      const double *__restrict__ f,
+                                                      const double *__restrict__ bcf,
+                                                      const int *__restrict__ nf, int *nlim,
+                                                      double *__restrict__ fo, double dtnow,
+                                                      double flower, double loops)
 {
     int i, j, k;
     if (!cell_of_thread(d, i, j, k)) return;
-    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1) return;
     const long long n = cid(d, i, j, k);
-    if (nf[n] == -1) return;
+    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1 || nf[n] == -1) {
+        fo[n] = f[n];
+        return;
+    }
     const long long ni = d.ni, nij = d.nij;
+    const double fu0 = fconv_face<0>(d, s, u, f, bcf, nf, nlim, dtnow, flower, i, j, k, n);
+    const double fu1 = fconv_face<0>(d, s, u, f, bcf, nf, nlim, dtnow, flower, i + 1, j, k, n + 1);
+    const double fv0 = fconv_face<1>(d, s, v, f, bcf, nf, nlim, dtnow, flower, i, j, k, n);
+    const double fv1 = fconv_face<1>(d, s, v, f, bcf, nf, nlim, dtnow, flower, i, j + 1, k, n + ni);
+    const double fw0 = fconv_face<2>(d, s, w, f, bcf, nf, nlim, dtnow, flower, i, j, k, n);
+    const double fw1 = fconv_face<2>(d, s, w, f, bcf, nf, nlim, dtnow, flower, i, j, k + 1, n + nij);
     double g0 = s.ggv[n] - (s.ggv[n] - s.ggv[n]) * loops;
-    double df = (met<0>(s, 4, i) * (flu[n + 1] - flu[n]) + met<1>(s, 4, j) * (flv[n + ni] - flv[n])
-                 + met<2>(s, 4, k) * (flw[n + nij] - flw[n]) + dtnow * 0.0);
-    f[n] = (g0 * f[n] + df) / s.ggv[n];
+    double df = (met<0>(s, 4, i) * (fu1 - fu0) + met<1>(s, 4, j) * (fv1 - fv0)
+                 + met<2>(s, 4, k) * (fw1 - fw0) + dtnow * 0.0);
+    fo[n] = (g0 * f[n] + df) / s.ggv[n];
 }
 
 __device__ __forceinline__ double cell_vol(const Static &s, int i, int j, int k, double g)
@@ -226,14 +239,27 @@ __global__ void __launch_bounds__(256) k_fmod_clamp(Dims d, const int *__restric
     if (f[n] < 0.0) f[n] = 0.0;
 }
 
+// Compensated summation (Neumaier): s + c carries the running sum with the
+// rounding error of every addition kept in c.
+__device__ __forceinline__ void comp_add(double &s, double &c, double x)
+{
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
 // Clip F to {0,1} beyond FLOWER/FUPPER and partial sums of FSUM and FCUT,
-// vf_fcut01.f:34-55.
+// vf_fcut01.f:34-55.  The volume sums are compensated sums in FP64: each
+// thread, each block's tree and the final pass over the blocks carry the
+// rounding errors along, so FSUM and FCUT are accurate to a few ulps of the
+// exact sums over any number of cells.  Partials: FSUM, FCUT, then their
+// compensations, REDUCE_BLOCKS each.
 __global__ void __launch_bounds__(REDUCE_THREADS) k_fcut(Dims d, Static s, const int *__restrict__ nf,
                                                          double *f, double flower, double fupper,
                                                          double *partial)
 {
-    __shared__ double sh[2][REDUCE_THREADS];
-    double fsum = 0.0, fcut = 0.0;
+    __shared__ double sh[4][REDUCE_THREADS];
+    double fsum = 0.0, fcut = 0.0, csum = 0.0, ccut = 0.0;
     for (long long n = (long long)blockIdx.x * REDUCE_THREADS + threadIdx.x; n < d.nc;
          n += (long long)REDUCE_BLOCKS * REDUCE_THREADS) {
         int i, j, k;
@@ -243,30 +269,50 @@ __global__ void __launch_bounds__(REDUCE_THREADS) k_fcut(Dims d, Static s, const
         double v = cell_vol(s, i, j, k, s.ggv[n]);
         double fv = f[n];
         if (fv > fupper) {
-            fcut = fcut + (fv - 1.0) * v;
+            comp_add(fcut, ccut, (fv - 1.0) * v);
             fv = 1.0;
             f[n] = fv;
         } else if (fv < flower) {
-            fcut = fcut + (fv - 0.0) * v;
+            comp_add(fcut, ccut, (fv - 0.0) * v);
             fv = 0.0;
             f[n] = fv;
         }
-        fsum = fsum + fv * v;
+        comp_add(fsum, csum, fv * v);
     }
     sh[0][threadIdx.x] = fsum;
     sh[1][threadIdx.x] = fcut;
+    sh[2][threadIdx.x] = csum;
+    sh[3][threadIdx.x] = ccut;
     __syncthreads();
     for (int off = REDUCE_THREADS / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                double a = sh[q][threadIdx.x], ca = sh[q + 2][threadIdx.x] + sh[q + 2][threadIdx.x + off];
+                comp_add(a, ca, sh[q][threadIdx.x + off]);
+                sh[q][threadIdx.x] = a;
+                sh[q + 2][threadIdx.x] = ca;
+            }
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partial[blockIdx.x] = sh[0][0];
-        partial[REDUCE_BLOCKS + blockIdx.x] = sh[1][0];
+#pragma unroll
+        for (int q = 0; q < 4; q++) partial[q * REDUCE_BLOCKS + blockIdx.x] = sh[q][0];
     }
+}
+
+// FSUM and FCUT from k_fcut's partials, compensated, in block order
+__global__ void k_fcut_final(const double *__restrict__ partial, double *out)
+{
+    if (threadIdx.x >= 2) return;
+    const int q = threadIdx.x;
+    double a = 0.0, c = 0.0;
+    for (int b = 0; b < REDUCE_BLOCKS; b++) {
+        comp_add(a, c, partial[q * REDUCE_BLOCKS + b]);
+        c += partial[(q + 2) * REDUCE_BLOCKS + b];
+    }
+    out[q] = a + c;
 }
 
 // Boundary F, vf_bwff.f:33-87 (types FIX and FREE; wave-maker and radiation
@@ -777,12 +823,12 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
     const double dt = c->prm.dtnow;
     // NLIM = 0 then FCONV (vf_f1cal.f:124-137)
     CAD_CHECK(hipMemsetAsync(f.nlim, 0, (size_t)d.nc * sizeof(int), st));
-    hipLaunchKernelGGL(k_fconv<0>, g, b, 0, st, d, s, f.uu, f.ff, f.bcf, f.nf, c->flf[0], f.nlim, dt, c->prm.flower);
-    hipLaunchKernelGGL(k_fconv<1>, g, b, 0, st, d, s, f.vv, f.ff, f.bcf, f.nf, c->flf[1], f.nlim, dt, c->prm.flower);
-    hipLaunchKernelGGL(k_fconv<2>, g, b, 0, st, d, s, f.ww, f.ff, f.bcf, f.nf, c->flf[2], f.nlim, dt, c->prm.flower);
-    // VF_FEULER (vf_f1cal.f:160)
-    hipLaunchKernelGGL(k_feuler, g, b, 0, st, d, s, c->flf[0], c->flf[1], c->flf[2], f.nf, f.ff, dt,
-                       (double)c->prm.loops);
+    // VF_FCONV and VF_FEULER (vf_f1cal.f:124-160), F out of place into the
+    // spare array, which then becomes F
+    hipLaunchKernelGGL(k_fconv_feuler, g, b, 0, st, d, s, f.uu, f.vv, f.ww, f.ff, f.bcf, f.nf, f.nlim,
+                       c->flf[0], dt, c->prm.flower, (double)c->prm.loops);
+    CAD_CHECK(hipGetLastError());
+    std::swap(f.ff, c->flf[0]);
     // VF_BWFFSF: fixed-level sides (vf_f1cal.f:180)
     RET_IF(cad_launch_bwffsf(c));
     // VF_FMOD1 (vf_f1cal.f:185)
@@ -793,11 +839,12 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
     // VF_FCUT01 (vf_f1cal.f:188)
     hipLaunchKernelGGL(k_fcut, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, d, s, f.nf, f.ff,
                        c->prm.flower, c->prm.fupper, c->red_partial);
+    hipLaunchKernelGGL(k_fcut_final, dim3(1), dim3(64), 0, st, c->red_partial, c->red_out);
     CAD_CHECK(hipGetLastError());
-    double sums[2];
-    RET_IF(cad_reduce_fetch(c, 2, sums));
-    *fsum = sums[0];
-    *fcut = sums[1];
+    CAD_CHECK(hipMemcpyAsync(c->host_red, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CAD_CHECK(hipStreamSynchronize(st));
+    *fsum = c->host_red[0];
+    *fcut = c->host_red[1];
     // VF_BWFF, VF_BSUWN3, VF_BSUWEM (vf_f1cal.f:192-194)
     if (d.numb > 0)
         hipLaunchKernelGGL(k_bwff, dim3((d.numb + 255) / 256), dim3(256), 0, st, d, c->indb, f.ff, f.bcf);
@@ -854,11 +901,12 @@ int cad_init_phase(cad_ctx *c, double wvlvl, const double *uvw, int flags, doubl
     RET_IF(cad_launch_bwffsf(c));
     hipLaunchKernelGGL(k_fcut, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, st, d, s, f.nf, f.ff,
                        c->prm.flower, c->prm.fupper, c->red_partial);
+    hipLaunchKernelGGL(k_fcut_final, dim3(1), dim3(64), 0, st, c->red_partial, c->red_out);
     CAD_CHECK(hipGetLastError());
-    double sums[2];
-    RET_IF(cad_reduce_fetch(c, 2, sums));
-    *fsum = sums[0];
-    *fcut = sums[1];
+    CAD_CHECK(hipMemcpyAsync(c->host_red, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CAD_CHECK(hipStreamSynchronize(st));
+    *fsum = c->host_red[0];
+    *fcut = c->host_red[1];
     if (d.numb > 0)
         hipLaunchKernelGGL(k_bwff, dim3((d.numb + 255) / 256), dim3(256), 0, st, d, c->indb, f.ff, f.bcf);
     hipLaunchKernelGGL(k_ifill, sgrid(d.nc), dim3(256), 0, st, d.nc, f.indc, -1);

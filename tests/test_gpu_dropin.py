@@ -105,7 +105,9 @@ def test_fortran_caller_on_recorded_systems(pkg, tmp_path, mode):
         _check(case, s, pt, float(w[2]), float(w[3]), int(w[1]))
 
 
-_STEP_RE = re.compile(r"STEP=\s*(\d+)\s*:.*?ITR\s*=\s*(\S+)", re.S)
+# a step's list line and the ITR on the line after it (the ">> FILE-... :
+# STEP=" lines of the output files are not step lines)
+_STEP_RE = re.compile(r"^ STEP=\s*(\d+)\s*:[^\n]*\n[^\n]*ITR\s*=\s*(\S+)", re.M)
 
 
 def _run_reference(exe, name, deck, tmp_path, timeout=600):

@@ -4,14 +4,17 @@ The reference (its Fortran, one rank) ran the 512x32x96 solitary run-up (40
 steps) and the 512x256x128 buildings tank (3 steps) restarted from the same
 initial state (tests/golden/make_golden_full.py).  Here the GPU runs the same
 tanks from the deck and the wave, through the C ABI, in the plain FP64 form,
-and must keep, at every step: ITR equal to the reference's, FSUM within 1e-12,
-||div|| within 1e-6; at the last step U, V, W, P, F on the recorded J planes
+and must keep, at every step: ITR equal to the reference's, FSUM within the
+reference's own rounding (it sums cell by cell; the bound of recursive
+summation, (n-1) 2^-53 FSUM over n cells) and the run's volume balance
+FSUM + FCUT to 1e-12, ||div|| within 1e-6; at the last step U, V, W, P, F on the recorded J planes
 within 1e-8 and every cell flag equal.  These are the sizes where the sweep
 tiles, the mailbox chain and the partial sums over millions of cells run as in
 the benchmark.
 
 The mixed form then runs the buildings tank for 200 steps beside the plain
-form (FP64, 1e-10): its true relative residual is at most 1e-8 at every step,
+form (FP64, 1e-10): its true residual is at most 1e-8 ||BB|| + CGEPSA at
+every step,
 and after 200 steps its fields are within 1e-6 of the plain form's with every
 flag equal.
 """
@@ -69,6 +72,8 @@ def test_full_size_plain_form_matches_reference(pkg, name):
     case = _case(pkg, traj, name)
     nsteps = int(traj["NSTEPS"])
     jj = [int(j) - 1 for j in fin["JPLANES"]]
+    cells = case.ni * case.nj * case.nk
+    prev = 0.0
     print(f"{name}: {case.ni}x{case.nj}x{case.nk}, {nsteps} steps", flush=True)
     with _context(pkg, case) as c:
         for s in range(1, nsteps + 1):
@@ -76,7 +81,15 @@ def test_full_size_plain_form_matches_reference(pkg, name):
             dv = c.cdiv00()
             print(f"{name} step {s} ITR={info['icgitr']} ref {int(traj['ITR'][s])}", flush=True)
             assert info["icgitr"] == int(traj["ITR"][s]), (name, s)
-            assert abs(info["fsum"] - traj["FSUM"][s]) <= 1e-12 * traj["FSUM"][s], (name, s)
+            # the reference adds FSUM up one cell at a time, so its value is
+            # off the exact sum by up to the bound of recursive summation,
+            # (n-1) u sum|x| (n terms, u = 2^-53); FSUM here is a compensated
+            # sum, within a few ulps of the exact one
+            assert abs(info["fsum"] - traj["FSUM"][s]) <= cells * 2.0**-53 * traj["FSUM"][s], (name, s)
+            # and the volume balance of this run itself, to 1e-12
+            if s > 1:
+                assert abs(info["fsum"] + info["fcut"] - prev) <= 1e-12 * prev, (name, s)
+            prev = info["fsum"]
             assert abs(dv - traj["DV"][s]) <= 1e-6 * traj["DV"][s], (name, s, dv, traj["DV"][s])
         out = c.download_state()
     assert _vel_slices_err(out, fin, jj) <= 1e-8
@@ -108,7 +121,9 @@ def test_full_size_mixed_form_200_steps(pkg):
         for s in range(steps):
             info = c.step()
             worst = max(worst, info["cgxnrm"])
-            assert info["cgxnrm"] <= 1e-8, (s, info["cgxnrm"])
+            # ||BB - A PT|| <= 1e-8 ||BB|| + CGEPSA, ||BB|| = CGBNRM
+            slack = c.params["cgepsa"] / info["cgbnrm"]
+            assert info["cgxnrm"] <= (1e-8 + slack) * (1 + 1e-6), (s, info["cgxnrm"])
         mixed = c.download_state()
     print(f"worst true residual {worst:.3e}", flush=True)
     vs = max(np.abs(plain[k]).max() for k in ("UU", "VV", "WW"))
@@ -117,3 +132,52 @@ def test_full_size_mixed_form_200_steps(pkg):
     for k in ("PP", "FF"):
         assert rel_err(mixed[k], plain[k]) <= 1e-6, k
     assert np.array_equal(mixed["NF"], plain["NF"])
+
+
+@pytest.mark.timeout(600)
+def test_full_size_first_step_phases_match_the_oracle(pkg):
+    """Step 1 of the 512x32x96 run-up tank phase by phase against the oracle
+    (the C restatement, pinned to the reference by the golden records, run
+    here on the host): the initial state, the predictor and the Poisson
+    matrix bit for bit, then the solve -- ITR equal and PT within 1e-10 --
+    on the same matrix; the oracle's ITR is the reference's own."""
+    import pyoracle as po
+    name = "runup_b"
+    traj, _ = _load(name)
+    case = _case(pkg, traj, name, 1)
+    lay = pkg.layout.from_text(case.deck)
+    state = lay.raw_state()
+    case.ic(lay.XX, lay.ZZ, lay.NF0, state)
+    arr = dict(lay.static_arrays())
+    arr.update({k: v.copy() for k, v in state.items()})
+    F = po.Fields(arr, lay.params())
+    assert F.init_state(lay.deck.wvlvl, lay.deck.uini, 0) == 0
+    keys = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")
+    with _context(pkg, case) as c:
+        out = c.download_state()
+        bad = [k for k in keys if not np.array_equal(out[k], F[k])]
+        print(f"initial state differs in {bad}", flush=True)
+        assert not bad
+        c.predictor()
+        assert F.predictor() == 0
+        out = c.download_state()
+        bad = [k for k in ("UU", "VV", "WW") if not np.array_equal(out[k], F[k])]
+        print(f"predictor differs in {bad}", flush=True)
+        assert not bad
+        c.vpcoef()
+        m = c.download_matrix()
+        rc, mo = F.vpcoef()
+        assert rc == 0
+        bad = [k for k in pkg.capi.MATRIX if not np.array_equal(m[k], mo[k])]
+        print(f"matrix differs in {bad}", flush=True)
+        assert not bad
+        info = c.vpsol()
+        pt = c.download_pt()
+    p = lay.params()
+    x, bn, xn, it = po.m1bcgs(mo, F["INDC"], cgpara=p["cgpara"], cgepsa=p["cgepsa"],
+                              cgepsr=p["cgepsr"], cgdiv=p["cgdiv"], icgtyp=p["icgtyp"],
+                              icgmax=p["icgmax"])
+    print(f"solve: ITR {info['icgitr']} oracle {it} reference {int(traj['ITR'][1])}", flush=True)
+    assert it == int(traj["ITR"][1])
+    assert info["icgitr"] == it
+    assert rel_err(pt, x) <= 1e-10

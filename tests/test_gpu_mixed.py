@@ -110,3 +110,31 @@ def test_mixed_form_with_boundary_and_timer_door_models(pkg, tmp_path, case):
     assert_vel_close(f"step {last} velocity", rec, ref, 1e-6)
     for k in ("PP", "FF"):
         assert_close(f"step {last} {k}", rec[k], ref[k], 1e-6)
+
+
+def test_mixed_falls_back_to_milu_bicgstab(pkg):
+    """With the mixed form's own iterations capped at one, its solve cannot
+    converge by itself: the reference's MILU-BiCGSTAB finishes it from the
+    current iterate, to the true residual at CGEPSR, PT within 1e-6 of the
+    reference's."""
+    case = "corner3d"
+    st = fx.static(case)
+    s = fx.steps(case)[-1]
+    r = fx.step(case, s)
+    c, p = _ctx(pkg, st, float(r["v1_in"]["DTNOW"][0]))
+    with c:
+        a = {k: r["v1_in"][k] for k in STATE}
+        a["BCF"] = r["f1_in"]["BCF"]
+        c.upload_state(a)
+        c.upload_matrix(r["ps_in"], r["ps_in"]["INDC"])
+        c.set_krylov_cap(1)
+        info = c.vpsol()
+        res = c.poisson_residual()
+        pt = c.download_pt()
+        stats = c.mixed_stats()
+    print(f"fallback: ITR {info['icgitr']} stats {stats} true residual {res:.2e}", flush=True)
+    assert stats["fallbacks"] == 1 and stats["fallback_iters"] > 0
+    assert info["icgitr"] == 1 + stats["fallback_iters"]
+    bn = info["cgbnrm"]
+    assert res * bn <= (p["cgepsr"] * bn + p["cgepsa"]) * (1.0 + 1e-6)
+    assert_close(f"{case} step {s} PT", pt, r["ps_out"]["PT"], 1e-6)
