@@ -370,6 +370,33 @@ void vf_vpsol_(double *dbuf, const double *ad, const double *ali, const double *
                const double *bb, double *pt, double *dd, double *xw, double *yy, double *r0,
                double *rr, double *pp, double *ap, double *ae, const int *indc);
 
+/* No counterpart in the reference: the VF_M1BCGS solve of vf_m1bcgs_ on
+ * arrays already resident on the current GPU, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It allocates nothing.  Arguments keep
+ * VF_M1BCGS's names and meaning (vf_m1bcgs.f:1-5) with MI,MJ,MK = NI,NJ,NK,
+ * all arrays as device pointers: AD..BB, X0 (in: initial guess; out: the
+ * solution where ITER >= 0), INDC, and the reference's work arrays DD, XW,
+ * YY, R0, RR, PP, AP, AE (NI*NJ*NK doubles each, contents on return
+ * unspecified).  BNORM, XNORM, ITER are host pointers with vf_m1bcgs_'s
+ * meaning; the result equals vf_m1bcgs_'s on the same system bit for bit.
+ * stream: a hipStream_t (NULL: the null stream); the call returns after the
+ * stream has finished the solve.  Everything else the solve needs lives in
+ * one library-owned device array of CAD_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends; grids beyond CAD_NOALLOC_MAX_CELLS cells, or whose
+ * sweep mailboxes do not fit the array, are refused with CAD_ERR_ARG.  Calls
+ * that overlap on one GPU share that array and race.  Returns CAD_OK, or a
+ * negative code with the cause in cad_last_error(). */
+#define CAD_NOALLOC_MAX_CELLS 2097152ull                                   /* 2^21 */
+#define CAD_NOALLOC_SCRATCH_BYTES ((72ull << 20) + (64ull << 10))          /* 72.06 MiB */
+int cad_m1bcgs_noalloc(void *stream, const double *ad, const double *ali, const double *alj,
+                       const double *alk, const double *aui, const double *auj,
+                       const double *auk, const double *bb, double *x0, double *dd, double *xw,
+                       double *yy, double *r0, double *rr, double *pp, double *ap, double *ae,
+                       const int *indc, int ni, int nj, int nk, double cgpara, double cgepsa,
+                       double cgepsr, double cgdiv, int icgtyp, int icgmax, double *bnorm,
+                       double *xnorm, int *iter);
+
 /* The mixed and learned forms --------------------------------------------
  * No counterparts in the reference (its only solver is VF_M1BCGS).  Both
  * stop on the true relative residual ||BB - A PT|| / ||BB|| <= CGEPSR and,

@@ -123,6 +123,8 @@ def lib():
     L.cad_time_vcycle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_time_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_unet_mac_per_voxel.restype = C.c_double
+    L.cad_m1bcgs_noalloc.argtypes = ([C.c_void_p] * 19 + [C.c_int] * 3 + [C.c_double] * 4 +
+                                     [C.c_int] * 2 + [PD, PD, PI])
     _lib = L
     return L
 
@@ -137,7 +139,7 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_get_wave_level", "cad_set_open", "cad_poisson_residual", "cad_time_precond",
            "cad_set_drag_df", "cad_set_damp", "cad_set_wave_theory", "cad_wave_theory_eval",
            "cad_set_wave_table2", "cad_set_wave2", "cad_set_flag_pass_limit",
-           "cad_set_true_residual_stop", "cad_set_krylov_cap",
+           "cad_set_true_residual_stop", "cad_m1bcgs_noalloc", "cad_set_krylov_cap",
            "cad_mixed_stats", "cad_time_vcycle", "cad_set_learned_weights", "cad_set_learned_mode", "cad_learned_stats",
            "cad_learned_box", "cad_learned_sample", "cad_time_learned", "cad_unet_param_count",
            "cad_unet_mac_per_voxel", "cad_unet_init_params", "cad_unet_create", "cad_unet_destroy",
@@ -576,3 +578,24 @@ def vpsol(mat, indc, cgpara=0.95, cgepsa=1.0e-15, cgepsr=1.0e-12, cgdiv=1.0e30, 
     L.vf_vpsol_(_pd(dbuf), *[_pd(a) for a in ms], _pd(pt), *[_pd(w) for w in wk], _pi(ic))
     return pt, pr.cgbnrm, pr.cgxnrm, pi.icgitr
 
+
+NOALLOC_MAX_CELLS = 2097152
+
+
+def m1bcgs_noalloc(ptrs, shape, stream=None, cgpara=0.95, cgepsa=1.0e-15, cgepsr=1.0e-12,
+                   cgdiv=1.0e30, icgtyp=1, icgmax=500):
+    """cad_m1bcgs_noalloc: VF_M1BCGS on arrays already on the GPU, allocating
+    nothing.  ptrs maps AD, ALI, ALJ, ALK, AUI, AUJ, AUK, BB, X0, INDC and the
+    work arrays DD, XW, YY, R0, RR, PP, AP, AE to device addresses of
+    (NI,NJ,NK) column-major arrays; shape = (NI,NJ,NK); stream a hipStream_t
+    (None: the null stream).  X0 holds the solution on return.  Returns
+    (rc, BNORM, XNORM, ITER); rc is CAD_OK or a negative code (not raised, so
+    that callers can check the refusals)."""
+    L = lib()
+    names = list(MATRIX) + ["X0", "DD", "XW", "YY", "R0", "RR", "PP", "AP", "AE", "INDC"]
+    bn, xn, it = C.c_double(), C.c_double(), C.c_int()
+    rc = L.cad_m1bcgs_noalloc(stream, *[C.c_void_p(int(ptrs[k])) for k in names],
+                              int(shape[0]), int(shape[1]), int(shape[2]), cgpara, cgepsa, cgepsr,
+                              cgdiv, int(icgtyp), int(icgmax), C.byref(bn), C.byref(xn),
+                              C.byref(it))
+    return rc, bn.value, xn.value, it.value

@@ -283,6 +283,8 @@ int cad_reduce_fetch(cad_ctx *c, int n, double *host_out);
 int cad_cdiv(cad_ctx *c, double *dv);
 int cad_sweep_alloc(cad_ctx *c);
 void cad_sweep_free(cad_ctx *c);
+unsigned long long cad_sweep_bytes(const cad_ctx *c);
+int cad_sweep_bind(cad_ctx *c, char *base, unsigned long long bytes, unsigned seq);
 int cad_sweep_dcmp(cad_ctx *c, double u, int *npos_host);
 int cad_sweep_minv(cad_ctx *c, double *x, const double *y);
 int cad_sweep_status(cad_ctx *c, int *code);

@@ -715,6 +715,97 @@ int cad_sweep_alloc(cad_ctx *c)
     return CAD_OK;
 }
 
+namespace {
+
+// Position of each tile in the dependency order that cad_sweep_alloc sorts on
+// the host: by anti-diagonal bj + bk, then by bj.
+__global__ void k_tile_order(int ntj, int ntk, int *order)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntj * ntk) return;
+    const int bj = t % ntj, diag = t % ntj + t / ntj;
+    int pos = 0;
+    for (int s = 0; s < diag; s++) pos += min(ntj - 1, s) - max(0, s - (ntk - 1)) + 1;
+    order[pos + bj - max(0, diag - (ntk - 1))] = t;
+}
+
+__global__ void k_clear_u64(unsigned long long *p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0ull;
+}
+
+constexpr unsigned long long align256(unsigned long long n) { return (n + 255ull) & ~255ull; }
+
+}  // namespace
+
+// Bytes of device memory the pipelined schedule's control arrays take for
+// the context's grid (tile orders and ranges of both shapes, mailboxes,
+// ticket, error flag), each part 256-byte aligned.
+unsigned long long cad_sweep_bytes(const cad_ctx *c)
+{
+    const Dims &d = c->d;
+    unsigned long long total = 0, lines = 0;
+    for (int sh = 0; sh < 2; sh++) {
+        const int tj = SHAPE_TJ[sh], tk = 64 / tj;
+        const unsigned long long nt =
+            (unsigned long long)((d.nj - 2 + tj - 1) / tj) * ((d.nk - 2 + tk - 1) / tk);
+        total += align256(nt * sizeof(int)) + align256(nt * 2 * sizeof(int));
+        lines = std::max(lines, nt * MB_LINES);
+    }
+    return total + align256(2 * lines * d.ni * 16) + 256 + 256;
+}
+
+// The same control arrays as cad_sweep_alloc, carved from [base, base +
+// bytes) that the caller owns instead of allocated: the tile orders are
+// written by a kernel, the ticket and the error flag are cleared by one, all
+// on c->stream.  The mailboxes are not cleared: their entries carry the
+// sequence numbers of earlier launches, and seq (the first launch's number)
+// must exceed every sequence number written into this memory before.
+int cad_sweep_bind(cad_ctx *c, char *base, unsigned long long bytes, unsigned seq)
+{
+    const Dims &d = c->d;
+    const unsigned long long need = cad_sweep_bytes(c);
+    if (need > bytes) {
+        cad_set_error("sweep control arrays need %llu bytes, %llu available", need, bytes);
+        return CAD_ERR_ARG;
+    }
+    char *p = base;
+    unsigned long long lines = 0;
+    for (int sh = 0; sh < 2; sh++) {
+        const int tj = SHAPE_TJ[sh], tk = 64 / tj;
+        c->sw_ntj[sh] = (d.nj - 2 + tj - 1) / tj;
+        c->sw_ntk[sh] = (d.nk - 2 + tk - 1) / tk;
+        c->sw_ntiles[sh] = c->sw_ntj[sh] * c->sw_ntk[sh];
+        const int nt = c->sw_ntiles[sh];
+        c->sw_order[sh] = reinterpret_cast<int *>(p);
+        p += align256((unsigned long long)nt * sizeof(int));
+        c->sw_range[sh] = reinterpret_cast<int *>(p);
+        p += align256((unsigned long long)nt * 2 * sizeof(int));
+        hipLaunchKernelGGL(k_tile_order, dim3((nt + 255) / 256), dim3(256), 0, c->stream,
+                           c->sw_ntj[sh], c->sw_ntk[sh], c->sw_order[sh]);
+        lines = std::max(lines, (unsigned long long)nt * MB_LINES);
+    }
+    c->sw_mb_bytes = lines * d.ni * 16;
+    if (2 * c->sw_mb_bytes > 0xFFFFFFF0ull || (unsigned long long)d.nc * 8 > 0xFFFFFFF0ull) {
+        cad_set_error("grid too large for 32-bit sweep offsets (%d x %d x %d)", d.ni, d.nj, d.nk);
+        return CAD_ERR_ARG;
+    }
+    c->sw_mb = p;
+    p += align256(2 * c->sw_mb_bytes);
+    c->sw_ticket = reinterpret_cast<unsigned long long *>(p);
+    p += 256;
+    c->sw_err = reinterpret_cast<int *>(p);
+    hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, c->stream, c->sw_ticket, 1);
+    hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, c->stream,
+                       reinterpret_cast<unsigned long long *>(c->sw_err), 1);
+    CAD_CHECK(hipGetLastError());
+    c->sw_base = 0;
+    c->sw_seq = seq;
+    c->sw_shape = 0;
+    return CAD_OK;
+}
+
 void cad_sweep_free(cad_ctx *c)
 {
     void *p[] = {c->sw_order[0], c->sw_order[1], c->sw_range[0], c->sw_range[1], c->sw_mb,
