@@ -952,17 +952,42 @@ void orc_mzax(int ni, int nj, int nk, const orc_matrix *m, const double *x, doub
             }
 }
 
-/* Masked dot product, lexicographic order.  vf_mzip.f:32-43. */
+static int dot_order = ORC_DOT_SERIAL;
+
+int orc_set_dot_order(int order)
+{
+    if (order < ORC_DOT_SERIAL || order > ORC_DOT_BLOCKED) return -1;
+    const int prev = dot_order;
+    dot_order = order;
+    return prev;
+}
+
+/* Masked dot product, lexicographic order.  vf_mzip.f:32-43.  The other
+ * summation orders (orc_set_dot_order) keep the same terms x*y, each rounded
+ * to double as in the reference. */
 double orc_mzip(int ni, int nj, int nk, const double *x, const double *y, const int *indc)
 {
     const size_t nij = (size_t)ni * nj;
-    double sum = 0.0;
+    double sum = 0.0, blk = 0.0;
+    long double ext = 0.0L;
+    size_t cnt = 0;
     for (int k = 2; k <= nk - 1; k++)
         for (int j = 2; j <= nj - 1; j++)
             for (int i = 2; i <= ni - 1; i++) {
                 size_t n = ID(i, j, k);
-                if (indc[n] != -1) sum = sum + x[n] * y[n];
+                if (indc[n] == -1) continue;
+                const double t = x[n] * y[n];
+                if (dot_order == ORC_DOT_SERIAL) {
+                    sum = sum + t;
+                } else if (dot_order == ORC_DOT_EXTENDED) {
+                    ext += (long double)t;
+                } else {
+                    blk = blk + t;
+                    if (++cnt % 1024 == 0) { sum = sum + blk; blk = 0.0; }
+                }
             }
+    if (dot_order == ORC_DOT_EXTENDED) return (double)ext;
+    if (dot_order == ORC_DOT_BLOCKED) return sum + blk;
     return sum;
 }
 

@@ -116,6 +116,14 @@ void orc_mzax(int numi, int numj, int numk, const orc_matrix *m, const double *x
               double *y, const int *indc);
 double orc_mzip(int numi, int numj, int numk, const double *x, const double *y,
                 const int *indc);
+/* Summation order of orc_mzip, for studying how the solve depends on the
+ * rounding of its dot products: ORC_DOT_SERIAL (the default) adds the terms
+ * one at a time in the reference's loop order; ORC_DOT_EXTENDED accumulates
+ * them in long double and rounds once; ORC_DOT_BLOCKED adds blocks of 1024
+ * terms serially and then the block sums.  Returns the previous order, or -1
+ * for an unknown one. */
+enum { ORC_DOT_SERIAL = 0, ORC_DOT_EXTENDED = 1, ORC_DOT_BLOCKED = 2 };
+int orc_set_dot_order(int order);
 /* phase (d): velocity / pressure correction and the boundary values after it */
 int orc_vmodif(orc_state *s, const double *pt);
 int orc_post_bc(orc_state *s);

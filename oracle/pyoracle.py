@@ -82,6 +82,8 @@ def lib():
         L.orc_mzax.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(OrcMatrix), PD, PD, PI]
         L.orc_mzip.argtypes = [C.c_int, C.c_int, C.c_int, PD, PD, PI]
         L.orc_mzip.restype = C.c_double
+        L.orc_set_dot_order.argtypes = [C.c_int]
+        L.orc_set_dot_order.restype = C.c_int
         L.orc_vmodif.argtypes = [C.POINTER(OrcState), PD]
         L.orc_post_bc.argtypes = [C.POINTER(OrcState)]
         L.orc_v1cal.argtypes = [C.POINTER(OrcState), PD]
@@ -302,6 +304,18 @@ def m1bcgs(mat, indc, x0=None, cgpara=0.95, cgepsa=1e-15, cgepsr=1e-12, cgdiv=1e
     lib().orc_m1bcgs(shape[0], shape[1], shape[2], C.byref(m), _pd(x), _pi(ic), cgpara, cgepsa,
                      cgepsr, cgdiv, icgtyp, icgmax, C.byref(bn), C.byref(xn), C.byref(it))
     return x, bn.value, xn.value, it.value
+
+
+DOT_SERIAL, DOT_EXTENDED, DOT_BLOCKED = 0, 1, 2
+
+
+def set_dot_order(order):
+    """Summation order of the solver's dot products (orc_set_dot_order):
+    DOT_SERIAL is the reference's; returns the previous order."""
+    prev = lib().orc_set_dot_order(int(order))
+    if prev < 0:
+        raise ValueError(f"unknown dot order {order}")
+    return prev
 
 
 def mzdcmp(mat, indc, cgpara=0.95, icgtyp=1):
