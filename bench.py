@@ -330,7 +330,7 @@ def cpu_baseline_reference(pkg, case, lay, state, flags, steps=1, timeout=600):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=os.environ.get("CADMAS_BENCH_WORKLOAD", "buildings_c"))
     ap.add_argument("--no-cpu-baseline", action="store_true")

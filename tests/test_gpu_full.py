@@ -18,8 +18,11 @@ The tests hold the GPU to what the reference's algorithm fixes:
   of the oracle's summation orders and PT within 5e-10 of the serial one;
 * every step: the solve meets the reference's stopping rule, its count is
   within 10 % of the reference's, FSUM within the reference's own rounding
-  plus 1e-10, the run's volume balance FSUM + FCUT to 1e-12 and ||div||
-  within 1e-6 of the reference's;
+  plus 1e-10 and ||div|| within 1e-6 of the reference's.  Both tanks take
+  water in through their upstream FREE boundary, so FSUM grows each step (by
+  6.8e-6 and 2.5e-4 in the reference's runs); FSUM against the reference's
+  covers that inflow.  Volume conservation to 1e-12 is tested on closed
+  tanks (tests/test_gpu_bore.py and the dam-break runs);
 * the last step: U, V, W, P, F on the recorded J planes within 1e-8 of the
   reference's and every cell flag equal.
 
@@ -83,7 +86,6 @@ def test_full_size_plain_form_matches_reference(pkg, name):
     nsteps = int(traj["NSTEPS"])
     jj = [int(j) - 1 for j in fin["JPLANES"]]
     cells = case.ni * case.nj * case.nk
-    prev = 0.0
     print(f"{name}: {case.ni}x{case.nj}x{case.nk}, {nsteps} steps", flush=True)
     with _context(pkg, case) as c:
         epsr, epsa = c.params["cgepsr"], c.params["cgepsa"]
@@ -104,10 +106,6 @@ def test_full_size_plain_form_matches_reference(pkg, name):
             # sum, within a few ulps of the exact one; F itself may differ by
             # what the solve's stopping rule leaves open
             assert ef <= cells * 2.0**-53 + 1e-10, (name, s)
-            # and the volume balance of this run itself, to 1e-12
-            if s > 1:
-                assert abs(info["fsum"] + info["fcut"] - prev) <= 1e-12 * prev, (name, s)
-            prev = info["fsum"]
             assert abs(dv - traj["DV"][s]) <= 1e-6 * traj["DV"][s], (name, s, dv, traj["DV"][s])
         out = c.download_state()
     ev = _vel_slices_err(out, fin, jj)
