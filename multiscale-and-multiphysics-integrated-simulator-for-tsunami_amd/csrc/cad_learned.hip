@@ -195,13 +195,20 @@ __global__ void __launch_bounds__(256) k_mdot(const unsigned long long *rng, con
             sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
-__global__ void k_mdot_final(const double *__restrict__ part, int nblk, int nv, double *__restrict__ out)
+// out[i] = sum over the nblk block partials of column i: block i of the grid
+// sums column i, each thread a fixed stride of partials, then a fixed tree
+__global__ void __launch_bounds__(256) k_mdot_final(const double *__restrict__ part, int nblk, int nv,
+                                                    double *__restrict__ out)
 {
-    const int i = threadIdx.x;
+    const int i = blockIdx.x;
     if (i >= nv) return;
     double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += part[(long long)b * (MAXM + 1) + i];
-    out[i] = s;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long long)b * (MAXM + 1) + i];
+    __shared__ double sh[4];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[i] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // w += sign * sum_i h_i V_i (active cells)
@@ -288,14 +295,25 @@ __global__ void __launch_bounds__(256) k_sumsq(const unsigned long long *rng, co
 }
 
 // s = sqrt(sum r^2 / count), 1/s (1 when r is zero)
-__global__ void k_rms_final(const double *__restrict__ part, int nblk, double *__restrict__ scal)
+__global__ void __launch_bounds__(256) k_rms_final(const double *__restrict__ part, int nblk,
+                                                   double *__restrict__ scal)
 {
-    if (threadIdx.x != 0) return;
     double a = 0.0, c = 0.0;
-    for (int b = 0; b < nblk; b++) {
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         a += part[(long long)b * (MAXM + 1)];
         c += part[(long long)b * (MAXM + 1) + 1];
     }
+    __shared__ double sh[2][4];
+    a = wave_sum(a);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    a = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    c = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     const double s = c > 0.0 ? sqrt(a / c) : 0.0;
     scal[0] = s;
     scal[1] = s > 0.0 ? 1.0 / s : 1.0;
@@ -363,7 +381,7 @@ int rms(cad_ctx *c, const double *r)
 {
     Learned *L = c->lrn;
     hipLaunchKernelGGL(k_sumsq, dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, c->act_rng, c->f.indc, r, L->part);
-    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(64), 0, c->stream, L->part, REDUCE_BLOCKS, L->scal);
+    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, c->stream, L->part, REDUCE_BLOCKS, L->scal);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
@@ -409,7 +427,7 @@ int mdot(cad_ctx *c, const double *V, int nv, const double *w, double *host)
     Learned *L = c->lrn;
     hipLaunchKernelGGL(k_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, c->act_rng, c->f.indc, V,
                        c->d.nc, nv, w, L->part);
-    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, c->stream, L->part, REDUCE_BLOCKS, nv, L->hdev);
+    hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, c->stream, L->part, REDUCE_BLOCKS, nv, L->hdev);
     CAD_CHECK(hipGetLastError());
     CAD_CHECK(hipMemcpyAsync(L->hpin, L->hdev, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CAD_CHECK(hipStreamSynchronize(c->stream));
