@@ -265,211 +265,274 @@ struct __attribute__((aligned(16))) Smem {
     int tile;                                // this workgroup's tile
 };
 
+// Per-lane maps of one tile.  Lane = jl + TJ*kl owns logical line L = lane
+// and walks it with a skew of jl + kl steps; the maps give its line, the
+// offsets of the operand loads and result stores of stage 0 (stage b adds
+// b*DIR*S cells), the mailbox entries the lane watches and those it feeds.
+template <int S, int DIR, int TJ>
+struct TileMap {
+    static constexpr int TK = 64 / TJ;
+    static constexpr int NG = Lay<S>::NG, LPI = Lay<S>::LPI;
+    static constexpr int LPS = 64 / S;       // lines per result store
+    static constexpr int HELPER = 1 + TJ;    // lane (1,1): reads lane (0,0)'s K entry
+    int ni, nj, nk;
+    long long nij;
+    int tile, bj, bk, jl, kl, skew, len;
+    int2 rg;
+    bool valid;
+    unsigned ld_off[NG], st_off[S];
+    unsigned stage_step;                     // byte step per stage (mod 2^32)
+    int hlo, hn, fsk, fn;                    // watched / fed windows of steps
+    unsigned h0, f0;
+
+    __device__ __forceinline__ int phys_j(int j) const { return DIR > 0 ? j : TJ - 1 - j; }
+    __device__ __forceinline__ int phys_k(int k) const { return DIR > 0 ? k : TK - 1 - k; }
+    __device__ __forceinline__ bool line_ok(int L) const
+    {
+        return 1 + bj * TJ + phys_j(L % TJ) <= nj - 2 && 1 + bk * TK + phys_k(L / TJ) <= nk - 2;
+    }
+    __device__ __forceinline__ long long line_base(int L) const
+    {
+        return (long long)(1 + bj * TJ + phys_j(L % TJ)) * ni + (long long)(1 + bk * TK + phys_k(L / TJ)) * nij;
+    }
+    // first cell (ascending I) of line L's window in stage 0
+    __device__ __forceinline__ int win0(int L) const
+    {
+        const int sk = L % TJ + L / TJ;
+        return DIR > 0 ? rg.x - sk : rg.y - (S - 1) + sk;
+    }
+    __device__ __forceinline__ unsigned mb_off(int tl, int l, int i) const
+    {
+        return (unsigned)((((long long)tl * MB_LINES + l) * ni + i) * 16);
+    }
+
+    __device__ __forceinline__ void init(const SweepGeom &geo, const SweepCtl &c, int t, int lane)
+    {
+        ni = geo.ni;
+        nj = geo.nj;
+        nk = geo.nk;
+        nij = geo.nij;
+        tile = t;
+        bj = t % geo.ntj;
+        bk = t / geo.ntj;
+        rg = c.range[t];
+        len = rg.y - rg.x + 1;
+        jl = lane % TJ;
+        kl = lane / TJ;
+        skew = jl + kl;
+        valid = line_ok(lane);
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+            const int L = q * LPI + lane % LPI;
+            const long long e = line_base(L) + win0(L) + 2 * (lane / LPI);
+            ld_off[q] = line_ok(L) && e >= 0 ? (unsigned)(e * 8) : OOB;
+        }
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            const int L = q * LPS + lane / S;
+            const long long e = line_base(L) + win0(L) + lane % S;
+            st_off[q] = line_ok(L) && e >= 0 ? (unsigned)(e * 8) : OOB;
+        }
+        stage_step = (unsigned)(DIR * S * 8);
+
+        // Mailbox reads.  A lane with jl == 0 reads the J-upstream tile's
+        // entry for its own line, a lane with kl == 0 (jl > 0) the K-upstream
+        // one; lane HELPER, which has no halo of its own, reads lane (0,0)'s
+        // K entry.
+        const unsigned half_k = (unsigned)c.mb_bytes;
+        const int pbj = bj - DIR, pbk = bk - DIR;
+        const int ptj = pbj >= 0 && pbj < geo.ntj ? pbj + bk * geo.ntj : -1;
+        const int ptk = pbk >= 0 && pbk < geo.ntk ? bj + pbk * geo.ntj : -1;
+        const bool has_cj = bj + DIR >= 0 && bj + DIR < geo.ntj;
+        const bool has_ck = bk + DIR >= 0 && bk + DIR < geo.ntk;
+        int hsrc = -1, hline = 0, hsk = skew;
+        bool hval = valid;
+        unsigned hbase = 0;
+        int2 hr = make_int2(1, 0);
+        if (jl == 0) {
+            if (ptj >= 0) {
+                hsrc = ptj;
+                hr = c.range[ptj];
+            }
+            hline = phys_k(kl);
+            if (kl == TK - 1) {
+                // the J-upstream corner cell (its lane 63) is published on
+                // its K side only
+                hbase = half_k;
+                hline = phys_j(TJ - 1);
+            }
+        } else if (kl == 0 || lane == HELPER) {
+            if (ptk >= 0) {
+                hsrc = ptk;
+                hr = c.range[ptk];
+            }
+            hbase = half_k;
+            hline = phys_j(jl);
+            if (lane == HELPER) {
+                hline = phys_j(0);
+                hsk = 0;
+                hval = line_ok(0);
+            }
+        }
+        // the watched entry of step t is at h0 + DIR*16*t for t in
+        // [hlo, hlo+hn): the steps where the watched cell is walked and lies
+        // in the producer's range
+        hlo = 0;
+        hn = 0;
+        h0 = 0;
+        if (hsrc >= 0 && hval) {
+            // cell_i(t, hsk) in [hr.x, hr.y] and 0 <= t - hsk < len
+            const int ta = DIR > 0 ? hr.x - rg.x + hsk : rg.y - hr.y + hsk;
+            const int tb = DIR > 0 ? hr.y - rg.x + hsk : rg.y - hr.x + hsk;
+            hlo = max(ta, hsk);
+            hn = max(min(tb, hsk + len - 1) - hlo + 1, 0);
+            h0 = hbase + mb_off(hsrc, hline, 0) + (unsigned)((DIR > 0 ? rg.x - hsk : rg.y + hsk) * 16);
+        }
+        // Mailbox writes, one per lane and step.  Lanes with kl == TK-1 feed
+        // the K-downstream tile; lanes with jl == TJ-1 (kl < TK-1) the
+        // J-downstream one.  The corner lane 63 writes its K entry only,
+        // which the J-downstream tile reads too, so it writes it when either
+        // downstream tile exists.
+        bool fok = false;
+        int fline = 0;
+        unsigned fbase = 0;
+        fsk = skew;
+        if (kl == TK - 1) {
+            fok = valid && (has_ck || (jl == TJ - 1 && has_cj));
+            fline = phys_j(jl);
+            fbase = half_k;
+        } else if (jl == TJ - 1) {
+            fok = valid && has_cj;
+            fline = phys_k(kl);
+        }
+        // the fed entry of step t is at f0 + DIR*16*t for t in [fsk, fsk+fn)
+        f0 = fbase + mb_off(t, fline, 0) + (unsigned)((DIR > 0 ? rg.x - fsk : rg.y + fsk) * 16);
+        fn = fok ? len : 0;
+    }
+    __device__ __forceinline__ unsigned h_off(int t) const
+    {
+        return (unsigned)(t - hlo) < (unsigned)hn ? h0 + (unsigned)(DIR * 16 * t) : OOB;
+    }
+    __device__ __forceinline__ unsigned f_off(int t) const
+    {
+        return (unsigned)(t - fsk) < (unsigned)fn ? f0 + (unsigned)(DIR * 16 * t) : OOB;
+    }
+    // offset of result store q of stage b for this lane, OOB outside the
+    // tile's active I range
+    __device__ __forceinline__ unsigned store_off(int q, int lane, int b) const
+    {
+        const int L = q * LPS + lane / S, e = lane % S;
+        const int skl = L % TJ + L / TJ;
+        const int i = (DIR > 0 ? rg.x - skl + S * b : rg.y - (S - 1) + skl - S * b) + e;
+        return st_off[q] == OOB || i < rg.x || i > rg.y ? OOB : st_off[q] + stage_step * (unsigned)b;
+    }
+};
+
+// issue the loads of stage b's operands into LDS buffer b&1
+template <class Op, class TM>
+__device__ __forceinline__ void load_stage(Smem<Op> &sm, const __amdgpu_buffer_rsrc_t *rs, const TM &m, int b)
+{
+    using LY = Lay<Op::S>;
+    const unsigned adv = m.stage_step * (unsigned)b;
+#pragma unroll
+    for (int s = 0; s < Op::NS; s++) {
+#pragma unroll
+        for (int q = 0; q < LY::NG; q++)
+            load_lds16(rs[s], &sm.st[b & 1][s][LY::gb(q)], m.ld_off[q] == OOB ? OOB : m.ld_off[q] + adv);
+    }
+}
+
+// results of stage b (in xo[b&1]): store q covers lines q*LPS .. q*LPS+LPS-1.
+// Every cell of the tile's active I range is stored, inactive ones as 0 (the
+// value the output array holds there), so that each line's window is one
+// contiguous run and HBM sees whole lines rather than masked pieces; cells
+// outside the range are not stored.
+template <class Op, class TM>
+__device__ __forceinline__ void store_stage(const Smem<Op> &sm, __amdgpu_buffer_rsrc_t rout, const TM &m,
+                                            int lane, int b)
+{
+    constexpr int S = Op::S;
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+        double x = sm.xo[b & 1][q * TM::LPS + lane / S][lane % S];
+        if (is_mark(x)) x = 0.0;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), rout, m.store_off(q, lane, b), 0, 0);
+    }
+}
+
+// The tile of this workgroup from the ticket (thread 0 draws it), or -1.
+__device__ __forceinline__ int draw_tile(const SweepGeom &g, const SweepCtl &c, int dir, int *slot)
+{
+    if (threadIdx.x == 0) {
+        const unsigned long long tk = atomicAdd(c.ticket, 1ull);
+        const long long tix = (long long)(tk - c.base);
+        if (tix < 0 || tix >= g.ntiles) {
+            atomicOr(c.err, 1);
+            *slot = -1;
+        } else {
+            *slot = c.order[dir > 0 ? tix : g.ntiles - 1 - tix];
+        }
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+// One step of a lane's walk: the reference's expression on the operands of
+// the cell and its three upstream neighbours' new values (xi from the lane's
+// previous step, xj / xk from lanes jl-1 / kl-1 one step earlier or, on the
+// tile's upstream faces, from the halo values hv and hk0 of lane (0,0)'s K
+// neighbour).
+template <class Op, int TJ>
+__device__ __forceinline__ double sweep_step(const double *v, bool act, double xcur, double hv, double hk0,
+                                             int jl, int kl, int lane)
+{
+    const double sj = from_lane_minus1(xcur);
+    const double sk = from_lane(xcur, lane - TJ);
+    const double xj = jl == 0 ? hv : sj;
+    const double xk = kl == 0 ? (jl == 0 ? hk0 : hv) : sk;
+    const double ev = Op::eval(v, xcur, xj, xk);
+    return act ? ev : 0.0;
+}
+
 template <class Op, int DIR, int TJ>
 __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
 {
-    constexpr int TK = 64 / TJ;
     constexpr int NS = Op::NS, S = Op::S;
     // mailbox read-ahead (steps); the ring is indexed by p % HD, so HD must
     // divide the stage length
     constexpr int HD = 4;
     static_assert(S % HD == 0, "ring depth must divide the stage length");
     using LY = Lay<S>;
+    using TM = TileMap<S, DIR, TJ>;
     __shared__ Smem<Op> sm;
     const int lane = threadIdx.x & 63;
     const bool loader = threadIdx.x >= 64;   // wave 1 moves memory, wave 0 computes
-    if (threadIdx.x == 0) {
-        const unsigned long long tk = atomicAdd(c.ticket, 1ull);
-        const long long tix = (long long)(tk - c.base);
-        if (tix < 0 || tix >= g.ntiles) {
-            atomicOr(c.err, 1);
-            sm.tile = -1;
-        } else {
-            sm.tile = c.order[DIR > 0 ? tix : g.ntiles - 1 - tix];
-        }
-    }
-    __syncthreads();
-    const int tile = __builtin_amdgcn_readfirstlane(sm.tile);
+    const int tile = draw_tile(g, c, DIR, &sm.tile);
     if (tile < 0) return;
-    const int bj = tile % g.ntj, bk = tile / g.ntj;
-    const int2 rg = c.range[tile];
-    if (rg.x > rg.y) return;
-    const int len = rg.y - rg.x + 1;
-
-    // geometry of logical line L = jl + TJ*kl
-    auto phys_j = [&] __device__(int jl) { return DIR > 0 ? jl : TJ - 1 - jl; };
-    auto phys_k = [&] __device__(int kl) { return DIR > 0 ? kl : TK - 1 - kl; };
-    auto line_ok = [&] __device__(int L) {
-        return 1 + bj * TJ + phys_j(L % TJ) <= g.nj - 2 && 1 + bk * TK + phys_k(L / TJ) <= g.nk - 2;
-    };
-    auto line_base = [&] __device__(int L) {
-        return (long long)(1 + bj * TJ + phys_j(L % TJ)) * g.ni +
-               (long long)(1 + bk * TK + phys_k(L / TJ)) * g.nij;
-    };
-    // first cell (ascending I) of line L's window in stage 0
-    auto win0 = [&] __device__(int L) {
-        const int sk = L % TJ + L / TJ;
-        return DIR > 0 ? rg.x - sk : rg.y - (S - 1) + sk;
-    };
-    auto cell_i = [&] __device__(int t, int sk) { return DIR > 0 ? rg.x + (t - sk) : rg.y - (t - sk); };
-
-    const int jl = lane % TJ, kl = lane / TJ;
-    const int skew = jl + kl;
-    const bool valid = line_ok(lane);
-
-    // per-lane byte offsets (stage 0) of the operand loads and result stores
-    unsigned ld_off[LY::NG];
-#pragma unroll
-    for (int q = 0; q < LY::NG; q++) {
-        const int L = q * LY::LPI + lane % LY::LPI;
-        const long long e = line_base(L) + win0(L) + 2 * (lane / LY::LPI);
-        ld_off[q] = line_ok(L) && e >= 0 ? (unsigned)(e * 8) : OOB;
-    }
-    constexpr int LPS = 64 / S;              // lines per result store
-    unsigned st_off[S];
-#pragma unroll
-    for (int q = 0; q < S; q++) {
-        const int L = q * LPS + lane / S;
-        const long long e = line_base(L) + win0(L) + lane % S;
-        st_off[q] = line_ok(L) && e >= 0 ? (unsigned)(e * 8) : OOB;
-    }
-    const unsigned stage_step = (unsigned)(DIR * S * 8);   // byte step per stage (mod 2^32)
-
-    // mailboxes
-    const __amdgpu_buffer_rsrc_t rmb = rsrc(c.mb, 2 * c.mb_bytes);
-    const unsigned half_k = (unsigned)c.mb_bytes;
-    auto mb_off = [&] __device__(int tl, int l, int i) {
-        return (unsigned)((((long long)tl * MB_LINES + l) * g.ni + i) * 16);
-    };
-    const unsigned seq = c.seq;
-    const int pbj = bj - DIR, pbk = bk - DIR;
-    const int ptj = pbj >= 0 && pbj < g.ntj ? pbj + bk * g.ntj : -1;
-    const int ptk = pbk >= 0 && pbk < g.ntk ? bj + pbk * g.ntj : -1;
-    const bool has_cj = bj + DIR >= 0 && bj + DIR < g.ntj;
-    const bool has_ck = bk + DIR >= 0 && bk + DIR < g.ntk;
-
-    // Mailbox reads.  A lane with jl == 0 reads the J-upstream tile's entry
-    // for its own line, a lane with kl == 0 (jl > 0) the K-upstream one; lane
-    // (1,1), which has no halo of its own, reads lane (0,0)'s K entry.
-    constexpr int HELPER = 1 + TJ;
-    int hsrc = -1, hline = 0, hsk = skew;
-    bool hval = valid;
-    unsigned hbase = 0;
-    int2 hr = make_int2(1, 0);
-    if (jl == 0) {
-        if (ptj >= 0) {
-            hsrc = ptj;
-            hr = c.range[ptj];
-        }
-        hline = phys_k(kl);
-        if (kl == TK - 1) {
-            // the J-upstream corner cell (its lane 63) is published on its K
-            // side only
-            hbase = half_k;
-            hline = phys_j(TJ - 1);
-        }
-    } else if (kl == 0 || lane == HELPER) {
-        if (ptk >= 0) {
-            hsrc = ptk;
-            hr = c.range[ptk];
-        }
-        hbase = half_k;
-        hline = phys_j(jl);
-        if (lane == HELPER) {
-            hline = phys_j(0);
-            hsk = 0;
-            hval = line_ok(0);
-        }
-    }
-    // the watched entry of step t is at h0 + DIR*16*t for t in [hlo, hlo+hn)
-    // (the steps where the watched cell is walked and lies in the producer's
-    // range), else not needed
-    int hlo = 0, hn = 0;             // window [hlo, hlo+hn)
-    unsigned h0 = 0;
-    if (hsrc >= 0 && hval) {
-        // cell_i(t, hsk) in [hr.x, hr.y] and 0 <= t - hsk < len
-        const int ta = DIR > 0 ? hr.x - rg.x + hsk : rg.y - hr.y + hsk;
-        const int tb = DIR > 0 ? hr.y - rg.x + hsk : rg.y - hr.x + hsk;
-        hlo = max(ta, hsk);
-        hn = max(min(tb, hsk + len - 1) - hlo + 1, 0);
-        h0 = hbase + mb_off(hsrc, hline, 0) + (unsigned)((DIR > 0 ? rg.x - hsk : rg.y + hsk) * 16);
-    }
-    auto h_off = [&] __device__(int t) {
-        return (unsigned)(t - hlo) < (unsigned)hn ? h0 + (unsigned)(DIR * 16 * t) : OOB;
-    };
-    // Mailbox writes, one per lane and step.  Lanes with kl == TK-1 feed the
-    // K-downstream tile; lanes with jl == TJ-1 (kl < TK-1) the J-downstream
-    // one.  The corner lane 63 writes its K entry only, which the J-downstream
-    // tile reads too, so it writes it when either downstream tile exists.
-    bool fok = false;
-    int fline = 0;
-    const int fsk = skew;
-    unsigned fbase = 0;
-    if (kl == TK - 1) {
-        fok = valid && (has_ck || (jl == TJ - 1 && has_cj));
-        fline = phys_j(jl);
-        fbase = half_k;
-    } else if (jl == TJ - 1) {
-        fok = valid && has_cj;
-        fline = phys_k(kl);
-    }
-    // the fed entry of step t is at f0 + DIR*16*t for t in [fsk, fsk+fn)
-    const unsigned f0 = fbase + mb_off(tile, fline, 0) + (unsigned)((DIR > 0 ? rg.x - fsk : rg.y + fsk) * 16);
-    const int fn = fok ? len : 0;
-    auto f_off = [&] __device__(int t) {
-        return (unsigned)(t - fsk) < (unsigned)fn ? f0 + (unsigned)(DIR * 16 * t) : OOB;
-    };
-
-    __amdgpu_buffer_rsrc_t rs[NS];
-#pragma unroll
-    for (int m = 0; m < NS; m++) rs[m] = rsrc(op.src[m], (unsigned long long)g.nc * 8);
-    const __amdgpu_buffer_rsrc_t rout = rsrc(op.out, (unsigned long long)g.nc * 8);
-
-    // issue the loads of stage b into LDS buffer b&1
-    auto load_stage = [&] __device__(int b) {
-        const unsigned adv = stage_step * (unsigned)b;
-#pragma unroll
-        for (int m = 0; m < NS; m++) {
-#pragma unroll
-            for (int q = 0; q < LY::NG; q++)
-                load_lds16(rs[m], &sm.st[b & 1][m][LY::gb(q)], ld_off[q] == OOB ? OOB : ld_off[q] + adv);
-        }
-    };
-
-    const int nsteps = len + (TJ - 1) + (TK - 1);
+    if (c.range[tile].x > c.range[tile].y) return;
+    TM m;
+    m.init(g, c, tile, lane);
+    const int nsteps = m.len + (TJ - 1) + (TM::TK - 1);
     const int nstage = (nsteps + S - 1) / S;
-
-    // results of stage b (in xo[b&1]): store q covers lines q*LPS .. q*LPS+LPS-1.
-    // Every cell of the tile's active I range is stored, inactive ones as 0
-    // (the value the output array holds there), so that each line's window is
-    // one contiguous run and HBM sees whole lines rather than masked pieces;
-    // cells outside the range are not stored.
-    auto store_stage = [&] __device__(int b) {
-        const unsigned adv = stage_step * (unsigned)b;
-#pragma unroll
-        for (int q = 0; q < S; q++) {
-            const int L = q * LPS + lane / S, e = lane % S;
-            const int skl = L % TJ + L / TJ;
-            const int i = (DIR > 0 ? rg.x - skl + S * b : rg.y - (S - 1) + skl - S * b) + e;
-            double x = sm.xo[b & 1][L][e];
-            if (is_mark(x)) x = 0.0;
-            const unsigned o = st_off[q] == OOB || i < rg.x || i > rg.y ? OOB : st_off[q] + adv;
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), rout, o, 0, 0);
-        }
-    };
+    const __amdgpu_buffer_rsrc_t rmb = rsrc(c.mb, 2 * c.mb_bytes);
+    const unsigned seq = c.seq;
 
     if (loader) {
+        __amdgpu_buffer_rsrc_t rs[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) rs[s] = rsrc(op.src[s], (unsigned long long)g.nc * 8);
+        const __amdgpu_buffer_rsrc_t rout = rsrc(op.out, (unsigned long long)g.nc * 8);
         // stage b+1's operands in while stage b computes; stage b-1's results out
-        load_stage(0);
+        load_stage(sm, rs, m, 0);
         __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
         lds_barrier();
         for (int b = 0; b < nstage; b++) {
-            if (b + 1 < nstage) load_stage(b + 1);
-            if (b > 0) store_stage(b - 1);
+            if (b + 1 < nstage) load_stage(sm, rs, m, b + 1);
+            if (b > 0) store_stage(sm, rout, m, lane, b - 1);
             __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
             lds_barrier();
         }
-        store_stage(nstage - 1);
+        store_stage(sm, rout, m, lane, nstage - 1);
         return;
     }
 
@@ -485,7 +548,7 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
     // stage do not also wait for the loads and stores of the last few steps.
 #pragma unroll
     for (int d = 0; d < HD; d++) {
-        ring[d] = mb_load(rmb, h_off(d));
+        ring[d] = mb_load(rmb, m.h_off(d));
         mb_store(rmb, OOB, 0.0, seq);
         // the stores have one address: keep each (no store merging)
         __asm__ __volatile__("" ::: "memory");
@@ -498,38 +561,32 @@ __global__ void __launch_bounds__(128) k_sweep(SweepGeom g, SweepCtl c, Op op)
         for (int p = 0; p < S; p++) {
             const int e = DIR > 0 ? p : S - 1 - p;
 #pragma unroll
-            for (int m = 0; m < NS; m++)
-                sv[p][m] = *reinterpret_cast<const double *>(&sm.st[b & 1][m][LY::addr(lane, e)]);
+            for (int s = 0; s < NS; s++)
+                sv[p][s] = *reinterpret_cast<const double *>(&sm.st[b & 1][s][LY::addr(lane, e)]);
         }
 #pragma unroll
         for (int p = 0; p < S; p++) {
             const int t = b * S + p;
-            const bool lv = valid && (unsigned)(t - skew) < (unsigned)len;
+            const bool lv = m.valid && (unsigned)(t - m.skew) < (unsigned)m.len;
             const int e = DIR > 0 ? p : S - 1 - p;
             const bool act = lv && Op::active(sv[p]);
-            const double xi = xcur;
-            const double sj = from_lane_minus1(xcur);
-            const double sk = from_lane(xcur, lane - TJ);
             // mailbox entry of this step (read HD steps ago); a stale one is
             // polled for, in a wave-uniform slow path
             v3u he = ring[p % HD];
-            const unsigned ho = h_off(t);
+            const unsigned ho = m.h_off(t);
             const bool act0 = (__builtin_amdgcn_ballot_w64(act) & 1ull) != 0;
-            const bool stale = ho != OOB && (lane == HELPER ? act0 : act) && he.z != seq;
+            const bool stale = ho != OOB && (lane == TM::HELPER ? act0 : act) && he.z != seq;
             if (__builtin_amdgcn_ballot_w64(stale)) {
                 if (stale) he = mb_wait(c.mb, 2 * c.mb_bytes, ho, seq, c.err);
             }
-            ring[p % HD] = mb_load(rmb, h_off(t + HD));
+            ring[p % HD] = mb_load(rmb, m.h_off(t + HD));
             const double hv = __builtin_bit_cast(double, v2u{he.x, he.y});
-            const double hk0 = read_lane(hv, HELPER);
-            const double xj = jl == 0 ? hv : sj;
-            const double xk = kl == 0 ? (jl == 0 ? hk0 : hv) : sk;
-            const double ev = Op::eval(sv[p], xi, xj, xk);
-            const double r = act ? ev : 0.0;
+            const double hk0 = read_lane(hv, TM::HELPER);
+            const double r = sweep_step<Op, TJ>(sv[p], act, xcur, hv, hk0, m.jl, m.kl, lane);
             if (Op::CHECK_PIVOT) bad_pivot = bad_pivot || (act && r <= 0.0);
             xcur = r;
             sm.xo[b & 1][lane][e] = act ? r : __builtin_bit_cast(double, MARK);
-            mb_store(rmb, f_off(t), r, seq);
+            mb_store(rmb, m.f_off(t), r, seq);
         }
         lds_barrier();
     }
@@ -663,10 +720,11 @@ int launch(cad_ctx *c, const Op &op)
     const int shape = c->sw_shape;
     SweepGeom g = geom(c, shape);
     SweepCtl s = ctl(c, shape);
-    if (shape == 0)
+    if (shape == 0) {
         hipLaunchKernelGGL((k_sweep<Op, DIR, 8>), dim3(g.ntiles), dim3(128), 0, c->stream, g, s, op);
-    else
+    } else {
         hipLaunchKernelGGL((k_sweep<Op, DIR, 16>), dim3(g.ntiles), dim3(128), 0, c->stream, g, s, op);
+    }
     CAD_CHECK(hipGetLastError());
     c->sw_base += (unsigned long long)g.ntiles;
     c->sw_seq += 1;
