@@ -31,14 +31,18 @@
 //
 // V-cycle.  Red-black Gauss-Seidel (one sweep = the cells with I+J+K even,
 // then odd) before and after the coarse correction; the residual of the fine
-// cells is summed into the coarse right-hand side in one kernel; the coarse
-// correction is added scaled by MG_SCALE (piecewise-constant prolongation
-// under-corrects smooth errors); the coarsest level is smoothed MG_COARSE
+// cells is summed into the coarse right-hand side in one kernel (after the
+// odd half-sweep only the even cells' residuals are not zero up to rounding);
+// the coarse correction is added scaled by MG_SCALE (piecewise-constant
+// prolongation under-corrects smooth errors), on the fly inside the even
+// half-sweep that follows it; the coarsest level is smoothed MG_COARSE
 // times.  Every level starts from x = 0, so the first (even) half of its
 // first sweep is x = b/AD, which the kernel that sets b writes at once.  The
 // levels of at most MG_TAIL_CELLS cells run as one workgroup (k_mg_tail), so
 // the small grids cost one launch instead of five per level.
 #include "cad_common.h"
+
+#include <climits>
 
 #define RET_IF(x)              \
     do {                       \
@@ -182,7 +186,25 @@ __global__ void __launch_bounds__(256) k_mg_smooth(MgLevelT<V> L, int color, int
     L.x[n] = (L.b[n] - row_off(L, n, i, j, k)) / L.ad[n];
 }
 
-// coarse right-hand side = sum of the fine residuals of the block; coarse x = 0
+// Coarse right-hand side = sum of the fine residuals of the block; coarse
+// x = 0.  It follows the odd half-sweep, after which an odd cell's residual
+// is zero up to rounding (its x was just solved from its row), so only the
+// block's four even cells are summed.
+template <typename V>
+__device__ __forceinline__ V block_residual(const MgLevelT<V> &F, int I, int J, int K)
+{
+    const Dims &fd = F.d;
+    V s = 0;
+    for (int c = 0; c < 8; c++) {
+        const int i = 2 * I - 2 + (c & 1), j = 2 * J - 2 + ((c >> 1) & 1), k = 2 * K - 2 + (c >> 2);
+        if (((i + j + k) & 1) != 0 || i > fd.ni - 1 || j > fd.nj - 1 || k > fd.nk - 1) continue;
+        const long long n = pos(F, i, j, k);
+        if (!F.act[n]) continue;
+        s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n, i, j, k);
+    }
+    return s;
+}
+
 template <typename V>
 __global__ void __launch_bounds__(256) k_mg_restrict(MgLevelT<V> F, MgLevelT<V> C)
 {
@@ -190,15 +212,7 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MgLevelT<V> F, MgLevelT<V> 
     const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int K = blockIdx.z * blockDim.z + threadIdx.z + 2;
     if (I > C.d.ni - 1 || J > C.d.nj - 1 || K > C.d.nk - 1) return;
-    const Dims &fd = F.d;
-    V s = 0;
-    for (int c = 0; c < 8; c++) {
-        const int i = 2 * I - 2 + (c & 1), j = 2 * J - 2 + ((c >> 1) & 1), k = 2 * K - 2 + (c >> 2);
-        if (i > fd.ni - 1 || j > fd.nj - 1 || k > fd.nk - 1) continue;
-        const long long n = pos(F, i, j, k);
-        if (!F.act[n]) continue;
-        s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n, i, j, k);
-    }
+    const V s = block_residual(F, I, J, K);
     const long long m = pos(C, I, J, K);
     const V bc = C.act[m] ? s : (V)0;
     C.b[m] = bc;
@@ -206,17 +220,35 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MgLevelT<V> F, MgLevelT<V> 
     C.x[m] = C.act[m] && ((I + J + K) & 1) == 0 ? bc / C.ad[m] : (V)0;
 }
 
-// x += MG_SCALE * coarse x of the block (active fine cells)
+// The coarse correction and the even half-sweep after it, in one pass.  The
+// correction x += MG_SCALE * coarse x of the block matters only at the odd
+// cells: the even half-sweep solves each even cell's x from its row, which
+// reads the odd neighbours only.  So each even cell reads its odd
+// neighbours' x plus their correction (what a separate correction pass would
+// have stored), and the odd cells, which the odd half-sweep that follows
+// solves from their rows in turn, are not corrected in memory.
 template <typename V>
-__global__ void __launch_bounds__(256) k_mg_prolong(MgLevelT<V> F, MgLevelT<V> C, int k0)
+__device__ __forceinline__ V corrected(const MgLevelT<V> &F, const MgLevelT<V> &C, int i, int j, int k)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const long long n = pos(F, i, j, k);
+    return F.act[n] ? F.x[n] + (V)MG_SCALE * C.x[pos(C, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)] : F.x[n];
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_mg_correct_even(MgLevelT<V> F, MgLevelT<V> C, int k0)
+{
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int k = blockIdx.z + k0;
-    if (i > F.d.ni - 1 || j > F.d.nj - 1 || k > F.d.nk - 1) return;
+    if (j > F.d.nj - 1 || k > F.d.nk - 1) return;
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = 2 * q + 1 + ((1 + j + k) & 1);
+    if (i < 2 || i > F.d.ni - 1) return;
     const long long n = pos(F, i, j, k);
     if (!F.act[n]) return;
-    F.x[n] += (V)MG_SCALE * C.x[pos(C, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)];
+    const V off = F.ali[n] * corrected(F, C, i - 1, j, k) + F.aui[n] * corrected(F, C, i + 1, j, k)
+                + F.alj[n] * corrected(F, C, i, j - 1, k) + F.auj[n] * corrected(F, C, i, j + 1, k)
+                + F.alk[n] * corrected(F, C, i, j, k - 1) + F.auk[n] * corrected(F, C, i, j, k + 1);
+    F.x[n] = (F.b[n] - off) / F.ad[n];
 }
 
 // level-0 right-hand side from a vector of the grid (natural order) on the K
@@ -251,8 +283,15 @@ __global__ void __launch_bounds__(256) k_mg_out(const MgLevelT<V> L, T *__restri
 
 // The V-cycle below level t0 inside one workgroup: for each level down to the
 // coarsest the odd half-sweep (the even half was set with b), the coarse
-// right-hand side; MG_COARSE sweeps on the coarsest; then up: correction and
-// one sweep.  Levels are a kernel argument (at most CAD_MG_MAX).
+// right-hand side; MG_COARSE sweeps on the coarsest; then up: the even
+// half-sweep with the correction, and the odd one.  The coarsest level, whose
+// sweeps are most of the tail's phases, is swept in LDS: its x and b are
+// copied in, each thread keeps its cell's row in registers, and x goes back
+// at the end, so those phases touch no memory.  Levels are a kernel argument
+// (at most CAD_MG_MAX).
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_LDS_CELLS = 8192;         // positions of the coarsest level held in LDS
+
 template <typename V>
 struct MgTail {
     MgLevelT<V> L[CAD_MG_MAX];
@@ -272,8 +311,62 @@ __device__ void tail_half(const MgLevelT<V> &L, int color)
     __syncthreads();
 }
 
+// the even half-sweep of F with the correction from C (k_mg_correct_even)
 template <typename V>
-__global__ void __launch_bounds__(1024) k_mg_tail(MgTail<V> T, int t0, int nl)
+__device__ void tail_correct_even(const MgLevelT<V> &F, const MgLevelT<V> &C)
+{
+    const int ni = F.d.ni - 2, nj = F.d.nj - 2, nk = F.d.nk - 2;
+    for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
+        const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
+        if (((i + j + k) & 1) != 0) continue;
+        const long long n = pos(F, i, j, k);
+        if (!F.act[n]) continue;
+        const V off = F.ali[n] * corrected(F, C, i - 1, j, k) + F.aui[n] * corrected(F, C, i + 1, j, k)
+                    + F.alj[n] * corrected(F, C, i, j - 1, k) + F.auj[n] * corrected(F, C, i, j + 1, k)
+                    + F.alk[n] * corrected(F, C, i, j, k - 1) + F.auk[n] * corrected(F, C, i, j, k + 1);
+        F.x[n] = (F.b[n] - off) / F.ad[n];
+    }
+    __syncthreads();
+}
+
+// MG_COARSE sweeps of the coarsest level B in LDS (the even half of the
+// first came with b); at most TAIL_THREADS cells and TAIL_LDS_CELLS
+// positions (the host sweeps larger coarsest levels in memory)
+template <typename V>
+__device__ void tail_coarsest(const MgLevelT<V> &B)
+{
+    __shared__ V xs[TAIL_LDS_CELLS], bs[TAIL_LDS_CELLS];
+    const int ns = B.ld * B.d.nj * B.d.nk;
+    for (int p = threadIdx.x; p < ns; p += blockDim.x) {
+        xs[p] = B.x[p];
+        bs[p] = B.b[p];
+    }
+    const int ni = B.d.ni - 2, nj = B.d.nj - 2, nk = B.d.nk - 2;
+    const int q = threadIdx.x;
+    const bool on = q < ni * nj * nk;
+    const int i = on ? q % ni + 2 : 2, j = on ? (q / ni) % nj + 2 : 2, k = on ? q / (ni * nj) + 2 : 2;
+    const long long n = pos(B, i, j, k);
+    const bool act = on && B.act[n];
+    const bool odd = ((i + j + k) & 1) != 0;
+    const V ad = act ? B.ad[n] : (V)1, li = act ? B.ali[n] : (V)0, ui = act ? B.aui[n] : (V)0;
+    const V lj = act ? B.alj[n] : (V)0, uj = act ? B.auj[n] : (V)0, lk = act ? B.alk[n] : (V)0;
+    const V uk = act ? B.auk[n] : (V)0;
+    const long long pi0 = pos(B, i - 1, j, k), pi1 = pos(B, i + 1, j, k), pj0 = pos(B, i, j - 1, k);
+    const long long pj1 = pos(B, i, j + 1, k), pk0 = pos(B, i, j, k - 1), pk1 = pos(B, i, j, k + 1);
+    __syncthreads();
+    for (int h = 1; h < 2 * MG_COARSE; h++) {
+        // h odd: the odd half, h even: the even half
+        if (act && odd == ((h & 1) != 0))
+            xs[n] = (bs[n] - (li * xs[pi0] + ui * xs[pi1] + lj * xs[pj0] + uj * xs[pj1] + lk * xs[pk0]
+                              + uk * xs[pk1])) / ad;
+        __syncthreads();
+    }
+    if (on) B.x[n] = xs[n];
+    __syncthreads();
+}
+
+template <typename V>
+__global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(MgTail<V> T, int t0, int nl, int lds_coarsest)
 {
     for (int l = t0; l < nl - 1; l++) {
         const MgLevelT<V> &F = T.L[l], &C = T.L[l + 1];
@@ -281,14 +374,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTail<V> T, int t0, int nl)
         const int ni = C.d.ni - 2, nj = C.d.nj - 2, nk = C.d.nk - 2;
         for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
             const int I = q % ni + 2, J = (q / ni) % nj + 2, K = q / (ni * nj) + 2;
-            V s = 0;
-            for (int c = 0; c < 8; c++) {
-                const int i = 2 * I - 2 + (c & 1), j = 2 * J - 2 + ((c >> 1) & 1), k = 2 * K - 2 + (c >> 2);
-                if (i > F.d.ni - 1 || j > F.d.nj - 1 || k > F.d.nk - 1) continue;
-                const long long n = pos(F, i, j, k);
-                if (!F.act[n]) continue;
-                s += F.b[n] - F.ad[n] * F.x[n] - row_off(F, n, i, j, k);
-            }
+            const V s = block_residual(F, I, J, K);
             const long long m = pos(C, I, J, K);
             const V bc = C.act[m] ? s : (V)0;
             C.b[m] = bc;
@@ -297,22 +383,18 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTail<V> T, int t0, int nl)
         __syncthreads();
     }
     const MgLevelT<V> &B = T.L[nl - 1];
-    tail_half(B, 1);
-    for (int s = 1; s < MG_COARSE; s++) {
-        tail_half(B, 0);
+    if (lds_coarsest) {
+        tail_coarsest(B);
+    } else {
         tail_half(B, 1);
+        for (int s = 1; s < MG_COARSE; s++) {
+            tail_half(B, 0);
+            tail_half(B, 1);
+        }
     }
     for (int l = nl - 2; l >= t0; l--) {
-        const MgLevelT<V> &F = T.L[l], &C = T.L[l + 1];
-        const int ni = F.d.ni - 2, nj = F.d.nj - 2, nk = F.d.nk - 2;
-        for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
-            const int i = q % ni + 2, j = (q / ni) % nj + 2, k = q / (ni * nj) + 2;
-            const long long n = pos(F, i, j, k);
-            if (F.act[n]) F.x[n] += (V)MG_SCALE * C.x[pos(C, (i + 2) / 2, (j + 2) / 2, (k + 2) / 2)];
-        }
-        __syncthreads();
-        tail_half(F, 0);
-        tail_half(F, 1);
+        tail_correct_even(T.L[l], T.L[l + 1]);
+        tail_half(T.L[l], 1);
     }
 }
 
@@ -450,15 +532,18 @@ int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, 
     }
     MgTail<V> tl;
     for (int l = 0; l < nl; l++) tl.L[l] = mg[l];
-    hipLaunchKernelGGL(k_mg_tail<V>, dim3(1), dim3(1024), 0, c->stream, tl, t0, nl);
+    const MgLevelT<V> &top = mg[nl - 1];
+    const int lds = (long long)(top.d.ni - 2) * (top.d.nj - 2) * (top.d.nk - 2) <= TAIL_THREADS &&
+                    (long long)top.ld * top.d.nj * top.d.nk <= TAIL_LDS_CELLS;
+    hipLaunchKernelGGL(k_mg_tail<V>, dim3(1), dim3(TAIL_THREADS), 0, c->stream, tl, t0, nl, lds);
     CAD_CHECK(hipGetLastError());
-    // up: correct, then one sweep
+    // up: correct, then one sweep (the correction rides on the even half)
     for (int l = t0 - 1; l >= 0; l--) {
         const MgLevelT<V> &F = mg[l], &C = mg[l + 1];
         const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
-        hipLaunchKernelGGL(k_mg_prolong<V>, lvl_grid(F.d, a0, a1), b, 0, c->stream, F, C, a0);
+        hipLaunchKernelGGL(k_mg_correct_even<V>, half_grid(F.d, a0, a1), b, 0, c->stream, F, C, a0);
         CAD_CHECK(hipGetLastError());
-        RET_IF(smooth(c, F, a0, a1, 0));
+        RET_IF(smooth(c, F, a0, a1, 1));
     }
     hipLaunchKernelGGL((k_mg_out<V, T>), lvl_grid(mg[0].d, k0, k1), b, 0, c->stream, mg[0], x, k0);
     CAD_CHECK(hipGetLastError());
@@ -469,16 +554,16 @@ int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, 
 // below the coarsest, three half-sweeps (each cell once in one of them: 7
 // coefficients, b, x written and the activity byte; x of the other colour's
 // neighbours read once: 10 values + 1 byte per cell and colour pair), the
-// residual-restriction (7 coefficients, b, x, activity, and a share of the
-// coarse b and x) and the prolongation (x read and written, activity, the
-// coarse x); on the coarsest level MG_COARSE full sweeps; per cell of level
+// residual-restriction of the even cells (7 coefficients, b, x, activity,
+// and a share of the coarse b and x) and the coarse x that the corrected
+// even half-sweep reads; on the coarsest level MG_COARSE full sweeps; per cell of level
 // 0's K range the FP64 input read with b and x written (AD and the activity
 // read), and x and the activity read with the FP64 output written.
 double vcycle_bytes(const cad_ctx *c, const Dims *dims, int nl, int vs)
 {
     const double half = (10.0 * vs + 1.0) / 2.0;
-    const double restr = 9.0 * vs + 1.0 + 2.0 * vs / 8.0;
-    const double prol = 2.0 * vs + 1.0 + vs / 8.0;
+    const double restr = (9.0 * vs + 1.0) / 2.0 + 2.0 * vs / 8.0;
+    const double prol = vs / 8.0;
     double by = 0.0;
     for (int l = 0; l < nl; l++) {
         const Dims &d = dims[l];
