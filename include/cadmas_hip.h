@@ -433,6 +433,12 @@ int cad_set_learned_weights(cad_ctx *ctx, const float *params, int n);
 int cad_set_learned_mode(cad_ctx *ctx, int mode, int restart);
 int cad_learned_stats(cad_ctx *ctx, long long *out6);
 int cad_learned_box(cad_ctx *ctx, int *box4);
+/* The context's current Poisson matrix on the network's box, FP32 on the
+ * device, coefficient-major (AD, ALI, AUI, ALJ, AUJ, ALK, AUK; 7*nx*ny*nz
+ * values, x fastest; AD = 1 and no couplings beyond the grid): for training
+ * losses that apply the matrix to the network's output.  No reference
+ * counterpart. */
+int cad_learned_box_matrix(cad_ctx *ctx, float *out);
 /* Average duration of the network's forward pass on the current box (HIP
  * events on the context's stream) and its work, 2 FLOP per multiply-add. */
 int cad_time_learned(cad_ctx *ctx, int reps, float *ms_avg, double *flops);
@@ -444,7 +450,11 @@ int cad_learned_sample(cad_ctx *ctx, int mode, const int *iters, int n, void *fe
  * (nb, nz+2, ny+2, nx+2, 8) with zero halos, outputs FP32 nb*nz*ny*nx (x
  * fastest).  train != 0 also holds the gradients and Adam's moments.
  * cad_unet_train_step: one Adam step on the loss mean over boxes of
- * sum m (y-e)^2 / sum m e^2; cad_unet_gradients: that loss and its gradient.
+ * sum m (y-e)^2 / sum m e^2; cad_unet_gradients: that loss and its gradient;
+ * cad_unet_train_step_dy: one Adam step from the caller's gradient dy
+ * (device FP32, the output's order) of any loss with respect to the output
+ * of the last cad_unet_forward; cad_unet_gradients_dy: the parameters'
+ * gradient from such a dy (host FP32), with no step.
  * stream: a hipStream_t or NULL (the network's own). */
 typedef struct cad_unet cad_unet;
 int cad_unet_param_count(void);
@@ -459,6 +469,8 @@ int cad_unet_forward(cad_unet *net);
 int cad_unet_get_output(cad_unet *net, float *dev);
 int cad_unet_set_depth(cad_unet *net, int nz);
 int cad_unet_train_step(cad_unet *net, const float *target, const float *mask, float lr, float *loss);
+int cad_unet_train_step_dy(cad_unet *net, const float *dy, float lr);
+int cad_unet_gradients_dy(cad_unet *net, const float *dy, float *grad);
 int cad_unet_gradients(cad_unet *net, const float *target, const float *mask, float *grad,
                        float *loss);
 

@@ -141,10 +141,11 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_set_wave_table2", "cad_set_wave2", "cad_set_flag_pass_limit",
            "cad_set_true_residual_stop", "cad_m1bcgs_noalloc", "cad_set_krylov_cap",
            "cad_mixed_stats", "cad_time_vcycle", "cad_set_learned_weights", "cad_set_learned_mode", "cad_learned_stats",
-           "cad_learned_box", "cad_learned_sample", "cad_time_learned", "cad_unet_param_count",
+           "cad_learned_box", "cad_learned_box_matrix", "cad_learned_sample", "cad_time_learned", "cad_unet_param_count",
            "cad_unet_mac_per_voxel", "cad_unet_init_params", "cad_unet_create", "cad_unet_destroy",
            "cad_unet_set_params", "cad_unet_get_params", "cad_unet_set_input", "cad_unet_forward",
-           "cad_unet_get_output", "cad_unet_set_depth", "cad_unet_train_step", "cad_unet_gradients")
+           "cad_unet_get_output", "cad_unet_set_depth", "cad_unet_train_step", "cad_unet_train_step_dy", "cad_unet_gradients_dy",
+           "cad_unet_gradients")
 
 class TheoryWave:
     """A theory wave evaluated on the host (cad_wave_theory_eval), with

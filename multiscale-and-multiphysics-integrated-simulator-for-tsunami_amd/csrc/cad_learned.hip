@@ -157,6 +157,28 @@ __global__ void k_box_target(Dims d, int nx, int ny, int kb, int nzr, const int 
     }
 }
 
+// the matrix rows of the box's voxels, FP32, coefficient-major: AD, ALI,
+// AUI, ALJ, AUJ, ALK, AUK (out[c * voxels + u]); outside the grid AD = 1 and
+// no couplings
+__global__ void k_box_matrix(Dims d, int nx, int ny, int kb, int nzr, const double *__restrict__ ad,
+                             const double *__restrict__ ali, const double *__restrict__ aui,
+                             const double *__restrict__ alj, const double *__restrict__ auj,
+                             const double *__restrict__ alk, const double *__restrict__ auk,
+                             float *__restrict__ out)
+{
+    const long long total = (long long)nx * ny * nzr;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int xq = (int)(u % nx), yq = (int)((u / nx) % ny), zq = (int)(u / ((long long)nx * ny));
+        const int i = xq + 2, j = yq + 2, k = zq + kb;
+        const bool in = i <= d.ni - 1 && j <= d.nj - 1 && k <= d.nk - 1;
+        const long long n = in ? cid(d, i, j, k) : 0;
+        const double *a[7] = {ad, ali, aui, alj, auj, alk, auk};
+#pragma unroll
+        for (int q = 0; q < 7; q++) out[q * total + u] = in ? (float)a[q][n] : (q == 0 ? 1.f : 0.f);
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -724,6 +746,25 @@ int cad_learned_box(cad_ctx *c, int *box)
     box[1] = c->lrn->ny;
     box[2] = c->lrn->nzr;
     box[3] = c->lrn->kb;
+    return CAD_OK;
+}
+
+// The context's current matrix on the network's box (k_box_matrix), for
+// training losses that apply it; out holds 7 * nx*ny*nz floats.
+int cad_learned_box_matrix(cad_ctx *c, float *out)
+{
+    if (!c || !out) {
+        cad_set_error("cad_learned_box_matrix: null argument");
+        return CAD_ERR_ARG;
+    }
+    RET_IF(learned_init(c));
+    bool empty = false;
+    RET_IF(setup(c, &empty));
+    const Learned *L = c->lrn;
+    hipLaunchKernelGGL(k_box_matrix, dim3(4096), dim3(256), 0, c->stream, c->d, L->nx, L->ny, L->kb, L->nzr,
+                       c->ad, c->ali, c->aui, c->alj, c->auj, c->alk, c->auk, out);
+    CAD_CHECK(hipGetLastError());
+    CAD_CHECK(hipStreamSynchronize(c->stream));
     return CAD_OK;
 }
 

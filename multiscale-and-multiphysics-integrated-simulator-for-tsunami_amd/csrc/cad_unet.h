@@ -29,6 +29,8 @@ int cad_unet_set_depth(cad_unet *u, int nz);
 int cad_unet_set_input(cad_unet *u, const void *dev);
 int cad_unet_get_output(cad_unet *u, float *dev);
 int cad_unet_train_step(cad_unet *u, const float *target, const float *mask, float lr, float *loss);
+int cad_unet_train_step_dy(cad_unet *u, const float *dy, float lr);
+int cad_unet_gradients_dy(cad_unet *u, const float *dy, float *host_grad);
 int cad_unet_gradients(cad_unet *u, const float *target, const float *mask, float *host_grad,
                        float *loss);
 }

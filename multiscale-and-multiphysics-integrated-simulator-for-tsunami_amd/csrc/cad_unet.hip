@@ -1042,6 +1042,43 @@ int cad_unet_train_step(cad_unet *u, const float *target, const float *mask, flo
     return CAD_OK;
 }
 
+// One Adam step from the gradient dy of a loss with respect to the output
+// of the last forward pass (computed by the caller, FP32 in the output's
+// order): backward through that pass, Adam, repack.
+int cad_unet_train_step_dy(cad_unet *u, const float *dy, float lr)
+{
+    if (!u || !u->train || !dy) {
+        cad_set_error("cad_unet_train_step_dy: not a training network or null argument");
+        return CAD_ERR_ARG;
+    }
+    RET_IF(ew(u, k_zero_f, u->nparam, u->nparam, u->gr));
+    CAD_CHECK(hipMemcpyAsync(u->dy, dy, nvox(u->g[0]) * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
+    RET_IF(backward(u));
+    u->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)u->step), c2 = 1.f - std::pow(b2, (float)u->step);
+    RET_IF(ew(u, k_adam, u->nparam, u->nparam, u->p, u->gr, u->m1, u->m2, lr, b1, b2, c1, c2, 1e-8f));
+    RET_IF(pack_all(u));
+    CAD_CHECK(hipStreamSynchronize(u->stream));
+    return CAD_OK;
+}
+
+// The gradient of every parameter from the caller's gradient dy with respect
+// to the output of the last forward pass (no step).
+int cad_unet_gradients_dy(cad_unet *u, const float *dy, float *host_grad)
+{
+    if (!u || !u->train || !dy || !host_grad) {
+        cad_set_error("cad_unet_gradients_dy: not a training network or null argument");
+        return CAD_ERR_ARG;
+    }
+    RET_IF(ew(u, k_zero_f, u->nparam, u->nparam, u->gr));
+    CAD_CHECK(hipMemcpyAsync(u->dy, dy, nvox(u->g[0]) * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
+    RET_IF(backward(u));
+    CAD_CHECK(hipMemcpyAsync(host_grad, u->gr, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->stream));
+    CAD_CHECK(hipStreamSynchronize(u->stream));
+    return CAD_OK;
+}
+
 int cad_unet_gradients(cad_unet *u, const float *target, const float *mask, float *host_grad, float *loss_out)
 {
     if (!u || !u->train || !target || !mask || !host_grad) {

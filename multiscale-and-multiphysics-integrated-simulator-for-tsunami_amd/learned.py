@@ -8,14 +8,22 @@ e_k - V(r_k)).  The network sees the residual scaled by its RMS over the
 active cells, GGV, F and the cell type; its target is the correction on the
 same scale.
 
+Two losses train it.  "error": the relative squared error of the correction
+over the active voxels of a crop's interior.  "residual": the relative
+squared residual that the correction leaves, sum m (rho - A y)^2 / sum m rho^2
+with rho = A e (the pair's residual on the same scale) and A the Poisson
+matrix on the crop (cad_learned_box_matrix), y taken as 0 on inactive voxels
+as the preconditioner does.  The second asks the network for what a local
+operator can give, the part of A^-1 that a crop sees, where the first also
+asks for the smooth error spread over the tank.
+
 Workload: the 512x256x128 buildings tank (configs.buildings_tank) with seeded
 layouts -- seeds 1..8 for training, 9..12 held out.  Each layout's flow is
 advanced with the mixed form and sampled every few steps; each sample gives
 one pair per k, and each pair several crops of 64x32x32 voxels placed where
-the water is.  The loss is the relative squared error of the correction over
-the active voxels of a crop's interior (a margin of 4 voxels at the crop's
-faces is left out: there the network sees zero padding where the full box has
-data).
+the water is.  Both losses are taken over the active voxels of a crop's
+interior (a margin of 4 voxels at the crop's faces is left out: there the
+network sees zero padding where the full box has data).
 
 Device memory comes from torch; the network's kernels are the library's.
 """
@@ -32,6 +40,7 @@ from . import unet as U
 CROP = (64, 32, 32)            # x, y, z voxels
 MARGIN = 4
 ITERS = (0, 1, 2, 4, 8, 16)    # plain-solver iterates sampled per system
+LOSSES = ("error", "residual")
 WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
 DEFAULT_WEIGHTS = WEIGHTS_DIR / "unet_buildings_512x256x128.npy"
 
@@ -46,6 +55,7 @@ def _bind():
     L.cad_learned_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.cad_learned_box.argtypes = [vp, C.POINTER(C.c_int)]
     L.cad_learned_sample.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, vp, vp, vp]
+    L.cad_learned_box_matrix.argtypes = [vp, vp]
     L._learned_bound = True
     return L
 
@@ -98,6 +108,38 @@ def sample(ctx, mode=1, iters=ITERS):
     return feat, tgt, msk, (nx, ny, nz, kb)
 
 
+def box_matrix(ctx):
+    """The context's current Poisson matrix on the network's box: torch FP32
+    CUDA tensor (7, nz, ny, nx) of the rows AD, ALI, AUI, ALJ, AUJ, ALK, AUK."""
+    import torch
+    nx, ny, nz, _ = box(ctx)
+    a = torch.empty((7, nz, ny, nx), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    capi._check(_bind().cad_learned_box_matrix(ctx._h, C.c_void_p(a.data_ptr())))
+    return a
+
+
+def apply_matrix(a, v):
+    """A v on crops: a (b, 7, z, y, x) the rows, v (b, z, y, x); values
+    beyond a crop are taken as 0 (only its faces' voxels see them)."""
+    import torch.nn.functional as Fn
+    p = Fn.pad(v, (1, 1, 1, 1, 1, 1))
+    c = p[:, 1:-1, 1:-1, 1:-1]
+    return (a[:, 0] * c + a[:, 1] * p[:, 1:-1, 1:-1, :-2] + a[:, 2] * p[:, 1:-1, 1:-1, 2:]
+            + a[:, 3] * p[:, 1:-1, :-2, 1:-1] + a[:, 4] * p[:, 1:-1, 2:, 1:-1]
+            + a[:, 5] * p[:, :-2, 1:-1, 1:-1] + a[:, 6] * p[:, 2:, 1:-1, 1:-1])
+
+
+def residual_loss(a, act, e, m, y):
+    """Mean over crops of sum m (rho - A (act y))^2 / sum m rho^2, rho = A e."""
+    import torch
+    with torch.no_grad():
+        rho = apply_matrix(a, e)
+        den = (m * rho * rho).sum(dim=(1, 2, 3)).clamp_min(1e-30)
+    res = m * (rho - apply_matrix(a, act * y))
+    return ((res * res).sum(dim=(1, 2, 3)) / den).mean()
+
+
 def _crop_origins(msk, count, rng, min_active=0.25):
     """Up to count crop origins (x0, y0, z0) whose interior holds at least
     min_active active voxels (fraction), drawn at random."""
@@ -120,20 +162,25 @@ def _crop_origins(msk, count, rng, min_active=0.25):
 
 
 class PairSet:
-    """Crops of pairs, resident on the GPU."""
+    """Crops of pairs, resident on the GPU; with the residual loss also the
+    matrix rows (7 FP32 per voxel) and the activity of each crop."""
 
-    def __init__(self, capacity):
+    def __init__(self, capacity, loss="error"):
         import torch
         cx, cy, cz = CROP
+        self.loss = loss
         self.x = torch.zeros((capacity, cz + 2, cy + 2, cx + 2, 8), dtype=torch.bfloat16, device="cuda")
         self.t = torch.zeros((capacity, cz, cy, cx), dtype=torch.float32, device="cuda")
         self.m = torch.zeros((capacity, cz, cy, cx), dtype=torch.float32, device="cuda")
+        if loss == "residual":
+            self.a = torch.zeros((capacity, 7, cz, cy, cx), dtype=torch.float32, device="cuda")
+            self.act = torch.zeros((capacity, cz, cy, cx), dtype=torch.float32, device="cuda")
         self.n = 0
         inner = torch.zeros((cz, cy, cx), dtype=torch.float32, device="cuda")
         inner[MARGIN:cz - MARGIN, MARGIN:cy - MARGIN, MARGIN:cx - MARGIN] = 1.0
         self.inner = inner
 
-    def add(self, feat, tgt, msk, crops_per_pair, rng):
+    def add(self, feat, tgt, msk, crops_per_pair, rng, a=None):
         cx, cy, cz = CROP
         for q in range(feat.shape[0]):
             for (x0, y0, z0) in _crop_origins(msk[q], crops_per_pair, rng):
@@ -144,6 +191,9 @@ class PairSet:
                                                       1 + x0:1 + x0 + cx, :]
                 self.t[i] = tgt[q, z0:z0 + cz, y0:y0 + cy, x0:x0 + cx]
                 self.m[i] = msk[q, z0:z0 + cz, y0:y0 + cy, x0:x0 + cx] * self.inner
+                if self.loss == "residual":
+                    self.a[i] = a[:, z0:z0 + cz, y0:y0 + cy, x0:x0 + cx]
+                    self.act[i] = msk[q, z0:z0 + cz, y0:y0 + cy, x0:x0 + cx]
                 self.n += 1
 
 
@@ -166,11 +216,11 @@ def _context(case, precision):
 
 
 def generate(seeds, samples=10, every=10, crops_per_pair=40, capacity=20000, size=(512, 256, 128),
-             nbuild=40, mode=1, log=print, seed=0):
+             nbuild=40, mode=1, log=print, seed=0, loss="error"):
     """The crop set from the layouts of the given seeds: per layout `samples`
     systems, `every` mixed-form steps apart."""
     rng = np.random.default_rng(seed)
-    ps = PairSet(capacity)
+    ps = PairSet(capacity, loss)
     for sd in seeds:
         t0 = time.time()
         case = configs.buildings_tank(*size, nbuild=nbuild, seed=sd, eps_r=1.0e-8)
@@ -179,15 +229,17 @@ def generate(seeds, samples=10, every=10, crops_per_pair=40, capacity=20000, siz
                 for _ in range(every):
                     ctx.step()
                 feat, tgt, msk, bx = sample(ctx, mode)
-                ps.add(feat, tgt, msk, crops_per_pair, rng)
-                del feat, tgt, msk
+                a = box_matrix(ctx) if loss == "residual" else None
+                ps.add(feat, tgt, msk, crops_per_pair, rng, a)
+                del feat, tgt, msk, a
         log(f"learned: layout seed {sd}: {ps.n} crops so far ({time.time() - t0:.1f} s, box {bx})")
     return ps
 
 
 def train(ps, steps=3000, batch=8, lr=1.0e-3, lr_min=1.0e-5, seed=0, p0=None, log=print):
-    """Adam on random batches of the crop set; cosine learning-rate decay.
-    Returns the parameters and the loss history (every 50 steps)."""
+    """Adam on random batches of the crop set with the set's loss; cosine
+    learning-rate decay.  Returns the parameters and the loss history (every
+    50 steps)."""
     import torch
     rng = np.random.default_rng(seed)
     cx, cy, cz = CROP
@@ -198,6 +250,7 @@ def train(ps, steps=3000, batch=8, lr=1.0e-3, lr_min=1.0e-5, seed=0, p0=None, lo
         xb = torch.empty((batch, cz + 2, cy + 2, cx + 2, 8), dtype=torch.bfloat16, device="cuda")
         tb = torch.empty((batch, cz, cy, cx), dtype=torch.float32, device="cuda")
         mb = torch.empty((batch, cz, cy, cx), dtype=torch.float32, device="cuda")
+        yb = torch.empty((batch, cz, cy, cx), dtype=torch.float32, device="cuda")
         t0 = time.time()
         run = []
         for s in range(steps):
@@ -208,7 +261,16 @@ def train(ps, steps=3000, batch=8, lr=1.0e-3, lr_min=1.0e-5, seed=0, p0=None, lo
             torch.cuda.synchronize()
             net.set_input(xb)
             a = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * s / steps))
-            run.append(net.train_step(tb, mb, a))
+            if ps.loss == "residual":
+                net.forward(yb)
+                y = yb.clone().requires_grad_(True)
+                loss = residual_loss(ps.a[idx], ps.act[idx], tb, mb, y)
+                loss.backward()
+                torch.cuda.synchronize()
+                net.train_step_dy(y.grad.contiguous(), a)
+                run.append(float(loss))
+            else:
+                run.append(net.train_step(tb, mb, a))
             if (s + 1) % 50 == 0:
                 hist.append(float(np.mean(run)))
                 run = []
@@ -219,16 +281,23 @@ def train(ps, steps=3000, batch=8, lr=1.0e-3, lr_min=1.0e-5, seed=0, p0=None, lo
 
 
 def evaluate(ps, p, batch=8):
-    """Mean loss of the network over a crop set (no training)."""
+    """Mean loss (the set's) of the network over a crop set (no training)."""
     import torch
     cx, cy, cz = CROP
     losses = []
     with U.UNet(cx, cy, cz, batch, train=True) as net:
         net.set_params(p)
+        yb = torch.empty((batch, cz, cy, cx), dtype=torch.float32, device="cuda")
         for i in range(0, ps.n - batch + 1, batch):
             net.set_input(ps.x[i:i + batch].contiguous())
-            loss, _ = net.gradients(ps.t[i:i + batch].contiguous(), ps.m[i:i + batch].contiguous())
-            losses.append(loss)
+            if ps.loss == "residual":
+                net.forward(yb)
+                with torch.no_grad():
+                    losses.append(float(residual_loss(ps.a[i:i + batch], ps.act[i:i + batch],
+                                                      ps.t[i:i + batch], ps.m[i:i + batch], yb)))
+            else:
+                loss, _ = net.gradients(ps.t[i:i + batch].contiguous(), ps.m[i:i + batch].contiguous())
+                losses.append(loss)
     return float(np.mean(losses)) if losses else float("nan")
 
 

@@ -41,6 +41,8 @@ def _bind():
     L.cad_unet_forward.argtypes = [vp]
     L.cad_unet_train_step.argtypes = [vp, vp, vp, C.c_float, C.POINTER(C.c_float)]
     L.cad_unet_gradients.argtypes = [vp, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.cad_unet_train_step_dy.argtypes = [vp, vp, C.c_float]
+    L.cad_unet_gradients_dy.argtypes = [vp, vp, C.POINTER(C.c_float)]
     L._unet_bound = True
     return L
 
@@ -136,6 +138,19 @@ class UNet:
                                                 C.c_void_p(mask.data_ptr()), float(lr),
                                                 C.byref(loss)))
         return loss.value
+
+    def train_step_dy(self, dy, lr):
+        """One Adam step from dy, the gradient (FP32 CUDA tensor in the
+        output's order) of a loss with respect to the last forward's output."""
+        assert dy.is_contiguous() and dy.numel() == self.nb * self.box[0] * self.box[1] * self.box[2]
+        capi._check(_bind().cad_unet_train_step_dy(self._h, C.c_void_p(dy.data_ptr()), float(lr)))
+
+    def gradients_dy(self, dy):
+        """Every parameter's gradient from dy (as for train_step_dy), no step."""
+        assert dy.is_contiguous() and dy.numel() == self.nb * self.box[0] * self.box[1] * self.box[2]
+        g = np.zeros(param_count(), np.float32)
+        capi._check(_bind().cad_unet_gradients_dy(self._h, C.c_void_p(dy.data_ptr()), _fp(g)))
+        return g
 
     def gradients(self, target, mask):
         """The loss and its gradient with respect to every parameter."""

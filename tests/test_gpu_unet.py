@@ -74,6 +74,44 @@ def test_unet_forward_and_gradients_match_torch(pkg, box, nb):
             assert rel <= 1e-1 and cos >= 0.99, (name, part, rel, cos)
 
 
+def test_unet_gradients_from_a_given_output_gradient_match_torch(pkg):
+    """The backward pass from a caller's output gradient (the residual loss's
+    path, cad_unet_gradients_dy): with dy random, every layer's gradient
+    against the reference's gradient of sum(dy * y).  This checks the path of
+    dy into the backward pass: a dy read in the wrong order or from the wrong
+    buffer gives gradients unrelated to the reference's (cosine near 0).  An
+    unmasked random dy sums to small gradients through much cancellation,
+    which BF16 activations widen, so the bar is relative error 2e-1 and
+    cosine 0.98 (measured worst: b3b's bias, 0.146 and 0.9896)."""
+    import torch
+    U = pkg.unet
+    box, nb = (64, 8, 8), 2
+    nx, ny, nz = box
+    p, pr, xf, xp, e, m = _setup(pkg, box, nb, 5)
+    dy = np.random.default_rng(9).standard_normal((nb, nz, ny, nx)).astype(np.float32)
+    with U.UNet(nx, ny, nz, nb, train=True) as net:
+        net.set_params(p)
+        net.set_input(xp.cuda())
+        y = torch.empty(nb * nz * ny * nx, dtype=torch.float32, device="cuda")
+        net.forward(y)
+        g = net.gradients_dy(torch.from_numpy(dy.ravel()).cuda())
+    prm = unet_ref.forward(U, pr, xf)
+    for v in prm.values():
+        v[0].requires_grad_(True)
+        v[1].requires_grad_(True)
+    yr = unet_ref.run(prm, xf)
+    (yr * torch.from_numpy(dy)).sum().backward()
+    gr = unet_ref.grad_vector(U, prm, U.param_count())
+    sl = U.layer_slices()
+    for name in U.NAMES + ("out",):
+        for part in (0, 1):
+            a, b = g[sl[name][part]], gr[sl[name][part]]
+            rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+            cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+            print(f"{name} {'wb'[part]}: rel {rel:.3e} cos {cos:.5f}", flush=True)
+            assert rel <= 2e-1 and cos >= 0.98, (name, part, rel, cos)
+
+
 def test_unet_adam_steps_lower_the_loss(pkg):
     import torch
     U = pkg.unet
