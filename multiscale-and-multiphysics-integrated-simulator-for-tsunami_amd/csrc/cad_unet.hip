@@ -14,19 +14,33 @@
 // skip connection's encoder output in the upper half, the upsampled coarse
 // features in the lower half), so the concatenation costs no copy.
 //
-// Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16:
-//   D (Cout x voxels) = W (Cout x K) * X^T (K x voxels),  K = 27*Cin taps x channels.
+// Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16, marched along z.
+// The z taps are folded into the output rows: for each dz in {-1, 0, +1}
+//   P_dz (Cout x voxels of plane p) = W_dz (Cout x K) * X_p^T (K x voxels),  K = 9*Cin (dx, dy, channel),
+// and input plane p adds P_dz to out(p - dz).
+// A block (4 waves, one y row each, NT groups of 16 voxels along x) walks a
+// run of z planes; its three FP32 accumulator sets hold out(p+1), out(p) and
+// out(p-1) while plane p is read, and out(p-1), then complete, is stored.
+// Every input fragment a wave loads feeds the three dz weight tiles, so an
+// input plane is read once per block, and the x and y taps are per-lane load
+// offsets within the plane, which the block's four rows share in L1.
 // A lane holds A[m = l%16][k = 8*(l/16)..+7] (8 weights, packed so that it is
-// one 16-byte load), B[k = 8*(l/16)..+7][n = l%16] (8 channels of one tap of
-// voxel n: one 16-byte load) and D[m = 4*(l/16)+i][n = l%16] (4 consecutive
-// output channels of one voxel: one 8-byte store).  A wave computes MT*16
-// output channels of NT*16 voxels along x.  The backward pass for the
-// activations is the same kernel with the taps flipped and the weight
-// matrix transposed (packed that way), and the weight gradients are an FP32
-// reduction over voxels.
+// one 16-byte load, staged in LDS), B[k = 8*(l/16)..+7][n = l%16] (8 channels
+// of one (dx,dy) tap of voxel n: one 16-byte load) and D[m = 4*(l/16)+i][n =
+// l%16] (4 consecutive output channels of one voxel: one 8-byte store).
+// Blocks of neighbouring rows are dealt to the same XCD, so the rows they
+// share stay in one L2.  The backward pass for the activations is the same
+// kernel with the taps flipped and the weight matrix transposed (packed that
+// way), and the weight gradients are an FP32 reduction over voxels.
+//
+// A convolution's input may come from two buffers (the decoder's skip
+// concatenation: channels below `split` from the upsampled coarse features,
+// the rest from the encoder's output), and its output may go to two, so
+// every buffer holds whole voxel records and no kernel writes part of one.
 #include "cad_common.h"
 #include "cad_unet.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -62,92 +76,197 @@ __device__ __forceinline__ long long tap_off(const UGeo &g, int t)
     return dx + (long long)g.px * (dy + (long long)g.py * dz);
 }
 
-// 3x3x3 convolution, CIN -> 16*MT channels, NT*16 voxels along x per wave.
-// wp: packed A fragments [MT][KC][64 lanes] of 8 BF16.  bias may be null.
-template <int CIN, int MT, int NT, bool RELU>
-__global__ void __launch_bounds__(256) k_conv3(UGeo g, const bf16 *__restrict__ in, int cs_in,
-                                               const bf16x8 *__restrict__ wp,
-                                               const float *__restrict__ bias,
-                                               bf16 *__restrict__ out, int cs_out)
+// K steps of 32 of the folded convolution for cin input channels (9 taps)
+__host__ __device__ constexpr int fold_kc(int cin) { return (9 * cin + 31) / 32; }
+// bytes of A fragments a launch of MT channel tiles stages in LDS, and the cap
+constexpr int conv_a_bytes(int cin, int mt) { return 3 * mt * fold_kc(cin) * 64 * 16; }
+// blocks stage at most this much in LDS; weights above CONV_A_WIDE take
+// blocks of 16 waves (one per CU) so that more rows share one staged copy
+constexpr int CONV_A_LDS_MAX = 112 * 1024;
+constexpr int CONV_A_WIDE = 40 * 1024;
+__host__ __device__ constexpr int conv_wx(int cin, int mt) { return conv_a_bytes(cin, mt) > CONV_A_WIDE ? 4 : 1; }
+
+// input channels [0, split) at p, [split, C) at p + delta, both with records
+// of cs channels (delta: the second half of a concatenation, allocated right
+// after the first so that one base and 32-bit lane offsets reach both)
+struct ConvIn {
+    const bf16 *p;
+    long long delta;
+    int cs, split;
+};
+struct ConvOut {
+    bf16 *p0, *p1;
+    int cs0, cs1, split;
+};
+__device__ __forceinline__ const bf16 *chan(const ConvIn &t, long long v, int c)
 {
-    constexpr int KC = (27 * CIN + 31) / 32;
-    const int lane = threadIdx.x & 63;
-    long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int ntx = g.nx / (16 * NT);
-    const int xt = (int)(w % ntx);
-    w /= ntx;
-    const int y = (int)(w % g.ny);
-    w /= g.ny;
-    const int z = (int)(w % g.nz);
-    const int b = (int)(w / g.nz);
-    if (b >= g.nb) return;
+    return c < t.split ? t.p + v * t.cs + c : t.p + t.delta + v * t.cs + (c - t.split);
+}
+__device__ __forceinline__ bf16 *chan(const ConvOut &t, long long v, int c)
+{
+    return c < t.split ? t.p0 + v * t.cs0 + c : t.p1 + v * t.cs1 + (c - t.split);
+}
+
+// 3x3x3 convolution, CIN -> 16*MT channels: output channel tiles m0..m0+MT-1
+// (wp points at tile m0).  A block is 4 rows (y) by WX wave columns (x) of
+// NT groups of 16 voxels, WX = conv_wx(CIN, MT).  Tiles (x, y, z run of zlen
+// planes, box) are numbered x fastest; XCD k (blocks b with b % 8 == k) takes
+// tiles [k per_xcd, (k+1) per_xcd).  Groups past nx and rows past ny are
+// idle.  wp: packed A fragments [MT][3 dz][KC][64 lanes] of 8 BF16.  bias may
+// be null.
+template <int CIN, int MT, int NT, bool RELU>
+__global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 ? 3 : 4)
+    k_conv3(UGeo g, ConvIn in, const bf16x8 *__restrict__ wp, const float *__restrict__ bias, ConvOut out, int m0,
+            int ntx, int nty, int nzc, int zlen, int per_xcd, int ntiles)
+{
+    constexpr int KC = fold_kc(CIN);
+    constexpr int WX = conv_wx(CIN, MT);
+    constexpr int NA = 3 * MT * KC * 64;
+    static_assert(conv_a_bytes(CIN, MT) <= CONV_A_LDS_MAX, "weights must fit in LDS");
+    __shared__ bf16x8 as[NA];
+    const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= ntiles) return;   // the whole block
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, zc = (tile / (ntx * nty)) % nzc;
+    const int b = tile / (ntx * nty * nzc);
+    const int z0 = zc * zlen, z1 = min(g.nz, z0 + zlen);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n = lane & 15, kg = lane >> 4;
-    const long long v0 = vox(g, xt * 16 * NT + n, y, z, b);
-    f4 acc[MT][NT];
+    const int y = 4 * ty + (wv & 3);
+    const int gi0 = (tx * WX + (wv >> 2)) * NT, ng = g.nx >> 4;
+    const bool yok = y < g.ny && gi0 < ng;
+    const int x0 = 16 * gi0 + n;
+    const int np = z1 - z0 + 2;   // input planes z0-1 .. z1
+    const int nstep = np * KC;
+    // Byte offsets of the K steps' taps for each lane, from the wave's base
+    // voxel (x = 16 gi0 - 1, y - 1, plane z0 - 1): the same for every plane.
+    // The loads then take a wave-uniform base and a 32-bit lane offset.
+    __shared__ unsigned boff[KC][64];
+    for (int e = threadIdx.x; e < KC * 64; e += blockDim.x) {
+        const int q = e >> 6, l = e & 63;
+        const int k0 = 32 * q + 8 * (l >> 4);
+        int t = k0 / CIN;
+        const int ci0 = k0 % CIN;
+        if (t > 8) t = 8;   // K padding: the weights there are zero
+        const int rel = (l & 15) + t % 3 + g.px * (t / 3);
+        const bool hi = ci0 >= in.split;
+        boff[q][l] = (unsigned)(2LL * ((long long)rel * in.cs + (hi ? in.delta + ci0 - in.split : ci0)));
+    }
+    // Loads go through a buffer resource at the wave's base voxel: a lane
+    // offset from the table, a wave-uniform plane and group offset.
+    const long long vb = vox(g, 16 * gi0 - 1, y - 1, z0 - 1, b);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(in.p + vb * in.cs), (short)0, -1, 0x00020000);
+    const int plane_bytes = 2 * g.px * g.py * in.cs;
+    // groups past nx read the wave's first group instead; their sums are not stored
+    int gofs[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) gofs[j] = (gi0 + j < ng ? j : 0) * 32 * in.cs;
+    auto load_b = [&](int st, bf16x8 *dst) {
+        const int pi = st / KC, q = st - pi * KC;
+        const int voff = (int)boff[q][lane];
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            dst[j] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                                    rs, voff, pi * plane_bytes + gofs[j], 0));
+    };
+    for (int e = threadIdx.x; e < NA; e += blockDim.x) as[e] = wp[e];
+    __syncthreads();
+    if (!yok) return;   // past ny or nx: no loads, no stores, no barrier after this
+    constexpr int PD = 2;   // steps in flight ahead of the one computed, in a ring of 3
+    bf16x8 ring[3][NT];
+#pragma unroll
+    for (int p = 0; p < PD; p++)
+        if (p < nstep) load_b(p, ring[p]);
+    const f4 zero = {0.f, 0.f, 0.f, 0.f};
+    f4 acc[3][MT][NT];   // slot (o - z0 + 2) % 3 holds out(o)
+#pragma unroll
+    for (int sl = 0; sl < 3; sl++)
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) acc[sl][m][j] = zero;
+    float bs[MT][4];
 #pragma unroll
     for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int j = 0; j < NT; j++) acc[m][j] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-    for (int q = 0; q < KC; q++) {
-        const int k0 = 32 * q + 8 * kg;
-        int t = k0 / CIN;
-        const int ci0 = k0 % CIN;
-        if (t > 26) t = 26;   // K padding: the weights there are zero
-        const long long off = tap_off(g, t);
-        bf16x8 bf[NT];
+        for (int i = 0; i < 4; i++) bs[m][i] = bias ? bias[16 * (m0 + m) + 4 * kg + i] : 0.f;
+#pragma unroll 1
+    for (int p0 = 0; p0 < np; p0 += 3) {
 #pragma unroll
-        for (int j = 0; j < NT; j++)
-            bf[j] = *reinterpret_cast<const bf16x8 *>(in + (v0 + 16 * j + off) * cs_in + ci0);
+        for (int u = 0; u < 3; u++) {
+            const int pi = p0 + u;   // input plane z0 - 1 + pi
+            if (pi >= np) break;
 #pragma unroll
-        for (int m = 0; m < MT; m++) {
-            const bf16x8 af = wp[((long long)m * KC + q) * 64 + lane];
+            for (int q = 0; q < KC; q++) {
+                const int st = pi * KC + q;
+                if (st + PD < nstep) load_b(st + PD, ring[(u * KC + q + PD) % 3]);
+                const bf16x8 *cur = ring[(u * KC + q) % 3];
 #pragma unroll
-            for (int j = 0; j < NT; j++)
-                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[j], acc[m][j], 0, 0, 0);
-        }
-    }
+                for (int m = 0; m < MT; m++)
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
-        const int co = 16 * m + 4 * kg;
-        float bs[4] = {0.f, 0.f, 0.f, 0.f};
-        if (bias)
+                    for (int d = 0; d < 3; d++) {
+                        // dz = d - 1 adds to out(p - dz), slot (u + 2 - d) % 3
+                        const int fa = ((3 * m + d) * KC + q) * 64 + lane;
+                        const bf16x8 af = as[fa];
 #pragma unroll
-            for (int i = 0; i < 4; i++) bs[i] = bias[co + i];
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-            bf16x4 o;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                float r = acc[m][j][i] + bs[i];
-                if (RELU) r = r > 0.f ? r : 0.f;
-                o[i] = (bf16)r;
+                        for (int j = 0; j < NT; j++)
+                            acc[(u + 2 - d) % 3][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                af, cur[j], acc[(u + 2 - d) % 3][m][j], 0, 0, 0);
+                    }
             }
-            *reinterpret_cast<bf16x4 *>(out + (v0 + 16 * j) * cs_out + co) = o;
+            // out(p - 1) is complete: slot u % 3
+            const int o = z0 - 2 + pi;
+            if (o >= z0 && o < z1) {
+                const long long v0 = vox(g, x0, y, o, b);
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    const int co = 16 * (m0 + m) + 4 * kg;
+#pragma unroll
+                    for (int j = 0; j < NT; j++) {
+                        if (gi0 + j >= ng) continue;
+                        bf16x4 ov;
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            float val = acc[u % 3][m][j][c] + bs[m][c];
+                            if (RELU) val = val > 0.f ? val : 0.f;
+                            ov[c] = (bf16)val;
+                        }
+                        *reinterpret_cast<bf16x4 *>(chan(out, v0 + 16 * j, co)) = ov;
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+#pragma unroll
+                for (int j = 0; j < NT; j++) acc[u % 3][m][j] = zero;
         }
     }
 }
 
-// Packs FP32 weights W[t][ci][co] (27 x cin x cout) into the A fragments of
-// k_conv3 for cin -> cout; flip: the backward-data operator, whose input has
-// cout channels and output cin, W'[t][co][ci] = W[26-t][ci][co].
+// Packs FP32 weights W[t][ci][co] (27 x cin x cout, t = dx+1 + 3(dy+1) + 9(dz+1))
+// into the A fragments of k_conv3 for cin -> cout: tile 3m + dzi holds output
+// channels 16m..16m+15 of the z tap dzi, its K runs over (dx, dy) taps and
+// channels.  flip: the backward-data operator, whose input has cout channels
+// and output cin, W'[t][co][ci] = W[26-t][ci][co].
 __global__ void k_pack(const float *__restrict__ w, int cin, int cout, int flip, bf16 *__restrict__ wp)
 {
     const int ki = flip ? cout : cin;   // input channels of the packed operator
     const int ko = flip ? cin : cout;   // its output channels
-    const int kc = (27 * ki + 31) / 32, mt = ko / 16;
-    const long long total = (long long)mt * kc * 64 * 8;
+    const int kc = fold_kc(ki), mt = ko / 16;
+    const long long total = 3LL * mt * kc * 64 * 8;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
         const int el = (int)(e % 8);
         const int lane = (int)((e / 8) % 64);
         const int q = (int)((e / 512) % kc);
-        const int m = (int)(e / (512LL * kc));
+        const int tile = (int)(e / (512LL * kc));
+        const int m = tile / 3, dzi = tile % 3;
         const int k = 32 * q + 8 * (lane >> 4) + el;
         const int o = 16 * m + (lane & 15);
         float v = 0.f;
-        const int t = k / ki, c = k % ki;
-        if (t < 27) {
+        const int tp = k / ki, c = k % ki;
+        if (tp < 9) {
+            const int t = tp + 9 * dzi;
             if (!flip)
                 v = w[((long long)t * cin + c) * cout + o];
             else
@@ -162,7 +281,7 @@ __global__ void k_pack(const float *__restrict__ w, int cin, int cout, int flip,
 // interior voxels; threads own 4x4 (ci, co) blocks, several thread groups
 // split each tile of voxels; partial sums are added atomically.
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(256) k_conv_dw(UGeo g, const bf16 *__restrict__ x, int cs_x,
+__global__ void __launch_bounds__(256) k_conv_dw(UGeo g, ConvIn x,
                                                  const bf16 *__restrict__ dy, int cs_dy,
                                                  long long per_block, float *__restrict__ dw,
                                                  float *__restrict__ db)
@@ -200,7 +319,7 @@ __global__ void __launch_bounds__(256) k_conv_dw(UGeo g, const bf16 *__restrict_
             float val = 0.f;
             if (u < u1) {
                 const long long v = vox_of(g, u);
-                val = c < CIN ? (float)x[(v + off) * cs_x + c] : (float)dy[v * cs_dy + (c - CIN)];
+                val = c < CIN ? (float)*chan(x, v + off, c) : (float)dy[v * cs_dy + (c - CIN)];
             }
             if (c < CIN)
                 xs[vi][c] = val;
@@ -272,150 +391,118 @@ __global__ void __launch_bounds__(256) k_conv_dw(UGeo g, const bf16 *__restrict_
     }
 }
 
+// Element-wise kernels run over the x rows of a level: block b covers
+// elements [256 (b % segs), 256 (b % segs + 1)) of row b / segs, an element
+// being (x, channel group of 8) with the group fastest; row = y + ny (z + nz box).
+__device__ __forceinline__ bool row_elem(const UGeo &g, int segs, int cg, int &x, int &c0, int &y, int &z,
+                                         int &b)
+{
+    const int seg = (int)(blockIdx.x % (unsigned)segs), row = (int)(blockIdx.x / (unsigned)segs);
+    const int t = seg * 256 + (int)threadIdx.x;
+    if (t >= g.nx * cg) return false;
+    x = t / cg;
+    c0 = (t - x * cg) * 8;
+    y = row % g.ny;
+    z = (row / g.ny) % g.nz;
+    b = row / (g.ny * g.nz);
+    return true;
+}
+
 // 2x2x2 average pooling of C channels (fine level gf -> coarse level gc)
-__global__ void k_pool(UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
+__global__ void k_pool(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
                        bf16 *__restrict__ out, int cs_out)
 {
-    const int cg = C / 8;
-    const long long total = (long long)gc.nx * gc.ny * gc.nz * gc.nb * cg;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int c0 = (int)(e % cg) * 8;
-        long long u = e / cg;
-        const int x = (int)(u % gc.nx);
-        u /= gc.nx;
-        const int y = (int)(u % gc.ny);
-        u /= gc.ny;
-        const int z = (int)(u % gc.nz);
-        const int b = (int)(u / gc.nz);
-        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int x, c0, y, z, b;
+    if (!row_elem(gc, segs, C / 8, x, c0, y, z, b)) return;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int d = 0; d < 8; d++) {
-            const long long vf = vox(gf, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2), b);
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(in + vf * cs_in + c0);
+    for (int d = 0; d < 8; d++) {
+        const long long vf = vox(gf, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2), b);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(in + vf * cs_in + c0);
 #pragma unroll
-            for (int i = 0; i < 8; i++) s[i] += (float)a[i];
-        }
-        bf16x8 o;
-#pragma unroll
-        for (int i = 0; i < 8; i++) o[i] = (bf16)(0.125f * s[i]);
-        *reinterpret_cast<bf16x8 *>(out + vox(gc, x, y, z, b) * cs_out + c0) = o;
+        for (int i = 0; i < 8; i++) s[i] += (float)a[i];
     }
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = (bf16)(0.125f * s[i]);
+    *reinterpret_cast<bf16x8 *>(out + vox(gc, x, y, z, b) * cs_out + c0) = o;
 }
 
 // backward of k_pool: the fine gradient (accumulated into) += coarse / 8
-__global__ void k_pool_bwd(UGeo gc, UGeo gf, const bf16 *__restrict__ dc, int cs_c, int C,
+__global__ void k_pool_bwd(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ dc, int cs_c, int C,
                            bf16 *__restrict__ df, int cs_f)
 {
-    const int cg = C / 8;
-    const long long total = (long long)gf.nx * gf.ny * gf.nz * gf.nb * cg;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int c0 = (int)(e % cg) * 8;
-        long long u = e / cg;
-        const int x = (int)(u % gf.nx);
-        u /= gf.nx;
-        const int y = (int)(u % gf.ny);
-        u /= gf.ny;
-        const int z = (int)(u % gf.nz);
-        const int b = (int)(u / gf.nz);
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dc + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_c + c0);
-        bf16x8 *p = reinterpret_cast<bf16x8 *>(df + vox(gf, x, y, z, b) * cs_f + c0);
-        bf16x8 o = *p;
+    int x, c0, y, z, b;
+    if (!row_elem(gf, segs, C / 8, x, c0, y, z, b)) return;
+    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dc + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_c + c0);
+    bf16x8 *p = reinterpret_cast<bf16x8 *>(df + vox(gf, x, y, z, b) * cs_f + c0);
+    bf16x8 o = *p;
 #pragma unroll
-        for (int i = 0; i < 8; i++) o[i] = (bf16)((float)o[i] + 0.125f * (float)a[i]);
-        *p = o;
-    }
+    for (int i = 0; i < 8; i++) o[i] = (bf16)((float)o[i] + 0.125f * (float)a[i]);
+    *p = o;
 }
 
 // nearest-neighbour upsampling of C channels (coarse gc -> fine gf)
-__global__ void k_up(UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
+__global__ void k_up(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
                      bf16 *__restrict__ out, int cs_out)
 {
-    const int cg = C / 8;
-    const long long total = (long long)gf.nx * gf.ny * gf.nz * gf.nb * cg;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int c0 = (int)(e % cg) * 8;
-        long long u = e / cg;
-        const int x = (int)(u % gf.nx);
-        u /= gf.nx;
-        const int y = (int)(u % gf.ny);
-        u /= gf.ny;
-        const int z = (int)(u % gf.nz);
-        const int b = (int)(u / gf.nz);
-        *reinterpret_cast<bf16x8 *>(out + vox(gf, x, y, z, b) * cs_out + c0) =
-            *reinterpret_cast<const bf16x8 *>(in + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_in + c0);
-    }
+    int x, c0, y, z, b;
+    if (!row_elem(gf, segs, C / 8, x, c0, y, z, b)) return;
+    *reinterpret_cast<bf16x8 *>(out + vox(gf, x, y, z, b) * cs_out + c0) =
+        *reinterpret_cast<const bf16x8 *>(in + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_in + c0);
 }
 
 // backward of k_up: coarse gradient = sum of its 8 fine voxels' gradients
-__global__ void k_up_bwd(UGeo gc, UGeo gf, const bf16 *__restrict__ df, int cs_f, int C,
+__global__ void k_up_bwd(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ df, int cs_f, int C,
                          bf16 *__restrict__ dc, int cs_c)
 {
-    const int cg = C / 8;
-    const long long total = (long long)gc.nx * gc.ny * gc.nz * gc.nb * cg;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int c0 = (int)(e % cg) * 8;
-        long long u = e / cg;
-        const int x = (int)(u % gc.nx);
-        u /= gc.nx;
-        const int y = (int)(u % gc.ny);
-        u /= gc.ny;
-        const int z = (int)(u % gc.nz);
-        const int b = (int)(u / gc.nz);
-        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int x, c0, y, z, b;
+    if (!row_elem(gc, segs, C / 8, x, c0, y, z, b)) return;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int d = 0; d < 8; d++) {
-            const long long vf = vox(gf, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2), b);
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(df + vf * cs_f + c0);
+    for (int d = 0; d < 8; d++) {
+        const long long vf = vox(gf, 2 * x + (d & 1), 2 * y + ((d >> 1) & 1), 2 * z + (d >> 2), b);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(df + vf * cs_f + c0);
 #pragma unroll
-            for (int i = 0; i < 8; i++) s[i] += (float)a[i];
-        }
-        bf16x8 o;
-#pragma unroll
-        for (int i = 0; i < 8; i++) o[i] = (bf16)s[i];
-        *reinterpret_cast<bf16x8 *>(dc + vox(gc, x, y, z, b) * cs_c + c0) = o;
+        for (int i = 0; i < 8; i++) s[i] += (float)a[i];
     }
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = (bf16)s[i];
+    *reinterpret_cast<bf16x8 *>(dc + vox(gc, x, y, z, b) * cs_c + c0) = o;
 }
 
 // dA *= (A > 0) over C channels: the gradient through a ReLU output A
-__global__ void k_relu_bwd(UGeo g, const bf16 *__restrict__ a, int cs_a, int C, bf16 *__restrict__ d,
+__global__ void k_relu_bwd(int segs, UGeo g, const bf16 *__restrict__ a, int cs_a, int C, bf16 *__restrict__ d,
                            int cs_d)
 {
-    const int cg = C / 8;
-    const long long total = (long long)g.nx * g.ny * g.nz * g.nb * cg;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int c0 = (int)(e % cg) * 8;
-        const long long v = vox_of(g, e / cg);
-        const bf16x8 av = *reinterpret_cast<const bf16x8 *>(a + v * cs_a + c0);
-        bf16x8 *p = reinterpret_cast<bf16x8 *>(d + v * cs_d + c0);
-        bf16x8 o = *p;
+    int x, c0, y, z, b;
+    if (!row_elem(g, segs, C / 8, x, c0, y, z, b)) return;
+    const long long v = vox(g, x, y, z, b);
+    const bf16x8 av = *reinterpret_cast<const bf16x8 *>(a + v * cs_a + c0);
+    bf16x8 *p = reinterpret_cast<bf16x8 *>(d + v * cs_d + c0);
+    bf16x8 o = *p;
 #pragma unroll
-        for (int i = 0; i < 8; i++)
-            if (!((float)av[i] > 0.f)) o[i] = (bf16)0.f;
-        *p = o;
-    }
+    for (int i = 0; i < 8; i++)
+        if (!((float)av[i] > 0.f)) o[i] = (bf16)0.f;
+    *p = o;
 }
 
 // output layer: y[u] = b + sum_c w[c] h[v][c] (FP32, interior order u)
-__global__ void k_out(UGeo g, const bf16 *__restrict__ h, const float *__restrict__ w, float *__restrict__ y)
+__global__ void k_out(int segs, UGeo g, const bf16 *__restrict__ h, const float *__restrict__ w,
+                      float *__restrict__ y)
 {
-    const long long total = (long long)g.nx * g.ny * g.nz * g.nb;
-    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
-         u += (long long)gridDim.x * blockDim.x) {
-        const long long v = vox_of(g, u);
-        const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(h + v * 16);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(h + v * 16 + 8);
-        float s = w[16];
+    int x, c0, yq, z, b;
+    if (!row_elem(g, segs, 1, x, c0, yq, z, b)) return;
+    const long long v = vox(g, x, yq, z, b);
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(h + v * 16);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(h + v * 16 + 8);
+    float s = w[16];
 #pragma unroll
-        for (int i = 0; i < 8; i++) s += w[i] * (float)a0[i];
+    for (int i = 0; i < 8; i++) s += w[i] * (float)a0[i];
 #pragma unroll
-        for (int i = 0; i < 8; i++) s += w[8 + i] * (float)a1[i];
-        y[u] = s;
-    }
+    for (int i = 0; i < 8; i++) s += w[8 + i] * (float)a1[i];
+    y[x + (long long)g.nx * (yq + (long long)g.ny * (z + (long long)g.nz * b))] = s;
 }
 
 // backward of k_out: dh[v][c] = w[c] dy[u]; dw[c] += sum dy h[c]; db += sum dy
@@ -562,6 +649,7 @@ unsigned grid_for(long long n) { return (unsigned)std::min<long long>((n + 255) 
 
 struct cad_unet {
     int device;
+    int ncu;                        // compute units of the device
     hipStream_t stream;
     bool own_stream;
     UGeo g[3];
@@ -571,9 +659,12 @@ struct cad_unet {
     bf16 *wp[NL], *wt[NL];          // packed forward and backward-data weights
     long long wp_n[NL], wt_n[NL];
     // activations (channels-last BF16 with halos)
-    bf16 *x, *h1a, *cat1, *p1, *h2a, *cat2, *p2, *h3a, *h3b, *h3c, *h2d, *h2e, *h2f, *h1d, *h1e;
+    // (cat1a, cat1b) and (cat2a, cat2b): the decoders' concatenated inputs,
+    // upsampled coarse features and the encoder's skip output
+    bf16 *x, *h1a, *cat1a, *cat1b, *p1, *h2a, *cat2a, *cat2b, *p2, *h3a, *h3b, *h3c, *h2d, *h2e, *h2f, *h1d, *h1e;
     // gradients
-    bf16 *dh1e, *dh1d, *dcat1, *dh2f, *dh2e, *dh2d, *dcat2, *dh3c, *dh3b, *dh3a, *dp2, *dh2a, *dp1, *dh1a;
+    bf16 *dh1e, *dh1d, *dcat1a, *dcat1b, *dh2f, *dh2e, *dh2d, *dcat2a, *dcat2b, *dh3c, *dh3b, *dh3a, *dp2, *dh2a,
+        *dp1, *dh1a;
     float *y, *dy, *esq, *loss;
     bool train;
     long long step;
@@ -601,64 +692,118 @@ int ualloc(cad_unet *u, T **p, size_t n)
 
 long long padded(const UGeo &g) { return (long long)g.px * g.py * g.pz * g.nb; }
 
-template <int CIN>
-int conv_ci(cad_unet *u, const UGeo &g, const bf16 *in, int cs_in, const bf16 *wp, const float *bias,
-            bf16 *out, int cs_out, int cout, bool relu)
+ConvIn one_in(const bf16 *p, int cs, int c) { return ConvIn{p, 0, cs, c}; }
+// a concatenation: channels [0, split) at p, the rest at p + delta, records of cs
+ConvIn cat_in(const bf16 *p, long long delta, int cs, int split) { return ConvIn{p, delta, cs, split}; }
+ConvOut one_out(bf16 *p, int cs, int c) { return ConvOut{p, nullptr, cs, 0, c}; }
+
+// waves a launch aims at: four per SIMD
+int conv_target_waves(const cad_unet *u) { return 16 * u->ncu; }
+
+template <int CIN, int MT, int NT>
+int conv_launch(cad_unet *u, const UGeo &g, const ConvIn &in, const bf16 *wp, const float *bias,
+                const ConvOut &out, int m0, bool relu)
 {
-    const int nt = (g.nx % 64) == 0 ? 4 : 1;
-    const long long waves = (long long)(g.nx / (16 * nt)) * g.ny * g.nz * g.nb;
-    const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
-    const bf16x8 *w = reinterpret_cast<const bf16x8 *>(wp);
-#define CAD_CONV(MT, NT, R) \
-    hipLaunchKernelGGL((k_conv3<CIN, MT, NT, R>), grid, blk, 0, u->stream, g, in, cs_in, w, bias, out, cs_out)
-#define CAD_CONV_NT(MT, R) \
-    do {                   \
-        if (nt == 4)       \
-            CAD_CONV(MT, 4, R); \
-        else               \
-            CAD_CONV(MT, 1, R); \
-    } while (0)
-    if (relu) {
-        if (cout == 16) CAD_CONV_NT(1, true);
-        else if (cout == 32) CAD_CONV_NT(2, true);
-        else if (cout == 64) CAD_CONV_NT(4, true);
-    } else {
-        if (cout == 16) CAD_CONV_NT(1, false);
-        else if (cout == 32) CAD_CONV_NT(2, false);
-        else if (cout == 64) CAD_CONV_NT(4, false);
+    constexpr int WX = conv_wx(CIN, MT);
+    const int ng = g.nx / 16;
+    const int ntx = (ng + NT * WX - 1) / (NT * WX), nty = (g.ny + 3) / 4;
+    const long long plane_tiles = (long long)ntx * nty * g.nb;
+    // z runs: the fewest that give the target wave count (each run reads two
+    // planes beyond its own)
+    const long long per_plane = plane_tiles * 4 * WX;
+    const long long want = (conv_target_waves(u) + per_plane - 1) / per_plane;
+    const int nzc0 = (int)std::min<long long>(std::max<long long>(want, 1), g.nz);
+    const int zlen = (g.nz + nzc0 - 1) / nzc0;
+    const int nzc = (g.nz + zlen - 1) / zlen;
+    const long long ntiles = plane_tiles * nzc;
+    // the kernel's load offsets (bytes from a wave's base voxel: lane part,
+    // plane part and group part) add up in 32 bits
+    const long long plane = 2LL * g.px * g.py * in.cs;
+    if (ntiles > (1LL << 30) ||
+        2 * (in.delta + (long long)(3 * g.px + 64) * in.cs) + (zlen + 2) * plane >= (1LL << 32) ||
+        plane >= (1LL << 31)) {
+        cad_set_error("cad_unet: box too large for the convolution's tiling and 32-bit load offsets");
+        return CAD_ERR_ARG;
     }
-#undef CAD_CONV_NT
-#undef CAD_CONV
+    const int per_xcd = (int)((ntiles + 7) / 8);
+    const dim3 grid((unsigned)(8 * per_xcd)), blk(256 * WX);
+    const bf16x8 *w = reinterpret_cast<const bf16x8 *>(wp);
+    if (relu)
+        hipLaunchKernelGGL((k_conv3<CIN, MT, NT, true>), grid, blk, 0, u->stream, g, in, w, bias, out, m0, ntx,
+                           nty, nzc, zlen, per_xcd, (int)ntiles);
+    else
+        hipLaunchKernelGGL((k_conv3<CIN, MT, NT, false>), grid, blk, 0, u->stream, g, in, w, bias, out, m0, ntx,
+                           nty, nzc, zlen, per_xcd, (int)ntiles);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
 
-int conv(cad_unet *u, const UGeo &g, int cin, const bf16 *in, int cs_in, const bf16 *wp, const float *bias,
-         bf16 *out, int cs_out, int cout, bool relu)
+// One launch per share of mt output-channel tiles (mt = 2 where the output
+// has more than one tile, 1 otherwise), each with the widest x groups (nt)
+// that still give the target wave count at one plane per z run.
+template <int CIN>
+int conv_ci(cad_unet *u, const UGeo &g, const ConvIn &in, const bf16 *wp, const float *bias, const ConvOut &out,
+            int cout, bool relu)
+{
+    const int mtot = cout / 16;
+    if (mtot < 1 || cout % 16 || g.nx % 16) {
+        cad_set_error("cad_unet: no convolution tiling for %d voxels along x and %d channels", g.nx, cout);
+        return CAD_ERR_ARG;
+    }
+    const int mt = mtot % 2 == 0 ? 2 : 1;
+    const int ng = g.nx / 16;
+    auto waves = [&](int nt, int wx) {
+        return (long long)((ng + nt * wx - 1) / (nt * wx)) * wx * 4 * ((g.ny + 3) / 4) * g.nb * g.nz;
+    };
+    const int wx = mt == 2 ? conv_wx(CIN, 2) : conv_wx(CIN, 1);
+    int nt = mt == 2 || wx == 4 ? 2 : 4;   // 12 accumulator quads at most; 8 in blocks of 16 waves
+    while (nt > 1 && waves(nt, wx) < conv_target_waves(u)) nt /= 2;
+    const long long tile = 3LL * fold_kc(CIN) * 64 * 8;   // BF16 per channel tile
+    for (int m0 = 0; m0 < mtot; m0 += mt) {
+        const bf16 *w = wp + m0 * tile;
+        int rc;
+        if (mt == 2)
+            rc = nt == 2 ? conv_launch<CIN, 2, 2>(u, g, in, w, bias, out, m0, relu)
+                         : conv_launch<CIN, 2, 1>(u, g, in, w, bias, out, m0, relu);
+        else if constexpr (conv_wx(CIN, 1) == 1)
+            rc = nt == 4   ? conv_launch<CIN, 1, 4>(u, g, in, w, bias, out, m0, relu)
+                 : nt == 2 ? conv_launch<CIN, 1, 2>(u, g, in, w, bias, out, m0, relu)
+                           : conv_launch<CIN, 1, 1>(u, g, in, w, bias, out, m0, relu);
+        else
+            rc = nt == 2 ? conv_launch<CIN, 1, 2>(u, g, in, w, bias, out, m0, relu)
+                         : conv_launch<CIN, 1, 1>(u, g, in, w, bias, out, m0, relu);
+        if (rc) return rc;
+    }
+    return CAD_OK;
+}
+
+int conv(cad_unet *u, const UGeo &g, int cin, const ConvIn &in, const bf16 *wp, const float *bias,
+         const ConvOut &out, int cout, bool relu)
 {
     switch (cin) {
-    case 8: return conv_ci<8>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
-    case 16: return conv_ci<16>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
-    case 32: return conv_ci<32>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
-    case 64: return conv_ci<64>(u, g, in, cs_in, wp, bias, out, cs_out, cout, relu);
+    case 8: return conv_ci<8>(u, g, in, wp, bias, out, cout, relu);
+    case 16: return conv_ci<16>(u, g, in, wp, bias, out, cout, relu);
+    case 32: return conv_ci<32>(u, g, in, wp, bias, out, cout, relu);
+    case 64: return conv_ci<64>(u, g, in, wp, bias, out, cout, relu);
     }
     cad_set_error("cad_unet: no convolution for %d input channels", cin);
     return CAD_ERR_ARG;
 }
 
-int layer_fwd(cad_unet *u, int l, const bf16 *in, int cs_in, bf16 *out, int cs_out)
+int layer_fwd(cad_unet *u, int l, const ConvIn &in, bf16 *out, int cs_out)
 {
     const LayerDef &L = LAYERS[l];
-    return conv(u, u->g[L.level], L.cin, in, cs_in, u->wp[l], u->p + u->bofs[l], out, cs_out, L.cout, true);
+    return conv(u, u->g[L.level], L.cin, in, u->wp[l], u->p + u->bofs[l], one_out(out, cs_out, L.cout), L.cout,
+                true);
 }
 
 // gradient of the layer's input (from dY at its output) and of its weights
-int layer_bwd(cad_unet *u, int l, const bf16 *in, int cs_in, const bf16 *dy, int cs_dy, bf16 *dx, int cs_dx)
+int layer_bwd(cad_unet *u, int l, const ConvIn &in, const bf16 *dy, int cs_dy, const ConvOut *dx)
 {
     const LayerDef &L = LAYERS[l];
     const UGeo &g = u->g[L.level];
     if (dx) {
-        int rc = conv(u, g, L.cout, dy, cs_dy, u->wt[l], nullptr, dx, cs_dx, L.cin, false);
+        int rc = conv(u, g, L.cout, one_in(dy, cs_dy, L.cout), u->wt[l], nullptr, *dx, L.cin, false);
         if (rc) return rc;
     }
     const long long total = (long long)g.nx * g.ny * g.nz * g.nb;
@@ -667,7 +812,7 @@ int layer_bwd(cad_unet *u, int l, const bf16 *in, int cs_in, const bf16 *dy, int
     float *dw = u->gr + u->wofs[l], *db = u->gr + u->bofs[l];
 #define CAD_DW(CI, CO)                                                                                \
     if (L.cin == CI && L.cout == CO)                                                                  \
-        hipLaunchKernelGGL((k_conv_dw<CI, CO>), grid, blk, 0, u->stream, g, in, cs_in, dy, cs_dy, per, dw, db);
+        hipLaunchKernelGGL((k_conv_dw<CI, CO>), grid, blk, 0, u->stream, g, in, dy, cs_dy, per, dw, db);
     CAD_DW(8, 16) CAD_DW(16, 16) CAD_DW(16, 32) CAD_DW(32, 32) CAD_DW(32, 64) CAD_DW(64, 64)
     CAD_DW(64, 32) CAD_DW(32, 16)
 #undef CAD_DW
@@ -697,6 +842,17 @@ int ew(cad_unet *u, K kern, long long n, A... args)
     return CAD_OK;
 }
 
+// an element-wise kernel over the rows of gi, cg elements per voxel
+template <typename K, typename... A>
+int ewr(cad_unet *u, K kern, const UGeo &gi, int cg, A... args)
+{
+    const int segs = (gi.nx * cg + 255) / 256;
+    const long long nblk = (long long)gi.ny * gi.nz * gi.nb * segs;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, u->stream, segs, args...);
+    CAD_CHECK(hipGetLastError());
+    return CAD_OK;
+}
+
 long long nvox(const UGeo &g) { return (long long)g.nx * g.ny * g.nz * g.nb; }
 
 }  // namespace
@@ -710,23 +866,23 @@ long long nvox(const UGeo &g) { return (long long)g.nx * g.ny * g.nz * g.nb; }
 int cad_unet_forward_impl(cad_unet *u)
 {
     const UGeo *g = u->g;
-    RET_IF(layer_fwd(u, E1A, u->x, 8, u->h1a, 16));
-    RET_IF(layer_fwd(u, E1B, u->h1a, 16, u->cat1 + 16, 32));
-    RET_IF(ew(u, k_pool, nvox(g[1]) * 2, g[1], g[0], u->cat1 + 16, 32, 16, u->p1, 16));
-    RET_IF(layer_fwd(u, E2A, u->p1, 16, u->h2a, 32));
-    RET_IF(layer_fwd(u, E2B, u->h2a, 32, u->cat2 + 32, 64));
-    RET_IF(ew(u, k_pool, nvox(g[2]) * 4, g[2], g[1], u->cat2 + 32, 64, 32, u->p2, 32));
-    RET_IF(layer_fwd(u, B3A, u->p2, 32, u->h3a, 64));
-    RET_IF(layer_fwd(u, B3B, u->h3a, 64, u->h3b, 64));
-    RET_IF(layer_fwd(u, U3, u->h3b, 64, u->h3c, 32));
-    RET_IF(ew(u, k_up, nvox(g[1]) * 4, g[2], g[1], u->h3c, 32, 32, u->cat2, 64));
-    RET_IF(layer_fwd(u, D2A, u->cat2, 64, u->h2d, 32));
-    RET_IF(layer_fwd(u, D2B, u->h2d, 32, u->h2e, 32));
-    RET_IF(layer_fwd(u, U2, u->h2e, 32, u->h2f, 16));
-    RET_IF(ew(u, k_up, nvox(g[0]) * 2, g[1], g[0], u->h2f, 16, 16, u->cat1, 32));
-    RET_IF(layer_fwd(u, D1A, u->cat1, 32, u->h1d, 16));
-    RET_IF(layer_fwd(u, D1B, u->h1d, 16, u->h1e, 16));
-    RET_IF(ew(u, k_out, nvox(g[0]), g[0], u->h1e, u->p + u->out_ofs, u->y));
+    RET_IF(layer_fwd(u, E1A, one_in(u->x, 8, 8), u->h1a, 16));
+    RET_IF(layer_fwd(u, E1B, one_in(u->h1a, 16, 16), u->cat1b, 16));
+    RET_IF(ewr(u, k_pool, g[1], 2, g[1], g[0], u->cat1b, 16, 16, u->p1, 16));
+    RET_IF(layer_fwd(u, E2A, one_in(u->p1, 16, 16), u->h2a, 32));
+    RET_IF(layer_fwd(u, E2B, one_in(u->h2a, 32, 32), u->cat2b, 32));
+    RET_IF(ewr(u, k_pool, g[2], 4, g[2], g[1], u->cat2b, 32, 32, u->p2, 32));
+    RET_IF(layer_fwd(u, B3A, one_in(u->p2, 32, 32), u->h3a, 64));
+    RET_IF(layer_fwd(u, B3B, one_in(u->h3a, 64, 64), u->h3b, 64));
+    RET_IF(layer_fwd(u, U3, one_in(u->h3b, 64, 64), u->h3c, 32));
+    RET_IF(ewr(u, k_up, g[1], 4, g[2], g[1], u->h3c, 32, 32, u->cat2a, 32));
+    RET_IF(layer_fwd(u, D2A, cat_in(u->cat2a, u->cat2b - u->cat2a, 32, 32), u->h2d, 32));
+    RET_IF(layer_fwd(u, D2B, one_in(u->h2d, 32, 32), u->h2e, 32));
+    RET_IF(layer_fwd(u, U2, one_in(u->h2e, 32, 32), u->h2f, 16));
+    RET_IF(ewr(u, k_up, g[0], 2, g[1], g[0], u->h2f, 16, 16, u->cat1a, 16));
+    RET_IF(layer_fwd(u, D1A, cat_in(u->cat1a, u->cat1b - u->cat1a, 16, 16), u->h1d, 16));
+    RET_IF(layer_fwd(u, D1B, one_in(u->h1d, 16, 16), u->h1e, 16));
+    RET_IF(ewr(u, k_out, g[0], 1, g[0], u->h1e, u->p + u->out_ofs, u->y));
     return CAD_OK;
 }
 
@@ -737,36 +893,42 @@ static int backward(cad_unet *u)
     hipLaunchKernelGGL(k_out_bwd, dim3(512), dim3(256), 0, u->stream, g[0], u->h1e, u->p + u->out_ofs, u->dy,
                        u->dh1e, gout);
     CAD_CHECK(hipGetLastError());
-    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->h1e, 16, 16, u->dh1e, 16));
-    RET_IF(layer_bwd(u, D1B, u->h1d, 16, u->dh1e, 16, u->dh1d, 16));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->h1d, 16, 16, u->dh1d, 16));
-    RET_IF(layer_bwd(u, D1A, u->cat1, 32, u->dh1d, 16, u->dcat1, 32));
-    // lower half of cat1: the upsampled h2f
-    RET_IF(ew(u, k_up_bwd, nvox(g[1]) * 2, g[1], g[0], u->dcat1, 32, 16, u->dh2f, 16));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 2, g[1], u->h2f, 16, 16, u->dh2f, 16));
-    RET_IF(layer_bwd(u, U2, u->h2e, 32, u->dh2f, 16, u->dh2e, 32));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->h2e, 32, 32, u->dh2e, 32));
-    RET_IF(layer_bwd(u, D2B, u->h2d, 32, u->dh2e, 32, u->dh2d, 32));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->h2d, 32, 32, u->dh2d, 32));
-    RET_IF(layer_bwd(u, D2A, u->cat2, 64, u->dh2d, 32, u->dcat2, 64));
-    RET_IF(ew(u, k_up_bwd, nvox(g[2]) * 4, g[2], g[1], u->dcat2, 64, 32, u->dh3c, 32));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[2]) * 4, g[2], u->h3c, 32, 32, u->dh3c, 32));
-    RET_IF(layer_bwd(u, U3, u->h3b, 64, u->dh3c, 32, u->dh3b, 64));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[2]) * 8, g[2], u->h3b, 64, 64, u->dh3b, 64));
-    RET_IF(layer_bwd(u, B3B, u->h3a, 64, u->dh3b, 64, u->dh3a, 64));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[2]) * 8, g[2], u->h3a, 64, 64, u->dh3a, 64));
-    RET_IF(layer_bwd(u, B3A, u->p2, 32, u->dh3a, 64, u->dp2, 32));
-    // upper half of cat2 (the skip, e2b's output): decoder part + pooling part
-    RET_IF(ew(u, k_pool_bwd, nvox(g[1]) * 4, g[2], g[1], u->dp2, 32, 32, u->dcat2 + 32, 64));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->cat2 + 32, 64, 32, u->dcat2 + 32, 64));
-    RET_IF(layer_bwd(u, E2B, u->h2a, 32, u->dcat2 + 32, 64, u->dh2a, 32));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[1]) * 4, g[1], u->h2a, 32, 32, u->dh2a, 32));
-    RET_IF(layer_bwd(u, E2A, u->p1, 16, u->dh2a, 32, u->dp1, 16));
-    RET_IF(ew(u, k_pool_bwd, nvox(g[0]) * 2, g[1], g[0], u->dp1, 16, 16, u->dcat1 + 16, 32));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->cat1 + 16, 32, 16, u->dcat1 + 16, 32));
-    RET_IF(layer_bwd(u, E1B, u->h1a, 16, u->dcat1 + 16, 32, u->dh1a, 16));
-    RET_IF(ew(u, k_relu_bwd, nvox(g[0]) * 2, g[0], u->h1a, 16, 16, u->dh1a, 16));
-    RET_IF(layer_bwd(u, E1A, u->x, 8, u->dh1a, 16, nullptr, 0));
+    const ConvOut dh1d = one_out(u->dh1d, 16, 16), dcat1{u->dcat1a, u->dcat1b, 16, 16, 16};
+    const ConvOut dh2e = one_out(u->dh2e, 32, 32), dh2d = one_out(u->dh2d, 32, 32);
+    const ConvOut dcat2{u->dcat2a, u->dcat2b, 32, 32, 32}, dh3b = one_out(u->dh3b, 64, 64);
+    const ConvOut dh3a = one_out(u->dh3a, 64, 64), dp2 = one_out(u->dp2, 32, 32);
+    const ConvOut dh2a = one_out(u->dh2a, 32, 32), dp1 = one_out(u->dp1, 16, 16);
+    const ConvOut dh1a = one_out(u->dh1a, 16, 16);
+    RET_IF(ewr(u, k_relu_bwd, g[0], 2, g[0], u->h1e, 16, 16, u->dh1e, 16));
+    RET_IF(layer_bwd(u, D1B, one_in(u->h1d, 16, 16), u->dh1e, 16, &dh1d));
+    RET_IF(ewr(u, k_relu_bwd, g[0], 2, g[0], u->h1d, 16, 16, u->dh1d, 16));
+    RET_IF(layer_bwd(u, D1A, cat_in(u->cat1a, u->cat1b - u->cat1a, 16, 16), u->dh1d, 16, &dcat1));
+    // the skip's first half: the upsampled h2f
+    RET_IF(ewr(u, k_up_bwd, g[1], 2, g[1], g[0], u->dcat1a, 16, 16, u->dh2f, 16));
+    RET_IF(ewr(u, k_relu_bwd, g[1], 2, g[1], u->h2f, 16, 16, u->dh2f, 16));
+    RET_IF(layer_bwd(u, U2, one_in(u->h2e, 32, 32), u->dh2f, 16, &dh2e));
+    RET_IF(ewr(u, k_relu_bwd, g[1], 4, g[1], u->h2e, 32, 32, u->dh2e, 32));
+    RET_IF(layer_bwd(u, D2B, one_in(u->h2d, 32, 32), u->dh2e, 32, &dh2d));
+    RET_IF(ewr(u, k_relu_bwd, g[1], 4, g[1], u->h2d, 32, 32, u->dh2d, 32));
+    RET_IF(layer_bwd(u, D2A, cat_in(u->cat2a, u->cat2b - u->cat2a, 32, 32), u->dh2d, 32, &dcat2));
+    RET_IF(ewr(u, k_up_bwd, g[2], 4, g[2], g[1], u->dcat2a, 32, 32, u->dh3c, 32));
+    RET_IF(ewr(u, k_relu_bwd, g[2], 4, g[2], u->h3c, 32, 32, u->dh3c, 32));
+    RET_IF(layer_bwd(u, U3, one_in(u->h3b, 64, 64), u->dh3c, 32, &dh3b));
+    RET_IF(ewr(u, k_relu_bwd, g[2], 8, g[2], u->h3b, 64, 64, u->dh3b, 64));
+    RET_IF(layer_bwd(u, B3B, one_in(u->h3a, 64, 64), u->dh3b, 64, &dh3a));
+    RET_IF(ewr(u, k_relu_bwd, g[2], 8, g[2], u->h3a, 64, 64, u->dh3a, 64));
+    RET_IF(layer_bwd(u, B3A, one_in(u->p2, 32, 32), u->dh3a, 64, &dp2));
+    // the skip's second half (e2b's output): decoder part + pooling part
+    RET_IF(ewr(u, k_pool_bwd, g[1], 4, g[2], g[1], u->dp2, 32, 32, u->dcat2b, 32));
+    RET_IF(ewr(u, k_relu_bwd, g[1], 4, g[1], u->cat2b, 32, 32, u->dcat2b, 32));
+    RET_IF(layer_bwd(u, E2B, one_in(u->h2a, 32, 32), u->dcat2b, 32, &dh2a));
+    RET_IF(ewr(u, k_relu_bwd, g[1], 4, g[1], u->h2a, 32, 32, u->dh2a, 32));
+    RET_IF(layer_bwd(u, E2A, one_in(u->p1, 16, 16), u->dh2a, 32, &dp1));
+    RET_IF(ewr(u, k_pool_bwd, g[0], 2, g[1], g[0], u->dp1, 16, 16, u->dcat1b, 16));
+    RET_IF(ewr(u, k_relu_bwd, g[0], 2, g[0], u->cat1b, 16, 16, u->dcat1b, 16));
+    RET_IF(layer_bwd(u, E1B, one_in(u->h1a, 16, 16), u->dcat1b, 16, &dh1a));
+    RET_IF(ewr(u, k_relu_bwd, g[0], 2, g[0], u->h1a, 16, 16, u->dh1a, 16));
+    RET_IF(layer_bwd(u, E1A, one_in(u->x, 8, 8), u->dh1a, 16, nullptr));
     return CAD_OK;
 }
 
@@ -789,14 +951,16 @@ int cad_unet_param_count(void)
 
 int cad_unet_create(int nx, int ny, int nz, int nb, int train, void *stream, cad_unet **out)
 {
-    if (!out || nx < 64 || nx % 64 || ny < 4 || ny % 4 || nz < 4 || nz % 4 || nb < 1) {
-        cad_set_error("cad_unet_create: box %d x %d x %d (x a multiple of 64, y and z of 4), batch %d",
+    if (!out || nx < 64 || nx % 64 || nx > 1024 || ny < 4 || ny % 4 || nz < 4 || nz % 4 || nb < 1) {
+        cad_set_error("cad_unet_create: box %d x %d x %d (x a multiple of 64 up to 1024, y and z of 4), "
+                      "batch %d",
                       nx, ny, nz, nb);
         return CAD_ERR_ARG;
     }
     *out = nullptr;
     cad_unet *u = new cad_unet();
     CAD_CHECK(hipGetDevice(&u->device));
+    CAD_CHECK(hipDeviceGetAttribute(&u->ncu, hipDeviceAttributeMultiprocessorCount, u->device));
     if (stream) {
         u->stream = reinterpret_cast<hipStream_t>(stream);
     } else {
@@ -835,20 +999,22 @@ int cad_unet_create(int nx, int ny, int nz, int nb, int train, void *stream, cad
     A(&u->p, u->nparam);
     for (int l = 0; l < NL; l++) {
         const LayerDef &L = LAYERS[l];
-        u->wp_n[l] = (long long)(L.cout / 16) * ((27 * L.cin + 31) / 32) * 64 * 8;
+        u->wp_n[l] = 3LL * (L.cout / 16) * fold_kc(L.cin) * 64 * 8;
         A(&u->wp[l], u->wp_n[l]);
         if (u->train && l != E1A) {
-            u->wt_n[l] = (long long)(L.cin / 16) * ((27 * L.cout + 31) / 32) * 64 * 8;
+            u->wt_n[l] = 3LL * (L.cin / 16) * fold_kc(L.cout) * 64 * 8;
             A(&u->wt[l], u->wt_n[l]);
         }
     }
     const long long v0 = padded(u->g[0]), v1 = padded(u->g[1]), v2 = padded(u->g[2]);
     A(&u->x, v0 * 8);
     A(&u->h1a, v0 * 16);
-    A(&u->cat1, v0 * 32);
+    A(&u->cat1a, v0 * 32);
+    if (!rc) u->cat1b = u->cat1a + v0 * 16;
     A(&u->p1, v1 * 16);
     A(&u->h2a, v1 * 32);
-    A(&u->cat2, v1 * 64);
+    A(&u->cat2a, v1 * 64);
+    if (!rc) u->cat2b = u->cat2a + v1 * 32;
     A(&u->p2, v2 * 32);
     A(&u->h3a, v2 * 64);
     A(&u->h3b, v2 * 64);
@@ -865,11 +1031,13 @@ int cad_unet_create(int nx, int ny, int nz, int nb, int train, void *stream, cad
         A(&u->m2, u->nparam);
         A(&u->dh1e, v0 * 16);
         A(&u->dh1d, v0 * 16);
-        A(&u->dcat1, v0 * 32);
+        A(&u->dcat1a, v0 * 16);
+        A(&u->dcat1b, v0 * 16);
         A(&u->dh2f, v1 * 16);
         A(&u->dh2e, v1 * 32);
         A(&u->dh2d, v1 * 32);
-        A(&u->dcat2, v1 * 64);
+        A(&u->dcat2a, v1 * 32);
+        A(&u->dcat2b, v1 * 32);
         A(&u->dh3c, v2 * 32);
         A(&u->dh3b, v2 * 64);
         A(&u->dh3a, v2 * 64);
@@ -960,8 +1128,8 @@ int cad_unet_set_depth(cad_unet *u, int nz)
     struct B {
         bf16 *a;
         int lvl, cs;
-    } bufs[] = {{u->x, 0, 8},     {u->h1a, 0, 16},  {u->cat1, 0, 32}, {u->h1d, 0, 16},
-                {u->h1e, 0, 16},  {u->p1, 1, 16},   {u->h2a, 1, 32},  {u->cat2, 1, 64},
+    } bufs[] = {{u->x, 0, 8},     {u->h1a, 0, 16},  {u->cat1a, 0, 16}, {u->cat1b, 0, 16}, {u->h1d, 0, 16},
+                {u->h1e, 0, 16},  {u->p1, 1, 16},   {u->h2a, 1, 32},  {u->cat2a, 1, 32}, {u->cat2b, 1, 32},
                 {u->h2d, 1, 32},  {u->h2e, 1, 32},  {u->h2f, 1, 16},  {u->p2, 2, 32},
                 {u->h3a, 2, 64},  {u->h3b, 2, 64},  {u->h3c, 2, 32}};
     for (const B &b : bufs) {

@@ -2,7 +2,10 @@
 plain float32 PyTorch statement of the same network on the CPU
 (tests/unet_ref.py): the output, the loss and every layer's parameter
 gradient, on random inputs and parameters (biases non-zero), for box sizes
-that take both wave tilings of the convolution (16 and 64 voxels along x).
+that take every tiling of the convolution: one wave per x row with idle
+16-voxel groups, two waves (the x shift crosses waves through LDS), eight
+and sixteen waves, and output channels split over two launches (1024 voxels
+along x).
 The bar is BF16's: relative L2 error of the output within 3e-2, of each
 layer's gradient within 1e-1 with cosine similarity above 0.99 -- a wrong tap,
 a transposed weight block or a misplaced channel slice gives errors of order
@@ -38,7 +41,8 @@ def _setup(pkg, box, nb, seed):
     return p, pr, xb.float(), xp, e, m
 
 
-@pytest.mark.parametrize("box,nb", [((64, 8, 8), 2), ((128, 4, 8), 1)])
+@pytest.mark.parametrize("box,nb", [((64, 8, 8), 2), ((128, 4, 8), 1), ((512, 4, 4), 1),
+                                    ((1024, 4, 4), 1)])
 def test_unet_forward_and_gradients_match_torch(pkg, box, nb):
     import torch
     U = pkg.unet
