@@ -243,6 +243,157 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
     }
 }
 
+// bytes of a staged input plane tile: 6 rows (the block's 4 and their y
+// neighbours) by the block's 16*NT*WX voxels and their x neighbours
+__host__ __device__ constexpr int conv_tile_bytes(int cin, int nt, int wx) { return 6 * (16 * nt * wx + 2) * cin * 2; }
+constexpr int CONV_STAGED_MAX = 112 * 1024;
+// (in blocks of 16 waves, 128 registers a lane: the staging registers fit
+// beside 8 accumulator quads, not 12)
+__host__ __device__ constexpr bool conv_staged(int cin, int mt, int nt)
+{
+    return conv_tile_bytes(cin, nt, conv_wx(cin, mt)) + conv_a_bytes(cin, mt) <= CONV_STAGED_MAX &&
+           (conv_wx(cin, mt) == 1 || mt * nt <= 2);
+}
+
+// k_conv3 with the input staged in LDS: for each input plane the block copies
+// its tile (16-byte chunks, each thread a fixed set of them) into LDS, and
+// the waves read their B fragments from there; the next plane's chunks are
+// loaded into registers while the current plane is computed.  Same tiling,
+// numbering and arguments as k_conv3.
+template <int CIN, int MT, int NT, bool RELU>
+__global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 ? 3 : 4)
+    k_conv3s(UGeo g, ConvIn in, const bf16x8 *__restrict__ wp, const float *__restrict__ bias, ConvOut out,
+             int m0, int ntx, int nty, int nzc, int zlen, int per_xcd, int ntiles)
+{
+    constexpr int KC = fold_kc(CIN);
+    constexpr int WX = conv_wx(CIN, MT);
+    constexpr int NA = 3 * MT * KC * 64;
+    constexpr int TW = 16 * NT * WX + 2;   // tile width in voxels
+    constexpr int CH = CIN / 8;            // 16-byte chunks per voxel
+    constexpr int NCH = 6 * TW * CH;       // chunks per tile
+    constexpr int NTHR = 256 * WX;
+    constexpr int PER = (NCH + NTHR - 1) / NTHR;
+    static_assert(conv_staged(CIN, MT, NT), "tile and weights must fit in LDS");
+    __shared__ bf16x8 as[NA];
+    __shared__ bf16x8 tl[NCH];
+    const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= ntiles) return;   // the whole block
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, zc = (tile / (ntx * nty)) % nzc;
+    const int b = tile / (ntx * nty * nzc);
+    const int z0 = zc * zlen, z1 = min(g.nz, z0 + zlen);
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n = lane & 15, kg = lane >> 4;
+    const int wy = wv & 3, wxi = wv >> 2;
+    const int xb = 16 * NT * WX * tx, y0 = 4 * ty;
+    const int y = y0 + wy;
+    const int gi0 = (xb >> 4) + wxi * NT, ng = g.nx >> 4;
+    const bool active = y < g.ny && gi0 < ng;
+    const int np = z1 - z0 + 2;   // input planes z0-1 .. z1
+    // this thread's chunks: byte offsets from the block's base voxel
+    // (xb - 1, y0 - 1, plane z0 - 1); chunks past the box's halo are skipped
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(in.p + vox(g, xb - 1, y0 - 1, z0 - 1, b) * in.cs), (short)0, -1, 0x00020000);
+    const int plane_bytes = 2 * g.px * g.py * in.cs;
+    int goff[PER];
+    bool gok[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int c = tid + i * NTHR;
+        const int r = c / (TW * CH), rem = c - r * (TW * CH);
+        const int col = rem / CH, c8 = 8 * (rem - col * CH);
+        gok[i] = c < NCH && xb - 1 + col <= g.nx && y0 - 1 + r <= g.ny;
+        const bool hi = c8 >= in.split;
+        goff[i] = (int)(2LL * ((long long)(r * g.px + col) * in.cs + (hi ? in.delta + c8 - in.split : c8)));
+    }
+    bf16x8 stg[PER];
+    auto fetch = [&](int pi) {
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (gok[i])
+                stg[i] = __builtin_bit_cast(bf16x8,
+                                            __builtin_amdgcn_raw_buffer_load_b128(rs, goff[i], pi * plane_bytes, 0));
+    };
+    fetch(0);
+    for (int e = tid; e < NA; e += NTHR) as[e] = wp[e];
+    const f4 zero = {0.f, 0.f, 0.f, 0.f};
+    f4 acc[3][MT][NT];   // slot (o - z0 + 2) % 3 holds out(o)
+#pragma unroll
+    for (int sl = 0; sl < 3; sl++)
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) acc[sl][m][j] = zero;
+    float bs[MT][4];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) bs[m][i] = bias ? bias[16 * (m0 + m) + 4 * kg + i] : 0.f;
+    // the wave's first B chunk: row wy (tap dy = -1), voxel wxi*16*NT + n (tap dx = -1)
+    const int lb = (wy * TW + wxi * 16 * NT + n) * CH;
+#pragma unroll 1
+    for (int p0 = 0; p0 < np; p0 += 3) {
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            const int pi = p0 + u;   // input plane z0 - 1 + pi
+            if (pi >= np) break;
+            __syncthreads();   // the previous tile is read
+#pragma unroll
+            for (int i = 0; i < PER; i++)
+                if (gok[i]) tl[tid + i * NTHR] = stg[i];
+            __syncthreads();
+            if (pi + 1 < np) fetch(pi + 1);
+            if (active) {
+#pragma unroll
+                for (int q = 0; q < KC; q++) {
+                    const int k0 = 32 * q + 8 * kg;
+                    int t = k0 / CIN;
+                    if (t > 8) t = 8;   // K padding: the weights there are zero
+                    const int bq = lb + ((t / 3) * TW + t % 3) * CH + (k0 % CIN) / 8;
+                    bf16x8 bf[NT];
+#pragma unroll
+                    for (int j = 0; j < NT; j++) bf[j] = tl[bq + 16 * j * CH];
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            // dz = d - 1 adds to out(p - dz), slot (u + 2 - d) % 3
+                            const bf16x8 af = as[((3 * m + d) * KC + q) * 64 + lane];
+#pragma unroll
+                            for (int j = 0; j < NT; j++)
+                                acc[(u + 2 - d) % 3][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    af, bf[j], acc[(u + 2 - d) % 3][m][j], 0, 0, 0);
+                        }
+                }
+                // out(p - 1) is complete: slot u % 3
+                const int o = z0 - 2 + pi;
+                if (o >= z0 && o < z1) {
+                    const long long v0 = vox(g, 16 * gi0 + n, y, o, b);
+#pragma unroll
+                    for (int m = 0; m < MT; m++) {
+                        const int co = 16 * (m0 + m) + 4 * kg;
+#pragma unroll
+                        for (int j = 0; j < NT; j++) {
+                            if (gi0 + j >= ng) continue;
+                            bf16x4 ov;
+#pragma unroll
+                            for (int c = 0; c < 4; c++) {
+                                float val = acc[u % 3][m][j][c] + bs[m][c];
+                                if (RELU) val = val > 0.f ? val : 0.f;
+                                ov[c] = (bf16)val;
+                            }
+                            *reinterpret_cast<bf16x4 *>(chan(out, v0 + 16 * j, co)) = ov;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < MT; m++)
+#pragma unroll
+                    for (int j = 0; j < NT; j++) acc[u % 3][m][j] = zero;
+            }
+        }
+    }
+}
+
 // Packs FP32 weights W[t][ci][co] (27 x cin x cout, t = dx+1 + 3(dy+1) + 9(dz+1))
 // into the A fragments of k_conv3 for cin -> cout: tile 3m + dzi holds output
 // channels 16m..16m+15 of the z tap dzi, its K runs over (dx, dy) taps and
@@ -728,12 +879,21 @@ int conv_launch(cad_unet *u, const UGeo &g, const ConvIn &in, const bf16 *wp, co
     const int per_xcd = (int)((ntiles + 7) / 8);
     const dim3 grid((unsigned)(8 * per_xcd)), blk(256 * WX);
     const bf16x8 *w = reinterpret_cast<const bf16x8 *>(wp);
-    if (relu)
-        hipLaunchKernelGGL((k_conv3<CIN, MT, NT, true>), grid, blk, 0, u->stream, g, in, w, bias, out, m0, ntx,
-                           nty, nzc, zlen, per_xcd, (int)ntiles);
-    else
-        hipLaunchKernelGGL((k_conv3<CIN, MT, NT, false>), grid, blk, 0, u->stream, g, in, w, bias, out, m0, ntx,
-                           nty, nzc, zlen, per_xcd, (int)ntiles);
+    if constexpr (conv_staged(CIN, MT, NT)) {
+        if (relu)
+            hipLaunchKernelGGL((k_conv3s<CIN, MT, NT, true>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
+                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+        else
+            hipLaunchKernelGGL((k_conv3s<CIN, MT, NT, false>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
+                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+    } else {
+        if (relu)
+            hipLaunchKernelGGL((k_conv3<CIN, MT, NT, true>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
+                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+        else
+            hipLaunchKernelGGL((k_conv3<CIN, MT, NT, false>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
+                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+    }
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
