@@ -48,9 +48,9 @@ struct Learned {
     double *vecs;                   // V (m+1), Z (m), then 6 scratch vectors
     int nvec;
     double *part;                   // REDUCE_BLOCKS x (MAXM+1) partial sums
-    double *hdev;                   // MAXM+1 coefficients
+    double *hdev;                   // Gram-Schmidt coefficients of both passes and |w|^2: 3 (MAXM+1)
     double *scal;                   // [0] s, [1] 1/s, [2] active count
-    double *hpin;                   // pinned host mirror (MAXM+1)
+    double *hpin;                   // pinned host mirror of hdev
     float *y;                       // network output
     unsigned long long prev_rng[2]; // active range of the previous solve
     int kb, nzr;                    // the current box placement
@@ -252,6 +252,80 @@ __global__ void __launch_bounds__(256) k_maxpy(const unsigned long long *rng, co
     }
 }
 
+// The second Gram-Schmidt pass fused with the first pass's update: on the
+// active cells w -= sum_i h_i V_i (h: the first pass's coefficients), then
+// the partials of (V_i, w) with the updated w, block order as k_mdot
+__global__ void __launch_bounds__(256) k_maxpy_mdot(const unsigned long long *rng, const int *__restrict__ mask,
+                                                    const double *__restrict__ V, long long nc, int nv,
+                                                    const double *__restrict__ h, double *__restrict__ w,
+                                                    double *__restrict__ part)
+{
+    __shared__ double hs[MAXM + 1];
+    if ((int)threadIdx.x < nv) hs[threadIdx.x] = -h[threadIdx.x];
+    __syncthreads();
+    double acc[MAXM + 1];
+#pragma unroll
+    for (int i = 0; i <= MAXM; i++) acc[i] = 0.0;
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (mask[n] == -1) continue;
+        double vn[MAXM + 1];
+        double s = w[n];
+#pragma unroll
+        for (int i = 0; i <= MAXM; i++)
+            if (i < nv) {
+                vn[i] = V[i * nc + n];
+                s += hs[i] * vn[i];
+            }
+        w[n] = s;
+#pragma unroll
+        for (int i = 0; i <= MAXM; i++)
+            if (i < nv) acc[i] += vn[i] * s;
+    }
+    __shared__ double sh[4][MAXM + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i <= MAXM; i++) {
+        if (i < nv) {
+            const double t = wave_sum(acc[i]);
+            if (lane == 0) sh[wv][i] = t;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv)
+        part[(long long)blockIdx.x * (MAXM + 1) + threadIdx.x] =
+            sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+
+// The second pass's update fused with the norm: on the active cells
+// w -= sum_i h_i V_i, then the partials of (w, w) in column 0, block order
+__global__ void __launch_bounds__(256) k_maxpy_sumsq(const unsigned long long *rng, const int *__restrict__ mask,
+                                                     const double *__restrict__ V, long long nc, int nv,
+                                                     const double *__restrict__ h, double *__restrict__ w,
+                                                     double *__restrict__ part)
+{
+    __shared__ double hs[MAXM + 1];
+    if ((int)threadIdx.x < nv) hs[threadIdx.x] = -h[threadIdx.x];
+    __syncthreads();
+    double acc = 0.0;
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (mask[n] == -1) continue;
+        double s = w[n];
+        for (int i = 0; i < nv; i++) s += hs[i] * V[i * nc + n];
+        w[n] = s;
+        acc += s * s;
+    }
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double t = wave_sum(acc);
+    if (lane == 0) sh[wv] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(long long)blockIdx.x * (MAXM + 1)] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 // w *= a (active cells)
 __global__ void k_vscale(const unsigned long long *rng, const int *__restrict__ mask, double a,
                          double *__restrict__ w)
@@ -375,10 +449,10 @@ int learned_init(cad_ctx *c)
     L->nvec = 2 * MAXM + 1 + 6;
     RET_IF(lalloc((void **)&L->vecs, (size_t)L->nvec * d.nc * sizeof(double)));
     RET_IF(lalloc((void **)&L->part, (size_t)REDUCE_BLOCKS * (MAXM + 1) * sizeof(double)));
-    RET_IF(lalloc((void **)&L->hdev, (MAXM + 1) * sizeof(double)));
+    RET_IF(lalloc((void **)&L->hdev, 3 * (MAXM + 1) * sizeof(double)));
     RET_IF(lalloc((void **)&L->scal, 4 * sizeof(double)));
     RET_IF(lalloc((void **)&L->y, (size_t)L->nx * L->ny * L->nzmax * sizeof(float)));
-    if (hipHostMalloc((void **)&L->hpin, (MAXM + 1) * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&L->hpin, 3 * (MAXM + 1) * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         cad_set_error("learned form: hipHostMalloc failed");
         return CAD_ERR_HIP;
     }
@@ -441,19 +515,6 @@ int precond(cad_ctx *c, const double *v, double *z)
     hipLaunchKernelGGL(k_scatter, dim3(VBLK), dim3(256), 0, c->stream, c->d, c->act_rng, L->nx, L->ny, L->kb,
                        L->nzr, c->f.indc, z1, L->scal, cad_unet_output(L->net), z);
     CAD_CHECK(hipGetLastError());
-    return CAD_OK;
-}
-
-int mdot(cad_ctx *c, const double *V, int nv, const double *w, double *host)
-{
-    Learned *L = c->lrn;
-    hipLaunchKernelGGL(k_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, c->act_rng, c->f.indc, V,
-                       c->d.nc, nv, w, L->part);
-    hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, c->stream, L->part, REDUCE_BLOCKS, nv, L->hdev);
-    CAD_CHECK(hipGetLastError());
-    CAD_CHECK(hipMemcpyAsync(L->hpin, L->hdev, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CAD_CHECK(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < nv; i++) host[i] = L->hpin[i];
     return CAD_OK;
 }
 
@@ -535,7 +596,7 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
     int iters = 0;
     double rnorm = 0.0;
     bool done = false, stalled = false;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
     double prev = -1.0;
     while (!done && !stalled) {
         // r = BB - A x into V_0
@@ -564,20 +625,25 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
             RET_IF(precond(c, Vv(c, j), Zv(c, j)));
             double *w = Vv(c, j + 1);
             RET_IF(cad_ext_apply_a(c, Zv(c, j), w));
-            // classical Gram-Schmidt, twice
-            RET_IF(mdot(c, Vv(c, 0), j + 1, w, h.data()));
-            CAD_CHECK(hipMemcpyAsync(L->hdev, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_maxpy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, j + 1,
-                               L->hdev, -1.0, w);
-            RET_IF(mdot(c, Vv(c, 0), j + 1, w, h2.data()));
-            CAD_CHECK(hipMemcpyAsync(L->hdev, h2.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_maxpy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, j + 1,
-                               L->hdev, -1.0, w);
+            // classical Gram-Schmidt, twice, with the coefficients kept on
+            // the device: h1 = V^T w; w -= V h1 and h2 = V^T w in one pass
+            // over V; w -= V h2 and |w|^2 in another; one read-back
+            double *h1d = L->hdev, *h2d = L->hdev + (MAXM + 1), *wwd = L->hdev + 2 * (MAXM + 1);
+            const int nv = j + 1;
+            hipLaunchKernelGGL(k_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, nv,
+                               w, L->part);
+            hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h1d);
+            hipLaunchKernelGGL(k_maxpy_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0),
+                               nc, nv, h1d, w, L->part);
+            hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h2d);
+            hipLaunchKernelGGL(k_maxpy_sumsq, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0),
+                               nc, nv, h2d, w, L->part);
+            hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, st, L->part, REDUCE_BLOCKS, 1, wwd);
             CAD_CHECK(hipGetLastError());
-            for (int i = 0; i <= j; i++) h[i] += h2[i];
-            double ww = 0.0;
-            RET_IF(cad_ext_dot(c, w, w, &ww));
-            const double hn = std::sqrt(ww);
+            CAD_CHECK(hipMemcpyAsync(L->hpin, L->hdev, 3 * (MAXM + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+            CAD_CHECK(hipStreamSynchronize(st));
+            for (int i = 0; i <= j; i++) h[i] = L->hpin[i] + L->hpin[MAXM + 1 + i];
+            const double hn = std::sqrt(L->hpin[2 * (MAXM + 1)]);
             iters++;
             // column j of H, rotated by the earlier Givens rotations
             for (int i = 0; i < j; i++) {

@@ -56,6 +56,15 @@ def test_unet_forward_and_gradients_match_torch(pkg, box, nb):
         net.forward(y)
         y = y.cpu().numpy().reshape(nb, nz, ny, nx)
         loss, g = net.gradients(torch.from_numpy(e.ravel()).cuda(), torch.from_numpy(m.ravel()).cuda())
+    # an inference network (upsampling read inside the decoder's convolution,
+    # output layer fused into the last one) gives the same output bit for bit
+    with U.UNet(nx, ny, nz, nb, train=False) as net:
+        net.set_params(p)
+        net.set_input(xp.cuda())
+        yi = torch.empty(nb * nz * ny * nx, dtype=torch.float32, device="cuda")
+        net.forward(yi)
+        yi = yi.cpu().numpy().reshape(nb, nz, ny, nx)
+    assert np.array_equal(yi, y)
     prm = unet_ref.forward(U, pr, xf)
     for v in prm.values():
         v[0].requires_grad_(True)
