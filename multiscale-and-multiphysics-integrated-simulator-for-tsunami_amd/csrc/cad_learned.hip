@@ -185,27 +185,29 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// partials of (V_i, w) for i < nv over the active cells (masked), block order
+// partials of (V_i, w) for i < nv over the active cells (masked), block
+// order; NV >= nv bounds the registers the sums take
+template <int NV>
 __global__ void __launch_bounds__(256) k_mdot(const unsigned long long *rng, const int *__restrict__ mask,
                                               const double *__restrict__ V, long long nc, int nv,
                                               const double *__restrict__ w, double *__restrict__ part)
 {
-    double acc[MAXM + 1];
+    double acc[NV];
 #pragma unroll
-    for (int i = 0; i <= MAXM; i++) acc[i] = 0.0;
+    for (int i = 0; i < NV; i++) acc[i] = 0.0;
     const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
     for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
          n += (long long)gridDim.x * blockDim.x) {
         if (mask[n] == -1) continue;
         const double wn = w[n];
 #pragma unroll
-        for (int i = 0; i <= MAXM; i++)
+        for (int i = 0; i < NV; i++)
             if (i < nv) acc[i] += V[i * nc + n] * wn;
     }
-    __shared__ double sh[4][MAXM + 1];
+    __shared__ double sh[4][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i <= MAXM; i++) {
+    for (int i = 0; i < NV; i++) {
         if (i < nv) {
             const double s = wave_sum(acc[i]);
             if (lane == 0) sh[wv][i] = s;
@@ -255,38 +257,39 @@ __global__ void __launch_bounds__(256) k_maxpy(const unsigned long long *rng, co
 // The second Gram-Schmidt pass fused with the first pass's update: on the
 // active cells w -= sum_i h_i V_i (h: the first pass's coefficients), then
 // the partials of (V_i, w) with the updated w, block order as k_mdot
+template <int NV>
 __global__ void __launch_bounds__(256) k_maxpy_mdot(const unsigned long long *rng, const int *__restrict__ mask,
                                                     const double *__restrict__ V, long long nc, int nv,
                                                     const double *__restrict__ h, double *__restrict__ w,
                                                     double *__restrict__ part)
 {
-    __shared__ double hs[MAXM + 1];
+    __shared__ double hs[NV];
     if ((int)threadIdx.x < nv) hs[threadIdx.x] = -h[threadIdx.x];
     __syncthreads();
-    double acc[MAXM + 1];
+    double acc[NV];
 #pragma unroll
-    for (int i = 0; i <= MAXM; i++) acc[i] = 0.0;
+    for (int i = 0; i < NV; i++) acc[i] = 0.0;
     const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
     for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
          n += (long long)gridDim.x * blockDim.x) {
         if (mask[n] == -1) continue;
-        double vn[MAXM + 1];
+        double vn[NV];
         double s = w[n];
 #pragma unroll
-        for (int i = 0; i <= MAXM; i++)
+        for (int i = 0; i < NV; i++)
             if (i < nv) {
                 vn[i] = V[i * nc + n];
                 s += hs[i] * vn[i];
             }
         w[n] = s;
 #pragma unroll
-        for (int i = 0; i <= MAXM; i++)
+        for (int i = 0; i < NV; i++)
             if (i < nv) acc[i] += vn[i] * s;
     }
-    __shared__ double sh[4][MAXM + 1];
+    __shared__ double sh[4][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i <= MAXM; i++) {
+    for (int i = 0; i < NV; i++) {
         if (i < nv) {
             const double t = wave_sum(acc[i]);
             if (lane == 0) sh[wv][i] = t;
@@ -526,14 +529,20 @@ int setup(cad_ctx *c, bool *empty)
     unsigned long long hr[2];
     RET_IF(cad_ext_active_range(c, hr));
     *empty = hr[0] >= hr[1];
-    // vectors read by the matvec must be 0 outside the new active range:
-    // clear what the previous solve's range may have left
+    // vectors read by the matvec must be 0 outside the new active range.
+    // Every solve writes its whole range (0 on the inactive cells in it) and
+    // nothing outside it, so only the cells of the previous range that fall
+    // outside the new one can hold values: clear those.
     const long long z0 = (long long)L->prev_rng[0], z1 = (long long)L->prev_rng[1];
-    if (z1 > z0)
+    const long long n0 = (long long)hr[0], n1 = (long long)hr[1];
+    const long long cut[2][2] = {{z0, std::min(z1, n0 < n1 ? n0 : z1)}, {std::max(z0, n0 < n1 ? n1 : z1), z1}};
+    for (const auto &iv : cut) {
+        if (iv[1] <= iv[0]) continue;
         for (int j = 0; j < MAXM; j++)
-            hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, z0, z1, Zv(c, j));
-    for (int q = 0; q < 6; q++)
-        if (z1 > z0) hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, z0, z1, Sv(c, q));
+            hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, iv[0], iv[1], Zv(c, j));
+        for (int q = 0; q < 6; q++)
+            hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, iv[0], iv[1], Sv(c, q));
+    }
     CAD_CHECK(hipGetLastError());
     L->prev_rng[0] = hr[0];
     L->prev_rng[1] = hr[1];
@@ -630,11 +639,24 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
             // over V; w -= V h2 and |w|^2 in another; one read-back
             double *h1d = L->hdev, *h2d = L->hdev + (MAXM + 1), *wwd = L->hdev + 2 * (MAXM + 1);
             const int nv = j + 1;
-            hipLaunchKernelGGL(k_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, nv,
-                               w, L->part);
-            hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h1d);
-            hipLaunchKernelGGL(k_maxpy_mdot, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0),
-                               nc, nv, h1d, w, L->part);
+            // register arrays sized to the basis: 8, 16, 24 or all MAXM+1 vectors
+#define CAD_GS(NVB)                                                                                           \
+    do {                                                                                                      \
+        hipLaunchKernelGGL(k_mdot<NVB>, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, \
+                           nv, w, L->part);                                                                   \
+        hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h1d);          \
+        hipLaunchKernelGGL(k_maxpy_mdot<NVB>, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask,         \
+                           Vv(c, 0), nc, nv, h1d, w, L->part);                                                \
+    } while (0)
+            if (nv <= 8)
+                CAD_GS(8);
+            else if (nv <= 16)
+                CAD_GS(16);
+            else if (nv <= 24)
+                CAD_GS(24);
+            else
+                CAD_GS(MAXM + 1);
+#undef CAD_GS
             hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h2d);
             hipLaunchKernelGGL(k_maxpy_sumsq, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0),
                                nc, nv, h2d, w, L->part);
