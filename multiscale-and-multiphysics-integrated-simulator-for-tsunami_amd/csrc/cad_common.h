@@ -298,6 +298,10 @@ void cad_mg_free(cad_ctx *c);
 int cad_mg_setup(cad_ctx *c);
 int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1);
 int cad_mg_apply32(cad_ctx *c, const float *y, float *x, int k0, int k1);
+// cad_mg_apply and r = y - A x on the V-cycle's FP32 level-0 matrix, with the
+// block partials of sum r^2 and of the row count at part[b * stride] and
+// part[b * stride + 1], b < REDUCE_BLOCKS
+int cad_mg_apply_resid(cad_ctx *c, const double *y, double *x, double *r, double *part, int stride, int k0, int k1);
 int cad_mg_time(cad_ctx *c, int reps, int fp64, float *ms_avg, double *bytes);
 int cad_milu_apply_impl(cad_ctx *c, const double *y, double *x);
 Static cad_static(const cad_ctx *c);

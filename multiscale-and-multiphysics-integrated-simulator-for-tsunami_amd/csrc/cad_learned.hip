@@ -485,35 +485,43 @@ int rms(cad_ctx *c, const double *r)
     return CAD_OK;
 }
 
-// features of r into the destination (network input or a sample buffer)
-int features(cad_ctx *c, const double *r, bf16 *dst, const UGeo &g)
+// features of r into the destination (network input or a sample buffer),
+// scaled by scal (set by rms, or from partials already in L->part)
+int features_scaled(cad_ctx *c, const double *r, bf16 *dst, const UGeo &g)
 {
     Learned *L = c->lrn;
-    RET_IF(rms(c, r));
     hipLaunchKernelGGL(k_features, dim3(4096), dim3(256), 0, c->stream, c->d, g, L->kb, L->nzr, r, L->scal,
                        c->ggv, c->f.ff, c->f.nf, c->f.indc, dst);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
 
+int features(cad_ctx *c, const double *r, bf16 *dst, const UGeo &g)
+{
+    RET_IF(rms(c, r));
+    return features_scaled(c, r, dst, g);
+}
+
 // z = P(v)
 int precond(cad_ctx *c, const double *v, double *z)
 {
     Learned *L = c->lrn;
-    const double *r = v;
     const double *z1 = nullptr;
-    if (L->mode == 1) {
-        double *t0 = Sv(c, 0), *t1 = Sv(c, 1);
-        RET_IF(cad_mg_apply(c, v, t0, c->mg_k0, c->mg_k1));
-        RET_IF(cad_ext_apply_a(c, t0, t1));
-        hipLaunchKernelGGL(k_vsub, dim3(VBLK), dim3(256), 0, c->stream, c->act_rng, c->f.indc, v, t1, t1);
-        CAD_CHECK(hipGetLastError());
-        r = t1;
-        z1 = t0;
-    }
     UGeo g = cad_unet_geo(L->net);
     g.nz = L->nzr;
-    RET_IF(features(c, r, reinterpret_cast<bf16 *>(cad_unet_input(L->net)), g));
+    bf16 *xin = reinterpret_cast<bf16 *>(cad_unet_input(L->net));
+    if (L->mode == 1) {
+        // t0 = V(v); t1 = v - A t0 on the V-cycle's FP32 level-0 matrix (the
+        // network's input needs no more), with its sum of squares
+        double *t0 = Sv(c, 0), *t1 = Sv(c, 1);
+        RET_IF(cad_mg_apply_resid(c, v, t0, t1, L->part, MAXM + 1, c->mg_k0, c->mg_k1));
+        hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, c->stream, L->part, REDUCE_BLOCKS, L->scal);
+        CAD_CHECK(hipGetLastError());
+        RET_IF(features_scaled(c, t1, xin, g));
+        z1 = t0;
+    } else {
+        RET_IF(features(c, v, xin, g));
+    }
     RET_IF(cad_unet_forward_impl(L->net));
     hipLaunchKernelGGL(k_scatter, dim3(VBLK), dim3(256), 0, c->stream, c->d, c->act_rng, L->nx, L->ny, L->kb,
                        L->nzr, c->f.indc, z1, L->scal, cad_unet_output(L->net), z);

@@ -281,6 +281,51 @@ __global__ void __launch_bounds__(256) k_mg_out(const MgLevelT<V> L, T *__restri
     x[nat(L.d, i, j, k)] = L.act[m] ? (T)L.x[m] : (T)0;
 }
 
+// k_mg_out and, on the same rows, the level's residual b - A x in V (the
+// learned preconditioner's input), with the block partials of its sum of
+// squares and of the row count at part[b * stride] and part[b * stride + 1]
+// (grid-stride over the K planes k0..k1, gridDim.x blocks of 256)
+template <typename V, typename T>
+__global__ void __launch_bounds__(256) k_mg_out_resid(const MgLevelT<V> L, T *__restrict__ x, T *__restrict__ r,
+                                                      int k0, int k1, double *__restrict__ part, int stride)
+{
+    const int ni = L.d.ni - 2, nj = L.d.nj - 2;
+    const long long total = (long long)ni * nj * (k1 - k0 + 1);
+    double a = 0.0, cnt = 0.0;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % ni) + 2;
+        const long long q = e / ni;
+        const int j = (int)(q % nj) + 2, k = (int)(q / nj) + k0;
+        const long long m = pos(L, i, j, k), n = nat(L.d, i, j, k);
+        if (!L.act[m]) {
+            x[n] = (T)0;
+            r[n] = (T)0;
+            continue;
+        }
+        const V xm = L.x[m];
+        const V rv = L.b[m] - L.ad[m] * xm - row_off(L, m, i, j, k);
+        x[n] = (T)xm;
+        r[n] = (T)rv;
+        a += (double)rv * (double)rv;
+        cnt += 1.0;
+    }
+    __shared__ double sh[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        cnt += __shfl_down(cnt, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[(long long)blockIdx.x * stride] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+        part[(long long)blockIdx.x * stride + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+    }
+}
+
 // The V-cycle below level t0 inside one workgroup: for each level down to the
 // coarsest the odd half-sweep (the even half was set with b), the coarse
 // right-hand side; MG_COARSE sweeps on the coarsest; then up: the even
@@ -510,9 +555,11 @@ int setup_levels(cad_ctx *c, MgLevelT<V> *mg, int nl)
 }
 
 // x = M^-1 y: one V-cycle from x = 0; k0..k1 limits level 0 to the K planes
-// that hold active cells
+// that hold active cells.  With r set, also r = y - A x on level 0's matrix
+// and the partials of its sum of squares (k_mg_out_resid, REDUCE_BLOCKS blocks).
 template <typename V, typename T>
-int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, int k1)
+int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, int k1, T *r = nullptr,
+           double *part = nullptr, int stride = 0)
 {
     const dim3 b(64, 4, 1);
     hipLaunchKernelGGL((k_mg_in<V, T>), lvl_grid(mg[0].d, k0, k1), b, 0, c->stream, mg[0], y, k0);
@@ -545,7 +592,11 @@ int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, 
         CAD_CHECK(hipGetLastError());
         RET_IF(smooth(c, F, a0, a1, 1));
     }
-    hipLaunchKernelGGL((k_mg_out<V, T>), lvl_grid(mg[0].d, k0, k1), b, 0, c->stream, mg[0], x, k0);
+    if (r)
+        hipLaunchKernelGGL((k_mg_out_resid<V, T>), dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, mg[0], x, r, k0, k1,
+                           part, stride);
+    else
+        hipLaunchKernelGGL((k_mg_out<V, T>), lvl_grid(mg[0].d, k0, k1), b, 0, c->stream, mg[0], x, k0);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
@@ -595,6 +646,11 @@ int cad_mg_setup(cad_ctx *c)
 int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1)
 {
     return vcycle(c, c->mg, c->mg_n, y, x, k0, k1);
+}
+
+int cad_mg_apply_resid(cad_ctx *c, const double *y, double *x, double *r, double *part, int stride, int k0, int k1)
+{
+    return vcycle(c, c->mg, c->mg_n, y, x, k0, k1, r, part, stride);
 }
 
 int cad_mg_apply32(cad_ctx *c, const float *y, float *x, int k0, int k1)
