@@ -910,14 +910,18 @@ int conv_ci(cad_unet *u, const UGeo &g, const ConvIn &in, const bf16 *wp, const 
         cad_set_error("cad_unet: no convolution tiling for %d voxels along x and %d channels", g.nx, cout);
         return CAD_ERR_ARG;
     }
-    const int mt = mtot % 2 == 0 ? 2 : 1;
     const int ng = g.nx / 16;
     auto waves = [&](int nt, int wx) {
         return (long long)((ng + nt * wx - 1) / (nt * wx)) * wx * 4 * ((g.ny + 3) / 4) * g.nb * g.nz;
     };
-    const int wx = mt == 2 ? conv_wx(CIN, 2) : conv_wx(CIN, 1);
+    const int mt = mtot % 2 == 0 ? 2 : 1;
+    const int wx = conv_wx(CIN, mt);
     int nt = mt == 2 || wx == 4 ? 2 : 4;   // 12 accumulator quads at most; 8 in blocks of 16 waves
     while (nt > 1 && waves(nt, wx) < conv_target_waves(u)) nt /= 2;
+    // a shape whose input tile and weights fit in LDS is preferred with one x
+    // group (measured: faster for 32 -> 32 channels at half resolution; one
+    // channel tile per launch for 64 input channels is not)
+    if (!conv_staged(CIN, mt, nt) && conv_staged(CIN, mt, 1)) nt = 1;
     const long long tile = 3LL * fold_kc(CIN) * 64 * 8;   // BF16 per channel tile
     for (int m0 = 0; m0 < mtot; m0 += mt) {
         const bf16 *w = wp + m0 * tile;
