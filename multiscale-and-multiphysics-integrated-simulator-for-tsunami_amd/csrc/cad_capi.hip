@@ -147,6 +147,9 @@ int cad_create(int numi, int numj, int numk, int numb, int device, cad_ctx **out
     cad_ctx *c = new cad_ctx();
     std::memset((void *)c, 0, sizeof(cad_ctx));
     c->device = device;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        c->ncu < 1)
+        c->ncu = 256;
     c->d.ni = numi;
     c->d.nj = numj;
     c->d.nk = numk;

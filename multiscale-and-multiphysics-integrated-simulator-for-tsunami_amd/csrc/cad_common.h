@@ -153,6 +153,7 @@ struct Fields {
 
 struct cad_ctx {
     int device;
+    int ncu;   // compute units of the device
     hipStream_t stream;
     Dims d;
     cad_params prm;

@@ -14,6 +14,7 @@
 //    listed cells at every stage of its iteration.
 #include "cad_common.h"
 
+#include <algorithm>
 #include <utility>
 
 namespace {
@@ -27,40 +28,39 @@ __device__ __forceinline__ long long fluid_off(const Dims &d, int c)
 }
 
 // ------------------------------------------------------------------ FCONV
-// Donor-acceptor flux through the face normal to A at cell (i,j,k) (index n),
-// vf_fconv.f:56-200: the flux, and NLIM(donor) = 1 where the flux is limited.
-template <int A>
-__device__ __forceinline__ double fconv_face(const Dims &d, const Static &s, const double *__restrict__ q,
-                                             const double *__restrict__ f,
-                                             const double *__restrict__ bcf,
-                                             const int *__restrict__ nf, int *nlim, double dtnow,
-                                             double flower, int i, int j, int k, long long n)
+// Donor-acceptor flux through a face normal to A, vf_fconv.f:56-200: the
+// flux, and NLIM(donor) = 1 where the flux is limited.  The face is at offset
+// FO (0: the cell's lower face, 1: its upper one) of a cell whose F, NF and
+// GGV along A at offset o (-2..+2) are fat(o), nat5(o), gat(o); q, gg, ind:
+// the face's velocity, porosity and index; idxf: the face's 1-based index
+// along A; inside: the face lies within the faces the reference updates
+// (2..NUMI on X, 2..NUMI-1 across); nc: the cell's linear index.
+template <int A, int FO, class FG, class NG, class GG>
+__device__ __forceinline__ double fconv_face_v(const Dims &d, const Static &s, const double *__restrict__ bcf,
+                                               int *nlim, double dtnow, double flower, double q, double gg, int ind,
+                                               FG fat, NG nat5, GG gat, int idxf, bool inside, long long nc)
 {
-    const int idx[3] = {i, j, k};
-    const int hi[3] = {A == 0 ? d.ni : d.ni - 1, A == 1 ? d.nj : d.nj - 1, A == 2 ? d.nk : d.nk - 1};
-    const int *ind = A == 0 ? s.indx : (A == 1 ? s.indy : s.indz);
-    const double *gga = A == 0 ? s.ggx : (A == 1 ? s.ggy : s.ggz);
-    if (i < 2 || j < 2 || k < 2 || i > hi[0] || j > hi[1] || k > hi[2] || ind[n] == -1) return 0.0;
-    const long long sa = stride(d, A);
-    double c = dtnow * gga[n] * q[n];
+    if (!inside || ind == -1) return 0.0;
+    double c = dtnow * gg * q;
     double ac = fabs(c);
-    int lsn, ild;
-    long long la, ld, ldm;
-    if (c >= 0.0) { lsn = 1; la = n; ld = n - sa; ldm = n - 2 * sa; ild = idx[A] - 1; }
-    else { lsn = -1; la = n - sa; ld = n; ldm = n + sa; ild = idx[A]; }
-    const int nn = nf[ld];
+    const bool pos = c >= 0.0;
+    // c >= 0: acceptor FO, donor FO-1, donor's upwind FO-2; else FO-1, FO, FO+1
+    const int lsn = pos ? 1 : -1;
+    const int ild = pos ? idxf - 1 : idxf;
+    const int ldo = pos ? FO - 1 : FO;
+    const int nn = pos ? nat5(FO - 1) : nat5(FO);
     double fd, fdmw, v, fa;
     if (nn != -1) {
-        fd = f[ld];
+        fd = pos ? fat(FO - 1) : fat(FO);
         fdmw = 1.0;
-        if (nf[ldm] != -1) fdmw = f[ldm];
-        v = met<A>(s, 2, ild) * s.ggv[ld];
+        if ((pos ? nat5(FO - 2) : nat5(FO + 1)) != -1) fdmw = pos ? fat(FO - 2) : fat(FO + 1);
+        v = met<A>(s, 2, ild) * (pos ? gat(FO - 1) : gat(FO));
     } else {
-        fd = ind[n] >= 1 ? bcf[ind[n] - 1] : 0.0;  // a donor obstacle face is a boundary face
+        fd = ind >= 1 ? bcf[ind - 1] : 0.0;  // a donor obstacle face is a boundary face
         fdmw = 1.0;
         v = met<A>(s, 2, ild);
     }
-    if (nf[la] != -1) fa = f[la];
+    if ((pos ? nat5(FO) : nat5(FO - 1)) != -1) fa = pos ? fat(FO) : fat(FO - 1);
     else fa = fd;
     double fad = fa;
     const bool tang = (nn == -1) || ((nn >= 1 && nn <= 6) && ((nn - 1) / 2 != A));
@@ -72,9 +72,41 @@ __device__ __forceinline__ double fconv_face(const Dims &d, const Static &s, con
     double cfx = dmaxd((fdm - fad) * ac - (fdm - fd) * v, 0.0);
     double fvx = fad * ac + cfx;
     double fvm = fd * v;
-    if (fvx > fvm) nlim[ld] = 1;
+    if (fvx > fvm) nlim[nc + ldo * stride(d, A)] = 1;
     return 0.0 - (double)lsn * dmind(fvx, fvm);
 }
+
+// One plane's tile of the staged VOF kernel: a block of VT_X x VT_Y cells
+// reads F and NF to +-2 in I and J, GGV to +-1, the X faces' U, GGX, INDX at
+// I and I+1 and the Y faces' V, GGY, INDY at J and J+1.  Tile positions
+// outside the arrays (ghost rows' neighbours) hold NF = -1 and zeros; inside
+// them, wrapped rows included, they hold what a direct read would.
+constexpr int VT_X = 64, VT_Y = 4;
+template <typename T, int W, int H>
+struct VRegion {
+    static constexpr int N = W * H, PER = (N + 255) / 256;
+    T r[PER];
+    // loads the region's cells of plane k at (ilo.., jlo..) into registers
+    __device__ __forceinline__ void fetch(const Dims &d, const T *__restrict__ a, T fill, int ilo, int jlo, int k)
+    {
+        const long long base = (long long)(k - 1) * d.nij;   // the plane
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = (int)threadIdx.x + p * 256;
+            const int ii = e % W, jj = e / W;
+            const long long n = base + ((ilo + ii - 1) + d.ni * (jlo + jj - 1));
+            r[p] = (e < N && n >= 0 && n < d.nc) ? a[n] : fill;
+        }
+    }
+    __device__ __forceinline__ void store(T *lds) const
+    {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = (int)threadIdx.x + p * 256;
+            if (e < N) lds[e] = r[p];
+        }
+    }
+};
 
 // VF_FCONV and VF_FEULER in one pass (vf_f1cal.f:124-160, vf_feuler.f:43-58;
 // static porosity, LOOPS=1, no source): each cell evaluates the fluxes of its
@@ -82,6 +114,9 @@ __device__ __forceinline__ double fconv_face(const Dims &d, const Static &s, con
 // cell, so both get its bits -- and writes its new F out of place (fo), so
 // that every flux reads the old F.  The flux arrays never reach HBM.  A face
 // with a limited flux sets NLIM of its donor from both sides (the same 1).
+// A block of 64 x 4 columns walks a run of kl K planes: each plane's I / J
+// neighbourhood is staged in LDS, the K neighbours of a thread's own column
+// roll through its registers, so every input is read from memory about once.
 __global__ void __launch_bounds__(256) k_fconv_feuler(Dims d, Static s, const double *__restrict__ u,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ w,
@@ -89,26 +124,138 @@ __global__ void __launch_bounds__(256) k_fconv_feuler(Dims d, Static s, const do
                                                       const double *__restrict__ bcf,
                                                       const int *__restrict__ nf, int *nlim,
                                                       double *__restrict__ fo, double dtnow,
-                                                      double flower, double loops)
+                                                      double flower, double loops, int kl)
 {
-    int i, j, k;
-    if (!cell_of_thread(d, i, j, k)) return;
-    const long long n = cid(d, i, j, k);
-    if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1 || nf[n] == -1) {
-        fo[n] = f[n];
-        return;
+    constexpr int FW = VT_X + 4, GW = VT_X + 2, UW = VT_X + 1;
+    __shared__ double sf[FW * (VT_Y + 4)], sg[GW * (VT_Y + 2)], su[UW * VT_Y], sgx[UW * VT_Y],
+        sv[VT_X * (VT_Y + 1)], sgy[VT_X * (VT_Y + 1)];
+    __shared__ int snf[FW * (VT_Y + 4)], six[UW * VT_Y], siy[VT_X * (VT_Y + 1)];
+    const int i0 = (int)blockIdx.x * VT_X + 1, j0 = (int)blockIdx.y * VT_Y + 1;
+    const int k0 = (int)blockIdx.z * kl + 1, k1 = min(d.nk, k0 + kl - 1);
+    const int tx = (int)threadIdx.x % VT_X, ty = (int)threadIdx.x / VT_X;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool cell = i <= d.ni && j <= d.nj;
+    VRegion<double, FW, VT_Y + 4> rf;
+    VRegion<int, FW, VT_Y + 4> rn;
+    VRegion<double, GW, VT_Y + 2> rg;
+    VRegion<double, UW, VT_Y> ru, rgx;
+    VRegion<int, UW, VT_Y> rix;
+    VRegion<double, VT_X, VT_Y + 1> rv, rgy;
+    VRegion<int, VT_X, VT_Y + 1> riy;
+    auto fetch = [&](int k) {
+        rf.fetch(d, f, 0.0, i0 - 2, j0 - 2, k);
+        rn.fetch(d, nf, -1, i0 - 2, j0 - 2, k);
+        rg.fetch(d, s.ggv, 0.0, i0 - 1, j0 - 1, k);
+        ru.fetch(d, u, 0.0, i0, j0, k);
+        rgx.fetch(d, s.ggx, 0.0, i0, j0, k);
+        rix.fetch(d, s.indx, -1, i0, j0, k);
+        rv.fetch(d, v, 0.0, i0, j0, k);
+        rgy.fetch(d, s.ggy, 0.0, i0, j0, k);
+        riy.fetch(d, s.indy, -1, i0, j0, k);
+    };
+    // the own column along K: F, NF at k-2..k+2, GGV at k-1..k+1, W, GGZ, INDZ at k, k+1
+    double cf[5], cg[3], cw[2], cgz[2];
+    int cn[5], ciz[2];
+    auto col = [&](int kk, auto &dst, auto *a, auto fill) {
+        const long long n = cid(d, i, j, kk);
+        dst = (cell && kk >= 1 && kk <= d.nk && n >= 0 && n < d.nc) ? a[n] : fill;
+    };
+#pragma unroll
+    for (int o = 0; o < 5; o++) {
+        col(k0 - 2 + o, cf[o], f, 0.0);
+        col(k0 - 2 + o, cn[o], nf, -1);
     }
-    const long long ni = d.ni, nij = d.nij;
-    const double fu0 = fconv_face<0>(d, s, u, f, bcf, nf, nlim, dtnow, flower, i, j, k, n);
-    const double fu1 = fconv_face<0>(d, s, u, f, bcf, nf, nlim, dtnow, flower, i + 1, j, k, n + 1);
-    const double fv0 = fconv_face<1>(d, s, v, f, bcf, nf, nlim, dtnow, flower, i, j, k, n);
-    const double fv1 = fconv_face<1>(d, s, v, f, bcf, nf, nlim, dtnow, flower, i, j + 1, k, n + ni);
-    const double fw0 = fconv_face<2>(d, s, w, f, bcf, nf, nlim, dtnow, flower, i, j, k, n);
-    const double fw1 = fconv_face<2>(d, s, w, f, bcf, nf, nlim, dtnow, flower, i, j, k + 1, n + nij);
-    double g0 = s.ggv[n] - (s.ggv[n] - s.ggv[n]) * loops;
-    double df = (met<0>(s, 4, i) * (fu1 - fu0) + met<1>(s, 4, j) * (fv1 - fv0)
-                 + met<2>(s, 4, k) * (fw1 - fw0) + dtnow * 0.0);
-    fo[n] = (g0 * f[n] + df) / s.ggv[n];
+#pragma unroll
+    for (int o = 0; o < 3; o++) col(k0 - 1 + o, cg[o], s.ggv, 0.0);
+#pragma unroll
+    for (int o = 0; o < 2; o++) {
+        col(k0 + o, cw[o], w, 0.0);
+        col(k0 + o, cgz[o], s.ggz, 0.0);
+        col(k0 + o, ciz[o], s.indz, -1);
+    }
+    for (int k = k0; k <= k1; k++) {
+        fetch(k);
+        __syncthreads();   // the previous plane's tile is read
+        rf.store(sf);
+        rn.store(snf);
+        rg.store(sg);
+        ru.store(su);
+        rgx.store(sgx);
+        rix.store(six);
+        rv.store(sv);
+        rgy.store(sgy);
+        riy.store(siy);
+        __syncthreads();
+        // the column's next values, in flight during this plane
+        double nfv, ngv, nwv, ngz;
+        int nnv, nizv;
+        col(k + 3, nfv, f, 0.0);
+        col(k + 3, nnv, nf, -1);
+        col(k + 2, ngv, s.ggv, 0.0);
+        col(k + 2, nwv, w, 0.0);
+        col(k + 2, ngz, s.ggz, 0.0);
+        col(k + 2, nizv, s.indz, -1);
+        if (cell) {
+            const long long n = cid(d, i, j, k);
+            if (i < 2 || i > d.ni - 1 || j < 2 || j > d.nj - 1 || k < 2 || k > d.nk - 1 || cn[2] == -1) {
+                fo[n] = cf[2];
+            } else {
+                // the cell in the F tile and in the GGV tile
+                const int fc = (ty + 2) * FW + tx + 2, gc = (ty + 1) * GW + tx + 1;
+                auto xf = [&](int o) { return sf[fc + o]; };
+                auto xn = [&](int o) { return snf[fc + o]; };
+                auto xg = [&](int o) { return sg[gc + o]; };
+                auto yf = [&](int o) { return sf[fc + o * FW]; };
+                auto yn = [&](int o) { return snf[fc + o * FW]; };
+                auto yg = [&](int o) { return sg[gc + o * GW]; };
+                auto zf = [&](int o) { return cf[o + 2]; };
+                auto zn = [&](int o) { return cn[o + 2]; };
+                auto zg = [&](int o) { return cg[o + 1]; };
+                const int ux = ty * UW + tx, vy = ty * VT_X + tx;
+                const int hi0 = d.ni, hi1 = d.nj - 1, hi2 = d.nk - 1;   // X faces run to ni
+                const int hj0 = d.ni - 1, hj1 = d.nj, hj2 = d.nk - 1;   // Y faces to nj
+                const int hk0 = d.ni - 1, hk1 = d.nj - 1, hk2 = d.nk;   // Z faces to nk
+                auto in_box = [](int a, int b, int c, int ha, int hb, int hc) {
+                    return !(a < 2 || b < 2 || c < 2 || a > ha || b > hb || c > hc);
+                };
+                const double fu0 = fconv_face_v<0, 0>(d, s, bcf, nlim, dtnow, flower, su[ux], sgx[ux], six[ux], xf,
+                                                      xn, xg, i, in_box(i, j, k, hi0, hi1, hi2), n);
+                const double fu1 = fconv_face_v<0, 1>(d, s, bcf, nlim, dtnow, flower, su[ux + 1], sgx[ux + 1],
+                                                      six[ux + 1], xf, xn, xg, i + 1,
+                                                      in_box(i + 1, j, k, hi0, hi1, hi2), n);
+                const double fv0 = fconv_face_v<1, 0>(d, s, bcf, nlim, dtnow, flower, sv[vy], sgy[vy], siy[vy], yf,
+                                                      yn, yg, j, in_box(i, j, k, hj0, hj1, hj2), n);
+                const double fv1 = fconv_face_v<1, 1>(d, s, bcf, nlim, dtnow, flower, sv[vy + VT_X],
+                                                      sgy[vy + VT_X], siy[vy + VT_X], yf, yn, yg, j + 1,
+                                                      in_box(i, j + 1, k, hj0, hj1, hj2), n);
+                const double fw0 = fconv_face_v<2, 0>(d, s, bcf, nlim, dtnow, flower, cw[0], cgz[0], ciz[0], zf, zn,
+                                                      zg, k, in_box(i, j, k, hk0, hk1, hk2), n);
+                const double fw1 = fconv_face_v<2, 1>(d, s, bcf, nlim, dtnow, flower, cw[1], cgz[1], ciz[1], zf, zn,
+                                                      zg, k + 1, in_box(i, j, k + 1, hk0, hk1, hk2), n);
+                double g0 = cg[1] - (cg[1] - cg[1]) * loops;
+                double df = (met<0>(s, 4, i) * (fu1 - fu0) + met<1>(s, 4, j) * (fv1 - fv0)
+                             + met<2>(s, 4, k) * (fw1 - fw0) + dtnow * 0.0);
+                fo[n] = (g0 * cf[2] + df) / cg[1];
+            }
+        }
+        // roll the column
+#pragma unroll
+        for (int o = 0; o < 4; o++) {
+            cf[o] = cf[o + 1];
+            cn[o] = cn[o + 1];
+        }
+        cf[4] = nfv;
+        cn[4] = nnv;
+        cg[0] = cg[1];
+        cg[1] = cg[2];
+        cg[2] = ngv;
+        cw[0] = cw[1];
+        cw[1] = nwv;
+        cgz[0] = cgz[1];
+        cgz[1] = ngz;
+        ciz[0] = ciz[1];
+        ciz[1] = nizv;
+    }
 }
 
 __device__ __forceinline__ double cell_vol(const Static &s, int i, int j, int k, double g)
@@ -825,8 +972,14 @@ int cad_vof_phase(cad_ctx *c, double *fsum, double *fcut)
     CAD_CHECK(hipMemsetAsync(f.nlim, 0, (size_t)d.nc * sizeof(int), st));
     // VF_FCONV and VF_FEULER (vf_f1cal.f:124-160), F out of place into the
     // spare array, which then becomes F
-    hipLaunchKernelGGL(k_fconv_feuler, g, b, 0, st, d, s, f.uu, f.vv, f.ww, f.ff, f.bcf, f.nf, f.nlim,
-                       c->flf[0], dt, c->prm.flower, (double)c->prm.loops);
+    // runs of K planes: about two resident blocks' worth of columns
+    const long long cols = (long long)((d.ni + VT_X - 1) / VT_X) * ((d.nj + VT_Y - 1) / VT_Y);
+    const int runs = (int)std::max(1LL, std::min<long long>(d.nk, (8LL * c->ncu + cols - 1) / cols));
+    const int kl = (d.nk + runs - 1) / runs;
+    const dim3 gv((unsigned)((d.ni + VT_X - 1) / VT_X), (unsigned)((d.nj + VT_Y - 1) / VT_Y),
+                  (unsigned)((d.nk + kl - 1) / kl));
+    hipLaunchKernelGGL(k_fconv_feuler, gv, dim3(256), 0, st, d, s, f.uu, f.vv, f.ww, f.ff, f.bcf, f.nf, f.nlim,
+                       c->flf[0], dt, c->prm.flower, (double)c->prm.loops, kl);
     CAD_CHECK(hipGetLastError());
     std::swap(f.ff, c->flf[0]);
     // VF_BWFFSF: fixed-level sides (vf_f1cal.f:180)
