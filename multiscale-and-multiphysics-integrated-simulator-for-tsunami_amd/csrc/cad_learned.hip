@@ -273,18 +273,24 @@ __global__ void __launch_bounds__(256) k_maxpy_mdot(const unsigned long long *rn
     for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
          n += (long long)gridDim.x * blockDim.x) {
         if (mask[n] == -1) continue;
-        double vn[NV];
+        // up to 16 basis values are held between the two loops; more would
+        // cost occupancy, so they are read again (from L2) past a compiler
+        // barrier
+        constexpr bool HOLD = NV <= 16;
+        double vn[HOLD ? NV : 1];
         double s = w[n];
 #pragma unroll
         for (int i = 0; i < NV; i++)
             if (i < nv) {
-                vn[i] = V[i * nc + n];
-                s += hs[i] * vn[i];
+                const double vi = V[i * nc + n];
+                if (HOLD) vn[i] = vi;
+                s += hs[i] * vi;
             }
         w[n] = s;
+        if (!HOLD) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
         for (int i = 0; i < NV; i++)
-            if (i < nv) acc[i] += vn[i] * s;
+            if (i < nv) acc[i] += (HOLD ? vn[i] : V[i * nc + n]) * s;
     }
     __shared__ double sh[4][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
