@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
 // bytes of a staged input plane tile: 6 rows (the block's 4 and their y
 // neighbours) by the block's 16*NT*WX voxels and their x neighbours
 __host__ __device__ constexpr int conv_tile_bytes(int cin, int nt, int wx) { return 6 * (16 * nt * wx + 2) * cin * 2; }
-constexpr int CONV_STAGED_MAX = 112 * 1024;
+constexpr int CONV_STAGED_MAX = 160 * 1024 - 2048;   // gfx950: 160 KB of LDS per workgroup
 // (in blocks of 16 waves, 128 registers a lane: the staging registers fit
 // beside 8 accumulator quads, not 12)
 __host__ __device__ constexpr bool conv_staged(int cin, int mt, int nt)
