@@ -52,13 +52,12 @@ __global__ void __launch_bounds__(256) k_pmodif(Dims d, const double *__restrict
     if (indc[n] != -1) p[n] = p[n] - rt * pt[n];
 }
 
-// Tangential velocity on faces with no fluid side, vf_bsuwt.f:47-463.
+// Tangential velocity on faces with no fluid side, vf_bsuwt.f:47-463: the
+// face normal to A at cell (i, j, k) of component q.
 template <int A>
-__global__ void __launch_bounds__(256) k_bsuwt(Dims d, Static s, const int *__restrict__ nf, double *q,
-                                               int ibsuw0)
+__device__ __forceinline__ void bsuwt_face(const Dims &d, const Static &s, const int *__restrict__ nf, double *q,
+                                           int ibsuw0, int i, int j, int k)
 {
-    int i, j, k;
-    if (!cell_of_thread(d, i, j, k)) return;
     const int idx[3] = {i, j, k};
     if (i < (A == 0 ? 3 : 2) || j < (A == 1 ? 3 : 2) || k < (A == 2 ? 3 : 2)) return;
     if (i > d.ni - 1 || j > d.nj - 1 || k > d.nk - 1) return;
@@ -103,6 +102,19 @@ __global__ void __launch_bounds__(256) k_bsuwt(Dims d, Static s, const int *__re
     if (nl == 8) q[n] = vr;
     else if (nr == 8) q[n] = vl;
     else q[n] = (dr * vl + dl * vr) / (dr + dl);
+}
+
+// The three components in one pass over the cells: each reads and writes
+// only its own velocity array, so the pass gives each component's result,
+// and the flags and face indices they share are read once.
+__global__ void __launch_bounds__(256) k_bsuwt(Dims d, Static s, const int *__restrict__ nf, double *u, double *v,
+                                               double *w, int ibsuw0)
+{
+    int i, j, k;
+    if (!cell_of_thread(d, i, j, k)) return;
+    bsuwt_face<0>(d, s, nf, u, ibsuw0, i, j, k);
+    bsuwt_face<1>(d, s, nf, v, ibsuw0, i, j, k);
+    bsuwt_face<2>(d, s, nf, w, ibsuw0, i, j, k);
 }
 
 // Tangential velocities across wall corners of surface cells, vf_bsuwt2.f:39-117.
@@ -223,9 +235,7 @@ int cad_launch_bsuwt(cad_ctx *c)
     Static s = cad_static(c);
     Fields &f = c->f;
     dim3 g = cell_grid(c->d), b = cell_block();
-    hipLaunchKernelGGL(k_bsuwt<0>, g, b, 0, c->stream, c->d, s, f.nf, f.uu, c->prm.ibsuw0);
-    hipLaunchKernelGGL(k_bsuwt<1>, g, b, 0, c->stream, c->d, s, f.nf, f.vv, c->prm.ibsuw0);
-    hipLaunchKernelGGL(k_bsuwt<2>, g, b, 0, c->stream, c->d, s, f.nf, f.ww, c->prm.ibsuw0);
+    hipLaunchKernelGGL(k_bsuwt, g, b, 0, c->stream, c->d, s, f.nf, f.uu, f.vv, f.ww, c->prm.ibsuw0);
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
