@@ -312,6 +312,9 @@ __device__ __forceinline__ double predict_face(const Dims &d, const Static &s, c
 // The three components of one cell's faces in one pass: the neighbourhood
 // each component reads (velocities, porosities, flags, viscosity) is shared,
 // so it comes from HBM once for all three instead of once per component.
+// A thread takes PRED_KU cells of its (I, J) column, so the K neighbours the
+// cells share are loaded once and all their loads are in flight together.
+constexpr int PRED_KU = 2;
 template <bool DF, bool SP>
 __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Damp dz,
                                                  const double *__restrict__ u,
@@ -326,12 +329,18 @@ __global__ void __launch_bounds__(256) k_predict(Dims d, Static s, DragDF df, Da
                                                  double *__restrict__ v2, double *__restrict__ w2,
                                                  cad_params prm)
 {
-    int i, j, k;
-    if (!cell_of_thread(d, i, j, k)) return;
-    const long long n = cid(d, i, j, k);
-    u2[n] = predict_face<0, DF, SP>(d, s, df, dz, u, v, w, p, bcu, nf, indc, prm, i, j, k, n);
-    v2[n] = predict_face<1, DF, SP>(d, s, df, dz, u, v, w, p, bcv, nf, indc, prm, i, j, k, n);
-    w2[n] = predict_face<2, DF, SP>(d, s, df, dz, u, v, w, p, bcw, nf, indc, prm, i, j, k, n);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > d.ni || j > d.nj) return;
+#pragma unroll
+    for (int kk = 0; kk < PRED_KU; kk++) {
+        const int k = blockIdx.z * PRED_KU + kk + 1;
+        if (k > d.nk) break;
+        const long long n = cid(d, i, j, k);
+        u2[n] = predict_face<0, DF, SP>(d, s, df, dz, u, v, w, p, bcu, nf, indc, prm, i, j, k, n);
+        v2[n] = predict_face<1, DF, SP>(d, s, df, dz, u, v, w, p, bcv, nf, indc, prm, i, j, k, n);
+        w2[n] = predict_face<2, DF, SP>(d, s, df, dz, u, v, w, p, bcw, nf, indc, prm, i, j, k, n);
+    }
 }
 
 // VF_BWUWN boundary-list pass (vf_bwuwn.f:84-254): normal velocity on wall and
@@ -378,6 +387,7 @@ void launch_predict(cad_ctx *c, const Static &s, const DragDF &df, const Damp &d
 {
     const Dims &d = c->d;
     dim3 b = cell_block(), g = cell_grid(d);
+    g.z = (unsigned)((d.nk + PRED_KU - 1) / PRED_KU);
     Fields &f = c->f;
     hipLaunchKernelGGL((k_predict<DF, SP>), g, b, 0, c->stream, d, s, df, dz, f.uu, f.vv, f.ww, f.pp,
                        f.bcu, f.bcv, f.bcw, f.nf, f.indc, c->u2, c->v2, c->w2, c->prm);
