@@ -94,9 +94,18 @@ struct ConvIn {
     long long delta;
     int cs, split;
 };
+// Output channels [0, split) to p0, the rest to p1 (p0 null: not stored).
+// Two optional extras of the same BF16 values: up, the 2x nearest-neighbour
+// upsampling into the next finer level (padded dims upx, upy, upz of ucs-
+// channel records), and y, the network's output layer y = ow[16] + sum_c
+// ow[c] h[c] in FP32 at the interior voxel number (a 16-channel output only).
 struct ConvOut {
     bf16 *p0, *p1;
     int cs0, cs1, split;
+    bf16 *up;
+    int ucs, upx, upy, upz;
+    const float *ow;
+    float *y;
 };
 __device__ __forceinline__ const bf16 *chan(const ConvIn &t, long long v, int c)
 {
@@ -107,6 +116,93 @@ __device__ __forceinline__ bf16 *chan(const ConvOut &t, long long v, int c)
     return c < t.split ? t.p0 + v * t.cs0 + c : t.p1 + v * t.cs1 + (c - t.split);
 }
 
+// v_permlane16_swap on two copies of x: rows (x0, x0, x2, x2) and (x1, x1, x3,
+// x3) of the wave's four 16-lane rows.  The second operand is an opaque copy:
+// given the same value twice, the compiler takes the two results as equal.
+__device__ __forceinline__ void rows_pair(unsigned x, unsigned &even, unsigned &odd)
+{
+    unsigned y = x;
+    __asm__("" : "+v"(y));
+    const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+    even = r[0];
+    odd = r[1];
+}
+// v_permlane32_swap on two copies of x: rows (x0, x1, x0, x1) and (x2, x3, x2, x3)
+__device__ __forceinline__ void halves_pair(unsigned x, unsigned &lo, unsigned &hi)
+{
+    unsigned y = x;
+    __asm__("" : "+v"(y));
+    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    lo = r[0];
+    hi = r[1];
+}
+
+// Stores out(o), complete in the accumulators a of a wave (NT groups of 16
+// voxels from x0 = 16 gi0 + n, row y, box b): bias, ReLU, BF16, then the
+// outputs out names: EPI 0 p0/p1 only, 1 also the upsampling (p0 may be
+// null), 2 also the output layer (p0 may be null).  Called uniformly across
+// the wave, so the output layer's channel sum may cross lanes.  ow: the
+// output layer's weights of this lane's four channels and its bias (EPI 2).
+template <int MT, int NT, bool RELU, int EPI>
+__device__ __forceinline__ void conv_store(const UGeo &g, const ConvOut &out, const f4 (&a)[MT][NT],
+                                           const float (&bs)[MT][4], const float (&ow)[5], int m0, int kg,
+                                           int gi0, int ng, int x0, int y, int o, int b)
+{
+    const long long v0 = vox(g, x0, y, o, b);
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int co = 16 * (m0 + m) + 4 * kg;
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            if (gi0 + j >= ng) continue;
+            bf16x4 ov;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                float val = a[m][j][c] + bs[m][c];
+                if (RELU) val = val > 0.f ? val : 0.f;
+                ov[c] = (bf16)val;
+            }
+            if (EPI == 0 || out.p0) *reinterpret_cast<bf16x4 *>(chan(out, v0 + 16 * j, co)) = ov;
+            if (EPI == 1) {
+                // Lane rows kg and kg ^ 1 hold channels 8 (kg / 2) .. +7 of the
+                // voxel between them; rows_pair gives every lane rows (2 r,
+                // 2 r + 1) of its pair, so each lane stores those 8
+                // channels (16 bytes) into fine voxel 2x + (kg & 1), and a
+                // wave's store covers whole fine records.
+                const uint2 w = __builtin_bit_cast(uint2, ov);
+                uint4 q;
+                rows_pair(w.x, q.x, q.z);
+                rows_pair(w.y, q.y, q.w);
+                const int xf = 2 * (x0 + 16 * j) + 1 + (kg & 1), yf = 2 * y + 1, zf = 2 * o + 1;
+                const int cf = 16 * (m0 + m) + 8 * (kg >> 1);
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const long long vf =
+                        xf + (long long)out.upx * ((yf + (d & 1)) +
+                                                   (long long)out.upy * ((zf + (d >> 1)) + (long long)out.upz * b));
+                    *reinterpret_cast<uint4 *>(out.up + vf * out.ucs + cf) = q;
+                }
+            }
+            if (EPI == 2) {
+                // channels 4 kg .. 4 kg + 3 here, the other twelve in the other
+                // lane rows: rows (0, 1) and (2, 3) summed in pairs, then the
+                // pair sums
+                float s = 0.f;
+#pragma unroll
+                for (int c = 0; c < 4; c++) s += ow[c] * (float)ov[c];
+                unsigned a, c;
+                rows_pair(__builtin_bit_cast(unsigned, s), a, c);
+                const float t = __builtin_bit_cast(float, a) + __builtin_bit_cast(float, c);
+                halves_pair(__builtin_bit_cast(unsigned, t), a, c);
+                s = __builtin_bit_cast(float, a) + __builtin_bit_cast(float, c);
+                if (kg == 0)
+                    out.y[(x0 + 16 * j) + (long long)g.nx * (y + (long long)g.ny * (o + (long long)g.nz * b))] =
+                        s + ow[4];
+            }
+        }
+    }
+}
+
 // 3x3x3 convolution, CIN -> 16*MT channels: output channel tiles m0..m0+MT-1
 // (wp points at tile m0).  A block is 4 rows (y) by WX wave columns (x) of
 // NT groups of 16 voxels, WX = conv_wx(CIN, MT).  Tiles (x, y, z run of zlen
@@ -114,7 +210,7 @@ __device__ __forceinline__ bf16 *chan(const ConvOut &t, long long v, int c)
 // tiles [k per_xcd, (k+1) per_xcd).  Groups past nx and rows past ny are
 // idle.  wp: packed A fragments [MT][3 dz][KC][64 lanes] of 8 BF16.  bias may
 // be null.
-template <int CIN, int MT, int NT, bool RELU>
+template <int CIN, int MT, int NT, bool RELU, int EPI>
 __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 ? 3 : 4)
     k_conv3(UGeo g, ConvIn in, const bf16x8 *__restrict__ wp, const float *__restrict__ bias, ConvOut out, int m0,
             int ntx, int nty, int nzc, int zlen, int per_xcd, int ntiles)
@@ -190,6 +286,12 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int i = 0; i < 4; i++) bs[m][i] = bias ? bias[16 * (m0 + m) + 4 * kg + i] : 0.f;
+    float ow[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if constexpr (EPI == 2) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) ow[c] = out.ow[4 * kg + c];
+        ow[4] = out.ow[16];
+    }
 #pragma unroll 1
     for (int p0 = 0; p0 < np; p0 += 3) {
 #pragma unroll
@@ -216,25 +318,8 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
             }
             // out(p - 1) is complete: slot u % 3
             const int o = z0 - 2 + pi;
-            if (o >= z0 && o < z1) {
-                const long long v0 = vox(g, x0, y, o, b);
-#pragma unroll
-                for (int m = 0; m < MT; m++) {
-                    const int co = 16 * (m0 + m) + 4 * kg;
-#pragma unroll
-                    for (int j = 0; j < NT; j++) {
-                        if (gi0 + j >= ng) continue;
-                        bf16x4 ov;
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            float val = acc[u % 3][m][j][c] + bs[m][c];
-                            if (RELU) val = val > 0.f ? val : 0.f;
-                            ov[c] = (bf16)val;
-                        }
-                        *reinterpret_cast<bf16x4 *>(chan(out, v0 + 16 * j, co)) = ov;
-                    }
-                }
-            }
+            if (o >= z0 && o < z1)
+                conv_store<MT, NT, RELU, EPI>(g, out, acc[u % 3], bs, ow, m0, kg, gi0, ng, x0, y, o, b);
 #pragma unroll
             for (int m = 0; m < MT; m++)
 #pragma unroll
@@ -260,7 +345,7 @@ __host__ __device__ constexpr bool conv_staged(int cin, int mt, int nt)
 // the waves read their B fragments from there; the next plane's chunks are
 // loaded into registers while the current plane is computed.  Same tiling,
 // numbering and arguments as k_conv3.
-template <int CIN, int MT, int NT, bool RELU>
+template <int CIN, int MT, int NT, bool RELU, int EPI>
 __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 ? 3 : 4)
     k_conv3s(UGeo g, ConvIn in, const bf16x8 *__restrict__ wp, const float *__restrict__ bias, ConvOut out,
              int m0, int ntx, int nty, int nzc, int zlen, int per_xcd, int ntiles)
@@ -328,6 +413,12 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int i = 0; i < 4; i++) bs[m][i] = bias ? bias[16 * (m0 + m) + 4 * kg + i] : 0.f;
+    float ow[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if constexpr (EPI == 2) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) ow[c] = out.ow[4 * kg + c];
+        ow[4] = out.ow[16];
+    }
     // the wave's first B chunk: row wy (tap dy = -1), voxel wxi*16*NT + n (tap dx = -1)
     const int lb = (wy * TW + wxi * 16 * NT + n) * CH;
 #pragma unroll 1
@@ -366,25 +457,9 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
                 }
                 // out(p - 1) is complete: slot u % 3
                 const int o = z0 - 2 + pi;
-                if (o >= z0 && o < z1) {
-                    const long long v0 = vox(g, 16 * gi0 + n, y, o, b);
-#pragma unroll
-                    for (int m = 0; m < MT; m++) {
-                        const int co = 16 * (m0 + m) + 4 * kg;
-#pragma unroll
-                        for (int j = 0; j < NT; j++) {
-                            if (gi0 + j >= ng) continue;
-                            bf16x4 ov;
-#pragma unroll
-                            for (int c = 0; c < 4; c++) {
-                                float val = acc[u % 3][m][j][c] + bs[m][c];
-                                if (RELU) val = val > 0.f ? val : 0.f;
-                                ov[c] = (bf16)val;
-                            }
-                            *reinterpret_cast<bf16x4 *>(chan(out, v0 + 16 * j, co)) = ov;
-                        }
-                    }
-                }
+                if (o >= z0 && o < z1)
+                    conv_store<MT, NT, RELU, EPI>(g, out, acc[u % 3], bs, ow, m0, kg, gi0, ng, 16 * gi0 + n, y, o,
+                                                  b);
 #pragma unroll
                 for (int m = 0; m < MT; m++)
 #pragma unroll
@@ -593,17 +668,8 @@ __global__ void k_pool_bwd(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ 
     *p = o;
 }
 
-// nearest-neighbour upsampling of C channels (coarse gc -> fine gf)
-__global__ void k_up(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ in, int cs_in, int C,
-                     bf16 *__restrict__ out, int cs_out)
-{
-    int x, c0, y, z, b;
-    if (!row_elem(gf, segs, C / 8, x, c0, y, z, b)) return;
-    *reinterpret_cast<bf16x8 *>(out + vox(gf, x, y, z, b) * cs_out + c0) =
-        *reinterpret_cast<const bf16x8 *>(in + vox(gc, x >> 1, y >> 1, z >> 1, b) * cs_in + c0);
-}
-
-// backward of k_up: coarse gradient = sum of its 8 fine voxels' gradients
+// backward of the 2x nearest-neighbour upsampling (stored by the coarse
+// layer's convolution): coarse gradient = sum of its 8 fine voxels' gradients
 __global__ void k_up_bwd(int segs, UGeo gc, UGeo gf, const bf16 *__restrict__ df, int cs_f, int C,
                          bf16 *__restrict__ dc, int cs_c)
 {
@@ -639,24 +705,8 @@ __global__ void k_relu_bwd(int segs, UGeo g, const bf16 *__restrict__ a, int cs_
     *p = o;
 }
 
-// output layer: y[u] = b + sum_c w[c] h[v][c] (FP32, interior order u)
-__global__ void k_out(int segs, UGeo g, const bf16 *__restrict__ h, const float *__restrict__ w,
-                      float *__restrict__ y)
-{
-    int x, c0, yq, z, b;
-    if (!row_elem(g, segs, 1, x, c0, yq, z, b)) return;
-    const long long v = vox(g, x, yq, z, b);
-    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(h + v * 16);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(h + v * 16 + 8);
-    float s = w[16];
-#pragma unroll
-    for (int i = 0; i < 8; i++) s += w[i] * (float)a0[i];
-#pragma unroll
-    for (int i = 0; i < 8; i++) s += w[8 + i] * (float)a1[i];
-    y[x + (long long)g.nx * (yq + (long long)g.ny * (z + (long long)g.nz * b))] = s;
-}
-
-// backward of k_out: dh[v][c] = w[c] dy[u]; dw[c] += sum dy h[c]; db += sum dy
+// backward of the output layer y = b + sum_c w[c] h[c] (applied by the last
+// convolution): dh[v][c] = w[c] dy[u]; dw[c] += sum dy h[c]; db += sum dy
 __global__ void __launch_bounds__(256) k_out_bwd(UGeo g, const bf16 *__restrict__ h,
                                                  const float *__restrict__ w,
                                                  const float *__restrict__ dy, bf16 *__restrict__ dh,
@@ -879,21 +929,32 @@ int conv_launch(cad_unet *u, const UGeo &g, const ConvIn &in, const bf16 *wp, co
     const int per_xcd = (int)((ntiles + 7) / 8);
     const dim3 grid((unsigned)(8 * per_xcd)), blk(256 * WX);
     const bf16x8 *w = reinterpret_cast<const bf16x8 *>(wp);
-    if constexpr (conv_staged(CIN, MT, NT)) {
+    // the fused epilogues exist where the network has them: the upsampled
+    // layers' (64 and 32 input channels) and the last layer's (16)
+    const int epi = out.up ? 1 : out.y ? 2 : 0;
+#define CAD_CONV(R, E)                                                                                       \
+    do {                                                                                                     \
+        if constexpr (conv_staged(CIN, MT, NT))                                                             \
+            hipLaunchKernelGGL((k_conv3s<CIN, MT, NT, R, E>), grid, blk, 0, u->stream, g, in, w, bias, out, m0, \
+                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);                                  \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_conv3<CIN, MT, NT, R, E>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,  \
+                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);                                  \
+    } while (0)
+    if (epi == 0) {
         if (relu)
-            hipLaunchKernelGGL((k_conv3s<CIN, MT, NT, true>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
-                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+            CAD_CONV(true, 0);
         else
-            hipLaunchKernelGGL((k_conv3s<CIN, MT, NT, false>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
-                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+            CAD_CONV(false, 0);
+    } else if (!relu || (epi == 1 && CIN != 32 && CIN != 64) || (epi == 2 && CIN != 16)) {
+        cad_set_error("cad_unet: no fused output for a layer of %d input channels", CIN);
+        return CAD_ERR_ARG;
+    } else if (epi == 1) {
+        if constexpr (CIN == 32 || CIN == 64) CAD_CONV(true, 1);
     } else {
-        if (relu)
-            hipLaunchKernelGGL((k_conv3<CIN, MT, NT, true>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
-                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
-        else
-            hipLaunchKernelGGL((k_conv3<CIN, MT, NT, false>), grid, blk, 0, u->stream, g, in, w, bias, out, m0,
-                               ntx, nty, nzc, zlen, per_xcd, (int)ntiles);
+        if constexpr (CIN == 16) CAD_CONV(true, 2);
     }
+#undef CAD_CONV
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
 }
@@ -908,6 +969,10 @@ int conv_ci(cad_unet *u, const UGeo &g, const ConvIn &in, const bf16 *wp, const 
     const int mtot = cout / 16;
     if (mtot < 1 || cout % 16 || g.nx % 16) {
         cad_set_error("cad_unet: no convolution tiling for %d voxels along x and %d channels", g.nx, cout);
+        return CAD_ERR_ARG;
+    }
+    if (out.y && cout != 16) {
+        cad_set_error("cad_unet: the output layer follows a 16-channel layer, not %d", cout);
         return CAD_ERR_ARG;
     }
     const int ng = g.nx / 16;
@@ -954,11 +1019,27 @@ int conv(cad_unet *u, const UGeo &g, int cin, const ConvIn &in, const bf16 *wp, 
     return CAD_ERR_ARG;
 }
 
-int layer_fwd(cad_unet *u, int l, const ConvIn &in, bf16 *out, int cs_out)
+int layer_fwd(cad_unet *u, int l, const ConvIn &in, const ConvOut &out)
 {
     const LayerDef &L = LAYERS[l];
-    return conv(u, u->g[L.level], L.cin, in, u->wp[l], u->p + u->bofs[l], one_out(out, cs_out, L.cout), L.cout,
-                true);
+    return conv(u, u->g[L.level], L.cin, in, u->wp[l], u->p + u->bofs[l], out, L.cout, true);
+}
+int layer_fwd(cad_unet *u, int l, const ConvIn &in, bf16 *out, int cs_out)
+{
+    return layer_fwd(u, l, in, one_out(out, cs_out, LAYERS[l].cout));
+}
+// an output of c channels stored at p (null: not stored) and upsampled into
+// the buffer up of the level below (records of cs_up channels)
+ConvOut up_out(const cad_unet *u, int level, bf16 *p, int cs, int c, bf16 *up, int cs_up)
+{
+    ConvOut o = one_out(p, cs, c);
+    const UGeo &gf = u->g[level - 1];
+    o.up = up;
+    o.ucs = cs_up;
+    o.upx = gf.px;
+    o.upy = gf.py;
+    o.upz = gf.pz;
+    return o;
 }
 
 // gradient of the layer's input (from dY at its output) and of its weights
@@ -1038,15 +1119,18 @@ int cad_unet_forward_impl(cad_unet *u)
     RET_IF(ewr(u, k_pool, g[2], 4, g[2], g[1], u->cat2b, 32, 32, u->p2, 32));
     RET_IF(layer_fwd(u, B3A, one_in(u->p2, 32, 32), u->h3a, 64));
     RET_IF(layer_fwd(u, B3B, one_in(u->h3a, 64, 64), u->h3b, 64));
-    RET_IF(layer_fwd(u, U3, one_in(u->h3b, 64, 64), u->h3c, 32));
-    RET_IF(ewr(u, k_up, g[1], 4, g[2], g[1], u->h3c, 32, 32, u->cat2a, 32));
+    // u3 and u2 store their outputs upsampled into the decoders' inputs, and
+    // d1b applies the output layer; the coarse outputs and h1e themselves
+    // are kept for the backward pass only
+    RET_IF(layer_fwd(u, U3, one_in(u->h3b, 64, 64), up_out(u, 2, u->train ? u->h3c : nullptr, 32, 32, u->cat2a, 32)));
     RET_IF(layer_fwd(u, D2A, cat_in(u->cat2a, u->cat2b - u->cat2a, 32, 32), u->h2d, 32));
     RET_IF(layer_fwd(u, D2B, one_in(u->h2d, 32, 32), u->h2e, 32));
-    RET_IF(layer_fwd(u, U2, one_in(u->h2e, 32, 32), u->h2f, 16));
-    RET_IF(ewr(u, k_up, g[0], 2, g[1], g[0], u->h2f, 16, 16, u->cat1a, 16));
+    RET_IF(layer_fwd(u, U2, one_in(u->h2e, 32, 32), up_out(u, 1, u->train ? u->h2f : nullptr, 16, 16, u->cat1a, 16)));
     RET_IF(layer_fwd(u, D1A, cat_in(u->cat1a, u->cat1b - u->cat1a, 16, 16), u->h1d, 16));
-    RET_IF(layer_fwd(u, D1B, one_in(u->h1d, 16, 16), u->h1e, 16));
-    RET_IF(ewr(u, k_out, g[0], 1, g[0], u->h1e, u->p + u->out_ofs, u->y));
+    ConvOut o1 = one_out(u->train ? u->h1e : nullptr, 16, 16);
+    o1.ow = u->p + u->out_ofs;
+    o1.y = u->y;
+    RET_IF(layer_fwd(u, D1B, one_in(u->h1d, 16, 16), o1));
     return CAD_OK;
 }
 

@@ -56,8 +56,9 @@ def test_unet_forward_and_gradients_match_torch(pkg, box, nb):
         net.forward(y)
         y = y.cpu().numpy().reshape(nb, nz, ny, nx)
         loss, g = net.gradients(torch.from_numpy(e.ravel()).cuda(), torch.from_numpy(m.ravel()).cuda())
-    # an inference network (upsampling read inside the decoder's convolution,
-    # output layer fused into the last one) gives the same output bit for bit
+    # an inference network (which stores neither the coarse outputs of the
+    # layers it upsamples nor the last hidden layer, only their fused
+    # products) gives the same output bit for bit
     with U.UNet(nx, ny, nz, nb, train=False) as net:
         net.set_params(p)
         net.set_input(xp.cuda())
