@@ -12,8 +12,9 @@
 // correction the V-cycle left out.  Mode 0 is the network alone, z = s N(v/s).
 // The network is nonlinear, hence flexible GMRES, which keeps the
 // preconditioned vectors Z beside the Arnoldi basis V.  Orthogonalisation is
-// classical Gram-Schmidt done twice, as batched dot products (one host read
-// per pass).  The stop test is on the true relative residual
+// classical Gram-Schmidt done twice with the second pass delayed by one step:
+// per step one reduction and one update pass over the basis, one host read.
+// The stop test is on the true relative residual
 // ||BB - A x|| / ||BB|| <= CGEPSR, recomputed in FP64 at every restart.  A
 // cycle that does not halve the residual, or more than ICGMAX iterations,
 // hands over to VF_M1BCGS from the current x with the same true-residual stop.
@@ -37,6 +38,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int MAXM = 32;            // largest restart length
 constexpr int VBLK = 2048;          // blocks of the vector kernels
+// columns of a block's Gram-Schmidt partials: 2 (MAXM-1) + 3 sums, and the
+// step's 2 (MAXM-1) + 6 coefficients
+constexpr int GS_STRIDE = 2 * MAXM + 4;
 
 }  // namespace
 
@@ -47,10 +51,11 @@ struct Learned {
     int m;                          // restart length
     double *vecs;                   // V (m+1), Z (m), then 6 scratch vectors
     int nvec;
-    double *part;                   // REDUCE_BLOCKS x (MAXM+1) partial sums
-    double *hdev;                   // Gram-Schmidt coefficients of both passes and |w|^2: 3 (MAXM+1)
+    double *part;                   // REDUCE_BLOCKS x GS_STRIDE partial sums
+    double *hdev;                   // a step's Gram-Schmidt sums, then its coefficients: 2 GS_STRIDE
     double *scal;                   // [0] s, [1] 1/s, [2] active count
     double *hpin;                   // pinned host mirror of hdev
+    hipEvent_t gs_read;             // the step's sums and coefficients are in hpin
     float *y;                       // network output
     unsigned long long prev_rng[2]; // active range of the previous solve
     int kb, nzr;                    // the current box placement
@@ -185,54 +190,139 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// partials of (V_i, w) for i < nv over the active cells (masked), block
-// order; NV >= nv bounds the registers the sums take
+// The delayed Gram-Schmidt's reduction, one pass over the active cells: with
+// the final basis V_0..V_{nv-1}, the candidate u (the previous step's new
+// vector, orthogonalised once) and w = A z, the block partials of
+//   (V_i, u) in columns 0..nv-1, (V_i, w) in nv..2nv-1, (u, u), (u, w), (w, w)
+// in 2nv..2nv+2 (row stride GS_STRIDE).  NV >= nv bounds the registers.
 template <int NV>
-__global__ void __launch_bounds__(256) k_mdot(const unsigned long long *rng, const int *__restrict__ mask,
-                                              const double *__restrict__ V, long long nc, int nv,
-                                              const double *__restrict__ w, double *__restrict__ part)
+__global__ void __launch_bounds__(256) k_dgs_reduce(const unsigned long long *rng, const int *__restrict__ mask,
+                                                    const double *__restrict__ V, long long nc, int nv,
+                                                    const double *__restrict__ u, const double *__restrict__ w,
+                                                    double *__restrict__ part)
 {
-    double acc[NV];
+    double as[NV], ap[NV], uu = 0.0, uw = 0.0, ww = 0.0;
 #pragma unroll
-    for (int i = 0; i < NV; i++) acc[i] = 0.0;
+    for (int i = 0; i < NV; i++) as[i] = ap[i] = 0.0;
     const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
     for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
          n += (long long)gridDim.x * blockDim.x) {
         if (mask[n] == -1) continue;
-        const double wn = w[n];
+        const double un = u[n], wn = w[n];
+        uu += un * un;
+        uw += un * wn;
+        ww += wn * wn;
 #pragma unroll
         for (int i = 0; i < NV; i++)
-            if (i < nv) acc[i] += V[i * nc + n] * wn;
+            if (i < nv) {
+                const double vi = V[i * nc + n];
+                as[i] += vi * un;
+                ap[i] += vi * wn;
+            }
     }
-    __shared__ double sh[4][NV];
+    __shared__ double sh[4][2 * NV + 3];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    auto put = [&](int col, double v) {
+        const double t = wave_sum(v);
+        if (lane == 0) sh[wv][col] = t;
+    };
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
+    for (int i = 0; i < NV; i++)
         if (i < nv) {
-            const double s = wave_sum(acc[i]);
-            if (lane == 0) sh[wv][i] = s;
+            put(i, as[i]);
+            put(nv + i, ap[i]);
         }
-    }
+    put(2 * nv, uu);
+    put(2 * nv + 1, uw);
+    put(2 * nv + 2, ww);
     __syncthreads();
-    if ((int)threadIdx.x < nv)
-        part[(long long)blockIdx.x * (MAXM + 1) + threadIdx.x] =
-            sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    if ((int)threadIdx.x < 2 * nv + 3)
+        part[(long long)blockIdx.x * GS_STRIDE + threadIdx.x] =
+            (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
-// out[i] = sum over the nblk block partials of column i: block i of the grid
-// sums column i, each thread a fixed stride of partials, then a fixed tree
-__global__ void __launch_bounds__(256) k_mdot_final(const double *__restrict__ part, int nblk, int nv,
+// out[i] = sum over the nblk block partials of column i (row stride
+// stride): block i of the grid sums column i, each thread a fixed stride of
+// partials, then a fixed tree
+__global__ void __launch_bounds__(256) k_cols_final(const double *__restrict__ part, int nblk, int ncol, int stride,
                                                     double *__restrict__ out)
 {
     const int i = blockIdx.x;
-    if (i >= nv) return;
+    if (i >= ncol) return;
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long long)b * (MAXM + 1) + i];
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long long)b * stride + i];
     __shared__ double sh[4];
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) out[i] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// The step's coefficients from the reduction's sums (s_i, p_i, uu, uw, ww),
+// one thread, fixed order.  The candidate u = V s + alpha v with
+// alpha^2 = uu - |s|^2; w's coefficient on the new v is h = (uw - s.p) / alpha
+// and its remainder's norm beta^2 = ww - |p|^2 - h^2 (Pythagoras).  cf:
+// s_i / alpha, p_i / beta, 1 / alpha, h / beta, 1 / beta (the update pass's
+// factors), then alpha, h, beta.  A non-positive alpha^2 or beta^2 (the
+// candidate or the remainder in the basis's span) gives 0 there and in the
+// factors that divide by it.
+__global__ void k_dgs_coef(int nv, const double *__restrict__ sum, double *__restrict__ cf)
+{
+    if (threadIdx.x != 0) return;
+    double ss = 0.0, sp = 0.0, pp = 0.0;
+    for (int i = 0; i < nv; i++) {
+        ss += sum[i] * sum[i];
+        sp += sum[i] * sum[nv + i];
+        pp += sum[nv + i] * sum[nv + i];
+    }
+    const double a2 = sum[2 * nv] - ss;
+    const double al = a2 > 0.0 ? sqrt(a2) : 0.0;
+    const double h = al > 0.0 ? (sum[2 * nv + 1] - sp) / al : 0.0;
+    const double b2 = sum[2 * nv + 2] - pp - h * h;
+    const double be = b2 > 0.0 ? sqrt(b2) : 0.0;
+    const double ia = al > 0.0 ? 1.0 / al : 0.0, ib = be > 0.0 ? 1.0 / be : 0.0;
+    for (int i = 0; i < nv; i++) {
+        cf[i] = sum[i] * ia;
+        cf[nv + i] = sum[nv + i] * ib;
+    }
+    cf[2 * nv] = ia;
+    cf[2 * nv + 1] = h * ib;
+    cf[2 * nv + 2] = ib;
+    cf[2 * nv + 3] = al;
+    cf[2 * nv + 4] = h;
+    cf[2 * nv + 5] = be;
+}
+
+// The delayed Gram-Schmidt's update, one pass over the active cells: the
+// candidate becomes the final basis vector v = (u - sum_i s_i V_i) / alpha (in
+// place of u), and w the next candidate (w - sum_i p_i V_i - h v) / beta (in
+// place of w), with the factors of k_dgs_coef.
+template <int NV>
+__global__ void __launch_bounds__(256) k_dgs_update(const unsigned long long *rng, const int *__restrict__ mask,
+                                                    const double *__restrict__ V, long long nc, int nv,
+                                                    const double *__restrict__ cf, double *__restrict__ u,
+                                                    double *__restrict__ w)
+{
+    __shared__ double c[2 * NV + 3];
+    for (int i = threadIdx.x; i < 2 * nv + 3; i += blockDim.x) c[i] = cf[i];
+    __syncthreads();
+    const double ia = c[2 * nv], hb = c[2 * nv + 1], ib = c[2 * nv + 2];
+    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
+    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
+         n += (long long)gridDim.x * blockDim.x) {
+        if (mask[n] == -1) continue;
+        double vs = 0.0, vp = 0.0;
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            if (i < nv) {
+                const double vi = V[i * nc + n];
+                vs += c[i] * vi;
+                vp += c[nv + i] * vi;
+            }
+        const double v = u[n] * ia - vs;
+        u[n] = v;
+        w[n] = w[n] * ib - vp - hb * v;
+    }
 }
 
 // w += sign * sum_i h_i V_i (active cells)
@@ -252,87 +342,6 @@ __global__ void __launch_bounds__(256) k_maxpy(const unsigned long long *rng, co
         for (int i = 0; i < nv; i++) s += hs[i] * V[i * nc + n];
         w[n] = s;
     }
-}
-
-// The second Gram-Schmidt pass fused with the first pass's update: on the
-// active cells w -= sum_i h_i V_i (h: the first pass's coefficients), then
-// the partials of (V_i, w) with the updated w, block order as k_mdot
-template <int NV>
-__global__ void __launch_bounds__(256) k_maxpy_mdot(const unsigned long long *rng, const int *__restrict__ mask,
-                                                    const double *__restrict__ V, long long nc, int nv,
-                                                    const double *__restrict__ h, double *__restrict__ w,
-                                                    double *__restrict__ part)
-{
-    __shared__ double hs[NV];
-    if ((int)threadIdx.x < nv) hs[threadIdx.x] = -h[threadIdx.x];
-    __syncthreads();
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; i++) acc[i] = 0.0;
-    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
-    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
-         n += (long long)gridDim.x * blockDim.x) {
-        if (mask[n] == -1) continue;
-        // up to 16 basis values are held between the two loops; more would
-        // cost occupancy, so they are read again (from L2) past a compiler
-        // barrier
-        constexpr bool HOLD = NV <= 16;
-        double vn[HOLD ? NV : 1];
-        double s = w[n];
-#pragma unroll
-        for (int i = 0; i < NV; i++)
-            if (i < nv) {
-                const double vi = V[i * nc + n];
-                if (HOLD) vn[i] = vi;
-                s += hs[i] * vi;
-            }
-        w[n] = s;
-        if (!HOLD) __asm__ __volatile__("" ::: "memory");
-#pragma unroll
-        for (int i = 0; i < NV; i++)
-            if (i < nv) acc[i] += (HOLD ? vn[i] : V[i * nc + n]) * s;
-    }
-    __shared__ double sh[4][NV];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-        if (i < nv) {
-            const double t = wave_sum(acc[i]);
-            if (lane == 0) sh[wv][i] = t;
-        }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nv)
-        part[(long long)blockIdx.x * (MAXM + 1) + threadIdx.x] =
-            sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
-}
-
-// The second pass's update fused with the norm: on the active cells
-// w -= sum_i h_i V_i, then the partials of (w, w) in column 0, block order
-__global__ void __launch_bounds__(256) k_maxpy_sumsq(const unsigned long long *rng, const int *__restrict__ mask,
-                                                     const double *__restrict__ V, long long nc, int nv,
-                                                     const double *__restrict__ h, double *__restrict__ w,
-                                                     double *__restrict__ part)
-{
-    __shared__ double hs[MAXM + 1];
-    if ((int)threadIdx.x < nv) hs[threadIdx.x] = -h[threadIdx.x];
-    __syncthreads();
-    double acc = 0.0;
-    const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
-    for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
-         n += (long long)gridDim.x * blockDim.x) {
-        if (mask[n] == -1) continue;
-        double s = w[n];
-        for (int i = 0; i < nv; i++) s += hs[i] * V[i * nc + n];
-        w[n] = s;
-        acc += s * s;
-    }
-    __shared__ double sh[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const double t = wave_sum(acc);
-    if (lane == 0) sh[wv] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) part[(long long)blockIdx.x * (MAXM + 1)] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // w *= a (active cells)
@@ -457,14 +466,15 @@ int learned_init(cad_ctx *c)
     RET_IF(cad_unet_create(L->nx, L->ny, L->nzmax, 1, 0, c->stream, &L->net));
     L->nvec = 2 * MAXM + 1 + 6;
     RET_IF(lalloc((void **)&L->vecs, (size_t)L->nvec * d.nc * sizeof(double)));
-    RET_IF(lalloc((void **)&L->part, (size_t)REDUCE_BLOCKS * (MAXM + 1) * sizeof(double)));
-    RET_IF(lalloc((void **)&L->hdev, 3 * (MAXM + 1) * sizeof(double)));
+    RET_IF(lalloc((void **)&L->part, (size_t)REDUCE_BLOCKS * GS_STRIDE * sizeof(double)));
+    RET_IF(lalloc((void **)&L->hdev, 2 * GS_STRIDE * sizeof(double)));
     RET_IF(lalloc((void **)&L->scal, 4 * sizeof(double)));
     RET_IF(lalloc((void **)&L->y, (size_t)L->nx * L->ny * L->nzmax * sizeof(float)));
-    if (hipHostMalloc((void **)&L->hpin, 3 * (MAXM + 1) * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&L->hpin, 2 * GS_STRIDE * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         cad_set_error("learned form: hipHostMalloc failed");
         return CAD_ERR_HIP;
     }
+    CAD_CHECK(hipEventCreateWithFlags(&L->gs_read, hipEventDisableTiming));
     hipLaunchKernelGGL(k_vzero, dim3(VBLK), dim3(256), 0, c->stream, 0LL, (long long)L->nvec * d.nc, L->vecs);
     CAD_CHECK(hipGetLastError());
     // weights until the caller sets trained ones: the library's initialiser
@@ -584,6 +594,7 @@ void cad_learned_free(cad_ctx *c)
     for (void *q : p)
         if (q) (void)hipFree(q);
     if (L->hpin) (void)hipHostFree(L->hpin);
+    if (L->gs_read) (void)hipEventDestroy(L->gs_read);
     delete L;
     c->lrn = nullptr;
 }
@@ -619,8 +630,33 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
     int iters = 0;
     double rnorm = 0.0;
     bool done = false, stalled = false;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
+    // R: the rotated Hessenberg columns (R[i * m + k]); Hr: the unrotated
+    // columns awaiting their final entries (Hr[k * (m + 1) + i])
+    std::vector<double> H((size_t)(m + 1) * m), Hr((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m),
+        col(m + 1);
     double prev = -1.0;
+    // column k of R from col (rows 0..k+1, unrotated): the final rotations
+    // G_0..G_{k-1}, then G_k from the column itself; returns |g_{k+1}| as it
+    // would be, and applies G_k to g only if commit
+    auto rotate = [&](int k, double sub, bool commit) {
+        for (int i = 0; i < k; i++) {
+            const double a = cs[i] * col[i] + sn[i] * col[i + 1];
+            col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+            col[i] = a;
+        }
+        const double den = std::hypot(col[k], sub);
+        const double ck = den > 0.0 ? col[k] / den : 1.0, sk = den > 0.0 ? sub / den : 0.0;
+        const double est = std::fabs(sk * g[k]);
+        if (commit) {
+            cs[k] = ck;
+            sn[k] = sk;
+            col[k] = den;
+            for (int i = 0; i <= k; i++) H[(size_t)i * m + k] = col[i];
+            g[k + 1] = -sk * g[k];
+            g[k] = ck * g[k];
+        }
+        return est;
+    };
     while (!done && !stalled) {
         // r = BB - A x into V_0
         RET_IF(cad_ext_residual(c, x, Vv(c, 0)));
@@ -642,25 +678,38 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
         hipLaunchKernelGGL(k_vscale, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, 1.0 / rnorm, Vv(c, 0));
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = rnorm;
-        int j = 0;
-        double est = rnorm;
-        for (; j < m && iters < cap; j++) {
+        // Arnoldi with classical Gram-Schmidt done twice, the second pass
+        // delayed by one step (DCGS2: Swirydowicz, Langou, Ananthan, Yang and
+        // Thomas, Numer. Linear Algebra Appl. 28, 2021): step j preconditions
+        // the candidate u_j in slot j (V_0 at j = 0), orthogonal to V_0..V_{j-1}
+        // after one pass, and one reduction over the basis gives both its
+        // second pass (s = V^T u_j) and w's first (p = V^T w).  The update pass
+        // then writes the final v_j = (u_j - V s) / alpha and the next
+        // candidate.  Column j-1 of the Hessenberg matrix gets its last terms
+        // here (u_j = V s + alpha v_j), column j is provisional until the next
+        // step; one read-back per step.
+        int j = 0;            // columns in R
+        double bprev = 0.0;   // the previous column's provisional subdiagonal
+        for (;;) {
+            const int nv = j;
             RET_IF(precond(c, Vv(c, j), Zv(c, j)));
             double *w = Vv(c, j + 1);
             RET_IF(cad_ext_apply_a(c, Zv(c, j), w));
-            // classical Gram-Schmidt, twice, with the coefficients kept on
-            // the device: h1 = V^T w; w -= V h1 and h2 = V^T w in one pass
-            // over V; w -= V h2 and |w|^2 in another; one read-back
-            double *h1d = L->hdev, *h2d = L->hdev + (MAXM + 1), *wwd = L->hdev + 2 * (MAXM + 1);
-            const int nv = j + 1;
-            // register arrays sized to the basis: 8, 16, 24 or all MAXM+1 vectors
-#define CAD_GS(NVB)                                                                                           \
-    do {                                                                                                      \
-        hipLaunchKernelGGL(k_mdot<NVB>, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc, \
-                           nv, w, L->part);                                                                   \
-        hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h1d);          \
-        hipLaunchKernelGGL(k_maxpy_mdot<NVB>, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask,         \
-                           Vv(c, 0), nc, nv, h1d, w, L->part);                                                \
+            double *sum = L->hdev, *cf = L->hdev + GS_STRIDE;
+            const bool last = j + 1 == m || iters + 1 >= cap;
+            // register arrays sized to the basis: 8, 16, 24 or 32 vectors
+#define CAD_GS(NVB)                                                                                              \
+    do {                                                                                                         \
+        hipLaunchKernelGGL(k_dgs_reduce<NVB>, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0),   \
+                           nc, nv, Vv(c, j), w, L->part);                                                        \
+        hipLaunchKernelGGL(k_cols_final, dim3(2 * nv + 3), dim3(256), 0, st, L->part, REDUCE_BLOCKS, 2 * nv + 3,  \
+                           GS_STRIDE, sum);                                                                      \
+        hipLaunchKernelGGL(k_dgs_coef, dim3(1), dim3(64), 0, st, nv, sum, cf);                                   \
+        CAD_CHECK(hipMemcpyAsync(L->hpin, L->hdev, 2 * GS_STRIDE * sizeof(double), hipMemcpyDeviceToHost, st)); \
+        CAD_CHECK(hipEventRecord(L->gs_read, st));                                                               \
+        if (!last)                                                                                               \
+            hipLaunchKernelGGL(k_dgs_update<NVB>, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0), nc,  \
+                               nv, cf, Vv(c, j), w);                                                             \
     } while (0)
             if (nv <= 8)
                 CAD_GS(8);
@@ -669,42 +718,37 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
             else if (nv <= 24)
                 CAD_GS(24);
             else
-                CAD_GS(MAXM + 1);
+                CAD_GS(MAXM);
 #undef CAD_GS
-            hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, st, L->part, REDUCE_BLOCKS, nv, h2d);
-            hipLaunchKernelGGL(k_maxpy_sumsq, dim3(REDUCE_BLOCKS), dim3(256), 0, st, c->act_rng, mask, Vv(c, 0),
-                               nc, nv, h2d, w, L->part);
-            hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, st, L->part, REDUCE_BLOCKS, 1, wwd);
             CAD_CHECK(hipGetLastError());
-            CAD_CHECK(hipMemcpyAsync(L->hpin, L->hdev, 3 * (MAXM + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-            CAD_CHECK(hipStreamSynchronize(st));
-            for (int i = 0; i <= j; i++) h[i] = L->hpin[i] + L->hpin[MAXM + 1 + i];
-            const double hn = std::sqrt(L->hpin[2 * (MAXM + 1)]);
+            // the update pass (if any) runs on while the host reads
+            CAD_CHECK(hipEventSynchronize(L->gs_read));
             iters++;
-            // column j of H, rotated by the earlier Givens rotations
-            for (int i = 0; i < j; i++) {
-                const double a = cs[i] * h[i] + sn[i] * h[i + 1];
-                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-                h[i] = a;
+            const double *hs = L->hpin, *hc = L->hpin + GS_STRIDE;
+            const double al = hc[2 * nv + 3], hj = hc[2 * nv + 4], be = hc[2 * nv + 5];
+            if (j == 0) {
+                g[0] = rnorm * al;   // r = rnorm u_0 = rnorm alpha v_0
+            } else {
+                // column j-1 final: A z_{j-1} = ... + bprev u_j, u_j = V s + alpha v_j
+                double *hk = &Hr[(size_t)(j - 1) * (m + 1)];
+                for (int i = 0; i < j; i++) col[i] = hk[i] + bprev * hs[i];
+                rotate(j - 1, bprev * al, true);
+                if (!(al > 0.0)) break;   // u_j in the span: the space is invariant
             }
-            const double den = std::hypot(h[j], hn);
-            cs[j] = den > 0.0 ? h[j] / den : 1.0;
-            sn[j] = den > 0.0 ? hn / den : 0.0;
-            h[j] = den;
-            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            est = std::fabs(g[j + 1]);
-            if (!(hn > 0.0) || !std::isfinite(est)) {
+            // column j, provisional: p_i, h_j and the remainder's norm
+            for (int i = 0; i < j; i++) col[i] = hs[nv + i];
+            col[j] = hj;
+            double *hk = &Hr[(size_t)j * (m + 1)];
+            for (int i = 0; i <= j; i++) hk[i] = col[i];
+            const double est = rotate(j, be, false);
+            if (!(be > 0.0) || !(est > tol) || !std::isfinite(est) || last) {
+                for (int i = 0; i <= j; i++) col[i] = hk[i];
+                rotate(j, be, true);
                 j++;
                 break;
             }
-            hipLaunchKernelGGL(k_vscale, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, 1.0 / hn, w);
-            CAD_CHECK(hipGetLastError());
-            if (est <= tol) {
-                j++;
-                break;
-            }
+            bprev = be;
+            j++;
         }
         // y = H^-1 g (upper triangular), x += Z y
         for (int i = j - 1; i >= 0; i--) {
