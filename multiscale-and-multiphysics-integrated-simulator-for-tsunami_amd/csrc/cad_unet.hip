@@ -44,6 +44,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -175,13 +176,14 @@ __device__ __forceinline__ void conv_store(const UGeo &g, const ConvOut &out, co
                 rows_pair(w.y, q.y, q.w);
                 const int xf = 2 * (x0 + 16 * j) + 1 + (kg & 1), yf = 2 * y + 1, zf = 2 * o + 1;
                 const int cf = 16 * (m0 + m) + 8 * (kg >> 1);
-#pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    const long long vf =
-                        xf + (long long)out.upx * ((yf + (d & 1)) +
-                                                   (long long)out.upy * ((zf + (d >> 1)) + (long long)out.upz * b));
-                    *reinterpret_cast<uint4 *>(out.up + vf * out.ucs + cf) = q;
-                }
+                const long long vf =
+                    xf + (long long)out.upx * (yf + (long long)out.upy * (zf + (long long)out.upz * b));
+                bf16 *pf = out.up + vf * out.ucs + cf;
+                const long long sy = (long long)out.upx * out.ucs, sz = sy * out.upy;   // uniform
+                *reinterpret_cast<uint4 *>(pf) = q;
+                *reinterpret_cast<uint4 *>(pf + sy) = q;
+                *reinterpret_cast<uint4 *>(pf + sz) = q;
+                *reinterpret_cast<uint4 *>(pf + sy + sz) = q;
             }
             if (EPI == 2) {
                 // channels 4 kg .. 4 kg + 3 here, the other twelve in the other
@@ -434,26 +436,63 @@ __global__ void __launch_bounds__(256 * conv_wx(CIN, MT), conv_wx(CIN, MT) == 1 
             __syncthreads();
             if (pi + 1 < np) fetch(pi + 1);
             if (active) {
+                // Plane pi adds to out(z0 + pi - d) for d = 0, 1, 2; the run's
+                // first two and last two planes have taps whose output lies
+                // outside it.  Blocks of 16 waves, whose runs are short, skip
+                // those products (on runs of two planes, half the work).
+                const int zl = z1 - z0;
+                // MASK: skip the taps whose output lies outside the run
+                auto mac = [&](auto mask) {
 #pragma unroll
-                for (int q = 0; q < KC; q++) {
-                    const int k0 = 32 * q + 8 * kg;
-                    int t = k0 / CIN;
-                    if (t > 8) t = 8;   // K padding: the weights there are zero
-                    const int bq = lb + ((t / 3) * TW + t % 3) * CH + (k0 % CIN) / 8;
-                    bf16x8 bf[NT];
+                    for (int q = 0; q < KC; q++) {
+                        const int k0 = 32 * q + 8 * kg;
+                        int t = k0 / CIN;
+                        if (t > 8) t = 8;   // K padding: the weights there are zero
+                        const int bq = lb + ((t / 3) * TW + t % 3) * CH + (k0 % CIN) / 8;
+                        bf16x8 bf[NT];
 #pragma unroll
-                    for (int j = 0; j < NT; j++) bf[j] = tl[bq + 16 * j * CH];
+                        for (int j = 0; j < NT; j++) bf[j] = tl[bq + 16 * j * CH];
 #pragma unroll
-                    for (int m = 0; m < MT; m++)
+                        for (int m = 0; m < MT; m++)
 #pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            // dz = d - 1 adds to out(p - dz), slot (u + 2 - d) % 3
-                            const bf16x8 af = as[((3 * m + d) * KC + q) * 64 + lane];
+                            for (int d = 0; d < 3; d++) {
+                                if (decltype(mask)::value && (pi - d < 0 || pi - d >= zl)) continue;
+                                // dz = d - 1 adds to out(p - dz), slot (u + 2 - d) % 3
+                                const bf16x8 af = as[((3 * m + d) * KC + q) * 64 + lane];
 #pragma unroll
-                            for (int j = 0; j < NT; j++)
-                                acc[(u + 2 - d) % 3][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    af, bf[j], acc[(u + 2 - d) % 3][m][j], 0, 0, 0);
-                        }
+                                for (int j = 0; j < NT; j++)
+                                    acc[(u + 2 - d) % 3][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        af, bf[j], acc[(u + 2 - d) % 3][m][j], 0, 0, 0);
+                            }
+                    }
+                };
+                if constexpr (WX == 4) {
+                    // two copies of the loop: a branch inside it costs more
+                    if (pi >= 2 && pi < zl)
+                        mac(std::false_type{});
+                    else
+                        mac(std::true_type{});
+                } else {
+#pragma unroll
+                    for (int q = 0; q < KC; q++) {
+                        const int k0 = 32 * q + 8 * kg;
+                        int t = k0 / CIN;
+                        if (t > 8) t = 8;
+                        const int bq = lb + ((t / 3) * TW + t % 3) * CH + (k0 % CIN) / 8;
+                        bf16x8 bf[NT];
+#pragma unroll
+                        for (int j = 0; j < NT; j++) bf[j] = tl[bq + 16 * j * CH];
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+#pragma unroll
+                            for (int d = 0; d < 3; d++) {
+                                const bf16x8 af = as[((3 * m + d) * KC + q) * 64 + lane];
+#pragma unroll
+                                for (int j = 0; j < NT; j++)
+                                    acc[(u + 2 - d) % 3][m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        af, bf[j], acc[(u + 2 - d) % 3][m][j], 0, 0, 0);
+                            }
+                    }
                 }
                 // out(p - 1) is complete: slot u % 3
                 const int o = z0 - 2 + pi;
