@@ -282,9 +282,11 @@ int cad_set_flag_pass_limit(cad_ctx *ctx, long long passes);
  * (vf_m1bcgs.f:144,219).  With rel > 0 it also requires the true relative
  * residual ||BB - A PT|| / ||BB|| <= rel, rechecked in FP64 whenever the
  * reference's test passes, and iterates on otherwise; 0 (the default)
- * restores the reference's test alone.  No counterpart in the reference: it
- * holds the plain form to the true residual the mixed and learned forms stop
- * on, so that their solves are compared at the same residual. */
+ * restores the reference's test alone.  The mixed and learned forms stop on
+ * the true residual ||BB - A PT|| <= CGEPSR ||BB|| + CGEPSA; with rel > 0
+ * also on ||BB - A PT|| <= rel ||BB||, so that CGEPSA cannot loosen it.  No
+ * counterpart in the reference: it holds the three forms to the same true
+ * relative residual, so that their solves are compared at it. */
 int cad_set_true_residual_stop(cad_ctx *ctx, double rel);
 
 /* Schedule of the MILU triangular sweeps (vf_mzdcmp.f, vf_mzfrwd.f,

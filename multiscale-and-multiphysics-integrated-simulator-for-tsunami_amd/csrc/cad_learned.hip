@@ -365,14 +365,14 @@ __global__ void k_vsub(const unsigned long long *rng, const int *__restrict__ ma
         if (mask[n] != -1) a[n] = b[n] - c[n];
 }
 
-// a = b (active cells)
+// a = b (active cells; every cell of the range when mask is null)
 __global__ void k_vcopy(const unsigned long long *rng, const int *__restrict__ mask,
                         const double *__restrict__ b, double *__restrict__ a)
 {
     const long long n0 = (long long)rng[0], n1 = (long long)rng[1];
     for (long long n = n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < n1;
          n += (long long)gridDim.x * blockDim.x)
-        if (mask[n] != -1) a[n] = b[n];
+        if (!mask || mask[n] != -1) a[n] = b[n];
 }
 
 __global__ void k_vzero(long long n0, long long n1, double *__restrict__ a)
@@ -621,9 +621,16 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
     double bn2 = 0.0;
     RET_IF(cad_ext_dot(c, c->bb, c->bb, &bn2));
     const double bnorm = std::sqrt(bn2);
-    const double tol = c->prm.cgepsr * bnorm + c->prm.cgepsa;
+    // the reference's stop on the true residual, held to the context's true
+    // relative residual when one is set
+    double tol = c->prm.cgepsr * bnorm + c->prm.cgepsa;
+    if (c->true_stop > 0.0) tol = std::fmin(tol, c->true_stop * bnorm);
     double *x = Sv(c, 2), *r = Sv(c, 3);
-    hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, x0, x);
+    // x is x0 on the whole range, the inactive cells' zeros included: the
+    // matvec reads them as neighbours, so the residual the solve stops on is
+    // then the one of the x0 it returns (the masked copy back leaves those
+    // cells of x0 as they are)
+    hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, nullptr, x0, x);
     CAD_CHECK(hipGetLastError());
     const int m = L->m;
     const int cap = cad_krylov_cap(c);
@@ -770,7 +777,7 @@ int cad_solve_learned(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_o
         // the same true relative residual
         L->fallbacks++;
         const double keep_stop = c->true_stop;
-        c->true_stop = c->prm.cgepsr;
+        c->true_stop = keep_stop > 0.0 ? std::fmin(keep_stop, c->prm.cgepsr) : c->prm.cgepsr;
         hipLaunchKernelGGL(k_vcopy, dim3(VBLK), dim3(256), 0, st, c->act_rng, mask, x, r);
         CAD_CHECK(hipGetLastError());
         double bn = 0.0, xn = 0.0;

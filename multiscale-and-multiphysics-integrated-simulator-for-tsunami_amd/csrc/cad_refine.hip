@@ -285,7 +285,10 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
     double bn2 = 0.0;
     RET_IF(cad_ext_dot(c, c->bb, c->bb, &bn2));
     const double bnorm = std::sqrt(bn2);
-    const double tol = c->prm.cgepsr * bnorm + c->prm.cgepsa;
+    // the reference's stop on the true residual, held to the context's true
+    // relative residual when one is set
+    double tol = c->prm.cgepsr * bnorm + c->prm.cgepsa;
+    if (c->true_stop > 0.0) tol = std::fmin(tol, c->true_stop * bnorm);
     const int cap = cad_krylov_cap(c);
     int inner_total = 0, outer = 0;
     double rn = 0.0, prev = -1.0;
@@ -343,7 +346,7 @@ int cad_solve_mixed(cad_ctx *c, double *x0, double *bnorm_out, double *xnorm_out
         hipLaunchKernelGGL(k_r_copy, vb, tb, 0, st, rng, L0, x, x0);
         CAD_CHECK(hipGetLastError());
         const double keep_stop = c->true_stop;
-        c->true_stop = c->prm.cgepsr;
+        c->true_stop = keep_stop > 0.0 ? std::fmin(keep_stop, c->prm.cgepsr) : c->prm.cgepsr;
         double bn = 0.0, xn = 0.0;
         int fit = 0;
         const int rc = cad_solve_m1bcgs(c, x0, &bn, &xn, &fit);
