@@ -151,9 +151,13 @@ int cad_set_wave_table2(cad_ctx *ctx, int nz, int nt, const double *zz, const do
 int cad_set_wave2(cad_ctx *ctx, int side, int l1, int l2, double depth, double period,
                   double ampl, double wvlvl);
 /* A theory wave-maker side, MODEL WAVE-BC <side> FUNC STOKES / CNOIDAL / STK-CND
- * (vf_iimdl.f:51-70; VF_CWMAK0/1/2, vf_wstk0.f, vf_wcnd0.f): func -2 fifth-order
- * Stokes, -1 third-order cnoidal, 0 one of the two by the Ursell number
- * g H T^2 / d^2 (Stokes up to 25, vf_csetup.f:101-111).  Columns l1..l2 as in
+ * / STREAM n (vf_iimdl.f:51-70; VF_CWMAK0/1/2, vf_wstk0.f, vf_wcnd0.f,
+ * vf_wsfmb2.f): func -2 fifth-order Stokes, -1 third-order cnoidal, 0 one of
+ * the two by the Ursell number g H T^2 / d^2 (Stokes up to 25,
+ * vf_csetup.f:101-111), 1..21 the stream function of order func (Dean's
+ * method B, second definition of the wave speed; CAD_ERR_ARG if it does not
+ * converge, as VF_CSETUP stops; the reference's order 22 writes past its own
+ * arrays and is refused).  Columns l1..l2 as in
  * cad_set_wave; DEPTH, HEIGHT, PERIOD, AMPL and W-LEVEL.  The phase origin is
  * where the wave's level crosses zero (VF_CWZERO).  Every VF_BWUWN from then on
  * sets the side's face velocities from the wave, scaled by the columns'
@@ -165,7 +169,8 @@ int cad_set_wave_theory(cad_ctx *ctx, int side, int func, int l1, int l2, double
                         double height, double period, double ampl, double wvlvl, double angle,
                         double ax, double ay);
 /* The theory wave itself, on the host and without a context: one VF_CWMAK0 of
- * the wave (func -2 Stokes, -1 cnoidal; depth, height in m, period in s), then
+ * the wave (func -2 Stokes, -1 cnoidal, 1..21 stream function; depth, height
+ * in m, period in s), then
  * optionally its wave length (length), its phase origin (VF_CWZERO, t0), its
  * level at phase wvt (VF_CWMAK1, level) and the velocities at nz heights zc
  * from the still water level (VF_CWMAK2, un, ut).  stk_state is VF_STK0's secant step carried from

@@ -171,8 +171,15 @@ class TheoryWave:
 
 
 # MODEL WAVE-BC FUNC words of the theory waves -> cad_set_wave_theory's func
-# (IBCTYP(2,side), Source code/vf_iimdl.f:58-63)
+# (IBCTYP(2,side), Source code/vf_iimdl.f:52-63); FUNC STREAM n gives n
 WAVE_FUNC = {"STK-CND": 0, "CNOIDAL": -1, "STOKES": -2}
+
+
+def wave_func(wv):
+    """cad_set_wave_theory's func for a deck's wave side (deck.waves[side])."""
+    if wv["func"] == "STREAM":
+        return int(wv["order"])
+    return WAVE_FUNC[wv["func"]]
 
 # schedules of the MILU triangular sweeps (bit-identical results)
 SWEEP_PIPELINED = 0
@@ -304,7 +311,7 @@ class Context:
                          float(wv["period"]), float(wv["ampl"]), float(wvlvl)))
             else:
                 _check(lib().cad_set_wave_theory(
-                    self._h, int(side), WAVE_FUNC[wv["func"]], int(l1), int(l2), float(wv["depth"]),
+                    self._h, int(side), wave_func(wv), int(l1), int(l2), float(wv["depth"]),
                     float(wv["height"]), float(wv["period"]), float(wv["ampl"]), float(wvlvl),
                     float(wv["angle"]), float(wv.get("ax", 0.0)), float(wv.get("ay", 0.0))))
 

@@ -443,8 +443,8 @@ int cad_wave_apply(cad_ctx *c, bool initial)
         const int q = jd - 1;
         if (!c->wv_on[q]) continue;
         any = true;
-        if (c->wv_kind[q] == -2 || c->wv_kind[q] == -1) {
-            // a theory side: STOKES or CNOIDAL (cad_wave_theory.hip)
+        if (c->wv_kind[q] == -2 || c->wv_kind[q] == -1 || c->wv_kind[q] >= 1) {
+            // a theory side: STOKES, CNOIDAL or STREAM n (cad_wave_theory.hip)
             const int rc = cad_wave_theory_apply(c, jd);
             if (rc) return rc;
             continue;

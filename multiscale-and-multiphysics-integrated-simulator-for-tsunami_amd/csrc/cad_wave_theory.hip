@@ -1,7 +1,8 @@
 // cad_wave_theory.hip -- the wave-maker's theory functions: MODEL WAVE-BC
-// <side> FUNC STOKES (fifth order), CNOIDAL (third order) and STK-CND (one of
-// the two by the Ursell number), applied in VF_BWUWN (vf_bwuwn.f:257-480)
-// through VF_CWMAK0/1/2 (vf_cwmak0.f, vf_cwmak1.f, vf_cwmak2.f).
+// <side> FUNC STOKES (fifth order), CNOIDAL (third order), STK-CND (one of
+// the two by the Ursell number) and STREAM n (the stream function of order
+// n, vf_wsfmb2.f), applied in VF_BWUWN (vf_bwuwn.f:257-480) through
+// VF_CWMAK0/1/2 (vf_cwmak0.f, vf_cwmak1.f, vf_cwmak2.f).
 //
 // The reference evaluates these waves in single precision, in CGS units, with
 // the C library's tanhf, coshf, sinhf, expf and sincosf (vf_wstk0.f,
@@ -28,6 +29,12 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+
+#define RET_IF_W(x)            \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
 
 namespace {
 
@@ -464,46 +471,685 @@ void cnd2(const Cnoidal &s, float x, float z1, float *u1, float *w1)
     *w1 = w * csd * z;
 }
 
+// ---- the stream-function wave (vf_wsfmb2.f) ------------------------------
+// Dean's stream-function method B with the second definition of the wave
+// speed: psi = (C - U) z - sum_n A_n sinh(n k (d + z)) cos(n k x) / (n k),
+// the coefficients A_1..A_N, the length L and psi at the surface fitted by
+// Newton least squares to the free-surface condition at 51 phases over half
+// a wave, the Bernoulli constant Q adjusted by halving steps until the
+// height comes out as asked.  The reference computes in double precision
+// from single-precision inputs (SI units, g = 9.8) and keeps its state in
+// COMMON blocks; VF_CWMAK0 starts it afresh each call (vf_prep runs again
+// and psi_eta is reset, vf_wsfmb2.f:43-50), so one Stream per VF_CWMAK0 call
+// holds the same values.  Literals written in single precision in the
+// reference keep their single-precision values here ((double)0.3f, ...).
+struct Stream {
+    // /vf_coef/
+    double d, t, h, u, a[24], rk, sigma, b, pseta;
+    int nn, nn1, nn2;
+    // /vf_cnst/, /vf_sico/
+    double pi, pi2, g, si[201], co[201];
+    int imax, imax1;
+    // /vf_parm/
+    double epsa, depsa[24], epsh, depsh;
+    int iteram, iterhm, irtry;
+    // /vf_euwt/: the phase and height caches of VF_SFMB1/2 are single precision
+    double sit, cot, sh, ch, eta, epse, depse;
+    float xl, xl2, z;
+    int iterem, iecon;
+    // /vf_iniv/, /vf_conv/, /vf_derv/, /vf_atmp/
+    double rkd0, rk0, rl0, b0, q0, deltq0, depsq, bq, hh;
+    int iconv, itera;
+    double e1, ekl[23], ek[23 * 23], detaa[23], a2[24];
+    // VF_SFMB1/2 locals read on their cached paths before being set there
+    // (kept from the previous call, as storage that outlives the call gives)
+    double ax1, uu1, ww1, uut1, wwt1, pp1;
+    double &EK(int k, int l) { return ek[(k - 1) + 23 * (l - 1)]; }   // EK(k,l), column-major
+};
+
+// vf_prep (vf_wsfmb2.f:224-277): constants and the 51 phases' sines
+void sf_prep(Stream &s)
+{
+    s.imax = 50;
+    s.epsa = (double)1.0e-3f;
+    s.epsh = (double)1.0e-3f;
+    s.epse = (double)1.0e-5f;
+    s.iteram = 50;
+    s.iterhm = 200;
+    s.iterem = 30;
+    s.irtry = 5;
+    s.pi = 3.14159265358979;
+    s.pi2 = 2.0 * s.pi;
+    s.g = 9.8;
+    const double df = s.pi / (double)(float)s.imax;
+    s.imax1 = s.imax + 1;
+    for (int i = 1; i <= s.imax1; i++) {
+        const double f = df * (double)(float)(i - 1);
+        s.si[i - 1] = std::sin(f);
+        s.co[i - 1] = std::cos(f);
+    }
+}
+
+// vf_wcdspr (vf_wsfmb2.f:344-362): kd of the linear dispersion relation with
+// a current, by Newton
+double sf_wcdspr(double w2, double usd)
+{
+    const double epsk = (double)1.0e-6f;
+    const double uw2 = 2.0 * usd * w2;
+    const double w = std::sqrt(w2);
+    double rkd = w / (1.0 + w * usd);
+    for (int i = 1; i <= 10; i++) {
+        const double th = std::tanh(rkd);
+        const double q = 1.0 - rkd * usd;
+        const double f = rkd * th - w2 * (q * q);
+        const double df = th + rkd * (1.0 - th * th) + uw2 * (1.0 - rkd * usd);
+        const double dkd = f / df;
+        rkd = rkd - dkd;
+        if (std::fabs(dkd / rkd) < epsk) return rkd;
+    }
+    return rkd;
+}
+
+// vf_init (vf_wsfmb2.f:279-342): first guesses and tolerances
+void sf_init(Stream &s)
+{
+    s.sigma = s.pi2 / s.t;
+    const double w2 = s.sigma * s.sigma * s.d / s.g;
+    const double usd = (s.u + s.pseta / s.d) / (s.sigma * s.d);
+    s.rkd0 = sf_wcdspr(w2, usd);
+    s.rk0 = s.rkd0 / s.d;
+    s.rl0 = s.pi2 / s.rk0;
+    s.b0 = s.rl0 / s.t - (s.u + s.pseta / s.d);
+    s.q0 = s.b0 * s.b0 / 2.0;
+    s.deltq0 = s.q0 * (double)0.01f;
+    s.depsq = s.h * s.h * s.rk0 / (8.0 * std::sinh(2.0 * s.rkd0)) * s.epsh;
+    s.depse = s.epse * s.d;
+    s.nn1 = s.nn + 1;
+    s.nn2 = s.nn + 2;
+    const double depsa1 = s.epsa * s.b0 * s.h / 2.0;
+    const double sh1 = std::sinh(s.rkd0), ch1 = std::cosh(s.rkd0);
+    double shn = sh1, chn = ch1;
+    for (int n = 1; n <= s.nn; n++) {
+        s.depsa[n - 1] = depsa1 / ((double)(float)n * shn);
+        const double c = chn * ch1 + shn * sh1;
+        shn = shn * ch1 + chn * sh1;
+        chn = c;
+    }
+    s.depsa[s.nn1 - 1] = s.epsa * s.rl0;
+    s.depsa[s.nn2 - 1] = s.epsa * s.b0 * s.h / 2.0;
+    s.depsh = s.epsh * s.h;
+}
+
+// vf_inita (vf_wsfmb2.f:468-493): starting coefficients
+void sf_inita(Stream &s)
+{
+    const double hinit = std::fmin((double)0.142f * s.rl0, (double)0.82f * s.d) * (double)0.1f;
+    s.a[0] = -(hinit * s.b0) / (2.0 * std::sinh(s.rkd0));
+    for (int n = 2; n <= s.nn; n++) s.a[n - 1] = 0.0;
+    s.a[s.nn1 - 1] = s.rl0;
+    s.a[s.nn2 - 1] = 0.0;
+    for (int k = 1; k <= s.nn1; k++) s.detaa[k - 1] = 0.0;
+}
+
+// vf_fact (vf_wsfmb2.f:495-512): pull the coefficients back towards the start
+void sf_fact(Stream &s)
+{
+    const double fac = (double)0.8f;
+    for (int n = 1; n <= s.nn; n++) s.a[n - 1] = s.a[n - 1] * fac;
+    s.a[s.nn1 - 1] = (s.a[s.nn1 - 1] - s.rl0) * fac + s.rl0;
+    s.a[s.nn2 - 1] = s.a[s.nn2 - 1] * fac;
+}
+
+// vf_caleta (vf_wsfmb2.f:745-800): the surface eta at the phase (sit, cot),
+// Newton from the current eta; iecon = 1 (eta = 0) if it does not converge
+void sf_caleta(Stream &s)
+{
+    const double rkd = s.rk * s.d;
+    s.iecon = 0;
+    for (int j = 1; j <= s.iterem; j++) {
+        double sitn = s.sit, cotn = s.cot;
+        const double rketa = s.rk * s.eta;
+        if (std::fabs(rketa) > 2.0) break;
+        const double rkdeta = rkd + rketa;
+        const double sh1 = std::sinh(rkdeta), ch1 = std::cosh(rkdeta);
+        double shn = sh1, chn = ch1;
+        double shc = s.b * s.eta - s.pseta;
+        double chc = s.b;
+        for (int n = 1; n <= s.nn; n++) {
+            const double ac = s.a[n - 1] * cotn;
+            shc = shc + ac * shn;
+            chc = chc + (double)(float)n * s.rk * ac * chn;
+            double c = cotn * s.cot - sitn * s.sit;
+            sitn = sitn * s.cot + cotn * s.sit;
+            cotn = c;
+            c = chn * ch1 + shn * sh1;
+            shn = shn * ch1 + chn * sh1;
+            chn = c;
+        }
+        const double dd = shc / chc;
+        s.eta = s.eta - dd;
+        if (std::fabs(dd) < s.depse) return;
+    }
+    s.iecon = 1;
+    s.eta = 0.0;
+}
+
+// vf_psieta (vf_wsfmb2.f:666-743): psi at the surface (the trapezoidal mean
+// over the phases) and the height hh = eta(crest) - eta(trough)
+void sf_psieta(Stream &s)
+{
+    const double rl = s.a[s.nn1 - 1];
+    s.rk = s.pi2 / rl;
+    s.b = rl / s.t - (s.u + s.pseta / s.d);
+    s.pseta = s.a[s.nn2 - 1];
+    s.eta = s.h / 2.0;
+    double sum = 0.0, etac = 0.0, etat = 0.0;
+    for (int i = 1; i <= s.imax1; i++) {
+        s.sit = s.si[i - 1];
+        s.cot = s.co[i - 1];
+        sf_caleta(s);
+        if (s.iecon != 0) return;
+        double sitn = s.sit, cotn = s.cot;
+        const double rkdeta = s.rk * (s.d + s.eta);
+        const double sh1 = std::sinh(rkdeta), ch1 = std::cosh(rkdeta);
+        double shn = sh1, chn = ch1;
+        double shc = 0.0;
+        for (int n = 1; n <= s.nn; n++) {
+            shc = shc + s.a[n - 1] * shn * cotn;
+            double c = cotn * s.cot - sitn * s.sit;
+            sitn = sitn * s.cot + cotn * s.sit;
+            cotn = c;
+            c = chn * ch1 + shn * sh1;
+            shn = shn * ch1 + chn * sh1;
+            chn = c;
+        }
+        if (i == 1) {
+            etac = s.eta;
+            shc = shc / 2.0;
+        } else if (i == s.imax1) {
+            etat = s.eta;
+            shc = shc / 2.0;
+        }
+        sum = sum + shc;
+    }
+    s.hh = etac - etat;
+    s.pseta = sum / (double)(float)s.imax;
+    s.a[s.nn2 - 1] = s.pseta;
+}
+
+// vf_dife (vf_wsfmb2.f:514-664): the error of the surface conditions, its
+// gradient ekl and Gauss-Newton matrix ek over the coefficients, and the
+// derivative of eta (detaa) for the next pass
+void sf_dife(Stream &s)
+{
+    const int nn = s.nn, nn1 = s.nn1;
+    double sdetaa[23], etaa[23], psza[23], psxa[23], shcn[23], qqk[23];
+    const double rl = s.a[nn1 - 1];
+    s.rk = s.pi2 / rl;
+    s.b = rl / s.t - (s.u + s.pseta / s.d);
+    s.pseta = s.a[s.nn2 - 1];
+    for (int k = 1; k <= nn1; k++) {
+        sdetaa[k - 1] = 0.0;
+        s.ekl[k - 1] = 0.0;
+        for (int l = 1; l <= nn1; l++) s.EK(k, l) = 0.0;
+    }
+    double se1 = 0.0;
+    s.e1 = (double)2.0e10f;
+    for (int i = 1; i <= s.imax1; i++) {
+        s.sit = s.si[i - 1];
+        s.cot = s.co[i - 1];
+        sf_caleta(s);
+        if (s.iecon != 0) return;
+        double sitn = s.sit, cotn = s.cot;
+        const double deta = s.d + s.eta;
+        const double rkdeta = s.rk * deta;
+        const double sh1 = std::sinh(rkdeta), ch1 = std::cosh(rkdeta);
+        double shn = sh1, chn = ch1;
+        double chc1 = 0.0, shs1 = 0.0, shc2 = 0.0, chs2 = 0.0, hchc1 = 0.0, hshc2 = 0.0, hchs2 = 0.0;
+        for (int n = 1; n <= nn; n++) {
+            const double rkn = (double)(float)n * s.rk;
+            psza[n - 1] = rkn * chn * cotn;
+            psxa[n - 1] = -(rkn * shn * sitn);
+            shcn[n - 1] = shn * cotn;
+            const double rkna = rkn * s.a[n - 1];
+            const double ach = rkna * chn, ash = rkna * shn;
+            const double achc = ach * cotn, ashs = ash * sitn;
+            const double ashc = rkn * ash * cotn, achs = rkn * ach * sitn;
+            chc1 = chc1 + achc;
+            shs1 = shs1 + ashs;
+            shc2 = shc2 + ashc;
+            chs2 = chs2 + achs;
+            hchc1 = hchc1 + achc * deta;
+            hshc2 = hshc2 + ashc * deta;
+            hchs2 = hchs2 + achs * deta;
+            double c = cotn * s.cot - sitn * s.sit;
+            sitn = sitn * s.cot + cotn * s.sit;
+            cotn = c;
+            c = chn * ch1 + shn * sh1;
+            shn = shn * ch1 + chn * sh1;
+            chn = c;
+        }
+        const double psz = chc1 + s.b, psx = -shs1, psze = shc2, psxe = -chs2;
+        psza[nn1 - 1] = 1.0 / s.t - (chc1 + hshc2) / rl;
+        psxa[nn1 - 1] = (shs1 + hchs2) / rl;
+        shcn[nn1 - 1] = s.eta / s.t - hchc1 / rl;
+        const double qq = (psz * psz + psx * psx) / 2.0 + s.g * s.eta - s.bq;
+        const double qe = psz * psze + psx * psxe + s.g;
+        const double fac = (i == 1 || i == s.imax1) ? 0.5 : 1.0;
+        for (int k = 1; k <= nn1; k++) {
+            etaa[k - 1] = (s.detaa[k - 1] - shcn[k - 1]) / psz;
+            const double qqkk = psz * psza[k - 1] + psx * psxa[k - 1] + qe * etaa[k - 1];
+            qqk[k - 1] = qqkk;
+            s.ekl[k - 1] = s.ekl[k - 1] + fac * 2.0 * qq * qqkk;
+            for (int l = 1; l <= k; l++) s.EK(k, l) = s.EK(k, l) + fac * 2.0 * qqkk * qqk[l - 1];
+        }
+        se1 = se1 + fac * qq * qq;
+        for (int n = 1; n <= nn; n++) sdetaa[n - 1] = sdetaa[n - 1] + fac * (chc1 * etaa[n - 1] + shcn[n - 1]);
+        sdetaa[nn1 - 1] = sdetaa[nn1 - 1] + fac * (chc1 * etaa[nn1 - 1] - hchc1 / rl);
+    }
+    const double fim = (double)(float)s.imax;
+    for (int k = 1; k <= nn1; k++) {
+        s.detaa[k - 1] = sdetaa[k - 1] / fim;
+        s.ekl[k - 1] = s.ekl[k - 1] / fim;
+        for (int l = 1; l <= k; l++) {
+            s.EK(k, l) = s.EK(k, l) / fim;
+            s.EK(l, k) = s.EK(k, l);
+        }
+    }
+    s.e1 = std::sqrt(se1 / fim) / (s.g * s.h);
+}
+
+// vf_ludcmp / vf_lubksb (vf_wsfmb2.f:943-1032): LU decomposition with
+// implicit partial pivoting (the last row of the largest scaled pivot) and
+// the substitutions, on the leading n x n of a 23 x 23 column-major matrix
+void sf_ludcmp(Stream &s, int n, int *indx, double tiny)
+{
+    double vv[23];
+    for (int i = 1; i <= n; i++) {
+        double aamax = 0.0;
+        for (int j = 1; j <= n; j++)
+            if (std::fabs(s.EK(i, j)) > aamax) aamax = std::fabs(s.EK(i, j));
+        vv[i - 1] = 1.0 / aamax;
+    }
+    for (int j = 1; j <= n; j++) {
+        for (int i = 1; i <= j - 1; i++) {
+            double sum = s.EK(i, j);
+            for (int k = 1; k <= i - 1; k++) sum = sum - s.EK(i, k) * s.EK(k, j);
+            s.EK(i, j) = sum;
+        }
+        double aamax = 0.0;
+        int imax = j;
+        for (int i = j; i <= n; i++) {
+            double sum = s.EK(i, j);
+            for (int k = 1; k <= j - 1; k++) sum = sum - s.EK(i, k) * s.EK(k, j);
+            s.EK(i, j) = sum;
+            const double dum = vv[i - 1] * std::fabs(sum);
+            if (dum >= aamax) {
+                imax = i;
+                aamax = dum;
+            }
+        }
+        if (j != imax) {
+            for (int k = 1; k <= n; k++) {
+                const double dum = s.EK(imax, k);
+                s.EK(imax, k) = s.EK(j, k);
+                s.EK(j, k) = dum;
+            }
+            vv[imax - 1] = vv[j - 1];
+        }
+        indx[j - 1] = imax;
+        if (s.EK(j, j) == 0.0) s.EK(j, j) = tiny;
+        if (j != n) {
+            const double dum = 1.0 / s.EK(j, j);
+            for (int i = j + 1; i <= n; i++) s.EK(i, j) = s.EK(i, j) * dum;
+        }
+    }
+}
+
+void sf_lubksb(Stream &s, int n, const int *indx, double *b)
+{
+    int ii = 0;
+    for (int i = 1; i <= n; i++) {
+        const int ll = indx[i - 1];
+        double sum = b[ll - 1];
+        b[ll - 1] = b[i - 1];
+        if (ii != 0) {
+            for (int j = ii; j <= i - 1; j++) sum = sum - s.EK(i, j) * b[j - 1];
+        } else if (sum != 0.0) {
+            ii = i;
+        }
+        b[i - 1] = sum;
+    }
+    for (int i = n; i >= 1; i--) {
+        double sum = b[i - 1];
+        for (int j = i + 1; j <= n; j++) sum = sum - s.EK(i, j) * b[j - 1];
+        b[i - 1] = sum / s.EK(i, i);
+    }
+}
+
+// vf_lstsq (vf_wsfmb2.f:364-466): Gauss-Newton on the coefficients for the
+// current Bernoulli constant; iconv = 0 when converged.  A first pass that
+// halves (eta lost, or the error grown) reads the reference's da before it
+// is set (vf_wsfmb2.f:446-450); here it reads 0.
+void sf_lstsq(Stream &s)
+{
+    double da[23] = {0.0};
+    int nans[23];
+    s.itera = 0;
+    s.hh = (double)1.0e10f;
+    s.iconv = 1;
+    int ifac = 1;
+    for (; ifac <= s.irtry; ifac++) {
+        sf_psieta(s);
+        if (s.iecon == 0) break;
+        sf_fact(s);
+    }
+    if (ifac > s.irtry) {
+        s.iecon = 2;
+        return;
+    }
+    const double sda1 = (double)1.0e10f;
+    double e11 = (double)1.0e10f;
+    for (s.itera = 1; s.itera <= s.iteram; s.itera++) {
+        s.iconv = 1;
+        bool halve = s.iecon != 0;
+        if (!halve) {
+            sf_dife(s);
+            halve = s.e1 > e11;
+        }
+        if (!halve) {
+            const double eps = s.a[s.nn1 - 1] * 1.0e-14;
+            sf_ludcmp(s, s.nn1, nans, eps);
+            sf_lubksb(s, s.nn1, nans, s.ekl);
+            s.iconv = 0;
+            double sda = 0.0;
+            for (int k = 1; k <= s.nn1; k++) {
+                const double f = std::fabs(s.ekl[k - 1] / s.depsa[k - 1]);
+                sda = sda + f;
+                if (f >= 1.0) s.iconv = 1;
+            }
+            double facd = 1.0;
+            if (sda > sda1 * 0.5 && s.itera > 4) facd = 0.5;
+            for (int k = 1; k <= s.nn1; k++) {
+                da[k - 1] = facd * s.ekl[k - 1];
+                s.a[k - 1] = s.a[k - 1] - da[k - 1];
+            }
+            e11 = s.e1;
+        } else {
+            // a step that made the error larger (or lost eta) is halved back
+            s.iconv = 0;
+            for (int k = 1; k <= s.nn1; k++) {
+                da[k - 1] = da[k - 1] / 2.0;
+                s.a[k - 1] = s.a[k - 1] + da[k - 1];
+                if (std::fabs(da[k - 1]) >= s.depsa[k - 1]) s.iconv = 1;
+            }
+        }
+        const double pseta0 = s.a[s.nn2 - 1];
+        sf_psieta(s);
+        if (s.iecon != 0) continue;
+        if (std::fabs(s.a[s.nn2 - 1] - pseta0) >= s.depsa[s.nn2 - 1]) s.iconv = 1;
+        if (s.iconv == 0) return;
+    }
+    s.iconv = 1;
+    s.itera = s.iteram + 1;
+}
+
+// VF_SFMB02C (vf_wsfmb2.f:24-222): the wave of order n1 in depth d1, period
+// t1, height h1 on a current u1 (m, s).  Returns ICONV1: 0 converged, -1
+// input out of range, otherwise iecon*10000 + iterations*100 + Gauss-Newton
+// passes (not converged); *rl1 is the wave length (single precision).  A
+// first outer pass that produces a negative height reads the reference's
+// ifg0 before it is set (vf_wsfmb2.f:123); here it reads 0.
+int sf_setup(Stream &s, int n1, float d1, float t1, float h1, float u1, float *rl1)
+{
+    sf_prep(s);
+    s.pseta = 0.0;
+    s.xl = 1.0e10f;
+    s.xl2 = 2.0e10f;
+    s.z = 1.0e10f;
+    s.ax1 = 0.0;
+    s.uu1 = s.ww1 = s.uut1 = s.wwt1 = s.pp1 = 0.0;
+    s.nn = n1;
+    s.d = d1;
+    s.t = t1;
+    s.h = h1;
+    s.u = u1;
+    s.hh = 0.0;
+    s.iecon = 0;
+    if (s.nn < 1 || s.nn > 22) s.nn = 5;
+    *rl1 = 0.0f;
+    if (s.d <= 0.0 || s.t <= 0.0 || s.h <= 0.0) return -1;
+    const double c0 = s.g * s.t / s.pi2;
+    const double rlo = c0 * s.t;
+    if (s.h / rlo > (double)0.3f) return -1;
+    if (s.h / s.d > 1.5) return -1;
+    if (s.u < -c0 / 2.0) return -1;
+    if (s.d > 2.0 * rlo) s.d = 2.0 * rlo;
+    sf_init(s);
+    s.iconv = 1;
+    s.bq = s.q0 * (double)1.01f;
+    double dq = s.deltq0, rdiv = 1.0;
+    sf_inita(s);
+    for (int k = 1; k <= s.nn2; k++) s.a2[k - 1] = s.a[k - 1];
+    int ifg0 = 0, ifg1 = 0, iterh = 1;
+    double hhh0 = 0.0;
+    bool conv = false;
+    for (; iterh <= s.iterhm; iterh++) {
+        sf_lstsq(s);
+        if (iterh == 1) hhh0 = s.hh - s.h;
+        const double hhh1 = s.hh - s.h;
+        // the flow of labels 74, 71, 72 (height too small: raise Q) and
+        // 21, 81, 82 (too large, or not converged: lower Q and restore A)
+        int path;   // 74, 21
+        if (s.iconv != 0) {
+            path = 21;
+        } else if (std::fabs(s.hh - s.h) < s.depsh) {
+            conv = true;
+            break;
+        } else {
+            path = hhh1 > 0.0 ? 21 : 74;
+        }
+        if (path == 21 && s.iconv != 0 && hhh1 < 0.0) path = 74;
+        if (path == 74) {
+            const bool to71 = iterh == 1 || hhh0 > 0.0 || hhh0 < hhh1;
+            if (!to71) {
+                dq = dq / 2.0;
+                if (ifg0 == 1) {
+                    s.bq = s.bq - dq;
+                    ifg1 = 0;
+                } else {
+                    s.bq = s.bq + dq;
+                    ifg1 = 1;
+                }
+            } else if (s.hh < 0.0) {
+                if (ifg0 == 1) {
+                    dq = dq / 2.0;
+                    s.bq = s.bq - dq;
+                    ifg1 = 0;
+                    if (iterh == 1) ifg0 = 0;
+                } else {
+                    s.bq = s.bq + dq;
+                    ifg1 = 1;
+                    if (iterh == 1) ifg0 = 1;
+                }
+            } else {
+                s.bq = s.bq + dq;
+                ifg1 = 1;
+                if (iterh == 1) ifg0 = 1;
+            }
+            for (int k = 1; k <= s.nn2; k++) s.a2[k - 1] = s.a[k - 1];
+        } else {
+            const bool to81 = iterh == 1 || hhh0 < 0.0 || hhh0 > hhh1;
+            if (!to81) {
+                dq = dq / 2.0;
+                if (ifg0 == 0) {
+                    s.bq = s.bq + dq;
+                    ifg1 = 1;
+                } else {
+                    s.bq = s.bq - dq;
+                    ifg1 = 0;
+                }
+            } else {
+                rdiv = 2.0;
+                dq = dq / rdiv;
+                s.bq = s.bq - dq;
+                ifg1 = 0;
+                if (iterh == 1) ifg0 = 0;
+            }
+            for (int k = 1; k <= s.nn2; k++) s.a[k - 1] = s.a2[k - 1];
+        }
+        if (iterh == 1) continue;
+        hhh0 = hhh1;
+        ifg0 = ifg1;
+        if (dq < s.depsq) break;
+    }
+    (void)rdiv;
+    *rl1 = (float)s.a[s.nn1 - 1];
+    if (conv) return 0;
+    return s.iecon * 10000 + iterh * 100 + s.itera;
+}
+
+// VF_SFMB1 (vf_wsfmb2.f:803-829): the surface level (m) at phase xl1
+float sf_level(Stream &s, float xl1)
+{
+    const double ax = std::fabs(xl1 - s.xl);
+    if (!(ax < (double)1.0e-7f)) {
+        s.xl = xl1;
+        const double rkx = s.pi2 * (double)s.xl;
+        s.sit = std::sin(rkx);
+        s.cot = std::cos(rkx);
+        s.ax1 = std::fabs(s.xl2 - s.xl);
+    }
+    if (!(s.ax1 <= (double)1.0e-7f)) {
+        s.eta = 0.0;
+        sf_caleta(s);
+        s.xl2 = s.xl;
+    }
+    return (float)s.eta;
+}
+
+// VF_SFMB2 (vf_wsfmb2.f:831-909): the horizontal and vertical velocity (m/s)
+// at phase xl1 and height z1 (m from the still water level)
+void sf_velocity(Stream &s, float xl1, float z1, float *u1, float *w1)
+{
+    bool same = false;
+    bool newz = true;
+    if (!(std::fabs(xl1 - s.xl) >= 1.0e-7f)) {
+        if (!(std::fabs(z1 - s.z) >= 1.0e-7f)) same = true;
+    } else {
+        s.xl = xl1;
+        const double rkx = s.pi2 * (double)s.xl;
+        s.sit = std::sin(rkx);
+        s.cot = std::cos(rkx);
+        if (std::fabs(z1 - s.z) < 1.0e-7f) newz = false;
+    }
+    if (!same) {
+        if (newz) {
+            s.z = z1;
+            double rkdz = s.rk * (s.d + (double)s.z);
+            if (rkdz < 0.0) rkdz = 0.0;
+            s.sh = std::sinh(rkdz);
+            s.ch = std::cosh(rkdz);
+        }
+        double chc = 0.0, shs = 0.0, chs = 0.0, shc = 0.0;
+        double sitn = s.sit, cotn = s.cot, shn = s.sh, chn = s.ch;
+        for (int n = 1; n <= s.nn; n++) {
+            const double an = (double)(float)n * s.rk * s.a[n - 1];
+            const double ach = an * chn, ash = an * shn;
+            const double sg = (double)(float)n * s.sigma;
+            chc = chc + ach * cotn;
+            shs = shs + ash * sitn;
+            chs = chs + ach * sitn * sg;
+            shc = shc + ash * cotn * sg;
+            double c = cotn * s.cot - sitn * s.sit;
+            sitn = sitn * s.cot + cotn * s.sit;
+            cotn = c;
+            c = chn * s.ch + shn * s.sh;
+            shn = shn * s.ch + chn * s.sh;
+            chn = c;
+        }
+        s.uu1 = -chc + s.u + s.pseta / s.d;
+        s.ww1 = -shs;
+        const double cc = s.sigma / s.rk;
+        s.uut1 = -(chs * (1.0 - s.uu1 / cc)) - shc * s.ww1 / cc;
+        s.wwt1 = shc * (1.0 - s.uu1 / cc) - chs * s.ww1 / cc;
+        const double du = s.uu1 - cc;
+        s.pp1 = (s.bq - (du * du + s.ww1 * s.ww1) / 2.0) / s.g - (double)s.z;
+    }
+    *u1 = (float)s.uu1;
+    *w1 = (float)s.ww1;
+}
+
 // ---- VF_CWMAK0/1/2 and VF_CWZERO -----------------------------------------
 
 struct Theory {
-    int kind;   // -2 Stokes, -1 cnoidal
+    int kind;   // -2 Stokes, -1 cnoidal, 1..21 the stream function of that order
     Stokes stk;
     Cnoidal cnd;
+    Stream sf;
+    float sf_rl;   // VF_SFMB02C's wave length (m)
 };
 
-// VF_CWMAK0 (vf_cwmak0.f): the wave of depth d, period t, height h (m, s)
-void cwmak0(cad_ctx *c, Theory &w, int kind, double d, double t, double h)
+// VF_CWMAK0 (vf_cwmak0.f): the wave of depth d, period t, height h (m, s);
+// CAD_ERR_ARG when the stream function does not converge (VF_CSETUP's
+// "(VF_SFMB02C) NOT CONVERGED.", vf_cwmak0.f:53-55)
+int cwmak0(cad_ctx *c, Theory &w, int kind, double d, double t, double h)
 {
     w.kind = kind;
+    if (kind >= 1) {
+        // SI units, no current (vf_cwmak0.f:43-55)
+        const int icn = sf_setup(w.sf, kind, (float)d, (float)t, (float)h, 0.0f, &w.sf_rl);
+        if (icn != 0) {
+            cad_set_error("VF_CSETUP: (VF_SFMB02C) NOT CONVERGED. (stream function of order %d, "
+                          "depth %g, period %g, height %g: code %d)", kind, d, t, h, icn);
+            return CAD_ERR_ARG;
+        }
+        return CAD_OK;
+    }
     const float r4d = (float)(d * 100.0), r4t = (float)t, r4h = (float)(h * 100.0);
     if (kind == -2)
         stk0(w.stk, c->wv_stk_dddk, 5, r4d, r4t, r4h);
     else
         cnd0(w.cnd, 3, r4d, r4t, r4h);
+    return CAD_OK;
 }
 
 // the wave length (m) VF_CWMAK0 returns (WVLENG)
 double cwlength(const Theory &w)
 {
+    if (w.kind >= 1) return (double)w.sf_rl;
     return (double)(w.kind == -2 ? w.stk.rl : w.cnd.rl) * 0.01;
 }
 
-// VF_CWMAK1: the level (m) at phase wvt
-double cwmak1(const Theory &w, double wvt)
+// VF_CWMAK1: the level (m) at phase wvt; for the stream function also the
+// VF_SFMB2 call at phase -1 that follows it (vf_cwmak1.f:33-37), which
+// resets VF_SFMB2's cached phase
+double cwmak1(Theory &w, double wvt)
 {
     const float r4tn = (float)wvt;
+    if (w.kind >= 1) {
+        const float z = sf_level(w.sf, r4tn);
+        float u, v;
+        sf_velocity(w.sf, -1.0f, 0.0f, &u, &v);
+        return (double)z;
+    }
     const float z = w.kind == -2 ? stk1(w.stk, r4tn) : cnd1(w.cnd, r4tn);
     return (double)z * 0.01;
 }
 
 // VF_CWMAK2: the normal and tangential velocity (m/s) at phase wvt, height zc
 // (m, from the still water level)
-void cwmak2(const Theory &w, double wvt, double zc, double *un, double *ut)
+void cwmak2(Theory &w, double wvt, double zc, double *un, double *ut)
 {
     const float r4tn = (float)wvt;
-    const float r4z = (float)(zc * 100.0);
     float u, v;
+    if (w.kind >= 1) {
+        sf_velocity(w.sf, r4tn, (float)zc, &u, &v);
+        *un = (double)u;
+        *ut = (double)v;
+        return;
+    }
+    const float r4z = (float)(zc * 100.0);
     if (w.kind == -2)
         stk2(w.stk, r4tn, r4z, &u, &v);
     else
@@ -513,10 +1159,11 @@ void cwmak2(const Theory &w, double wvt, double zc, double *un, double *ut)
 }
 
 // VF_CWZERO (vf_cwzero.f): the phase in (0.5, 1] where the level crosses zero,
-// by 100 steps down from 1 then 20 bisections (|level| < 1e-8 ends early)
-double cwzero(const Theory &w)
+// by 100 steps down from 1 then 20 bisections (|level| < 1e-8, 1e-6 for the
+// stream function, ends early)
+double cwzero(Theory &w)
 {
-    const float eps = 1.0e-8f;
+    const float eps = w.kind >= 1 ? 1.0e-6f : 1.0e-8f;
     double wvzero = 0.5;
     float tn1 = 1.0f;
     float z1 = (float)cwmak1(w, (double)tn1);
@@ -609,9 +1256,10 @@ int cad_wave_theory_eval(int func, double depth, double height, double period, f
                          double wvt, int nz, const double *zc, double *un, double *ut,
                          double *level, double *t0, double *length)
 {
-    if ((func != -2 && func != -1) || !stk_state || (nz > 0 && (!zc || !un || !ut)) || nz < 0) {
-        cad_set_error("cad_wave_theory_eval: func -2 (Stokes) or -1 (cnoidal), the state, and "
-                      "nz >= 0 heights with their results");
+    if ((func != -2 && func != -1 && (func < 1 || func > 21)) || !stk_state || (nz > 0 && (!zc || !un || !ut)) ||
+        nz < 0) {
+        cad_set_error("cad_wave_theory_eval: func -2 (Stokes), -1 (cnoidal) or 1..21 (stream function "
+                      "order), the state, and nz >= 0 heights with their results");
         return CAD_ERR_ARG;
     }
     if (!(depth >= ZERO_REF) || !(height >= ZERO_REF) || !(period >= ZERO_REF)) {
@@ -621,10 +1269,14 @@ int cad_wave_theory_eval(int func, double depth, double height, double period, f
     Theory w;
     w.kind = func;
     const float r4d = (float)(depth * 100.0), r4t = (float)period, r4h = (float)(height * 100.0);
-    if (func == -2)
+    if (func >= 1) {
+        const int rc = cwmak0(nullptr, w, func, depth, period, height);
+        if (rc) return rc;
+    } else if (func == -2) {
         stk0(w.stk, *stk_state, 5, r4d, r4t, r4h);
-    else
+    } else {
         cnd0(w.cnd, 3, r4d, r4t, r4h);
+    }
     if (length) *length = cwlength(w);
     if (t0) *t0 = cwzero(w);
     if (level) *level = cwmak1(w, wvt);
@@ -650,8 +1302,9 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
         cad_set_error("cad_set_wave_theory: side %d (1..4 = X-, X+, Y-, Y+)", side);
         return CAD_ERR_ARG;
     }
-    if (func < -2 || func > 0) {
-        cad_set_error("cad_set_wave_theory: function %d (0 STK-CND, -1 CNOIDAL, -2 STOKES)", func);
+    if (func < -2 || func > 21) {
+        cad_set_error("cad_set_wave_theory: function %d (0 STK-CND, -1 CNOIDAL, -2 STOKES, 1..21 STREAM "
+                      "of that order; the reference's order 22 writes past its own arrays)", func);
         return CAD_ERR_ARG;
     }
     if (c->op_on[side - 1]) {
@@ -703,7 +1356,7 @@ int cad_wave_theory_side(cad_ctx *c, int side, int func, int l1, int l2, double 
     // length, and the phase origin where the level crosses zero
     // (vf_csetup.f:113-125)
     Theory w;
-    cwmak0(c, w, kind, depth, period, height);
+    RET_IF_W(cwmak0(c, w, kind, depth, period, height));
     const int q = side - 1;
     c->wv_on[q] = 1;
     c->wv_kind[q] = kind;
@@ -739,7 +1392,7 @@ int cad_wave_theory_apply(cad_ctx *c, int jd)
         if (a < 1.0) aw = 0.5 * std::sin(PI_REF * (a - 0.5)) + 0.5;
     }
     Theory w;
-    cwmak0(c, w, c->wv_kind[q], dep, t, c->wv_h[q]);
+    RET_IF_W(cwmak0(c, w, c->wv_kind[q], dep, t, c->wv_h[q]));
     const double wvt = (double)r4tn;
     const double wvz = cwmak1(w, wvt);
     c->wv_level[q] = wvz * aw;

@@ -398,11 +398,20 @@ def _parse_model(d, w):
     if key == "FUNC":
         if len(w) < 5:
             raise DeckError("MODEL WAVE-BC FUNC: SYNTAX ERROR.")
-        if w[4] not in ("MATRIX", "MATRIX2", "STOKES", "CNOIDAL", "STK-CND"):
-            if w[4] == "STREAM":
-                raise UnsupportedDeck("MODEL WAVE-BC FUNC STREAM: the stream-function wave is not "
-                                      "implemented")
+        if w[4] not in ("MATRIX", "MATRIX2", "STOKES", "CNOIDAL", "STK-CND", "STREAM"):
             raise DeckError(f"MODEL WAVE-BC FUNC {w[4]}: UNKNOWN WORD.")
+        if w[4] == "STREAM":
+            # the order n of the stream function (vf_iimdl.f:52-57)
+            if len(w) < 6:
+                raise DeckError("MODEL WAVE-BC FUNC STREAM: SYNTAX ERROR.")
+            n = _int(w[5])
+            if n < 1 or n > 22:
+                raise DeckError("MODEL WAVE-BC FUNC STREAM: INVALID VALUE.")
+            if n == 22:
+                raise UnsupportedDeck("MODEL WAVE-BC FUNC STREAM 22: at order 22 the reference's "
+                                      "VF_DIFE writes past its SHCN(22) (vf_wsfmb2.f:519,612), so "
+                                      "the order has no defined result")
+            wv["order"] = n
         wv["func"] = w[4]
     elif key in ("DEPTH", "HEIGHT", "PERIOD", "AMPL"):
         if len(w) < 5:

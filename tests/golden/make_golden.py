@@ -88,6 +88,9 @@ def _cases():
                          [1, 2, 100, 300, 600]),
         "cnoidal_flume": (lambda: cfg.theory_flume("STK-CND", H=0.03, period=2.0,
                                                    name="cnoidal_flume"), [1, 2, 100, 300, 600]),
+        # the fifth-order stream-function wave (FUNC STREAM 5, vf_wsfmb2.f)
+        "stream_flume": (lambda: cfg.theory_flume("STREAM 5", H=0.03, period=1.2, name="stream_flume"),
+                         [1, 2, 100, 300, 600]),
         # the Stokes wave at 25 degrees to the x- side's normal (MODEL WAVE-BC ANGLE)
         "oblique_basin": (lambda: cfg.theory_flume("STOKES", ni=30, nj=16, nk=16, h=0.16, period=0.8,
                                                    steps=300, angle=(25.0, 0.0, 0.1),
@@ -118,6 +121,8 @@ _KEEP = {
                          "f1_out": None} for s in (100, 300, 600)},
     "cnoidal_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
                           "f1_out": None} for s in (100, 300, 600)},
+    "stream_flume": {s: {"v1_in": None, "ps_in": ("US", "VS", "WS"), "v1_out": None, "f1_in": None,
+                         "f1_out": None} for s in (100, 300, 600)},
 }
 
 

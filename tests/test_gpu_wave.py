@@ -1,5 +1,5 @@
 """The wave-makers (MODEL WAVE-BC X- FUNC MATRIX with a LEVEL-ON <name>.mtb
-table; FUNC STOKES and STK-CND, the theory waves) and the radiation boundary
+table; FUNC STOKES, STK-CND and STREAM, the theory waves) and the radiation boundary
 (MODEL OPEN-BC X+ FUNC TYPE1) on the GPU.
 
 Fixtures (tests/golden/make_golden.py): "wave_flume", a 60x4x20 flume at rest
@@ -9,6 +9,8 @@ radiation boundary at the far end that the wave leaves through, 1000 steps
 recorded at 1, 2, 250, 500, 750, 1000; "stokes_flume" and "cnoidal_flume",
 40x4x20 flumes driven by a fifth-order Stokes and (by the Ursell number) a
 third-order cnoidal wave, 600 steps recorded at 1, 2, 100, 300, 600;
+"stream_flume", the same flume driven by the fifth-order stream-function wave
+(FUNC STREAM 5, H = 3 cm, T = 1.2 s), recorded at the same steps;
 "wave_off", the 40-cell flume driven by the same wave as a LEVEL-OFF table
 (no level column: profiles unscaled), 400 steps recorded at 1, 2, 100, 200,
 400; "wave_only", the same flume with its x- side held at the wave's level by
@@ -31,9 +33,9 @@ from helpers import assert_close, assert_same_int, assert_vel_close, rel_err
 pytestmark = pytest.mark.gpu
 
 CASES = ["wave_flume", "wave_open", "wave_off", "wave_only", "wave_two", "stokes_flume",
-         "cnoidal_flume", "oblique_basin"]
+         "cnoidal_flume", "oblique_basin", "stream_flume"]
 RSL = {"wave_flume": (100, 400, 100), "wave_open": (250, 1000, 250),
-       "stokes_flume": (300, 600, 300), "cnoidal_flume": (300, 600, 300),
+       "stokes_flume": (300, 600, 300), "cnoidal_flume": (300, 600, 300), "stream_flume": (300, 600, 300),
        "oblique_basin": (100, 300, 200), "wave_off": (200, 400, 200),
        "wave_only": (200, 400, 200), "wave_two": (200, 400, 200)}
 STATE = ("UU", "VV", "WW", "PP", "FF", "BCU", "BCV", "BCW", "BCP", "BCF", "NF", "INDC")

@@ -49,7 +49,7 @@ def test_metrics_closed_form(pkg):
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB\n"
      "TIME CONST 1e-3\n", "DeckError"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
-     "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nMODEL WAVE-BC X- FUNC STREAM 5\n", "UnsupportedDeck"),
+     "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nMODEL WAVE-BC X- FUNC STREAM 22\n", "UnsupportedDeck"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nOPTION T-DOOR BUB OFF\n"
      "OPTION T-DOOR DROP OFF\nTIME CONST 1e-3\nPARALLEL X 3\n", "UnsupportedDeck"),
     ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nFOO 1\n", "DeckError"),
@@ -99,10 +99,11 @@ def test_deck_drag_df_words(pkg):
 
 
 def test_deck_theory_wave_words(pkg):
-    """MODEL WAVE-BC FUNC STOKES / CNOIDAL / STK-CND and ANGLE (Source
-    code/vf_iimdl.f:51-70, 107-111) with the set-up checks of
+    """MODEL WAVE-BC FUNC STOKES / CNOIDAL / STK-CND / STREAM n and ANGLE
+    (Source code/vf_iimdl.f:51-70, 107-111) with the set-up checks of
     vf_csetup.f:86-100: a theory wave needs DEPTH, HEIGHT and PERIOD, the table
-    takes no ANGLE; STREAM is outside what is produced here."""
+    takes no ANGLE; STREAM takes an order 1..22 (22 has no defined result in
+    the reference and is refused as unsupported)."""
     base = ("GRID X 0 1 2 END\nGRID Y 0 1 2 END\nGRID Z 0 1 2 END\nTIME CONST 1e-3\n"
             "B.C. D VP SLIP\nB.C. D F FREE\n")
     full = ("MODEL WAVE-BC X- DEPTH 0.2\nMODEL WAVE-BC X- HEIGHT 0.02\n"
@@ -115,9 +116,14 @@ def test_deck_theory_wave_words(pkg):
                        "MODEL WAVE-BC X- PERIOD 1.0\n")
     with pytest.raises(pkg.deck.DeckError):
         pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC AIRY\n" + full)
-    for func in ("STREAM 5",):
-        with pytest.raises(pkg.deck.UnsupportedDeck):
+    d = pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STREAM 5\n" + full)
+    assert d.waves[1]["func"] == "STREAM" and d.waves[1]["order"] == 5
+    assert pkg.capi.wave_func(d.waves[1]) == 5
+    for func in ("STREAM", "STREAM 0", "STREAM 23"):   # SYNTAX ERROR / INVALID VALUE
+        with pytest.raises(pkg.deck.DeckError):
             pkg.deck.parse(base + f"MODEL WAVE-BC X- FUNC {func}\n" + full)
+    with pytest.raises(pkg.deck.UnsupportedDeck):
+        pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STREAM 22\n" + full)
     d = pkg.deck.parse(base + "MODEL WAVE-BC X- FUNC STOKES\n" + full +
                        "MODEL WAVE-BC X- ANGLE 10.0 0.5 -0.25\n")
     assert (d.waves[1]["angle"], d.waves[1]["ax"], d.waves[1]["ay"]) == (10.0, 0.5, -0.25)

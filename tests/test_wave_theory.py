@@ -1,6 +1,7 @@
-"""The wave-maker's theory functions (MODEL WAVE-BC FUNC STOKES / STK-CND;
-Source code/vf_cwmak0.f, vf_cwmak1.f, vf_cwmak2.f, vf_wstk0.f, vf_wcnd0.f,
-vf_cwzero.f), evaluated on the host by cad_wave_theory_eval, against the
+"""The wave-maker's theory functions (MODEL WAVE-BC FUNC STOKES / STK-CND /
+STREAM; Source code/vf_cwmak0.f, vf_cwmak1.f, vf_cwmak2.f, vf_wstk0.f,
+vf_wcnd0.f, vf_wsfmb2.f, vf_cwzero.f), evaluated on the host by
+cad_wave_theory_eval, against the
 boundary values the reference wrote on the wave faces (vf_bwuwn.f:331-380).
 
 The reference calls VF_CWMAK0 once at set-up (VF_CSETUP) and once in every
@@ -12,19 +13,25 @@ and flags (vf_bwuwn.f:300-330).  No GPU: the evaluation is host code.
 Fixtures (tests/golden/make_golden.py): "stokes_flume" (fifth-order Stokes,
 H = 2 cm, T = 1 s in 0.2 m) and "cnoidal_flume" (STK-CND with an Ursell
 number of 29, so the third-order cnoidal wave), 40x4x20, ramped in over one
-period, recorded at steps 1, 2, 100, 300, 600.
+period, recorded at steps 1, 2, 100, 300, 600; "stream_flume" (the
+fifth-order stream-function wave, H = 3 cm, T = 1.2 s, FUNC STREAM 5), the
+same flume and steps.  The stream function's set-up is a least-squares
+iteration in double precision from single-precision inputs; a one-literal
+change there (1.01 for the reference's single-precision 1.01) already moves
+the faces' bits.
 """
 import numpy as np
 import pytest
 
 import fixtures as fx
 
-CASES = ["stokes_flume", "cnoidal_flume", "oblique_basin"]
+CASES = ["stokes_flume", "cnoidal_flume", "oblique_basin", "stream_flume"]
 
 
 def _kind(pkg, wv, grz0):
-    """FUNC -> the wave evaluated (STK-CND by the Ursell number, vf_csetup.f:101-111)."""
-    f = pkg.capi.WAVE_FUNC[wv["func"]]
+    """FUNC -> the wave evaluated (STK-CND by the Ursell number, vf_csetup.f:101-111;
+    STREAM n -> n)."""
+    f = pkg.capi.wave_func(wv)
     if f == 0:
         wus = grz0 * wv["height"] * wv["period"] * wv["period"] / wv["depth"] / wv["depth"]
         f = -2 if wus <= 25.0 else -1
@@ -84,7 +91,7 @@ def test_theory_wave_faces_match_reference(pkg, case):
     wv = d.waves[1]
     grz0 = fx.deck_params(st["DECK"])["grz0"]
     kind = _kind(pkg, wv, grz0)
-    assert kind == (-1 if case == "cnoidal_flume" else -2)
+    assert kind == {"cnoidal_flume": -1, "stream_flume": 5}.get(case, -2)
     dep, t, wvlvl = wv["depth"], wv["period"], d.wvlvl
     zz, indx, xx, yy = st["ZZ"], st["INDX"], st["XX"], st["YY"]
     # oblique incidence (vf_bwuwn.f:297-325): the wave's direction, and each
@@ -103,8 +110,9 @@ def test_theory_wave_faces_match_reference(pkg, case):
         w = pkg.capi.TheoryWave(kind, dep, wv["height"], t)
         _, _, _, t0 = w.call(origin=True)            # VF_CSETUP
         dl = w.length
-        for _ in range(s):                           # VF_CINIT, steps 1 .. s-1
-            w.call()
+        if kind < 0:
+            for _ in range(s):                       # VF_CINIT, steps 1 .. s-1 (VF_STK0's state)
+                w.call()
         tnow = float(r["v1_in"]["TNOW"][0])
         wvt = _phase(t0, tnow, t)
         aw = _ramp(tnow, t, wv["ampl"])
