@@ -255,6 +255,7 @@ struct cad_ctx {
     int mg64_n;
     unsigned long long mg_span;   // cells in the active index range of the current solve
     int mg_k0, mg_k1;             // K planes of level 0's passes in the current solve
+    int mg_set_k0, mg_set_k1;     // K planes level 0's rows were last built on (0: none yet)
     // the learned form's network, Krylov basis and counters (cad_learned.hip),
     // allocated when its weights are set
     struct Learned *lrn;

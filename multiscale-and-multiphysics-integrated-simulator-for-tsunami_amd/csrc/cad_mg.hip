@@ -42,6 +42,7 @@
 // the small grids cost one launch instead of five per level.
 #include "cad_common.h"
 
+#include <algorithm>
 #include <climits>
 
 #define RET_IF(x)              \
@@ -80,10 +81,11 @@ __global__ void __launch_bounds__(256) k_mg_fine(Dims d, const double *__restric
                                                  const double *__restrict__ aui,
                                                  const double *__restrict__ auj,
                                                  const double *__restrict__ auk,
-                                                 const int *__restrict__ indc, MgLevelT<V> L)
+                                                 const int *__restrict__ indc, MgLevelT<V> L,
+                                                 long long n0, long long n1)
 {
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= d.nc) return;
+    const long long n = n0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n1) return;
     const int i = (int)(n % d.ni) + 1, j = (int)((n / d.ni) % d.nj) + 1, k = (int)(n / d.nij) + 1;
     const long long m = pos(L, i, j, k);
     const bool a = indc[n] != -1 && ad[n] > 0.0;
@@ -109,13 +111,14 @@ __global__ void __launch_bounds__(256) k_mg_fine(Dims d, const double *__restric
     }
 }
 
-// Galerkin coarse row of coarse interior cell (I,J,K); one thread per cell
+// Galerkin coarse row of coarse interior cell (I,J,K), K from k0; one thread
+// per cell
 template <typename V>
-__global__ void __launch_bounds__(256) k_mg_coarsen(MgLevelT<V> F, MgLevelT<V> C)
+__global__ void __launch_bounds__(256) k_mg_coarsen(MgLevelT<V> F, MgLevelT<V> C, int k0)
 {
     const int I = blockIdx.x * blockDim.x + threadIdx.x + 2;
     const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int K = blockIdx.z * blockDim.z + threadIdx.z + 2;
+    const int K = blockIdx.z + k0;
     if (I > C.d.ni - 1 || J > C.d.nj - 1 || K > C.d.nk - 1) return;
     const Dims &fd = F.d;
     V s_ad = 0, s_li = 0, s_lj = 0, s_lk = 0, s_ui = 0, s_uj = 0, s_uk = 0;
@@ -539,16 +542,23 @@ int alloc_levels(const Dims &d0, MgLevelT<V> *mg, int &nl, bool natural)
     return CAD_OK;
 }
 
+// The levels' rows from the context's matrix on level 0's K planes lo..hi
+// and, level by level, the coarse planes whose rows read those (coarse plane
+// K reads fine planes 2K-3 .. 2K); every other row keeps what it holds.
 template <typename V>
-int setup_levels(cad_ctx *c, MgLevelT<V> *mg, int nl)
+int setup_levels(cad_ctx *c, MgLevelT<V> *mg, int nl, int lo, int hi)
 {
     const Dims &d = c->d;
-    hipLaunchKernelGGL(k_mg_fine<V>, dim3((unsigned)((d.nc + 255) / 256)), dim3(256), 0, c->stream, d,
-                       c->ad, c->ali, c->alj, c->alk, c->aui, c->auj, c->auk, c->f.indc, mg[0]);
+    const long long n0 = (long long)(lo - 1) * d.nij, n1 = (long long)hi * d.nij;
+    hipLaunchKernelGGL(k_mg_fine<V>, dim3((unsigned)((n1 - n0 + 255) / 256)), dim3(256), 0, c->stream, d,
+                       c->ad, c->ali, c->alj, c->alk, c->aui, c->auj, c->auk, c->f.indc, mg[0], n0, n1);
     for (int l = 1; l < nl; l++) {
         const Dims &cd = mg[l].d;
-        hipLaunchKernelGGL(k_mg_coarsen<V>, lvl_grid(cd, 2, cd.nk - 1), dim3(64, 4, 1), 0, c->stream,
-                           mg[l - 1], mg[l]);
+        lo = std::max(2, (lo + 1) / 2);
+        hi = std::min(cd.nk - 1, (hi + 3) / 2);
+        if (lo > hi) break;
+        hipLaunchKernelGGL(k_mg_coarsen<V>, lvl_grid(cd, lo, hi), dim3(64, 4, 1), 0, c->stream, mg[l - 1],
+                           mg[l], lo);
     }
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
@@ -630,17 +640,29 @@ double vcycle_bytes(const cad_ctx *c, const Dims *dims, int nl, int vs)
 
 void cad_mg_free(cad_ctx *c)
 {
+    c->mg_set_k0 = c->mg_set_k1 = 0;
     free_levels(c->mg, c->mg_n);
     free_levels(c->mg64, c->mg64_n);
 }
 
 int cad_mg_alloc(cad_ctx *c) { return alloc_levels(c->d, c->mg, c->mg_n, true); }
 
-// the hierarchy of the context's current matrix
+// the hierarchy of the context's current matrix.  Outside the K planes that
+// hold active cells every row is inactive (INDC = -1 there), as a freshly
+// allocated level's are, so the rows are rebuilt only on the planes of this
+// solve and of the last build (where rows may have been active); the rows
+// are those of a build over every plane.
 int cad_mg_setup(cad_ctx *c)
 {
     RET_IF(cad_mg_alloc(c));
-    return setup_levels(c, c->mg, c->mg_n);
+    int lo = c->mg_k0, hi = c->mg_k1;
+    if (c->mg_set_k0 > 0) {
+        lo = std::min(lo, c->mg_set_k0);
+        hi = std::max(hi, c->mg_set_k1);
+    }
+    c->mg_set_k0 = c->mg_k0;
+    c->mg_set_k1 = c->mg_k1;
+    return setup_levels(c, c->mg, c->mg_n, lo, hi);
 }
 
 int cad_mg_apply(cad_ctx *c, const double *y, double *x, int k0, int k1)
@@ -672,7 +694,7 @@ int cad_mg_time(cad_ctx *c, int reps, int fp64, float *ms_avg, double *bytes)
     if (reps < 1) reps = 1;
     if (fp64) {
         RET_IF(alloc_levels(c->d, c->mg64, c->mg64_n, false));
-        RET_IF(setup_levels(c, c->mg64, c->mg64_n));
+        RET_IF(setup_levels(c, c->mg64, c->mg64_n, 1, c->d.nk));
     }
     hipEvent_t a, b;
     CAD_CHECK(hipEventCreate(&a));
