@@ -209,11 +209,11 @@ __device__ __forceinline__ V block_residual(const MgLevelT<V> &F, int I, int J, 
 }
 
 template <typename V>
-__global__ void __launch_bounds__(256) k_mg_restrict(MgLevelT<V> F, MgLevelT<V> C)
+__global__ void __launch_bounds__(256) k_mg_restrict(MgLevelT<V> F, MgLevelT<V> C, int k0)
 {
     const int I = blockIdx.x * blockDim.x + threadIdx.x + 2;
     const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int K = blockIdx.z * blockDim.z + threadIdx.z + 2;
+    const int K = blockIdx.z + k0;
     if (I > C.d.ni - 1 || J > C.d.nj - 1 || K > C.d.nk - 1) return;
     const V s = block_residual(F, I, J, K);
     const long long m = pos(C, I, J, K);
@@ -579,12 +579,24 @@ int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, 
     while (t0 > 0 && (long long)(mg[t0 - 1].d.ni - 2) * (mg[t0 - 1].d.nj - 2) * (mg[t0 - 1].d.nk - 2) <=
                          MG_TAIL_CELLS)
         t0--;
+    // Each level's passes run on the K planes that can hold its active rows:
+    // level 0's k0..k1, and on a coarser level the planes of their parents
+    // (fine plane k lies in coarse plane (k + 2) / 2).  Rows outside are
+    // inactive, and no active row couples to them (coarsening sums only the
+    // couplings between active rows), so what they hold is never used.
+    int lo[CAD_MG_MAX], hi[CAD_MG_MAX];
+    lo[0] = k0;
+    hi[0] = k1;
+    for (int l = 1; l < nl; l++) {
+        lo[l] = std::max(2, (lo[l - 1] + 2) / 2);
+        hi[l] = std::min(mg[l].d.nk - 1, (hi[l - 1] + 2) / 2);
+    }
     // down: the odd half-sweep (the even half came with b), the coarse b
     for (int l = 0; l < t0; l++) {
         const MgLevelT<V> &F = mg[l], &C = mg[l + 1];
-        const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
-        RET_IF(smooth(c, F, a0, a1, 1));
-        hipLaunchKernelGGL(k_mg_restrict<V>, lvl_grid(C.d, 2, C.d.nk - 1), b, 0, c->stream, F, C);
+        RET_IF(smooth(c, F, lo[l], hi[l], 1));
+        hipLaunchKernelGGL(k_mg_restrict<V>, lvl_grid(C.d, lo[l + 1], hi[l + 1]), b, 0, c->stream, F, C,
+                           lo[l + 1]);
         CAD_CHECK(hipGetLastError());
     }
     MgTail<V> tl;
@@ -597,10 +609,9 @@ int vcycle(cad_ctx *c, const MgLevelT<V> *mg, int nl, const T *y, T *x, int k0, 
     // up: correct, then one sweep (the correction rides on the even half)
     for (int l = t0 - 1; l >= 0; l--) {
         const MgLevelT<V> &F = mg[l], &C = mg[l + 1];
-        const int a0 = l == 0 ? k0 : 2, a1 = l == 0 ? k1 : F.d.nk - 1;
-        hipLaunchKernelGGL(k_mg_correct_even<V>, half_grid(F.d, a0, a1), b, 0, c->stream, F, C, a0);
+        hipLaunchKernelGGL(k_mg_correct_even<V>, half_grid(F.d, lo[l], hi[l]), b, 0, c->stream, F, C, lo[l]);
         CAD_CHECK(hipGetLastError());
-        RET_IF(smooth(c, F, a0, a1, 1));
+        RET_IF(smooth(c, F, lo[l], hi[l], 1));
     }
     if (r)
         hipLaunchKernelGGL((k_mg_out_resid<V, T>), dim3(REDUCE_BLOCKS), dim3(256), 0, c->stream, mg[0], x, r, k0, k1,
