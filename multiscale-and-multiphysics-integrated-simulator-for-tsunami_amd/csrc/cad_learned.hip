@@ -59,6 +59,7 @@ struct Learned {
     float *y;                       // network output
     unsigned long long prev_rng[2]; // active range of the previous solve
     int kb, nzr;                    // the current box placement
+    bool xin_static;                // the network input holds this solve's flag channels
     long long last_iters, last_fallback_iters, solves, fallbacks, total_iters;
     double last_resid;
 };
@@ -102,6 +103,29 @@ __global__ void k_features(Dims d, UGeo g, int kb, int nzr, const double *__rest
             o[7] = (bf16)(act ? 1.f : 0.f);
         }
         *reinterpret_cast<bf16x8 *>(x + uvox(g, xq, yq, zq) * 8) = o;
+    }
+}
+
+// channel 0 of the network input (the scaled residual) on the box's active
+// voxels; the other channels, and the inactive voxels' zeros, are the ones
+// k_features wrote at the start of the solve (the flags do not change within
+// it).  Same value as k_features writes.
+__global__ void k_features_r(Dims d, UGeo g, int kb, int nzr, const double *__restrict__ r,
+                             const double *__restrict__ scal, const int *__restrict__ indc,
+                             bf16 *__restrict__ x)
+{
+    const long long total = (long long)g.nx * g.ny * nzr;
+    const double inv = scal[1];
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int xq = (int)(e % g.nx);
+        const int yq = (int)((e / g.nx) % g.ny);
+        const int zq = (int)(e / ((long long)g.nx * g.ny));
+        const int i = xq + 2, j = yq + 2, k = zq + kb;
+        if (i > d.ni - 1 || j > d.nj - 1 || k > d.nk - 1) continue;
+        const long long n = cid(d, i, j, k);
+        if (indc[n] == -1) continue;
+        x[uvox(g, xq, yq, zq) * 8] = (bf16)(float)(r[n] * inv);
     }
 }
 
@@ -533,10 +557,18 @@ int precond(cad_ctx *c, const double *v, double *z)
         RET_IF(cad_mg_apply_resid(c, v, t0, t1, L->part, MAXM + 1, c->mg_k0, c->mg_k1));
         hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, c->stream, L->part, REDUCE_BLOCKS, L->scal);
         CAD_CHECK(hipGetLastError());
-        RET_IF(features_scaled(c, t1, xin, g));
+        if (L->xin_static) {
+            hipLaunchKernelGGL(k_features_r, dim3(4096), dim3(256), 0, c->stream, c->d, g, L->kb, L->nzr, t1,
+                               L->scal, c->f.indc, xin);
+            CAD_CHECK(hipGetLastError());
+        } else {
+            RET_IF(features_scaled(c, t1, xin, g));
+            L->xin_static = true;
+        }
         z1 = t0;
     } else {
         RET_IF(features(c, v, xin, g));
+        L->xin_static = true;
     }
     RET_IF(cad_unet_forward_impl(L->net));
     hipLaunchKernelGGL(k_scatter, dim3(VBLK), dim3(256), 0, c->stream, c->d, c->act_rng, L->nx, L->ny, L->kb,
@@ -580,6 +612,7 @@ int setup(cad_ctx *c, bool *empty)
     L->kb = k0;
     L->nzr = std::min(((k1 - k0 + 1) + 3) / 4 * 4, L->nzmax);
     RET_IF(cad_unet_set_depth(L->net, L->nzr));
+    L->xin_static = false;   // new flags: the network input is written whole once
     return CAD_OK;
 }
 
