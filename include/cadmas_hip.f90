@@ -26,6 +26,7 @@ module cadmas_hip
   integer(c_int), parameter :: CAD_ERR_WAVE = -50
   integer(c_int), parameter :: CAD_PRECISION_FP64 = 0
   integer(c_int), parameter :: CAD_PRECISION_MIXED = 1
+  integer(c_int), parameter :: CAD_PRECISION_LEARNED = 2
   integer(c_int), parameter :: CAD_INIT_VELOCITY = 1
   integer(c_int), parameter :: CAD_INIT_PRESSURE = 2
 
@@ -288,6 +289,32 @@ module cadmas_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: rel
       integer(c_int) :: cad_poisson_residual
+    end function
+
+    ! every form held to ||BB - A PT|| <= rel ||BB|| (rel > 0; 0 restores
+    ! each form's own test)
+    function cad_set_true_residual_stop(ctx, rel) bind(c, name='cad_set_true_residual_stop')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: rel
+      integer(c_int) :: cad_set_true_residual_stop
+    end function
+
+    ! the learned form's network parameters (n FP32 values) and mode
+    ! (0: network alone, 1: V-cycle then network) with the restart length
+    function cad_set_learned_weights(ctx, params, n) bind(c, name='cad_set_learned_weights')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: params(*)
+      integer(c_int), value :: n
+      integer(c_int) :: cad_set_learned_weights
+    end function
+
+    function cad_set_learned_mode(ctx, mode, restart) bind(c, name='cad_set_learned_mode')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode, restart
+      integer(c_int) :: cad_set_learned_mode
     end function
 
     function cad_cdiv00(ctx, dv) bind(c, name='cad_cdiv00')
