@@ -271,6 +271,22 @@ int cad_time_precond(cad_ctx *ctx, int reps, float *ms_avg, double *bytes);
  * of active cells (INDC != -1) that the solver's vector operations touch. */
 int cad_time_minv(cad_ctx *ctx, int reps, float *ms_avg);
 int cad_count_active(cad_ctx *ctx, long long *active);
+/* In-solve timing of the forms' dominant kernels (no counterpart in the
+ * reference; measurement support).  With on = n >= 1, every n-th launch
+ * inside the solves of the plain form's M^-1 (CAD_KERNEL_MINV), the mixed
+ * form's FP32 V-cycle (CAD_KERNEL_VCYCLE) and the learned form's network pass
+ * (CAD_KERNEL_UNET) is bracketed by HIP events on the context's stream
+ * (without a system-scope fence), with its algorithmic work: the V-cycle's
+ * bytes from the level sizes and the solve's K range, the pass's FLOP (2 per
+ * multiply-add) on the solve's box, 0 for M^-1 (96 B per active cell, which
+ * the caller counts); on = 0 stops.  Turning it on clears the totals;
+ * cad_kernel_timing gives the average duration and work per timed launch and
+ * the number of timed launches since then. */
+#define CAD_KERNEL_MINV 0
+#define CAD_KERNEL_VCYCLE 1
+#define CAD_KERNEL_UNET 2
+int cad_set_kernel_timing(cad_ctx *ctx, int on);
+int cad_kernel_timing(cad_ctx *ctx, int which, double *ms_avg, double *work_avg, long long *count);
 
 /* Cap on the flag-fix passes of VF_FNFINI (vf_fnfini.f:108-180, repeated
  * until no surface cell is left without fluid on one side and gas on the

@@ -123,6 +123,8 @@ def lib():
     L.cad_time_vcycle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_time_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), PD]
     L.cad_unet_mac_per_voxel.restype = C.c_double
+    L.cad_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+    L.cad_kernel_timing.argtypes = [C.c_void_p, C.c_int, PD, PD, C.POINTER(C.c_longlong)]
     L.cad_m1bcgs_noalloc.argtypes = ([C.c_void_p] * 19 + [C.c_int] * 3 + [C.c_double] * 4 +
                                      [C.c_int] * 2 + [PD, PD, PI])
     _lib = L
@@ -145,7 +147,11 @@ EXPORTS = ("cad_last_error", "cad_create", "cad_destroy", "cad_set_params", "cad
            "cad_unet_mac_per_voxel", "cad_unet_init_params", "cad_unet_create", "cad_unet_destroy",
            "cad_unet_set_params", "cad_unet_get_params", "cad_unet_set_input", "cad_unet_forward",
            "cad_unet_get_output", "cad_unet_set_depth", "cad_unet_train_step", "cad_unet_train_step_dy", "cad_unet_gradients_dy",
-           "cad_unet_gradients")
+           "cad_unet_gradients", "cad_set_kernel_timing", "cad_kernel_timing")
+
+# the forms' dominant kernels for Context.kernel_timing
+KERNEL_MINV, KERNEL_VCYCLE, KERNEL_UNET = 0, 1, 2
+
 
 class TheoryWave:
     """A theory wave evaluated on the host (cad_wave_theory_eval), with
@@ -483,6 +489,21 @@ class Context:
         ms, fl = C.c_float(), C.c_double()
         _check(lib().cad_time_learned(self._h, int(reps), C.byref(ms), C.byref(fl)))
         return ms.value, fl.value
+
+    def set_kernel_timing(self, every):
+        """Start (clearing the totals) or stop (every = 0 or False) the in-solve
+        timing of the forms' dominant kernels: HIP events around every
+        `every`-th launch inside the solves (True: each launch)."""
+        _check(lib().cad_set_kernel_timing(self._h, int(every)))
+
+    def kernel_timing(self, which):
+        """(average ms, average algorithmic work, timed launches) of kernel
+        `which` (KERNEL_MINV, KERNEL_VCYCLE, KERNEL_UNET) inside the solves
+        since the timing was turned on; work is bytes for the V-cycle, FLOP
+        for the network pass and 0 for M^-1."""
+        ms, wk, n = C.c_double(), C.c_double(), C.c_longlong()
+        _check(lib().cad_kernel_timing(self._h, int(which), C.byref(ms), C.byref(wk), C.byref(n)))
+        return ms.value, wk.value, n.value
 
     def set_sweep_schedule(self, schedule):
         _check(lib().cad_set_sweep_schedule(self._h, int(schedule)))

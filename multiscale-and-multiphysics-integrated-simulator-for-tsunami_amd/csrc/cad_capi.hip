@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <vector>
 
 static thread_local char g_err[1024] = "";
@@ -231,6 +232,7 @@ void cad_destroy(cad_ctx *c)
     cad_wave_theory_free(c);
     cad_mg_free(c);
     cad_learned_free(c);
+    cad_kt_free(c);
     delete[] c->zz_host;
     delete[] c->xx_host;
     delete[] c->yy_host;
@@ -840,6 +842,101 @@ int cad_time_minv(cad_ctx *c, int reps, float *ms_avg)
     RET_IF(ready(c));
     if (!ms_avg) return CAD_ERR_ARG;
     return cad_time_minv_impl(c, reps, ms_avg);
+}
+
+// ---- in-solve kernel timing ---------------------------------------------
+
+// takes the oldest recorded pair of kernel w; its end event was recorded at
+// least one solve iteration ago, so the wait returns at once in practice
+static int kt_take(cad_ktimer *t, int w)
+{
+    const int s = (int)(t->done[w] % cad_ktimer::PAIRS);
+    CAD_CHECK(hipEventSynchronize(t->ev[w][2 * s + 1]));
+    float ms = 0.f;
+    CAD_CHECK(hipEventElapsedTime(&ms, t->ev[w][2 * s], t->ev[w][2 * s + 1]));
+    t->ms[w] += ms;
+    t->total_work[w] += t->work[w][s];
+    t->done[w]++;
+    return CAD_OK;
+}
+
+int cad_kt_begin(cad_ctx *c, int w)
+{
+    cad_ktimer *t = c->kt;
+    if (!t || !t->on) return CAD_OK;
+    t->open[w] = t->seen[w]++ % t->every == 0;
+    if (!t->open[w]) return CAD_OK;
+    if (t->next[w] - t->done[w] >= cad_ktimer::PAIRS) RET_IF(kt_take(t, w));
+    const int s = (int)(t->next[w] % cad_ktimer::PAIRS);
+    CAD_CHECK(hipEventRecord(t->ev[w][2 * s], c->stream));
+    return CAD_OK;
+}
+
+int cad_kt_end(cad_ctx *c, int w, double work)
+{
+    cad_ktimer *t = c->kt;
+    if (!t || !t->on || !t->open[w]) return CAD_OK;
+    t->open[w] = false;
+    const int s = (int)(t->next[w] % cad_ktimer::PAIRS);
+    CAD_CHECK(hipEventRecord(t->ev[w][2 * s + 1], c->stream));
+    t->work[w][s] = work;
+    t->next[w]++;
+    return CAD_OK;
+}
+
+void cad_kt_free(cad_ctx *c)
+{
+    if (!c->kt) return;
+    for (auto &row : c->kt->ev)
+        for (auto &e : row)
+            if (e) hipEventDestroy(e);
+    delete c->kt;
+    c->kt = nullptr;
+}
+
+int cad_set_kernel_timing(cad_ctx *c, int on)
+{
+    RET_IF(ready(c));
+    if (on < 0) return CAD_ERR_ARG;
+    if (on && !c->kt) {
+        c->kt = new (std::nothrow) cad_ktimer();
+        if (!c->kt) {
+            cad_set_error("cad_set_kernel_timing: out of host memory");
+            return CAD_ERR_STATE;
+        }
+        for (auto &row : c->kt->ev)
+            for (auto &e : row) CAD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    }
+    if (!c->kt) return CAD_OK;
+    if (on) {
+        CAD_CHECK(hipStreamSynchronize(c->stream));   // no pair of an earlier period is in flight
+        for (int w = 0; w < CAD_KT_N; w++) {
+            c->kt->next[w] = c->kt->done[w] = c->kt->seen[w] = 0;
+            c->kt->open[w] = false;
+            c->kt->ms[w] = c->kt->total_work[w] = 0.0;
+        }
+        c->kt->every = on;
+    }
+    c->kt->on = on != 0;
+    return CAD_OK;
+}
+
+int cad_kernel_timing(cad_ctx *c, int which, double *ms_avg, double *work_avg, long long *count)
+{
+    RET_IF(ready(c));
+    if (which < 0 || which >= CAD_KT_N || !ms_avg || !work_avg || !count) return CAD_ERR_ARG;
+    *ms_avg = *work_avg = 0.0;
+    *count = 0;
+    cad_ktimer *t = c->kt;
+    if (!t) return CAD_OK;
+    while (t->done[which] < t->next[which]) RET_IF(kt_take(t, which));
+    const long long n = t->done[which];
+    *count = n;
+    if (n > 0) {
+        *ms_avg = t->ms[which] / (double)n;
+        *work_avg = t->total_work[which] / (double)n;
+    }
+    return CAD_OK;
 }
 
 int cad_count_active(cad_ctx *c, long long *active)

@@ -265,7 +265,35 @@ struct cad_ctx {
     int mixed_fallback_iters;
     // cap on the mixed and learned forms' own iterations (0: 200)
     int krylov_cap;
+    // in-solve timing of the forms' dominant launches (cad_set_kernel_timing)
+    struct cad_ktimer *kt;
 };
+
+// ---- in-solve kernel timing --------------------------------------------
+// While on, every n-th launch of a form's dominant kernel inside the solves
+// (the plain form's M^-1, the mixed form's FP32 V-cycle, the learned form's
+// network pass) is bracketed by a pair of HIP events on the library stream,
+// with the launch's algorithmic work (bytes or FLOP; 0 where the caller
+// supplies it).  The events carry no system-scope fence, so recording them
+// flushes no cache.  Pairs sit in a ring per kernel; a slot's elapsed time is
+// taken when the slot comes round again, and all are taken when read.
+enum { CAD_KT_MINV = 0, CAD_KT_VCYCLE = 1, CAD_KT_UNET = 2, CAD_KT_N = 3 };
+struct cad_ktimer {
+    static constexpr int PAIRS = 256;
+    bool on;
+    int every;                                   // time launches 0, n, 2n, ...
+    long long seen[CAD_KT_N];                    // launches since turned on
+    bool open[CAD_KT_N];                         // a pair's start is recorded
+    hipEvent_t ev[CAD_KT_N][2 * PAIRS];
+    double work[CAD_KT_N][PAIRS];
+    long long next[CAD_KT_N], done[CAD_KT_N];   // pairs recorded, pairs taken
+    double ms[CAD_KT_N], total_work[CAD_KT_N];
+};
+extern "C" {
+int cad_kt_begin(cad_ctx *c, int which);
+int cad_kt_end(cad_ctx *c, int which, double work);
+void cad_kt_free(cad_ctx *c);
+}
 
 // ---- launchers implemented in the phase files ---------------------------
 int cad_launch_predictor(cad_ctx *c);

@@ -570,7 +570,9 @@ int precond(cad_ctx *c, const double *v, double *z)
         RET_IF(features(c, v, xin, g));
         L->xin_static = true;
     }
+    RET_IF(cad_kt_begin(c, CAD_KT_UNET));
     RET_IF(cad_unet_forward_impl(L->net));
+    RET_IF(cad_kt_end(c, CAD_KT_UNET, 2.0 * cad_unet_mac_per_voxel() * (double)L->nx * L->ny * L->nzr));
     hipLaunchKernelGGL(k_scatter, dim3(VBLK), dim3(256), 0, c->stream, c->d, c->act_rng, L->nx, L->ny, L->kb,
                        L->nzr, c->f.indc, z1, L->scal, cad_unet_output(L->net), z);
     CAD_CHECK(hipGetLastError());

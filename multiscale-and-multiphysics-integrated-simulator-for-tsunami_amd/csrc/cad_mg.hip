@@ -688,7 +688,15 @@ int cad_mg_apply_resid(cad_ctx *c, const double *y, double *x, double *r, double
 
 int cad_mg_apply32(cad_ctx *c, const float *y, float *x, int k0, int k1)
 {
-    return vcycle(c, c->mg, c->mg_n, y, x, k0, k1);
+    RET_IF(cad_kt_begin(c, CAD_KT_VCYCLE));
+    RET_IF(vcycle(c, c->mg, c->mg_n, y, x, k0, k1));
+    double by = 0.0;
+    if (c->kt && c->kt->on) {
+        Dims dims[CAD_MG_MAX];
+        for (int l = 0; l < c->mg_n; l++) dims[l] = c->mg[l].d;
+        by = vcycle_bytes(c, dims, c->mg_n, 4);
+    }
+    return cad_kt_end(c, CAD_KT_VCYCLE, by);
 }
 
 // One V-cycle on the hierarchy and K range of the last mixed or learned

@@ -507,7 +507,7 @@ static int sweep_dcmp(cad_ctx *c, double u, int *npos_host)
 }
 
 // x = M^-1 y  (vf_mzminv.f:38-41)
-static int apply_minv(cad_ctx *c, double *x, const double *y)
+static int minv_launch(cad_ctx *c, double *x, const double *y)
 {
     if (c->sweep_schedule != CAD_SWEEP_HYPERPLANE) return cad_sweep_minv(c, x, y);
     const Dims &d = c->d;
@@ -528,6 +528,14 @@ static int apply_minv(cad_ctx *c, double *x, const double *y)
     }
     CAD_CHECK(hipGetLastError());
     return CAD_OK;
+}
+
+static int apply_minv(cad_ctx *c, double *x, const double *y)
+{
+    int rc = cad_kt_begin(c, CAD_KT_MINV);
+    if (!rc) rc = minv_launch(c, x, y);
+    if (!rc) rc = cad_kt_end(c, CAD_KT_MINV, 0.0);
+    return rc;
 }
 
 #define RET_IF_(x)             \
