@@ -905,7 +905,14 @@ int cad_set_kernel_timing(cad_ctx *c, int on)
             return CAD_ERR_STATE;
         }
         for (auto &row : c->kt->ev)
-            for (auto &e : row) CAD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            for (auto &e : row) {
+                const hipError_t err = hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+                if (err != hipSuccess) {
+                    cad_kt_free(c);   // no timer with missing events is kept
+                    cad_set_error("cad_set_kernel_timing: hipEventCreateWithFlags: %s", hipGetErrorString(err));
+                    return CAD_ERR_HIP;
+                }
+            }
     }
     if (!c->kt) return CAD_OK;
     if (on) {
